@@ -1,0 +1,98 @@
+"""ctypes declarations of the hgx C ABI (include/hgx.h) and a loader that fails loudly.
+
+The shared library is ``hipgisaxs_amd/libhgx.so`` built by ``python -m hipgisaxs_amd.build``.
+There is no fallback: if the library is missing or a HIP device is absent the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhgx.so"
+
+HGX_OK = 0
+ERRORS = {
+    -1: "invalid argument",
+    -2: "out of memory",
+    -3: "device error",
+    -4: "no q-grid",
+    -5: "unsupported",
+    -6: "bad handle",
+}
+
+PREC_FP64, PREC_FP32C = 0, 1
+MESH_APPROX8, MESH_EXACT9 = 0, 1
+FF_APPROX, FF_EXACT = 0, 1
+FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN = 1, 2
+SF_COMPAT_LB2 = 1
+SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}
+SHAPE_DIMS = {"sphere": 1, "cylinder": 2, "box": 3, "cube": 1, "prism3": 2, "prism6": 2, "pyramid": 4}
+MAX_NODES = 40
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class ShapeParams(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 4),
+        ("val", _dp * 4),
+        ("wt", _dp * 4),
+        ("tau", C.c_double),
+        ("eta", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "hgx_version": (C.c_char_p, []),
+    "hgx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "hgx_destroy": (C.c_int, [_vp]),
+    "hgx_set_stream": (C.c_int, [_vp, _vp]),
+    "hgx_synchronize": (C.c_int, [_vp]),
+    "hgx_last_error": (C.c_char_p, [_vp]),
+    "hgx_profile_begin": (C.c_int, [_vp]),
+    "hgx_profile_end": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int64)]),
+    "hgx_qgrid_qspace": (C.c_int, [_vp, C.c_int, C.c_int] + [C.c_double] * 6),
+    "hgx_qgrid_explicit": (C.c_int, [_vp, C.c_int, C.c_int] + [C.c_double] * 6),
+    "hgx_qgrid_detector": (C.c_int, [_vp, C.c_int, C.c_int] + [C.c_double] * 6),
+    "hgx_qgrid_set": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "hgx_qz_extended": (C.c_int, [_vp] + [C.c_double] * 4),
+    "hgx_qgrid_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hgx_qgrid_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "hgx_mesh_upload": (C.c_int, [_vp, C.c_int, C.c_int64, _vp, C.POINTER(C.c_int)]),
+    "hgx_mesh_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_mesh_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hgx_ff_numeric": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "hgx_ff_analytic": (C.c_int, [_vp, C.c_int, C.POINTER(ShapeParams), C.c_int, _vp, _vp, _vp]),
+    "hgx_sf_lattice": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp]),
+    "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),
+    "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
+    "hgx_param_distribution": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         _dp, _dp, C.c_int]),
+    "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libhgx.so (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hipgisaxs_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)

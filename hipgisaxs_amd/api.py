@@ -1,0 +1,236 @@
+"""Python face of the hgx C ABI: a Context owning a q-grid and meshes, with device outputs.
+
+Device memory and the stream come from PyTorch (torch tensors on ``cuda``); every compute call
+runs the HIP kernels of ``libhgx.so`` on the stream torch is currently using, so results can be
+consumed by torch ops without extra synchronisation. Nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+class HgxError(RuntimeError):
+    pass
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One hgx context (device, stream, persistent q-grid and meshes)."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise HgxError("no HIP device available")
+        self._torch = torch
+        self.lib = N.load()
+        self.device = device
+        h = C.c_void_p()
+        rc = self.lib.hgx_create(device, C.byref(h))
+        if rc != N.HGX_OK:
+            raise HgxError(f"hgx_create failed: {N.ERRORS.get(rc, rc)}")
+        self.h = h
+        self.follow_torch_stream()
+
+    # ----- plumbing ---------------------------------------------------------------------------
+    def follow_torch_stream(self) -> None:
+        s = self._torch.cuda.current_stream(self.device).cuda_stream
+        self._check(self.lib.hgx_set_stream(self.h, C.c_void_p(s)), "set_stream")
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != N.HGX_OK:
+            msg = self.lib.hgx_last_error(self.h)
+            raise HgxError(f"{what}: {N.ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.hgx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        self._check(self.lib.hgx_synchronize(self.h), "synchronize")
+
+    def profile_begin(self) -> None:
+        self._check(self.lib.hgx_profile_begin(self.h), "profile_begin")
+
+    def profile_end(self) -> dict:
+        mm, am = C.c_double(), C.c_double()
+        mn, an = C.c_int64(), C.c_int64()
+        self._check(self.lib.hgx_profile_end(self.h, C.byref(mm), C.byref(mn), C.byref(am), C.byref(an)),
+                    "profile_end")
+        return {"main_ms": mm.value, "main_launches": mn.value, "aux_ms": am.value, "aux_launches": an.value}
+
+    def _out(self, n: int, out=None):
+        torch = self._torch
+        if out is None:
+            return torch.empty(n, dtype=torch.float64, device=f"cuda:{self.device}")
+        if out.dtype != torch.float64 or out.numel() < n or not out.is_contiguous() or not out.is_cuda:
+            raise HgxError("output must be a contiguous float64 cuda tensor of sufficient size")
+        return out
+
+    @staticmethod
+    def as_complex(t):
+        import torch
+
+        return torch.view_as_complex(t.view(-1, 2))
+
+    # ----- q-grid -----------------------------------------------------------------------------
+    def qgrid_qspace(self, ncol, nrow, qy_min, qz_min, qy_max, qz_max, alpha_i, k0):
+        self._check(self.lib.hgx_qgrid_qspace(self.h, ncol, nrow, qy_min, qz_min, qy_max, qz_max, alpha_i, k0),
+                    "qgrid_qspace")
+
+    def qgrid_explicit(self, ncol, nrow, qy_min, qz_min, qy_max, qz_max, alpha_i, k0):
+        self._check(self.lib.hgx_qgrid_explicit(self.h, ncol, nrow, qy_min, qz_min, qy_max, qz_max, alpha_i, k0),
+                    "qgrid_explicit")
+
+    def qgrid_detector(self, ncol, nrow, pixel, distance, col_center, row_horizon, alpha_i, k0):
+        self._check(self.lib.hgx_qgrid_detector(self.h, ncol, nrow, pixel, distance, col_center, row_horizon,
+                                                alpha_i, k0), "qgrid_detector")
+
+    def qgrid_set(self, ncol, nrow, qx, qy, qz_ext, alpha_f=None):
+        qx, qy = _f64(qx).ravel(), _f64(qy).ravel()
+        qz = np.asarray(qz_ext)
+        if np.iscomplexobj(qz):
+            qz = np.stack([qz.real, qz.imag], -1)
+        qz = _f64(qz).ravel()
+        npix = ncol * nrow
+        if qx.size != npix or qy.size != npix or qz.size not in (2 * npix, 8 * npix):
+            raise HgxError("qgrid_set: array sizes do not match ncol*nrow")
+        nb = qz.size // (2 * npix)
+        af = None if alpha_f is None else _f64(alpha_f).ravel()
+        self._check(self.lib.hgx_qgrid_set(self.h, ncol, nrow, nb, _ptr(qx), _ptr(qy), _ptr(qz), _ptr(af)),
+                    "qgrid_set")
+
+    def qz_extended(self, k0, alpha_i, dnl=0j):
+        dnl = complex(dnl)
+        self._check(self.lib.hgx_qz_extended(self.h, k0, alpha_i, dnl.real, dnl.imag), "qz_extended")
+
+    def info(self):
+        npix, nq = C.c_int64(), C.c_int64()
+        nrow, ncol, nb = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.hgx_qgrid_info(self.h, C.byref(npix), C.byref(nq), C.byref(nrow), C.byref(ncol),
+                                            C.byref(nb)), "qgrid_info")
+        return {"npix": npix.value, "nq": nq.value, "nrow": nrow.value, "ncol": ncol.value, "nbranch": nb.value}
+
+    def qgrid_get(self):
+        i = self.info()
+        qx = np.empty(i["npix"])
+        qy = np.empty(i["npix"])
+        qz = np.empty(2 * i["nq"])
+        af = np.full(i["nrow"], np.nan)
+        self._check(self.lib.hgx_qgrid_get(self.h, _ptr(qx), _ptr(qy), _ptr(qz), _ptr(af)), "qgrid_get")
+        return qx, qy, qz[0::2] + 1j * qz[1::2], af
+
+    # ----- meshes -----------------------------------------------------------------------------
+    def upload_mesh(self, records, kind: str = "exact9") -> int:
+        k = {"approx8": N.MESH_APPROX8, "exact9": N.MESH_EXACT9}[kind]
+        width = 8 if k == N.MESH_APPROX8 else 9
+        r = _f64(records).reshape(-1, width)
+        mid = C.c_int()
+        self._check(self.lib.hgx_mesh_upload(self.h, k, r.shape[0], _ptr(r), C.byref(mid)), "mesh_upload")
+        return mid.value
+
+    def release_mesh(self, mesh_id: int) -> None:
+        self._check(self.lib.hgx_mesh_release(self.h, mesh_id), "mesh_release")
+
+    def mesh_info(self, mesh_id: int):
+        n, v, r = C.c_int64(), C.c_double(), C.c_double()
+        self._check(self.lib.hgx_mesh_info(self.h, mesh_id, C.byref(n), C.byref(v), C.byref(r)), "mesh_info")
+        return {"ntri": n.value, "volume": v.value, "radius": r.value}
+
+    # ----- compute ----------------------------------------------------------------------------
+    def ff_numeric(self, mesh_id, method="approx", precision="fp64", rot=None, transvec=None, flags=0, out=None):
+        m = {"approx": N.FF_APPROX, "exact": N.FF_EXACT}[method]
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        nq = self.info()["nq"]
+        o = self._out(2 * nq, out)
+        R = None if rot is None else _f64(rot).ravel()
+        tv = None if transvec is None else _f64(transvec).ravel()
+        self._check(self.lib.hgx_ff_numeric(self.h, mesh_id, m, p, flags, _ptr(R), _ptr(tv), C.c_void_p(o.data_ptr())),
+                    "ff_numeric")
+        return o
+
+    def ff_analytic(self, shape: str, dims: Sequence, precision="fp64", rot=None, transvec=None, tau=0.0, eta=0.0,
+                    out=None):
+        """dims: per shape dimension a (values, weights) pair of 1-D arrays."""
+        sp = N.ShapeParams()
+        keep = []
+        if len(dims) != N.SHAPE_DIMS[shape]:
+            raise HgxError(f"{shape} takes {N.SHAPE_DIMS[shape]} dimensions")
+        for d, (v, w) in enumerate(dims):
+            v, w = _f64(v).ravel(), _f64(w).ravel()
+            keep += [v, w]
+            sp.n[d] = v.size
+            sp.val[d] = v.ctypes.data_as(C.POINTER(C.c_double))
+            sp.wt[d] = w.ctypes.data_as(C.POINTER(C.c_double))
+        sp.tau, sp.eta = tau, eta
+        nq = self.info()["nq"]
+        o = self._out(2 * nq, out)
+        R = None if rot is None else _f64(rot).ravel()
+        tv = None if transvec is None else _f64(transvec).ravel()
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        self._check(self.lib.hgx_ff_analytic(self.h, N.SHAPES[shape], C.byref(sp), p, _ptr(R), _ptr(tv),
+                                             C.c_void_p(o.data_ptr())), "ff_analytic")
+        return o
+
+    def sf_lattice(self, la, lb, lc, repet, center=(0, 0, 0), dw_sigma=0.0, flags=0, rot=None, out=None):
+        a, b, c, n, ce = (_f64(x).ravel() for x in (la, lb, lc, repet, center))
+        nq = self.info()["nq"]
+        o = self._out(2 * nq, out)
+        R = None if rot is None else _f64(rot).ravel()
+        self._check(self.lib.hgx_sf_lattice(self.h, _ptr(a), _ptr(b), _ptr(c), _ptr(n), _ptr(ce), dw_sigma, flags,
+                                            _ptr(R), C.c_void_p(o.data_ptr())), "sf_lattice")
+        return o
+
+    def fresnel(self, delta, beta, thickness, order, k0, alpha_i, out=None):
+        d, b, t = _f64(delta).ravel(), _f64(beta).ravel(), _f64(thickness).ravel()
+        npix = self.info()["npix"]
+        o = self._out(8 * npix, out)
+        self._check(self.lib.hgx_fresnel(self.h, d.size, _ptr(d), _ptr(b), _ptr(t), order, k0, alpha_i,
+                                         C.c_void_p(o.data_ptr())), "fresnel")
+        return o
+
+    def dwba_intensity(self, ff, fc=None, sf=None, weight=1.0, accumulate=False, out=None):
+        npix = self.info()["npix"]
+        o = self._out(npix, out)
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        self._check(self.lib.hgx_dwba_intensity(self.h, ptr(fc), ptr(sf), ptr(ff), weight, int(accumulate),
+                                                C.c_void_p(o.data_ptr())), "dwba_intensity")
+        return o
+
+
+def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0):
+    """AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284) through the C ABI."""
+    lib = N.load()
+    cap = max(4 * nvalues + 8, 16)
+    v = np.empty(cap)
+    w = np.empty(cap)
+    n = lib.hgx_param_distribution(pmin, pmax, nvalues, stat, mean, sd, v.ctypes.data_as(C.POINTER(C.c_double)),
+                                   w.ctypes.data_as(C.POINTER(C.c_double)), cap)
+    if n < 0:
+        raise HgxError(f"param_distribution: {N.ERRORS.get(n, n)}")
+    return v[:n].copy(), w[:n].copy()
+
+
+def tune_numeric(qpt64: int, qpt32: int, target_wg: int) -> None:
+    rc = N.load().hgx_tune_numeric(qpt64, qpt32, target_wg)
+    if rc != N.HGX_OK:
+        raise HgxError("tune_numeric: invalid knob values")

@@ -1,0 +1,534 @@
+// ff_numeric.hip -- triangulated-mesh form factors on gfx950.
+//
+// Approximate (centroid) method, src/ff/cpu/ff_tri_cpu.cpp:200-246:
+//     F(q) = -i / |q|^2 * sum_t (q . s_t n_t) exp(i q . c_t)
+// Exact (flat triangle) method, src/ff/cpu/ff_tri_cpu.cpp:46-195.
+//
+// Work decomposition (both precisions, so the FP64/FP32 comparison is like for like):
+//   * a q-tile is kWG threads x QPT q-points per thread (q-points strided by kWG: coalesced);
+//   * a triangle chunk is a run of 128-triangle blocks of the Morton-sorted mesh;
+//   * grid = (q-tiles, chunks). With one chunk the kernel finishes the q-points itself, otherwise
+//     it writes FP64 partial sums [chunk][q] and k_ff_finish reduces them in a fixed order
+//     (deterministic; no atomics).
+//   * triangle records are wave-uniform: the loop index depends only on blockIdx and the loop
+//     counter, so the records arrive through scalar loads into SGPRs and feed the VALU directly.
+//
+// FP32C path: per block, coordinates are FP32 relative to the block's FP64 origin o_b, the phase
+// is carried in turns (q/2pi . (c - o_b)), reduced with v_fract and fed to the hardware
+// v_sin/v_cos (which take turns); the block's FP32 sum (plain, or Kahan) is rotated by
+// exp(i q . o_b) and added to an FP64 accumulator. For |q| R < kSeriesQR (R = mesh radius about
+// its centroid) the result is replaced by the FP64 moment series of the same sum, which has no
+// cancellation at small q.
+#include "hgx_internal.hpp"
+
+namespace hgx {
+
+constexpr int kWG = 256;
+constexpr double kSeriesQR = 0.1;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kInvTwoPi = 0.15915494309189533576888376337251;
+
+struct QView {
+  const double* __restrict__ qx;
+  const double* __restrict__ qy;
+  const double* __restrict__ qz;
+  int64_t npix, nq;
+};
+
+struct Epi {
+  Vec3 tv;
+  int has_tv;
+  double volume;
+  // FP32C small-q series
+  const double* moments;
+  double origin[3];
+  double radius;
+  int use_series;
+};
+
+__device__ inline void load_q(const QView& g, const Rot& R, int64_t q, cd mq[3]) {
+  int64_t p = q % g.npix;
+  cd qz = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+  rotate_q(R, g.qx[p], g.qy[p], qz, mq);
+}
+
+__constant__ double c_fact[kSeriesOrder + 1] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0};
+
+// sum_{k=0..K} (i^k / k!) sum_t s_t (q.n_t) (q.(c_t - o))^k from the mesh moments.
+__device__ cd moment_series(const double* __restrict__ M, const cd mq[3]) {
+  cd p[3][kSeriesOrder + 1];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    p[j][0] = cd{1.0, 0.0};
+#pragma unroll
+    for (int k = 1; k <= kSeriesOrder; ++k) p[j][k] = p[j][k - 1] * mq[j];
+  }
+  cd total = cd{0.0, 0.0};
+  cd ik = cd{1.0, 0.0};
+  int idx = 0;
+  for (int k = 0; k <= kSeriesOrder; ++k) {
+    cd Tk = cd{0.0, 0.0};
+    for (int i = 0; i < 3; ++i) {
+      cd s = cd{0.0, 0.0};
+      for (int a = k; a >= 0; --a)
+        for (int b = k - a; b >= 0; --b) {
+          int c = k - a - b;
+          double coef = c_fact[k] / (c_fact[a] * c_fact[b] * c_fact[c]);
+          s = s + (coef * M[idx++]) * (p[0][a] * p[1][b] * p[2][c]);
+        }
+      Tk = Tk + mq[i] * s;
+    }
+    total = total + (1.0 / c_fact[k]) * (ik * Tk);
+    ik = mul_i(ik);
+  }
+  return total;
+}
+
+// Final value of one q-point from the raw sum P = sum_t (q . s n) exp(i q . c) (FP64 path), or
+// P = sum_b exp(i q.o_b) sum_{t in b} ... (FP32C path, identical value).
+__device__ inline cd finish_q(const Epi& e, const cd mq[3], cd P) {
+  double q2 = norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]);
+  cd F;
+  if (q2 == 0.0) {
+    F = cd{e.volume, 0.0};
+  } else if (e.use_series && sqrt(q2) * e.radius < kSeriesQR) {
+    cd S = moment_series(e.moments, mq);
+    cd ph = mq[0] * e.origin[0] + mq[1] * e.origin[1] + mq[2] * e.origin[2];
+    F = (1.0 / q2) * mul_mi(S * cexpi_d(ph));
+  } else {
+    F = (1.0 / q2) * mul_mi(P);
+  }
+  if (e.has_tv) F = F * cexpi_d(mq[0] * e.tv.v[0] + mq[1] * e.tv.v[1] + mq[2] * e.tv.v[2]);
+  return F;
+}
+
+// ----- FP64 plain approximate kernel ---------------------------------------------------------
+
+template <int QPT, bool CQ>
+__global__ __launch_bounds__(kWG) void k_ff_approx64(QView g, Rot R, const double* __restrict__ rec, int64_t ntri,
+                                                     int64_t tchunk, int nchunks, double* __restrict__ part,
+                                                     double* __restrict__ out, Epi e) {
+  const int64_t qbase = (int64_t)blockIdx.x * (kWG * QPT) + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int64_t t0 = (int64_t)chunk * tchunk;
+  const int64_t t1 = min(ntri, t0 + tchunk);
+  cd mq[QPT][3];
+  double ar[QPT], ai[QPT];
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    load_q(g, R, q < g.nq ? q : g.nq - 1, mq[k]);
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+  }
+  for (int64_t t = t0; t < t1; ++t) {
+    const double* r = rec + 8 * t;
+    const double snx = r[0], sny = r[1], snz = r[2], cx = r[3], cy = r[4], cz = r[5];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+      if (!CQ) {
+        double qn = mq[k][0].r * snx + mq[k][1].r * sny + mq[k][2].r * snz;
+        double qt = mq[k][0].r * cx + mq[k][1].r * cy + mq[k][2].r * cz;
+        double s, c;
+        sincos(qt, &s, &c);
+        ar[k] += qn * c;
+        ai[k] += qn * s;
+      } else {
+        cd qn = mq[k][0] * snx + mq[k][1] * sny + mq[k][2] * snz;
+        cd qt = mq[k][0] * cx + mq[k][1] * cy + mq[k][2] * cz;
+        cd E = cexpi_d(qt);
+        ar[k] += qn.r * E.r - qn.i * E.i;
+        ai[k] += qn.r * E.i + qn.i * E.r;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    if (q >= g.nq) continue;
+    if (nchunks == 1) {
+      cd F = finish_q(e, mq[k], cd{ar[k], ai[k]});
+      out[2 * q] = F.r;
+      out[2 * q + 1] = F.i;
+    } else {
+      part[2 * ((int64_t)chunk * g.nq + q)] = ar[k];
+      part[2 * ((int64_t)chunk * g.nq + q) + 1] = ai[k];
+    }
+  }
+}
+
+// ----- FP32 compensated approximate kernel ----------------------------------------------------
+
+template <bool KAHAN>
+struct Acc32 {
+  float sr, si, cr, ci;
+  __device__ inline void zero() { sr = si = cr = ci = 0.0f; }
+  __device__ inline void add(float xr, float xi) {
+    if (KAHAN) {
+      float yr = xr - cr, yi = xi - ci;
+      float tr = sr + yr, ti = si + yi;
+      cr = (tr - sr) - yr;
+      ci = (ti - si) - yi;
+      sr = tr;
+      si = ti;
+    } else {
+      sr += xr;
+      si += xi;
+    }
+  }
+};
+
+template <int QPT, bool CQ, bool KAHAN>
+__global__ __launch_bounds__(kWG) void k_ff_approx32(QView g, Rot R, const float* __restrict__ rec,
+                                                     const double* __restrict__ borig, int64_t nblocks,
+                                                     int64_t bchunk, int nchunks, double* __restrict__ part,
+                                                     double* __restrict__ out, Epi e) {
+  const int64_t qbase = (int64_t)blockIdx.x * (kWG * QPT) + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int64_t b0 = (int64_t)chunk * bchunk;
+  const int64_t b1 = min(nblocks, b0 + bchunk);
+  cd mq[QPT][3];
+  float mx[QPT], my[QPT], mz[QPT], ux[QPT], uy[QPT], uz[QPT];
+  float mxi[QPT], myi[QPT], mzi[QPT];  // imaginary parts (complex q only)
+  double ar[QPT], ai[QPT];
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    load_q(g, R, q < g.nq ? q : g.nq - 1, mq[k]);
+    mx[k] = (float)mq[k][0].r;
+    my[k] = (float)mq[k][1].r;
+    mz[k] = (float)mq[k][2].r;
+    ux[k] = (float)(mq[k][0].r * kInvTwoPi);
+    uy[k] = (float)(mq[k][1].r * kInvTwoPi);
+    uz[k] = (float)(mq[k][2].r * kInvTwoPi);
+    // attenuation exp(-Im(q).c) = exp2(-log2(e) Im(q).c)
+    mxi[k] = (float)(mq[k][0].i * 1.4426950408889634);
+    myi[k] = (float)(mq[k][1].i * 1.4426950408889634);
+    mzi[k] = (float)(mq[k][2].i * 1.4426950408889634);
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    Acc32<KAHAN> acc[QPT];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) acc[k].zero();
+    const float* rb = rec + (int64_t)8 * kBlockTri * b;
+#pragma unroll 4
+    for (int t = 0; t < kBlockTri; ++t) {
+      const float* r = rb + 8 * t;
+      const float snx = r[0], sny = r[1], snz = r[2], cx = r[3], cy = r[4], cz = r[5];
+#pragma unroll
+      for (int k = 0; k < QPT; ++k) {
+        float ph = __builtin_amdgcn_fractf(ux[k] * cx + uy[k] * cy + uz[k] * cz);
+        float s = __builtin_amdgcn_sinf(ph);
+        float c = __builtin_amdgcn_cosf(ph);
+        if (!CQ) {
+          float qn = mx[k] * snx + my[k] * sny + mz[k] * snz;
+          acc[k].add(qn * c, qn * s);
+        } else {
+          float att = __builtin_amdgcn_exp2f(-(mxi[k] * cx + myi[k] * cy + mzi[k] * cz));
+          float qnr = (mx[k] * snx + my[k] * sny + mz[k] * snz) * att;
+          float qni = (mxi[k] * snx + myi[k] * sny + mzi[k] * snz) * (att * 0.69314718055994531f);
+          acc[k].add(qnr * c - qni * s, qnr * s + qni * c);
+        }
+      }
+    }
+    // rotate the block sum by exp(i q . o_b): FP64 phase in turns, reduced in FP64, then the
+    // hardware sin/cos on the reduced FP32 angle
+    const double ox = borig[4 * b], oy = borig[4 * b + 1], oz = borig[4 * b + 2];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+      double tb = (mq[k][0].r * ox + mq[k][1].r * oy + mq[k][2].r * oz) * kInvTwoPi;
+      float fr = (float)(tb - rint(tb));
+      double cb = (double)__builtin_amdgcn_cosf(fr), sb = (double)__builtin_amdgcn_sinf(fr);
+      if (CQ) {
+        double m = exp(-(mq[k][0].i * ox + mq[k][1].i * oy + mq[k][2].i * oz));
+        cb *= m;
+        sb *= m;
+      }
+      double pr = (double)acc[k].sr - (double)acc[k].cr, pi = (double)acc[k].si - (double)acc[k].ci;
+      if (!KAHAN) {
+        pr = (double)acc[k].sr;
+        pi = (double)acc[k].si;
+      }
+      ar[k] += pr * cb - pi * sb;
+      ai[k] += pr * sb + pi * cb;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    if (q >= g.nq) continue;
+    if (nchunks == 1) {
+      cd F = finish_q(e, mq[k], cd{ar[k], ai[k]});
+      out[2 * q] = F.r;
+      out[2 * q + 1] = F.i;
+    } else {
+      part[2 * ((int64_t)chunk * g.nq + q)] = ar[k];
+      part[2 * ((int64_t)chunk * g.nq + q) + 1] = ai[k];
+    }
+  }
+}
+
+// Reduces partial sums over chunks in chunk order and applies the per-q epilogue.
+__global__ __launch_bounds__(kWG) void k_ff_finish(QView g, Rot R, int nchunks, const double* __restrict__ part,
+                                                   double* __restrict__ out, Epi e) {
+  int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (q >= g.nq) return;
+  cd mq[3];
+  load_q(g, R, q, mq);
+  double pr = 0.0, pi = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    pr += part[2 * ((int64_t)c * g.nq + q)];
+    pi += part[2 * ((int64_t)c * g.nq + q) + 1];
+  }
+  cd F = finish_q(e, mq, cd{pr, pi});
+  out[2 * q] = F.r;
+  out[2 * q + 1] = F.i;
+}
+
+// ----- FP64 exact-triangle kernel -------------------------------------------------------------
+
+// One flat triangle, src/ff/cpu/ff_tri_cpu.cpp:46-153 (phase exp(-i q.v)); mq is already
+// negated for the exp(+i q.r) convention.
+__device__ cd exact_triangle(const cd mq[3], double q_sqr, const double* __restrict__ v, double tiny) {
+  double e[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e[0][i] = v[3 + i] - v[i];
+    e[1][i] = v[6 + i] - v[3 + i];
+    e[2][i] = v[i] - v[6 + i];
+  }
+  double nt[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                  e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+  double ntn = sqrt(nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2]);
+  double area = 0.5 * ntn;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nt[i] /= ntn;
+  cd q_nt = mq[0] * nt[0] + mq[1] * nt[1] + mq[2] * nt[2];
+  double proj_tq = q_sqr - norm2(q_nt);
+  if (fabs(proj_tq) < tiny) {
+    cd q_v = mq[0] * v[0] + mq[1] * v[1] + mq[2] * v[2];
+    return (area / q_sqr) * (mul_i(q_nt) * cexpi_d(-q_v));
+  }
+  cd ff = cd{0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double ne[3] = {e[k][1] * nt[2] - e[k][2] * nt[1], e[k][2] * nt[0] - e[k][0] * nt[2],
+                    e[k][0] * nt[1] - e[k][1] * nt[0]};
+    double nen = sqrt(ne[0] * ne[0] + ne[1] * ne[1] + ne[2] * ne[2]);
+    cd q_ne = (mq[0] * ne[0] + mq[1] * ne[1] + mq[2] * ne[2]) * (1.0 / nen);
+    double proj_eq = proj_tq - norm2(q_ne);
+    double elen = sqrt(e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2]);
+    const double* va = v + 3 * k;
+    const double* vb = v + 3 * ((k + 1) % 3);
+    cd q_va = mq[0] * va[0] + mq[1] * va[1] + mq[2] * va[2];
+    if (fabs(proj_eq) < tiny) {
+      double f0 = elen / (q_sqr * proj_tq);
+      ff = ff + f0 * (-(q_nt * q_ne) * cexpi_d(-q_va));
+    } else {
+      double f0 = q_sqr * proj_tq * proj_eq;
+      cd q_nv = (mq[0] * e[k][0] + mq[1] * e[k][1] + mq[2] * e[k][2]) * (1.0 / elen);
+      cd q_vb = mq[0] * vb[0] + mq[1] * vb[1] + mq[2] * vb[2];
+      // +i: the reference's -i here (src/ff/cpu/ff_tri_cpu.cpp:132,146) has the opposite sign of
+      // its cases 1-2 and of the volume integral
+      cd c0 = mul_i(q_nt * q_ne * q_nv);
+      ff = ff + (1.0 / f0) * (c0 * (cexpi_d(-q_va) - cexpi_d(-q_vb)));
+    }
+  }
+  return ff;
+}
+
+__global__ __launch_bounds__(kWG) void k_ff_exact64(QView g, Rot R, double sign, const double* __restrict__ tri,
+                                                    int64_t ntri, int64_t tchunk, int nchunks, double tiny,
+                                                    double* __restrict__ part, double* __restrict__ out, Epi e) {
+  int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int64_t t0 = (int64_t)chunk * tchunk;
+  const int64_t t1 = min(ntri, t0 + tchunk);
+  cd mq[3];
+  load_q(g, R, q < g.nq ? q : g.nq - 1, mq);
+  cd ms[3] = {sign * mq[0], sign * mq[1], sign * mq[2]};
+  double q2 = norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]);
+  cd acc = cd{0.0, 0.0};
+  if (q2 != 0.0)
+    for (int64_t t = t0; t < t1; ++t) acc = acc + exact_triangle(ms, q2, tri + 9 * t, tiny);
+  if (q >= g.nq) return;
+  if (nchunks == 1) {
+    cd F = q2 == 0.0 ? cd{e.volume, 0.0} : acc;
+    if (e.has_tv) F = F * cexpi_d(mq[0] * e.tv.v[0] + mq[1] * e.tv.v[1] + mq[2] * e.tv.v[2]);
+    out[2 * q] = F.r;
+    out[2 * q + 1] = F.i;
+  } else {
+    part[2 * ((int64_t)chunk * g.nq + q)] = acc.r;
+    part[2 * ((int64_t)chunk * g.nq + q) + 1] = acc.i;
+  }
+}
+
+__global__ __launch_bounds__(kWG) void k_ff_exact_finish(QView g, Rot R, int nchunks, const double* __restrict__ part,
+                                                         double* __restrict__ out, Epi e) {
+  int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (q >= g.nq) return;
+  cd mq[3];
+  load_q(g, R, q, mq);
+  double q2 = norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]);
+  cd F = cd{0.0, 0.0};
+  for (int c = 0; c < nchunks; ++c) F = F + cd{part[2 * ((int64_t)c * g.nq + q)], part[2 * ((int64_t)c * g.nq + q) + 1]};
+  if (q2 == 0.0) F = cd{e.volume, 0.0};
+  if (e.has_tv) F = F * cexpi_d(mq[0] * e.tv.v[0] + mq[1] * e.tv.v[1] + mq[2] * e.tv.v[2]);
+  out[2 * q] = F.r;
+  out[2 * q + 1] = F.i;
+}
+
+// ----- host-side launch planning --------------------------------------------------------------
+
+struct Plan {
+  int qpt;
+  unsigned nqt;
+  int nchunks;
+  int64_t per_chunk;  // blocks (FP32C) or triangles (FP64 / exact) per chunk
+};
+
+// Enough workgroups to fill 256 CUs several times over, without splitting below one block.
+static Plan make_plan(int64_t nq, int64_t units, int64_t min_units, int qpt, int target_wg) {
+  Plan p;
+  p.qpt = qpt;
+  p.nqt = ceil_div(nq, (int64_t)kWG * qpt);
+  int64_t want = std::max<int64_t>(1, (target_wg + p.nqt - 1) / p.nqt);
+  int64_t maxc = std::max<int64_t>(1, units / std::max<int64_t>(1, min_units));
+  want = std::min(want, maxc);
+  p.per_chunk = (units + want - 1) / want;
+  p.nchunks = (int)((units + p.per_chunk - 1) / p.per_chunk);
+  return p;
+}
+
+template <int QPT>
+static void launch_approx64(hipStream_t s, const Plan& p, QView g, Rot R, const Mesh& m, bool cq, double* part,
+                            double* out, Epi e) {
+  dim3 grid(p.nqt, p.nchunks);
+  if (cq)
+    k_ff_approx64<QPT, true><<<grid, kWG, 0, s>>>(g, R, m.d_rec64, m.ntri, p.per_chunk, p.nchunks, part, out, e);
+  else
+    k_ff_approx64<QPT, false><<<grid, kWG, 0, s>>>(g, R, m.d_rec64, m.ntri, p.per_chunk, p.nchunks, part, out, e);
+}
+
+template <int QPT>
+static void launch_approx32(hipStream_t s, const Plan& p, QView g, Rot R, const Mesh& m, bool cq, bool kahan,
+                            double* part, double* out, Epi e) {
+  dim3 grid(p.nqt, p.nchunks);
+#define HGX_L32(CQ, KH) \
+  k_ff_approx32<QPT, CQ, KH><<<grid, kWG, 0, s>>>(g, R, m.d_rec32, m.d_borig, m.nblocks, p.per_chunk, p.nchunks, part, out, e)
+  if (cq) {
+    if (kahan) HGX_L32(true, true); else HGX_L32(true, false);
+  } else {
+    if (kahan) HGX_L32(false, true); else HGX_L32(false, false);
+  }
+#undef HGX_L32
+}
+
+}  // namespace hgx
+
+using namespace hgx;
+
+// Tuning knobs for the numeric kernels (set by the autotuner; defaults are measured choices).
+static int g_qpt64 = 2, g_qpt32 = 4, g_target_wg = 2048;
+
+extern "C" int hgx_tune_numeric(int qpt64, int qpt32, int target_wg) {
+  auto ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8; };
+  if (!ok(qpt64) || !ok(qpt32) || target_wg < 1) return HGX_ERR_INVALID_ARG;
+  g_qpt64 = qpt64;
+  g_qpt32 = qpt32;
+  g_target_wg = target_wg;
+  return HGX_OK;
+}
+
+extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision, int flags, const double rot[9],
+                              const double transvec[3], double* ff) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ff_numeric: no q-grid");
+  if (mesh_id < 0 || mesh_id >= (int)c->meshes.size() || !c->meshes[mesh_id].live)
+    return set_error(c, HGX_ERR_BAD_HANDLE, "ff_numeric: unknown mesh %d", mesh_id);
+  if (!ff) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: null output");
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
+    return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: bad precision %d", precision);
+  const Mesh& m = c->meshes[mesh_id];
+  if (int rc = set_device(c)) return rc;
+  QView g{c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq};
+  Rot R = make_rot(rot);
+  Epi e;
+  e.tv = make_vec(transvec);
+  e.has_tv = transvec != nullptr;
+  e.volume = m.volume;
+  e.moments = m.d_moments;
+  for (int k = 0; k < 3; ++k) e.origin[k] = m.origin[k];
+  e.radius = m.radius;
+  e.use_series = 0;
+  const bool cq = !c->qz_real;  // a rotation keeps real q real
+  if (method == HGX_FF_APPROX) {
+    Plan p;
+    if (precision == HGX_PREC_FP64)
+      p = make_plan(c->nq, m.ntri, kBlockTri, g_qpt64, g_target_wg);
+    else
+      p = make_plan(c->nq, m.nblocks, 1, g_qpt32, g_target_wg);
+    double* part = nullptr;
+    if (p.nchunks > 1) {
+      if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;
+      part = reinterpret_cast<double*>(c->d_work);
+    }
+    {
+    LaunchTimer lt(c, true);
+    if (precision == HGX_PREC_FP64) {
+      switch (p.qpt) {
+        case 1: launch_approx64<1>(c->stream, p, g, R, m, cq, part, ff, e); break;
+        case 2: launch_approx64<2>(c->stream, p, g, R, m, cq, part, ff, e); break;
+        case 4: launch_approx64<4>(c->stream, p, g, R, m, cq, part, ff, e); break;
+        default: launch_approx64<8>(c->stream, p, g, R, m, cq, part, ff, e); break;
+      }
+    } else {
+      e.use_series = 1;
+      bool kahan = (flags & HGX_FLAG_ACC_KAHAN) != 0;
+      switch (p.qpt) {
+        case 1: launch_approx32<1>(c->stream, p, g, R, m, cq, kahan, part, ff, e); break;
+        case 2: launch_approx32<2>(c->stream, p, g, R, m, cq, kahan, part, ff, e); break;
+        case 4: launch_approx32<4>(c->stream, p, g, R, m, cq, kahan, part, ff, e); break;
+        default: launch_approx32<8>(c->stream, p, g, R, m, cq, kahan, part, ff, e); break;
+      }
+    }
+    }
+    HGX_CHECK_LAUNCH(c);
+    if (p.nchunks > 1) {
+      LaunchTimer lt(c, false);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
+      HGX_CHECK_LAUNCH(c);
+    }
+    return HGX_OK;
+  }
+  if (method == HGX_FF_EXACT) {
+    if (m.kind != HGX_MESH_EXACT9 || !m.d_tri9)
+      return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: exact method needs a vertex (exact9) mesh");
+    if (precision != HGX_PREC_FP64)
+      return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric: exact method is FP64 only");
+    Plan p = make_plan(c->nq, m.ntri, 64, 1, g_target_wg);
+    double* part = nullptr;
+    if (p.nchunks > 1) {
+      if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;
+      part = reinterpret_cast<double*>(c->d_work);
+    }
+    double sign = (flags & HGX_FLAG_EXACT_REFERENCE_SIGN) ? 1.0 : -1.0;
+    const double tiny = 2.2250738585072014e-308;  // DBL_MIN, the reference CPU threshold
+    dim3 grid(p.nqt, p.nchunks);
+    {
+      LaunchTimer lt(c, true);
+      k_ff_exact64<<<grid, kWG, 0, c->stream>>>(g, R, sign, m.d_tri9, m.ntri, p.per_chunk, p.nchunks, tiny, part, ff, e);
+    }
+    HGX_CHECK_LAUNCH(c);
+    if (p.nchunks > 1) {
+      LaunchTimer lt(c, false);
+      k_ff_exact_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
+      HGX_CHECK_LAUNCH(c);
+    }
+    return HGX_OK;
+  }
+  return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: bad method %d", method);
+}

@@ -1,0 +1,182 @@
+// hgx_internal.hpp -- context state, error plumbing and device helpers shared by the hgx kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "hgx.h"
+
+namespace hgx {
+
+// One uploaded triangle mesh. Triangles are stored in a spatially sorted order (Morton order of
+// the centroids) and grouped in blocks of kBlockTri; every block carries its own FP64 origin so the
+// FP32 path only ever sees coordinates relative to a nearby point.
+constexpr int kBlockTri = 128;
+constexpr int kSeriesOrder = 8;  // small-|q| moment series: terms k = 0..kSeriesOrder
+// number of moment components: sum_k 3 * (k+1)(k+2)/2
+constexpr int kMomentCount = 3 * 165;
+
+struct Mesh {
+  bool live = false;
+  int kind = 0;
+  int64_t ntri = 0;       // triangles (after dropping degenerate ones)
+  int64_t nblocks = 0;    // ceil(ntri / kBlockTri)
+  double volume = 0.0;    // sum_t s_t (n_t . c_t) / 3
+  double origin[3] = {0, 0, 0};
+  double radius = 0.0;    // max_t |c_t - origin|
+  // device arrays
+  double* d_rec64 = nullptr;   // [ntri][8]: s*nx, s*ny, s*nz, cx, cy, cz, 0, 0 (absolute, FP64)
+  float* d_rec32 = nullptr;    // [nblocks*kBlockTri][8]: s*n (3), c - block origin (3), 0, 0 (FP32)
+  double* d_borig = nullptr;   // [nblocks][4]: block origin x, y, z, 0
+  double* d_tri9 = nullptr;    // [ntri][9] vertices (exact meshes only), same order as d_rec64
+  double* d_moments = nullptr; // [kMomentCount] moments about origin (see ff_numeric.hip)
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  // q-grid
+  int ncol = 0, nrow = 0, nbranch = 0;
+  int64_t npix = 0, nq = 0;
+  bool qz_real = true;      // all Im(qz) == 0
+  bool have_alpha = false;
+  double* d_qx = nullptr;   // [npix]
+  double* d_qy = nullptr;   // [npix]
+  double* d_qz = nullptr;   // [2*nq] interleaved complex
+  double* d_alpha = nullptr;// [nrow]
+  // meshes
+  std::vector<Mesh> meshes;
+  // scratch for partial sums
+  void* d_work = nullptr;
+  size_t work_bytes = 0;
+  std::string last_error;
+  // optional HIP-event timing of kernel launches (hgx_profile_begin / hgx_profile_end)
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_main, ev_aux, ev_pool;
+};
+
+// Brackets one kernel launch with events when profiling is on. `main` marks the hot kernel of
+// the call (the triangle / node loop), aux everything else (finish, epilogues).
+struct LaunchTimer {
+  Context* c;
+  bool main;
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  LaunchTimer(Context* ctx, bool is_main);
+  ~LaunchTimer();
+};
+
+// ----- error plumbing ----------------------------------------------------------------------
+
+int set_error(Context* c, int code, const char* fmt, ...);
+
+#define HGX_CHECK_HIP(ctx, expr)                                                              \
+  do {                                                                                        \
+    hipError_t e__ = (expr);                                                                  \
+    if (e__ != hipSuccess)                                                                    \
+      return ::hgx::set_error((ctx), HGX_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr,          \
+                              hipGetErrorString(e__), __FILE__, __LINE__);                     \
+  } while (0)
+
+// launches are followed by hipGetLastError so a bad launch is reported on the call that made it
+#define HGX_CHECK_LAUNCH(ctx) HGX_CHECK_HIP(ctx, hipGetLastError())
+
+int ensure_work(Context* c, size_t bytes);
+int set_device(Context* c);
+
+// ----- device helpers ------------------------------------------------------------------------
+
+struct cd {
+  double r, i;
+};
+
+__host__ __device__ inline cd mk(double r, double i) { return cd{r, i}; }
+__host__ __device__ inline cd operator+(cd a, cd b) { return cd{a.r + b.r, a.i + b.i}; }
+__host__ __device__ inline cd operator-(cd a, cd b) { return cd{a.r - b.r, a.i - b.i}; }
+__host__ __device__ inline cd operator-(cd a) { return cd{-a.r, -a.i}; }
+__host__ __device__ inline cd operator*(cd a, cd b) { return cd{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__host__ __device__ inline cd operator*(double s, cd a) { return cd{s * a.r, s * a.i}; }
+__host__ __device__ inline cd operator*(cd a, double s) { return cd{s * a.r, s * a.i}; }
+__host__ __device__ inline double norm2(cd a) { return a.r * a.r + a.i * a.i; }
+__host__ __device__ inline cd conj(cd a) { return cd{a.r, -a.i}; }
+__device__ inline cd cdiv(cd a, cd b) {
+  // Smith's algorithm keeps the quotient finite for widely scaled operands
+  if (fabs(b.r) >= fabs(b.i)) {
+    double t = b.i / b.r, d = b.r + b.i * t;
+    return cd{(a.r + a.i * t) / d, (a.i - a.r * t) / d};
+  }
+  double t = b.r / b.i, d = b.r * t + b.i;
+  return cd{(a.r * t + a.i) / d, (a.i * t - a.r) / d};
+}
+__device__ inline cd mul_i(cd a) { return cd{-a.i, a.r}; }      // i * a
+__device__ inline cd mul_mi(cd a) { return cd{a.i, -a.r}; }     // -i * a
+__device__ inline cd cexpi_d(cd t) {                             // exp(i t)
+  double s, c;
+  sincos(t.r, &s, &c);
+  double m = exp(-t.i);
+  return cd{m * c, m * s};
+}
+__device__ inline cd cexp_d(cd z) {
+  double s, c;
+  sincos(z.i, &s, &c);
+  double m = exp(z.r);
+  return cd{m * c, m * s};
+}
+__device__ inline cd csqrt_d(cd z) {
+  // principal branch, as std::sqrt(std::complex)
+  double a = hypot(z.r, z.i);
+  if (a == 0.0) return cd{0.0, 0.0};
+  if (z.r >= 0.0) {
+    double t = sqrt(0.5 * (a + z.r));
+    return cd{t, z.i / (2.0 * t)};
+  }
+  double t = sqrt(0.5 * (a - z.r));
+  return cd{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
+}
+__device__ inline cd csin_d(cd z) {  // sin(a+ib) = sin a cosh b + i cos a sinh b
+  double s, c;
+  sincos(z.r, &s, &c);
+  return cd{s * cosh(z.i), c * sinh(z.i)};
+}
+__device__ inline cd ccos_d(cd z) {  // cos(a+ib) = cos a cosh b - i sin a sinh b
+  double s, c;
+  sincos(z.r, &s, &c);
+  return cd{c * cosh(z.i), -s * sinh(z.i)};
+}
+__device__ inline double cabs_d(cd z) { return hypot(z.r, z.i); }
+
+// Rotation of a q-point: mq = R (qx, qy, qz) with complex qz (matrix.hpp:123-130).
+struct Rot {
+  double m[9];
+};
+__device__ inline void rotate_q(const Rot& R, double qx, double qy, cd qz, cd mq[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    mq[j].r = R.m[3 * j] * qx + R.m[3 * j + 1] * qy + R.m[3 * j + 2] * qz.r;
+    mq[j].i = R.m[3 * j + 2] * qz.i;
+  }
+}
+
+struct Vec3 {
+  double v[3];
+};
+
+inline Rot make_rot(const double* r) {
+  Rot R;
+  for (int i = 0; i < 9; ++i) R.m[i] = r ? r[i] : ((i % 4 == 0) ? 1.0 : 0.0);
+  return R;
+}
+inline Vec3 make_vec(const double* v) {
+  Vec3 o;
+  for (int i = 0; i < 3; ++i) o.v[i] = v ? v[i] : 0.0;
+  return o;
+}
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace hgx

@@ -1,0 +1,200 @@
+/* hgx.h -- C ABI of the MI355X (gfx950) GISAXS form-factor / structure-factor / DWBA engine.
+ *
+ * Plain C: no torch, no STL, no HIP types in the signatures. Every compute entry point writes into
+ * caller-owned DEVICE memory (a pointer from hipMalloc or any framework allocator on the context's
+ * device) and enqueues its kernels on the context's stream; nothing blocks unless documented.
+ * Inputs given as host pointers are small (rotations, parameter lists, layer stacks) or are
+ * uploaded once (q-grids, meshes) and persist in the context.
+ *
+ * Conventions (from the reference, paths relative to the reference root):
+ *   - q-grid: per-pixel real qx[npix], qy[npix]; complex qz[nq] with nq = nbranch * npix
+ *     (nbranch = 4 for DWBA, 1 for Born/SAXS); q-point i <-> pixel i % npix, branch i / npix
+ *     (src/model/qgrid.cpp:102-127). Row r of the detector holds pixels [r*ncol, (r+1)*ncol).
+ *   - complex arrays are interleaved double pairs (re, im): a complex[nq] output is double[2*nq].
+ *   - rot is a row-major 3x3 matrix applied to q: mq = rot * q (include/numerics/matrix.hpp:123-130).
+ *   - lengths in nm, q in 1/nm, angles in radians unless a parameter says degrees.
+ *
+ * Error handling: every function returns HGX_OK (0) or a negative HGX_ERR_* code and never exits
+ * the process; hgx_last_error(ctx) returns a message for the last failure on that context.
+ * A context is not thread-safe; distinct contexts are independent (no global state).
+ */
+#ifndef HGX_H_
+#define HGX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HGX_OK 0
+#define HGX_ERR_INVALID_ARG (-1)
+#define HGX_ERR_OOM (-2)
+#define HGX_ERR_DEVICE (-3)
+#define HGX_ERR_NO_QGRID (-4)
+#define HGX_ERR_UNSUPPORTED (-5)
+#define HGX_ERR_BAD_HANDLE (-6)
+
+/* precision of a compute call */
+#define HGX_PREC_FP64 0  /* plain: FP64 throughout */
+#define HGX_PREC_FP32C 1 /* FP32 bulk, compensated accumulation, FP64 where cancellation needs it */
+
+/* mesh record kinds */
+#define HGX_MESH_APPROX8 0 /* 8 doubles per triangle: area, nx, ny, nz, cx, cy, cz, pad */
+#define HGX_MESH_EXACT9 1  /* 9 doubles per triangle: v1, v2, v3 (counter-clockwise, outward) */
+
+/* numeric form-factor methods */
+#define HGX_FF_APPROX 0 /* centroid rule */
+#define HGX_FF_EXACT 1  /* exact flat-triangle integral */
+
+/* flags for hgx_ff_numeric */
+#define HGX_FLAG_EXACT_REFERENCE_SIGN 1 /* exact method with the reference's exp(-iq.r) sign */
+#define HGX_FLAG_ACC_KAHAN 2            /* FP32C: per-pair Kahan instead of FP32-block/FP64-carry */
+
+/* flags for hgx_sf_lattice */
+#define HGX_SF_COMPAT_LB2 1 /* reproduce the reference's lb[2]*lb[2] term (src/sf/sf.cpp:212) */
+
+/* analytic shapes */
+#define HGX_SHAPE_SPHERE 0   /* dims: [radius] */
+#define HGX_SHAPE_CYLINDER 1 /* dims: [radius, height] */
+#define HGX_SHAPE_BOX 2      /* dims: [xsize, ysize, height] */
+#define HGX_SHAPE_CUBE 3     /* dims: [edge] */
+#define HGX_SHAPE_PRISM3 4   /* dims: [edge, height]; uses tau, eta */
+#define HGX_SHAPE_PRISM6 5   /* dims: [edge, height]; uses tau, eta */
+#define HGX_SHAPE_PYRAMID 6  /* dims: [xsize, ysize, height, baseangle in degrees] */
+
+#define HGX_MAX_NODES 40 /* distribution nodes per dimension */
+
+typedef struct hgx_ctx hgx_ctx;
+
+/* Distribution nodes of one analytic shape: dimension d has n[d] node values val[d][0..n-1] with
+ * weights wt[d][0..n-1]; the form factor is sum over the node grid of prod_d wt * F(nodes).
+ * (src/ff/ff_ana.cpp:235-284 builds such lists; the node grid per shape is the reference's loop
+ * nest, e.g. src/ff/ff_ana_pyramid.cpp:133-143.) Host pointers. */
+typedef struct hgx_shape_params {
+  int n[4];
+  const double* val[4];
+  const double* wt[4];
+  double tau; /* prism tilt angles (radians), src/ff/ff_ana_prism.cpp:94 */
+  double eta;
+} hgx_shape_params;
+
+/* ---- context ----------------------------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` with its own non-blocking stream.
+ * Replaces the implicit global state of the reference's GPU classes (init_gpu,
+ * include/init/gpu/init_gpu.cuh:25-32; __constant__ symbols src/ff/gpu/ff_ana_gpu.cu:30-32). */
+int hgx_create(int device, hgx_ctx** out);
+int hgx_destroy(hgx_ctx* ctx);
+/* Use an external hipStream_t (passed as void*) for all further work; NULL restores the own stream. */
+int hgx_set_stream(hgx_ctx* ctx, void* hip_stream);
+int hgx_synchronize(hgx_ctx* ctx);
+const char* hgx_last_error(const hgx_ctx* ctx);
+const char* hgx_version(void);
+/* Launch timing with HIP events recorded on the context's stream around every kernel launch made
+ * by later calls: "main" kernels are the hot loops (triangle / node / lattice loops), "aux" the
+ * finishing and setup kernels. hgx_profile_end synchronises the stream and returns the summed
+ * durations (ms) and launch counts since hgx_profile_begin. Replaces the reference's per-call
+ * CUDA-event timing (src/ff/gpu/ff_tri_gpu.cu:413-415, 508-512). */
+int hgx_profile_begin(hgx_ctx* ctx);
+int hgx_profile_end(hgx_ctx* ctx, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches);
+
+/* ---- q-grid (persistent per context) ----------------------------------------------------- */
+
+/* QGrid::create, q-space region (src/model/qgrid.cpp:30-99), built on the device.
+ * nbranch becomes 1 (qz real); call hgx_qz_extended for the DWBA branches. */
+int hgx_qgrid_qspace(hgx_ctx* ctx, int ncol, int nrow, double qy_min, double qz_min, double qy_max,
+                     double qz_max, double alpha_i, double k0);
+/* Rectangular grid qx = 0, qy over columns, qz over rows (row 0 = highest qz). No reference
+ * counterpart; the reference's region_qspace always derives qx from the Ewald sphere. */
+int hgx_qgrid_explicit(hgx_ctx* ctx, int ncol, int nrow, double qy_min, double qz_min, double qy_max,
+                       double qz_max, double alpha_i, double k0);
+/* Flat detector: 2theta = atan((col - col_center) p / D), alpha_f = atan((nrow-1-row + 0.5 -
+ * row_horizon) p / D), then the Ewald-sphere q of src/model/qgrid.cpp:76-84. */
+int hgx_qgrid_detector(hgx_ctx* ctx, int ncol, int nrow, double pixel, double distance, double col_center,
+                       double row_horizon, double alpha_i, double k0);
+/* Explicit upload of a grid from host arrays: qx, qy [npix]; qz complex [nbranch*npix];
+ * alpha_f [nrow] may be NULL (then hgx_fresnel is unavailable). npix = nrow * ncol. */
+int hgx_qgrid_set(hgx_ctx* ctx, int ncol, int nrow, int nbranch, const double* qx, const double* qy,
+                  const double* qz_ext, const double* alpha_f);
+/* QGrid::create_qz_extended (src/model/qgrid.cpp:102-127): replaces qz by the 4 DWBA branches
+ * [kzf-kzi, -kzf-kzi, kzf+kzi, -kzf+kzi] for a particle in a layer with 1-n^2 = dnl. */
+int hgx_qz_extended(hgx_ctx* ctx, double k0, double alpha_i, double dnl_re, double dnl_im);
+int hgx_qgrid_info(const hgx_ctx* ctx, int64_t* npix, int64_t* nq, int* nrow, int* ncol, int* nbranch);
+/* Copies the grid back to host arrays (any pointer may be NULL). */
+int hgx_qgrid_get(hgx_ctx* ctx, double* qx, double* qy, double* qz_ext, double* alpha_f);
+
+/* ---- meshes ------------------------------------------------------------------------------- */
+
+/* Uploads a triangle mesh (host array of ntri records of the given kind) once; the reference
+ * re-reads the mesh file on every call instead (src/ff/ff_num.cpp:972-1095). Exact9 meshes can
+ * be evaluated with both methods (the approx records are derived as the OBJ reader does,
+ * src/file/objectshape_reader.cpp:271-300); approx8 meshes only with HGX_FF_APPROX. */
+int hgx_mesh_upload(hgx_ctx* ctx, int kind, int64_t ntri, const double* records, int* mesh_id);
+int hgx_mesh_release(hgx_ctx* ctx, int mesh_id);
+int hgx_mesh_info(const hgx_ctx* ctx, int mesh_id, int64_t* ntri, double* volume, double* radius);
+
+/* ---- form factors ------------------------------------------------------------------------- */
+
+/* Numeric (triangulated) form factor over the whole q-grid into ff[2*nq] (device).
+ *  HGX_FF_APPROX replaces NumericFormFactorC/G::compute_approx_triangle
+ *    (src/ff/cpu/ff_tri_cpu.cpp:200-246; src/ff/gpu/ff_tri_gpu.cu:395-521;
+ *     signature include/ff/gpu/ff_num_gpu.cuh:55-58);
+ *  HGX_FF_EXACT replaces compute_exact_triangle (src/ff/cpu/ff_tri_cpu.cpp:46-195).
+ * transvec (may be NULL) multiplies by exp(i q.t); the reference ignores the unit-cell location
+ * for numeric shapes (src/ff/ff.cpp:44), so pass NULL to reproduce it.
+ * At q = 0 the result is the mesh volume (the reference divides 0/0 there). */
+int hgx_ff_numeric(hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, const double rot[9],
+                   const double transvec[3], double* ff);
+
+/* Analytic form factor, replaces AnalyticFormFactorG::compute_<shape>
+ * (include/ff/gpu/ff_ana_gpu.cuh:54-94) with the CPU formulas of src/ff/ff_ana_<shape>.cpp. */
+int hgx_ff_analytic(hgx_ctx* ctx, int shape, const hgx_shape_params* params, int precision,
+                    const double rot[9], const double transvec[3], double* ff);
+
+/* ---- structure factor --------------------------------------------------------------------- */
+
+/* Finite-lattice structure factor into sf[2*nq], replaces StructureFactor::compute_structure_factor
+ * and StructureFactorG::compute (src/sf/sf.cpp:85-284; include/sf/gpu/sf_gpu.cuh:30-70).
+ * la, lb, lc: lattice vectors already multiplied by the grain scaling; repet: repetitions (values
+ * < 1 become 1); center: grain centre; dw_sigma: Debye-Waller sigma (0 = off; not in the
+ * reference). */
+int hgx_sf_lattice(hgx_ctx* ctx, const double la[3], const double lb[3], const double lc[3],
+                   const double repet[3], const double center[3], double dw_sigma, int flags,
+                   const double rot[9], double* sf);
+
+/* ---- Fresnel coefficients and DWBA -------------------------------------------------------- */
+
+/* MultiLayer::propagation_coeffs (src/model/multilayer.cpp:110-155) on the grid's alpha_f rows:
+ * fc[2*4*npix] = [Ti conj Tf, Ti conj Rf, Ri conj Tf, Ri conj Rf]. Layers below the vacuum are
+ * given top to bottom (substrate last) by delta, beta, thickness; order is the particle's layer
+ * (0 = vacuum). */
+int hgx_fresnel(hgx_ctx* ctx, int nl, const double* delta, const double* beta, const double* thickness,
+                int order, double k0, double alpha_i, double* fc);
+
+/* I[p] (+)= |weight * sum_b fc_b sf_b ff_b|^2 over the grid's branches (inline in the reference:
+ * src/sim/hipgisaxs_main.cpp:1084-1105). fc and/or sf may be NULL (treated as 1). */
+int hgx_dwba_intensity(hgx_ctx* ctx, const double* fc, const double* sf, const double* ff, double weight,
+                       int accumulate, double* I);
+
+/* ---- host helpers ------------------------------------------------------------------------- */
+
+/* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values of a shape
+ * parameter by the reference's accumulating do-while over [pmin, pmax] with nvalues points (which
+ * can yield nvalues +- 1 nodes), weights for stat 0 (none: one weight 1), 1 (uniform: 1 each),
+ * 2 (gaussian: unnormalised pdf, mean replaced by the node midpoint when not finite). Writes at
+ * most cap nodes; returns the node count or a negative error code. Host arithmetic only. */
+int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd, double* vals,
+                           double* wts, int cap);
+
+/* Launch-shape knobs of the numeric form-factor kernels (q-points per thread for the FP64 and
+ * FP32C kernels, each 1, 2, 4 or 8, and the workgroup count the triangle split aims for).
+ * Process-wide; set by the autotuner. No reference counterpart (the reference hard-codes its
+ * tiles, include/common/parameters.hpp:37-48). */
+int hgx_tune_numeric(int qpt64, int qpt32, int target_wg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGX_H_ */

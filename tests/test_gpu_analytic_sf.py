@@ -1,0 +1,122 @@
+"""GPU parity (FP64, relL2 <= 1e-10) of q-grids, analytic form factors, the lattice structure
+factor, Fresnel coefficients and the DWBA combination against the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+K0 = 2 * np.pi / (1.23984 / 12.0)
+AI = np.radians(0.15)
+
+
+def _gpu(ctx, t):
+    return ctx.as_complex(t).cpu().numpy()
+
+
+def _rot(az, ax=0.0):
+    c, s = np.cos(az), np.sin(az)
+    cx, sx = np.cos(ax), np.sin(ax)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]) @ np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+
+
+@pytest.fixture(scope="module")
+def det_grid(oracle_mod):
+    O = oracle_mod
+    g = O.qgrid_detector(96, 80, 0.172e6 * 8, 2.0e9, 47.5, 0.0, AI, K0)
+    return O.qz_extended(g, K0, AI, 0j)
+
+
+def _upload(ctx, g):
+    ctx.qgrid_set(g.ncol, g.nrow, g.qx, g.qy, g.qz, g.alpha_f)
+
+
+def test_qgrid_builders_match_oracle(gpu_ctx, oracle_mod):
+    O = oracle_mod
+    cases = [
+        ("qspace", (70, 50, -1.0, 0.0, 2.0, 2.5, np.radians(0.16), 50.67)),
+        ("explicit", (64, 48, -2.0, 0.0, 2.0, 2.0, np.radians(0.2), 50.67)),
+        ("detector", (64, 48, 0.172e6, 2.0e9, 31.5, 0.0, AI, K0)),
+    ]
+    for kind, args in cases:
+        getattr(gpu_ctx, f"qgrid_{kind}")(*args)
+        gx, gy, gz, ga = gpu_ctx.qgrid_get()
+        ref = getattr(O, f"qgrid_{kind}")(*args)
+        for a, b in ((gx, ref.qx), (gy, ref.qy), (gz.real, ref.qz.real), (ga, ref.alpha_f)):
+            assert np.max(np.abs(a - b)) <= 1e-13 * max(1.0, np.max(np.abs(b))), kind
+        gpu_ctx.qz_extended(50.67, np.radians(0.2), 2 * 4.88e-6 + 2j * 7.37e-8)
+        gx, gy, gz, ga = gpu_ctx.qgrid_get()
+        ref4 = O.qz_extended(ref, 50.67, np.radians(0.2), 2 * 4.88e-6 + 2j * 7.37e-8)
+        assert rel_l2(gz, ref4.qz) <= 1e-14
+
+
+SHAPES = {
+    "sphere": [(np.linspace(3, 7, 21), np.exp(-0.5 * ((np.linspace(3, 7, 21) - 5) / 0.5) ** 2))],
+    "cylinder": [([3.0, 4.0, 4.5], [0.2, 0.5, 0.3]), ([10.0, 12.0], [0.6, 0.4])],
+    "box": [([4.0, 5.0], [1, 1]), ([6.0], [1.0]), ([8.0, 9.0, 10.0], [0.2, 0.3, 0.5])],
+    "cube": [([5.0, 6.0], [0.5, 0.5])],
+    "prism3": [([6.0, 7.0], [1, 1]), ([5.0], [1.0])],
+    "prism6": [([6.0], [1.0]), ([5.0, 5.5], [1, 1])],
+    "pyramid": [([16.0], [1.0]), ([16.0, 17.0], [1, 1]), ([4.0], [1.0]), ([54.73], [1.0])],
+}
+
+
+@pytest.mark.parametrize("shape", sorted(SHAPES))
+def test_analytic_fp64_matches_oracle(gpu_ctx, oracle_mod, det_grid, shape):
+    O = oracle_mod
+    _upload(gpu_ctx, det_grid)
+    R = _rot(0.3, 0.05)
+    tv = np.array([1.0, -2.0, 0.5])
+    ref = O.ff_analytic(shape, SHAPES[shape], det_grid, R, tv, tau=0.1, eta=0.2)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_analytic(shape, SHAPES[shape], "fp64", rot=R, transvec=tv, tau=0.1, eta=0.2))
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("lattice", ["bcc", "hcp"])
+@pytest.mark.parametrize("compat", [0, 1])
+def test_sf_lattice_matches_oracle(gpu_ctx, oracle_mod, det_grid, lattice, compat):
+    O = oracle_mod
+    _upload(gpu_ctx, det_grid)
+    a = 20.0 if lattice == "bcc" else 15.0
+    vecs = {"bcc": ([1, 0, 0], [0, 1, 0], [0.5, 0.5, 0.5]), "hcp": ([1, 0, 0], [0.5, 0.866, 0], [0.5, 0.2887, 0.8165])}
+    la, lb, lc = (a * np.array(v, dtype=float) for v in vecs[lattice])
+    rep = (10, 10, 5) if lattice == "bcc" else (20, 20, 1)
+    R = _rot(np.pi / 4)
+    ce = np.array([0.5, 1.0, -2.0])
+    ref = O.sf_lattice(la, lb, lc, rep, ce, det_grid, dw_sigma=0.5, flags=compat, rot=R)
+    got = _gpu(gpu_ctx, gpu_ctx.sf_lattice(la, lb, lc, rep, ce, dw_sigma=0.5, flags=compat, rot=R))
+    assert rel_l2(got, ref) <= 1e-10
+
+
+def test_fresnel_matches_oracle(gpu_ctx, oracle_mod, det_grid):
+    O = oracle_mod
+    _upload(gpu_ctx, det_grid)
+    cases = [
+        ([4.88e-6], [7.37e-8], [0.0], 0),
+        ([3.0e-6, 4.88e-6], [1e-8, 7.37e-8], [25.0, 0.0], 0),
+        ([3.0e-6, 6.0e-6, 4.88e-6], [1e-8, 2e-8, 7.37e-8], [25.0, 10.0, 0.0], 1),
+    ]
+    for d, b, t, order in cases:
+        ref = O.fresnel(det_grid, d, b, t, order, K0, AI)
+        got = _gpu(gpu_ctx, gpu_ctx.fresnel(d, b, t, order, K0, AI))
+        assert rel_l2(got, ref) <= 1e-12
+
+
+def test_dwba_matches_oracle(gpu_ctx, oracle_mod, det_grid):
+    O = oracle_mod
+    _upload(gpu_ctx, det_grid)
+    import torch
+
+    rng = np.random.default_rng(3)
+    nq = det_grid.nq
+    f = rng.standard_normal(nq) + 1j * rng.standard_normal(nq)
+    s = rng.standard_normal(nq) + 1j * rng.standard_normal(nq)
+    fc = rng.standard_normal(nq) + 1j * rng.standard_normal(nq)
+    dev = lambda z: torch.from_numpy(np.stack([z.real, z.imag], -1).ravel().copy()).cuda()
+    I0 = rng.uniform(0, 1, det_grid.npix)
+    ref = O.dwba_intensity(det_grid.npix, 4, fc, s, f, 0.7, I0)
+    out = torch.from_numpy(I0.copy()).cuda()
+    got = gpu_ctx.dwba_intensity(dev(f), dev(fc), dev(s), weight=0.7, accumulate=True, out=out).cpu().numpy()
+    assert rel_l2(got, ref) <= 1e-14

@@ -1,0 +1,187 @@
+"""GPU parity of the numeric (triangulated) form factors against the FP64 oracle, through the C ABI.
+
+Bars: plain FP64 path relL2 <= 1e-10; FP32C path relL2 <= 1e-5 (the stated FP32 tolerance).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+K0_10KEV = 2 * np.pi / (1.23984 / 10.0)
+ALPHA_I = np.radians(0.2)
+
+
+def _rot(ax, ay, az):
+    cx, sx, cy, sy, cz, sz = np.cos(ax), np.sin(ax), np.cos(ay), np.sin(ay), np.cos(az), np.sin(az)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, -sy], [0, 1, 0], [sy, 0, cy]])  # include/numerics/matrix.hpp:53-56
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def _pyramid_grid(O, n=48, dwba=True, dnl=0j):
+    g = O.qgrid_explicit(n, n, -2.0, 0.0, 2.0, 2.0, ALPHA_I, K0_10KEV)
+    return O.qz_extended(g, K0_10KEV, ALPHA_I, dnl) if dwba else g
+
+
+def _upload_grid(ctx, g):
+    ctx.qgrid_set(g.ncol, g.nrow, g.qx, g.qy, g.qz, g.alpha_f)
+
+
+def _gpu(ctx, t):
+    return ctx.as_complex(t).cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def pyramid_case(gpu_ctx, oracle_mod):
+    from hipgisaxs_amd import meshes
+
+    tri = meshes.pyramid()
+    mid = gpu_ctx.upload_mesh(tri, "exact9")
+    return tri, mid
+
+
+def test_mesh_upload_volume(gpu_ctx, pyramid_case):
+    tri, mid = pyramid_case
+    info = gpu_ctx.mesh_info(mid)
+    assert info["ntri"] == 2048
+    assert abs(info["volume"] - 4000.0 / 3.0) < 1e-9
+
+
+@pytest.mark.parametrize("rot", [None, (0.3, -0.2, 0.7)])
+def test_approx_fp64_matches_oracle(gpu_ctx, oracle_mod, pyramid_case, rot):
+    O = oracle_mod
+    tri, mid = pyramid_case
+    g = _pyramid_grid(O)
+    _upload_grid(gpu_ctx, g)
+    R = None if rot is None else _rot(*rot)
+    ref = O.ff_approx(O.tri9_to_rec8(tri), g, R)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64", rot=R))
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("kahan", [False, True])
+def test_approx_fp32c_matches_oracle(gpu_ctx, oracle_mod, pyramid_case, kahan):
+    O = oracle_mod
+    tri, mid = pyramid_case
+    g = _pyramid_grid(O)
+    _upload_grid(gpu_ctx, g)
+    ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp32c", flags=2 if kahan else 0))
+    err = rel_l2(got, ref)
+    print(f"fp32c kahan={kahan} relL2={err:.3e}")
+    assert err <= 1e-5
+
+
+def test_approx_complex_q(gpu_ctx, oracle_mod, pyramid_case):
+    """Particle in an absorbing layer: complex qz branches (qgrid.cpp:102-127)."""
+    O = oracle_mod
+    tri, mid = pyramid_case
+    g = _pyramid_grid(O, n=32, dnl=2 * 4.88e-6 + 2j * 7.37e-8)
+    assert np.any(g.qz.imag != 0)
+    _upload_grid(gpu_ctx, g)
+    R = _rot(0.1, 0.4, -0.3)
+    ref = O.ff_approx(O.tri9_to_rec8(tri), g, R)
+    got64 = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64", rot=R))
+    got32 = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp32c", rot=R))
+    assert rel_l2(got64, ref) <= 1e-10
+    assert rel_l2(got32, ref) <= 1e-5
+
+
+def test_approx_q_zero_is_volume(gpu_ctx, oracle_mod, pyramid_case):
+    O = oracle_mod
+    tri, mid = pyramid_case
+    qx = np.array([0.0, 1e-4, 0.0, 0.003])
+    qy = np.array([0.0, 0.0, 2e-4, 0.004])
+    qz = np.array([0.0, 0.0, 1e-4, 0.001]) + 0j
+    g = O.Grid(4, 1, qx, qy, qz)
+    _upload_grid(gpu_ctx, g)
+    for prec, tol in (("fp64", 1e-9), ("fp32c", 1e-6)):
+        got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", prec))
+        assert abs(got[0] - 4000.0 / 3.0) < 1e-9 * 4000
+        ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+        assert rel_l2(got[1:], ref[1:]) <= tol, prec
+
+
+def test_approx_translation_phase(gpu_ctx, oracle_mod, pyramid_case):
+    O = oracle_mod
+    tri, mid = pyramid_case
+    g = _pyramid_grid(O, n=24)
+    _upload_grid(gpu_ctx, g)
+    t = np.array([3.0, -2.0, 1.5])
+    base = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64"))
+    moved = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64", transvec=t))
+    q = g.q_rotated()
+    assert rel_l2(moved, base * np.exp(1j * (t @ q))) <= 1e-12
+
+
+def test_chunked_reduction_matches_single_pass(gpu_ctx, oracle_mod, pyramid_case):
+    """Forces the triangle split (partials + finish kernel) and the single pass; both agree."""
+    from hipgisaxs_amd.api import tune_numeric
+
+    O = oracle_mod
+    tri, mid = pyramid_case
+    g = _pyramid_grid(O, n=40)
+    _upload_grid(gpu_ctx, g)
+    ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+    try:
+        for qpt64, qpt32, twg in ((1, 1, 1), (2, 4, 100000), (8, 8, 64)):
+            tune_numeric(qpt64, qpt32, twg)
+            a = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64"))
+            b = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp32c"))
+            assert rel_l2(a, ref) <= 1e-10
+            assert rel_l2(b, ref) <= 1e-5
+    finally:
+        tune_numeric(2, 4, 2048)
+
+
+def test_exact_fp64_box_and_pyramid(gpu_ctx, oracle_mod):
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    g = _pyramid_grid(O, n=32)
+    _upload_grid(gpu_ctx, g)
+    for tri in (meshes.box(4.0, 6.0, 8.0), meshes.pyramid(side_levels=2, base_nx=8, base_ny=4)):
+        mid = gpu_ctx.upload_mesh(tri, "exact9")
+        R = _rot(0.2, 0.1, -0.5)
+        ref = O.ff_exact(tri, g, R)
+        got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64", rot=R))
+        assert rel_l2(got, ref) <= 1e-10
+        ref_m = O.ff_exact(tri, g, R, sign=-1)
+        got_m = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64", rot=R, flags=1))
+        assert rel_l2(got_m, ref_m) <= 1e-10
+        gpu_ctx.release_mesh(mid)
+
+
+def test_exact_box_matches_analytic_box(gpu_ctx, oracle_mod):
+    """External truth: the exact method on a 12-triangle box is the analytic box."""
+    from hipgisaxs_amd import meshes
+
+    rng = np.random.default_rng(7)
+    n = 64
+    qx, qy, qz = rng.uniform(-2, 2, (3, n))
+    g = oracle_mod.Grid(n, 1, qx, qy, qz + 0j)
+    _upload_grid(gpu_ctx, g)
+    L, W, H = 4.0, 6.0, 8.0
+    mid = gpu_ctx.upload_mesh(meshes.box(L, W, H), "exact9")
+    got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64"))
+    s = lambda x: np.sinc(x / np.pi)
+    ana = L * W * H * s(qx * L / 2) * s(qy * W / 2) * s(qz * H / 2) * np.exp(1j * qz * H / 2)
+    assert rel_l2(got, ana) <= 1e-9
+    gpu_ctx.release_mesh(mid)
+
+
+def test_invalid_arguments_return_codes(gpu_ctx):
+    from hipgisaxs_amd.api import HgxError
+
+    with pytest.raises(HgxError):
+        gpu_ctx.ff_numeric(12345, "approx", "fp64")
+    with pytest.raises(HgxError):
+        gpu_ctx.upload_mesh(np.zeros((3, 9)), "exact9")  # all degenerate
+    bad = np.ones((1, 9))
+    bad[0, 4] = np.nan
+    with pytest.raises(HgxError):
+        gpu_ctx.upload_mesh(bad, "exact9")
