@@ -40,6 +40,7 @@ class ShapeParams(C.Structure):
         ("wt", _dp * 4),
         ("tau", C.c_double),
         ("eta", C.c_double),
+        ("flags", C.c_int),
     ]
 
 
@@ -49,6 +50,7 @@ _SIGS = {
     "hgx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "hgx_destroy": (C.c_int, [_vp]),
     "hgx_set_stream": (C.c_int, [_vp, _vp]),
+    "hgx_use_own_stream": (C.c_int, [_vp]),
     "hgx_synchronize": (C.c_int, [_vp]),
     "hgx_last_error": (C.c_char_p, [_vp]),
     "hgx_profile_begin": (C.c_int, [_vp]),

@@ -169,7 +169,7 @@ class Context:
         return o
 
     def ff_analytic(self, shape: str, dims: Sequence, precision="fp64", rot=None, transvec=None, tau=0.0, eta=0.0,
-                    out=None):
+                    flags=0, out=None):
         """dims: per shape dimension a (values, weights) pair of 1-D arrays."""
         sp = N.ShapeParams()
         keep = []
@@ -181,7 +181,7 @@ class Context:
             sp.n[d] = v.size
             sp.val[d] = v.ctypes.data_as(C.POINTER(C.c_double))
             sp.wt[d] = w.ctypes.data_as(C.POINTER(C.c_double))
-        sp.tau, sp.eta = tau, eta
+        sp.tau, sp.eta, sp.flags = tau, eta, flags
         nq = self.info()["nq"]
         o = self._out(2 * nq, out)
         R = None if rot is None else _f64(rot).ravel()

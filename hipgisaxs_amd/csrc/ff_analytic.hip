@@ -15,6 +15,7 @@ struct Nodes {
   double val[4][HGX_MAX_NODES];
   double wt[4][HGX_MAX_NODES];
   double tan_tau, sin_eta, cos_eta;
+  int flags;
 };
 
 struct QViewA {
@@ -166,8 +167,11 @@ __device__ cd shape_ff<HGX_SHAPE_PRISM6>(const cd mq[3], const Nodes& P) {
   return acc;
 }
 
-// truncated pyramid, src/ff/ff_ana_pyramid.cpp:35-74
-__device__ cd pyramid_one(cd qx, cd qy, cd qz, double length, double width, double height, double angle) {
+// truncated pyramid, src/ff/ff_ana_pyramid.cpp:35-74. By default the exponentials are
+// exp(+i q1 H), exp(-i q2 H) (published form, the pyramid's volume integral); compat flips them as
+// the reference does (:47-60) without flipping the +-i factors of k2, k4.
+__device__ cd pyramid_one(cd qx, cd qy, cd qz, double length, double width, double height, double angle,
+                          bool compat) {
   double a = angle * kPi / 180.;
   double tan_a = tan(a);
   if ((2 * height / length >= tan_a) || (2 * height / width >= tan_a)) return cd{0.0, 0.0};
@@ -177,11 +181,11 @@ __device__ cd pyramid_one(cd qx, cd qy, cd qz, double length, double width, doub
   cd q2 = 0.5 * ((1.0 / tan_a) * (qx - qy) - qz);
   cd q3 = 0.5 * ((1.0 / tan_a) * (qx + qy) + qz);
   cd q4 = 0.5 * ((1.0 / tan_a) * (qx + qy) - qz);
-  // sinc(qh) exp(-/+ i qh)
-  cd a1 = sinc_free(q1 * height) * cexpi_d(-(q1 * height));
-  cd a2 = sinc_free(q2 * height) * cexpi_d(q2 * height);
-  cd a3 = sinc_free(q3 * height) * cexpi_d(-(q3 * height));
-  cd a4 = sinc_free(q4 * height) * cexpi_d(q4 * height);
+  const double e1 = compat ? -height : height;
+  cd a1 = sinc_free(q1 * height) * cexpi_d(e1 * q1);
+  cd a2 = sinc_free(q2 * height) * cexpi_d(-e1 * q2);
+  cd a3 = sinc_free(q3 * height) * cexpi_d(e1 * q3);
+  cd a4 = sinc_free(q4 * height) * cexpi_d(-e1 * q4);
   cd k1 = a1 + a2;
   cd k2 = mul_mi(a1) + mul_i(a2);
   cd k3 = a3 + a4;
@@ -203,7 +207,8 @@ __device__ cd shape_ff<HGX_SHAPE_PYRAMID>(const cd mq[3], const Nodes& P) {
       for (int ih = 0; ih < P.n[2]; ++ih)
         for (int ib = 0; ib < P.n[3]; ++ib) {
           double prob = P.wt[0][ix] * P.wt[1][iy] * P.wt[2][ih] * P.wt[3][ib];
-          acc = acc + prob * pyramid_one(mq[0], mq[1], mq[2], P.val[0][ix], P.val[1][iy], P.val[2][ih], P.val[3][ib]);
+          acc = acc + prob * pyramid_one(mq[0], mq[1], mq[2], P.val[0][ix], P.val[1][iy], P.val[2][ih], P.val[3][ib],
+                                         (P.flags & HGX_SHAPE_COMPAT_PYRAMID) != 0);
         }
   return acc;
 }
@@ -254,6 +259,7 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   P.tan_tau = std::tan(params->tau);
   P.sin_eta = std::sin(params->eta);
   P.cos_eta = std::cos(params->eta);
+  P.flags = params->flags;
   if (int rc = set_device(c)) return rc;
   QViewA g{c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq};
   Rot R = make_rot(rot);

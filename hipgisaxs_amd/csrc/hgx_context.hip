@@ -236,7 +236,15 @@ int hgx_destroy(hgx_ctx* h) {
 int hgx_set_stream(hgx_ctx* h, void* stream) {
   Context* c = ctx_of(h);
   if (!c) return HGX_ERR_BAD_HANDLE;
-  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
+  // NULL is the device's null stream (what PyTorch's default stream is), not "own stream"
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  return HGX_OK;
+}
+
+int hgx_use_own_stream(hgx_ctx* h) {
+  Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  c->stream = c->own_stream;
   return HGX_OK;
 }
 

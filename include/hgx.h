@@ -77,7 +77,13 @@ typedef struct hgx_shape_params {
   const double* wt[4];
   double tau; /* prism tilt angles (radians), src/ff/ff_ana_prism.cpp:94 */
   double eta;
+  int flags;  /* HGX_SHAPE_COMPAT_* bits */
 } hgx_shape_params;
+
+/* Pyramid: evaluate the reference's k1..k4 (src/ff/ff_ana_pyramid.cpp:47-60), whose exponentials
+ * have the opposite sign of the published truncated-pyramid form factor while the +-i factors keep
+ * theirs; by default the published form (the volume integral of the pyramid) is used. */
+#define HGX_SHAPE_COMPAT_PYRAMID 1
 
 /* ---- context ----------------------------------------------------------------------------- */
 
@@ -86,8 +92,10 @@ typedef struct hgx_shape_params {
  * include/init/gpu/init_gpu.cuh:25-32; __constant__ symbols src/ff/gpu/ff_ana_gpu.cu:30-32). */
 int hgx_create(int device, hgx_ctx** out);
 int hgx_destroy(hgx_ctx* ctx);
-/* Use an external hipStream_t (passed as void*) for all further work; NULL restores the own stream. */
+/* Use an external hipStream_t (passed as void*) for all further work; NULL is the null stream. */
 int hgx_set_stream(hgx_ctx* ctx, void* hip_stream);
+/* Go back to the context's own non-blocking stream. */
+int hgx_use_own_stream(hgx_ctx* ctx);
 int hgx_synchronize(hgx_ctx* ctx);
 const char* hgx_last_error(const hgx_ctx* ctx);
 const char* hgx_version(void);

@@ -35,10 +35,11 @@ def load():
             "hgxo_ff_cylinder": (None, [_i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_box": (None, [_i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
                                    _vp]),
-            "hgxo_ff_prism3": (None, [_i, _vp, _vp, _i, _vp, _vp, _d, _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "hgxo_ff_cube": (None, [_i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "hgxo_ff_prism3":(None, [_i, _vp, _vp, _i, _vp, _vp, _d, _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_prism6": (None, [_i, _vp, _vp, _i, _vp, _vp, _d, _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
-            "hgxo_ff_pyramid": (None, [_i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp,
-                                       _vp, _vp, _vp, _vp]),
+            "hgxo_ff_pyramid": (None, [_i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i, _i64, _i64, _vp,
+                                       _vp, _vp, _vp, _vp, _vp]),
             "hgxo_sf_lattice": (None, [_vp, _vp, _vp, _vp, _vp, _d, _i, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_dwba_intensity": (None, [_i64, _i, _vp, _vp, _vp, _d, _vp]),
             "hgxo_tri9_to_rec8": (_i64, [_i64, _vp, _vp]),
@@ -176,7 +177,7 @@ def _dims(dims):
     return out
 
 
-def ff_analytic(shape: str, dims, g: Grid, rot=None, transvec=None, tau=0.0, eta=0.0) -> np.ndarray:
+def ff_analytic(shape: str, dims, g: Grid, rot=None, transvec=None, tau=0.0, eta=0.0, compat=0) -> np.ndarray:
     npix, nq, qx, qy, _, R = _qargs(g, rot)
     qz = _c2r(g.qz)
     tv = None if transvec is None else _a(transvec)
@@ -194,14 +195,13 @@ def ff_analytic(shape: str, dims, g: Grid, rot=None, transvec=None, tau=0.0, eta
     elif shape == "box":
         lib.hgxo_ff_box(*flat, *tail)
     elif shape == "cube":
-        n, v, w = D[0]
-        lib.hgxo_ff_box(n, _p(v), _p(w), n, _p(v), _p(w), n, _p(v), _p(w), *tail)
+        lib.hgxo_ff_cube(*flat, *tail)
     elif shape == "prism3":
         lib.hgxo_ff_prism3(*flat, tau, eta, *tail)
     elif shape == "prism6":
         lib.hgxo_ff_prism6(*flat, tau, eta, *tail)
     elif shape == "pyramid":
-        lib.hgxo_ff_pyramid(*flat, *tail)
+        lib.hgxo_ff_pyramid(*flat, int(compat), *tail)
     else:
         raise ValueError(shape)
     return _r2c(out)

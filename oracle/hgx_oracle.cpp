@@ -503,6 +503,28 @@ void hgxo_ff_box(int nx, const double* x, const double* wx, int ny, const double
   }
 }
 
+// a13: FormFactorCube / compute_cube (src/ff/ff_ana_cube.cpp:35-42, :103-113): one edge node list,
+// F = sum_i w_i L_i^3 sinc(qx L/2) sinc(qy L/2) sinc(qz L/2) exp(i qz L/2).
+// Pinned by: the box closed form with L = W = H.
+void hgxo_ff_cube(int nx, const double* x, const double* wx, int64_t npix, int64_t nq, const double* qx,
+                  const double* qy, const double* qz, const double* rot, const double* transvec, double* ff) {
+  const cplx I(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; ++i) {
+    int64_t j = i % npix;
+    cplx mq[3];
+    rotate_q(rot, qx[j], qy[j], qz_at(qz, i), mq);
+    cplx acc = 0.0;
+    for (int ix = 0; ix < nx; ++ix) {
+      double L = x[ix];
+      cplx ev = std::exp(I * 0.5 * mq[2] * L);
+      cplx sv = sinc_free(0.5 * mq[0] * L) * sinc_free(0.5 * mq[1] * L) * sinc_free(0.5 * mq[2] * L);
+      acc += (L * L * L) * ev * sv * wx[ix];
+    }
+    store(ff, i, acc * trans_phase(mq, transvec));
+  }
+}
+
 // a14: compute_prism, triangular prism (src/ff/ff_ana_prism.cpp:34-122; CPU formula :87-113).
 // Node sum carries weights w_l w_h (the reference CPU ignores the weights: pass unit weights).
 // Pinned by: the exact-triangle mesh of the same prism (tests).
@@ -568,24 +590,34 @@ void hgxo_ff_prism6(int nl, const double* l, const double* wl, int nh, const dou
 // a15: FormFactorPyramid / compute_pyramid (src/ff/ff_ana_pyramid.cpp:35-156), truncated square
 // pyramid; returns 0 when 2H/L or 2H/W >= tan(angle) (untruncated) and when |qx qy| < 1e-20.
 // Weighted 4-D node sum (:133-143). angle in degrees.
-// Pinned by: the exact-triangle mesh of the same truncated pyramid (tests).
-static cplx pyramid_one(cplx qx, cplx qy, cplx qz, double length, double width, double height, double angle) {
+// compat = 1 evaluates the reference's k1..k4 (:47-60), whose exponentials exp(-/+ i q_j H) carry
+// the opposite sign of the published truncated-pyramid form factor while the +-i factors of k2, k4
+// keep theirs, so it matches neither the pyramid nor its mirror image. compat = 0 (default) uses
+// exp(+/- i q_j H), the volume integral over the pyramid with base z = 0 and top z = H.
+// Pinned by: compat = 0 against Gauss-Legendre slice integration of the pyramid and against the
+// exact-triangle mesh (tests/test_oracle_analytic.py); compat = 1 only against a literal
+// transcription of the reference formula (parity unpinned by any external truth).
+static cplx pyramid_one(cplx qx, cplx qy, cplx qz, double length, double width, double height, double angle,
+                        int compat) {
   double a = angle * kPi / 180.;
   double tan_a = std::tan(a);
   if ((2 * height / length >= tan_a) || (2 * height / width >= tan_a)) return 0.0;
   cplx tmp = qx * qy;
   if (std::abs(tmp) < 1.0E-20) return 0.0;
   const cplx P_J(0, 1), N_J(0, -1);
+  const cplx E1 = compat ? N_J : P_J, E2 = compat ? P_J : N_J;
   cplx q1 = 0.5 * ((qx - qy) / tan_a + qz);
   cplx q2 = 0.5 * ((qx - qy) / tan_a - qz);
   cplx q3 = 0.5 * ((qx + qy) / tan_a + qz);
   cplx q4 = 0.5 * ((qx + qy) / tan_a - qz);
-  cplx k1 = sinc_free(q1 * height) * std::exp(N_J * q1 * height) + sinc_free(q2 * height) * std::exp(P_J * q2 * height);
-  cplx k2 = sinc_free(q1 * height) * std::exp(N_J * q1 * height) * N_J +
-            sinc_free(q2 * height) * std::exp(P_J * q2 * height) * P_J;
-  cplx k3 = sinc_free(q3 * height) * std::exp(N_J * q3 * height) + sinc_free(q4 * height) * std::exp(P_J * q4 * height);
-  cplx k4 = sinc_free(q3 * height) * std::exp(N_J * q3 * height) * N_J +
-            sinc_free(q4 * height) * std::exp(P_J * q4 * height) * P_J;
+  cplx a1 = sinc_free(q1 * height) * std::exp(E1 * q1 * height);
+  cplx a2 = sinc_free(q2 * height) * std::exp(E2 * q2 * height);
+  cplx a3 = sinc_free(q3 * height) * std::exp(E1 * q3 * height);
+  cplx a4 = sinc_free(q4 * height) * std::exp(E2 * q4 * height);
+  cplx k1 = a1 + a2;
+  cplx k2 = a1 * N_J + a2 * P_J;
+  cplx k3 = a3 + a4;
+  cplx k4 = a3 * N_J + a4 * P_J;
   cplx t1 = k1 * std::cos(0.5 * (qx * length - qy * width));
   cplx t2 = k2 * std::sin(0.5 * (qx * length - qy * width));
   cplx t3 = k3 * std::cos(0.5 * (qx * length + qy * width));
@@ -594,8 +626,8 @@ static cplx pyramid_one(cplx qx, cplx qy, cplx qz, double length, double width, 
 }
 
 void hgxo_ff_pyramid(int nx, const double* x, const double* wx, int ny, const double* yv, const double* wy, int nh,
-                     const double* h, const double* wh, int nb, const double* b, const double* wb, int64_t npix,
-                     int64_t nq, const double* qx, const double* qy, const double* qz, const double* rot,
+                     const double* h, const double* wh, int nb, const double* b, const double* wb, int compat,
+                     int64_t npix, int64_t nq, const double* qx, const double* qy, const double* qz, const double* rot,
                      const double* transvec, double* ff) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < nq; ++i) {
@@ -608,7 +640,7 @@ void hgxo_ff_pyramid(int nx, const double* x, const double* wx, int ny, const do
         for (int ih = 0; ih < nh; ++ih)
           for (int ib = 0; ib < nb; ++ib) {
             double prob = wx[ix] * wy[iy] * wh[ih] * wb[ib];
-            acc += pyramid_one(mq[0], mq[1], mq[2], x[ix], yv[iy], h[ih], b[ib]) * prob;
+            acc += pyramid_one(mq[0], mq[1], mq[2], x[ix], yv[iy], h[ih], b[ib], compat) * prob;
           }
     store(ff, i, acc * trans_phase(mq, transvec));
   }

@@ -74,6 +74,14 @@ def test_analytic_fp64_matches_oracle(gpu_ctx, oracle_mod, det_grid, shape):
     assert rel_l2(got, ref) <= 1e-10
 
 
+def test_pyramid_reference_compat_matches_oracle(gpu_ctx, oracle_mod, det_grid):
+    _upload(gpu_ctx, det_grid)
+    d = SHAPES["pyramid"]
+    ref = oracle_mod.ff_analytic("pyramid", d, det_grid, compat=1)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_analytic("pyramid", d, "fp64", flags=1))
+    assert rel_l2(got, ref) <= 1e-10
+
+
 @pytest.mark.parametrize("lattice", ["bcc", "hcp"])
 @pytest.mark.parametrize("compat", [0, 1])
 def test_sf_lattice_matches_oracle(gpu_ctx, oracle_mod, det_grid, lattice, compat):

@@ -1,0 +1,138 @@
+"""Pins the oracle's analytic form factors: scipy special functions, closed forms, and the exact
+triangle method on meshes of the same solids (whose own truth is pinned in test_oracle_ff.py)."""
+import numpy as np
+import pytest
+from scipy import special
+
+from conftest import rel_l2
+
+S = lambda x: np.sinc(x / np.pi)
+
+
+def _grid(O, n=50, seed=2, scale=1.5):
+    rng = np.random.default_rng(seed)
+    qx, qy, qz = rng.uniform(-scale, scale, (3, n))
+    return O.Grid(n, 1, qx, qy, qz + 0j), qx, qy, qz
+
+
+def test_sphere_is_spherical_bessel(oracle_mod):
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    r = np.array([3.0, 5.0, 6.5])
+    w = np.array([0.2, 0.5, 0.3])
+    q = np.sqrt(qx ** 2 + qy ** 2 + qz ** 2)
+    ref = sum(wi * 4 * np.pi * R ** 3 * special.spherical_jn(1, q * R) / (q * R) * np.exp(1j * qz * R)
+              for R, wi in zip(r, w))
+    assert rel_l2(O.ff_analytic("sphere", [(r, w)], g), ref) < 1e-12
+
+
+def test_cylinder_is_bessel_j1(oracle_mod):
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    qp = np.hypot(qx, qy)
+    R, H = 4.0, 12.0
+    ref = 2 * np.pi * R ** 2 * H * special.j1(qp * R) / (qp * R) * S(qz * H / 2) * np.exp(1j * qz * H / 2)
+    assert rel_l2(O.ff_analytic("cylinder", [([R], [1.0]), ([H], [1.0])], g), ref) < 1e-12
+
+
+def test_box_and_cube_closed_form(oracle_mod):
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    L, W, H = 4.0, 6.0, 8.0
+    box = lambda a, b, c: a * b * c * S(qx * a / 2) * S(qy * b / 2) * S(qz * c / 2) * np.exp(1j * qz * c / 2)
+    got = O.ff_analytic("box", [([L, 5.0], [0.3, 0.7]), ([W], [1.0]), ([H], [1.0])], g)
+    assert rel_l2(got, 0.3 * box(L, W, H) + 0.7 * box(5.0, W, H)) < 1e-13
+    got = O.ff_analytic("cube", [([5.0, 6.0], [0.5, 0.5])], g)
+    assert rel_l2(got, 0.5 * box(5.0, 5.0, 5.0) + 0.5 * box(6.0, 6.0, 6.0)) < 1e-13
+
+
+def test_prism3_is_equilateral_prism_mesh(oracle_mod):
+    """prism3 with edge l, height h: equilateral triangle of side 2l centred on its centroid,
+    z in [0, h] (identified against the exact-triangle method)."""
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    g, *_ = _grid(O)
+    l, h = 3.0, 5.0
+    s3 = np.sqrt(3.0)
+    tri = meshes.prism_polygon(np.array([[-l, -l / s3], [l, -l / s3], [0.0, 2 * l / s3]]), h)
+    assert rel_l2(O.ff_analytic("prism3", [([l], [1.0]), ([h], [1.0])], g), O.ff_exact(tri, g)) < 1e-10
+
+
+def test_prism6_is_hexagonal_prism_mesh(oracle_mod):
+    """prism6 with edge l: regular hexagon of circumradius 2l/sqrt(3), a vertex on +x, z in [0, h]."""
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    g, *_ = _grid(O)
+    l, h = 3.0, 5.0
+    ang = np.arange(6) * np.pi / 3
+    R = 2 * l / np.sqrt(3.0)
+    tri = meshes.prism_polygon(np.stack([R * np.cos(ang), R * np.sin(ang)], 1), h)
+    assert rel_l2(O.ff_analytic("prism6", [([l], [1.0]), ([h], [1.0])], g), O.ff_exact(tri, g)) < 1e-10
+
+
+def test_pyramid_published_form_is_the_truncated_pyramid(oracle_mod):
+    """Truth: Gauss-Legendre integration over z of the rectangular cross-sections
+    (half-widths L/2 - z/tan(b), W/2 - z/tan(b)); also the exact-triangle mesh."""
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    L, W, H, b = 16.0, 12.0, 4.0, 54.73
+    dims = [([L], [1.0]), ([W], [1.0]), ([H], [1.0]), ([b], [1.0])]
+    x, w = np.polynomial.legendre.leggauss(200)
+    z = (x + 1) * H / 2
+    wz = w * H / 2
+    a, c = L / 2 - z / np.tan(np.radians(b)), W / 2 - z / np.tan(np.radians(b))
+    ref = np.array([np.sum(wz * 4 * a * c * S(qx[i] * a) * S(qy[i] * c) * np.exp(1j * qz[i] * z))
+                    for i in range(len(qx))])
+    got = O.ff_analytic("pyramid", dims, g)
+    assert rel_l2(got, ref) < 1e-12
+    assert rel_l2(got, O.ff_exact(meshes.truncated_pyramid(L, W, H, b), g)) < 1e-7
+
+
+def test_pyramid_reference_compat_form(oracle_mod):
+    """compat=1 is the reference's expression (src/ff/ff_ana_pyramid.cpp:47-64) transcribed here
+    independently; it is not the pyramid's form factor (parity with it is unpinned by truth)."""
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    L, W, H, b = 16.0, 12.0, 4.0, 54.73
+    ta = np.tan(np.radians(b))
+    q1, q2 = 0.5 * ((qx - qy) / ta + qz), 0.5 * ((qx - qy) / ta - qz)
+    q3, q4 = 0.5 * ((qx + qy) / ta + qz), 0.5 * ((qx + qy) / ta - qz)
+    e = lambda x: np.exp(1j * x)
+    k1 = S(q1 * H) * e(-q1 * H) + S(q2 * H) * e(q2 * H)
+    k2 = -1j * S(q1 * H) * e(-q1 * H) + 1j * S(q2 * H) * e(q2 * H)
+    k3 = S(q3 * H) * e(-q3 * H) + S(q4 * H) * e(q4 * H)
+    k4 = -1j * S(q3 * H) * e(-q3 * H) + 1j * S(q4 * H) * e(q4 * H)
+    am, ap = 0.5 * (qx * L - qy * W), 0.5 * (qx * L + qy * W)
+    ref = H / (qx * qy) * (k1 * np.cos(am) + k2 * np.sin(am) - k3 * np.cos(ap) - k4 * np.sin(ap))
+    dims = [([L], [1.0]), ([W], [1.0]), ([H], [1.0]), ([b], [1.0])]
+    assert rel_l2(O.ff_analytic("pyramid", dims, g, compat=1), ref) < 1e-12
+    # untruncated pyramid (2H/L >= tan(angle)) returns 0 (src/ff/ff_ana_pyramid.cpp:40-41)
+    dims0 = [([20.0], [1.0]), ([20.0], [1.0]), ([10.0], [1.0]), ([45.0], [1.0])]
+    assert np.all(O.ff_analytic("pyramid", dims0, g) == 0)
+
+
+def test_translation_phase(oracle_mod):
+    O = oracle_mod
+    g, qx, qy, qz = _grid(O)
+    t = np.array([1.0, -2.0, 0.5])
+    a = O.ff_analytic("sphere", [([5.0], [1.0])], g)
+    b = O.ff_analytic("sphere", [([5.0], [1.0])], g, transvec=t)
+    assert rel_l2(b, a * np.exp(1j * (qx * t[0] + qy * t[1] + qz * t[2]))) < 1e-13
+
+
+def test_param_distribution_reference_grid(oracle_mod):
+    """Node values follow the reference's accumulating do-while; gaussian weights are the pdf."""
+    O = oracle_mod
+    v, w = O.param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)
+    # the accumulating do-while can drop or add the last node (here it stops at 20 nodes)
+    assert 20 <= v.size <= 22
+    assert np.allclose(v, 3.0 + 0.2 * np.arange(v.size), atol=1e-12)
+    from scipy.stats import norm
+
+    assert np.allclose(w, norm.pdf(v, 5.0, 0.5), rtol=1e-13)
+    v, w = O.param_distribution(2.0, 2.0, 1, 0)
+    assert v.tolist() == [2.0] and w.tolist() == [1.0]
