@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build is single-GPU); use --gpus 1", file=sys.stderr)
@@ -98,6 +100,15 @@ def main():
 
     import torch
     from hipgisaxs_amd.api import Context
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 3}))
+        return
 
     dev = torch.device("cuda:0")
     t_setup = time.perf_counter()

@@ -53,6 +53,10 @@ _SIGS = {
     "hgx_use_own_stream": (C.c_int, [_vp]),
     "hgx_synchronize": (C.c_int, [_vp]),
     "hgx_last_error": (C.c_char_p, [_vp]),
+    "hgx_device_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "hgx_device_free": (C.c_int, [_vp, _vp]),
+    "hgx_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "hgx_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "hgx_profile_begin": (C.c_int, [_vp]),
     "hgx_profile_end": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_int64)]),

@@ -59,6 +59,26 @@ def build_hip(jobs: int = 8) -> Path:
     return LIB
 
 
+HIG_SRC = PKG / "host" / "hig_gpu.cpp"
+HIG_LIB = PKG / "libhgx_hig.so"
+SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
+SHIM_LIB = ROOT / "tests" / "cpp" / "libhig_shim.so"
+
+
+def build_host_adapters() -> Path:
+    """The reference-interface C++ classes (host code on the C ABI) and the test shim that calls them."""
+    cxx = shutil.which("g++") or "g++"
+    common = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'host'}"]
+    tmp = HIG_LIB.with_suffix(".so.tmp")
+    _run([cxx, *common, str(HIG_SRC), "-o", str(tmp), f"-L{PKG}", "-lhgx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HIG_LIB)
+    tmp = SHIM_LIB.with_suffix(".so.tmp")
+    _run([cxx, *common, str(SHIM_SRC), "-o", str(tmp), f"-L{PKG}", "-lhgx_hig", "-lhgx",
+          "-Wl,-rpath,$ORIGIN/../../hipgisaxs_amd"])
+    os.replace(tmp, SHIM_LIB)
+    return HIG_LIB
+
+
 def build_oracle() -> Path:
     """Compiles the FP64 OpenMP oracle (test infrastructure) into oracle/liboracle.so."""
     cxx = shutil.which("g++") or "g++"
@@ -70,6 +90,7 @@ def build_oracle() -> Path:
 
 def build_all() -> None:
     build_hip()
+    build_host_adapters()
     build_oracle()
 
 
@@ -77,5 +98,6 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     if "all" in what or "hip" in what:
         print("built", build_hip())
+        print("built", build_host_adapters())
     if "all" in what or "oracle" in what:
         print("built", build_oracle())

@@ -256,6 +256,50 @@ int hgx_synchronize(hgx_ctx* h) {
   return HGX_OK;
 }
 
+int hgx_device_alloc(hgx_ctx* h, size_t bytes, void** ptr) {
+  Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!ptr) return set_error(c, HGX_ERR_INVALID_ARG, "device_alloc: null output pointer");
+  *ptr = nullptr;
+  if (int rc = set_device(c)) return rc;
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return set_error(c, HGX_ERR_OOM, "device_alloc of %zu bytes failed", bytes);
+  }
+  return HGX_OK;
+}
+
+int hgx_device_free(hgx_ctx* h, void* ptr) {
+  Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!ptr) return HGX_OK;
+  if (int rc = set_device(c)) return rc;
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  HGX_CHECK_HIP(c, hipFree(ptr));
+  return HGX_OK;
+}
+
+int hgx_memcpy_d2h(hgx_ctx* h, void* dst, const void* src, size_t bytes) {
+  Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!dst || !src) return set_error(c, HGX_ERR_INVALID_ARG, "memcpy_d2h: null pointer");
+  if (int rc = set_device(c)) return rc;
+  HGX_CHECK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return HGX_OK;
+}
+
+int hgx_memcpy_h2d(hgx_ctx* h, void* dst, const void* src, size_t bytes) {
+  Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!dst || !src) return set_error(c, HGX_ERR_INVALID_ARG, "memcpy_h2d: null pointer");
+  if (int rc = set_device(c)) return rc;
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  HGX_CHECK_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return HGX_OK;
+}
+
 int hgx_profile_begin(hgx_ctx* h) {
   Context* c = ctx_of(h);
   if (!c) return HGX_ERR_BAD_HANDLE;

@@ -21,6 +21,7 @@
 #ifndef HGX_H_
 #define HGX_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -106,6 +107,15 @@ const char* hgx_version(void);
  * CUDA-event timing (src/ff/gpu/ff_tri_gpu.cu:413-415, 508-512). */
 int hgx_profile_begin(hgx_ctx* ctx);
 int hgx_profile_end(hgx_ctx* ctx, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches);
+
+/* Device memory on the context's device, for callers without a HIP toolchain of their own (the C++
+ * adapters, cgo/JNI-style bindings). The reference allocates and frees per call instead
+ * (src/ff/gpu/ff_tri_gpu.cu:440-518). memcpy_d2h synchronises the context's stream; memcpy_h2d
+ * waits for it first. */
+int hgx_device_alloc(hgx_ctx* ctx, size_t bytes, void** ptr);
+int hgx_device_free(hgx_ctx* ctx, void* ptr);
+int hgx_memcpy_d2h(hgx_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+int hgx_memcpy_h2d(hgx_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
 
 /* ---- q-grid (persistent per context) ----------------------------------------------------- */
 

@@ -1,0 +1,99 @@
+"""The drop-in C++ classes (reference GPU-class interfaces on the C ABI), driven by reference-style
+C++ callers (tests/cpp/hig_adapter_shim.cpp), against the FP64 oracle."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+ROOT = Path(__file__).resolve().parents[1]
+SHIM = ROOT / "tests" / "cpp" / "libhig_shim.so"
+
+
+def _shim():
+    lib = C.CDLL(str(SHIM))
+    for name in ("shim_numeric_approx", "shim_numeric_exact", "shim_analytic", "shim_sf", "shim_fresnel"):
+        getattr(lib, name).restype = C.c_int
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c2r(z):
+    return np.ascontiguousarray(np.stack([z.real, z.imag], -1).ravel())
+
+
+def test_adapter_library_exports_reference_classes():
+    import subprocess
+
+    syms = subprocess.run(["nm", "-DC", str(ROOT / "hipgisaxs_amd" / "libhgx_hig.so")], capture_output=True,
+                          text=True).stdout
+    for s in ("hig::NumericFormFactorG::compute_approx_triangle", "hig::NumericFormFactorG::compute_exact_triangle",
+              "hig::AnalyticFormFactorG::compute_sphere", "hig::AnalyticFormFactorG::compute_pyramid",
+              "hig::StructureFactorG::compute", "hig::StructureFactorG::get_sf",
+              "hig::MultiLayerG::propagation_coeffs", "hig::dwba_intensity"):
+        assert s in syms, s
+    assert SHIM.exists()
+
+
+@pytest.mark.gpu
+def test_adapters_match_oracle(oracle_mod):
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    lib = _shim()
+    k0, ai = 2 * np.pi / 0.123984, np.radians(0.2)
+    g = O.qz_extended(O.qgrid_explicit(24, 20, -2.0, 0.0, 2.0, 2.0, ai, k0), k0, ai)
+    qx, qy, qz = g.qx.copy(), g.qy.copy(), _c2r(g.qz)
+    nqy, nqz = g.npix, g.nq
+    R = np.array([[0.8, -0.6, 0.0], [0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])
+    tri = meshes.pyramid(side_levels=2, base_nx=8, base_ny=4)
+    rec = O.tri9_to_rec8(tri)
+    out = np.empty(2 * nqz)
+    # numeric, approximate (plain FP64 and FP32C)
+    for prec, tol in ((0, 1e-10), (1, 1e-5)):
+        n = lib.shim_numeric_approx(nqy, _p(qx), _p(qy), nqz, _p(qz), rec.shape[0], _p(np.ascontiguousarray(rec)),
+                                    _p(R.ravel().copy()), prec, _p(out))
+        assert n == rec.shape[0]
+        assert rel_l2(out[0::2] + 1j * out[1::2], O.ff_approx(rec, g, R)) <= tol
+    # numeric, exact
+    n = lib.shim_numeric_exact(nqy, _p(qx), _p(qy), nqz, _p(qz), tri.shape[0], _p(np.ascontiguousarray(tri)),
+                               _p(R.ravel().copy()), _p(out))
+    assert n == tri.shape[0]
+    assert rel_l2(out[0::2] + 1j * out[1::2], O.ff_exact(tri, g, R)) <= 1e-10
+    # analytic sphere with a gaussian radius distribution, and the pyramid in both forms
+    v, w = O.param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)
+    tv = np.array([1.0, 2.0, -0.5])
+    lib.shim_analytic(0, nqy, _p(qx), _p(qy), nqz, _p(qz), _p(np.array([v.size], np.int32)), _p(v), _p(w),
+                      _p(R.ravel().copy()), _p(tv), 0, _p(out))
+    assert rel_l2(out[0::2] + 1j * out[1::2], O.ff_analytic("sphere", [(v, w)], g, R, tv)) <= 1e-10
+    dims = [([16.0], [1.0]), ([14.0, 16.0], [0.5, 0.5]), ([4.0], [1.0]), ([54.73], [1.0])]
+    counts = np.array([len(d[0]) for d in dims], np.int32)
+    vals = np.concatenate([np.asarray(d[0], float) for d in dims])
+    wts = np.concatenate([np.asarray(d[1], float) for d in dims])
+    for compat in (0, 1):
+        lib.shim_analytic(1, nqy, _p(qx), _p(qy), nqz, _p(qz), _p(counts), _p(vals), _p(wts), _p(R.ravel().copy()),
+                          _p(tv), compat, _p(out))
+        ref = O.ff_analytic("pyramid", dims, g, R, tv, compat=compat)
+        assert rel_l2(out[0::2] + 1j * out[1::2], ref) <= 1e-10
+    # structure factor: bcc, scaled unit vectors
+    abc = np.array([1.0, 0, 0, 0, 1.0, 0, 0.5, 0.5, 0.5])
+    rep, sc, ce = np.array([10.0, 10, 5]), np.array([20.0, 20, 20]), np.array([0.0, 1.0, -1.0])
+    assert lib.shim_sf(nqy, _p(qx), _p(qy), nqz, _p(qz), _p(abc), _p(rep), _p(sc), _p(ce), _p(R.ravel().copy()),
+                       C.c_double(0.5), _p(out)) == 0
+    ref = O.sf_lattice(20 * abc[:3], 20 * abc[3:6], 20 * abc[6:], rep, ce, g, 0.5, 0, R)
+    assert rel_l2(out[0::2] + 1j * out[1::2], ref) <= 1e-10
+    # Fresnel coefficients on the grid's rows
+    d, b, t = np.array([4.88e-6]), np.array([7.37e-8]), np.array([0.0])
+    fc = np.empty(8 * nqy)
+    assert lib.shim_fresnel(1, _p(d), _p(b), _p(t), 0, C.c_double(k0), C.c_double(ai), g.nrow, _p(g.alpha_f),
+                            g.ncol, _p(fc)) == 0
+    assert rel_l2(fc[0::2] + 1j * fc[1::2], O.fresnel(g, d, b, t, 0, k0, ai)) <= 1e-12
