@@ -82,6 +82,92 @@ def cpu_baseline(shapes, budget_s=12.0):
     }, (s, rows, out)
 
 
+def _det_grid(ctx, side, pixel_mm, alpha_i, k0, dwba=True):
+    ctx.qgrid_detector(side, side, pixel_mm, 2000.0, (side - 1) / 2.0, 0.0, alpha_i, k0)
+    if dwba:
+        ctx.qz_extended(k0, alpha_i, 0j)
+
+
+def bench_lattice(reps=5):
+    """Spheres R ~ N(5, 0.5) nm (21 nodes on [3, 7]) on bcc a = 20 nm, 10x10x5, 8 azimuths, DW 0.5 nm,
+    1024^2 detector (2 m, 172 um), 12 keV, alpha_i = 0.15 deg, DWBA over Si: fused kernel, both paths."""
+    import torch
+    from hipgisaxs_amd.api import Context, lattice_intensity, param_distribution
+
+    k0 = 2 * math.pi / (1.23984 / 12.0)
+    ai = math.radians(0.15)
+    ctx = Context(0)
+    _det_grid(ctx, 1024, 0.172, ai, k0)
+    fc = ctx.fresnel([3.39e-6], [3.55e-8], [0.0], 0, k0, ai)
+    v, w = param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)
+    dims = [(v, w / w.sum())]
+    a = 20.0
+    la, lb, lc = (a, 0, 0), (0, a, 0), (a / 2, a / 2, a / 2)
+    rots = np.stack([np.array([[math.cos(p), -math.sin(p), 0], [math.sin(p), math.cos(p), 0], [0, 0, 1]])
+                     for p in (j * math.pi / 4 for j in range(8))])
+    dn2 = -(2 * 6.53e-6 + 2j * 1.6e-8)
+    out = {}
+    res = {}
+    for prec in ("fp32c", "fp64"):
+        I = torch.empty(1024 * 1024, dtype=torch.float64, device="cuda:0")
+        kw = dict(dw_sigma=0.5, dn2=dn2, precision=prec, fc=fc, out=I)
+        lattice_intensity(ctx, "sphere", dims, la, lb, lc, (10, 10, 5), rots, None, **kw)
+        torch.cuda.synchronize()
+        ctx.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lattice_intensity(ctx, "sphere", dims, la, lb, lc, (10, 10, 5), rots, None, **kw)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        prof = ctx.profile_end()
+        samples = 4 * 1024 * 1024 * 8 * v.size
+        out[prec] = {"ms": 1e3 * wall, "kernel_ms": prof["main_ms"] / reps, "q_pt_samples_per_s": samples / wall}
+        res[prec] = I
+    d = torch.linalg.vector_norm(res["fp32c"] - res["fp64"]) / torch.linalg.vector_norm(res["fp64"])
+    out["relL2_fp32c_vs_fp64"] = float(d)
+    out["speedup_fp32c_vs_fp64"] = out["fp32c"]["q_pt_samples_per_s"] / out["fp64"]["q_pt_samples_per_s"]
+    out["workload"] = "spheres N(5,0.5) 21 nodes, bcc a=20 10x10x5, 8 azimuths, DW 0.5, 1024^2 DWBA, fused"
+    ctx.close()
+    return out
+
+
+def bench_fit(evals=20):
+    """Cylinders R=4, H=12 on hcp a=15, 20x20x1, 512^2 detector (2 m, 344 um), 12 keV, 0.15 deg, DWBA;
+    target = Poisson(1e4 I(theta*)/max I) (seeded); chi^2 and its 6-parameter gradient."""
+    import torch
+    from hipgisaxs_amd.api import Context, FitPlan
+
+    k0 = 2 * math.pi / (1.23984 / 12.0)
+    ai = math.radians(0.15)
+    ctx = Context(0)
+    _det_grid(ctx, 512, 0.344, ai, k0)
+    fc = ctx.fresnel([3.39e-6], [3.55e-8], [0.0], 0, k0, ai)
+    ua, ub, uc = (1.0, 0, 0), (0.5, 0.866, 0), (0.5, 0.2887, 0.8165)
+    theta_true = np.array([4.0, 12.0, 15.0, 0.0, 0.5, 1.0])
+    I = torch.empty(512 * 512, dtype=torch.float64, device="cuda:0")
+    zero = torch.zeros_like(I)
+    plan0 = FitPlan(ctx, "cylinder", ua, ub, uc, (20, 20, 1), zero, fc=fc, dn2=-(2 * 6.53e-6 + 2j * 1.6e-8))
+    plan0.fwd(theta_true, I_model=I)
+    rng = np.random.default_rng(SEED + 4)
+    Ih = I.cpu().numpy()
+    obs = torch.from_numpy(rng.poisson(1e4 * Ih / Ih.max()).astype(np.float64) * (Ih.max() / 1e4)).cuda()
+    plan = FitPlan(ctx, "cylinder", ua, ub, uc, (20, 20, 1), obs, fc=fc, dn2=-(2 * 6.53e-6 + 2j * 1.6e-8))
+    theta = theta_true * np.array([1.02, 0.98, 1.01, 1.0, 1.1, 0.95]) + np.array([0, 0, 0, 0.01, 0, 0])
+    plan.bwd(theta)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        plan.fwd(theta)
+    t_fwd = (time.perf_counter() - t0) / evals
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        chi2, grad = plan.bwd(theta)
+    t_bwd = (time.perf_counter() - t0) / evals
+    ctx.close()
+    return {"chi2_grad_evals_per_s": 1.0 / t_bwd, "ms_fwd": 1e3 * t_fwd, "ms_grad": 1e3 * t_bwd,
+            "grad_over_fwd": t_bwd / t_fwd, "fd_equivalent_fwds": 13, "chi2": chi2,
+            "workload": "cylinders R4 H12, hcp a=15 20x20x1, 512^2 DWBA, 6 parameters, FP64 adjoint"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +176,7 @@ def main():
     ap.add_argument("--shapes", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the lattice and chi^2 measurements")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
@@ -215,6 +302,11 @@ def main():
             "aux_ms_per_step": aux32 / args.steps,
         },
     }
+    if not args.no_extra:
+        result["lattice_spheres_bcc"] = bench_lattice()
+        print(f"[bench] lattice done", file=sys.stderr, flush=True)
+        result["chi2_cylinders_hcp"] = bench_fit()
+        print(f"[bench] fit done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb, (s, rows, ref) = cpu_baseline(shapes)
         got = Context.as_complex(s["out32"]).cpu().numpy()[: rows * s["side"]]

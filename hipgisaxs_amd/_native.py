@@ -44,6 +44,40 @@ class ShapeParams(C.Structure):
     ]
 
 
+class LatticeModel(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int),
+        ("params", ShapeParams),
+        ("la", C.c_double * 3),
+        ("lb", C.c_double * 3),
+        ("lc", C.c_double * 3),
+        ("repet", C.c_double * 3),
+        ("center", C.c_double * 3),
+        ("dw_sigma", C.c_double),
+        ("dn2_re", C.c_double),
+        ("dn2_im", C.c_double),
+        ("sf_flags", C.c_int),
+        ("ngrains", C.c_int),
+        ("rot", _dp),
+        ("weight", _dp),
+    ]
+
+
+class FitModel(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int),
+        ("ua", C.c_double * 3),
+        ("ub", C.c_double * 3),
+        ("uc", C.c_double * 3),
+        ("repet", C.c_double * 3),
+        ("center", C.c_double * 3),
+        ("dn2_re", C.c_double),
+        ("dn2_im", C.c_double),
+        ("rot0", C.c_double * 9),
+        ("sf_flags", C.c_int),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "hgx_version": (C.c_char_p, []),
@@ -76,6 +110,11 @@ _SIGS = {
     "hgx_sf_lattice": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp]),
     "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),
     "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
+    "hgx_lattice_intensity": (C.c_int, [_vp, C.POINTER(LatticeModel), C.c_int, _vp, C.c_int, _vp]),
+    "hgx_fit_plan_create": (C.c_int, [_vp, C.POINTER(FitModel), _vp, _vp, _vp, C.POINTER(C.c_int)]),
+    "hgx_fit_plan_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_chi2_fwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _vp]),
+    "hgx_chi2_bwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _dp]),
     "hgx_param_distribution": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                          _dp, _dp, C.c_int]),
     "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),

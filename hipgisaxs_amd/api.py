@@ -217,6 +217,87 @@ class Context:
         return o
 
 
+def _fill_shape_params(sp, dims, keep):
+    for d, (v, w) in enumerate(dims):
+        v, w = _f64(v).ravel(), _f64(w).ravel()
+        keep += [v, w]
+        sp.n[d] = v.size
+        sp.val[d] = v.ctypes.data_as(C.POINTER(C.c_double))
+        sp.wt[d] = w.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lattice_intensity(ctx: Context, shape: str, dims, la, lb, lc, repet, rots, weights=None, center=(0, 0, 0),
+                      dw_sigma=0.0, dn2=1.0, sf_flags=0, precision="fp64", fc=None, accumulate=False, out=None):
+    """hgx_lattice_intensity: I[p] (+)= sum_g |w_g dn2 sum_b fc_b F(R_g q_b) S(R_g q_b)|^2."""
+    m = N.LatticeModel()
+    keep = []
+    m.shape = N.SHAPES[shape]
+    _fill_shape_params(m.params, dims, keep)
+    for name, v in (("la", la), ("lb", lb), ("lc", lc), ("repet", repet), ("center", center)):
+        getattr(m, name)[:] = [float(x) for x in v]
+    m.dw_sigma = dw_sigma
+    m.dn2_re, m.dn2_im = complex(dn2).real, complex(dn2).imag
+    m.sf_flags = sf_flags
+    R = _f64(rots).reshape(-1, 9)
+    keep.append(R)
+    m.ngrains = R.shape[0]
+    m.rot = R.ctypes.data_as(C.POINTER(C.c_double))
+    if weights is not None:
+        W = _f64(weights).ravel()
+        keep.append(W)
+        m.weight = W.ctypes.data_as(C.POINTER(C.c_double))
+    npix = ctx.info()["npix"]
+    o = ctx._out(npix, out)
+    p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+    ctx._check(ctx.lib.hgx_lattice_intensity(ctx.h, C.byref(m), p, None if fc is None else C.c_void_p(fc.data_ptr()),
+                                             int(accumulate), C.c_void_p(o.data_ptr())), "lattice_intensity")
+    return o
+
+
+class FitPlan:
+    """hgx_fit_plan_create / hgx_chi2_fwd / hgx_chi2_bwd. theta = (radius, height, a, tilt, sigma, scale)."""
+
+    def __init__(self, ctx: Context, shape: str, ua, ub, uc, repet, I_obs, fc=None, mask=None, center=(0, 0, 0),
+                 dn2=1.0, rot0=None, sf_flags=0):
+        m = N.FitModel()
+        m.shape = N.SHAPES[shape]
+        for name, v in (("ua", ua), ("ub", ub), ("uc", uc), ("repet", repet), ("center", center)):
+            getattr(m, name)[:] = [float(x) for x in v]
+        m.dn2_re, m.dn2_im = complex(dn2).real, complex(dn2).imag
+        m.rot0[:] = [float(x) for x in (np.eye(3) if rot0 is None else np.asarray(rot0)).ravel()]
+        m.sf_flags = sf_flags
+        self.ctx = ctx
+        self._keep = (I_obs, fc, mask)  # the plan reads these device buffers by pointer
+        pid = C.c_int()
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        ctx._check(ctx.lib.hgx_fit_plan_create(ctx.h, C.byref(m), ptr(fc), ptr(I_obs), ptr(mask), C.byref(pid)),
+                   "fit_plan_create")
+        self.id = pid.value
+
+    def fwd(self, theta, precision="fp64", I_model=None) -> float:
+        th = _f64(theta).ravel()
+        chi2 = C.c_double()
+        self.ctx._check(self.ctx.lib.hgx_chi2_fwd(self.ctx.h, self.id, th.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision],
+                                                  C.byref(chi2),
+                                                  None if I_model is None else C.c_void_p(I_model.data_ptr())),
+                        "chi2_fwd")
+        return chi2.value
+
+    def bwd(self, theta, precision="fp64"):
+        th = _f64(theta).ravel()
+        chi2 = C.c_double()
+        g = np.zeros(6)
+        self.ctx._check(self.ctx.lib.hgx_chi2_bwd(self.ctx.h, self.id, th.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision],
+                                                  C.byref(chi2), g.ctypes.data_as(C.POINTER(C.c_double))),
+                        "chi2_bwd")
+        return chi2.value, g
+
+    def release(self):
+        self.ctx._check(self.ctx.lib.hgx_fit_plan_release(self.ctx.h, self.id), "fit_plan_release")
+
+
 def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0):
     """AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284) through the C ABI."""
     lib = N.load()

@@ -37,8 +37,18 @@ struct Mesh {
   double* d_moments = nullptr; // [kMomentCount] moments about origin (see ff_numeric.hip)
 };
 
+// A chi^2 plan: model constants and the device inputs it reads (caller-owned).
+struct FitPlan {
+  bool live = false;
+  hgx_fit_model model;
+  const double* fc = nullptr;
+  const double* obs = nullptr;
+  const double* mask = nullptr;
+};
+
 struct Context {
   int device = 0;
+  std::vector<FitPlan> fit_plans;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   // q-grid

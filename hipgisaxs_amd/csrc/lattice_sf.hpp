@@ -1,0 +1,48 @@
+// lattice_sf.hpp -- FP64 finite-lattice structure factor of the rotated q-vector (the reference's
+// src/sf/sf.cpp:165-245 in closed Dirichlet form), shared by the SF kernel and the fused kernels.
+#pragma once
+
+#include "hgx_internal.hpp"
+
+namespace hgx {
+
+constexpr double kPiD = 3.141592653589793238462643383279502884;
+
+// ----- structure factor -------------------------------------------------------------------------
+//
+// The reference (src/sf/sf.cpp:197-245) evaluates, per lattice direction with x = l.q,
+//     pow(e, (1-N)/2) * (1 - pow(e, N)) / (1 - e),   e = exp(i x),
+// with the principal-branch pow of std::complex, and N where |1 - e| vanishes. pow(e, p) is
+// exp(i p x') with x' = wrap(Re x) + i Im x (wrap into (-pi, pi]), so the product is the
+// Dirichlet kernel D_N(x') = sin(N x'/2) / sin(x'/2), evaluated here in that closed form, which
+// keeps full relative accuracy next to the Bragg peaks where the ratio form cancels.
+__device__ inline cd dirichlet(cd x, double N) {
+  double w = x.r - 2.0 * kPiD * rint(x.r * (0.5 / kPiD));
+  cd xp = cd{w, x.i};
+  cd h = 0.5 * xp;
+  cd den = csin_d(h);
+  if (cabs_d(den) < 1e-300) return cd{N, 0.0};
+  return cdiv(csin_d(N * h), den);
+}
+
+struct SFArgs {
+  double la[3], lb[3], lc[3];
+  double N[3];
+  double center[3];
+  double dw;  // sigma^2 / 2
+  int compat_lb2;
+};
+
+// S(q) for one rotated q-vector: product of the three Dirichlet kernels, the grain-centre phase
+// and the Debye-Waller damping.
+__device__ inline cd sf_value(const cd mq[3], const SFArgs& a) {
+  cd xa = mq[0] * a.la[0] + mq[1] * a.la[1] + mq[2] * a.la[2];
+  cd xb = mq[0] * a.lb[0] + mq[1] * a.lb[1] + (a.compat_lb2 ? cd{a.lb[2] * a.lb[2], 0.0} : mq[2] * a.lb[2]);
+  cd xc = mq[0] * a.lc[0] + mq[1] * a.lc[1] + mq[2] * a.lc[2];
+  cd s = dirichlet(xa, a.N[0]) * dirichlet(xb, a.N[1]) * dirichlet(xc, a.N[2]);
+  s = s * cexpi_d(mq[0] * a.center[0] + mq[1] * a.center[1] + mq[2] * a.center[2]);
+  if (a.dw != 0.0) s = exp(-a.dw * (norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]))) * s;
+  return s;
+}
+
+}  // namespace hgx

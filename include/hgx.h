@@ -195,6 +195,59 @@ int hgx_fresnel(hgx_ctx* ctx, int nl, const double* delta, const double* beta, c
 int hgx_dwba_intensity(hgx_ctx* ctx, const double* fc, const double* sf, const double* ff, double weight,
                        int accumulate, double* I);
 
+/* ---- fused grain-ensemble intensity ------------------------------------------------------- */
+
+/* A lattice of identical analytic particles seen by several grain orientations. */
+typedef struct hgx_lattice_model {
+  int shape;                  /* HGX_SHAPE_SPHERE (dims: radius) or HGX_SHAPE_CYLINDER (radius, height) */
+  hgx_shape_params params;    /* distribution nodes and weights */
+  double la[3], lb[3], lc[3]; /* lattice vectors already multiplied by the grain scaling */
+  double repet[3];            /* repetitions (values < 1 become 1) */
+  double center[3];           /* grain centre */
+  double dw_sigma;            /* Debye-Waller sigma (0 = off) */
+  double dn2_re, dn2_im;      /* contrast (1-n^2)_layer - (1-n^2)_particle (hipgisaxs_main.cpp:983) */
+  int sf_flags;               /* HGX_SF_* */
+  int ngrains;                /* >= 1 grain orientations */
+  const double* rot;          /* host [ngrains][9], mq = rot_g q for both form and structure factor */
+  const double* weight;       /* host [ngrains] grain weights w_g, or NULL for 1 */
+} hgx_lattice_model;
+
+/* I[p] (+)= sum_g |w_g dn2 sum_b fc_b F(R_g q_b) S(R_g q_b)|^2 over the grid's branches b: the grain
+ * loop, form factor, structure factor and DWBA sum of HipGISAXS::run_gisaxs
+ * (src/sim/hipgisaxs_main.cpp:925-1100) fused into one pass per pixel. fc (device, [2*nq]) may be
+ * NULL for the Born approximation. FP32C requires real q. */
+int hgx_lattice_intensity(hgx_ctx* ctx, const hgx_lattice_model* model, int precision, const double* fc,
+                          int accumulate, double* I);
+
+/* ---- chi^2 and its gradient (forward / backward pair) ------------------------------------- */
+
+/* One-grain model fitted by hgx_chi2_*: I(p) = s |sum_b fc_b dn2 F(m_b) S(m_b)|^2 with
+ * m_b = R_x(tilt) rot0 q_b and theta = {radius, height, lattice constant a, tilt (rad), Debye-Waller
+ * sigma, scale s}; the lattice vectors are a * (ua, ub, uc). */
+typedef struct hgx_fit_model {
+  int shape;                  /* HGX_SHAPE_CYLINDER (radius, height) or HGX_SHAPE_SPHERE (radius) */
+  double ua[3], ub[3], uc[3]; /* lattice vectors for a = 1 (Lattice::construct_vectors) */
+  double repet[3];
+  double center[3];
+  double dn2_re, dn2_im;      /* contrast */
+  double rot0[9];             /* fixed grain orientation applied before the tilt */
+  int sf_flags;               /* HGX_SF_* */
+} hgx_fit_model;
+
+/* Binds a model to device inputs (kept by pointer, caller-owned): fc [2*nq] or NULL (Born), the
+ * observed image I_obs [npix], an optional 0/1 mask [npix]. The reference's objective is
+ * HipGISAXSObjectiveFunction::operator() (src/analyzer/objective_func_hipgisaxs.cpp:245-332), whose
+ * gradient TAO's LMVM takes by central differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). */
+int hgx_fit_plan_create(hgx_ctx* ctx, const hgx_fit_model* model, const double* fc, const double* I_obs,
+                        const double* mask, int* plan);
+int hgx_fit_plan_release(hgx_ctx* ctx, int plan);
+/* chi2 = sum_p mask_p (I_p(theta) - O_p)^2 / max(O_p, 1); I_model (device [npix]) may be NULL.
+ * Synchronises the stream (the scalar is returned to the host). */
+int hgx_chi2_fwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, double* chi2, double* I_model);
+/* chi2 and dchi2/dtheta (6 components, the height component is 0 for spheres) by the hand-written
+ * adjoint: one pass over the pixels that recomputes the amplitudes instead of storing them. */
+int hgx_chi2_bwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, double* chi2, double grad[6]);
+
 /* ---- host helpers ------------------------------------------------------------------------- */
 
 /* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values of a shape
