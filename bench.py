@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the lattice and chi^2 measurements")
+    ap.add_argument("--mode", choices=("batch", "per-shape"), default="batch",
+                    help="batch: one launch for all shapes (hipGraph replay); per-shape: one call per shape")
+    ap.add_argument("--no-tune", action="store_true", help="skip the work-unit autotuning")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
@@ -186,7 +189,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from hipgisaxs_amd.api import Context
+    from hipgisaxs_amd.api import Batch, Context
 
     if args.copy_probe:
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
@@ -201,15 +204,15 @@ def main():
     t_setup = time.perf_counter()
     shapes = make_workload(args.shapes)
     sides = sorted({s["side"] for s in shapes})
-    ctxs = {}
+    grid_ctx = {}
     for side in sides:
         c = Context(0)
         c.qgrid_explicit(side, side, -2.0, 0.0, 2.0, 2.0, 0.0, 1e3)
-        ctxs[side] = c
+        grid_ctx[side] = c
+    main_ctx = Context(0)  # holds every mesh and the batches; grids live in grid_ctx
     for s in shapes:
-        c = ctxs[s["side"]]
-        s["mid"] = c.upload_mesh(s["tri"], "exact9")
-        s["ntri"] = c.mesh_info(s["mid"])["ntri"]
+        s["mid"] = main_ctx.upload_mesh(s["tri"], "exact9")
+        s["ntri"] = main_ctx.mesh_info(s["mid"])["ntri"]
         s["out32"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
         s["out64"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
     pairs = float(sum(s["ntri"] * s["side"] ** 2 for s in shapes))
@@ -217,31 +220,51 @@ def main():
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s: {len(shapes)} shapes, "
           f"{sum(s['ntri'] for s in shapes)} triangles, {pairs:.3e} pairs/step", file=sys.stderr, flush=True)
 
-    def step(prec):
-        key = "out32" if prec == "fp32c" else "out64"
-        for s in shapes:
-            ctxs[s["side"]].ff_numeric(s["mid"], "approx", prec, flags=flags if prec == "fp32c" else 0, out=s[key])
+    items = {prec: [(grid_ctx[s["side"]], s["mid"], None, s["out32" if prec == "fp32c" else "out64"])
+                    for s in shapes] for prec in ("fp32c", "fp64")}
+    tuned = {}
+    if args.mode == "batch" and not args.no_tune:
+        from hipgisaxs_amd.autotune import autotune_batch
 
-    def timed(prec, steps, warmup):
+        best, times = autotune_batch(main_ctx, items["fp32c"], "fp32c", flags)
+        tuned = {"qpt": best[0], "blocks_per_unit": best[1],
+                 "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
+
+    def make_step(prec):
+        if args.mode == "batch":
+            b = Batch(main_ctx, items[prec], precision=prec, flags=flags if prec == "fp32c" else 0)
+            return lambda graph: b.run(use_graph=graph), b
+        def per_shape(graph):
+            for s in shapes:
+                grid_ctx[s["side"]].ff_numeric(s["mid"], "approx", prec, flags=flags if prec == "fp32c" else 0,
+                                               out=s["out32" if prec == "fp32c" else "out64"])
+        return per_shape, None
+
+    def timed(step, steps, warmup, graph):
         for _ in range(warmup):
-            step(prec)
+            step(graph)
         torch.cuda.synchronize()
-        for c in ctxs.values():
+        for c in [main_ctx, *grid_ctx.values()]:
             c.profile_begin()
         t0 = time.perf_counter()
         for _ in range(steps):
-            step(prec)
+            step(graph)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        prof = [c.profile_end() for c in ctxs.values()]
-        main_ms = sum(p["main_ms"] for p in prof)
-        main_n = sum(p["main_launches"] for p in prof)
-        aux_ms = sum(p["aux_ms"] for p in prof)
-        return wall, main_ms, main_n, aux_ms
+        prof = [c.profile_end() for c in [main_ctx, *grid_ctx.values()]]
+        return wall, sum(p["main_ms"] for p in prof), sum(p["main_launches"] for p in prof), \
+            sum(p["aux_ms"] for p in prof)
 
-    wall32, k32_ms, n32, aux32 = timed("fp32c", args.steps, args.warmup)
+    step32, b32 = make_step("fp32c")
+    use_graph = args.mode == "batch"
+    # headline: the whole step replayed from the captured hipGraph
+    wall32, _, _, _ = timed(step32, args.steps, args.warmup, use_graph)
     print(f"[bench] fp32c {args.steps} steps: {wall32:.3f}s", file=sys.stderr, flush=True)
-    wall64, k64_ms, n64, aux64 = timed("fp64", args.fp64_steps, 1 if args.warmup else 0)
+    # roofline: the same kernels launched directly, so the hot kernel's own events are recorded
+    _, k32_ms, n32, aux32 = timed(step32, args.steps, 0, False)
+    step64, b64 = make_step("fp64")
+    wall64, k64_ms, n64, aux64 = timed(step64, args.fp64_steps, 1 if args.warmup else 0, use_graph)
     print(f"[bench] fp64 {args.fp64_steps} steps: {wall64:.3f}s", file=sys.stderr, flush=True)
 
     # accuracy of the FP32C outputs against the plain FP64 GPU outputs (all shapes)
@@ -259,7 +282,7 @@ def main():
     value32 = pairs * args.steps / wall32
     value64 = pairs * args.fp64_steps / wall64
     avg_launch_ms = k32_ms / max(n32, 1)
-    flops_per_launch = FLOP_PER_PAIR * pairs / len(shapes)
+    flops_per_launch = FLOP_PER_PAIR * pairs * args.steps / max(n32, 1)
     achieved = flops_per_launch / (avg_launch_ms * 1e-3)
     result = {
         "metric": METRIC,
@@ -282,6 +305,8 @@ def main():
             "triangles": int(sum(s["ntri"] for s in shapes)),
             "pairs_per_step": pairs,
             "accumulation": "kahan" if args.kahan else "fp32-block/fp64-carry",
+            "mode": args.mode + (" (hipGraph replay)" if use_graph else ""),
+            "tuning": tuned,
         },
         "fp64_plain": {"value": value64, "ms_per_step": 1e3 * wall64 / args.fp64_steps,
                        "kernel_ms_per_step": k64_ms / args.fp64_steps},
@@ -290,7 +315,7 @@ def main():
         "relL2_fp32c_vs_fp64_gpu_worst_shape": worst,
         "roofline": {
             "bound": "valu",
-            "kernel": "k_ff_approx32",
+            "kernel": "k_batch32" if args.mode == "batch" else "k_ff_approx32",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
@@ -302,6 +327,8 @@ def main():
             "aux_ms_per_step": aux32 / args.steps,
         },
     }
+    if b32 is not None:
+        result["config"]["work_units"] = b32.info()["units"]
     if not args.no_extra:
         result["lattice_spheres_bcc"] = bench_lattice()
         print(f"[bench] lattice done", file=sys.stderr, flush=True)

@@ -78,6 +78,15 @@ class FitModel(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("grid", _vp),
+        ("mesh_id", C.c_int),
+        ("rot", C.c_double * 9),
+        ("ff", _vp),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "hgx_version": (C.c_char_p, []),
@@ -111,6 +120,11 @@ _SIGS = {
     "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),
     "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "hgx_lattice_intensity": (C.c_int, [_vp, C.POINTER(LatticeModel), C.c_int, _vp, C.c_int, _vp]),
+    "hgx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(BatchItem), C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "hgx_batch_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_tune_batch": (C.c_int, [C.c_int, C.c_int]),
     "hgx_fit_plan_create": (C.c_int, [_vp, C.POINTER(FitModel), _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_fit_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_chi2_fwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _vp]),

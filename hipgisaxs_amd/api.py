@@ -254,6 +254,41 @@ def lattice_intensity(ctx: Context, shape: str, dims, la, lb, lc, repet, rots, w
     return o
 
 
+class Batch:
+    """hgx_batch_create / hgx_batch_run: items are (grid_context_or_None, mesh_id, rot or None, out tensor)."""
+
+    def __init__(self, ctx: Context, items, precision="fp32c", flags=0):
+        arr = (N.BatchItem * len(items))()
+        self._keep = []
+        for k, (gctx, mid, rot, out) in enumerate(items):
+            arr[k].grid = None if gctx is None else gctx.h
+            arr[k].mesh_id = mid
+            arr[k].rot[:] = [float(x) for x in (np.eye(3) if rot is None else np.asarray(rot)).ravel()]
+            arr[k].ff = C.c_void_p(out.data_ptr())
+            self._keep.append(out)
+        self.ctx = ctx
+        bid = C.c_int()
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        ctx._check(ctx.lib.hgx_batch_create(ctx.h, len(items), arr, p, flags, C.byref(bid)), "batch_create")
+        self.id = bid.value
+
+    def run(self, use_graph=False):
+        self.ctx._check(self.ctx.lib.hgx_batch_run(self.ctx.h, self.id, int(use_graph)), "batch_run")
+
+    def info(self):
+        u, p = C.c_int64(), C.c_double()
+        self.ctx._check(self.ctx.lib.hgx_batch_info(self.ctx.h, self.id, C.byref(u), C.byref(p)), "batch_info")
+        return {"units": u.value, "pairs": p.value}
+
+    def release(self):
+        self.ctx._check(self.ctx.lib.hgx_batch_release(self.ctx.h, self.id), "batch_release")
+
+
+def tune_batch(qpt: int, blocks_per_unit: int) -> None:
+    if N.load().hgx_tune_batch(qpt, blocks_per_unit) != N.HGX_OK:
+        raise HgxError("tune_batch: invalid knob values")
+
+
 class FitPlan:
     """hgx_fit_plan_create / hgx_chi2_fwd / hgx_chi2_bwd. theta = (radius, height, a, tilt, sigma, scale)."""
 

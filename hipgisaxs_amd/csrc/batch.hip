@@ -1,0 +1,338 @@
+// batch.hip -- many numeric form factors (own mesh, own q-grid, own rotation each) in one launch.
+//
+// The batch is cut into work units (item, q-tile of kWG x QPT q-points, chunk of triangle blocks) of
+// similar cost, sorted longest-first and dispatched as one grid, so the hardware dispatcher hands
+// the next unit to whichever CU frees up (longest-processing-time-first list scheduling). Uneven
+// meshes (500 - 50 000 triangles) and grids (128^2 - 1024^2) then load-balance in a single launch.
+// Units write disjoint FP64 partial slots; one finishing launch reduces them in chunk order
+// (deterministic) and applies the per-q epilogue. Both launches can be replayed from a hipGraph.
+// The reference computes one shape per call with its own allocations and copies
+// (src/ff/gpu/ff_tri_gpu.cu:395-521).
+#include <algorithm>
+
+#include "numeric_core.hpp"
+
+namespace hgx {
+
+struct BatchItemDev {
+  QView g;
+  Rot R;
+  Epi e;
+  const float* rec32;
+  const double* borig;
+  const double* rec64;
+  int64_t nblocks;
+  int64_t ntri;
+  int64_t bchunk;  // blocks per chunk
+  int nchunks;
+  double* part;
+  double* out;
+};
+
+struct BatchUnit {
+  int item;
+  int qtile;
+  int chunk;
+  int pad;
+};
+
+struct FinishUnit {
+  int item;
+  int qblock;
+};
+
+template <int QPT>
+__device__ inline void store_unit(const BatchItemDev& it, int chunk, int64_t qbase, const double (&ar)[QPT],
+                                  const double (&ai)[QPT]) {
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    if (q >= it.g.nq) continue;
+    // raw sums; k_batch_finish applies the epilogue (in place when there is a single chunk)
+    double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)chunk * it.g.nq + q);
+    dst[0] = ar[k];
+    dst[1] = ai[k];
+  }
+}
+
+template <int QPT, bool CQ, bool KAHAN>
+__global__ __launch_bounds__(kWG) void k_batch32(const BatchItemDev* __restrict__ items,
+                                                 const BatchUnit* __restrict__ units) {
+  const BatchUnit u = units[blockIdx.x];
+  const BatchItemDev& it = items[u.item];
+  const int64_t qbase = (int64_t)u.qtile * (kWG * QPT) + threadIdx.x;
+  const int64_t b0 = (int64_t)u.chunk * it.bchunk;
+  const int64_t b1 = min(it.nblocks, b0 + it.bchunk);
+  double ar[QPT], ai[QPT];
+  approx32_blocks<QPT, CQ, KAHAN>(it.g, it.R, qbase, it.rec32, it.borig, b0, b1, ar, ai);
+  store_unit<QPT>(it, u.chunk, qbase, ar, ai);
+}
+
+template <int QPT, bool CQ>
+__global__ __launch_bounds__(kWG) void k_batch64(const BatchItemDev* __restrict__ items,
+                                                 const BatchUnit* __restrict__ units) {
+  const BatchUnit u = units[blockIdx.x];
+  const BatchItemDev& it = items[u.item];
+  const int64_t qbase = (int64_t)u.qtile * (kWG * QPT) + threadIdx.x;
+  const int64_t t0 = (int64_t)u.chunk * it.bchunk * kBlockTri;
+  const int64_t t1 = min(it.ntri, t0 + it.bchunk * kBlockTri);
+  cd mq[QPT][3];
+  double ar[QPT], ai[QPT];
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    load_q<CQ>(it.g, it.R, q < it.g.nq ? q : it.g.nq - 1, mq[k]);
+  }
+  approx64_tris<QPT, CQ>(mq, it.rec64, t0, t1, ar, ai);
+  store_unit<QPT>(it, u.chunk, qbase, ar, ai);
+}
+
+__global__ __launch_bounds__(kWG) void k_batch_finish(const BatchItemDev* __restrict__ items,
+                                                      const FinishUnit* __restrict__ units) {
+  const FinishUnit u = units[blockIdx.x];
+  const BatchItemDev& it = items[u.item];
+  const int64_t q = (int64_t)u.qblock * kWG + threadIdx.x;
+  if (q >= it.g.nq) return;
+  cd mq[3];
+  load_q(it.g, it.R, q, mq);
+  double pr = 0.0, pi = 0.0;
+  if (it.nchunks == 1) {
+    pr = it.out[2 * q];
+    pi = it.out[2 * q + 1];
+  }
+  for (int c = 0; it.nchunks > 1 && c < it.nchunks; ++c) {
+    pr += it.part[2 * ((int64_t)c * it.g.nq + q)];
+    pi += it.part[2 * ((int64_t)c * it.g.nq + q) + 1];
+  }
+  cd F = finish_q(it.e, mq, cd{pr, pi});
+  it.out[2 * q] = F.r;
+  it.out[2 * q + 1] = F.i;
+}
+
+}  // namespace hgx
+
+using namespace hgx;
+
+namespace {
+
+int g_batch_qpt = 4;
+int g_batch_bpu = 16;  // triangle blocks per work unit
+
+using Batch = BatchState;
+
+std::vector<Batch>& batches(Context* c) { return c->batches; }
+
+template <int QPT>
+void launch_batch(const Batch& b, hipStream_t s) {
+  if (b.precision == HGX_PREC_FP64) {
+    if (b.cq) k_batch64<QPT, true><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+    else k_batch64<QPT, false><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+  } else {
+    bool kahan = (b.flags & HGX_FLAG_ACC_KAHAN) != 0;
+    if (b.cq) {
+      if (kahan) k_batch32<QPT, true, true><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+      else k_batch32<QPT, true, false><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+    } else {
+      if (kahan) k_batch32<QPT, false, true><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+      else k_batch32<QPT, false, false><<<b.nunits, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const BatchUnit*>(b.d_units));
+    }
+  }
+}
+
+void launch_main(const Batch& b, hipStream_t s) {
+  switch (b.qpt) {
+    case 1: launch_batch<1>(b, s); break;
+    case 2: launch_batch<2>(b, s); break;
+    case 4: launch_batch<4>(b, s); break;
+    default: launch_batch<8>(b, s); break;
+  }
+}
+
+void launch_finish(const Batch& b, hipStream_t s) {
+  if (b.nfinish > 0) k_batch_finish<<<b.nfinish, kWG, 0, s>>>(static_cast<const BatchItemDev*>(b.d_items), static_cast<const FinishUnit*>(b.d_finish));
+}
+
+}  // namespace
+
+namespace hgx {
+void free_batch_state(BatchState& b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  if (b.d_items) (void)hipFree(b.d_items);
+  if (b.d_units) (void)hipFree(b.d_units);
+  if (b.d_finish) (void)hipFree(b.d_finish);
+  if (b.d_part) (void)hipFree(b.d_part);
+  b = BatchState();
+}
+}  // namespace hgx
+
+extern "C" int hgx_tune_batch(int qpt, int blocks_per_unit) {
+  if ((qpt != 1 && qpt != 2 && qpt != 4 && qpt != 8) || blocks_per_unit < 1) return HGX_ERR_INVALID_ARG;
+  g_batch_qpt = qpt;
+  g_batch_bpu = blocks_per_unit;
+  return HGX_OK;
+}
+
+extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, int precision, int flags, int* batch_id) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (n < 1 || !items || !batch_id) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: need items");
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
+    return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: bad precision");
+  if (int rc = set_device(c)) return rc;
+  Batch b;
+  b.precision = precision;
+  b.flags = flags;
+  b.qpt = g_batch_qpt;
+  const int64_t bpu = g_batch_bpu;
+  std::vector<BatchItemDev> it(n);
+  std::vector<int64_t> part_off(n, -1);
+  int64_t part_total = 0;
+  for (int i = 0; i < n; ++i) {
+    const hgx_batch_item& x = items[i];
+    Context* gc = x.grid ? reinterpret_cast<Context*>(x.grid) : c;
+    if (!gc->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "batch_create: item %d has no q-grid", i);
+    if (gc->device != c->device) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: item %d on another device", i);
+    if (x.mesh_id < 0 || x.mesh_id >= (int)c->meshes.size() || !c->meshes[x.mesh_id].live)
+      return set_error(c, HGX_ERR_BAD_HANDLE, "batch_create: item %d: unknown mesh %d", i, x.mesh_id);
+    if (!x.ff) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: item %d has no output", i);
+    const Mesh& m = c->meshes[x.mesh_id];
+    BatchItemDev& d = it[i];
+    d.g = QView{gc->d_qx, gc->d_qy, gc->d_qz, gc->npix, gc->nq};
+    d.R = make_rot(x.rot);
+    d.e.tv = make_vec(nullptr);
+    d.e.has_tv = 0;
+    d.e.volume = m.volume;
+    d.e.moments = m.d_moments;
+    for (int k = 0; k < 3; ++k) d.e.origin[k] = m.origin[k];
+    d.e.radius = m.radius;
+    d.e.use_series = precision == HGX_PREC_FP32C ? 1 : 0;
+    d.rec32 = m.d_rec32;
+    d.borig = m.d_borig;
+    d.rec64 = m.d_rec64;
+    d.nblocks = m.nblocks;
+    d.ntri = m.ntri;
+    d.bchunk = std::min<int64_t>(bpu, m.nblocks);
+    d.nchunks = (int)((m.nblocks + d.bchunk - 1) / d.bchunk);
+    d.out = x.ff;
+    d.part = nullptr;
+    if (d.nchunks > 1) {
+      part_off[i] = part_total;
+      part_total += (int64_t)d.nchunks * gc->nq * 2;
+    }
+    b.cq = b.cq || !gc->qz_real;
+    b.pairs += (double)gc->nq * (double)m.ntri;
+  }
+  // work units, longest first
+  struct U {
+    double cost;
+    BatchUnit u;
+  };
+  std::vector<U> us;
+  std::vector<FinishUnit> fin;
+  const int64_t qtile = (int64_t)kWG * b.qpt;
+  for (int i = 0; i < n; ++i) {
+    const BatchItemDev& d = it[i];
+    int64_t nqt = (d.g.nq + qtile - 1) / qtile;
+    for (int64_t t = 0; t < nqt; ++t) {
+      double nqpts = (double)std::min<int64_t>(qtile, d.g.nq - t * qtile);
+      for (int ch = 0; ch < d.nchunks; ++ch) {
+        int64_t nb = std::min<int64_t>(d.bchunk, d.nblocks - ch * d.bchunk);
+        us.push_back(U{nqpts * (double)nb, BatchUnit{i, (int)t, ch, 0}});
+      }
+    }
+    for (int64_t qb = 0; qb < (d.g.nq + kWG - 1) / kWG; ++qb) fin.push_back(FinishUnit{i, (int)qb});
+  }
+  std::stable_sort(us.begin(), us.end(), [](const U& a, const U& b2) { return a.cost > b2.cost; });
+  std::vector<BatchUnit> units(us.size());
+  for (size_t k = 0; k < us.size(); ++k) units[k] = us[k].u;
+  if (units.size() > 0x7fffffff) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: too many work units");
+  b.nunits = (int)units.size();
+  b.nfinish = (int)fin.size();
+  auto fail = [&](const char* what) {
+    (void)hipGetLastError();
+    free_batch_state(b);
+    return set_error(c, HGX_ERR_OOM, "batch_create: allocation of %s failed", what);
+  };
+  if (part_total > 0 && hipMalloc(&b.d_part, part_total * sizeof(double)) != hipSuccess) return fail("partials");
+  for (int i = 0; i < n; ++i)
+    if (part_off[i] >= 0) it[i].part = b.d_part + part_off[i];
+  if (hipMalloc(&b.d_items, it.size() * sizeof(BatchItemDev)) != hipSuccess) return fail("items");
+  if (hipMalloc(&b.d_units, units.size() * sizeof(BatchUnit)) != hipSuccess) return fail("units");
+  if (!fin.empty() && hipMalloc(&b.d_finish, fin.size() * sizeof(FinishUnit)) != hipSuccess) return fail("finish");
+  HGX_CHECK_HIP(c, hipMemcpy(b.d_items, it.data(), it.size() * sizeof(BatchItemDev), hipMemcpyHostToDevice));
+  HGX_CHECK_HIP(c, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(BatchUnit), hipMemcpyHostToDevice));
+  if (!fin.empty())
+    HGX_CHECK_HIP(c, hipMemcpy(b.d_finish, fin.data(), fin.size() * sizeof(FinishUnit), hipMemcpyHostToDevice));
+  b.live = true;
+  auto& B = batches(c);
+  for (size_t i = 0; i < B.size(); ++i)
+    if (!B[i].live) {
+      B[i] = b;
+      *batch_id = (int)i;
+      return HGX_OK;
+    }
+  B.push_back(b);
+  *batch_id = (int)B.size() - 1;
+  return HGX_OK;
+}
+
+extern "C" int hgx_batch_run(hgx_ctx* h, int batch_id, int use_graph) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  auto& B = batches(c);
+  if (batch_id < 0 || batch_id >= (int)B.size() || !B[batch_id].live)
+    return set_error(c, HGX_ERR_BAD_HANDLE, "batch_run: unknown batch %d", batch_id);
+  if (int rc = set_device(c)) return rc;
+  Batch& b = B[batch_id];
+  if (!use_graph) {
+    {
+      LaunchTimer lt(c, true);
+      launch_main(b, c->stream);
+    }
+    HGX_CHECK_LAUNCH(c);
+    {
+      LaunchTimer lt(c, false);
+      launch_finish(b, c->stream);
+    }
+    HGX_CHECK_LAUNCH(c);
+    return HGX_OK;
+  }
+  if (!b.exec) {
+    // capture both launches once on a private stream, then replay on the context's stream
+    hipStream_t cap;
+    HGX_CHECK_HIP(c, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    HGX_CHECK_HIP(c, hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    launch_main(b, cap);
+    launch_finish(b, cap);
+    hipError_t e1 = hipStreamEndCapture(cap, &b.graph);
+    (void)hipStreamDestroy(cap);
+    if (e1 != hipSuccess) return set_error(c, HGX_ERR_DEVICE, "batch_run: capture failed: %s", hipGetErrorString(e1));
+    HGX_CHECK_HIP(c, hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  }
+  LaunchTimer lt(c, true);
+  HGX_CHECK_HIP(c, hipGraphLaunch(b.exec, c->stream));
+  return HGX_OK;
+}
+
+extern "C" int hgx_batch_info(hgx_ctx* h, int batch_id, int64_t* units, double* pairs) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  auto& B = batches(c);
+  if (batch_id < 0 || batch_id >= (int)B.size() || !B[batch_id].live) return HGX_ERR_BAD_HANDLE;
+  if (units) *units = B[batch_id].nunits;
+  if (pairs) *pairs = B[batch_id].pairs;
+  return HGX_OK;
+}
+
+extern "C" int hgx_batch_release(hgx_ctx* h, int batch_id) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  auto& B = batches(c);
+  if (batch_id < 0 || batch_id >= (int)B.size() || !B[batch_id].live)
+    return set_error(c, HGX_ERR_BAD_HANDLE, "batch_release: unknown batch %d", batch_id);
+  if (int rc = set_device(c)) return rc;
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  free_batch_state(B[batch_id]);
+  return HGX_OK;
+}

@@ -19,88 +19,9 @@
 // exp(i q . o_b) and added to an FP64 accumulator. For |q| R < kSeriesQR (R = mesh radius about
 // its centroid) the result is replaced by the FP64 moment series of the same sum, which has no
 // cancellation at small q.
-#include "hgx_internal.hpp"
+#include "numeric_core.hpp"
 
 namespace hgx {
-
-constexpr int kWG = 256;
-constexpr double kSeriesQR = 0.1;
-constexpr double kTwoPi = 6.283185307179586476925286766559;
-constexpr double kInvTwoPi = 0.15915494309189533576888376337251;
-
-struct QView {
-  const double* __restrict__ qx;
-  const double* __restrict__ qy;
-  const double* __restrict__ qz;
-  int64_t npix, nq;
-};
-
-struct Epi {
-  Vec3 tv;
-  int has_tv;
-  double volume;
-  // FP32C small-q series
-  const double* moments;
-  double origin[3];
-  double radius;
-  int use_series;
-};
-
-__device__ inline void load_q(const QView& g, const Rot& R, int64_t q, cd mq[3]) {
-  int64_t p = q % g.npix;
-  cd qz = cd{g.qz[2 * q], g.qz[2 * q + 1]};
-  rotate_q(R, g.qx[p], g.qy[p], qz, mq);
-}
-
-__constant__ double c_fact[kSeriesOrder + 1] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0};
-
-// sum_{k=0..K} (i^k / k!) sum_t s_t (q.n_t) (q.(c_t - o))^k from the mesh moments.
-__device__ cd moment_series(const double* __restrict__ M, const cd mq[3]) {
-  cd p[3][kSeriesOrder + 1];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    p[j][0] = cd{1.0, 0.0};
-#pragma unroll
-    for (int k = 1; k <= kSeriesOrder; ++k) p[j][k] = p[j][k - 1] * mq[j];
-  }
-  cd total = cd{0.0, 0.0};
-  cd ik = cd{1.0, 0.0};
-  int idx = 0;
-  for (int k = 0; k <= kSeriesOrder; ++k) {
-    cd Tk = cd{0.0, 0.0};
-    for (int i = 0; i < 3; ++i) {
-      cd s = cd{0.0, 0.0};
-      for (int a = k; a >= 0; --a)
-        for (int b = k - a; b >= 0; --b) {
-          int c = k - a - b;
-          double coef = c_fact[k] / (c_fact[a] * c_fact[b] * c_fact[c]);
-          s = s + (coef * M[idx++]) * (p[0][a] * p[1][b] * p[2][c]);
-        }
-      Tk = Tk + mq[i] * s;
-    }
-    total = total + (1.0 / c_fact[k]) * (ik * Tk);
-    ik = mul_i(ik);
-  }
-  return total;
-}
-
-// Final value of one q-point from the raw sum P = sum_t (q . s n) exp(i q . c) (FP64 path), or
-// P = sum_b exp(i q.o_b) sum_{t in b} ... (FP32C path, identical value).
-__device__ inline cd finish_q(const Epi& e, const cd mq[3], cd P) {
-  double q2 = norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]);
-  cd F;
-  if (q2 == 0.0) {
-    F = cd{e.volume, 0.0};
-  } else if (e.use_series && sqrt(q2) * e.radius < kSeriesQR) {
-    cd S = moment_series(e.moments, mq);
-    cd ph = mq[0] * e.origin[0] + mq[1] * e.origin[1] + mq[2] * e.origin[2];
-    F = (1.0 / q2) * mul_mi(S * cexpi_d(ph));
-  } else {
-    F = (1.0 / q2) * mul_mi(P);
-  }
-  if (e.has_tv) F = F * cexpi_d(mq[0] * e.tv.v[0] + mq[1] * e.tv.v[1] + mq[2] * e.tv.v[2]);
-  return F;
-}
 
 // ----- FP64 plain approximate kernel ---------------------------------------------------------
 
@@ -117,66 +38,21 @@ __global__ __launch_bounds__(kWG) void k_ff_approx64(QView g, Rot R, const doubl
 #pragma unroll
   for (int k = 0; k < QPT; ++k) {
     int64_t q = qbase + (int64_t)k * kWG;
-    load_q(g, R, q < g.nq ? q : g.nq - 1, mq[k]);
-    ar[k] = 0.0;
-    ai[k] = 0.0;
+    load_q<CQ>(g, R, q < g.nq ? q : g.nq - 1, mq[k]);
   }
-  for (int64_t t = t0; t < t1; ++t) {
-    const double* r = rec + 8 * t;
-    const double snx = r[0], sny = r[1], snz = r[2], cx = r[3], cy = r[4], cz = r[5];
-#pragma unroll
-    for (int k = 0; k < QPT; ++k) {
-      if (!CQ) {
-        double qn = mq[k][0].r * snx + mq[k][1].r * sny + mq[k][2].r * snz;
-        double qt = mq[k][0].r * cx + mq[k][1].r * cy + mq[k][2].r * cz;
-        double s, c;
-        sincos(qt, &s, &c);
-        ar[k] += qn * c;
-        ai[k] += qn * s;
-      } else {
-        cd qn = mq[k][0] * snx + mq[k][1] * sny + mq[k][2] * snz;
-        cd qt = mq[k][0] * cx + mq[k][1] * cy + mq[k][2] * cz;
-        cd E = cexpi_d(qt);
-        ar[k] += qn.r * E.r - qn.i * E.i;
-        ai[k] += qn.r * E.i + qn.i * E.r;
-      }
-    }
-  }
+  approx64_tris<QPT, CQ>(mq, rec, t0, t1, ar, ai);
 #pragma unroll
   for (int k = 0; k < QPT; ++k) {
     int64_t q = qbase + (int64_t)k * kWG;
     if (q >= g.nq) continue;
-    if (nchunks == 1) {
-      cd F = finish_q(e, mq[k], cd{ar[k], ai[k]});
-      out[2 * q] = F.r;
-      out[2 * q + 1] = F.i;
-    } else {
-      part[2 * ((int64_t)chunk * g.nq + q)] = ar[k];
-      part[2 * ((int64_t)chunk * g.nq + q) + 1] = ai[k];
-    }
+    // raw sums; k_ff_finish applies the epilogue (in place when there is a single chunk)
+    double* dst = nchunks == 1 ? out + 2 * q : part + 2 * ((int64_t)chunk * g.nq + q);
+    dst[0] = ar[k];
+    dst[1] = ai[k];
   }
 }
 
 // ----- FP32 compensated approximate kernel ----------------------------------------------------
-
-template <bool KAHAN>
-struct Acc32 {
-  float sr, si, cr, ci;
-  __device__ inline void zero() { sr = si = cr = ci = 0.0f; }
-  __device__ inline void add(float xr, float xi) {
-    if (KAHAN) {
-      float yr = xr - cr, yi = xi - ci;
-      float tr = sr + yr, ti = si + yi;
-      cr = (tr - sr) - yr;
-      ci = (ti - si) - yi;
-      sr = tr;
-      si = ti;
-    } else {
-      sr += xr;
-      si += xi;
-    }
-  }
-};
 
 template <int QPT, bool CQ, bool KAHAN>
 __global__ __launch_bounds__(kWG) void k_ff_approx32(QView g, Rot R, const float* __restrict__ rec,
@@ -187,92 +63,21 @@ __global__ __launch_bounds__(kWG) void k_ff_approx32(QView g, Rot R, const float
   const int chunk = blockIdx.y;
   const int64_t b0 = (int64_t)chunk * bchunk;
   const int64_t b1 = min(nblocks, b0 + bchunk);
-  cd mq[QPT][3];
-  float mx[QPT], my[QPT], mz[QPT], ux[QPT], uy[QPT], uz[QPT];
-  float mxi[QPT], myi[QPT], mzi[QPT];  // imaginary parts (complex q only)
   double ar[QPT], ai[QPT];
-#pragma unroll
-  for (int k = 0; k < QPT; ++k) {
-    int64_t q = qbase + (int64_t)k * kWG;
-    load_q(g, R, q < g.nq ? q : g.nq - 1, mq[k]);
-    mx[k] = (float)mq[k][0].r;
-    my[k] = (float)mq[k][1].r;
-    mz[k] = (float)mq[k][2].r;
-    ux[k] = (float)(mq[k][0].r * kInvTwoPi);
-    uy[k] = (float)(mq[k][1].r * kInvTwoPi);
-    uz[k] = (float)(mq[k][2].r * kInvTwoPi);
-    // attenuation exp(-Im(q).c) = exp2(-log2(e) Im(q).c)
-    mxi[k] = (float)(mq[k][0].i * 1.4426950408889634);
-    myi[k] = (float)(mq[k][1].i * 1.4426950408889634);
-    mzi[k] = (float)(mq[k][2].i * 1.4426950408889634);
-    ar[k] = 0.0;
-    ai[k] = 0.0;
-  }
-  for (int64_t b = b0; b < b1; ++b) {
-    Acc32<KAHAN> acc[QPT];
-#pragma unroll
-    for (int k = 0; k < QPT; ++k) acc[k].zero();
-    const float* rb = rec + (int64_t)8 * kBlockTri * b;
-#pragma unroll 4
-    for (int t = 0; t < kBlockTri; ++t) {
-      const float* r = rb + 8 * t;
-      const float snx = r[0], sny = r[1], snz = r[2], cx = r[3], cy = r[4], cz = r[5];
-#pragma unroll
-      for (int k = 0; k < QPT; ++k) {
-        // v_sin/v_cos take turns and reduce internally (measured max abs error 1.25e-7 up to
-        // +-250 turns, 2.6e-7 after an explicit v_fract); block-relative |ph| stays far below that
-        float ph = ux[k] * cx + uy[k] * cy + uz[k] * cz;
-        float s = __builtin_amdgcn_sinf(ph);
-        float c = __builtin_amdgcn_cosf(ph);
-        if (!CQ) {
-          float qn = mx[k] * snx + my[k] * sny + mz[k] * snz;
-          acc[k].add(qn * c, qn * s);
-        } else {
-          float att = __builtin_amdgcn_exp2f(-(mxi[k] * cx + myi[k] * cy + mzi[k] * cz));
-          float qnr = (mx[k] * snx + my[k] * sny + mz[k] * snz) * att;
-          float qni = (mxi[k] * snx + myi[k] * sny + mzi[k] * snz) * (att * 0.69314718055994531f);
-          acc[k].add(qnr * c - qni * s, qnr * s + qni * c);
-        }
-      }
-    }
-    // rotate the block sum by exp(i q . o_b): FP64 phase in turns, reduced in FP64, then the
-    // hardware sin/cos on the reduced FP32 angle
-    const double ox = borig[4 * b], oy = borig[4 * b + 1], oz = borig[4 * b + 2];
-#pragma unroll
-    for (int k = 0; k < QPT; ++k) {
-      double tb = (mq[k][0].r * ox + mq[k][1].r * oy + mq[k][2].r * oz) * kInvTwoPi;
-      float fr = (float)(tb - rint(tb));
-      double cb = (double)__builtin_amdgcn_cosf(fr), sb = (double)__builtin_amdgcn_sinf(fr);
-      if (CQ) {
-        double m = exp(-(mq[k][0].i * ox + mq[k][1].i * oy + mq[k][2].i * oz));
-        cb *= m;
-        sb *= m;
-      }
-      double pr = (double)acc[k].sr - (double)acc[k].cr, pi = (double)acc[k].si - (double)acc[k].ci;
-      if (!KAHAN) {
-        pr = (double)acc[k].sr;
-        pi = (double)acc[k].si;
-      }
-      ar[k] += pr * cb - pi * sb;
-      ai[k] += pr * sb + pi * cb;
-    }
-  }
+  approx32_blocks<QPT, CQ, KAHAN>(g, R, qbase, rec, borig, b0, b1, ar, ai);
 #pragma unroll
   for (int k = 0; k < QPT; ++k) {
     int64_t q = qbase + (int64_t)k * kWG;
     if (q >= g.nq) continue;
-    if (nchunks == 1) {
-      cd F = finish_q(e, mq[k], cd{ar[k], ai[k]});
-      out[2 * q] = F.r;
-      out[2 * q + 1] = F.i;
-    } else {
-      part[2 * ((int64_t)chunk * g.nq + q)] = ar[k];
-      part[2 * ((int64_t)chunk * g.nq + q) + 1] = ai[k];
-    }
+    // raw sums; k_ff_finish applies the epilogue (in place when there is a single chunk)
+    double* dst = nchunks == 1 ? out + 2 * q : part + 2 * ((int64_t)chunk * g.nq + q);
+    dst[0] = ar[k];
+    dst[1] = ai[k];
   }
 }
 
-// Reduces partial sums over chunks in chunk order and applies the per-q epilogue.
+// Reduces partial sums over chunks in chunk order (or takes the raw sum the hot kernel left in
+// `out` when there is one chunk) and applies the per-q epilogue, including the small-|q| series.
 __global__ __launch_bounds__(kWG) void k_ff_finish(QView g, Rot R, int nchunks, const double* __restrict__ part,
                                                    double* __restrict__ out, Epi e) {
   int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
@@ -280,7 +85,11 @@ __global__ __launch_bounds__(kWG) void k_ff_finish(QView g, Rot R, int nchunks, 
   cd mq[3];
   load_q(g, R, q, mq);
   double pr = 0.0, pi = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
+  if (nchunks == 1) {
+    pr = out[2 * q];
+    pi = out[2 * q + 1];
+  }
+  for (int c = 0; nchunks > 1 && c < nchunks; ++c) {
     pr += part[2 * ((int64_t)c * g.nq + q)];
     pi += part[2 * ((int64_t)c * g.nq + q) + 1];
   }
@@ -499,7 +308,7 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     }
     }
     HGX_CHECK_LAUNCH(c);
-    if (p.nchunks > 1) {
+    {
       LaunchTimer lt(c, false);
       k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
       HGX_CHECK_LAUNCH(c);

@@ -211,13 +211,28 @@ __global__ __launch_bounds__(kFitWG) void k_fit(QViewF g, const double* __restri
   }
 }
 
-// Sums the per-block partials in block order: one workgroup, thread k owns output k.
-__global__ void k_fit_finish(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
-  int k = threadIdx.x;
-  if (k >= kNOut) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * kNOut + k];
-  out[k] = s;
+// Sums the per-block partials: one workgroup of 256 threads; thread j adds blocks j, j+256, ... for
+// every output, then a fixed-shape LDS tree combines the 256 sums (same order on every run).
+__global__ __launch_bounds__(256) void k_fit_finish(const double* __restrict__ part, int nblocks,
+                                                    double* __restrict__ out) {
+  __shared__ double red[kNOut][256];
+  const int j = threadIdx.x;
+  double s[kNOut];
+#pragma unroll
+  for (int k = 0; k < kNOut; ++k) s[k] = 0.0;
+  for (int b = j; b < nblocks; b += 256)
+#pragma unroll
+    for (int k = 0; k < kNOut; ++k) s[k] += part[(int64_t)b * kNOut + k];
+#pragma unroll
+  for (int k = 0; k < kNOut; ++k) red[k][j] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (j < w)
+#pragma unroll
+      for (int k = 0; k < kNOut; ++k) red[k][j] += red[k][j + w];
+    __syncthreads();
+  }
+  if (j < kNOut) out[j] = red[j][0];
 }
 
 static std::vector<FitPlan>& plans(Context* c) { return c->fit_plans; }
@@ -263,7 +278,7 @@ static int run_fit(Context* c, int plan, const double theta[6], int precision, b
   HGX_CHECK_LAUNCH(c);
   {
     LaunchTimer lt(c, false);
-    k_fit_finish<<<1, 64, 0, c->stream>>>(part, (int)nblk, res);
+    k_fit_finish<<<1, 256, 0, c->stream>>>(part, (int)nblk, res);
   }
   HGX_CHECK_LAUNCH(c);
   double host[kNOut];

@@ -221,6 +221,7 @@ int hgx_destroy(hgx_ctx* h) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& m : c->meshes) free_mesh(m);
+  for (auto& b : c->batches) free_batch_state(b);
   free_qgrid(c);
   for (auto* v : {&c->ev_main, &c->ev_aux, &c->ev_pool})
     for (auto& e : *v) {

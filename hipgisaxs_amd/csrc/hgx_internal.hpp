@@ -46,9 +46,28 @@ struct FitPlan {
   const double* mask = nullptr;
 };
 
+// A prepared batch of numeric form factors (batch.hip): device work tables, partial-sum buffer
+// and the captured hipGraph.
+struct BatchState {
+  bool live = false;
+  int precision = 0, flags = 0, qpt = 4;
+  bool cq = false;
+  int nunits = 0, nfinish = 0;
+  void* d_items = nullptr;
+  void* d_units = nullptr;
+  void* d_finish = nullptr;
+  double* d_part = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double pairs = 0.0;
+};
+
+void free_batch_state(BatchState& b);
+
 struct Context {
   int device = 0;
   std::vector<FitPlan> fit_plans;
+  std::vector<BatchState> batches;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   // q-grid

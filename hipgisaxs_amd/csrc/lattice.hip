@@ -90,17 +90,27 @@ __global__ __launch_bounds__(256) void k_lattice64(QViewL g, const double* __res
   int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npix) return;
   const double qx = g.qx[p], qy = g.qy[p];
+  // the pixel's branch data, read once for all grains
+  // (fixed-bound unrolled loops keep the arrays in registers)
+  cd qzb[4], fcb[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= a.nb) break;
+    int64_t q = p + b * g.npix;
+    qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+    fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+  }
   double acc = 0.0;
   for (int o = 0; o < a.ng; ++o) {
     cd amp = cd{0.0, 0.0};
-    for (int b = 0; b < a.nb; ++b) {
-      int64_t q = p + b * g.npix;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= a.nb) break;
       cd mq[3];
-      rot_g(a, o, qx, qy, cd{g.qz[2 * q], g.qz[2 * q + 1]}, mq);
+      rot_g(a, o, qx, qy, qzb[b], mq);
       cd F = a.shape == HGX_SHAPE_SPHERE ? sphere64(mq, a) : cylinder64(mq, a);
       cd S = sf_value(mq, a.sf);
-      cd f = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
-      amp = amp + f * F * S;
+      amp = amp + fcb[b] * F * S;
     }
     amp = a.wg[o] * (cd{a.dn2r, a.dn2i} * amp);
     acc += norm2(amp);
@@ -180,12 +190,24 @@ __global__ __launch_bounds__(256) void k_lattice32(QViewL g, const double* __res
   int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npix) return;
   const double qx = g.qx[p], qy = g.qy[p];
+  // the pixel's branch data, read once for all grains
+  // (fixed-bound unrolled loops keep the arrays in registers)
+  double qzb[4];
+  cd fcb[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= a.nb) break;
+    int64_t q = p + b * g.npix;
+    qzb[b] = g.qz[2 * q];
+    fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+  }
   double acc = 0.0;
   for (int o = 0; o < a.ng; ++o) {
     cd amp = cd{0.0, 0.0};
-    for (int b = 0; b < a.nb; ++b) {
-      int64_t q = p + b * g.npix;
-      const double qz = g.qz[2 * q];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= a.nb) break;
+      const double qz = qzb[b];
       double m[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) m[j] = a.rot[o][3 * j] * qx + a.rot[o][3 * j + 1] * qy + a.rot[o][3 * j + 2] * qz;
@@ -195,8 +217,7 @@ __global__ __launch_bounds__(256) void k_lattice32(QViewL g, const double* __res
       else
         F = cylinder32((float)sqrt(m[0] * m[0] + m[1] * m[1]), (float)m[2], a);
       cf FS = F * sf32(m, a);
-      cd f = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
-      amp = amp + f * cd{(double)FS.r, (double)FS.i};
+      amp = amp + fcb[b] * cd{(double)FS.r, (double)FS.i};
     }
     amp = a.wg[o] * (cd{a.dn2r, a.dn2i} * amp);
     acc += norm2(amp);

@@ -195,6 +195,32 @@ int hgx_fresnel(hgx_ctx* ctx, int nl, const double* delta, const double* beta, c
 int hgx_dwba_intensity(hgx_ctx* ctx, const double* fc, const double* sf, const double* ff, double weight,
                        int accumulate, double* I);
 
+/* ---- batches of numeric form factors -------------------------------------------------------- */
+
+/* One item of a batch: a mesh of the batch's context evaluated on the q-grid held by `grid` (any
+ * context on the same device; NULL = the batch's context) under rotation rot, into ff (device,
+ * [2 * nq of that grid]). */
+typedef struct hgx_batch_item {
+  hgx_ctx* grid;
+  int mesh_id;
+  double rot[9];
+  double* ff;
+} hgx_batch_item;
+
+/* Prepares a batch: cuts it into work units (item, q-tile, run of triangle blocks) of similar cost,
+ * orders them longest first and allocates the partial-sum buffer once. No reference counterpart: the
+ * reference evaluates one shape per call (src/ff/gpu/ff_tri_gpu.cu:395-521). The grids and meshes the
+ * items name must stay unchanged while the batch lives. */
+int hgx_batch_create(hgx_ctx* ctx, int nitems, const hgx_batch_item* items, int precision, int flags, int* batch);
+/* Runs the batch: one launch over all work units plus one finishing launch. use_graph = 1 replays them
+ * from a hipGraph captured on the first such call (no allocation, no host work per run). */
+int hgx_batch_run(hgx_ctx* ctx, int batch, int use_graph);
+int hgx_batch_info(hgx_ctx* ctx, int batch, int64_t* units, double* pairs);
+int hgx_batch_release(hgx_ctx* ctx, int batch);
+/* Work-unit shape of later batches: q-points per thread (1, 2, 4, 8) and triangle blocks (128
+ * triangles each) per unit. Process-wide; set by the autotuner. */
+int hgx_tune_batch(int qpt, int blocks_per_unit);
+
 /* ---- fused grain-ensemble intensity ------------------------------------------------------- */
 
 /* A lattice of identical analytic particles seen by several grain orientations. */
