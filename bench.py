@@ -181,6 +181,8 @@ def main():
                     help="batch: one launch for all shapes (hipGraph replay); per-shape: one call per shape")
     ap.add_argument("--no-tune", action="store_true", help="skip the work-unit autotuning")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
+    ap.add_argument("--no-mfma", action="store_true",
+                    help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -216,7 +218,7 @@ def main():
         s["out32"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
         s["out64"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
     pairs = float(sum(s["ntri"] * s["side"] ** 2 for s in shapes))
-    flags = 2 if args.kahan else 0
+    flags = (2 if args.kahan else 0) | (4 if args.no_mfma else 0)
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s: {len(shapes)} shapes, "
           f"{sum(s['ntri'] for s in shapes)} triangles, {pairs:.3e} pairs/step", file=sys.stderr, flush=True)
 

@@ -130,6 +130,11 @@ class Context:
                                             C.byref(nb)), "qgrid_info")
         return {"npix": npix.value, "nq": nq.value, "nrow": nrow.value, "ncol": ncol.value, "nbranch": nb.value}
 
+    def is_product(self) -> bool:
+        v = C.c_int()
+        self._check(self.lib.hgx_qgrid_is_product(self.h, C.byref(v)), "qgrid_is_product")
+        return bool(v.value)
+
     def qgrid_get(self):
         i = self.info()
         qx = np.empty(i["npix"])
@@ -287,6 +292,13 @@ class Batch:
 def tune_batch(qpt: int, blocks_per_unit: int) -> None:
     if N.load().hgx_tune_batch(qpt, blocks_per_unit) != N.HGX_OK:
         raise HgxError("tune_batch: invalid knob values")
+
+
+def tune_product(blocks_per_unit: int, complex_mults: int = 3) -> None:
+    """Work-unit size (128-triangle blocks) of the product-grid matrix-core kernel and 3 or 4 real
+    products per complex product."""
+    if N.load().hgx_tune_product(blocks_per_unit, complex_mults) != N.HGX_OK:
+        raise HgxError("tune_product: invalid knob values")
 
 
 class FitPlan:

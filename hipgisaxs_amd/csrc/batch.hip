@@ -140,6 +140,10 @@ void launch_batch(const Batch& b, hipStream_t s) {
 }
 
 void launch_main(const Batch& b, hipStream_t s) {
+  if (b.npunits > 0)
+    product_launch_units(s, static_cast<const ProductItem*>(b.d_pitems), static_cast<const ProductUnit*>(b.d_punits),
+                         b.npunits, b.cq, b.gauss);
+  if (b.nunits == 0) return;
   switch (b.qpt) {
     case 1: launch_batch<1>(b, s); break;
     case 2: launch_batch<2>(b, s); break;
@@ -162,6 +166,9 @@ void free_batch_state(BatchState& b) {
   if (b.d_units) (void)hipFree(b.d_units);
   if (b.d_finish) (void)hipFree(b.d_finish);
   if (b.d_part) (void)hipFree(b.d_part);
+  if (b.d_pitems) (void)hipFree(b.d_pitems);
+  if (b.d_punits) (void)hipFree(b.d_punits);
+  if (b.d_prot) (void)hipFree(b.d_prot);
   b = BatchState();
 }
 }  // namespace hgx
@@ -184,10 +191,13 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   b.precision = precision;
   b.flags = flags;
   b.qpt = g_batch_qpt;
+  b.gauss = product_gauss();
   const int64_t bpu = g_batch_bpu;
   std::vector<BatchItemDev> it(n);
-  std::vector<int64_t> part_off(n, -1);
-  int64_t part_total = 0;
+  std::vector<ProductItem> pit(n);
+  std::vector<char> is_prod(n, 0);
+  std::vector<int64_t> part_off(n, -1), rot_off(n, -1);
+  int64_t part_total = 0, rot_total = 0;  // doubles, bytes
   for (int i = 0; i < n; ++i) {
     const hgx_batch_item& x = items[i];
     Context* gc = x.grid ? reinterpret_cast<Context*>(x.grid) : c;
@@ -213,7 +223,32 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     d.nblocks = m.nblocks;
     d.ntri = m.ntri;
     d.bchunk = std::min<int64_t>(bpu, m.nblocks);
+    const bool prod = precision == HGX_PREC_FP32C && gc->product && !(flags & HGX_FLAG_NO_MFMA);
+    if (prod) {
+      // product grid: complex matrix products on the matrix cores (product_mfma.hip)
+      is_prod[i] = 1;
+      d.bchunk = std::min<int64_t>(product_blocks_per_unit(), m.nblocks);
+      ProductItem& pi = pit[i];
+      pi.qx = gc->d_qx;
+      pi.qy = gc->d_qy;
+      pi.qz = gc->d_qz;
+      pi.nblocks = m.nblocks;
+      pi.bchunk = d.bchunk;
+      pi.nq = gc->nq;
+      pi.ncol = gc->ncol;
+      pi.nz = gc->nbranch * gc->nrow;
+      bool identity = true;
+      for (int k = 0; k < 9; ++k) identity = identity && x.rot[k] == ((k % 4 == 0) ? 1.0 : 0.0);
+      if (identity) {
+        pi.rec = m.d_rec32;
+        pi.borig = m.d_borig;
+      } else {
+        rot_off[i] = rot_total;
+        rot_total += (int64_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double));
+      }
+    }
     d.nchunks = (int)((m.nblocks + d.bchunk - 1) / d.bchunk);
+    pit[i].nchunks = d.nchunks;
     d.out = x.ff;
     d.part = nullptr;
     if (d.nchunks > 1) {
@@ -229,10 +264,26 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     BatchUnit u;
   };
   std::vector<U> us;
+  struct PU {
+    double cost;
+    ProductUnit u;
+  };
+  std::vector<PU> pus;
   std::vector<FinishUnit> fin;
   const int64_t qtile = (int64_t)kWG * b.qpt;
   for (int i = 0; i < n; ++i) {
     const BatchItemDev& d = it[i];
+    for (int64_t qb = 0; qb < (d.g.nq + kWG - 1) / kWG; ++qb) fin.push_back(FinishUnit{i, (int)qb});
+    if (is_prod[i]) {
+      const ProductItem& pi = pit[i];
+      int nzt = (int)ceil_div(pi.nz, kProductTile), nyt = (int)ceil_div(pi.ncol, kProductTile);
+      for (int ch = 0; ch < d.nchunks; ++ch) {
+        int64_t nb = std::min<int64_t>(d.bchunk, d.nblocks - ch * d.bchunk);
+        for (int zt = 0; zt < nzt; ++zt)
+          for (int yt = 0; yt < nyt; ++yt) pus.push_back(PU{(double)nb, ProductUnit{i, zt, yt, ch}});
+      }
+      continue;
+    }
     int64_t nqt = (d.g.nq + qtile - 1) / qtile;
     for (int64_t t = 0; t < nqt; ++t) {
       double nqpts = (double)std::min<int64_t>(qtile, d.g.nq - t * qtile);
@@ -241,13 +292,17 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
         us.push_back(U{nqpts * (double)nb, BatchUnit{i, (int)t, ch, 0}});
       }
     }
-    for (int64_t qb = 0; qb < (d.g.nq + kWG - 1) / kWG; ++qb) fin.push_back(FinishUnit{i, (int)qb});
   }
   std::stable_sort(us.begin(), us.end(), [](const U& a, const U& b2) { return a.cost > b2.cost; });
+  std::stable_sort(pus.begin(), pus.end(), [](const PU& a, const PU& b2) { return a.cost > b2.cost; });
   std::vector<BatchUnit> units(us.size());
   for (size_t k = 0; k < us.size(); ++k) units[k] = us[k].u;
-  if (units.size() > 0x7fffffff) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: too many work units");
+  std::vector<ProductUnit> punits(pus.size());
+  for (size_t k = 0; k < pus.size(); ++k) punits[k] = pus[k].u;
+  if (units.size() > 0x7fffffff || punits.size() > 0x7fffffff)
+    return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: too many work units");
   b.nunits = (int)units.size();
+  b.npunits = (int)punits.size();
   b.nfinish = (int)fin.size();
   auto fail = [&](const char* what) {
     (void)hipGetLastError();
@@ -256,12 +311,37 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   };
   if (part_total > 0 && hipMalloc(&b.d_part, part_total * sizeof(double)) != hipSuccess) return fail("partials");
   for (int i = 0; i < n; ++i)
-    if (part_off[i] >= 0) it[i].part = b.d_part + part_off[i];
+    if (part_off[i] >= 0) it[i].part = pit[i].part = b.d_part + part_off[i];
+  for (int i = 0; i < n; ++i) pit[i].out = it[i].out;
+  if (rot_total > 0) {
+    if (hipMalloc(&b.d_prot, rot_total) != hipSuccess) return fail("rotated records");
+    for (int i = 0; i < n; ++i) {
+      if (rot_off[i] < 0) continue;
+      const Mesh& m = c->meshes[items[i].mesh_id];
+      char* base = static_cast<char*>(b.d_prot) + rot_off[i];
+      float* rr = reinterpret_cast<float*>(base);
+      double* ro = reinterpret_cast<double*>(base + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
+      if (int rc = product_rotate_mesh(c, m, make_rot(items[i].rot), rr, ro, c->stream)) {
+        free_batch_state(b);
+        return rc;
+      }
+      pit[i].rec = rr;
+      pit[i].borig = ro;
+    }
+  }
+  if (b.npunits > 0) {
+    if (hipMalloc(&b.d_pitems, pit.size() * sizeof(ProductItem)) != hipSuccess) return fail("product items");
+    if (hipMalloc(&b.d_punits, punits.size() * sizeof(ProductUnit)) != hipSuccess) return fail("product units");
+    HGX_CHECK_HIP(c, hipMemcpy(b.d_pitems, pit.data(), pit.size() * sizeof(ProductItem), hipMemcpyHostToDevice));
+    HGX_CHECK_HIP(c, hipMemcpy(b.d_punits, punits.data(), punits.size() * sizeof(ProductUnit), hipMemcpyHostToDevice));
+  }
   if (hipMalloc(&b.d_items, it.size() * sizeof(BatchItemDev)) != hipSuccess) return fail("items");
-  if (hipMalloc(&b.d_units, units.size() * sizeof(BatchUnit)) != hipSuccess) return fail("units");
+  if (!units.empty() && hipMalloc(&b.d_units, units.size() * sizeof(BatchUnit)) != hipSuccess) return fail("units");
   if (!fin.empty() && hipMalloc(&b.d_finish, fin.size() * sizeof(FinishUnit)) != hipSuccess) return fail("finish");
   HGX_CHECK_HIP(c, hipMemcpy(b.d_items, it.data(), it.size() * sizeof(BatchItemDev), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(BatchUnit), hipMemcpyHostToDevice));
+  if (!units.empty())
+    HGX_CHECK_HIP(c, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(BatchUnit), hipMemcpyHostToDevice));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));  // rotated records ready before the first run
   if (!fin.empty())
     HGX_CHECK_HIP(c, hipMemcpy(b.d_finish, fin.data(), fin.size() * sizeof(FinishUnit), hipMemcpyHostToDevice));
   b.live = true;
@@ -320,7 +400,7 @@ extern "C" int hgx_batch_info(hgx_ctx* h, int batch_id, int64_t* units, double* 
   if (!c) return HGX_ERR_BAD_HANDLE;
   auto& B = batches(c);
   if (batch_id < 0 || batch_id >= (int)B.size() || !B[batch_id].live) return HGX_ERR_BAD_HANDLE;
-  if (units) *units = B[batch_id].nunits;
+  if (units) *units = (int64_t)B[batch_id].nunits + B[batch_id].npunits;
   if (pairs) *pairs = B[batch_id].pairs;
   return HGX_OK;
 }

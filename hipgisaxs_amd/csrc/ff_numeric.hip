@@ -276,6 +276,57 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
   e.radius = m.radius;
   e.use_series = 0;
   const bool cq = !c->qz_real;  // a rotation keeps real q real
+  if (method == HGX_FF_APPROX && precision == HGX_PREC_FP32C && c->product && !(flags & HGX_FLAG_NO_MFMA)) {
+    // product grid: complex matrix products on the matrix cores (product_mfma.hip)
+    e.use_series = 1;
+    const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
+                                             R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
+    ProductItem it{};
+    it.qx = c->d_qx;
+    it.qy = c->d_qy;
+    it.qz = c->d_qz;
+    it.nblocks = m.nblocks;
+    it.nq = c->nq;
+    it.ncol = c->ncol;
+    it.nz = c->nbranch * c->nrow;
+    const int64_t tiles = (int64_t)ceil_div(it.nz, kProductTile) * ceil_div(it.ncol, kProductTile);
+    // enough waves to fill the chip several times over, without splitting below one block
+    int64_t want = std::max<int64_t>(1, (int64_t)(4 * g_target_wg + tiles - 1) / tiles);
+    want = std::min<int64_t>(want, m.nblocks);
+    it.bchunk = (m.nblocks + want - 1) / want;
+    it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
+    if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
+    size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
+    size_t rec_bytes = identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
+    if (int rc = ensure_work(c, part_bytes + rec_bytes)) return rc;
+    char* w = static_cast<char*>(c->d_work);
+    it.part = it.nchunks > 1 ? reinterpret_cast<double*>(w) : nullptr;
+    it.out = ff;
+    if (identity) {
+      it.rec = m.d_rec32;
+      it.borig = m.d_borig;
+    } else {
+      float* rr = reinterpret_cast<float*>(w + part_bytes);
+      double* ro = reinterpret_cast<double*>(w + part_bytes + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
+      {
+        LaunchTimer lt(c, false);
+        if (int rc = product_rotate_mesh(c, m, R, rr, ro, c->stream)) return rc;
+      }
+      it.rec = rr;
+      it.borig = ro;
+    }
+    {
+      LaunchTimer lt(c, true);
+      product_launch_single(c->stream, it, cq, product_gauss());
+    }
+    HGX_CHECK_LAUNCH(c);
+    {
+      LaunchTimer lt(c, false);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
+      HGX_CHECK_LAUNCH(c);
+    }
+    return HGX_OK;
+  }
   if (method == HGX_FF_APPROX) {
     Plan p;
     if (precision == HGX_PREC_FP64)

@@ -80,6 +80,7 @@ static void free_qgrid(Context* c) {
   c->npix = c->nq = 0;
   c->ncol = c->nrow = c->nbranch = 0;
   c->have_alpha = false;
+  c->product = false;
 }
 
 // Allocates grid arrays for nrow x ncol pixels and nbranch branches.
@@ -182,11 +183,44 @@ __global__ void k_qz_extended(int64_t npix, const double* __restrict__ qz_in, do
   }
 }
 
+// Clears *flag unless every q-point equals the product of the grid's axes: qx = qx[0],
+// qy = qy[column] of row 0 and qz = qz of column 0 of its (branch, row).
+__global__ void k_check_product(int ncol, int64_t npix, int nbranch, const double* __restrict__ qx,
+                                const double* __restrict__ qy, const double* __restrict__ qz, int* __restrict__ flag) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  int64_t row = p / ncol, j = p % ncol;
+  bool ok = qx[p] == qx[0] && qy[p] == qy[j];
+  for (int b = 0; b < nbranch; ++b) {
+    int64_t q = b * npix + p, q0 = b * npix + row * ncol;
+    ok = ok && qz[2 * q] == qz[2 * q0] && qz[2 * q + 1] == qz[2 * q0 + 1];
+  }
+  if (!ok) *flag = 0;  // every writer stores the same value
+}
+
 }  // namespace hgx
 
 using namespace hgx;
 
 static Context* ctx_of(hgx_ctx* c) { return reinterpret_cast<Context*>(c); }
+
+// Decides whether the current grid is a product grid (run once per grid change; synchronous).
+static int update_product(Context* c) {
+  c->product = false;
+  if (!c->d_flag && hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(c, HGX_ERR_OOM, "q-grid: flag allocation failed");
+  }
+  int one = 1, got = 0;
+  HGX_CHECK_HIP(c, hipMemcpyAsync(c->d_flag, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  k_check_product<<<ceil_div(c->npix, 256), 256, 0, c->stream>>>(c->ncol, c->npix, c->nbranch, c->d_qx, c->d_qy,
+                                                                 c->d_qz, c->d_flag);
+  HGX_CHECK_LAUNCH(c);
+  HGX_CHECK_HIP(c, hipMemcpyAsync(&got, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->product = got != 0;
+  return HGX_OK;
+}
 static const Context* ctx_of(const hgx_ctx* c) { return reinterpret_cast<const Context*>(c); }
 
 extern "C" {
@@ -229,6 +263,7 @@ int hgx_destroy(hgx_ctx* h) {
       (void)hipEventDestroy(e.second);
     }
   if (c->d_work) (void)hipFree(c->d_work);
+  if (c->d_flag) (void)hipFree(c->d_flag);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return HGX_OK;
@@ -372,7 +407,7 @@ int hgx_qgrid_explicit(hgx_ctx* h, int ncol, int nrow, double qy_min, double qz_
   HGX_CHECK_LAUNCH(c);
   c->qz_real = true;
   c->have_alpha = true;
-  return HGX_OK;
+  return update_product(c);
 }
 
 int hgx_qgrid_detector(hgx_ctx* h, int ncol, int nrow, double pixel, double distance, double col_center,
@@ -408,7 +443,7 @@ int hgx_qgrid_set(hgx_ctx* h, int ncol, int nrow, int nbranch, const double* qx,
   bool real = true;
   for (int64_t i = 0; i < c->nq && real; ++i) real = qz_ext[2 * i + 1] == 0.0;
   c->qz_real = real;
-  return HGX_OK;
+  return update_product(c);
 }
 
 int hgx_qz_extended(hgx_ctx* h, double k0, double alpha_i, double dnl_re, double dnl_im) {
@@ -431,7 +466,8 @@ int hgx_qz_extended(hgx_ctx* h, double k0, double alpha_i, double dnl_re, double
   c->nq = 4 * c->npix;
   // kzi and kzf are real exactly when the layer is lossless and every kzf0^2 >= k0^2 dnl
   c->qz_real = (dnl_re == 0.0 && dnl_im == 0.0);
-  return HGX_OK;
+  // the branches depend on the row only, so a product grid stays one
+  return c->product ? update_product(c) : HGX_OK;
 }
 
 int hgx_qgrid_info(const hgx_ctx* h, int64_t* npix, int64_t* nq, int* nrow, int* ncol, int* nbranch) {
@@ -442,6 +478,13 @@ int hgx_qgrid_info(const hgx_ctx* h, int64_t* npix, int64_t* nq, int* nrow, int*
   if (nrow) *nrow = c->nrow;
   if (ncol) *ncol = c->ncol;
   if (nbranch) *nbranch = c->nbranch;
+  return c->d_qz ? HGX_OK : HGX_ERR_NO_QGRID;
+}
+
+int hgx_qgrid_is_product(const hgx_ctx* h, int* is_product) {
+  const Context* c = ctx_of(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (is_product) *is_product = c->product ? 1 : 0;
   return c->d_qz ? HGX_OK : HGX_ERR_NO_QGRID;
 }
 
@@ -587,6 +630,17 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
     accumulate_moments(o, d, M.data());
   }
   m.radius = rad;
+  {
+    // A closed surface has sum_t s_t n_t = 0 (divergence theorem); what the sum holds then is
+    // rounding, which the series would divide by |q| and so blow up as q -> 0. Zero it for meshes
+    // closed to rounding; open meshes keep their (physical) 1/|q| term.
+    double asum = 0.0;
+    for (int64_t t = 0; t < ntri; ++t)
+      asum += std::sqrt(rec64[8 * t] * rec64[8 * t] + rec64[8 * t + 1] * rec64[8 * t + 1] +
+                        rec64[8 * t + 2] * rec64[8 * t + 2]);
+    double n0 = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+    if (n0 <= 1e-8 * asum) M[0] = M[1] = M[2] = 0.0;
+  }
   for (int64_t b = 0; b < m.nblocks; ++b) {
     int64_t t0 = b * kBlockTri, t1 = std::min<int64_t>(ntri, t0 + kBlockTri);
     double bo[3] = {0, 0, 0};

@@ -57,12 +57,56 @@ struct BatchState {
   void* d_units = nullptr;
   void* d_finish = nullptr;
   double* d_part = nullptr;
+  // product-grid items (FP32C on product grids): their own unit list and rotated records
+  bool gauss = true;
+  int npunits = 0;
+  void* d_pitems = nullptr;
+  void* d_punits = nullptr;
+  void* d_prot = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   double pairs = 0.0;
 };
 
 void free_batch_state(BatchState& b);
+
+// ----- product-grid numeric form factor on the matrix cores (product_mfma.hip) ------------------
+//
+// A q-grid is a product grid when q(row i, column j, branch b) = (qx0, qy_j, qz_{b,i}): the Born
+// q-space grid with qx = 0 and its four DWBA branches are. On such a grid the approximate-triangle
+// sum factorizes per triangle, exp(i q.c) = exp(i(qx0 cx + qy cy)) exp(i qz cz), so the sum over
+// triangles is a complex matrix product (z-rows x triangles) . (triangles x y-columns).
+
+// One form factor for the product-grid kernel. Triangle records and block origins are already
+// rotated (R^T applied), so the kernel works with the unrotated grid axes.
+struct ProductItem {
+  const double* qx;     // per-pixel grid arrays; qx[0] is qx0, qy[0..ncol) is the qy axis,
+  const double* qy;     // qz row zi (zi = branch * nrow + row) is at qz + 2 * zi * ncol
+  const double* qz;
+  const float* rec;     // [nblocks * kBlockTri][8] rotated s*n (3), c - o_b (3), 0, 0
+  const double* borig;  // [nblocks][4] rotated block origins
+  int64_t nblocks;
+  int64_t bchunk;       // blocks per chunk
+  int64_t nq;
+  int nchunks;
+  int ncol;
+  int nz;               // nbranch * nrow
+  int pad;
+  double* part;         // [nchunks][nq] complex partial sums (nchunks > 1)
+  double* out;          // [nq] complex raw sums (nchunks == 1)
+};
+
+// A work unit: one 32 (z) x 32 (y) tile of one item over one chunk of triangle blocks.
+struct ProductUnit {
+  int item;
+  int ztile;
+  int ytile;
+  int chunk;
+};
+
+constexpr int kProductTile = 32;
+constexpr int kProductWaves = 4;  // work units (waves) per workgroup
+
 
 struct Context {
   int device = 0;
@@ -74,6 +118,8 @@ struct Context {
   int ncol = 0, nrow = 0, nbranch = 0;
   int64_t npix = 0, nq = 0;
   bool qz_real = true;      // all Im(qz) == 0
+  bool product = false;     // q(row, col, branch) = (qx0, qy[col], qz[branch, row]) exactly
+  int* d_flag = nullptr;    // device scratch flag for the product-grid check
   bool have_alpha = false;
   double* d_qx = nullptr;   // [npix]
   double* d_qy = nullptr;   // [npix]
@@ -207,5 +253,17 @@ inline Vec3 make_vec(const double* v) {
 }
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Rotates an item's block-relative FP32 records and FP64 block origins by R^T.
+int product_rotate_mesh(Context* c, const Mesh& m, const Rot& R, float* rec_out, double* borig_out,
+                        hipStream_t s);
+// Launches the unit list of a batch (units sorted by the caller).
+void product_launch_units(hipStream_t s, const ProductItem* d_items, const ProductUnit* d_units, int nunits,
+                          bool cq, bool gauss);
+// Launches every (z-tile, y-tile, chunk) unit of a single item.
+void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, bool gauss);
+// Tuning: triangle blocks per work unit and 3- or 4-multiplication complex products.
+int product_blocks_per_unit();
+bool product_gauss();
 
 }  // namespace hgx

@@ -1,0 +1,280 @@
+// product_mfma.hip -- FP32C approximate-triangle form factor on product q-grids, on the matrix cores.
+//
+// On a product grid q(zi, yj) = (qx0, qy_j, qz_zi) (zi runs over branch-major rows) the raw sum of
+// the approximate method (src/ff/cpu/ff_tri_cpu.cpp:200-246)
+//     S(zi, yj) = sum_t (q . s_t n_t) exp(i q . c_t)
+// splits per triangle into a z factor and a y factor:
+//     q . s n = alpha_t(yj) + beta_t(zi),  alpha = s (qx0 nx + qy ny),  beta = s qz nz
+//     exp(i q.c) = Ey_t(yj) Ez_t(zi),      Ey = exp(i (qx0 cx + qy cy)),  Ez = exp(i qz cz)
+// so S = sum_t Ez_t(zi) [alpha_t Ey_t](yj) + [beta_t Ez_t](zi) Ey_t(yj): a complex matrix product
+// of (z-rows x 2 triangles-terms) by (2 triangle-terms x y-columns). Each wave owns a 32 x 32 tile
+// of S and runs it on v_mfma_f32_32x32x2_f32: the MFMA's K = 2 is two triangles (lane half h
+// supplies triangle 2p + h), the A operand (lane row r) is the z side, the B operand (lane column
+// r) the y side. Every lane computes its own table entries (one sincos pair for y, one for z per
+// triangle), so there is no LDS traffic and no barrier. Complex products use either three real
+// products (P1 = sum Xr Yr, P2 = sum Xi Yi, P3 = sum (Xr + Xi)(Yr + Yi); Re = P1 - P2,
+// Im = P3 - P1 - P2) or four.
+//
+// Precision follows the FP32C path of the per-pair kernels: within a 128-triangle block the
+// coordinates are FP32 relative to the block's FP64 origin o_b and phases are in turns for the
+// hardware v_sin/v_cos; the FP32 block sum is rotated by exp(i q . o_b) (FP64 phase, reduced before
+// the FP32 sincos) and added to FP64 accumulators. Output is the raw sum; the finishing kernels
+// apply -i/|q|^2 and the small-|q| series exactly as for the per-pair path.
+#include "numeric_core.hpp"
+
+namespace hgx {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+static int g_product_bpu = 16;
+static bool g_product_gauss = true;
+
+int product_blocks_per_unit() { return g_product_bpu; }
+bool product_gauss() { return g_product_gauss; }
+
+__global__ __launch_bounds__(256) void k_rotate_records(const float* __restrict__ rec, const double* __restrict__ borig,
+                                                        int64_t nblocks, Rot R, float* __restrict__ rec_out,
+                                                        double* __restrict__ borig_out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nblocks * kBlockTri) {
+    const float* a = rec + 8 * t;
+    float* o = rec_out + 8 * t;
+    // (R^T v)_k = sum_j R[j][k] v_j
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      o[k] = (float)(R.m[k] * a[0] + R.m[3 + k] * a[1] + R.m[6 + k] * a[2]);
+      o[3 + k] = (float)(R.m[k] * a[3] + R.m[3 + k] * a[4] + R.m[6 + k] * a[5]);
+    }
+    o[6] = 0.0f;
+    o[7] = 0.0f;
+  }
+  if (t < nblocks) {
+    const double* a = borig + 4 * t;
+    double* o = borig_out + 4 * t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = R.m[k] * a[0] + R.m[3 + k] * a[1] + R.m[6 + k] * a[2];
+    o[3] = 0.0;
+  }
+}
+
+__device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// accumulator register rr of lane (h = lane / 32) holds tile row 8 (rr / 4) + 4 h + rr % 4
+__device__ inline int acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (rr & 3); }
+
+template <bool CQ, bool GAUSS>
+__global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restrict__ items,
+                                                      const ProductUnit* __restrict__ units, int nunits,
+                                                      ProductItem single, int nzt, int nyt) {
+  // the wave index is uniform across the wave; saying so keeps the unit and item loads scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int uidx = (int)blockIdx.x * kProductWaves + wave;  // host keeps unit counts below 2^31
+  if (uidx >= nunits) return;  // whole waves leave; the kernel has no barriers
+  ProductUnit u;
+  if (units) {
+    u = units[uidx];
+  } else {
+    // single item: unit = (chunk, ztile, ytile), y fastest
+    u.item = 0;
+    u.ytile = uidx % nyt;
+    u.ztile = (uidx / nyt) % nzt;
+    u.chunk = uidx / (nyt * nzt);
+  }
+  // wave-uniform copy (scalar loads)
+  const ProductItem it = units ? items[u.item] : single;
+  const int r = lane & 31, h = lane >> 5;
+  const int z0 = u.ztile * kProductTile, y0 = u.ytile * kProductTile;
+  const int ncol = it.ncol, nz = it.nz;
+  // this lane's operand row (z) and column (y), clamped: an out-of-range row or column of the
+  // operands only feeds the same row or column of the product, which is never stored
+  const int zr = min(z0 + r, nz - 1), yc = min(y0 + r, ncol - 1);
+  const double qx0 = it.qx[0];
+  const double qyv = it.qy[yc];
+  const double qzr = it.qz[2 * (int64_t)zr * ncol];
+  const double qzi = CQ ? it.qz[2 * (int64_t)zr * ncol + 1] : 0.0;
+  const float ux0 = (float)(qx0 * kInvTwoPi), uy = (float)(qyv * kInvTwoPi), uz = (float)(qzr * kInvTwoPi);
+  const float qx0f = (float)qx0, qyf = (float)qyv, qzrf = (float)qzr, qzif = (float)qzi;
+  const float azf = (float)(qzi * 1.4426950408889634);  // log2(e) Im(qz): exp(-Im(qz) c) = exp2(-azf c)
+
+  double accr[16], acci[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) accr[k] = acci[k] = 0.0;
+
+  const int64_t b0 = (int64_t)u.chunk * it.bchunk;
+  const int64_t b1 = min(it.nblocks, b0 + it.bchunk);
+  // A block's records (128 x 32 B) are staged per wave in LDS; the next block's are loaded into
+  // registers while the current block runs, so global-memory latency stays off the matrix pipe.
+  // Only this wave touches its LDS slice, and LDS executes one wave's instructions in order, so
+  // wave-scope fences (which keep the compiler from reordering) are all the ordering needed.
+  __shared__ float4 s_rec[kProductWaves][2 * kBlockTri];
+  float4* const srec = s_rec[wave];
+  const float4* gsrc = reinterpret_cast<const float4*>(it.rec + 8 * b0 * kBlockTri) + lane;
+  float4 pf0 = {0, 0, 0, 0}, pf1 = pf0, pf2 = pf0, pf3 = pf0;
+  if (b0 < b1) {
+    pf0 = gsrc[0];
+    pf1 = gsrc[64];
+    pf2 = gsrc[128];
+    pf3 = gsrc[192];
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    srec[lane] = pf0;
+    srec[lane + 64] = pf1;
+    srec[lane + 128] = pf2;
+    srec[lane + 192] = pf3;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (b + 1 < b1) {
+      gsrc += 2 * kBlockTri;
+      pf0 = gsrc[0];
+      pf1 = gsrc[64];
+      pf2 = gsrc[128];
+      pf3 = gsrc[192];
+    }
+    f32x16 P1, P2, P3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = 0.0f;
+#pragma unroll 2
+    for (int p = 0; p < kBlockTri / 2; ++p) {
+      // this lane's triangle 2p + h: record float4s 2 (2p + h) and 2 (2p + h) + 1
+      const float4 ra = srec[4 * p + 2 * h];
+      const float4 rc = srec[4 * p + 2 * h + 1];
+      const float snx = ra.x, sny = ra.y, snz = ra.z, dcx = ra.w, dcy = rc.x, dcz = rc.y;
+      // y side: Ey = exp(i (qx0 dcx + qy dcy)), Y1 = alpha Ey, Y2 = Ey
+      const float phy = __builtin_fmaf(uy, dcy, ux0 * dcx);
+      const float eyr = __builtin_amdgcn_cosf(phy), eyi = __builtin_amdgcn_sinf(phy);
+      const float al = __builtin_fmaf(qyf, sny, qx0f * snx);
+      const float y1r = al * eyr, y1i = al * eyi;
+      // z side: Ez = exp(i qz dcz), Z1 = Ez, Z2 = beta Ez with beta = qz snz
+      const float phz = uz * dcz;
+      float ezr = __builtin_amdgcn_cosf(phz), ezi = __builtin_amdgcn_sinf(phz);
+      float z2r, z2i;
+      if (CQ) {
+        const float att = __builtin_amdgcn_exp2f(-azf * dcz);
+        ezr *= att;
+        ezi *= att;
+        const float br = qzrf * snz, bi = qzif * snz;
+        z2r = br * ezr - bi * ezi;
+        z2i = br * ezi + bi * ezr;
+      } else {
+        const float br = qzrf * snz;
+        z2r = br * ezr;
+        z2i = br * ezi;
+      }
+      if (GAUSS) {
+        P1 = mfma32(ezr, y1r, P1);
+        P1 = mfma32(z2r, eyr, P1);
+        P2 = mfma32(ezi, y1i, P2);
+        P2 = mfma32(z2i, eyi, P2);
+        P3 = mfma32(ezr + ezi, y1r + y1i, P3);
+        P3 = mfma32(z2r + z2i, eyr + eyi, P3);
+      } else {
+        // P1 = real part, P2 = imaginary part
+        P1 = mfma32(ezr, y1r, P1);
+        P1 = mfma32(-ezi, y1i, P1);
+        P1 = mfma32(z2r, eyr, P1);
+        P1 = mfma32(-z2i, eyi, P1);
+        P2 = mfma32(ezr, y1i, P2);
+        P2 = mfma32(ezi, y1r, P2);
+        P2 = mfma32(z2r, eyi, P2);
+        P2 = mfma32(z2i, eyr, P2);
+      }
+    }
+    // rotate the block sum by exp(i q . o_b) = exp(i (qx0 ox + qy oy)) exp(i qz oz), FP64 phases
+    // reduced before the FP32 sincos. Each lane evaluates the z factor of its operand row; lane i
+    // (i < 32) holds row z0 + i, which the accumulator layout reads back with cross-lane shuffles.
+    const double ox = it.borig[4 * b], oy = it.borig[4 * b + 1], oz = it.borig[4 * b + 2];
+    const double ty = (qx0 * ox + qyv * oy) * kInvTwoPi;
+    const float fty = (float)(ty - rint(ty));
+    const double fyr = (double)__builtin_amdgcn_cosf(fty), fyi = (double)__builtin_amdgcn_sinf(fty);
+    const double tz = qzr * oz * kInvTwoPi;
+    const float ftz = (float)(tz - rint(tz));
+    double fzr = (double)__builtin_amdgcn_cosf(ftz), fzi = (double)__builtin_amdgcn_sinf(ftz);
+    if (CQ) {
+      const double m = (double)__builtin_amdgcn_exp2f((float)(-qzi * oz * 1.4426950408889634));
+      fzr *= m;
+      fzi *= m;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int src = acc_row(k, h);
+      const double zr_k = __shfl(fzr, src), zi_k = __shfl(fzi, src);
+      const double fr = fyr * zr_k - fyi * zi_k, fi = fyr * zi_k + fyi * zr_k;
+      double sr, si;
+      if (GAUSS) {
+        sr = (double)P1[k] - (double)P2[k];
+        si = (double)P3[k] - (double)P1[k] - (double)P2[k];
+      } else {
+        sr = (double)P1[k];
+        si = (double)P2[k];
+      }
+      accr[k] += sr * fr - si * fi;
+      acci[k] += sr * fi + si * fr;
+    }
+  }
+  // raw sums: in place when the item has one chunk, else into the chunk's partial slot
+  const int yj = y0 + r;
+  if (yj >= ncol) return;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int zi = z0 + acc_row(k, h);
+    if (zi >= nz) continue;
+    const int64_t q = (int64_t)zi * ncol + yj;
+    double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)u.chunk * it.nq + q);
+    dst[0] = accr[k];
+    dst[1] = acci[k];
+  }
+}
+
+int product_rotate_mesh(Context* c, const Mesh& m, const Rot& R, float* rec_out, double* borig_out,
+                        hipStream_t s) {
+  int64_t n = m.nblocks * kBlockTri;
+  k_rotate_records<<<ceil_div(n, 256), 256, 0, s>>>(m.d_rec32, m.d_borig, m.nblocks, R, rec_out, borig_out);
+  HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}
+
+template <bool CQ, bool G>
+static void launch_t(hipStream_t s, const ProductItem* items, const ProductUnit* units, int nunits,
+                     const ProductItem& single, int nzt, int nyt) {
+  unsigned grid = ceil_div(nunits, kProductWaves);
+  k_ff_product32<CQ, G><<<grid, 256, 0, s>>>(items, units, nunits, single, nzt, nyt);
+}
+
+static void launch_any(hipStream_t s, const ProductItem* items, const ProductUnit* units, int nunits,
+                       const ProductItem& single, int nzt, int nyt, bool cq, bool gauss) {
+  if (nunits <= 0) return;
+  if (cq) {
+    if (gauss) launch_t<true, true>(s, items, units, nunits, single, nzt, nyt);
+    else launch_t<true, false>(s, items, units, nunits, single, nzt, nyt);
+  } else {
+    if (gauss) launch_t<false, true>(s, items, units, nunits, single, nzt, nyt);
+    else launch_t<false, false>(s, items, units, nunits, single, nzt, nyt);
+  }
+}
+
+void product_launch_units(hipStream_t s, const ProductItem* d_items, const ProductUnit* d_units, int nunits,
+                          bool cq, bool gauss) {
+  ProductItem none{};
+  launch_any(s, d_items, d_units, nunits, none, 1, 1, cq, gauss);
+}
+
+void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, bool gauss) {
+  int nzt = (int)ceil_div(item.nz, kProductTile), nyt = (int)ceil_div(item.ncol, kProductTile);
+  int nunits = nzt * nyt * item.nchunks;  // callers check the count fits
+  launch_any(s, nullptr, nullptr, nunits, item, nzt, nyt, cq, gauss);
+}
+
+}  // namespace hgx
+
+using namespace hgx;
+
+extern "C" int hgx_tune_product(int blocks_per_unit, int complex_mults) {
+  if (blocks_per_unit < 1 || (complex_mults != 3 && complex_mults != 4)) return HGX_ERR_INVALID_ARG;
+  g_product_bpu = blocks_per_unit;
+  g_product_gauss = complex_mults == 3;
+  return HGX_OK;
+}

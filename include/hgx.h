@@ -51,6 +51,7 @@ extern "C" {
 /* flags for hgx_ff_numeric */
 #define HGX_FLAG_EXACT_REFERENCE_SIGN 1 /* exact method with the reference's exp(-iq.r) sign */
 #define HGX_FLAG_ACC_KAHAN 2            /* FP32C: per-pair Kahan instead of FP32-block/FP64-carry */
+#define HGX_FLAG_NO_MFMA 4              /* FP32C approx: per-pair VALU kernels even on a product grid */
 
 /* flags for hgx_sf_lattice */
 #define HGX_SF_COMPAT_LB2 1 /* reproduce the reference's lb[2]*lb[2] term (src/sf/sf.cpp:212) */
@@ -139,6 +140,11 @@ int hgx_qgrid_set(hgx_ctx* ctx, int ncol, int nrow, int nbranch, const double* q
  * [kzf-kzi, -kzf-kzi, kzf+kzi, -kzf+kzi] for a particle in a layer with 1-n^2 = dnl. */
 int hgx_qz_extended(hgx_ctx* ctx, double k0, double alpha_i, double dnl_re, double dnl_im);
 int hgx_qgrid_info(const hgx_ctx* ctx, int64_t* npix, int64_t* nq, int* nrow, int* ncol, int* nbranch);
+/* *is_product = 1 when every q-point is exactly (qx[0], qy of its column, qz of its branch and row),
+ * as for hgx_qgrid_explicit and its DWBA branches (checked on the device whenever the grid changes).
+ * FP32C approximate-method form factors on such grids run as complex matrix products on the
+ * matrix cores. No reference counterpart. */
+int hgx_qgrid_is_product(const hgx_ctx* ctx, int* is_product);
 /* Copies the grid back to host arrays (any pointer may be NULL). */
 int hgx_qgrid_get(hgx_ctx* ctx, double* qx, double* qy, double* qz_ext, double* alpha_f);
 
@@ -289,6 +295,11 @@ int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, doub
  * Process-wide; set by the autotuner. No reference counterpart (the reference hard-codes its
  * tiles, include/common/parameters.hpp:37-48). */
 int hgx_tune_numeric(int qpt64, int qpt32, int target_wg);
+
+/* Work-unit shape of the product-grid kernel (single calls and later batches): triangle blocks (128
+ * triangles each) per unit, and 3 (Gauss) or 4 real products per complex product. Process-wide; set
+ * by the autotuner. No reference counterpart. */
+int hgx_tune_product(int blocks_per_unit, int complex_mults);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,169 @@
+"""Product-grid FP32C numeric form factor on the matrix cores (hgx_qgrid_is_product, product_mfma.hip).
+
+The kernel evaluates the approximate-triangle sum as complex matrix products; it must agree with the
+oracle to the FP32C bar on grids whose sides are not multiples of the 32 x 32 tile, on meshes smaller
+than one 128-triangle block, under rotation, on complex (DWBA, absorbing layer) branches, with either
+complex-product form, chunked or not, and with the per-pair VALU kernels (HGX_FLAG_NO_MFMA)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+K0 = 2 * np.pi / (1.23984 / 10.0)
+AI = np.radians(0.2)
+
+
+def _rot(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1.0]])
+    ry = np.array([[cb, 0, sb], [0, 1.0, 0], [-sb, 0, cb]])
+    rx = np.array([[1.0, 0, 0], [0, cc, -sc], [0, sc, cc]])
+    return rz @ ry @ rx
+
+
+def _grid(ctx, O, ncol, nrow, dwba=None):
+    ctx.qgrid_explicit(ncol, nrow, -2.0, 0.0, 2.0, 2.0, AI, K0)
+    g = O.qgrid_explicit(ncol, nrow, -2.0, 0.0, 2.0, 2.0, AI, K0)
+    if dwba is not None:
+        ctx.qz_extended(K0, AI, dwba)
+        g = O.qz_extended(g, K0, AI, dwba)
+    return g
+
+
+def test_product_grid_detection(oracle_mod):
+    from hipgisaxs_amd.api import Context
+
+    O = oracle_mod
+    c = Context(0)
+    try:
+        c.qgrid_explicit(40, 23, -2.0, 0.0, 2.0, 2.0, AI, K0)
+        assert c.is_product()
+        c.qz_extended(K0, AI, 2e-6 + 1e-8j)
+        assert c.is_product()
+        c.qgrid_detector(64, 48, 0.172, 2000.0, 31.5, 0.0, AI, K0)
+        assert not c.is_product()
+        g = O.qgrid_explicit(16, 8, -2.0, 0.0, 2.0, 2.0, AI, K0)
+        qy = g.qy.copy()
+        c.qgrid_set(16, 8, g.qx, qy, g.qz, g.alpha_f)
+        assert c.is_product()
+        qy[37] += 1e-9  # one pixel off the product
+        c.qgrid_set(16, 8, g.qx, qy, g.qz, g.alpha_f)
+        assert not c.is_product()
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("ncol,nrow,ntri,dwba,rotated", [
+    (40, 23, 900, None, False),
+    (64, 64, 3000, None, True),
+    (33, 31, 60, None, True),           # fewer triangles than one block
+    (48, 20, 1500, 2e-6 + 1e-8j, True),  # complex branches (absorbing layer), nz = 4 * 20
+    (37, 45, 700, 0j, False),            # real branches
+])
+@pytest.mark.parametrize("mults", [3, 4])
+def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, mults):
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Context, tune_product
+
+    O = oracle_mod
+    rng = np.random.default_rng(ncol * 1000 + nrow)
+    tri = meshes.convex_polyhedron(rng, rng.uniform(4, 12, 3), ntri)
+    R = _rot(0.3, -0.7, 1.1) if rotated else None
+    c = Context(0)
+    try:
+        tune_product(16, mults)
+        g = _grid(c, O, ncol, nrow, dwba)
+        assert c.is_product()
+        mid = c.upload_mesh(tri, "exact9")
+        ref = O.ff_approx(O.tri9_to_rec8(tri), g, R)
+        got = c.as_complex(c.ff_numeric(mid, "approx", "fp32c", rot=R)).cpu().numpy()
+        valu = c.as_complex(c.ff_numeric(mid, "approx", "fp32c", rot=R, flags=4)).cpu().numpy()
+        # The oracle's direct FP64 sum loses about eps / (|q| R) relative accuracy to cancellation:
+        # at the rounding-level |q| ~ 1e-16 that odd-sized grids produce, it is not a reference.
+        # There F(q) -> V (the mesh volume), which the FP32C small-|q| series gives.
+        info = c.mesh_info(mid)
+        qn = np.sqrt((np.abs(g.q_rotated(R)) ** 2).sum(0))
+        tiny = qn * info["radius"] < 1e-6
+        if tiny.any():
+            for f in (got, valu):
+                assert np.max(np.abs(f[tiny] - info["volume"])) <= 1e-6 * info["volume"]
+        ok = ~tiny
+        got, valu, ref = got[ok], valu[ok], ref[ok]
+        e_mfma, e_valu = rel_l2(got, ref), rel_l2(valu, ref)
+        print(f"{ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} mults={mults}: "
+              f"mfma {e_mfma:.2e} valu {e_valu:.2e}")
+        assert np.all(np.isfinite(got))
+        assert e_mfma <= 1e-5
+        assert e_valu <= 1e-5
+        c.release_mesh(mid)
+    finally:
+        tune_product(16, 3)
+        c.close()
+
+
+def test_product_chunked_matches_single_chunk(oracle_mod):
+    """Many triangle chunks (partials reduced by the finishing kernel) vs one chunk."""
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Context, tune_numeric
+
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    tri = meshes.convex_polyhedron(rng, np.array([9.0, 7.0, 5.0]), 6000)
+    c = Context(0)
+    try:
+        g = _grid(c, O, 96, 64)
+        mid = c.upload_mesh(tri, "exact9")
+        tune_numeric(2, 4, 1)  # few waves wanted: one chunk
+        one = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+        tune_numeric(2, 4, 1 << 16)  # many waves wanted: one block per chunk
+        many = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+        ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+        assert rel_l2(one, ref) <= 1e-5
+        assert rel_l2(many, ref) <= 1e-5
+        assert rel_l2(one, many) <= 1e-6
+    finally:
+        tune_numeric(2, 4, 2048)
+        c.close()
+
+
+def test_product_batch_mixed_with_valu_items(oracle_mod):
+    """A batch holding product-grid items (matrix cores) and detector-grid items (per-pair kernels)."""
+    import torch
+
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Batch, Context
+
+    O = oracle_mod
+    rng = np.random.default_rng(21)
+    main = Context(0)
+    gp = Context(0)
+    gd = Context(0)
+    try:
+        gp.qgrid_explicit(50, 34, -2.0, 0.0, 2.0, 2.0, AI, K0)
+        gd.qgrid_detector(40, 30, 0.172, 2000.0, 19.5, 0.0, AI, K0)
+        ref_p = O.qgrid_explicit(50, 34, -2.0, 0.0, 2.0, 2.0, AI, K0)
+        ref_d = O.qgrid_detector(40, 30, 0.172, 2000.0, 19.5, 0.0, AI, K0)
+        shapes = []
+        for k in range(4):
+            tri = meshes.convex_polyhedron(rng, rng.uniform(3, 10, 3), int(rng.integers(200, 2500)))
+            R = _rot(0.2 * k, 0.1 * k, -0.3 * k)
+            grid, ref = (gp, ref_p) if k % 2 == 0 else (gd, ref_d)
+            shapes.append((tri, R, grid, ref))
+        mids = [main.upload_mesh(t, "exact9") for t, _, _, _ in shapes]
+        outs = [torch.full((2 * gr.info()["nq"],), float("nan"), dtype=torch.float64, device="cuda")
+                for _, _, gr, _ in shapes]
+        b = Batch(main, [(gr, m, R, o) for (t, R, gr, _), m, o in zip(shapes, mids, outs)], precision="fp32c")
+        for use_graph in (False, True, True):
+            for o in outs:
+                o.fill_(float("nan"))
+            b.run(use_graph=use_graph)
+            torch.cuda.synchronize()
+            for (t, R, gr, ref), o in zip(shapes, outs):
+                want = O.ff_approx(O.tri9_to_rec8(t), ref, R)
+                assert rel_l2(main.as_complex(o).cpu().numpy(), want) <= 1e-5, use_graph
+        b.release()
+    finally:
+        for c in (main, gp, gd):
+            c.close()
