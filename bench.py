@@ -27,9 +27,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "q-pt·triangles/s + χ²-grad evals/s, FP32-comp vs plain FP64 GPU @ relL2≤1e-5"
 SEED = 20261203
-PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector (packed) peak
+PEAK_FP32 = 157.3e12  # MI355X dense FP32 peak (matrix cores; also packed vector FMA)
 PEAK_FP64 = 78.6e12
-FLOP_PER_PAIR = 33  # approximate-triangle pair: q.n 10 + q.c 10 + phase 2 + scale 3 + cmul 6 + acc 2
+FLOP_PER_PAIR = 33  # approximate-triangle pair, direct form: q.n 10 + q.c 10 + phase 2 + scale 3 + cmul 6 + acc 2
+# product-grid form: two complex multiply-adds per (q, triangle) pair, 8 flop each (6 + 2)
+FLOP_PER_PAIR_PRODUCT = 16
 
 
 def make_workload(nshapes=64, seed=SEED + 3):
@@ -224,13 +226,23 @@ def main():
 
     items = {prec: [(grid_ctx[s["side"]], s["mid"], None, s["out32" if prec == "fp32c" else "out64"])
                     for s in shapes] for prec in ("fp32c", "fp64")}
+    # every grid here is a product grid (qx = 0, qy per column, qz per row): FP32C runs on the
+    # matrix cores unless --no-mfma
+    product = all(c.is_product() for c in grid_ctx.values()) and not args.no_mfma
     tuned = {}
+    mults = 3
     if args.mode == "batch" and not args.no_tune:
-        from hipgisaxs_amd.autotune import autotune_batch
+        from hipgisaxs_amd.autotune import autotune_batch, autotune_product
 
-        best, times = autotune_batch(main_ctx, items["fp32c"], "fp32c", flags)
-        tuned = {"qpt": best[0], "blocks_per_unit": best[1],
-                 "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+        if product:
+            best, times = autotune_product(main_ctx, items["fp32c"], flags)
+            mults = best[1]
+            tuned = {"blocks_per_unit": best[0], "complex_mults": best[1],
+                     "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+        else:
+            best, times = autotune_batch(main_ctx, items["fp32c"], "fp32c", flags)
+            tuned = {"qpt": best[0], "blocks_per_unit": best[1],
+                     "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
 
     def make_step(prec):
@@ -284,8 +296,9 @@ def main():
     value32 = pairs * args.steps / wall32
     value64 = pairs * args.fp64_steps / wall64
     avg_launch_ms = k32_ms / max(n32, 1)
-    flops_per_launch = FLOP_PER_PAIR * pairs * args.steps / max(n32, 1)
-    achieved = flops_per_launch / (avg_launch_ms * 1e-3)
+    pairs_per_launch = pairs * args.steps / max(n32, 1)
+    fpp = FLOP_PER_PAIR_PRODUCT if product else FLOP_PER_PAIR
+    achieved = fpp * pairs_per_launch / (avg_launch_ms * 1e-3)
     result = {
         "metric": METRIC,
         "value": value32,
@@ -316,21 +329,33 @@ def main():
         "relL2_fp32c_vs_fp64_gpu": rel_fp64,
         "relL2_fp32c_vs_fp64_gpu_worst_shape": worst,
         "roofline": {
-            "bound": "valu",
-            "kernel": "k_batch32" if args.mode == "batch" else "k_ff_approx32",
+            "bound": "mfma" if product else "valu",
+            "kernel": "k_ff_product32" if product else ("k_batch32" if args.mode == "batch" else "k_ff_approx32"),
             "achieved": achieved / 1e12,
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": None,
-            "flop_per_pair": FLOP_PER_PAIR,
+            "flop_per_pair": fpp,
             "avg_launch_ms": avg_launch_ms,
             "launches": n32,
             "aux_ms_per_step": aux32 / args.steps,
         },
     }
+    if product:
+        # what the matrix pipe actually executes: 2 x (3 or 4) real products of 2 flop per pair
+        executed = 2 * mults * 2 * pairs_per_launch / (avg_launch_ms * 1e-3)
+        result["roofline"]["mfma_executed_tflops"] = executed / 1e12
+        result["roofline"]["mfma_pipe_frac"] = executed / PEAK_FP32
+        result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * pairs_per_launch / (avg_launch_ms * 1e-3) / 1e12
     if b32 is not None:
         result["config"]["work_units"] = b32.info()["units"]
+    if product and not args.no_extra:
+        # the same batch on the per-pair VALU kernels (no matrix cores), for comparison
+        bv = Batch(main_ctx, items["fp32c"], precision="fp32c", flags=flags | 4)
+        wall_v, kv_ms, nv, _ = timed(lambda g: bv.run(use_graph=g), 2, 1, True)
+        result["fp32c_per_pair_valu"] = {"value": pairs * 2 / wall_v, "ms_per_step": 1e3 * wall_v / 2}
+        bv.release()
     if not args.no_extra:
         result["lattice_spheres_bcc"] = bench_lattice()
         print(f"[bench] lattice done", file=sys.stderr, flush=True)

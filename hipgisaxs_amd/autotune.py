@@ -1,33 +1,61 @@
-"""Autotuner for the launch shapes of the numeric form-factor kernels.
+"""Autotuner for the launch shapes of the batched numeric form-factor kernels.
 
-Times each candidate (q-points per thread, triangle blocks per work unit) on the caller's own batch
-with HIP events (the library's launch timing) and keeps the fastest. The reference's counterpart is
-the hyperblock-size search of its legacy CPU path (src/ff/cpu/ff_num_cpu.cpp:116-176).
+Times each candidate on the caller's own batch with HIP events (the library's launch timing) and
+keeps the fastest; a candidate replaces the default only when it is faster by more than `margin`,
+so repeated runs settle on the same shape. Two kernel families are tuned:
+
+* the per-pair VALU kernels: (q-points per thread, triangle blocks per work unit);
+* the product-grid matrix-core kernel: (triangle blocks per work unit, real products per complex
+  product: 3 or 4).
+
+The reference's counterpart is the hyperblock-size search of its legacy CPU path
+(src/ff/cpu/ff_num_cpu.cpp:116-176).
 """
 from __future__ import annotations
 
-from .api import Batch, Context, tune_batch
+from .api import Batch, Context, tune_batch, tune_product
 
-DEFAULT_CANDIDATES = ((4, 8), (4, 16), (4, 32), (8, 16), (2, 16), (8, 32))
+VALU_DEFAULT = (4, 16)
+VALU_CANDIDATES = (VALU_DEFAULT, (4, 8), (4, 32), (8, 16), (2, 16), (8, 32))
+PRODUCT_DEFAULT = (16, 3)
+PRODUCT_CANDIDATES = (PRODUCT_DEFAULT, (8, 3), (32, 3), (64, 3), (16, 4))
+
+
+def _time_batch(ctx: Context, items, precision, flags, reps):
+    import torch
+
+    b = Batch(ctx, items, precision=precision, flags=flags)
+    b.run()
+    torch.cuda.synchronize()
+    ctx.profile_begin()
+    for _ in range(reps):
+        b.run()
+    prof = ctx.profile_end()
+    b.release()
+    return (prof["main_ms"] + prof["aux_ms"]) / reps
+
+
+def _search(setter, default, candidates, run, margin):
+    times = {}
+    for cand in candidates:
+        setter(*cand)
+        times[cand] = run()
+    best = min(times, key=times.get)
+    if default in times and times[best] > (1.0 - margin) * times[default]:
+        best = default
+    setter(*best)
+    return best, times
 
 
 def autotune_batch(ctx: Context, items, precision: str = "fp32c", flags: int = 0,
-                   candidates=DEFAULT_CANDIDATES, reps: int = 1):
-    """Returns ((qpt, blocks_per_unit), {candidate: ms}) and leaves the best setting active."""
-    import torch
+                   candidates=VALU_CANDIDATES, reps: int = 1, margin: float = 0.02):
+    """Per-pair kernels: returns ((qpt, blocks_per_unit), {candidate: ms}), leaves the best active."""
+    return _search(tune_batch, VALU_DEFAULT, candidates,
+                   lambda: _time_batch(ctx, items, precision, flags, reps), margin)
 
-    times = {}
-    for qpt, bpu in candidates:
-        tune_batch(qpt, bpu)
-        b = Batch(ctx, items, precision=precision, flags=flags)
-        b.run()
-        torch.cuda.synchronize()
-        ctx.profile_begin()
-        for _ in range(reps):
-            b.run()
-        prof = ctx.profile_end()
-        times[(qpt, bpu)] = (prof["main_ms"] + prof["aux_ms"]) / reps
-        b.release()
-    best = min(times, key=times.get)
-    tune_batch(*best)
-    return best, times
+
+def autotune_product(ctx: Context, items, flags: int = 0, candidates=PRODUCT_CANDIDATES, reps: int = 1,
+                     margin: float = 0.02):
+    """Product-grid kernel (FP32C): returns ((blocks_per_unit, complex_mults), {candidate: ms})."""
+    return _search(tune_product, PRODUCT_DEFAULT, candidates,
+                   lambda: _time_batch(ctx, items, "fp32c", flags, reps), margin)

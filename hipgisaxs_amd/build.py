@@ -59,6 +59,19 @@ def build_hip(jobs: int = 8) -> Path:
     return LIB
 
 
+PEAKS_SRC = PKG / "peaks.hip"
+PEAKS_BIN = PKG / "hgx_peaks"
+
+
+def build_peaks() -> Path:
+    """The throughput microbenchmark (FP32/FP64 FMA, v_sin, v_exp, FP32 MFMA): a standalone executable."""
+    tmp = PEAKS_BIN.with_suffix(".tmp")
+    _run([HIPCC, "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", str(PEAKS_SRC), "-o", str(tmp)])
+    os.replace(tmp, PEAKS_BIN)
+    return PEAKS_BIN
+
+
 HIG_SRC = PKG / "host" / "hig_gpu.cpp"
 HIG_LIB = PKG / "libhgx_hig.so"
 SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
@@ -91,6 +104,7 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_hip()
     build_host_adapters()
+    build_peaks()
     build_oracle()
 
 
@@ -99,5 +113,6 @@ if __name__ == "__main__":
     if "all" in what or "hip" in what:
         print("built", build_hip())
         print("built", build_host_adapters())
+        print("built", build_peaks())
     if "all" in what or "oracle" in what:
         print("built", build_oracle())

@@ -293,7 +293,8 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     // enough waves to fill the chip several times over, without splitting below one block
     int64_t want = std::max<int64_t>(1, (int64_t)(4 * g_target_wg + tiles - 1) / tiles);
     want = std::min<int64_t>(want, m.nblocks);
-    it.bchunk = (m.nblocks + want - 1) / want;
+    // chunks accumulate in FP32: at most product_blocks_per_unit() blocks each
+    it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit());
     it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
     if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
     size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;

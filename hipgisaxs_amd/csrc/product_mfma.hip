@@ -15,11 +15,13 @@
 // products (P1 = sum Xr Yr, P2 = sum Xi Yi, P3 = sum (Xr + Xi)(Yr + Yi); Re = P1 - P2,
 // Im = P3 - P1 - P2) or four.
 //
-// Precision follows the FP32C path of the per-pair kernels: within a 128-triangle block the
-// coordinates are FP32 relative to the block's FP64 origin o_b and phases are in turns for the
-// hardware v_sin/v_cos; the FP32 block sum is rotated by exp(i q . o_b) (FP64 phase, reduced before
-// the FP32 sincos) and added to FP64 accumulators. Output is the raw sum; the finishing kernels
-// apply -i/|q|^2 and the small-|q| series exactly as for the per-pair path.
+// Precision: within a 128-triangle block the coordinates are FP32 relative to the block's FP64
+// origin o_b. A phase is q.o_b / 2pi (FP64, reduced to [-1/2, 1/2] turns once per block and lane)
+// plus q.(c - o_b) / 2pi (FP32, small), fed in turns to the hardware v_sin/v_cos, so each table
+// entry is accurate to FP32 rounding whatever |q| |c|. The sums accumulate in FP32 in the
+// matrix-core accumulators over one chunk of triangle blocks (16 blocks by default) and chunks are
+// added in FP64 by the finishing kernel, which also applies -i/|q|^2 and the small-|q| series
+// exactly as for the per-pair path.
 #include "numeric_core.hpp"
 
 namespace hgx {
@@ -65,7 +67,7 @@ __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
 __device__ inline int acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (rr & 3); }
 
 template <bool CQ, bool GAUSS>
-__global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_ff_product32(const ProductItem* __restrict__ items,
                                                       const ProductUnit* __restrict__ units, int nunits,
                                                       ProductItem single, int nzt, int nyt) {
   // the wave index is uniform across the wave; saying so keeps the unit and item loads scalar
@@ -98,10 +100,6 @@ __global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restr
   const float qx0f = (float)qx0, qyf = (float)qyv, qzrf = (float)qzr, qzif = (float)qzi;
   const float azf = (float)(qzi * 1.4426950408889634);  // log2(e) Im(qz): exp(-Im(qz) c) = exp2(-azf c)
 
-  double accr[16], acci[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) accr[k] = acci[k] = 0.0;
-
   const int64_t b0 = (int64_t)u.chunk * it.bchunk;
   const int64_t b1 = min(it.nblocks, b0 + it.bchunk);
   // A block's records (128 x 32 B) are staged per wave in LDS; the next block's are loaded into
@@ -118,7 +116,16 @@ __global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restr
     pf2 = gsrc[128];
     pf3 = gsrc[192];
   }
+  // Each block's sums accumulate in the matrix-core accumulators (FP32) and are then added to the
+  // chunk's FP32 sums, so no FP32 sum absorbs more than 128 triangles or 16 blocks directly;
+  // chunks are summed in FP64 by the finishing kernel.
+  f32x16 C1, C2, C3;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) C1[k] = C2[k] = C3[k] = 0.0f;
   for (int64_t b = b0; b < b1; ++b) {
+    f32x16 P1, P2, P3;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = 0.0f;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     srec[lane] = pf0;
@@ -134,26 +141,30 @@ __global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restr
       pf2 = gsrc[128];
       pf3 = gsrc[192];
     }
-    f32x16 P1, P2, P3;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = 0.0f;
+    // phase of the block origin for this lane's column and row, in turns, reduced in FP64:
+    // q . c = q . o_b + q . (c - o_b), the first reduced exactly, the second small
+    const double ox = it.borig[4 * b], oy = it.borig[4 * b + 1], oz = it.borig[4 * b + 2];
+    const double ty = (qx0 * ox + qyv * oy) * kInvTwoPi;
+    const double tz = qzr * oz * kInvTwoPi;
+    const float by = (float)(ty - rint(ty)), bz = (float)(tz - rint(tz));
+    const float bza = CQ ? (float)(-qzi * oz * 1.4426950408889634) : 0.0f;  // log2 of exp(-Im(qz) oz)
 #pragma unroll 2
     for (int p = 0; p < kBlockTri / 2; ++p) {
       // this lane's triangle 2p + h: record float4s 2 (2p + h) and 2 (2p + h) + 1
       const float4 ra = srec[4 * p + 2 * h];
       const float4 rc = srec[4 * p + 2 * h + 1];
       const float snx = ra.x, sny = ra.y, snz = ra.z, dcx = ra.w, dcy = rc.x, dcz = rc.y;
-      // y side: Ey = exp(i (qx0 dcx + qy dcy)), Y1 = alpha Ey, Y2 = Ey
-      const float phy = __builtin_fmaf(uy, dcy, ux0 * dcx);
+      // y side: Ey = exp(i (qx0 cx + qy cy)), Y1 = alpha Ey, Y2 = Ey
+      const float phy = __builtin_fmaf(uy, dcy, __builtin_fmaf(ux0, dcx, by));
       const float eyr = __builtin_amdgcn_cosf(phy), eyi = __builtin_amdgcn_sinf(phy);
       const float al = __builtin_fmaf(qyf, sny, qx0f * snx);
       const float y1r = al * eyr, y1i = al * eyi;
-      // z side: Ez = exp(i qz dcz), Z1 = Ez, Z2 = beta Ez with beta = qz snz
-      const float phz = uz * dcz;
+      // z side: Ez = exp(i qz cz), Z1 = Ez, Z2 = beta Ez with beta = qz snz
+      const float phz = __builtin_fmaf(uz, dcz, bz);
       float ezr = __builtin_amdgcn_cosf(phz), ezi = __builtin_amdgcn_sinf(phz);
       float z2r, z2i;
       if (CQ) {
-        const float att = __builtin_amdgcn_exp2f(-azf * dcz);
+        const float att = __builtin_amdgcn_exp2f(__builtin_fmaf(-azf, dcz, bza));
         ezr *= att;
         ezi *= att;
         const float br = qzrf * snz, bi = qzif * snz;
@@ -183,36 +194,11 @@ __global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restr
         P2 = mfma32(z2i, eyr, P2);
       }
     }
-    // rotate the block sum by exp(i q . o_b) = exp(i (qx0 ox + qy oy)) exp(i qz oz), FP64 phases
-    // reduced before the FP32 sincos. Each lane evaluates the z factor of its operand row; lane i
-    // (i < 32) holds row z0 + i, which the accumulator layout reads back with cross-lane shuffles.
-    const double ox = it.borig[4 * b], oy = it.borig[4 * b + 1], oz = it.borig[4 * b + 2];
-    const double ty = (qx0 * ox + qyv * oy) * kInvTwoPi;
-    const float fty = (float)(ty - rint(ty));
-    const double fyr = (double)__builtin_amdgcn_cosf(fty), fyi = (double)__builtin_amdgcn_sinf(fty);
-    const double tz = qzr * oz * kInvTwoPi;
-    const float ftz = (float)(tz - rint(tz));
-    double fzr = (double)__builtin_amdgcn_cosf(ftz), fzi = (double)__builtin_amdgcn_sinf(ftz);
-    if (CQ) {
-      const double m = (double)__builtin_amdgcn_exp2f((float)(-qzi * oz * 1.4426950408889634));
-      fzr *= m;
-      fzi *= m;
-    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const int src = acc_row(k, h);
-      const double zr_k = __shfl(fzr, src), zi_k = __shfl(fzi, src);
-      const double fr = fyr * zr_k - fyi * zi_k, fi = fyr * zi_k + fyi * zr_k;
-      double sr, si;
-      if (GAUSS) {
-        sr = (double)P1[k] - (double)P2[k];
-        si = (double)P3[k] - (double)P1[k] - (double)P2[k];
-      } else {
-        sr = (double)P1[k];
-        si = (double)P2[k];
-      }
-      accr[k] += sr * fr - si * fi;
-      acci[k] += sr * fi + si * fr;
+      C1[k] += P1[k];
+      C2[k] += P2[k];
+      if (GAUSS) C3[k] += P3[k];
     }
   }
   // raw sums: in place when the item has one chunk, else into the chunk's partial slot
@@ -224,8 +210,13 @@ __global__ __launch_bounds__(256) void k_ff_product32(const ProductItem* __restr
     if (zi >= nz) continue;
     const int64_t q = (int64_t)zi * ncol + yj;
     double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)u.chunk * it.nq + q);
-    dst[0] = accr[k];
-    dst[1] = acci[k];
+    if (GAUSS) {
+      dst[0] = (double)C1[k] - (double)C2[k];
+      dst[1] = (double)C3[k] - (double)C1[k] - (double)C2[k];
+    } else {
+      dst[0] = (double)C1[k];
+      dst[1] = (double)C2[k];
+    }
   }
 }
 

