@@ -30,8 +30,16 @@ SEED = 20261203
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 peak (matrix cores; also packed vector FMA)
 PEAK_FP64 = 78.6e12
 FLOP_PER_PAIR = 33  # approximate-triangle pair, direct form: q.n 10 + q.c 10 + phase 2 + scale 3 + cmul 6 + acc 2
-# product-grid form: two complex multiply-adds per (q, triangle) pair, 8 flop each (6 + 2)
-FLOP_PER_PAIR_PRODUCT = 16
+# Product-grid form: two complex multiply-adds per (q, triangle) pair, each done as 3 (Gauss) or 4
+# real multiply-adds of 2 flop on the matrix cores: 12 or 16 flop per pair.
+def flop_per_pair_product(mults):
+    return 2 * mults * 2
+
+
+# HBM bytes per launch of k_ff_product32 on this workload at the default shape (16 blocks per unit,
+# Gauss form), from rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
+# device copy) in KiB per dispatch: (94933.4 x 2 + 2218240.0) x 1024.
+PRODUCT_TRAFFIC_BYTES = {(16, 3): (94933.4 * 2 + 2218240.0) * 1024}
 
 
 def make_workload(nshapes=64, seed=SEED + 3):
@@ -230,13 +238,13 @@ def main():
     # matrix cores unless --no-mfma
     product = all(c.is_product() for c in grid_ctx.values()) and not args.no_mfma
     tuned = {}
-    mults = 3
+    mults, bpu_p = 3, 16
     if args.mode == "batch" and not args.no_tune:
         from hipgisaxs_amd.autotune import autotune_batch, autotune_product
 
         if product:
             best, times = autotune_product(main_ctx, items["fp32c"], flags)
-            mults = best[1]
+            bpu_p, mults = best
             tuned = {"blocks_per_unit": best[0], "complex_mults": best[1],
                      "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
         else:
@@ -297,7 +305,7 @@ def main():
     value64 = pairs * args.fp64_steps / wall64
     avg_launch_ms = k32_ms / max(n32, 1)
     pairs_per_launch = pairs * args.steps / max(n32, 1)
-    fpp = FLOP_PER_PAIR_PRODUCT if product else FLOP_PER_PAIR
+    fpp = flop_per_pair_product(mults) if product else FLOP_PER_PAIR
     achieved = fpp * pairs_per_launch / (avg_launch_ms * 1e-3)
     result = {
         "metric": METRIC,
@@ -343,11 +351,14 @@ def main():
         },
     }
     if product:
-        # what the matrix pipe actually executes: 2 x (3 or 4) real products of 2 flop per pair
-        executed = 2 * mults * 2 * pairs_per_launch / (avg_launch_ms * 1e-3)
-        result["roofline"]["mfma_executed_tflops"] = executed / 1e12
-        result["roofline"]["mfma_pipe_frac"] = executed / PEAK_FP32
-        result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * pairs_per_launch / (avg_launch_ms * 1e-3) / 1e12
+        rate = pairs_per_launch / (avg_launch_ms * 1e-3)
+        # the same pairs counted as complex arithmetic (8 flop per complex multiply-add) and as the
+        # direct per-pair form (33 flop): rates of equivalent work, not of matrix-core issue
+        result["roofline"]["complex_form_tflops"] = 16 * rate / 1e12
+        result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * rate / 1e12
+        shape_key = (bpu_p, mults)
+        if len(shapes) == 64 and not args.kahan and shape_key in PRODUCT_TRAFFIC_BYTES:
+            result["roofline"]["traffic"] = PRODUCT_TRAFFIC_BYTES[shape_key]
     if b32 is not None:
         result["config"]["work_units"] = b32.info()["units"]
     if product and not args.no_extra:
