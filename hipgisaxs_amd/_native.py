@@ -116,6 +116,7 @@ _SIGS = {
     "hgx_mesh_release": (C.c_int, [_vp, C.c_int]),
     "hgx_mesh_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hgx_ff_numeric": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "hgx_ff_numeric_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hgx_ff_analytic": (C.c_int, [_vp, C.c_int, C.POINTER(ShapeParams), C.c_int, _vp, _vp, _vp]),
     "hgx_sf_lattice": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp]),
     "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),

@@ -151,6 +151,9 @@ class Context:
         r = _f64(records).reshape(-1, width)
         mid = C.c_int()
         self._check(self.lib.hgx_mesh_upload(self.h, k, r.shape[0], _ptr(r), C.byref(mid)), "mesh_upload")
+        if not hasattr(self, "_uploaded"):
+            self._uploaded = {}
+        self._uploaded[mid.value] = r.shape[0]
         return mid.value
 
     def release_mesh(self, mesh_id: int) -> None:
@@ -172,6 +175,22 @@ class Context:
         self._check(self.lib.hgx_ff_numeric(self.h, mesh_id, m, p, flags, _ptr(R), _ptr(tv), C.c_void_p(o.data_ptr())),
                     "ff_numeric")
         return o
+
+    def ff_numeric_adjoint(self, mesh_id, W, precision="fp64", rot=None, out=None):
+        """Record adjoint of L = Re sum_q W(q) F(q) (F: approximate method): [ntri_uploaded, 6] device
+        tensor of (dL/d(s n)_t, dL/dc_t). W: complex device tensor viewed as [2*nq] float64."""
+        import torch
+
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        nq = self.info()["nq"]
+        if W.dtype != torch.float64 or W.numel() != 2 * nq or not W.is_cuda:
+            raise HgxError("ff_numeric_adjoint: W must be a float64 device tensor of 2*nq values")
+        n = self._uploaded[mesh_id]
+        o = self._out(6 * n, out)
+        R = None if rot is None else _f64(rot).ravel()
+        self._check(self.lib.hgx_ff_numeric_adjoint(self.h, mesh_id, p, 0, _ptr(R), C.c_void_p(W.data_ptr()),
+                                                    C.c_void_p(o.data_ptr())), "ff_numeric_adjoint")
+        return o.view(n, 6)
 
     def ff_analytic(self, shape: str, dims: Sequence, precision="fp64", rot=None, transvec=None, tau=0.0, eta=0.0,
                     flags=0, out=None):

@@ -1,0 +1,408 @@
+// adjoint.hip -- record adjoint of the approximate-triangle form factor.
+//
+// For L = Re sum_q W(q) F(q), with F(q) = -i / |q|^2 sum_t (q . A_t) exp(i q . c_t) (A_t = s_t n_t,
+// src/ff/cpu/ff_tri_cpu.cpp:200-246) and W given per q-point, the gradient with respect to every
+// triangle record is
+//     dL/dA_t = Re sum_q Wt(q) q exp(i q.c_t),              Wt = -i W / |q|^2
+//     dL/dc_t = Re i sum_q Wt(q) (q . A_t) q exp(i q.c_t) = Re i (M_t A_t),
+//     M_t = sum_q Wt(q) q q^T exp(i q.c_t)
+// plus, at q = 0 where F = V = sum_t A_t . c_t / 3, Re W(0) (c_t, A_t) / 3. Any mesh
+// parameterisation's gradient follows by the chain rule through (A_t, c_t). The reference has no
+// such path; it differentiates by finite differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224).
+//
+// Rotation: as in the forward kernels, records are rotated by R^T and the grid is left alone; the
+// gradient is rotated back by R at the end.
+//
+// Plain FP64 (any grid): one thread per triangle, q-points in chunks (wave-uniform q data through
+// scalar loads), FP64 partials per chunk reduced in order.
+//
+// FP32C on product grids q = (qx0, qy_y, qz_z): the sums are bilinear forms
+//     B[f, g](t) = sum_{z, y} f(z) g(y) Wt(z, y) Ez_t(z) Ey_t(y)
+// with f in {1, qz, qz^2} and g in {1, qy, qy^2}; V_t and M_t are combinations of the six forms
+// B11, B1y, B1yy, Bz1, Bzy, Bzz. For each f the kernel forms X = (f Ez)^T Wt -- a complex matrix
+// product (triangles x z) . (z x y) on v_mfma_f32_32x32x2_f32, Gauss's three real products -- and then
+// contracts each row of X with g(y) Ey_t(y) across the wave's 32 columns. Phases come from FP64
+// absolute centroids reduced to turns per table entry, so no block origins are needed.
+#include "numeric_core.hpp"
+
+namespace hgx {
+
+typedef float f32x16a __attribute__((ext_vector_type(16)));
+
+constexpr int kAdjTile = 32;
+constexpr int kAdjForms = 6;  // B11, B1y, B1yy, Bz1, Bzy, Bzz
+
+// Rotated FP64 records (A' = R^T A, c' = R^T c) from the mesh's [ntri][8] FP64 records.
+__global__ __launch_bounds__(256) void k_adj_rotate(const double* __restrict__ rec, int64_t ntri, Rot R,
+                                                    double* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntri) return;
+  const double* a = rec + 8 * t;
+  double* o = out + 8 * t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o[k] = R.m[k] * a[0] + R.m[3 + k] * a[1] + R.m[6 + k] * a[2];
+    o[3 + k] = R.m[k] * a[3] + R.m[3 + k] * a[4] + R.m[6 + k] * a[5];
+  }
+  o[6] = o[7] = 0.0;
+}
+
+// Wt = -i W / |q|^2 (0 at q = 0), FP64 or FP32.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_weights(QView g, const double* __restrict__ W, T* __restrict__ wt) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= g.nq) return;
+  int64_t p = q % g.npix;
+  double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = g.qz[2 * q + 1];
+  double q2 = qx * qx + qy * qy + zr * zr + zi * zi;
+  double s = q2 == 0.0 ? 0.0 : 1.0 / q2;
+  // -i (a + ib) = b - ia
+  wt[2 * q] = (T)(W[2 * q + 1] * s);
+  wt[2 * q + 1] = (T)(-W[2 * q] * s);
+}
+
+// w0 = sum over q-points with |q| = 0 of Re W(q): one workgroup, fixed-order tree (deterministic).
+__global__ __launch_bounds__(256) void k_adj_q0(QView g, const double* __restrict__ W, double* __restrict__ w0) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int64_t q = threadIdx.x; q < g.nq; q += 256) {
+    int64_t p = q % g.npix;
+    double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = g.qz[2 * q + 1];
+    if (qx * qx + qy * qy + zr * zr + zi * zi == 0.0) acc += W[2 * q];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *w0 = s[0];
+}
+
+// ----- plain FP64, any grid ----------------------------------------------------------------------
+
+template <bool CQ>
+__global__ __launch_bounds__(256) void k_adjoint64(QView g, const double* __restrict__ wt, const double* __restrict__ rec,
+                                                   int64_t ntri, int64_t qchunk, double* __restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tc = t < ntri ? t : ntri - 1;
+  const int chunk = blockIdx.y;
+  const int64_t q0 = (int64_t)chunk * qchunk, q1 = min(g.nq, q0 + qchunk);
+  const double* r = rec + 8 * tc;
+  const double ax = r[0], ay = r[1], az = r[2], cx = r[3], cy = r[4], cz = r[5];
+  double gA[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
+  for (int64_t q = q0; q < q1; ++q) {
+    // q-point data are the same for every lane: scalar loads
+    const int64_t p = q % g.npix;
+    const double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = CQ ? g.qz[2 * q + 1] : 0.0;
+    const cd w = cd{wt[2 * q], wt[2 * q + 1]};
+    // e = exp(i q.c) with complex qz
+    double sn, cs;
+    sincos(qx * cx + qy * cy + zr * cz, &sn, &cs);
+    double m = CQ ? exp(-zi * cz) : 1.0;
+    const cd wa = w * cd{m * cs, m * sn};
+    const cd qa = cd{qx * ax + qy * ay + zr * az, zi * az};
+    const cd wc = mul_i(wa * qa);
+    gA[0] += wa.r * qx;
+    gA[1] += wa.r * qy;
+    gA[2] += wa.r * zr - wa.i * zi;
+    gc[0] += wc.r * qx;
+    gc[1] += wc.r * qy;
+    gc[2] += wc.r * zr - wc.i * zi;
+  }
+  if (t >= ntri) return;
+  double* o = part + 6 * ((int64_t)chunk * ntri + t);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o[k] = gA[k];
+    o[3 + k] = gc[k];
+  }
+}
+
+// Rotates (dL/dA', dL/dc') back by R, adds the q = 0 term and scatters to the caller's order.
+__device__ inline void adj_store(const Rot& R, const double* __restrict__ rec, int64_t t, double w0, double gA[3],
+                                 double gc[3], const int64_t* __restrict__ src, double* __restrict__ grad) {
+  const double* r = rec + 8 * t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gA[k] += w0 * r[3 + k] / 3.0;
+    gc[k] += w0 * r[k] / 3.0;
+  }
+  double* o = grad + 6 * src[t];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    o[i] = R.m[3 * i] * gA[0] + R.m[3 * i + 1] * gA[1] + R.m[3 * i + 2] * gA[2];
+    o[3 + i] = R.m[3 * i] * gc[0] + R.m[3 * i + 1] * gc[1] + R.m[3 * i + 2] * gc[2];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adjoint64_finish(const double* __restrict__ part, int nchunks, int64_t ntri,
+                                                          const double* __restrict__ rec, Rot R,
+                                                          const double* __restrict__ w0p,
+                                                          const int64_t* __restrict__ src, double* __restrict__ grad) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntri) return;
+  double gA[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
+  for (int c = 0; c < nchunks; ++c) {
+    const double* o = part + 6 * ((int64_t)c * ntri + t);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gA[k] += o[k];
+      gc[k] += o[3 + k];
+    }
+  }
+  adj_store(R, rec, t, *w0p, gA, gc, src, grad);
+}
+
+__global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ----- FP32C on product grids (matrix cores) -------------------------------------------------------
+
+struct AdjItem {
+  const double* qx;  // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
+  const double* qy;
+  const double* qz;
+  const float* wt;   // [nq] complex Wt = -i W / |q|^2
+  const double* rec; // [ntri][8] rotated FP64 records
+  double* part;      // [kAdjForms][nyt][ntri] complex
+  int64_t ntri;
+  int ncol, nz, nyt, ntt;
+};
+
+__device__ inline f32x16a mfma32a(float a, float b, f32x16a c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+__device__ inline int adj_acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (rr & 3); }
+
+// One wave = (f-pass, 32 triangles, 32 y-columns) over all z. Lane row r of the A operand is triangle
+// t0 + r at z = 2s + h; lane column r of the B operand is Wt(z = 2s + h, y0 + r).
+template <bool CQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adjoint_product32(AdjItem it,
+                                                                                                      int nunits) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int uidx = (int)blockIdx.x * 4 + wave;
+  if (uidx >= nunits) return;  // whole waves leave; no barriers below
+  const int yt = uidx % it.nyt;
+  const int tt = (uidx / it.nyt) % it.ntt;
+  const int fp = uidx / (it.nyt * it.ntt);  // 0: f = 1, 1: f = qz, 2: f = qz^2
+  const int r = lane & 31, h = lane >> 5;
+  const int ncol = it.ncol, nz = it.nz;
+  const int64_t t0 = (int64_t)tt * kAdjTile;
+  const int64_t ta = min(t0 + r, it.ntri - 1);
+  const int yc = min(yt * kAdjTile + r, ncol - 1);
+  const double czt = it.rec[8 * ta + 5];
+  const float* wcol = it.wt + 2 * (int64_t)yc;
+
+  f32x16a P1, P2, P3, C1, C2, C3;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = C1[k] = C2[k] = C3[k] = 0.0f;
+  const int nsteps = (nz + 1) / 2;
+  for (int s = 0; s < nsteps; ++s) {
+    const int z = 2 * s + h;
+    const int zc = min(z, nz - 1);
+    const double qzr = it.qz[2 * (int64_t)zc * ncol];
+    const double qzi = CQ ? it.qz[2 * (int64_t)zc * ncol + 1] : 0.0;
+    const double ph = qzr * czt * kInvTwoPi;
+    const float fph = (float)(ph - rint(ph));
+    float er = __builtin_amdgcn_cosf(fph), ei = __builtin_amdgcn_sinf(fph);
+    if (CQ) {
+      const float att = __builtin_amdgcn_exp2f((float)(-qzi * czt * 1.4426950408889634));
+      er *= att;
+      ei *= att;
+    }
+    float ar = er, ai = ei;
+    if (fp > 0) {
+      const float fr = (float)qzr, fi = (float)qzi;
+      float xr = fr * er - fi * ei, xi = fr * ei + fi * er;
+      if (fp > 1) {
+        const float yr = fr * xr - fi * xi, yi = fr * xi + fi * xr;
+        xr = yr;
+        xi = yi;
+      }
+      ar = xr;
+      ai = xi;
+    }
+    if (z >= nz) ar = ai = 0.0f;
+    const float2 w = *reinterpret_cast<const float2*>(wcol + 2 * (int64_t)zc * ncol);
+    P1 = mfma32a(ar, w.x, P1);
+    P2 = mfma32a(ai, w.y, P2);
+    P3 = mfma32a(ar + ai, w.x + w.y, P3);
+    if ((s & 63) == 63) {  // no FP32 sum absorbs more than 128 z-values directly
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        C1[k] += P1[k];
+        C2[k] += P2[k];
+        C3[k] += P3[k];
+        P1[k] = P2[k] = P3[k] = 0.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    C1[k] += P1[k];
+    C2[k] += P2[k];
+    C3[k] += P3[k];
+  }
+  // X[t][y] for this lane's column y and 16 triangle rows; contract with g(y) Ey_t(y) over the 32
+  // columns of the wave's half
+  const int yv = yt * kAdjTile + r;
+  const bool yok = yv < ncol;
+  const double qx0 = it.qx[0], qyv = it.qy[yc];
+  const int nform = 3 - fp;
+  const int form0 = fp == 0 ? 0 : (fp == 1 ? 3 : 5);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int64_t trow = t0 + adj_acc_row(k, h);
+    const int64_t tc = min(trow, it.ntri - 1);
+    const double ph = (qx0 * it.rec[8 * tc + 3] + qyv * it.rec[8 * tc + 4]) * kInvTwoPi;
+    const float fph = (float)(ph - rint(ph));
+    const double eyr = (double)__builtin_amdgcn_cosf(fph), eyi = (double)__builtin_amdgcn_sinf(fph);
+    const double xr = (double)C1[k] - (double)C2[k];
+    const double xi = (double)C3[k] - (double)C1[k] - (double)C2[k];
+    double yr = yok ? xr * eyr - xi * eyi : 0.0;
+    double yi = yok ? xr * eyi + xi * eyr : 0.0;
+    for (int f = 0; f < nform; ++f) {
+      double sr = yr, si = yi;
+#pragma unroll
+      for (int m = 16; m >= 1; m >>= 1) {
+        sr += __shfl_xor(sr, m, 32);
+        si += __shfl_xor(si, m, 32);
+      }
+      if (r == 0 && trow < it.ntri) {
+        double* o = it.part + 2 * (((int64_t)(form0 + f) * it.nyt + yt) * it.ntri + trow);
+        o[0] = sr;
+        o[1] = si;
+      }
+      yr *= qyv;  // next form: one more power of qy
+      yi *= qyv;
+    }
+  }
+}
+
+// Sums the forms over y-tiles (in order) and assembles (dL/dA', dL/dc') per triangle.
+__global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __restrict__ part, int nyt, int64_t ntri,
+                                                                const double* __restrict__ rec,
+                                                                const double* __restrict__ qxp, Rot R,
+                                                                const double* __restrict__ w0p,
+                                                                const int64_t* __restrict__ src,
+                                                                double* __restrict__ grad) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntri) return;
+  const double qx0 = qxp[0];
+  cd B[kAdjForms];
+#pragma unroll
+  for (int f = 0; f < kAdjForms; ++f) {
+    B[f] = cd{0.0, 0.0};
+    for (int y = 0; y < nyt; ++y) {
+      const double* o = part + 2 * (((int64_t)f * nyt + y) * ntri + t);
+      B[f] = B[f] + cd{o[0], o[1]};
+    }
+  }
+  // V = (qx0 B11, B1y, Bz1); M = [[qx0^2 B11, qx0 B1y, qx0 Bz1], [., B1yy, Bzy], [., ., Bzz]]
+  const cd V[3] = {qx0 * B[0], B[1], B[3]};
+  const cd M[3][3] = {{(qx0 * qx0) * B[0], qx0 * B[1], qx0 * B[3]},
+                      {qx0 * B[1], B[2], B[4]},
+                      {qx0 * B[3], B[4], B[5]}};
+  const double* r = rec + 8 * t;
+  double gA[3], gc[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gA[i] = V[i].r;
+    cd ma = M[i][0] * r[0] + M[i][1] * r[1] + M[i][2] * r[2];
+    gc[i] = -ma.i;  // Re(i z) = -Im z
+  }
+  adj_store(R, rec, t, *w0p, gA, gc, src, grad);
+}
+
+}  // namespace hgx
+
+using namespace hgx;
+
+extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, int flags, const double rot[9],
+                                      const double* W, double* grad) {
+  (void)flags;
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ff_numeric_adjoint: no q-grid");
+  if (mesh_id < 0 || mesh_id >= (int)c->meshes.size() || !c->meshes[mesh_id].live)
+    return set_error(c, HGX_ERR_BAD_HANDLE, "ff_numeric_adjoint: unknown mesh %d", mesh_id);
+  if (!W || !grad) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: null weights or output");
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
+    return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: bad precision %d", precision);
+  if (precision == HGX_PREC_FP32C && !c->product)
+    return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric_adjoint: FP32C needs a product q-grid (use FP64)");
+  const Mesh& m = c->meshes[mesh_id];
+  if (int rc = set_device(c)) return rc;
+  QView g{c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq};
+  Rot R = make_rot(rot);
+  const bool cq = !c->qz_real;
+  // workspace: rotated records | w0 | weights | partials
+  const int nyt = (int)ceil_div(c->ncol, kAdjTile);
+  const int64_t qchunk = 4096;
+  const int nqc = (int)ceil_div(c->nq, qchunk);
+  const size_t rec_b = (size_t)m.ntri * 8 * sizeof(double);
+  const size_t w0_b = 256;
+  const size_t wt_b = (size_t)c->nq * 2 * (precision == HGX_PREC_FP64 ? sizeof(double) : sizeof(float));
+  const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
+                                                   : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + part_b)) return rc;
+  char* w = static_cast<char*>(c->d_work);
+  double* rrec = reinterpret_cast<double*>(w);
+  double* w0 = reinterpret_cast<double*>(w + al(rec_b));
+  void* wt = w + al(rec_b) + w0_b;
+  double* part = reinterpret_cast<double*>(w + al(rec_b) + w0_b + al(wt_b));
+  k_adj_rotate<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(m.d_rec64, m.ntri, R, rrec);
+  HGX_CHECK_LAUNCH(c);
+  k_adj_q0<<<1, 256, 0, c->stream>>>(g, W, w0);
+  HGX_CHECK_LAUNCH(c);
+  if (m.ntri_in > m.ntri) {
+    k_adj_zero<<<ceil_div(m.ntri_in * 6, 256), 256, 0, c->stream>>>(grad, m.ntri_in * 6);
+    HGX_CHECK_LAUNCH(c);
+  }
+  if (precision == HGX_PREC_FP64) {
+    k_adj_weights<double><<<ceil_div(c->nq, 256), 256, 0, c->stream>>>(g, W, static_cast<double*>(wt));
+    HGX_CHECK_LAUNCH(c);
+    dim3 grid(ceil_div(m.ntri, 256), nqc);
+    {
+      LaunchTimer lt(c, true);
+      if (cq) k_adjoint64<true><<<grid, 256, 0, c->stream>>>(g, static_cast<double*>(wt), rrec, m.ntri, qchunk, part);
+      else k_adjoint64<false><<<grid, 256, 0, c->stream>>>(g, static_cast<double*>(wt), rrec, m.ntri, qchunk, part);
+    }
+    HGX_CHECK_LAUNCH(c);
+    k_adjoint64_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nqc, m.ntri, rrec, R, w0, m.d_src, grad);
+    HGX_CHECK_LAUNCH(c);
+    return HGX_OK;
+  }
+  k_adj_weights<float><<<ceil_div(c->nq, 256), 256, 0, c->stream>>>(g, W, static_cast<float*>(wt));
+  HGX_CHECK_LAUNCH(c);
+  AdjItem it;
+  it.qx = c->d_qx;
+  it.qy = c->d_qy;
+  it.qz = c->d_qz;
+  it.wt = static_cast<const float*>(wt);
+  it.rec = rrec;
+  it.part = part;
+  it.ntri = m.ntri;
+  it.ncol = c->ncol;
+  it.nz = c->nbranch * c->nrow;
+  it.nyt = nyt;
+  it.ntt = (int)ceil_div(m.ntri, kAdjTile);
+  const double units = 3.0 * it.nyt * it.ntt;
+  if (units > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
+  {
+    LaunchTimer lt(c, true);
+    const int nunits = (int)units;
+    if (cq) k_adjoint_product32<true><<<ceil_div(nunits, 4), 256, 0, c->stream>>>(it, nunits);
+    else k_adjoint_product32<false><<<ceil_div(nunits, 4), 256, 0, c->stream>>>(it, nunits);
+  }
+  HGX_CHECK_LAUNCH(c);
+  k_adjoint_product_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nyt, m.ntri, rrec, c->d_qx, R, w0,
+                                                                         m.d_src, grad);
+  HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}

@@ -68,6 +68,7 @@ static void free_mesh(Mesh& m) {
   if (m.d_borig) (void)hipFree(m.d_borig);
   if (m.d_tri9) (void)hipFree(m.d_tri9);
   if (m.d_moments) (void)hipFree(m.d_moments);
+  if (m.d_src) (void)hipFree(m.d_src);
   m = Mesh();
 }
 
@@ -544,6 +545,7 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   // the way the OBJ reader does (src/file/objectshape_reader.cpp:271-300).
   std::vector<double> s_n_c;  // 7 per triangle
   std::vector<double> tri9;
+  std::vector<int64_t> kept;  // caller's index of every kept triangle
   s_n_c.reserve(7 * ntri_in);
   for (int64_t t = 0; t < ntri_in; ++t) {
     if (kind == HGX_MESH_APPROX8) {
@@ -553,6 +555,7 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
       if (!finite) return set_error(c, HGX_ERR_INVALID_ARG, "mesh_upload: non-finite record %lld", (long long)t);
       if (r[0] == 0.0) continue;
       s_n_c.insert(s_n_c.end(), r, r + 7);
+      kept.push_back(t);
     } else {
       const double* v = rec + 9 * t;
       for (int k = 0; k < 9; ++k)
@@ -571,6 +574,7 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
                       (v[1] + v[4] + v[7]) / 3.0, (v[2] + v[5] + v[8]) / 3.0};
       s_n_c.insert(s_n_c.end(), r7, r7 + 7);
       tri9.insert(tri9.end(), v, v + 9);
+      kept.push_back(t);
     }
   }
   int64_t ntri = (int64_t)(s_n_c.size() / 7);
@@ -600,6 +604,9 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   Mesh m;
   m.kind = kind;
   m.ntri = ntri;
+  m.ntri_in = ntri_in;
+  std::vector<int64_t> src(ntri);
+  for (int64_t t = 0; t < ntri; ++t) src[t] = kept[order[t]];
   m.nblocks = (ntri + kBlockTri - 1) / kBlockTri;
   std::vector<double> rec64(8 * ntri, 0.0);
   std::vector<float> rec32(8 * m.nblocks * kBlockTri, 0.0f);
@@ -669,11 +676,13 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   if (hipMalloc(&m.d_rec32, rec32.size() * sizeof(float)) != hipSuccess) return fail("rec32");
   if (hipMalloc(&m.d_borig, borig.size() * sizeof(double)) != hipSuccess) return fail("block origins");
   if (hipMalloc(&m.d_moments, M.size() * sizeof(double)) != hipSuccess) return fail("moments");
+  if (hipMalloc(&m.d_src, src.size() * sizeof(int64_t)) != hipSuccess) return fail("index map");
   if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_tri9, tri9s.size() * sizeof(double)) != hipSuccess) return fail("tri9");
   HGX_CHECK_HIP(c, hipMemcpy(m.d_rec64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
   HGX_CHECK_HIP(c, hipMemcpy(m.d_rec32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
   HGX_CHECK_HIP(c, hipMemcpy(m.d_borig, borig.data(), borig.size() * sizeof(double), hipMemcpyHostToDevice));
   HGX_CHECK_HIP(c, hipMemcpy(m.d_moments, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  HGX_CHECK_HIP(c, hipMemcpy(m.d_src, src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   if (kind == HGX_MESH_EXACT9)
     HGX_CHECK_HIP(c, hipMemcpy(m.d_tri9, tri9s.data(), tri9s.size() * sizeof(double), hipMemcpyHostToDevice));
   m.live = true;

@@ -25,6 +25,7 @@ struct Mesh {
   bool live = false;
   int kind = 0;
   int64_t ntri = 0;       // triangles (after dropping degenerate ones)
+  int64_t ntri_in = 0;    // triangles as uploaded
   int64_t nblocks = 0;    // ceil(ntri / kBlockTri)
   double volume = 0.0;    // sum_t s_t (n_t . c_t) / 3
   double origin[3] = {0, 0, 0};
@@ -35,6 +36,7 @@ struct Mesh {
   double* d_borig = nullptr;   // [nblocks][4]: block origin x, y, z, 0
   double* d_tri9 = nullptr;    // [ntri][9] vertices (exact meshes only), same order as d_rec64
   double* d_moments = nullptr; // [kMomentCount] moments about origin (see ff_numeric.hip)
+  int64_t* d_src = nullptr;    // [ntri] caller's index of each stored (sorted) triangle
 };
 
 // A chi^2 plan: model constants and the device inputs it reads (caller-owned).

@@ -171,6 +171,16 @@ int hgx_mesh_info(const hgx_ctx* ctx, int mesh_id, int64_t* ntri, double* volume
 int hgx_ff_numeric(hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, const double rot[9],
                    const double transvec[3], double* ff);
 
+/* Record adjoint of the approximate numeric form factor. For L = Re sum_q W(q) F(q), F as computed by
+ * hgx_ff_numeric(HGX_FF_APPROX) for the same mesh and rotation and W complex [2*nq] on the device,
+ * writes grad[6 * ntri_uploaded] (device) = (dL/d(s n)_t, dL/dc_t) per triangle in the order the mesh
+ * was uploaded (zero for triangles dropped as degenerate). Gradients of any mesh parameterisation
+ * follow by the chain rule through (s n_t, c_t). HGX_PREC_FP64 works on any grid; HGX_PREC_FP32C needs
+ * a product grid (matrix cores). No reference counterpart: the reference differentiates by finite
+ * differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). */
+int hgx_ff_numeric_adjoint(hgx_ctx* ctx, int mesh_id, int precision, int flags, const double rot[9],
+                           const double* W, double* grad);
+
 /* Analytic form factor, replaces AnalyticFormFactorG::compute_<shape>
  * (include/ff/gpu/ff_ana_gpu.cuh:54-94) with the CPU formulas of src/ff/ff_ana_<shape>.cpp. */
 int hgx_ff_analytic(hgx_ctx* ctx, int shape, const hgx_shape_params* params, int precision,

@@ -30,6 +30,7 @@ def load():
             "hgxo_fresnel": (_i, [_i, _vp, _vp, _vp, _i, _d, _d, _i, _i, _vp, _vp]),
             "hgxo_ff_approx": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_exact": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i, _d, _vp]),
+            "hgxo_ff_approx_adjoint": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_param_distribution": (_i, [_d, _d, _i, _i, _d, _d, _vp, _vp, _i]),
             "hgxo_ff_sphere": (None, [_i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_cylinder": (None, [_i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -142,6 +143,17 @@ def ff_approx(rec8, g: Grid, rot=None) -> np.ndarray:
     out = np.empty(2 * nq)
     load().hgxo_ff_approx(r.shape[0], _p(r), npix, nq, qx, qy, _p(qz), _p(R), _p(out))
     return _r2c(out)
+
+
+def ff_approx_adjoint(rec8, g: Grid, W, rot=None) -> np.ndarray:
+    """Record adjoint of L = Re sum_q W(q) F(q): [ntri, 6] = (dL/dA_t, dL/dc_t), A_t = s_t n_t."""
+    r = _a(rec8).reshape(-1, 8)
+    npix, nq, qx, qy, _, R = _qargs(g, rot)
+    qz = _c2r(g.qz)
+    w = _c2r(np.asarray(W, dtype=np.complex128).ravel())
+    out = np.empty(6 * r.shape[0])
+    load().hgxo_ff_approx_adjoint(r.shape[0], _p(r), npix, nq, qx, qy, _p(qz), _p(R), _p(w), _p(out))
+    return out.reshape(-1, 6)
 
 
 def ff_exact(tri9, g: Grid, rot=None, sign=+1, tiny=0.0) -> np.ndarray:

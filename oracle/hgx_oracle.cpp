@@ -252,6 +252,57 @@ void hgxo_ff_approx(int64_t ntri, const double* rec, int64_t npix, int64_t nq, c
   }
 }
 
+// Record adjoint of the approximate form factor (no reference counterpart: the reference has
+// only finite-difference gradients, src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). For
+// L = Re sum_q W(q) F(q), with F as in hgxo_ff_approx and A_t = s_t n_t:
+//   dL/dA_t = Re sum_q W (-i / |q|^2) mq exp(i mq.c_t)
+//   dL/dc_t = Re sum_q W (mq.A_t) / |q|^2 mq exp(i mq.c_t)
+// and at q = 0 (F = V = sum_t A_t.c_t / 3): dL/dA_t += Re W c_t / 3, dL/dc_t += Re W A_t / 3.
+// grad[t] = (dL/dA_t, dL/dc_t). Pinned by central finite differences of hgxo_ff_approx.
+void hgxo_ff_approx_adjoint(int64_t ntri, const double* rec, int64_t npix, int64_t nq, const double* qx,
+                            const double* qy, const double* qz, const double* rot, const double* W,
+                            double* grad) {
+  std::vector<cplx> mq(3 * nq);
+  std::vector<double> inv_q2(nq);
+  for (int64_t iz = 0; iz < nq; ++iz) {
+    int64_t iy = iz % npix;
+    rotate_q(rot, qx[iy], qy[iy], qz_at(qz, iz), &mq[3 * iz]);
+    double q2 = std::norm(mq[3 * iz]) + std::norm(mq[3 * iz + 1]) + std::norm(mq[3 * iz + 2]);
+    inv_q2[iz] = q2 == 0.0 ? 0.0 : 1.0 / q2;
+  }
+  const cplx nj(0.0, -1.0), np(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < ntri; ++t) {
+    const double* r = rec + 8 * t;
+    const double A[3] = {r[0] * r[1], r[0] * r[2], r[0] * r[3]};
+    const double c[3] = {r[4], r[5], r[6]};
+    double gA[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
+    for (int64_t iz = 0; iz < nq; ++iz) {
+      const cplx w(W[2 * iz], W[2 * iz + 1]);
+      const cplx* m = &mq[3 * iz];
+      if (inv_q2[iz] == 0.0) {  // q = 0: F = V
+        for (int k = 0; k < 3; ++k) {
+          gA[k] += w.real() * c[k] / 3.0;
+          gc[k] += w.real() * A[k] / 3.0;
+        }
+        continue;
+      }
+      cplx e = std::exp(np * (m[0] * c[0] + m[1] * c[1] + m[2] * c[2]));
+      cplx qa = m[0] * A[0] + m[1] * A[1] + m[2] * A[2];
+      cplx wa = w * nj * inv_q2[iz] * e;   // multiplies mq for dL/dA
+      cplx wc = w * qa * inv_q2[iz] * e;   // multiplies mq for dL/dc
+      for (int k = 0; k < 3; ++k) {
+        gA[k] += (wa * m[k]).real();
+        gc[k] += (wc * m[k]).real();
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      grad[6 * t + k] = gA[k];
+      grad[6 * t + 3 + k] = gc[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // a8: FormFactorTriangle / compute_exact_triangle (src/ff/cpu/ff_tri_cpu.cpp:46-195).
 // The reference evaluates the exact (Green's theorem) triangle integral with phase exp(-i q.v);
