@@ -78,6 +78,20 @@ class FitModel(C.Structure):
     ]
 
 
+class GratingModel(C.Structure):
+    _fields_ = [
+        ("length", C.c_double),
+        ("pitch", C.c_double),
+        ("periods", C.c_int),
+        ("n_bottom", C.c_int),
+        ("n_side", C.c_int),
+        ("n_top", C.c_int),
+        ("n_length", C.c_int),
+        ("dn2_re", C.c_double),
+        ("dn2_im", C.c_double),
+    ]
+
+
 class BatchItem(C.Structure):
     _fields_ = [
         ("grid", _vp),
@@ -135,6 +149,13 @@ _SIGS = {
                                          _dp, _dp, C.c_int]),
     "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product": (C.c_int, [C.c_int, C.c_int]),
+    "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
+    "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_grating_set_theta": (C.c_int, [_vp, C.c_int, _vp]),
+    "hgx_grating_eval": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hgx_grating_lbfgs": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hgx_grating_get": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "hgx_grating_intensity": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
 }
 
 _lib = None

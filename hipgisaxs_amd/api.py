@@ -381,3 +381,51 @@ def tune_numeric(qpt64: int, qpt32: int, target_wg: int) -> None:
     rc = N.load().hgx_tune_numeric(qpt64, qpt32, target_wg)
     if rc != N.HGX_OK:
         raise HgxError("tune_numeric: invalid knob values")
+
+
+class GratingPlan:
+    """hgx_grating_*: batched fit of trapezoidal grating lines. theta rows = (top width, sidewall angle in
+    rad, height, scale)."""
+
+    def __init__(self, ctx: Context, ncand: int, I_obs, fc=None, mask=None, length=100.0, pitch=100.0,
+                 periods=200, n_bottom=64, n_side=40, n_top=64, n_length=256, dn2=1.0 + 0j):
+        self.ctx, self.ncand = ctx, ncand
+        m = N.GratingModel()
+        m.length, m.pitch, m.periods = length, pitch, periods
+        m.n_bottom, m.n_side, m.n_top, m.n_length = n_bottom, n_side, n_top, n_length
+        m.dn2_re, m.dn2_im = complex(dn2).real, complex(dn2).imag
+        self._keep = (I_obs, fc, mask)
+        pid = C.c_int()
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+        ctx._check(ctx.lib.hgx_grating_plan_create(ctx.h, C.byref(m), ncand, ptr(fc), ptr(I_obs), ptr(mask),
+                                                   C.byref(pid)), "grating_plan_create")
+        self.id = pid.value
+
+    def set_theta(self, theta):
+        t = _f64(theta).reshape(self.ncand, 4)
+        self.ctx._check(self.ctx.lib.hgx_grating_set_theta(self.ctx.h, self.id, _ptr(t)), "grating_set_theta")
+
+    def eval(self, use_graph=False):
+        self.ctx._check(self.ctx.lib.hgx_grating_eval(self.ctx.h, self.id, int(use_graph)), "grating_eval")
+
+    def lbfgs(self, iterations: int):
+        self.ctx._check(self.ctx.lib.hgx_grating_lbfgs(self.ctx.h, self.id, iterations), "grating_lbfgs")
+
+    def get(self):
+        th = np.empty((self.ncand, 4))
+        c2 = np.empty(self.ncand)
+        g = np.empty((self.ncand, 4))
+        self.ctx._check(self.ctx.lib.hgx_grating_get(self.ctx.h, self.id, _ptr(th), _ptr(c2), _ptr(g)), "grating_get")
+        return th, c2, g
+
+    def intensity(self, cand: int, out=None):
+        npix = self.ctx.info()["npix"]
+        o = self.ctx._out(npix, out)
+        self.ctx._check(self.ctx.lib.hgx_grating_intensity(self.ctx.h, self.id, cand, C.c_void_p(o.data_ptr())),
+                        "grating_intensity")
+        return o
+
+    def release(self):
+        if self.id is not None:
+            self.ctx._check(self.ctx.lib.hgx_grating_plan_release(self.ctx.h, self.id), "grating_plan_release")
+            self.id = None

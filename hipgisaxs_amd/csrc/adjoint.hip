@@ -23,6 +23,7 @@
 // product (triangles x z) . (z x y) on v_mfma_f32_32x32x2_f32, Gauss's three real products -- and then
 // contracts each row of X with g(y) Ey_t(y) across the wave's 32 columns. Phases come from FP64
 // absolute centroids reduced to turns per table entry, so no block origins are needed.
+#include "adjoint_core.hpp"
 #include "numeric_core.hpp"
 
 namespace hgx {
@@ -161,17 +162,6 @@ __global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ p, int64_
 
 // ----- FP32C on product grids (matrix cores) -------------------------------------------------------
 
-struct AdjItem {
-  const double* qx;  // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
-  const double* qy;
-  const double* qz;
-  const float* wt;   // [nq] complex Wt = -i W / |q|^2
-  const double* rec; // [ntri][8] rotated FP64 records
-  double* part;      // [kAdjForms][nyt][ntri] complex
-  int64_t ntri;
-  int ncol, nz, nyt, ntt;
-};
-
 __device__ inline f32x16a mfma32a(float a, float b, f32x16a c) {
   return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
 }
@@ -180,15 +170,19 @@ __device__ inline int adj_acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h 
 
 // One wave = (f-pass, 32 triangles, 32 y-columns) over all z. Lane row r of the A operand is triangle
 // t0 + r at z = 2s + h; lane column r of the B operand is Wt(z = 2s + h, y0 + r).
+// Units: item = uidx / upi, then (f-pass, triangle tile, y-tile) with the y-tile fastest; items == NULL
+// means the single item passed by value.
 template <bool CQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adjoint_product32(AdjItem it,
-                                                                                                      int nunits) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adjoint_product32(
+    const AdjItem* __restrict__ items, AdjItem single, int upi, int nunits) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int uidx = (int)blockIdx.x * 4 + wave;
   if (uidx >= nunits) return;  // whole waves leave; no barriers below
-  const int yt = uidx % it.nyt;
-  const int tt = (uidx / it.nyt) % it.ntt;
-  const int fp = uidx / (it.nyt * it.ntt);  // 0: f = 1, 1: f = qz, 2: f = qz^2
+  const int item = uidx / upi, ul = uidx % upi;
+  const AdjItem it = items ? items[item] : single;  // wave-uniform copy (scalar loads)
+  const int yt = ul % it.nyt;
+  const int tt = (ul / it.nyt) % it.ntt;
+  const int fp = ul / (it.nyt * it.ntt);  // 0: f = 1, 1: f = qz, 2: f = qz^2
   const int r = lane & 31, h = lane >> 5;
   const int ncol = it.ncol, nz = it.nz;
   const int64_t t0 = (int64_t)tt * kAdjTile;
@@ -292,30 +286,16 @@ __global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __
                                                                 double* __restrict__ grad) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntri) return;
-  const double qx0 = qxp[0];
-  cd B[kAdjForms];
-#pragma unroll
-  for (int f = 0; f < kAdjForms; ++f) {
-    B[f] = cd{0.0, 0.0};
-    for (int y = 0; y < nyt; ++y) {
-      const double* o = part + 2 * (((int64_t)f * nyt + y) * ntri + t);
-      B[f] = B[f] + cd{o[0], o[1]};
-    }
-  }
-  // V = (qx0 B11, B1y, Bz1); M = [[qx0^2 B11, qx0 B1y, qx0 Bz1], [., B1yy, Bzy], [., ., Bzz]]
-  const cd V[3] = {qx0 * B[0], B[1], B[3]};
-  const cd M[3][3] = {{(qx0 * qx0) * B[0], qx0 * B[1], qx0 * B[3]},
-                      {qx0 * B[1], B[2], B[4]},
-                      {qx0 * B[3], B[4], B[5]}};
-  const double* r = rec + 8 * t;
   double gA[3], gc[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    gA[i] = V[i].r;
-    cd ma = M[i][0] * r[0] + M[i][1] * r[1] + M[i][2] * r[2];
-    gc[i] = -ma.i;  // Re(i z) = -Im z
-  }
+  adj_assemble(part, nyt, ntri, t, rec, qxp[0], gA, gc);
   adj_store(R, rec, t, *w0p, gA, gc, src, grad);
+}
+
+void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
+                            bool cq) {
+  if (nunits <= 0) return;
+  if (cq) k_adjoint_product32<true><<<ceil_div(nunits, 4), 256, 0, s>>>(d_items, single, upi, nunits);
+  else k_adjoint_product32<false><<<ceil_div(nunits, 4), 256, 0, s>>>(d_items, single, upi, nunits);
 }
 
 }  // namespace hgx
@@ -396,9 +376,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   if (units > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
   {
     LaunchTimer lt(c, true);
-    const int nunits = (int)units;
-    if (cq) k_adjoint_product32<true><<<ceil_div(nunits, 4), 256, 0, c->stream>>>(it, nunits);
-    else k_adjoint_product32<false><<<ceil_div(nunits, 4), 256, 0, c->stream>>>(it, nunits);
+    adjoint_product_launch(c->stream, nullptr, it, (int)units, (int)units, cq);
   }
   HGX_CHECK_LAUNCH(c);
   k_adjoint_product_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nyt, m.ntri, rrec, c->d_qx, R, w0,

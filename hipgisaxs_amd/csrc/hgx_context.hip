@@ -257,6 +257,7 @@ int hgx_destroy(hgx_ctx* h) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& m : c->meshes) free_mesh(m);
   for (auto& b : c->batches) free_batch_state(b);
+  free_grating_plans(c);
   free_qgrid(c);
   for (auto* v : {&c->ev_main, &c->ev_aux, &c->ev_pool})
     for (auto& e : *v) {

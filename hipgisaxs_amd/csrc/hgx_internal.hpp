@@ -72,6 +72,10 @@ struct BatchState {
 
 void free_batch_state(BatchState& b);
 
+struct GratingPlan;  // grating.hip
+struct Context;
+void free_grating_plans(Context* c);
+
 // ----- product-grid numeric form factor on the matrix cores (product_mfma.hip) ------------------
 //
 // A q-grid is a product grid when q(row i, column j, branch b) = (qx0, qy_j, qz_{b,i}): the Born
@@ -106,6 +110,18 @@ struct ProductUnit {
   int chunk;
 };
 
+// One mesh for the product-grid adjoint kernel (adjoint.hip).
+struct AdjItem {
+  const double* qx;  // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
+  const double* qy;
+  const double* qz;
+  const float* wt;   // [nq] complex Wt = -i W / |q|^2
+  const double* rec; // [ntri][8] rotated FP64 records (A', c')
+  double* part;      // [6 forms][nyt][ntri] complex: B11, B1y, B1yy, Bz1, Bzy, Bzz per y-tile
+  int64_t ntri;
+  int ncol, nz, nyt, ntt;
+};
+
 constexpr int kProductTile = 32;
 constexpr int kProductWaves = 4;  // work units (waves) per workgroup
 
@@ -114,6 +130,7 @@ struct Context {
   int device = 0;
   std::vector<FitPlan> fit_plans;
   std::vector<BatchState> batches;
+  std::vector<GratingPlan*> gratings;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   // q-grid
@@ -264,6 +281,10 @@ void product_launch_units(hipStream_t s, const ProductItem* d_items, const Produ
                           bool cq, bool gauss);
 // Launches every (z-tile, y-tile, chunk) unit of a single item.
 void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, bool gauss);
+// Launches the product-grid adjoint over items (d_items == NULL: the single item by value);
+// upi units per item (3 f-passes x triangle tiles x y-tiles).
+void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
+                            bool cq);
 // Tuning: triangle blocks per work unit and 3- or 4-multiplication complex products.
 int product_blocks_per_unit();
 bool product_gauss();

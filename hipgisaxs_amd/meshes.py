@@ -136,3 +136,39 @@ def convex_polyhedron(rng: np.random.Generator, semi_axes, target_tri: int) -> n
 def mesh_volume(tri: np.ndarray) -> float:
     t = np.asarray(tri, dtype=np.float64).reshape(-1, 3, 3)
     return float(np.einsum("ij,ij->", t[:, 0], np.cross(t[:, 1], t[:, 2])) / 6.0)
+
+
+def trapezoid_line(top, angle, height, length, n_bottom, n_side, n_top, n_length):
+    """Closed mesh of a line of trapezoidal cross-section: base [-wb/2, wb/2] on z = 0 with
+    wb = top + 2 height cot(angle), top [-top/2, top/2] at z = height, extruded along x over
+    [-length/2, length/2]. Perimeter in n_bottom + 2 n_side + n_top segments (counter-clockwise in
+    y-z), n_length segments along x, two triangles per quad, caps fanned from the corner centroid
+    (0, height/2). Same triangles as the device mesher of the grating fit. Returns [ntri, 9]."""
+    wb = top + 2.0 * height / np.tan(angle)
+    C = np.array([[-wb / 2, 0.0], [wb / 2, 0.0], [top / 2, height], [-top / 2, height]])
+    P = []
+    for a, n in ((0, n_bottom), (1, n_side), (2, n_top), (3, n_side)):
+        b = (a + 1) % 4
+        for i in range(n):
+            u = i / n
+            P.append((1 - u) * C[a] + u * C[b])
+    P = np.array(P)
+    npr = len(P)
+    xs = -length / 2 + length * np.arange(n_length + 1) / n_length
+    tris = []
+    for i in range(n_length):
+        for k in range(npr):
+            k1 = (k + 1) % npr
+            a = (xs[i], *P[k])
+            b = (xs[i], *P[k1])
+            c = (xs[i + 1], *P[k1])
+            d = (xs[i + 1], *P[k])
+            tris.append((*a, *b, *c))
+            tris.append((*a, *c, *d))
+    for ci, x in ((0, -length / 2), (1, length / 2)):
+        M = (x, 0.0, height / 2)
+        for k in range(npr):
+            k1 = (k + 1) % npr
+            pa, pb = (P[k], P[k1]) if ci else (P[k1], P[k])
+            tris.append((*M, x, *pa, x, *pb))
+    return np.array(tris, dtype=np.float64)

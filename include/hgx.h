@@ -290,6 +290,46 @@ int hgx_chi2_fwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, d
  * adjoint: one pass over the pixels that recomputes the amplitudes instead of storing them. */
 int hgx_chi2_bwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, double* chi2, double grad[6]);
 
+/* ---- batched trapezoid-grating fit --------------------------------------------------------- */
+
+/* Lines of trapezoidal cross-section in the y-z plane (base on z = 0) extruded along x over `length`,
+ * repeated `periods` times along y with `pitch`. Each candidate's line is meshed on the device: the
+ * cross-section perimeter in n_bottom + 2 n_side + n_top segments (a multiple of 16) and the length in
+ * n_length segments (a multiple of 8), two triangles per quad, end caps fanned from the corner
+ * centroid. Parameters theta = (top width, sidewall angle in rad, height, scale); the bottom width is
+ * top + 2 height cot(angle). */
+typedef struct hgx_grating_model {
+  double length, pitch;
+  int periods;
+  int n_bottom, n_side, n_top, n_length;
+  double dn2_re, dn2_im; /* contrast (1 - n^2) of the line material */
+} hgx_grating_model;
+
+/* I(theta) = scale |dn2 SF(qy) sum_b fc_b F_b(q_b; theta)|^2 with F the approximate-triangle form
+ * factor of the line's mesh and SF = sin(N qy pitch / 2) / sin(qy pitch / 2); chi^2 = sum_p mask_p
+ * (I_p - O_p)^2 / max(O_p, 1), leaving out pixels with |q_b| R < 0.1 on any branch (R: the line's
+ * radius). The context's q-grid must be a product grid (FP32C on the matrix cores). fc [2*nq] or NULL
+ * (Born), I_obs [npix] and mask [npix] (or NULL) are device arrays read by pointer. ncand candidates
+ * are evaluated together; device memory is bounded by running them in groups. No reference
+ * counterpart: the reference fits with TAO/PSO and finite-difference gradients
+ * (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224, src/analyzer/hipgisaxs_fit_pso.cpp). */
+int hgx_grating_plan_create(hgx_ctx* ctx, const hgx_grating_model* model, int ncand, const double* fc,
+                            const double* I_obs, const double* mask, int* plan);
+int hgx_grating_plan_release(hgx_ctx* ctx, int plan);
+/* Sets every candidate's theta (host [ncand][4]) and resets its optimiser state. */
+int hgx_grating_set_theta(hgx_ctx* ctx, int plan, const double* theta);
+/* chi^2 and its gradient (adjoint, one pass) for every candidate at its theta; use_graph replays the
+ * evaluation from a hipGraph captured on first use. */
+int hgx_grating_eval(hgx_ctx* ctx, int plan, int use_graph);
+/* `iterations` L-BFGS iterations (memory 6, Armijo backtracking on the step) for every candidate, on the
+ * device: one chi^2 + gradient evaluation per iteration, each iteration one hipGraph launch. */
+int hgx_grating_lbfgs(hgx_ctx* ctx, int plan, int iterations);
+/* Host copies of each candidate's accepted theta [ncand][4], chi^2 [ncand] and gradient [ncand][4]
+ * (any may be NULL). Synchronises the stream. */
+int hgx_grating_get(hgx_ctx* ctx, int plan, double* theta, double* chi2, double* grad);
+/* Model image of candidate cand at its accepted theta into I [npix] (device). */
+int hgx_grating_intensity(hgx_ctx* ctx, int plan, int cand, double* I);
+
 /* ---- host helpers ------------------------------------------------------------------------- */
 
 /* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values of a shape
