@@ -178,6 +178,71 @@ def bench_fit(evals=20):
             "workload": "cylinders R4 H12, hcp a=15 20x20x1, 512^2 DWBA, 6 parameters, FP64 adjoint"}
 
 
+def bench_grating(ncand=16, iters=2):
+    """Trapezoidal grating lines (top 20 nm, sidewall 82 deg, height 40 nm; pitch 100 nm, 200 periods,
+    length 100 nm) meshed to 106 912 triangles, 1024^2 q-space grid qy [-2, 2], qz [0, 2] with four
+    DWBA branches over Si, 10 keV, 0.2 deg; seeded Poisson target at 1e4 peak counts. ncand candidates
+    (perturbed starts) run forward + adjoint + L-BFGS update per hipGraph step; the 256-candidate x
+    200-iteration run is extrapolated from the measured per-iteration time (candidates are independent
+    work in one launch, so the time is linear in their number)."""
+    import torch
+    from hipgisaxs_amd.api import Context, GratingPlan
+
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    ctx = Context(0)
+    ctx.qgrid_explicit(1024, 1024, -2.0, 0.0, 2.0, 2.0, ai, k0)
+    ctx.qz_extended(k0, ai, 0j)
+    fc = ctx.fresnel([4.88e-6], [7.37e-8], [0.0], 0, k0, ai)
+    model = dict(length=100.0, pitch=100.0, periods=200, n_bottom=64, n_side=40, n_top=64, n_length=256,
+                 dn2=2e-5 + 1e-7j)
+    truth = np.array([20.0, math.radians(82.0), 40.0, 1.0])
+    npix = 1024 * 1024
+    zero = torch.zeros(npix, dtype=torch.float64, device="cuda:0")
+    p1 = GratingPlan(ctx, 1, zero, fc=fc, **model)
+    p1.set_theta(truth[None, :])
+    I = p1.intensity(0).cpu().numpy()
+    p1.release()
+    truth[3] = 1e4 / I.max()
+    rng = np.random.default_rng(SEED + 5)
+    obs = torch.from_numpy(rng.poisson(1e4 * I / I.max()).astype(np.float64)).cuda()
+    starts = truth * (1.0 + rng.uniform(-0.08, 0.08, size=(ncand, 4)))
+    plan = GratingPlan(ctx, ncand, obs, fc=fc, **model)
+    plan.set_theta(starts)
+    plan.eval(use_graph=True)  # capture + first evaluation
+    torch.cuda.synchronize()
+    _, chi0, _ = plan.get()
+    plan.lbfgs(1)  # capture of the L-BFGS step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.lbfgs(iters)
+    torch.cuda.synchronize()
+    t_iter = (time.perf_counter() - t0) / iters
+    _, chi1, _ = plan.get()
+    # one forward alone (mesh + FP32C form factor + intensity) for the gradient / forward cost ratio
+    Ibuf = torch.empty(npix, dtype=torch.float64, device="cuda:0")
+    plan.intensity(0, out=Ibuf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(4):
+        plan.intensity(k, out=Ibuf)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / 4
+    plan.release()
+    ctx.close()
+    nslots = (256 // 8 * 208 // 8 + 2 * 208 // 16) * 128
+    pairs = 4 * npix * nslots  # forward (q-point x triangle slot) pairs per candidate
+    return {"grad_evals_per_s": ncand / t_iter, "ms_per_iteration": 1e3 * t_iter, "candidates": ncand,
+            "ms_forward_per_candidate": 1e3 * t_fwd, "grad_over_fwd": t_iter / ncand / t_fwd,
+            "forward_pairs_per_candidate": pairs,
+            "fwd_plus_adjoint_pair_rate": ncand * pairs / t_iter,
+            "extrapolated_s_256x200": 256 / ncand * t_iter * 200,
+            "chi2_median_start": float(np.median(chi0)), "chi2_median_after": float(np.median(chi1)),
+            "iterations_timed": iters,
+            "workload": "trapezoid lines 106912 triangles, pitch 100 x 200, 1024^2 DWBA (4 branches), "
+                        "FP32C forward + adjoint + L-BFGS per hipGraph step"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,6 +437,8 @@ def main():
         print(f"[bench] lattice done", file=sys.stderr, flush=True)
         result["chi2_cylinders_hcp"] = bench_fit()
         print(f"[bench] fit done", file=sys.stderr, flush=True)
+        result["grating_fit"] = bench_grating()
+        print(f"[bench] grating done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb, (s, rows, ref) = cpu_baseline(shapes)
         got = Context.as_complex(s["out32"]).cpu().numpy()[: rows * s["side"]]

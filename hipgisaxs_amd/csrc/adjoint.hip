@@ -168,83 +168,128 @@ __device__ inline f32x16a mfma32a(float a, float b, f32x16a c) {
 
 __device__ inline int adj_acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (rr & 3); }
 
-// One wave = (f-pass, 32 triangles, 32 y-columns) over all z. Lane row r of the A operand is triangle
-// t0 + r at z = 2s + h; lane column r of the B operand is Wt(z = 2s + h, y0 + r).
-// Units: item = uidx / upi, then (f-pass, triangle tile, y-tile) with the y-tile fastest; items == NULL
-// means the single item passed by value.
+// One workgroup = (item, f-pass, 128-triangle block, 32 y-columns) over all z; wave w takes the block's
+// triangles 32 w .. 32 w + 31. Per chunk of 64 z-rows the workgroup stages the weight tile
+// Wt[64 z][32 y] and, per z, the block's reduced base phase q_z o_bz / 2pi and the FP32 axis values in
+// LDS, so each weight is read from memory once per block instead of once per wave, and the inner loop
+// has no FP64 work: phase = base(z) + qz(z) (c_z - o_bz) / 2pi in turns for the hardware sin/cos.
+// MFMA: lane row r of the A operand is triangle t0 + r at z = 2s + h; lane column r of the B operand is
+// Wt(z = 2s + h, y0 + r).
+constexpr int kAdjZChunk = 64;
+
+struct AdjZ {
+  float base, uz, qzr, qzi, attb, az, pad0, pad1;
+};
+
+template <bool CQ>
+__device__ inline void adj_fetch(const AdjItem& it, int z0, int y0, double oz, int tid, float2 (&w)[8], AdjZ& d) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int idx = tid + 256 * e, zl = idx / kAdjTile, yl = idx % kAdjTile;
+    const int z = z0 + zl, y = y0 + yl;
+    w[e] = make_float2(0.0f, 0.0f);
+    if (z < it.nz && y < it.ncol) w[e] = *reinterpret_cast<const float2*>(it.wt + 2 * ((int64_t)z * it.ncol + y));
+  }
+  d = AdjZ{};
+  const int z = z0 + tid;
+  if (tid < kAdjZChunk && z < it.nz) {
+    const double qr = it.qz[2 * (int64_t)z * it.ncol], qi = CQ ? it.qz[2 * (int64_t)z * it.ncol + 1] : 0.0;
+    const double tb = qr * oz * kInvTwoPi;
+    d.base = (float)(tb - rint(tb));
+    d.uz = (float)(qr * kInvTwoPi);
+    d.qzr = (float)qr;
+    d.qzi = (float)qi;
+    d.attb = (float)(-qi * oz * 1.4426950408889634);
+    d.az = (float)(qi * 1.4426950408889634);
+  }
+}
+
 template <bool CQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adjoint_product32(
-    const AdjItem* __restrict__ items, AdjItem single, int upi, int nunits) {
+    const AdjItem* __restrict__ items, AdjItem single, int upi) {
+  // two LDS buffers: chunk k is read from one while chunk k + 1 is written into the other
+  __shared__ float2 s_w[2][kAdjZChunk][kAdjTile];
+  __shared__ AdjZ s_z[2][kAdjZChunk];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int uidx = (int)blockIdx.x * 4 + wave;
-  if (uidx >= nunits) return;  // whole waves leave; no barriers below
-  const int item = uidx / upi, ul = uidx % upi;
-  const AdjItem it = items ? items[item] : single;  // wave-uniform copy (scalar loads)
+  const int tid = threadIdx.x;
+  const int item = blockIdx.x / upi, ul = blockIdx.x % upi;
+  const AdjItem it = items ? items[item] : single;  // workgroup-uniform copy (scalar loads)
   const int yt = ul % it.nyt;
-  const int tt = (ul / it.nyt) % it.ntt;
-  const int fp = ul / (it.nyt * it.ntt);  // 0: f = 1, 1: f = qz, 2: f = qz^2
+  const int blk = (ul / it.nyt) % it.nblocks;
+  const int fp = ul / (it.nyt * it.nblocks);  // 0: f = 1, 1: f = qz, 2: f = qz^2
   const int r = lane & 31, h = lane >> 5;
   const int ncol = it.ncol, nz = it.nz;
-  const int64_t t0 = (int64_t)tt * kAdjTile;
-  const int64_t ta = min(t0 + r, it.ntri - 1);
-  const int yc = min(yt * kAdjTile + r, ncol - 1);
-  const double czt = it.rec[8 * ta + 5];
-  const float* wcol = it.wt + 2 * (int64_t)yc;
+  const int64_t t0 = (int64_t)blk * kBlockTri + wave * kAdjTile;
+  const float dcz = it.rec32[8 * (t0 + r) + 5];
+  const double oz = it.borig[4 * blk + 2];
+  const int y0 = yt * kAdjTile;
+
+  float2 wr[8];
+  AdjZ dr;
+  adj_fetch<CQ>(it, 0, y0, oz, tid, wr, dr);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) s_w[0][(tid + 256 * e) / kAdjTile][(tid + 256 * e) % kAdjTile] = wr[e];
+  if (tid < kAdjZChunk) s_z[0][tid] = dr;
+  __syncthreads();
 
   f32x16a P1, P2, P3, C1, C2, C3;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = C1[k] = C2[k] = C3[k] = 0.0f;
-  const int nsteps = (nz + 1) / 2;
-  for (int s = 0; s < nsteps; ++s) {
-    const int z = 2 * s + h;
-    const int zc = min(z, nz - 1);
-    const double qzr = it.qz[2 * (int64_t)zc * ncol];
-    const double qzi = CQ ? it.qz[2 * (int64_t)zc * ncol + 1] : 0.0;
-    const double ph = qzr * czt * kInvTwoPi;
-    const float fph = (float)(ph - rint(ph));
-    float er = __builtin_amdgcn_cosf(fph), ei = __builtin_amdgcn_sinf(fph);
-    if (CQ) {
-      const float att = __builtin_amdgcn_exp2f((float)(-qzi * czt * 1.4426950408889634));
-      er *= att;
-      ei *= att;
-    }
-    float ar = er, ai = ei;
-    if (fp > 0) {
-      const float fr = (float)qzr, fi = (float)qzi;
-      float xr = fr * er - fi * ei, xi = fr * ei + fi * er;
-      if (fp > 1) {
-        const float yr = fr * xr - fi * xi, yi = fr * xi + fi * xr;
-        xr = yr;
-        xi = yi;
-      }
-      ar = xr;
-      ai = xi;
-    }
-    if (z >= nz) ar = ai = 0.0f;
-    const float2 w = *reinterpret_cast<const float2*>(wcol + 2 * (int64_t)zc * ncol);
-    P1 = mfma32a(ar, w.x, P1);
-    P2 = mfma32a(ai, w.y, P2);
-    P3 = mfma32a(ar + ai, w.x + w.y, P3);
-    if ((s & 63) == 63) {  // no FP32 sum absorbs more than 128 z-values directly
+  for (int k = 0; k < 16; ++k) C1[k] = C2[k] = C3[k] = 0.0f;
+  const int nchunk = (nz + kAdjZChunk - 1) / kAdjZChunk;
+  for (int ck = 0; ck < nchunk; ++ck) {
+    const int cur = ck & 1;
+    const bool more = ck + 1 < nchunk;
+    if (more) adj_fetch<CQ>(it, (ck + 1) * kAdjZChunk, y0, oz, tid, wr, dr);  // in flight during the MFMAs
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        C1[k] += P1[k];
-        C2[k] += P2[k];
-        C3[k] += P3[k];
-        P1[k] = P2[k] = P3[k] = 0.0f;
+    for (int k = 0; k < 16; ++k) P1[k] = P2[k] = P3[k] = 0.0f;
+#pragma unroll 2
+    for (int s = 0; s < kAdjZChunk / 2; ++s) {
+      const int zl = 2 * s + h;
+      const AdjZ d = s_z[cur][zl];
+      const float2 w = s_w[cur][zl][r];
+      const float ph = __builtin_fmaf(d.uz, dcz, d.base);
+      float er = __builtin_amdgcn_cosf(ph), ei = __builtin_amdgcn_sinf(ph);
+      if (CQ) {
+        const float att = __builtin_amdgcn_exp2f(__builtin_fmaf(-d.az, dcz, d.attb));
+        er *= att;
+        ei *= att;
       }
+      float ar = er, ai = ei;
+      if (fp > 0) {
+        float xr = d.qzr * er - d.qzi * ei, xi = d.qzr * ei + d.qzi * er;
+        if (fp > 1) {
+          const float yr = d.qzr * xr - d.qzi * xi, yi = d.qzr * xi + d.qzi * xr;
+          xr = yr;
+          xi = yi;
+        }
+        ar = xr;
+        ai = xi;
+      }
+      P1 = mfma32a(ar, w.x, P1);
+      P2 = mfma32a(ai, w.y, P2);
+      P3 = mfma32a(ar + ai, w.x + w.y, P3);
     }
-  }
+    // each chunk's FP32 sums (64 z-rows) are added to the unit's FP32 sums
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    C1[k] += P1[k];
-    C2[k] += P2[k];
-    C3[k] += P3[k];
+    for (int k = 0; k < 16; ++k) {
+      C1[k] += P1[k];
+      C2[k] += P2[k];
+      C3[k] += P3[k];
+    }
+    if (more) {
+      // buffer cur ^ 1 was last read in chunk ck - 1, which every wave finished before the
+      // barrier that ended it
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s_w[cur ^ 1][(tid + 256 * e) / kAdjTile][(tid + 256 * e) % kAdjTile] = wr[e];
+      if (tid < kAdjZChunk) s_z[cur ^ 1][tid] = dr;
+    }
+    __syncthreads();
   }
   // X[t][y] for this lane's column y and 16 triangle rows; contract with g(y) Ey_t(y) over the 32
   // columns of the wave's half
-  const int yv = yt * kAdjTile + r;
+  const int yv = y0 + r;
   const bool yok = yv < ncol;
+  const int yc = min(yv, ncol - 1);
   const double qx0 = it.qx[0], qyv = it.qy[yc];
   const int nform = 3 - fp;
   const int form0 = fp == 0 ? 0 : (fp == 1 ? 3 : 5);
@@ -294,8 +339,8 @@ __global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq) {
   if (nunits <= 0) return;
-  if (cq) k_adjoint_product32<true><<<ceil_div(nunits, 4), 256, 0, s>>>(d_items, single, upi, nunits);
-  else k_adjoint_product32<false><<<ceil_div(nunits, 4), 256, 0, s>>>(d_items, single, upi, nunits);
+  if (cq) k_adjoint_product32<true><<<nunits, 256, 0, s>>>(d_items, single, upi);
+  else k_adjoint_product32<false><<<nunits, 256, 0, s>>>(d_items, single, upi);
 }
 
 }  // namespace hgx
@@ -325,17 +370,22 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   const int64_t qchunk = 4096;
   const int nqc = (int)ceil_div(c->nq, qchunk);
   const size_t rec_b = (size_t)m.ntri * 8 * sizeof(double);
+  const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
+                                           R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
+  const size_t r32_b = (precision == HGX_PREC_FP32C && !identity)
+                           ? (size_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double)) : 0;
   const size_t w0_b = 256;
   const size_t wt_b = (size_t)c->nq * 2 * (precision == HGX_PREC_FP64 ? sizeof(double) : sizeof(float));
   const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
                                                    : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + part_b)) return rc;
+  if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + al(part_b) + r32_b)) return rc;
   char* w = static_cast<char*>(c->d_work);
   double* rrec = reinterpret_cast<double*>(w);
   double* w0 = reinterpret_cast<double*>(w + al(rec_b));
   void* wt = w + al(rec_b) + w0_b;
   double* part = reinterpret_cast<double*>(w + al(rec_b) + w0_b + al(wt_b));
+  char* r32 = w + al(rec_b) + w0_b + al(wt_b) + al(part_b);
   k_adj_rotate<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(m.d_rec64, m.ntri, R, rrec);
   HGX_CHECK_LAUNCH(c);
   k_adj_q0<<<1, 256, 0, c->stream>>>(g, W, w0);
@@ -366,13 +416,23 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   it.qz = c->d_qz;
   it.wt = static_cast<const float*>(wt);
   it.rec = rrec;
+  if (identity) {
+    it.rec32 = m.d_rec32;
+    it.borig = m.d_borig;
+  } else {
+    float* rr = reinterpret_cast<float*>(r32);
+    double* ro = reinterpret_cast<double*>(r32 + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
+    if (int rc = product_rotate_mesh(c, m, R, rr, ro, c->stream)) return rc;
+    it.rec32 = rr;
+    it.borig = ro;
+  }
   it.part = part;
   it.ntri = m.ntri;
+  it.nblocks = (int)m.nblocks;
   it.ncol = c->ncol;
   it.nz = c->nbranch * c->nrow;
   it.nyt = nyt;
-  it.ntt = (int)ceil_div(m.ntri, kAdjTile);
-  const double units = 3.0 * it.nyt * it.ntt;
+  const double units = 3.0 * it.nyt * it.nblocks;
   if (units > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
   {
     LaunchTimer lt(c, true);

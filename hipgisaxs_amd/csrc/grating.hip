@@ -707,7 +707,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   p.nyt = (int)ceil_div(c->ncol, 32);
   p.ntt = (int)(gm.nslots / 32);
   p.upi_fwd = nzt * nyt_f * p.nchunks;
-  p.upi_adj = 3 * p.ntt * p.nyt;
+  p.upi_adj = 3 * gm.nblocks * p.nyt;
   p.npb = (int)ceil_div(c->npix, 256);
   p.ngb = (int)ceil_div(gm.nslots, 256);
   // group size: per-candidate device memory within a 24 GiB budget
@@ -758,12 +758,14 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
     a.qz = c->d_qz;
     a.wt = p.d_wt + (int64_t)i * c->nq * 2;
     a.rec = p.d_rec64 + (int64_t)i * gm.nslots * 8;
+    a.rec32 = q.rec;
+    a.borig = q.borig;
     a.part = p.d_apart + (int64_t)i * 6 * p.nyt * gm.nslots * 2;
     a.ntri = gm.nslots;
+    a.nblocks = gm.nblocks;
     a.ncol = c->ncol;
     a.nz = nz;
     a.nyt = p.nyt;
-    a.ntt = p.ntt;
   }
   // forward units, item-major so a short last group runs a prefix
   std::vector<ProductUnit> units;

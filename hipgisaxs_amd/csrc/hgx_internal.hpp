@@ -112,14 +112,17 @@ struct ProductUnit {
 
 // One mesh for the product-grid adjoint kernel (adjoint.hip).
 struct AdjItem {
-  const double* qx;  // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
+  const double* qx;     // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
   const double* qy;
   const double* qz;
-  const float* wt;   // [nq] complex Wt = -i W / |q|^2
-  const double* rec; // [ntri][8] rotated FP64 records (A', c')
-  double* part;      // [6 forms][nyt][ntri] complex: B11, B1y, B1yy, Bz1, Bzy, Bzz per y-tile
+  const float* wt;      // [nq] complex Wt = -i W / |q|^2
+  const double* rec;    // [ntri][8] rotated FP64 records (A', c')
+  const float* rec32;   // [nblocks * kBlockTri][8] rotated FP32 records (A', c' - o_b)
+  const double* borig;  // [nblocks][4] rotated block origins
+  double* part;         // [6 forms][nyt][ntri] complex: B11, B1y, B1yy, Bz1, Bzy, Bzz per y-tile
   int64_t ntri;
-  int ncol, nz, nyt, ntt;
+  int nblocks;
+  int ncol, nz, nyt;
 };
 
 constexpr int kProductTile = 32;
@@ -282,7 +285,7 @@ void product_launch_units(hipStream_t s, const ProductItem* d_items, const Produ
 // Launches every (z-tile, y-tile, chunk) unit of a single item.
 void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, bool gauss);
 // Launches the product-grid adjoint over items (d_items == NULL: the single item by value);
-// upi units per item (3 f-passes x triangle tiles x y-tiles).
+// upi workgroup units per item (3 f-passes x triangle blocks x y-tiles).
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq);
 // Tuning: triangle blocks per work unit and 3- or 4-multiplication complex products.
