@@ -1,4 +1,5 @@
-// ff_analytic.hip -- analytic form factors (FP64), one thread per q-point, node loops in registers.
+// ff_analytic.hip -- analytic form factors (FP64 for either precision argument), one thread per
+// q-point, node loops in registers.
 // Formulas are the CPU ones of the reference (src/ff/ff_ana_<shape>.cpp), which the reference's
 // GPU kernels do not all follow (src/ff/gpu/ff_ana_prism_gpu.cu:39-57, ff_ana_prism6_gpu.cu:47).
 #include <cmath>

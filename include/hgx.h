@@ -182,7 +182,9 @@ int hgx_ff_numeric_adjoint(hgx_ctx* ctx, int mesh_id, int precision, int flags, 
                            const double* W, double* grad);
 
 /* Analytic form factor, replaces AnalyticFormFactorG::compute_<shape>
- * (include/ff/gpu/ff_ana_gpu.cuh:54-94) with the CPU formulas of src/ff/ff_ana_<shape>.cpp. */
+ * (include/ff/gpu/ff_ana_gpu.cuh:54-94) with the CPU formulas of src/ff/ff_ana_<shape>.cpp.
+ * Evaluated in FP64 for either precision value (HGX_PREC_FP32C is accepted for interface symmetry);
+ * the FP32C analytic path is the fused grain-ensemble kernel, hgx_lattice_intensity. */
 int hgx_ff_analytic(hgx_ctx* ctx, int shape, const hgx_shape_params* params, int precision,
                     const double rot[9], const double transvec[3], double* ff);
 
