@@ -56,39 +56,71 @@ def make_workload(nshapes=64, seed=SEED + 3):
     return shapes
 
 
-def cpu_baseline(shapes, budget_s=12.0):
-    """Times the FP64 OpenMP oracle on a bounded sample: one mid-size shape, a band of q-rows."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(shapes, run_s=4.0, repeats=3):
+    """Times the FP64 OpenMP oracle on a bounded sample of the workload: one mid-size shape, a band of its
+    q-grid rows sized to about run_s per run, median of `repeats` runs; then the same loop as the reference
+    writes it (a heap-allocated rotation per pair) on a band sized to about run_s, once."""
     import oracle as O
 
     s = sorted(shapes, key=lambda d: d["tri"].shape[0])[len(shapes) // 2]
     rec = O.tri9_to_rec8(s["tri"])
     side = s["side"]
     g = O.qgrid_explicit(side, side, -2.0, 0.0, 2.0, 2.0, 0.0, 1e3)
-    g = O.Grid(side, side, g.qx, g.qy, g.qz.real + 0j, None)
-    # calibrate on a few rows, then size the sample to about budget_s
-    rows = 2
-    while True:
-        sub = O.Grid(side, rows, g.qx[: rows * side], g.qy[: rows * side], g.qz[: rows * side])
+
+    def band(rows):
+        return O.Grid(side, rows, g.qx[: rows * side], g.qy[: rows * side], g.qz[: rows * side] + 0j)
+
+    def calibrate(fn):
+        rows = 2
+        while True:
+            t0 = time.perf_counter()
+            fn(rec, band(rows))
+            dt = time.perf_counter() - t0
+            if dt > 0.3 or rows >= side:
+                return int(min(side, max(1, rows * run_s / max(dt, 1e-3))))
+            rows = min(side, rows * 4)
+
+    rows = calibrate(O.ff_approx)
+    sub = band(rows)
+    times = []
+    for _ in range(repeats):
         t0 = time.perf_counter()
-        O.ff_approx(rec, sub)
-        dt = time.perf_counter() - t0
-        if dt > 0.5 or rows >= side:
-            break
-        rows = min(side, rows * 4)
-    rows = int(min(side, max(1, rows * budget_s / max(dt, 1e-3))))
-    sub = O.Grid(side, rows, g.qx[: rows * side], g.qy[: rows * side], g.qz[: rows * side])
-    t0 = time.perf_counter()
-    out = O.ff_approx(rec, sub)
-    dt = time.perf_counter() - t0
+        out = O.ff_approx(rec, sub)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     pairs = rows * side * rec.shape[0]
+    rows_w = calibrate(O.ff_approx_as_written)
+    t0 = time.perf_counter()
+    O.ff_approx_as_written(rec, band(rows_w))
+    dt_w = time.perf_counter() - t0
+    pairs_w = rows_w * side * rec.shape[0]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": pairs / dt,
         "unit": "q-pt·triangles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"FP64 OpenMP oracle, approximate method, shape with {rec.shape[0]} triangles x "
-                  f"{rows} rows of its {side}x{side} grid ({pairs:.3e} pairs, {dt:.1f} s)",
+        "sample": f"FP64 OpenMP oracle (rotation hoisted per q), approximate method, shape with {rec.shape[0]} "
+                  f"triangles x {rows} rows of its {side}x{side} grid ({pairs:.3e} pairs), median of {repeats} "
+                  f"runs of {dt:.1f} s",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "reference_as_written": {
+            "value": pairs_w / dt_w,
+            "sample": f"same shape, {rows_w} rows ({pairs_w:.3e} pairs, {dt_w:.1f} s), one heap-allocated "
+                      f"rotation per pair as src/ff/cpu/ff_tri_cpu.cpp:233",
+        },
     }, (s, rows, out)
 
 
@@ -96,6 +128,48 @@ def _det_grid(ctx, side, pixel_mm, alpha_i, k0, dwba=True):
     ctx.qgrid_detector(side, side, pixel_mm, 2000.0, (side - 1) / 2.0, 0.0, alpha_i, k0)
     if dwba:
         ctx.qz_extended(k0, alpha_i, 0j)
+
+
+def bench_pyramid_fp64(reps=5):
+    """The plain-method configuration at full size: the 2048-triangle square pyramid (base 20 nm, height
+    10 nm) on the 256^2 q-space grid qy [-2, 2], qz [0, 2] with its four DWBA branches (10 keV,
+    0.2 deg, Si substrate), FP64 approximate-triangle form factor and DWBA intensity, on the GPU and with
+    the CPU oracle on the GPU box's host cores, and the relL2 between them (bar 1e-10)."""
+    import torch
+    import oracle as O
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Context
+
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    tri = meshes.pyramid()
+    g = O.qz_extended(O.qgrid_explicit(256, 256, -2.0, 0.0, 2.0, 2.0, ai, k0), k0, ai)
+    fc = O.fresnel(g, [4.88e-6], [7.37e-8], [0.0], 0, k0, ai)
+    ctx = Context(0)
+    ctx.qgrid_explicit(256, 256, -2.0, 0.0, 2.0, 2.0, ai, k0)
+    ctx.qz_extended(k0, ai, 0j)
+    mid = ctx.upload_mesh(tri, "exact9")
+    fcd = torch.from_numpy(np.stack([fc.real, fc.imag], -1).ravel()).cuda()
+    ff = ctx.ff_numeric(mid, "approx", "fp64")
+    I = ctx.dwba_intensity(ff, fc=fcd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ff_numeric(mid, "approx", "fp64", out=ff)
+        ctx.dwba_intensity(ff, fc=fcd, out=I)
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    F_ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+    I_ref = O.dwba_intensity(g.npix, 4, fc, None, F_ref)
+    t_cpu = time.perf_counter() - t0
+    got = I.cpu().numpy()
+    pairs = 4 * 256 * 256 * tri.shape[0]
+    ctx.close()
+    return {"gpu_ms": 1e3 * t_gpu, "cpu_ms": 1e3 * t_cpu, "gpu_pairs_per_s": pairs / t_gpu,
+            "cpu_pairs_per_s": pairs / t_cpu, "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "relL2_intensity_gpu_vs_oracle": float(np.linalg.norm(got - I_ref) / np.linalg.norm(I_ref)),
+            "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA"}
 
 
 def bench_lattice(reps=5):
@@ -433,6 +507,8 @@ def main():
         result["fp32c_per_pair_valu"] = {"value": pairs * 2 / wall_v, "ms_per_step": 1e3 * wall_v / 2}
         bv.release()
     if not args.no_extra:
+        result["pyramid_fp64"] = bench_pyramid_fp64()
+        print(f"[bench] pyramid done", file=sys.stderr, flush=True)
         result["lattice_spheres_bcc"] = bench_lattice()
         print(f"[bench] lattice done", file=sys.stderr, flush=True)
         result["chi2_cylinders_hcp"] = bench_fit()

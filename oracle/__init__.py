@@ -31,6 +31,7 @@ def load():
             "hgxo_ff_approx": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_exact": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i, _d, _vp]),
             "hgxo_ff_approx_adjoint": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "hgxo_ff_approx_as_written": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_param_distribution": (_i, [_d, _d, _i, _i, _d, _d, _vp, _vp, _i]),
             "hgxo_ff_sphere": (None, [_i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_cylinder": (None, [_i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -142,6 +143,16 @@ def ff_approx(rec8, g: Grid, rot=None) -> np.ndarray:
     qz = _c2r(g.qz)
     out = np.empty(2 * nq)
     load().hgxo_ff_approx(r.shape[0], _p(r), npix, nq, qx, qy, _p(qz), _p(R), _p(out))
+    return _r2c(out)
+
+
+def ff_approx_as_written(rec8, g: Grid, rot=None) -> np.ndarray:
+    """ff_approx with the reference's per-pair rotation into a heap-allocated vector (timing baseline)."""
+    r = _a(rec8).reshape(-1, 8)
+    npix, nq, qx, qy, _, R = _qargs(g, rot)
+    qz = _c2r(g.qz)
+    out = np.empty(2 * nq)
+    load().hgxo_ff_approx_as_written(r.shape[0], _p(r), npix, nq, qx, qy, _p(qz), _p(R), _p(out))
     return _r2c(out)
 
 

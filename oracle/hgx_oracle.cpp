@@ -252,6 +252,41 @@ void hgxo_ff_approx(int64_t ntri, const double* rec, int64_t npix, int64_t nq, c
   }
 }
 
+// The same sum evaluated the way the reference's CPU loop is written (src/ff/cpu/ff_tri_cpu.cpp:219-242):
+// the q-point is rotated again for every (q, triangle) pair into a freshly allocated std::vector
+// (:233), i.e. one heap allocation per pair. Timed beside hgxo_ff_approx as the "reference as written"
+// CPU baseline; its values equal hgxo_ff_approx's (same arithmetic, q = 0 handled the same way).
+void hgxo_ff_approx_as_written(int64_t ntri, const double* rec, int64_t npix, int64_t nq, const double* qx,
+                               const double* qy, const double* qz, const double* rot, double* ff) {
+  double vol0 = 0.0;
+  for (int64_t t = 0; t < ntri; ++t) {
+    const double* r = rec + 8 * t;
+    vol0 += r[0] * (r[1] * r[4] + r[2] * r[5] + r[3] * r[6]);
+  }
+  vol0 /= 3.0;
+  const cplx nj(0.0, -1.0), np(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t iz = 0; iz < nq; ++iz) {
+    int64_t iy = iz % npix;
+    cplx acc = 0.0;
+    bool zero = false;
+    for (int64_t t = 0; t < ntri && !zero; ++t) {
+      const double* r = rec + 8 * t;
+      std::vector<cplx> mq(3);
+      rotate_q(rot, qx[iy], qy[iy], qz_at(qz, iz), mq.data());
+      double q2 = std::norm(mq[0]) + std::norm(mq[1]) + std::norm(mq[2]);
+      if (q2 == 0.0) {
+        zero = true;
+        break;
+      }
+      cplx qn = mq[0] * r[1] + mq[1] * r[2] + mq[2] * r[3];
+      cplx qt = mq[0] * r[4] + mq[1] * r[5] + mq[2] * r[6];
+      acc += (nj * qn * r[0] * std::exp(np * qt) / q2);
+    }
+    store(ff, iz, zero ? cplx(vol0, 0.0) : acc);
+  }
+}
+
 // Record adjoint of the approximate form factor (no reference counterpart: the reference has
 // only finite-difference gradients, src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). For
 // L = Re sum_q W(q) F(q), with F as in hgxo_ff_approx and A_t = s_t n_t:

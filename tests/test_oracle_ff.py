@@ -134,3 +134,18 @@ def test_conjugate_symmetry_rotation_translation(oracle_mod, method):
     d = np.array([1.5, -2.0, 0.7])
     tt = (t.reshape(-1, 3, 3) + d).reshape(-1, 9)
     assert rel_l2(f(tt, g), f(t, g) * np.exp(1j * (qx * d[0] + qy * d[1] + qz * d[2]))) < 1e-10
+
+
+def test_as_written_variant_equals_oracle(oracle_mod):
+    """The per-pair-allocation timing variant computes the same values as the oracle."""
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    rng = np.random.default_rng(9)
+    tri = meshes.convex_polyhedron(rng, np.array([5.0, 4.0, 3.0]), 200)
+    rec = O.tri9_to_rec8(tri)
+    g = O.qz_extended(O.qgrid_explicit(9, 7, -1.0, 0.0, 1.0, 1.0, 0.003, 50.0), 50.0, 0.003, 1e-6 + 1e-8j)
+    R = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    a = O.ff_approx(rec, g, R)
+    b = O.ff_approx_as_written(rec, g, R)
+    assert np.array_equal(a, b)
