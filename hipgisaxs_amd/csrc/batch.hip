@@ -309,6 +309,14 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     free_batch_state(b);
     return set_error(c, HGX_ERR_OOM, "batch_create: allocation of %s failed", what);
   };
+  auto h2d = [](void* dst, const void* src, size_t n) {
+    return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  auto copy_fail = [&]() {
+    hipError_t e = hipGetLastError();
+    free_batch_state(b);
+    return set_error(c, HGX_ERR_DEVICE, "batch_create: copy to the device failed: %s", hipGetErrorString(e));
+  };
   if (part_total > 0 && hipMalloc(&b.d_part, part_total * sizeof(double)) != hipSuccess) return fail("partials");
   for (int i = 0; i < n; ++i)
     if (part_off[i] >= 0) it[i].part = pit[i].part = b.d_part + part_off[i];
@@ -332,18 +340,18 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   if (b.npunits > 0) {
     if (hipMalloc(&b.d_pitems, pit.size() * sizeof(ProductItem)) != hipSuccess) return fail("product items");
     if (hipMalloc(&b.d_punits, punits.size() * sizeof(ProductUnit)) != hipSuccess) return fail("product units");
-    HGX_CHECK_HIP(c, hipMemcpy(b.d_pitems, pit.data(), pit.size() * sizeof(ProductItem), hipMemcpyHostToDevice));
-    HGX_CHECK_HIP(c, hipMemcpy(b.d_punits, punits.data(), punits.size() * sizeof(ProductUnit), hipMemcpyHostToDevice));
+    if (!h2d(b.d_pitems, pit.data(), pit.size() * sizeof(ProductItem)) ||
+        !h2d(b.d_punits, punits.data(), punits.size() * sizeof(ProductUnit)))
+      return copy_fail();
   }
   if (hipMalloc(&b.d_items, it.size() * sizeof(BatchItemDev)) != hipSuccess) return fail("items");
   if (!units.empty() && hipMalloc(&b.d_units, units.size() * sizeof(BatchUnit)) != hipSuccess) return fail("units");
   if (!fin.empty() && hipMalloc(&b.d_finish, fin.size() * sizeof(FinishUnit)) != hipSuccess) return fail("finish");
-  HGX_CHECK_HIP(c, hipMemcpy(b.d_items, it.data(), it.size() * sizeof(BatchItemDev), hipMemcpyHostToDevice));
-  if (!units.empty())
-    HGX_CHECK_HIP(c, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(BatchUnit), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));  // rotated records ready before the first run
-  if (!fin.empty())
-    HGX_CHECK_HIP(c, hipMemcpy(b.d_finish, fin.data(), fin.size() * sizeof(FinishUnit), hipMemcpyHostToDevice));
+  if (!h2d(b.d_items, it.data(), it.size() * sizeof(BatchItemDev)) ||
+      (!units.empty() && !h2d(b.d_units, units.data(), units.size() * sizeof(BatchUnit))) ||
+      (!fin.empty() && !h2d(b.d_finish, fin.data(), fin.size() * sizeof(FinishUnit))) ||
+      hipStreamSynchronize(c->stream) != hipSuccess)  // rotated records ready before the first run
+    return copy_fail();
   b.live = true;
   auto& B = batches(c);
   for (size_t i = 0; i < B.size(); ++i)

@@ -777,11 +777,15 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   if (hipMalloc(&p.d_pitems, sizeof(ProductItem) * G) != hipSuccess) return fail("items");
   if (hipMalloc(&p.d_punits, sizeof(ProductUnit) * units.size()) != hipSuccess) return fail("units");
   if (hipMalloc(&p.d_aitems, sizeof(AdjItem) * G) != hipSuccess) return fail("adjoint items");
-  HGX_CHECK_HIP(c, hipMemcpy(p.d_pitems, pit.data(), sizeof(ProductItem) * G, hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(p.d_punits, units.data(), sizeof(ProductUnit) * units.size(), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(p.d_aitems, ait.data(), sizeof(AdjItem) * G, hipMemcpyHostToDevice));
   std::vector<GState> st(ncand);
-  HGX_CHECK_HIP(c, hipMemcpy(p.d_state, st.data(), sizeof(GState) * ncand, hipMemcpyHostToDevice));
+  if (hipMemcpy(p.d_pitems, pit.data(), sizeof(ProductItem) * G, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p.d_punits, units.data(), sizeof(ProductUnit) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p.d_aitems, ait.data(), sizeof(AdjItem) * G, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p.d_state, st.data(), sizeof(GState) * ncand, hipMemcpyHostToDevice) != hipSuccess) {
+    hipError_t e = hipGetLastError();
+    free_plan(p);
+    return set_error(c, HGX_ERR_DEVICE, "grating_plan_create: copy to the device failed: %s", hipGetErrorString(e));
+  }
   p.live = true;
   auto& P = gplans(c);
   for (size_t i = 0; i < P.size(); ++i)

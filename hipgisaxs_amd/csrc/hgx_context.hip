@@ -679,13 +679,19 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   if (hipMalloc(&m.d_moments, M.size() * sizeof(double)) != hipSuccess) return fail("moments");
   if (hipMalloc(&m.d_src, src.size() * sizeof(int64_t)) != hipSuccess) return fail("index map");
   if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_tri9, tri9s.size() * sizeof(double)) != hipSuccess) return fail("tri9");
-  HGX_CHECK_HIP(c, hipMemcpy(m.d_rec64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(m.d_rec32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(m.d_borig, borig.data(), borig.size() * sizeof(double), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(m.d_moments, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
-  HGX_CHECK_HIP(c, hipMemcpy(m.d_src, src.data(), src.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  if (kind == HGX_MESH_EXACT9)
-    HGX_CHECK_HIP(c, hipMemcpy(m.d_tri9, tri9s.data(), tri9s.size() * sizeof(double), hipMemcpyHostToDevice));
+  auto h2d = [](void* dst, const void* src, size_t n) {
+    return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!h2d(m.d_rec64, rec64.data(), rec64.size() * sizeof(double)) ||
+      !h2d(m.d_rec32, rec32.data(), rec32.size() * sizeof(float)) ||
+      !h2d(m.d_borig, borig.data(), borig.size() * sizeof(double)) ||
+      !h2d(m.d_moments, M.data(), M.size() * sizeof(double)) ||
+      !h2d(m.d_src, src.data(), src.size() * sizeof(int64_t)) ||
+      (kind == HGX_MESH_EXACT9 && !h2d(m.d_tri9, tri9s.data(), tri9s.size() * sizeof(double)))) {
+    hipError_t e = hipGetLastError();
+    free_mesh(m);
+    return set_error(c, HGX_ERR_DEVICE, "mesh_upload: copy to the device failed: %s", hipGetErrorString(e));
+  }
   m.live = true;
   int id = -1;
   for (size_t i = 0; i < c->meshes.size(); ++i)
