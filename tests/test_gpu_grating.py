@@ -212,3 +212,47 @@ def test_grating_lbfgs_recovers_parameters(oracle_mod):
         plan.release()
     finally:
         c.close()
+
+
+def test_grating_full_size_image_matches_fp64_path():
+    """The grating workload at full size (106 912 triangles, 1024^2 grid, four DWBA branches): the plan's
+    FP32C model image against the plain FP64 GPU path on the same mesh built on the host."""
+    import math
+
+    import torch
+
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Context, GratingPlan
+
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    model = dict(length=100.0, pitch=100.0, periods=200, n_bottom=64, n_side=40, n_top=64, n_length=256)
+    dn2 = 2e-5 + 1e-7j
+    th = np.array([20.0, math.radians(82.0), 40.0, 3.0])
+    c = Context(0)
+    try:
+        c.qgrid_explicit(1024, 1024, -2.0, 0.0, 2.0, 2.0, ai, k0)
+        c.qz_extended(k0, ai, 0j)
+        fc = c.fresnel([4.88e-6], [7.37e-8], [0.0], 0, k0, ai)
+        zero = torch.zeros(1024 * 1024, dtype=torch.float64, device="cuda")
+        plan = GratingPlan(c, 1, zero, fc=fc, dn2=dn2, **model)
+        plan.set_theta(th[None, :])
+        I32 = plan.intensity(0).cpu().numpy()
+        plan.release()
+        tri = meshes.trapezoid_line(th[0], th[1], th[2], model["length"], model["n_bottom"], model["n_side"],
+                                    model["n_top"], model["n_length"])
+        mid = c.upload_mesh(tri, "exact9")
+        F = c.as_complex(c.ff_numeric(mid, "approx", "fp64")).reshape(4, -1)
+        A = (dn2 * (c.as_complex(fc).reshape(4, -1) * F).sum(0)).cpu().numpy()
+        qx, qy, qz, _ = c.qgrid_get()
+        sf = _dirichlet(qy * model["pitch"], model["periods"])
+        I64 = th[3] * sf ** 2 * np.abs(A) ** 2
+        wb = th[0] + 2 * th[2] / np.tan(th[1])
+        R = np.sqrt(model["length"] ** 2 / 4 + wb ** 2 / 4 + th[2] ** 2 / 4)
+        qn = np.sqrt(qx[None] ** 2 + qy[None] ** 2 + np.abs(qz.reshape(4, -1)) ** 2)
+        keep = ~(qn * R < 0.1).any(0)
+        e = rel_l2(I32[keep], I64[keep])
+        print(f"grating 1024^2 x {len(tri)}: relL2 {e:.2e}")
+        assert e <= 1e-5
+    finally:
+        c.close()
