@@ -32,14 +32,18 @@ PEAK_FP64 = 78.6e12
 FLOP_PER_PAIR = 33  # approximate-triangle pair, direct form: q.n 10 + q.c 10 + phase 2 + scale 3 + cmul 6 + acc 2
 # Product-grid form: two complex multiply-adds per (q, triangle) pair, each done as 3 (Gauss) or 4
 # real multiply-adds of 2 flop on the matrix cores: 12 or 16 flop per pair.
-def flop_per_pair_product(mults):
-    return 2 * mults * 2
+def flop_per_pair_product(mults, operands="fp32"):
+    # split fp16: each real product is three fp16 products (high x high, low x high, high x low)
+    return 2 * mults * 2 * (3 if operands == "f16x2" else 1)
+
+
+PEAK_FP16 = 2.5e15  # MI355X dense fp16 matrix-core peak
 
 
 # HBM bytes per launch of k_ff_product32 on this workload at the default shape (16 blocks per unit,
 # Gauss form), from rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
 # device copy) in KiB per dispatch: (94933.4 x 2 + 2218240.0) x 1024.
-PRODUCT_TRAFFIC_BYTES = {(16, 3): (94933.4 * 2 + 2218240.0) * 1024}
+PRODUCT_TRAFFIC_BYTES = {(16, 3, "fp32"): (94933.4 * 2 + 2218240.0) * 1024}
 
 
 def make_workload(nshapes=64, seed=SEED + 3):
@@ -329,6 +333,8 @@ def main():
     ap.add_argument("--mode", choices=("batch", "per-shape"), default="batch",
                     help="batch: one launch for all shapes (hipGraph replay); per-shape: one call per shape")
     ap.add_argument("--no-tune", action="store_true", help="skip the work-unit autotuning")
+    ap.add_argument("--operands", choices=("fp32", "f16x2"), default=None,
+                    help="product-grid operand precision when not autotuning (default: the library's)")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
     ap.add_argument("--no-mfma", action="store_true",
                     help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
@@ -377,15 +383,21 @@ def main():
     # matrix cores unless --no-mfma
     product = all(c.is_product() for c in grid_ctx.values()) and not args.no_mfma
     tuned = {}
-    mults, bpu_p = 3, 16
+    from hipgisaxs_amd.api import tune_product
+    from hipgisaxs_amd.autotune import PRODUCT_DEFAULT
+
+    bpu_p, mults, operands = PRODUCT_DEFAULT
+    if args.operands is not None:
+        mults, operands = (4, "f16x2") if args.operands == "f16x2" else (3, "fp32")
+    tune_product(bpu_p, mults, operands)
     if args.mode == "batch" and not args.no_tune:
         from hipgisaxs_amd.autotune import autotune_batch, autotune_product
 
         if product:
             best, times = autotune_product(main_ctx, items["fp32c"], flags)
-            bpu_p, mults = best
-            tuned = {"blocks_per_unit": best[0], "complex_mults": best[1],
-                     "candidates_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in times.items()}}
+            bpu_p, mults, operands = best
+            tuned = {"blocks_per_unit": best[0], "complex_mults": best[1], "operands": best[2],
+                     "candidates_ms": {f"{k[0]}x{k[1]}x{k[2]}": round(v, 3) for k, v in times.items()}}
         else:
             best, times = autotune_batch(main_ctx, items["fp32c"], "fp32c", flags)
             tuned = {"qpt": best[0], "blocks_per_unit": best[1],
@@ -444,8 +456,10 @@ def main():
     value64 = pairs * args.fp64_steps / wall64
     avg_launch_ms = k32_ms / max(n32, 1)
     pairs_per_launch = pairs * args.steps / max(n32, 1)
-    fpp = flop_per_pair_product(mults) if product else FLOP_PER_PAIR
+    fpp = flop_per_pair_product(mults, operands) if product else FLOP_PER_PAIR
     achieved = fpp * pairs_per_launch / (avg_launch_ms * 1e-3)
+    split16 = product and operands == "f16x2"
+    peak = PEAK_FP16 if split16 else PEAK_FP32
     result = {
         "metric": METRIC,
         "value": value32,
@@ -457,7 +471,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "f16x2-split/fp32-acc" if split16 else "fp32",
         "data": "synthetic",
         "config": {
             "workload": "batched numeric FF: 64 seeded convex polyhedra (ellipsoid semi-axes U[3,15] nm), "
@@ -477,11 +491,12 @@ def main():
         "relL2_fp32c_vs_fp64_gpu_worst_shape": worst,
         "roofline": {
             "bound": "mfma" if product else "valu",
-            "kernel": "k_ff_product32" if product else ("k_batch32" if args.mode == "batch" else "k_ff_approx32"),
+            "kernel": ("k_ff_product16" if split16 else "k_ff_product32") if product else
+                      ("k_batch32" if args.mode == "batch" else "k_ff_approx32"),
             "achieved": achieved / 1e12,
-            "peak": PEAK_FP32 / 1e12,
+            "peak": peak / 1e12,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32,
+            "frac": achieved / peak,
             "traffic": None,
             "flop_per_pair": fpp,
             "avg_launch_ms": avg_launch_ms,
@@ -495,7 +510,7 @@ def main():
         # direct per-pair form (33 flop): rates of equivalent work, not of matrix-core issue
         result["roofline"]["complex_form_tflops"] = 16 * rate / 1e12
         result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * rate / 1e12
-        shape_key = (bpu_p, mults)
+        shape_key = (bpu_p, mults, operands)
         if len(shapes) == 64 and not args.kahan and shape_key in PRODUCT_TRAFFIC_BYTES:
             result["roofline"]["traffic"] = PRODUCT_TRAFFIC_BYTES[shape_key]
     if b32 is not None:

@@ -313,10 +313,16 @@ def tune_batch(qpt: int, blocks_per_unit: int) -> None:
         raise HgxError("tune_batch: invalid knob values")
 
 
-def tune_product(blocks_per_unit: int, complex_mults: int = 3) -> None:
-    """Work-unit size (128-triangle blocks) of the product-grid matrix-core kernel and 3 or 4 real
-    products per complex product."""
-    if N.load().hgx_tune_product(blocks_per_unit, complex_mults) != N.HGX_OK:
+# the library's own product-kernel setting (hgx_tune_product's values at load)
+PRODUCT_DEFAULT = (16, 4, "f16x2")
+
+
+def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "fp32") -> None:
+    """Work-unit size (128-triangle blocks) of the product-grid matrix-core kernel, 3 or 4 real
+    products per complex product, and the operand precision: "fp32" or "f16x2" (each operand split
+    into two fp16 halves; 4 real products only)."""
+    ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
+    if operands not in ops or N.load().hgx_tune_product(blocks_per_unit, complex_mults, ops[operands]) != N.HGX_OK:
         raise HgxError("tune_product: invalid knob values")
 
 

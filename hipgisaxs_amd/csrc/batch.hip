@@ -142,7 +142,7 @@ void launch_batch(const Batch& b, hipStream_t s) {
 void launch_main(const Batch& b, hipStream_t s) {
   if (b.npunits > 0)
     product_launch_units(s, static_cast<const ProductItem*>(b.d_pitems), static_cast<const ProductUnit*>(b.d_punits),
-                         b.npunits, b.cq, b.gauss);
+                         b.npunits, b.cq, b.pmode);
   if (b.nunits == 0) return;
   switch (b.qpt) {
     case 1: launch_batch<1>(b, s); break;
@@ -191,7 +191,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   b.precision = precision;
   b.flags = flags;
   b.qpt = g_batch_qpt;
-  b.gauss = product_gauss();
+  b.pmode = product_mode();
   const int64_t bpu = g_batch_bpu;
   std::vector<BatchItemDev> it(n);
   std::vector<ProductItem> pit(n);
@@ -237,6 +237,9 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
       pi.nq = gc->nq;
       pi.ncol = gc->ncol;
       pi.nz = gc->nbranch * gc->nrow;
+      pi.snmax = (float)m.snmax;
+      pi.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
+                        m.radius);
       bool identity = true;
       for (int k = 0; k < 9; ++k) identity = identity && x.rot[k] == ((k % 4 == 0) ? 1.0 : 0.0);
       if (identity) {

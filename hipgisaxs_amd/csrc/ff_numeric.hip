@@ -289,6 +289,9 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     it.nq = c->nq;
     it.ncol = c->ncol;
     it.nz = c->nbranch * c->nrow;
+    it.snmax = (float)m.snmax;
+    it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
+                      m.radius);
     const int64_t tiles = (int64_t)ceil_div(it.nz, kProductTile) * ceil_div(it.ncol, kProductTile);
     // enough waves to fill the chip several times over, without splitting below one block
     int64_t want = std::max<int64_t>(1, (int64_t)(4 * g_target_wg + tiles - 1) / tiles);
@@ -318,7 +321,7 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     }
     {
       LaunchTimer lt(c, true);
-      product_launch_single(c->stream, it, cq, product_gauss());
+      product_launch_single(c->stream, it, cq, product_mode());
     }
     HGX_CHECK_LAUNCH(c);
     {

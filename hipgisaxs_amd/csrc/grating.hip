@@ -600,7 +600,10 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   k_grating_mesh<<<gmesh, 256, 0, s>>>(gm, p.d_state, cand0, p.d_rec64, p.d_drec);
   dim3 gblk(gm.nblocks, gsize);
   k_grating_blocks<<<gblk, 128, 0, s>>>(gm, p.d_rec64, p.d_rec32, p.d_borig);
-  product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq, product_gauss());
+  // the trapezoid meshes are built on the device, so the split-fp16 operand scales (which need the
+  // largest triangle) are not known here: the grating's forward pass keeps FP32 operands
+  product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq,
+                       product_mode() == kProductF16x2 ? kProductF32Gauss : product_mode());
   dim3 gpix(p.npb, gsize);
   k_grating_pixel<<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.d_wt, p.d_cpart,
                                        want_I ? Iout : nullptr);

@@ -642,10 +642,14 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
     // A closed surface has sum_t s_t n_t = 0 (divergence theorem); what the sum holds then is
     // rounding, which the series would divide by |q| and so blow up as q -> 0. Zero it for meshes
     // closed to rounding; open meshes keep their (physical) 1/|q| term.
-    double asum = 0.0;
-    for (int64_t t = 0; t < ntri; ++t)
-      asum += std::sqrt(rec64[8 * t] * rec64[8 * t] + rec64[8 * t + 1] * rec64[8 * t + 1] +
-                        rec64[8 * t + 2] * rec64[8 * t + 2]);
+    double asum = 0.0, amax = 0.0;
+    for (int64_t t = 0; t < ntri; ++t) {
+      double a = std::sqrt(rec64[8 * t] * rec64[8 * t] + rec64[8 * t + 1] * rec64[8 * t + 1] +
+                           rec64[8 * t + 2] * rec64[8 * t + 2]);
+      asum += a;
+      amax = std::max(amax, a);
+    }
+    m.snmax = amax;
     double n0 = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
     if (n0 <= 1e-8 * asum) M[0] = M[1] = M[2] = 0.0;
   }

@@ -30,6 +30,7 @@ struct Mesh {
   double volume = 0.0;    // sum_t s_t (n_t . c_t) / 3
   double origin[3] = {0, 0, 0};
   double radius = 0.0;    // max_t |c_t - origin|
+  double snmax = 0.0;     // max_t |s_t n_t|
   // device arrays
   double* d_rec64 = nullptr;   // [ntri][8]: s*nx, s*ny, s*nz, cx, cy, cz, 0, 0 (absolute, FP64)
   float* d_rec32 = nullptr;    // [nblocks*kBlockTri][8]: s*n (3), c - block origin (3), 0, 0 (FP32)
@@ -60,7 +61,7 @@ struct BatchState {
   void* d_finish = nullptr;
   double* d_part = nullptr;
   // product-grid items (FP32C on product grids): their own unit list and rotated records
-  bool gauss = true;
+  int pmode = 0;
   int npunits = 0;
   void* d_pitems = nullptr;
   void* d_punits = nullptr;
@@ -97,6 +98,8 @@ struct ProductItem {
   int nchunks;
   int ncol;
   int nz;               // nbranch * nrow
+  float snmax;          // max_t |s_t n_t| over the mesh (split-fp16 operand scaling)
+  float rmax;           // max_t |c_t| over the mesh (bounds exp(-Im(qz) c_z) for complex qz)
   int pad;
   double* part;         // [nchunks][nq] complex partial sums (nchunks > 1)
   double* out;          // [nq] complex raw sums (nchunks == 1)
@@ -127,6 +130,11 @@ struct AdjItem {
 
 constexpr int kProductTile = 32;
 constexpr int kProductWaves = 4;  // work units (waves) per workgroup
+// product kernel variants: FP32 operands with 4 or 3 (Gauss) real products per complex product,
+// or each FP32 operand split into two fp16 halves (4 real products, 3 cross terms each)
+constexpr int kProductF32x4 = 0;
+constexpr int kProductF32Gauss = 1;
+constexpr int kProductF16x2 = 2;
 
 
 struct Context {
@@ -281,15 +289,15 @@ int product_rotate_mesh(Context* c, const Mesh& m, const Rot& R, float* rec_out,
                         hipStream_t s);
 // Launches the unit list of a batch (units sorted by the caller).
 void product_launch_units(hipStream_t s, const ProductItem* d_items, const ProductUnit* d_units, int nunits,
-                          bool cq, bool gauss);
+                          bool cq, int mode);
 // Launches every (z-tile, y-tile, chunk) unit of a single item.
-void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, bool gauss);
+void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, int mode);
 // Launches the product-grid adjoint over items (d_items == NULL: the single item by value);
 // upi workgroup units per item (3 f-passes x triangle blocks x y-tiles).
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq);
-// Tuning: triangle blocks per work unit and 3- or 4-multiplication complex products.
+// Tuning: triangle blocks per work unit and the kernel variant (kProduct* below).
 int product_blocks_per_unit();
-bool product_gauss();
+int product_mode();
 
 }  // namespace hgx

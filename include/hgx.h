@@ -348,10 +348,16 @@ int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, doub
  * tiles, include/common/parameters.hpp:37-48). */
 int hgx_tune_numeric(int qpt64, int qpt32, int target_wg);
 
+/* Operand precision of the product-grid kernel. */
+#define HGX_PRODUCT_FP32 0  /* FP32 operands on v_mfma_f32_32x32x2_f32 */
+#define HGX_PRODUCT_F16X2 1 /* each FP32 operand split into two fp16 halves on v_mfma_f32_32x32x16_f16 */
+
 /* Work-unit shape of the product-grid kernel (single calls and later batches): triangle blocks (128
- * triangles each) per unit, and 3 (Gauss) or 4 real products per complex product. Process-wide; set
- * by the autotuner. No reference counterpart. */
-int hgx_tune_product(int blocks_per_unit, int complex_mults);
+ * triangles each) per unit, 3 (Gauss) or 4 real products per complex product (HGX_PRODUCT_F16X2
+ * takes 4 only), and the operand precision. Default (16, 4, HGX_PRODUCT_F16X2). Process-wide; set by
+ * the autotuner. The grating fit's forward pass keeps FP32 operands whatever is set. No reference
+ * counterpart. */
+int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
 
 #ifdef __cplusplus
 }

@@ -16,9 +16,10 @@ AI = math.radians(0.2)
 
 def test_numeric_fp32c_full_size_grid():
     """A 50 000-triangle polyhedron on the 1024^2 Born grid (the largest shape of the batched workload):
-    matrix-core FP32C and per-pair FP32C against plain FP64, and Friedel symmetry F(-q) = conj F(q)."""
+    matrix-core FP32C (FP32 and split-fp16 operands) and per-pair FP32C against plain FP64, and
+    Friedel symmetry F(-q) = conj F(q)."""
     from hipgisaxs_amd import meshes
-    from hipgisaxs_amd.api import Context
+    from hipgisaxs_amd.api import PRODUCT_DEFAULT, Context, tune_product
 
     rng = np.random.default_rng(20261203)
     tri = meshes.convex_polyhedron(rng, np.array([15.0, 11.0, 7.0]), 50000)
@@ -27,11 +28,17 @@ def test_numeric_fp32c_full_size_grid():
         c.qgrid_explicit(1024, 1024, -2.0, 0.0, 2.0, 2.0, 0.0, 1e3)
         mid = c.upload_mesh(tri, "exact9")
         f64 = c.as_complex(c.ff_numeric(mid, "approx", "fp64")).cpu().numpy()
-        f32 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         fva = c.as_complex(c.ff_numeric(mid, "approx", "fp32c", flags=4)).cpu().numpy()
-        e_mfma, e_valu = rel_l2(f32, f64), rel_l2(fva, f64)
-        print(f"1024^2 x 50k: mfma {e_mfma:.2e} valu {e_valu:.2e}")
-        assert e_mfma <= 1e-5 and e_valu <= 1e-5
+        try:
+            tune_product(16, 3, "fp32")
+            f32 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+            tune_product(16, 4, "f16x2")
+            f16 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+        finally:
+            tune_product(*PRODUCT_DEFAULT)
+        e_mfma, e_valu, e_16 = rel_l2(f32, f64), rel_l2(fva, f64), rel_l2(f16, f64)
+        print(f"1024^2 x 50k: mfma {e_mfma:.2e} valu {e_valu:.2e} split-fp16 mfma {e_16:.2e}")
+        assert e_mfma <= 1e-5 and e_valu <= 1e-5 and e_16 <= 1e-5
         # Friedel: a grid symmetric in qy with qz -> -qz gives the conjugate
         c.qgrid_explicit(512, 256, -2.0, -2.0, 2.0, 0.0, 0.0, 1e3)  # qz in [-2, 0]
         lo = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy().reshape(256, 512)

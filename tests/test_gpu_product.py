@@ -2,8 +2,9 @@
 
 The kernel evaluates the approximate-triangle sum as complex matrix products; it must agree with the
 oracle to the FP32C bar on grids whose sides are not multiples of the 32 x 32 tile, on meshes smaller
-than one 128-triangle block, under rotation, on complex (DWBA, absorbing layer) branches, with either
-complex-product form, chunked or not, and with the per-pair VALU kernels (HGX_FLAG_NO_MFMA)."""
+than one 128-triangle block, under rotation, on complex (DWBA, absorbing layer) branches, with FP32
+operands (either complex-product form) and split-fp16 operands, chunked or not, and with the per-pair
+VALU kernels (HGX_FLAG_NO_MFMA)."""
 import numpy as np
 import pytest
 
@@ -62,10 +63,10 @@ def test_product_grid_detection(oracle_mod):
     (48, 20, 1500, 2e-6 + 1e-8j, True),  # complex branches (absorbing layer), nz = 4 * 20
     (37, 45, 700, 0j, False),            # real branches
 ])
-@pytest.mark.parametrize("mults", [3, 4])
-def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, mults):
+@pytest.mark.parametrize("mults,operands", [(3, "fp32"), (4, "fp32"), (4, "f16x2")])
+def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, mults, operands):
     from hipgisaxs_amd import meshes
-    from hipgisaxs_amd.api import Context, tune_product
+    from hipgisaxs_amd.api import PRODUCT_DEFAULT, Context, tune_product
 
     O = oracle_mod
     rng = np.random.default_rng(ncol * 1000 + nrow)
@@ -73,7 +74,7 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
     R = _rot(0.3, -0.7, 1.1) if rotated else None
     c = Context(0)
     try:
-        tune_product(16, mults)
+        tune_product(16, mults, operands)
         g = _grid(c, O, ncol, nrow, dwba)
         assert c.is_product()
         mid = c.upload_mesh(tri, "exact9")
@@ -92,21 +93,22 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
         ok = ~tiny
         got, valu, ref = got[ok], valu[ok], ref[ok]
         e_mfma, e_valu = rel_l2(got, ref), rel_l2(valu, ref)
-        print(f"{ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} mults={mults}: "
+        print(f"{ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} mults={mults} {operands}: "
               f"mfma {e_mfma:.2e} valu {e_valu:.2e}")
         assert np.all(np.isfinite(got))
         assert e_mfma <= 1e-5
         assert e_valu <= 1e-5
         c.release_mesh(mid)
     finally:
-        tune_product(16, 3)
+        tune_product(*PRODUCT_DEFAULT)
         c.close()
 
 
-def test_product_chunked_matches_single_chunk(oracle_mod):
+@pytest.mark.parametrize("operands", ["fp32", "f16x2"])
+def test_product_chunked_matches_single_chunk(oracle_mod, operands):
     """Many triangle chunks (partials reduced by the finishing kernel) vs one chunk."""
     from hipgisaxs_amd import meshes
-    from hipgisaxs_amd.api import Context, tune_numeric
+    from hipgisaxs_amd.api import PRODUCT_DEFAULT, Context, tune_numeric, tune_product
 
     O = oracle_mod
     rng = np.random.default_rng(3)
@@ -115,6 +117,7 @@ def test_product_chunked_matches_single_chunk(oracle_mod):
     try:
         g = _grid(c, O, 96, 64)
         mid = c.upload_mesh(tri, "exact9")
+        tune_product(16, 4 if operands == "f16x2" else 3, operands)
         tune_numeric(2, 4, 1)  # few waves wanted: one chunk
         one = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         tune_numeric(2, 4, 1 << 16)  # many waves wanted: one block per chunk
@@ -127,15 +130,17 @@ def test_product_chunked_matches_single_chunk(oracle_mod):
         assert d <= 5e-6
     finally:
         tune_numeric(2, 4, 2048)
+        tune_product(*PRODUCT_DEFAULT)
         c.close()
 
 
-def test_product_batch_mixed_with_valu_items(oracle_mod):
+@pytest.mark.parametrize("operands", ["fp32", "f16x2"])
+def test_product_batch_mixed_with_valu_items(oracle_mod, operands):
     """A batch holding product-grid items (matrix cores) and detector-grid items (per-pair kernels)."""
     import torch
 
     from hipgisaxs_amd import meshes
-    from hipgisaxs_amd.api import Batch, Context
+    from hipgisaxs_amd.api import PRODUCT_DEFAULT, Batch, Context, tune_product
 
     O = oracle_mod
     rng = np.random.default_rng(21)
@@ -154,6 +159,7 @@ def test_product_batch_mixed_with_valu_items(oracle_mod):
             grid, ref = (gp, ref_p) if k % 2 == 0 else (gd, ref_d)
             shapes.append((tri, R, grid, ref))
         mids = [main.upload_mesh(t, "exact9") for t, _, _, _ in shapes]
+        tune_product(16, 4 if operands == "f16x2" else 3, operands)  # read when the batch is made
         outs = [torch.full((2 * gr.info()["nq"],), float("nan"), dtype=torch.float64, device="cuda")
                 for _, _, gr, _ in shapes]
         b = Batch(main, [(gr, m, R, o) for (t, R, gr, _), m, o in zip(shapes, mids, outs)], precision="fp32c")
@@ -167,5 +173,51 @@ def test_product_batch_mixed_with_valu_items(oracle_mod):
                 assert rel_l2(main.as_complex(o).cpu().numpy(), want) <= 1e-5, use_graph
         b.release()
     finally:
+        tune_product(*PRODUCT_DEFAULT)
         for c in (main, gp, gd):
             c.close()
+
+
+@pytest.mark.parametrize("case", ["mixed_sizes", "far_absorbing", "near_origin"])
+def test_product_f16x2_scaling_edges(oracle_mod, case):
+    """Split-fp16 operands are scaled by powers of two chosen from |q|, the largest triangle and the
+    mesh extent: triangles 1000x smaller than the largest, a mesh 60 nm from the origin on
+    absorbing (complex qz) branches, and a grid close to q = 0 must all stay at the FP32C bar."""
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import PRODUCT_DEFAULT, Context, tune_product
+
+    O = oracle_mod
+    rng = np.random.default_rng(77)
+    tri = meshes.convex_polyhedron(rng, np.array([8.0, 6.0, 5.0]), 400)
+    qy0, qz0, qy1, qz1, dwba = -2.0, 0.0, 2.0, 2.0, None
+    if case == "mixed_sizes":
+        tri = np.concatenate([meshes.subdivide(tri[:4], 5), tri[4:]])
+    elif case == "far_absorbing":
+        tri = tri + np.tile(np.array([10.0, -20.0, 60.0]), 3)
+        dwba = 2e-6 + 2e-7j
+    else:
+        qy0, qz0, qy1, qz1 = -0.1, 0.0, 0.1, 0.1
+    c = Context(0)
+    try:
+        c.qgrid_explicit(45, 38, qy0, qz0, qy1, qz1, AI, K0)
+        g = O.qgrid_explicit(45, 38, qy0, qz0, qy1, qz1, AI, K0)
+        if dwba is not None:
+            c.qz_extended(K0, AI, dwba)
+            g = O.qz_extended(g, K0, AI, dwba)
+        assert c.is_product()
+        mid = c.upload_mesh(tri, "exact9")
+        ref = O.ff_approx(O.tri9_to_rec8(tri), g)
+        info = c.mesh_info(mid)
+        qn = np.sqrt((np.abs(g.q_rotated(None)) ** 2).sum(0))
+        ok = qn * info["radius"] >= 1e-6
+        errs = {}
+        for ops in ("fp32", "f16x2"):
+            tune_product(16, 4, ops)
+            got = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+            assert np.all(np.isfinite(got))
+            errs[ops] = rel_l2(got[ok], ref[ok])
+        print(f"{case}: fp32 {errs['fp32']:.2e} f16x2 {errs['f16x2']:.2e}")
+        assert errs["f16x2"] <= 1e-5
+    finally:
+        tune_product(*PRODUCT_DEFAULT)
+        c.close()
