@@ -40,10 +40,11 @@ def flop_per_pair_product(mults, operands="fp32"):
 PEAK_FP16 = 2.5e15  # MI355X dense fp16 matrix-core peak
 
 
-# HBM bytes per launch of k_ff_product32 on this workload at the default shape (16 blocks per unit,
-# Gauss form), from rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
-# device copy) in KiB per dispatch: (94933.4 x 2 + 2218240.0) x 1024.
-PRODUCT_TRAFFIC_BYTES = {(16, 3, "fp32"): (94933.4 * 2 + 2218240.0) * 1024}
+# HBM bytes per launch of the product kernel on this workload, per work-unit shape, from rocprofv3
+# --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB device copy) in KiB per
+# dispatch: k_ff_product32 (16 blocks per unit, Gauss form) and k_ff_product16s (the default).
+PRODUCT_TRAFFIC_BYTES = {(16, 3, "fp32"): (94933.4 * 2 + 2218240.0) * 1024,
+                         (64, 4, "f16x2", 2, True, True): (99335.0 * 2 + 694528.0) * 1024}
 
 
 def make_workload(nshapes=64, seed=SEED + 3):
@@ -332,9 +333,17 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the lattice and chi^2 measurements")
     ap.add_argument("--mode", choices=("batch", "per-shape"), default="batch",
                     help="batch: one launch for all shapes (hipGraph replay); per-shape: one call per shape")
-    ap.add_argument("--no-tune", action="store_true", help="skip the work-unit autotuning")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the work-unit shape first (default: the library's default shape)")
+    ap.add_argument("--no-tune", action="store_true", help="(default) skip the work-unit autotuning")
     ap.add_argument("--operands", choices=("fp32", "f16x2"), default=None,
                     help="product-grid operand precision when not autotuning (default: the library's)")
+    ap.add_argument("--row-tiles", type=int, default=None, choices=(1, 2, 4),
+                    help="split-fp16 product kernel: row tiles per wave when not autotuning")
+    ap.add_argument("--flat", action="store_true", help="with --row-tiles 4: one-level accumulation")
+    ap.add_argument("--no-share", action="store_true", help="with --row-tiles: row operands built per wave")
+    ap.add_argument("--blocks-per-unit", type=int, default=None,
+                    help="product kernel: triangle blocks per work unit when not autotuning")
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
     ap.add_argument("--no-mfma", action="store_true",
                     help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
@@ -386,18 +395,22 @@ def main():
     from hipgisaxs_amd.api import tune_product
     from hipgisaxs_amd.autotune import PRODUCT_DEFAULT
 
-    bpu_p, mults, operands = PRODUCT_DEFAULT
+    pcfg = tuple(PRODUCT_DEFAULT)
     if args.operands is not None:
-        mults, operands = (4, "f16x2") if args.operands == "f16x2" else (3, "fp32")
-    tune_product(bpu_p, mults, operands)
-    if args.mode == "batch" and not args.no_tune:
+        pcfg = (16, 3, "fp32") if args.operands == "fp32" else pcfg
+    if args.row_tiles is not None:
+        pcfg = (pcfg[0], 4, "f16x2", args.row_tiles, not args.flat, not args.no_share)
+    if args.blocks_per_unit is not None:
+        pcfg = (args.blocks_per_unit,) + pcfg[1:]
+    tune_product(*pcfg)
+    if args.mode == "batch" and args.tune and not args.no_tune:
         from hipgisaxs_amd.autotune import autotune_batch, autotune_product
 
         if product:
             best, times = autotune_product(main_ctx, items["fp32c"], flags)
-            bpu_p, mults, operands = best
-            tuned = {"blocks_per_unit": best[0], "complex_mults": best[1], "operands": best[2],
-                     "candidates_ms": {f"{k[0]}x{k[1]}x{k[2]}": round(v, 3) for k, v in times.items()}}
+            pcfg = tuple(best)
+            tuned = {"product_config": "x".join(map(str, best)),
+                     "candidates_ms": {"x".join(map(str, k)): round(v, 3) for k, v in times.items()}}
         else:
             best, times = autotune_batch(main_ctx, items["fp32c"], "fp32c", flags)
             tuned = {"qpt": best[0], "blocks_per_unit": best[1],
@@ -456,6 +469,7 @@ def main():
     value64 = pairs * args.fp64_steps / wall64
     avg_launch_ms = k32_ms / max(n32, 1)
     pairs_per_launch = pairs * args.steps / max(n32, 1)
+    bpu_p, mults, operands = pcfg[:3]
     fpp = flop_per_pair_product(mults, operands) if product else FLOP_PER_PAIR
     achieved = fpp * pairs_per_launch / (avg_launch_ms * 1e-3)
     split16 = product and operands == "f16x2"
@@ -491,7 +505,8 @@ def main():
         "relL2_fp32c_vs_fp64_gpu_worst_shape": worst,
         "roofline": {
             "bound": "mfma" if product else "valu",
-            "kernel": ("k_ff_product16" if split16 else "k_ff_product32") if product else
+            "kernel": (("k_ff_product16s" if len(pcfg) > 5 and pcfg[5] else "k_ff_product16") if split16
+                       else "k_ff_product32") if product else
                       ("k_batch32" if args.mode == "batch" else "k_ff_approx32"),
             "achieved": achieved / 1e12,
             "peak": peak / 1e12,
@@ -510,9 +525,8 @@ def main():
         # direct per-pair form (33 flop): rates of equivalent work, not of matrix-core issue
         result["roofline"]["complex_form_tflops"] = 16 * rate / 1e12
         result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * rate / 1e12
-        shape_key = (bpu_p, mults, operands)
-        if len(shapes) == 64 and not args.kahan and shape_key in PRODUCT_TRAFFIC_BYTES:
-            result["roofline"]["traffic"] = PRODUCT_TRAFFIC_BYTES[shape_key]
+        if len(shapes) == 64 and not args.kahan and pcfg in PRODUCT_TRAFFIC_BYTES:
+            result["roofline"]["traffic"] = PRODUCT_TRAFFIC_BYTES[pcfg]
     if b32 is not None:
         result["config"]["work_units"] = b32.info()["units"]
     if product and not args.no_extra:
@@ -521,6 +535,23 @@ def main():
         wall_v, kv_ms, nv, _ = timed(lambda g: bv.run(use_graph=g), 2, 1, True)
         result["fp32c_per_pair_valu"] = {"value": pairs * 2 / wall_v, "ms_per_step": 1e3 * wall_v / 2}
         bv.release()
+        # the one-level accumulation variant (FP32 sums over a whole work unit): faster, less accurate
+        from hipgisaxs_amd.autotune import PRODUCT_ONE_LEVEL
+
+        tune_product(*PRODUCT_ONE_LEVEL)
+        b1 = Batch(main_ctx, items["fp32c"], precision="fp32c", flags=flags)
+        wall_1, k1_ms, n1, _ = timed(lambda g: b1.run(use_graph=g), args.steps, 1, True)
+        b1.release()
+        num1 = den1 = 0.0
+        for s_ in shapes:
+            num1 += float(torch.linalg.vector_norm(s_["out32"] - s_["out64"])) ** 2
+            den1 += float(torch.linalg.vector_norm(s_["out64"])) ** 2
+        result["fp32c_one_level"] = {"config": "x".join(map(str, PRODUCT_ONE_LEVEL)),
+                                     "value": pairs * args.steps / wall_1, "ms_per_step": 1e3 * wall_1 / args.steps,
+                                     "relL2_vs_fp64_gpu": math.sqrt(num1 / den1)}
+        tune_product(*pcfg)
+        step32(False)  # the headline outputs again, for the oracle comparison below
+        torch.cuda.synchronize()
     if not args.no_extra:
         result["pyramid_fp64"] = bench_pyramid_fp64()
         print(f"[bench] pyramid done", file=sys.stderr, flush=True)

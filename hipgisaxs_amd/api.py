@@ -313,17 +313,25 @@ def tune_batch(qpt: int, blocks_per_unit: int) -> None:
         raise HgxError("tune_batch: invalid knob values")
 
 
-# the library's own product-kernel setting (hgx_tune_product's values at load)
-PRODUCT_DEFAULT = (16, 4, "f16x2")
+# the library's own product-kernel setting (hgx_tune_product / hgx_tune_product_tile at load):
+# (blocks per unit, real products per complex product, operands, row tiles, two-level, shared rows)
+PRODUCT_DEFAULT = (64, 4, "f16x2", 2, True, True)
 
 
-def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "fp32") -> None:
+def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "fp32", row_tiles: int = 2,
+                 two_level: bool = True, shared_rows: bool = True) -> None:
     """Work-unit size (128-triangle blocks) of the product-grid matrix-core kernel, 3 or 4 real
     products per complex product, and the operand precision: "fp32" or "f16x2" (each operand split
-    into two fp16 halves; 4 real products only)."""
+    into two fp16 halves; 4 real products only). For "f16x2" also the 32-row tiles per wave (1, 2,
+    4), two-level accumulation (False only with 4 row tiles) and whether the row operands are built
+    once per workgroup and shared (2 row tiles two-level, or 4 row tiles one-level)."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
-    if operands not in ops or N.load().hgx_tune_product(blocks_per_unit, complex_mults, ops[operands]) != N.HGX_OK:
+    lib = N.load()
+    if operands not in ops or lib.hgx_tune_product(blocks_per_unit, complex_mults, ops[operands]) != N.HGX_OK:
         raise HgxError("tune_product: invalid knob values")
+    if operands == "f16x2" and lib.hgx_tune_product_tile(int(row_tiles), int(bool(two_level)),
+                                                         int(bool(shared_rows))) != N.HGX_OK:
+        raise HgxError("tune_product: invalid row tiles / accumulation / sharing")
 
 
 class FitPlan:

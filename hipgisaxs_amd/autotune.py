@@ -6,7 +6,8 @@ so repeated runs settle on the same shape. Two kernel families are tuned:
 
 * the per-pair VALU kernels: (q-points per thread, triangle blocks per work unit);
 * the product-grid matrix-core kernel: (triangle blocks per work unit, real products per complex
-  product: 3 or 4, operand precision: FP32 or split fp16).
+  product: 3 or 4, operand precision: FP32 or split fp16, and for split fp16 the row tiles per wave
+  and two-level accumulation).
 
 The reference's counterpart is the hyperblock-size search of its legacy CPU path
 (src/ff/cpu/ff_num_cpu.cpp:116-176).
@@ -17,8 +18,12 @@ from .api import PRODUCT_DEFAULT, Batch, Context, tune_batch, tune_product
 
 VALU_DEFAULT = (4, 16)
 VALU_CANDIDATES = (VALU_DEFAULT, (4, 8), (4, 32), (8, 16), (2, 16), (8, 32))
-PRODUCT_CANDIDATES = (PRODUCT_DEFAULT, (8, 4, "f16x2"), (32, 4, "f16x2"), (64, 4, "f16x2"),
-                      (16, 3, "fp32"), (32, 3, "fp32"))
+# two-level accumulation only: the one-level variants (FP32 sums over a whole work unit) are about
+# 10x less accurate and are measured separately
+PRODUCT_CANDIDATES = (PRODUCT_DEFAULT, (16, 4, "f16x2", 2, True, True), (32, 4, "f16x2", 2, True, True),
+                      (64, 4, "f16x2", 2, True, False), (64, 4, "f16x2", 4, True, False),
+                      (16, 4, "f16x2", 1, True, False), (16, 3, "fp32"))
+PRODUCT_ONE_LEVEL = (64, 4, "f16x2", 4, False, True)
 
 
 def _time_batch(ctx: Context, items, precision, flags, reps):
@@ -56,6 +61,6 @@ def autotune_batch(ctx: Context, items, precision: str = "fp32c", flags: int = 0
 
 def autotune_product(ctx: Context, items, flags: int = 0, candidates=PRODUCT_CANDIDATES, reps: int = 1,
                      margin: float = 0.02):
-    """Product-grid kernel (FP32C): returns ((blocks_per_unit, complex_mults, operands), {candidate: ms})."""
+    """Product-grid kernel (FP32C): returns (best candidate tuple as taken by tune_product, {candidate: ms})."""
     return _search(tune_product, PRODUCT_DEFAULT, candidates,
                    lambda: _time_batch(ctx, items, "fp32c", flags, reps), margin)

@@ -279,7 +279,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     for (int64_t qb = 0; qb < (d.g.nq + kWG - 1) / kWG; ++qb) fin.push_back(FinishUnit{i, (int)qb});
     if (is_prod[i]) {
       const ProductItem& pi = pit[i];
-      int nzt = (int)ceil_div(pi.nz, kProductTile), nyt = (int)ceil_div(pi.ncol, kProductTile);
+      int nzt = (int)ceil_div(pi.nz, product_zrows(b.pmode)), nyt = (int)ceil_div(pi.ncol, product_ycols(b.pmode));
       for (int ch = 0; ch < d.nchunks; ++ch) {
         int64_t nb = std::min<int64_t>(d.bchunk, d.nblocks - ch * d.bchunk);
         for (int zt = 0; zt < nzt; ++zt)

@@ -292,9 +292,11 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     it.snmax = (float)m.snmax;
     it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
                       m.radius);
-    const int64_t tiles = (int64_t)ceil_div(it.nz, kProductTile) * ceil_div(it.ncol, kProductTile);
+    const int pmode = product_mode();
+    const int64_t tiles = (int64_t)ceil_div(it.nz, product_zrows(pmode)) * ceil_div(it.ncol, product_ycols(pmode));
     // enough waves to fill the chip several times over, without splitting below one block
-    int64_t want = std::max<int64_t>(1, (int64_t)(4 * g_target_wg + tiles - 1) / tiles);
+    const int64_t upw = product_is_shared(pmode) ? 1 : kProductWaves;  // work units per workgroup
+    int64_t want = std::max<int64_t>(1, (int64_t)(upw * g_target_wg + tiles - 1) / tiles);
     want = std::min<int64_t>(want, m.nblocks);
     // chunks accumulate in FP32: at most product_blocks_per_unit() blocks each
     it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit());
@@ -321,7 +323,7 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     }
     {
       LaunchTimer lt(c, true);
-      product_launch_single(c->stream, it, cq, product_mode());
+      product_launch_single(c->stream, it, cq, pmode);
     }
     HGX_CHECK_LAUNCH(c);
     {

@@ -603,7 +603,7 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   // the trapezoid meshes are built on the device, so the split-fp16 operand scales (which need the
   // largest triangle) are not known here: the grating's forward pass keeps FP32 operands
   product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq,
-                       product_mode() == kProductF16x2 ? kProductF32Gauss : product_mode());
+                       product_is_f16(product_mode()) ? kProductF32Gauss : product_mode());
   dim3 gpix(p.npb, gsize);
   k_grating_pixel<<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.d_wt, p.d_cpart,
                                        want_I ? Iout : nullptr);

@@ -131,10 +131,27 @@ struct AdjItem {
 constexpr int kProductTile = 32;
 constexpr int kProductWaves = 4;  // work units (waves) per workgroup
 // product kernel variants: FP32 operands with 4 or 3 (Gauss) real products per complex product,
-// or each FP32 operand split into two fp16 halves (4 real products, 3 cross terms each)
+// or each FP32 operand split into two fp16 halves (4 real products, 3 cross terms each) with 1, 2
+// or 4 row tiles of 32 per wave (two-level FP32 accumulation, or in place over the chunk: Flat)
 constexpr int kProductF32x4 = 0;
 constexpr int kProductF32Gauss = 1;
 constexpr int kProductF16x2 = 2;
+constexpr int kProductF16x2Z2 = 3;
+constexpr int kProductF16x2Z4 = 4;
+constexpr int kProductF16x2Z4Flat = 5;
+// the same with the z operands built once per workgroup and shared through LDS: one work unit per
+// workgroup, 32 x row tiles rows by 4 x 32 columns
+constexpr int kProductF16x2S2 = 6;
+constexpr int kProductF16x2S4Flat = 7;
+inline bool product_is_f16(int mode) { return mode >= kProductF16x2; }
+inline bool product_is_shared(int mode) { return mode >= kProductF16x2S2; }
+// rows (z) and columns (y) of one work unit's tile for a kernel variant
+inline int product_zrows(int mode) {
+  return (mode == kProductF16x2Z2 || mode == kProductF16x2S2)           ? 2 * kProductTile
+         : (mode == kProductF16x2Z4 || mode == kProductF16x2Z4Flat || mode == kProductF16x2S4Flat) ? 4 * kProductTile
+                                                                                 : kProductTile;
+}
+inline int product_ycols(int mode) { return product_is_shared(mode) ? kProductWaves * kProductTile : kProductTile; }
 
 
 struct Context {

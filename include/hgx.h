@@ -354,10 +354,19 @@ int hgx_tune_numeric(int qpt64, int qpt32, int target_wg);
 
 /* Work-unit shape of the product-grid kernel (single calls and later batches): triangle blocks (128
  * triangles each) per unit, 3 (Gauss) or 4 real products per complex product (HGX_PRODUCT_F16X2
- * takes 4 only), and the operand precision. Default (16, 4, HGX_PRODUCT_F16X2). Process-wide; set by
+ * takes 4 only), and the operand precision. Default (64, 4, HGX_PRODUCT_F16X2). Process-wide; set by
  * the autotuner. The grating fit's forward pass keeps FP32 operands whatever is set. No reference
  * counterpart. */
 int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
+
+/* Tile of the split-fp16 product-grid kernel. row_tiles: 32-row tiles per wave (1, 2 or 4); each
+ * wave builds its column's operands once for all its rows. two_level = 1 adds each 128-triangle
+ * block's sums to the work unit's sums after the block; 0 accumulates in place over the unit
+ * (row_tiles 4 only: fewer registers, about 10x larger FP32 rounding error). shared_rows = 1 builds
+ * the row operands once per workgroup of four waves (adjacent column tiles) and shares them through
+ * LDS (row_tiles 2 with two_level, or 4 without). Default (2, 1, 1). Process-wide; set by the
+ * autotuner; applies to single calls and to batches created afterwards. No reference counterpart. */
+int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows);
 
 #ifdef __cplusplus
 }

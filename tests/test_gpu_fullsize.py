@@ -32,13 +32,19 @@ def test_numeric_fp32c_full_size_grid():
         try:
             tune_product(16, 3, "fp32")
             f32 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
-            tune_product(16, 4, "f16x2")
-            f16 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
+            f16 = {}
+            for variant in ((16, 4, "f16x2", 1, True, False), (64, 4, "f16x2", 2, True, False),
+                            (64, 4, "f16x2", 4, True, False), (64, 4, "f16x2", 4, False, False),
+                            (64, 4, "f16x2", 2, True, True), (64, 4, "f16x2", 4, False, True)):
+                tune_product(*variant)
+                f16[variant] = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         finally:
             tune_product(*PRODUCT_DEFAULT)
-        e_mfma, e_valu, e_16 = rel_l2(f32, f64), rel_l2(fva, f64), rel_l2(f16, f64)
-        print(f"1024^2 x 50k: mfma {e_mfma:.2e} valu {e_valu:.2e} split-fp16 mfma {e_16:.2e}")
-        assert e_mfma <= 1e-5 and e_valu <= 1e-5 and e_16 <= 1e-5
+        e_mfma, e_valu = rel_l2(f32, f64), rel_l2(fva, f64)
+        e_16 = {"x".join(map(str, k)): rel_l2(v, f64) for k, v in f16.items()}
+        print(f"1024^2 x 50k: mfma {e_mfma:.2e} valu {e_valu:.2e} split-fp16 mfma",
+              {k: f"{v:.2e}" for k, v in e_16.items()})
+        assert e_mfma <= 1e-5 and e_valu <= 1e-5 and max(e_16.values()) <= 1e-5
         # Friedel: a grid symmetric in qy with qz -> -qz gives the conjugate
         c.qgrid_explicit(512, 256, -2.0, -2.0, 2.0, 0.0, 0.0, 1e3)  # qz in [-2, 0]
         lo = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy().reshape(256, 512)

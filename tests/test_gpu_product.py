@@ -56,6 +56,12 @@ def test_product_grid_detection(oracle_mod):
         c.close()
 
 
+PRODUCT_VARIANTS = [(16, 3, "fp32"), (16, 4, "fp32"), (16, 4, "f16x2", 1, True, False),
+                    (16, 4, "f16x2", 2, True, False), (16, 4, "f16x2", 4, True, False),
+                    (16, 4, "f16x2", 4, False, False), (16, 4, "f16x2", 2, True, True),
+                    (16, 4, "f16x2", 4, False, True)]
+
+
 @pytest.mark.parametrize("ncol,nrow,ntri,dwba,rotated", [
     (40, 23, 900, None, False),
     (64, 64, 3000, None, True),
@@ -63,8 +69,8 @@ def test_product_grid_detection(oracle_mod):
     (48, 20, 1500, 2e-6 + 1e-8j, True),  # complex branches (absorbing layer), nz = 4 * 20
     (37, 45, 700, 0j, False),            # real branches
 ])
-@pytest.mark.parametrize("mults,operands", [(3, "fp32"), (4, "fp32"), (4, "f16x2")])
-def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, mults, operands):
+@pytest.mark.parametrize("variant", PRODUCT_VARIANTS, ids=lambda v: "x".join(map(str, v)))
+def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, variant):
     from hipgisaxs_amd import meshes
     from hipgisaxs_amd.api import PRODUCT_DEFAULT, Context, tune_product
 
@@ -74,7 +80,7 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
     R = _rot(0.3, -0.7, 1.1) if rotated else None
     c = Context(0)
     try:
-        tune_product(16, mults, operands)
+        tune_product(*variant)
         g = _grid(c, O, ncol, nrow, dwba)
         assert c.is_product()
         mid = c.upload_mesh(tri, "exact9")
@@ -93,7 +99,7 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
         ok = ~tiny
         got, valu, ref = got[ok], valu[ok], ref[ok]
         e_mfma, e_valu = rel_l2(got, ref), rel_l2(valu, ref)
-        print(f"{ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} mults={mults} {operands}: "
+        print(f"{ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} {variant}: "
               f"mfma {e_mfma:.2e} valu {e_valu:.2e}")
         assert np.all(np.isfinite(got))
         assert e_mfma <= 1e-5
@@ -211,13 +217,13 @@ def test_product_f16x2_scaling_edges(oracle_mod, case):
         qn = np.sqrt((np.abs(g.q_rotated(None)) ** 2).sum(0))
         ok = qn * info["radius"] >= 1e-6
         errs = {}
-        for ops in ("fp32", "f16x2"):
-            tune_product(16, 4, ops)
+        for variant in PRODUCT_VARIANTS[1:]:
+            tune_product(*variant)
             got = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
             assert np.all(np.isfinite(got))
-            errs[ops] = rel_l2(got[ok], ref[ok])
-        print(f"{case}: fp32 {errs['fp32']:.2e} f16x2 {errs['f16x2']:.2e}")
-        assert errs["f16x2"] <= 1e-5
+            errs["x".join(map(str, variant))] = rel_l2(got[ok], ref[ok])
+        print(case, {k: f"{v:.2e}" for k, v in errs.items()})
+        assert max(errs.values()) <= 1e-5
     finally:
         tune_product(*PRODUCT_DEFAULT)
         c.close()
