@@ -72,13 +72,16 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(shapes, run_s=4.0, repeats=3):
-    """Times the FP64 OpenMP oracle on a bounded sample of the workload: one mid-size shape, a band of its
-    q-grid rows sized to about run_s per run, median of `repeats` runs; then the same loop as the reference
-    writes it (a heap-allocated rotation per pair) on a band sized to about run_s, once."""
+def cpu_baseline(shapes, run_s=5.0, repeats=3):
+    """Times the FP64 OpenMP oracle on a bounded sample of the workload (about 20 s of CPU work in all):
+    the median-size shape among those on the largest grid, a band of its q-grid rows sized to about run_s
+    per run, median of `repeats` runs; then the same loop as the reference writes it (a heap-allocated
+    rotation per pair) on a band sized to about run_s, once."""
     import oracle as O
 
-    s = sorted(shapes, key=lambda d: d["tri"].shape[0])[len(shapes) // 2]
+    big = max(d["side"] for d in shapes)
+    pool = sorted((d for d in shapes if d["side"] == big), key=lambda d: d["tri"].shape[0])
+    s = pool[len(pool) // 2]
     rec = O.tri9_to_rec8(s["tri"])
     side = s["side"]
     g = O.qgrid_explicit(side, side, -2.0, 0.0, 2.0, 2.0, 0.0, 1e3)

@@ -324,7 +324,7 @@ def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "
     products per complex product, and the operand precision: "fp32" or "f16x2" (each operand split
     into two fp16 halves; 4 real products only). For "f16x2" also the 32-row tiles per wave (1, 2,
     4), two-level accumulation (False only with 4 row tiles) and whether the row operands are built
-    once per workgroup and shared (2 row tiles two-level, or 4 row tiles one-level)."""
+    once per workgroup and shared (2 or 4 row tiles)."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
     lib = N.load()
     if operands not in ops or lib.hgx_tune_product(blocks_per_unit, complex_mults, ops[operands]) != N.HGX_OK:

@@ -21,6 +21,7 @@ VALU_CANDIDATES = (VALU_DEFAULT, (4, 8), (4, 32), (8, 16), (2, 16), (8, 32))
 # two-level accumulation only: the one-level variants (FP32 sums over a whole work unit) are about
 # 10x less accurate and are measured separately
 PRODUCT_CANDIDATES = (PRODUCT_DEFAULT, (16, 4, "f16x2", 2, True, True), (32, 4, "f16x2", 2, True, True),
+                      (64, 4, "f16x2", 4, True, True), (32, 4, "f16x2", 4, True, True),
                       (64, 4, "f16x2", 2, True, False), (64, 4, "f16x2", 4, True, False),
                       (16, 4, "f16x2", 1, True, False), (16, 3, "fp32"))
 PRODUCT_ONE_LEVEL = (64, 4, "f16x2", 4, False, True)

@@ -233,9 +233,12 @@ __global__ __launch_bounds__(256) void k_grating_mesh(GMesh gm, const GState* __
 
 // Block origins (mean centroid of the block's real triangles, fixed-order LDS tree) and the FP32
 // block-relative records the forward kernel reads.
+// Block origins and FP32 records of every block; also each block's largest |A_t| and |c_t| (the
+// split-fp16 forward's operand scales need bounds on both), reduced per item by k_grating_scales.
 __global__ __launch_bounds__(128) void k_grating_blocks(GMesh gm, const double* __restrict__ rec64,
-                                                        float* __restrict__ rec32, double* __restrict__ borig) {
-  __shared__ double sh[4][128];
+                                                        float* __restrict__ rec32, double* __restrict__ borig,
+                                                        float* __restrict__ bmax) {
+  __shared__ double sh[6][128];
   const int item = blockIdx.y, B = blockIdx.x, t = threadIdx.x;
   const int64_t s = (int64_t)B * 128 + t;
   const double* r = rec64 + ((int64_t)item * gm.nslots + s) * 8;
@@ -244,11 +247,16 @@ __global__ __launch_bounds__(128) void k_grating_blocks(GMesh gm, const double* 
   sh[1][t] = real ? r[4] : 0.0;
   sh[2][t] = real ? r[5] : 0.0;
   sh[3][t] = real ? 1.0 : 0.0;
+  sh[4][t] = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  sh[5][t] = real ? sqrt(r[3] * r[3] + r[4] * r[4] + r[5] * r[5]) : 0.0;
   __syncthreads();
   for (int w = 64; w > 0; w >>= 1) {
-    if (t < w)
+    if (t < w) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + w];
+      sh[4][t] = fmax(sh[4][t], sh[4][t + w]);
+      sh[5][t] = fmax(sh[5][t], sh[5][t + w]);
+    }
     __syncthreads();
   }
   const double n = sh[3][0] > 0 ? sh[3][0] : 1.0;
@@ -259,6 +267,10 @@ __global__ __launch_bounds__(128) void k_grating_blocks(GMesh gm, const double* 
     bo[1] = o[1];
     bo[2] = o[2];
     bo[3] = 0.0;
+    // raised by 1e-6 relative, so the bounds still hold after rounding to FP32
+    float* bm = bmax + ((int64_t)item * gm.nblocks + B) * 2;
+    bm[0] = (float)(sh[4][0] * (1.0 + 1e-6));
+    bm[1] = (float)(sh[5][0] * (1.0 + 1e-6));
   }
   float* f = rec32 + ((int64_t)item * gm.nslots + s) * 8;
 #pragma unroll
@@ -267,6 +279,32 @@ __global__ __launch_bounds__(128) void k_grating_blocks(GMesh gm, const double* 
     f[3 + k] = real ? (float)(r[3 + k] - o[k]) : 0.0f;
   }
   f[6] = f[7] = 0.0f;
+}
+
+// Largest |s n| and |c| over each item's blocks into its product-kernel item (one workgroup per item).
+__global__ __launch_bounds__(256) void k_grating_scales(int nblocks, const float* __restrict__ bmax,
+                                                        ProductItem* __restrict__ pitems) {
+  __shared__ float sh[2][256];
+  const int item = blockIdx.x, t = threadIdx.x;
+  float a = 0.0f, c = 0.0f;
+  for (int b = t; b < nblocks; b += 256) {
+    a = fmaxf(a, bmax[((int64_t)item * nblocks + b) * 2]);
+    c = fmaxf(c, bmax[((int64_t)item * nblocks + b) * 2 + 1]);
+  }
+  sh[0][t] = a;
+  sh[1][t] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] = fmaxf(sh[0][t], sh[0][t + w]);
+      sh[1][t] = fmaxf(sh[1][t], sh[1][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    pitems[item].snmax = sh[0][0];
+    pitems[item].rmax = sh[1][0];
+  }
 }
 
 // ----- pixel pass: amplitude, intensity, chi^2, adjoint weights --------------------------------
@@ -553,12 +591,14 @@ struct GratingPlan {
   int ncand = 0, G = 0, ngroups = 0;
   int nchunks = 1, nyt = 0, ntt = 0, upi_fwd = 0, upi_adj = 0;
   int npb = 0, ngb = 0;  // pixel blocks, gradient blocks per item
+  int pmode = 0;         // product-kernel variant of the forward pass, fixed at creation
   bool cq = false;
   GState* d_state = nullptr;
   double* d_rec64 = nullptr;
   double* d_drec = nullptr;
   float* d_rec32 = nullptr;
   double* d_borig = nullptr;
+  float* d_bmax = nullptr;
   double* d_fbuf = nullptr;
   float* d_wt = nullptr;
   double* d_apart = nullptr;
@@ -585,7 +625,7 @@ void free_plan(GratingPlan& p) {
   if (p.step_exec) (void)hipGraphExecDestroy(p.step_exec);
   if (p.eval_graph) (void)hipGraphDestroy(p.eval_graph);
   if (p.step_graph) (void)hipGraphDestroy(p.step_graph);
-  void* ptrs[] = {p.d_state, p.d_rec64, p.d_drec, p.d_rec32, p.d_borig, p.d_fbuf, p.d_wt,
+  void* ptrs[] = {p.d_state, p.d_rec64, p.d_drec, p.d_rec32, p.d_borig, p.d_bmax, p.d_fbuf, p.d_wt,
                   p.d_apart, p.d_gpart, p.d_cpart, p.d_pitems, p.d_punits, p.d_aitems};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -599,11 +639,10 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   dim3 gmesh(ceil_div(gm.nslots, 256), gsize);
   k_grating_mesh<<<gmesh, 256, 0, s>>>(gm, p.d_state, cand0, p.d_rec64, p.d_drec);
   dim3 gblk(gm.nblocks, gsize);
-  k_grating_blocks<<<gblk, 128, 0, s>>>(gm, p.d_rec64, p.d_rec32, p.d_borig);
-  // the trapezoid meshes are built on the device, so the split-fp16 operand scales (which need the
-  // largest triangle) are not known here: the grating's forward pass keeps FP32 operands
-  product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq,
-                       product_is_f16(product_mode()) ? kProductF32Gauss : product_mode());
+  k_grating_blocks<<<gblk, 128, 0, s>>>(gm, p.d_rec64, p.d_rec32, p.d_borig, p.d_bmax);
+  // the meshes are built here, so the split-fp16 operand scales' bounds are too
+  k_grating_scales<<<gsize, 256, 0, s>>>(gm.nblocks, p.d_bmax, p.d_pitems);
+  product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq, p.pmode);
   dim3 gpix(p.npb, gsize);
   k_grating_pixel<<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.d_wt, p.d_cpart,
                                        want_I ? Iout : nullptr);
@@ -704,7 +743,8 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   pa.L = m.length;
   p.cq = !c->qz_real;
   const int nz = c->nbranch * c->nrow;
-  const int nzt = (int)ceil_div(nz, kProductTile), nyt_f = (int)ceil_div(c->ncol, kProductTile);
+  p.pmode = product_mode();
+  const int nzt = (int)ceil_div(nz, product_zrows(p.pmode)), nyt_f = (int)ceil_div(c->ncol, product_ycols(p.pmode));
   const int64_t bchunk = std::min<int64_t>(kGChunkBlocks, gm.nblocks);
   p.nchunks = (int)((gm.nblocks + bchunk - 1) / bchunk);
   p.nyt = (int)ceil_div(c->ncol, 32);
@@ -714,7 +754,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   p.npb = (int)ceil_div(c->npix, 256);
   p.ngb = (int)ceil_div(gm.nslots, 256);
   // group size: per-candidate device memory within a 24 GiB budget
-  const double per = (double)gm.nslots * (8 * 8 + 18 * 8 + 8 * 4) + (double)gm.nblocks * 32 +
+  const double per = (double)gm.nslots * (8 * 8 + 18 * 8 + 8 * 4) + (double)gm.nblocks * 40 +
                      (double)p.nchunks * c->nq * 16 + (double)c->nq * 8 + 6.0 * p.nyt * gm.nslots * 16 +
                      (double)(p.npb * 2 + p.ngb * 3) * 8;
   p.G = (int)std::max(1.0, std::min((double)ncand, 24.0 * (1 << 30) / per));
@@ -732,6 +772,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   if (hipMalloc(&p.d_drec, sizeof(double) * 18 * gm.nslots * G) != hipSuccess) return fail("record derivatives");
   if (hipMalloc(&p.d_rec32, sizeof(float) * 8 * gm.nslots * G) != hipSuccess) return fail("FP32 records");
   if (hipMalloc(&p.d_borig, sizeof(double) * 4 * gm.nblocks * G) != hipSuccess) return fail("block origins");
+  if (hipMalloc(&p.d_bmax, sizeof(float) * 2 * gm.nblocks * G) != hipSuccess) return fail("block bounds");
   if (hipMalloc(&p.d_fbuf, sizeof(double) * 2 * p.nchunks * c->nq * G) != hipSuccess) return fail("sums");
   if (hipMalloc(&p.d_wt, sizeof(float) * 2 * c->nq * G) != hipSuccess) return fail("weights");
   if (hipMalloc(&p.d_apart, sizeof(double) * 2 * 6 * p.nyt * gm.nslots * G) != hipSuccess) return fail("adjoint");

@@ -143,12 +143,15 @@ constexpr int kProductF16x2Z4Flat = 5;
 // workgroup, 32 x row tiles rows by 4 x 32 columns
 constexpr int kProductF16x2S2 = 6;
 constexpr int kProductF16x2S4Flat = 7;
+constexpr int kProductF16x2S4 = 8;
 inline bool product_is_f16(int mode) { return mode >= kProductF16x2; }
 inline bool product_is_shared(int mode) { return mode >= kProductF16x2S2; }
 // rows (z) and columns (y) of one work unit's tile for a kernel variant
 inline int product_zrows(int mode) {
   return (mode == kProductF16x2Z2 || mode == kProductF16x2S2)           ? 2 * kProductTile
-         : (mode == kProductF16x2Z4 || mode == kProductF16x2Z4Flat || mode == kProductF16x2S4Flat) ? 4 * kProductTile
+         : (mode == kProductF16x2Z4 || mode == kProductF16x2Z4Flat || mode == kProductF16x2S4Flat ||
+            mode == kProductF16x2S4)
+             ? 4 * kProductTile
                                                                                  : kProductTile;
 }
 inline int product_ycols(int mode) { return product_is_shared(mode) ? kProductWaves * kProductTile : kProductTile; }

@@ -38,7 +38,7 @@ static int g_product_shared = 1;
 int product_blocks_per_unit() { return g_product_bpu; }
 int product_mode() {
   if (g_product_operands == HGX_PRODUCT_FP32) return g_product_mults == 3 ? kProductF32Gauss : kProductF32x4;
-  if (g_product_shared) return g_product_mz == 4 ? kProductF16x2S4Flat : kProductF16x2S2;
+  if (g_product_shared) return g_product_mz == 4 ? (g_product_two ? kProductF16x2S4 : kProductF16x2S4Flat) : kProductF16x2S2;
   if (g_product_mz == 4) return g_product_two ? kProductF16x2Z4 : kProductF16x2Z4Flat;
   return g_product_mz == 2 ? kProductF16x2Z2 : kProductF16x2;
 }
@@ -531,7 +531,7 @@ __device__ inline YOps y_step(const float4* __restrict__ rb, int ps, int h, floa
 }
 
 template <bool CQ, int MZ, bool TWO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MZ == 4 && TWO ? 1 : 2, 8)))
 void k_ff_product16s(const ProductItem* __restrict__ items, const ProductUnit* __restrict__ units, int nunits,
                      ProductItem single, int nzt, int nyg) {
   static_assert(MZ == 2 || MZ == 4, "the workgroup tile's rows are built 64 per pass");
@@ -770,6 +770,7 @@ static void launch16_mode(hipStream_t s, const ProductItem* items, const Product
   switch (mode) {
     case kProductF16x2S2: launch16s_t<CQ, 2, true>(s, items, units, nunits, single, nzt, nyt); break;
     case kProductF16x2S4Flat: launch16s_t<CQ, 4, false>(s, items, units, nunits, single, nzt, nyt); break;
+    case kProductF16x2S4: launch16s_t<CQ, 4, true>(s, items, units, nunits, single, nzt, nyt); break;
     case kProductF16x2Z2: launch16_t<CQ, 2, true>(s, items, units, nunits, single, nzt, nyt); break;
     case kProductF16x2Z4: launch16_t<CQ, 4, true>(s, items, units, nunits, single, nzt, nyt); break;
     case kProductF16x2Z4Flat: launch16_t<CQ, 4, false>(s, items, units, nunits, single, nzt, nyt); break;
@@ -821,7 +822,7 @@ extern "C" int hgx_tune_product(int blocks_per_unit, int complex_mults, int oper
 extern "C" int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows) {
   if (!(row_tiles == 1 || row_tiles == 2 || row_tiles == 4)) return HGX_ERR_INVALID_ARG;
   if (!two_level && row_tiles != 4) return HGX_ERR_INVALID_ARG;
-  if (shared_rows && !((row_tiles == 2 && two_level) || (row_tiles == 4 && !two_level))) return HGX_ERR_INVALID_ARG;
+  if (shared_rows && row_tiles == 1) return HGX_ERR_INVALID_ARG;
   g_product_mz = row_tiles;
   g_product_two = two_level ? 1 : 0;
   g_product_shared = shared_rows ? 1 : 0;

@@ -364,7 +364,7 @@ int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
  * block's sums to the work unit's sums after the block; 0 accumulates in place over the unit
  * (row_tiles 4 only: fewer registers, about 10x larger FP32 rounding error). shared_rows = 1 builds
  * the row operands once per workgroup of four waves (adjacent column tiles) and shares them through
- * LDS (row_tiles 2 with two_level, or 4 without). Default (2, 1, 1). Process-wide; set by the
+ * LDS (row_tiles 2 or 4). Default (2, 1, 1). Process-wide; set by the
  * autotuner; applies to single calls and to batches created afterwards. No reference counterpart. */
 int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows);
 

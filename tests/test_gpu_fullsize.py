@@ -35,7 +35,8 @@ def test_numeric_fp32c_full_size_grid():
             f16 = {}
             for variant in ((16, 4, "f16x2", 1, True, False), (64, 4, "f16x2", 2, True, False),
                             (64, 4, "f16x2", 4, True, False), (64, 4, "f16x2", 4, False, False),
-                            (64, 4, "f16x2", 2, True, True), (64, 4, "f16x2", 4, False, True)):
+                            (64, 4, "f16x2", 2, True, True), (64, 4, "f16x2", 4, False, True),
+                            (64, 4, "f16x2", 4, True, True)):
                 tune_product(*variant)
                 f16[variant] = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         finally:

@@ -59,7 +59,7 @@ def test_product_grid_detection(oracle_mod):
 PRODUCT_VARIANTS = [(16, 3, "fp32"), (16, 4, "fp32"), (16, 4, "f16x2", 1, True, False),
                     (16, 4, "f16x2", 2, True, False), (16, 4, "f16x2", 4, True, False),
                     (16, 4, "f16x2", 4, False, False), (16, 4, "f16x2", 2, True, True),
-                    (16, 4, "f16x2", 4, False, True)]
+                    (16, 4, "f16x2", 4, False, True), (16, 4, "f16x2", 4, True, True)]
 
 
 @pytest.mark.parametrize("ncol,nrow,ntri,dwba,rotated", [
