@@ -151,6 +151,7 @@ _SIGS = {
     "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product_tile": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "hgx_tune_adjoint": (C.c_int, [C.c_int]),
     "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_grating_set_theta": (C.c_int, [_vp, C.c_int, _vp]),

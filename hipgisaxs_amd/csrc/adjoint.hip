@@ -336,9 +336,330 @@ __global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __
   adj_store(R, rec, t, *w0p, gA, gc, src, grad);
 }
 
+// ----- split fp16 on product grids ----------------------------------------------------------------
+//
+// The same bilinear forms on v_mfma_f32_32x32x16_f16 (K = 16 z values per instruction, lane half h
+// holding z = 8 h .. 8 h + 7), every FP32 operand x split into x_h = fp16(x) and x_l = fp16(x - x_h)
+// and each real product taken as x_h y_h + x_l y_h + x_h y_l, as in the forward product kernel.
+// Complex products in Gauss's form: P1 = sum Ar Wr, P2 = sum Ai Wi, P3 = sum (Ar + Ai)(Wr + Wi),
+// X = (P1 - P2, P3 - P1 - P2), i.e. 9 MFMAs per 16 z values and 32 x 32 tile.
+// Weights are split once per evaluation (k_adj_wmax, k_adj_wsplit), scaled per column by 2^sw(y) so
+// that |Wr| + |Wi| <= 2^14; the A operands f(z) Ez_t(z) are built per lane and scaled by 2^sa, uniform
+// over the workgroup, chosen from the largest |f(z)| times the attenuation bound. X is unscaled by
+// 2^-(sa + sw(y)) before the contraction with g(y) Ey_t(y).
+//
+// A workgroup = (item, f-pass, 128-triangle block, 64 columns): wave w takes triangles 32 w .. 32 w +
+// 31 and both 32-column tiles, so each A operand feeds two tiles; the weights of one 16-z step are
+// staged in LDS for the 4 waves (double-buffered, one barrier per step). Work units run block-fastest,
+// so workgroups running together share the same weights in L2.
+
+constexpr int kAdjZMax = 256;   // z rows per partial column maximum
+constexpr int kAdjNY = 2;       // 32-column tiles per wave
+constexpr int kAdjYG = kAdjTile * kAdjNY;
+
+__global__ __launch_bounds__(256) void k_adj_wmax(const float* __restrict__ wt, int64_t wt_stride, int nz, int ncol,
+                                                  float* __restrict__ pmax, int64_t pmax_stride) {
+  const int y = blockIdx.x * 256 + threadIdx.x, zc = blockIdx.y, item = blockIdx.z;
+  if (y >= ncol) return;
+  const float* w = wt + item * wt_stride;
+  float m = 0.0f;
+  const int z1 = min(nz, (zc + 1) * kAdjZMax);
+  for (int z = zc * kAdjZMax; z < z1; ++z) {
+    const float2 v = *reinterpret_cast<const float2*>(w + 2 * ((int64_t)z * ncol + y));
+    m = fmaxf(m, fabsf(v.x) + fabsf(v.y));
+  }
+  pmax[item * pmax_stride + (int64_t)zc * ncol + y] = m;
+}
+
+// exponent e with v <= 2^e (v > 0)
+__device__ inline int adj_exp2_bound(float v) {
+  int e;
+  (void)frexpf(v, &e);
+  return e;
+}
+
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef float fx2 __attribute__((ext_vector_type(2)));
+
+__device__ inline h16x2 a16pk(float a, float b) { return __builtin_convertvector((fx2){a, b}, h16x2); }
+
+// 8 FP32 values to their high and low fp16 halves
+__device__ inline void adj_split8(const float (&x)[8], h16x8& hi, h16x8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const h16x2 hh = a16pk(x[e], x[e + 1]);
+    const h16x2 ll = a16pk(x[e] - (float)hh[0], x[e + 1] - (float)hh[1]);
+    hi[e] = hh[0];
+    hi[e + 1] = hh[1];
+    lo[e] = ll[0];
+    lo[e + 1] = ll[1];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt, int64_t wt_stride, int nz, int ncol,
+                                                    const float* __restrict__ pmax, int64_t pmax_stride,
+                                                    uint4* __restrict__ w16, int64_t w16_stride,
+                                                    int* __restrict__ wexp, int64_t wexp_stride) {
+  const int y = blockIdx.x * 256 + threadIdx.x, zo = blockIdx.y, item = blockIdx.z;
+  if (y >= ncol) return;
+  const int nzo = (nz + 7) / 8, nzc = (nz + kAdjZMax - 1) / kAdjZMax;
+  float m = 0.0f;
+  for (int zc = 0; zc < nzc; ++zc) m = fmaxf(m, pmax[item * pmax_stride + (int64_t)zc * ncol + y]);
+  const int sw = m > 0.0f ? max(-100, min(100, 14 - adj_exp2_bound(m))) : 0;
+  if (zo == 0) wexp[item * wexp_stride + y] = sw;
+  const float sc = __builtin_ldexpf(1.0f, sw);
+  const float* w = wt + item * wt_stride;
+  float f[3][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int z = 8 * zo + e;
+    float wr = 0.0f, wi = 0.0f;
+    if (z < nz) {
+      const float2 v = *reinterpret_cast<const float2*>(w + 2 * ((int64_t)z * ncol + y));
+      wr = v.x * sc;
+      wi = v.y * sc;
+    }
+    f[0][e] = wr;
+    f[1][e] = wi;
+    f[2][e] = wr + wi;
+  }
+  uint4* o = w16 + item * w16_stride;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    h16x8 hi, lo;
+    adj_split8(f[k], hi, lo);
+    o[((int64_t)(2 * k) * nzo + zo) * ncol + y] = __builtin_bit_cast(uint4, hi);
+    o[((int64_t)(2 * k + 1) * nzo + zo) * ncol + y] = __builtin_bit_cast(uint4, lo);
+  }
+}
+
+__device__ inline f32x16a mfma16a(h16x8 a, h16x8 b, f32x16a c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+template <bool CQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adjoint_product16(
+    const AdjItem* __restrict__ items, AdjItem single, int upi) {
+  // weights of one step: [buffer][form * 2 + half][z octet h][column]; per-z data of one step
+  __shared__ uint4 s_w[2][6][2][kAdjYG];
+  __shared__ float s_zb[2][16], s_zu[2][16], s_zfr[2][16], s_zfi[2][16], s_zab[2][16], s_zaz[2][16];
+  __shared__ float s_red[256];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int item = blockIdx.x / upi, ul = blockIdx.x % upi;
+  const AdjItem it = items ? items[item] : single;  // workgroup-uniform copy (scalar loads)
+  const int ncol = it.ncol, nz = it.nz, nzo = it.nzo;
+  const int nyg = (ncol + kAdjYG - 1) / kAdjYG;
+  const int blk = ul % it.nblocks;
+  const int yg = (ul / it.nblocks) % nyg;
+  const int fp = ul / (it.nblocks * nyg);  // 0: f = 1, 1: f = qz, 2: f = qz^2
+  const int r = lane & 31, h = lane >> 5;
+  const int64_t t0 = (int64_t)blk * kBlockTri + wave * kAdjTile;
+  const float dcz = it.rec32[8 * (t0 + r) + 5];
+  const double oz = it.borig[4 * blk + 2];
+  const int y0 = yg * kAdjYG;
+
+  // A scale 2^sa: |f(z)| att <= 2^(14 - sa) / sqrt(2) over all z (the Gauss sum Ar + Ai is up to
+  // sqrt(2) |A|), with att <= exp(|Im qz| (|oz| + max |c_z - oz|)) over the block's triangles
+  float czmax = 0.0f;
+  if (CQ) {
+    s_red[tid] = tid < kBlockTri ? fabsf(it.rec32[8 * ((int64_t)blk * kBlockTri + tid) + 5]) : 0.0f;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] = fmaxf(s_red[tid], s_red[tid + w]);
+      __syncthreads();
+    }
+    czmax = (float)fabs(oz) + s_red[0];
+    __syncthreads();
+  }
+  float lmax = -1e30f;  // log2 of |f(z)| att
+  for (int z = tid; z < nz; z += 256) {
+    const double qr = it.qz[2 * (int64_t)z * ncol], qi = CQ ? it.qz[2 * (int64_t)z * ncol + 1] : 0.0;
+    const double a = sqrt(qr * qr + qi * qi);
+    const float l = fp == 0 ? 0.0f : (a > 0.0 ? (float)(fp * log2(a)) : -1e30f);
+    lmax = fmaxf(lmax, l + (CQ ? (float)(fabs(qi) * 1.4426950408889634) * czmax : 0.0f));
+  }
+  s_red[tid] = lmax;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) s_red[tid] = fmaxf(s_red[tid], s_red[tid + w]);
+    __syncthreads();
+  }
+  const int sa = s_red[0] > -1e29f ? max(-100, min(100, 14 - (int)ceilf(s_red[0] + 0.5f))) : 0;
+
+  const int nks = (nz + 15) / 16;
+  // one step's staged data: 3 uint4 of weights per thread, one z entry for threads < 16
+  uint4 wv[3];
+  float zb = 0.f, zu = 0.f, zfr = 0.f, zfi = 0.f, zab = 0.f, zaz = 0.f;
+  auto fetch = [&](int ks) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = tid + 256 * q, fh = i / (2 * kAdjYG), hh = (i / kAdjYG) & 1, col = i % kAdjYG;
+      const int zo = 2 * ks + hh, y = y0 + col;
+      wv[q] = make_uint4(0, 0, 0, 0);
+      if (zo < nzo && y < ncol) wv[q] = it.w16[((int64_t)fh * nzo + zo) * ncol + y];
+    }
+    if (tid < 16) {
+      const int z = 16 * ks + tid;
+      zb = zu = zfr = zfi = zab = zaz = 0.0f;
+      if (z < nz) {
+        const double qr = it.qz[2 * (int64_t)z * ncol], qi = CQ ? it.qz[2 * (int64_t)z * ncol + 1] : 0.0;
+        const double tb = qr * oz * kInvTwoPi;
+        zb = (float)(tb - rint(tb));
+        zu = (float)(qr * kInvTwoPi);
+        double fr = 1.0, fi = 0.0;
+        for (int k = 0; k < fp; ++k) {
+          const double nr = fr * qr - fi * qi;
+          fi = fr * qi + fi * qr;
+          fr = nr;
+        }
+        zfr = (float)__builtin_ldexp(fr, sa);
+        zfi = (float)__builtin_ldexp(fi, sa);
+        zab = (float)(-qi * oz * 1.4426950408889634);
+        zaz = (float)(qi * 1.4426950408889634);
+      }
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = tid + 256 * q, fh = i / (2 * kAdjYG), hh = (i / kAdjYG) & 1, col = i % kAdjYG;
+      s_w[buf][fh][hh][col] = wv[q];
+    }
+    if (tid < 16) {
+      s_zb[buf][tid] = zb;
+      s_zu[buf][tid] = zu;
+      s_zfr[buf][tid] = zfr;
+      s_zfi[buf][tid] = zfi;
+      s_zab[buf][tid] = zab;
+      s_zaz[buf][tid] = zaz;
+    }
+  };
+  fetch(0);
+  stage(0);
+  __syncthreads();
+
+  f32x16a P[3][kAdjNY];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < kAdjNY; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) P[k][j][e] = 0.0f;
+  for (int ks = 0; ks < nks; ++ks) {
+    const int cur = ks & 1;
+    const bool more = ks + 1 < nks;
+    if (more) fetch(ks + 1);  // in flight during this step's work
+    // A operands: triangle t0 + r at z = 16 ks + 8 h + e
+    float ar[3][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int zl = 8 * h + e;
+      const float ph = __builtin_fmaf(s_zu[cur][zl], dcz, s_zb[cur][zl]);
+      float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+      if (CQ) {
+        const float att = __builtin_amdgcn_exp2f(__builtin_fmaf(-s_zaz[cur][zl], dcz, s_zab[cur][zl]));
+        c *= att;
+        sn *= att;
+      }
+      const float fr = s_zfr[cur][zl], fi = s_zfi[cur][zl];
+      const float re = fr * c - fi * sn, im = fr * sn + fi * c;
+      ar[0][e] = re;
+      ar[1][e] = im;
+      ar[2][e] = re + im;
+    }
+    h16x8 AH[3], AL[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) adj_split8(ar[k], AH[k], AL[k]);
+#pragma unroll
+    for (int j = 0; j < kAdjNY; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const h16x8 BH = __builtin_bit_cast(h16x8, s_w[cur][2 * k][h][kAdjTile * j + r]);
+        const h16x8 BL = __builtin_bit_cast(h16x8, s_w[cur][2 * k + 1][h][kAdjTile * j + r]);
+        P[k][j] = mfma16a(AH[k], BH, P[k][j]);
+        P[k][j] = mfma16a(AL[k], BH, P[k][j]);
+        P[k][j] = mfma16a(AH[k], BL, P[k][j]);
+      }
+    // the other buffer's last reads were in step ks - 1, which every wave finished before the
+    // barrier that ended it
+    if (more) stage(cur ^ 1);
+    __syncthreads();
+  }
+  // X[t][y] for this lane's column and 16 triangle rows of each tile; contract with g(y) Ey_t(y) over
+  // the tile's 32 columns
+  const double qx0 = it.qx[0];
+  const int nform = 3 - fp;
+  const int form0 = fp == 0 ? 0 : (fp == 1 ? 3 : 5);
+#pragma unroll
+  for (int j = 0; j < kAdjNY; ++j) {
+    const int ytile = 2 * yg + j;
+    if (ytile >= it.nyt) continue;  // wave-uniform
+    const int yv = y0 + kAdjTile * j + r;
+    const bool yok = yv < ncol;
+    const int yc = min(yv, ncol - 1);
+    const double qyv = it.qy[yc];
+    const double unscale = __builtin_ldexp(1.0, -(sa + it.wexp[yc]));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t trow = t0 + adj_acc_row(k, h);
+      const int64_t tc = min(trow, it.ntri - 1);
+      const double ph = (qx0 * it.rec[8 * tc + 3] + qyv * it.rec[8 * tc + 4]) * kInvTwoPi;
+      const float fph = (float)(ph - rint(ph));
+      const double eyr = (double)__builtin_amdgcn_cosf(fph), eyi = (double)__builtin_amdgcn_sinf(fph);
+      const double p1 = P[0][j][k], p2 = P[1][j][k], p3 = P[2][j][k];
+      const double xr = (p1 - p2) * unscale, xi = (p3 - p1 - p2) * unscale;
+      double yr = yok ? xr * eyr - xi * eyi : 0.0;
+      double yi = yok ? xr * eyi + xi * eyr : 0.0;
+      for (int f = 0; f < nform; ++f) {
+        double sr = yr, si = yi;
+#pragma unroll
+        for (int m = 16; m >= 1; m >>= 1) {
+          sr += __shfl_xor(sr, m, 32);
+          si += __shfl_xor(si, m, 32);
+        }
+        if (r == 0 && trow < it.ntri) {
+          double* o = it.part + 2 * (((int64_t)(form0 + f) * it.nyt + ytile) * it.ntri + trow);
+          o[0] = sr;
+          o[1] = si;
+        }
+        yr *= qyv;  // next form: one more power of qy
+        yi *= qyv;
+      }
+    }
+  }
+}
+
+static bool g_adjoint_f16 = true;
+
+bool adjoint_product_f16() { return g_adjoint_f16; }
+
+int64_t adjoint_product_units(const AdjItem& it, bool f16) {
+  if (f16) return 3 * (int64_t)it.nblocks * ceil_div(it.ncol, kAdjYG);
+  return 3 * (int64_t)it.nblocks * it.nyt;
+}
+
+int64_t adjoint_pmax_count(int nz, int ncol) { return (int64_t)ceil_div(nz, kAdjZMax) * ncol; }
+
+void adjoint_split_weights(hipStream_t s, int nitems, const float* wt, int64_t wt_stride, int nz, int ncol,
+                           float* pmax, int64_t pmax_stride, uint4* w16, int64_t w16_stride, int* wexp,
+                           int64_t wexp_stride) {
+  if (nitems <= 0) return;
+  dim3 gm(ceil_div(ncol, 256), ceil_div(nz, kAdjZMax), nitems);
+  k_adj_wmax<<<gm, 256, 0, s>>>(wt, wt_stride, nz, ncol, pmax, pmax_stride);
+  dim3 gs(ceil_div(ncol, 256), ceil_div(nz, 8), nitems);
+  k_adj_wsplit<<<gs, 256, 0, s>>>(wt, wt_stride, nz, ncol, pmax, pmax_stride, w16, w16_stride, wexp, wexp_stride);
+}
+
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
-                            bool cq) {
+                            bool cq, bool f16) {
   if (nunits <= 0) return;
+  if (f16) {
+    if (cq) k_adjoint_product16<true><<<nunits, 256, 0, s>>>(d_items, single, upi);
+    else k_adjoint_product16<false><<<nunits, 256, 0, s>>>(d_items, single, upi);
+    return;
+  }
   if (cq) k_adjoint_product32<true><<<nunits, 256, 0, s>>>(d_items, single, upi);
   else k_adjoint_product32<false><<<nunits, 256, 0, s>>>(d_items, single, upi);
 }
@@ -378,14 +699,25 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   const size_t wt_b = (size_t)c->nq * 2 * (precision == HGX_PREC_FP64 ? sizeof(double) : sizeof(float));
   const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
                                                    : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
+  // split-fp16 weights (FP32C with the fp16 matrix cores): column maxima, 12 B per q-point, column scales
+  const bool f16 = precision == HGX_PREC_FP32C && adjoint_product_f16();
+  const int nz = c->nbranch * c->nrow;
+  const int nzo = (int)ceil_div(nz, 8);
+  const size_t pm_b = f16 ? (size_t)adjoint_pmax_count(nz, c->ncol) * sizeof(float) : 0;
+  const size_t w16_b = f16 ? (size_t)6 * nzo * c->ncol * sizeof(uint4) : 0;
+  const size_t we_b = f16 ? (size_t)c->ncol * sizeof(int) : 0;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + al(part_b) + r32_b)) return rc;
+  if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + al(part_b) + al(r32_b) + al(pm_b) + al(w16_b) + we_b))
+    return rc;
   char* w = static_cast<char*>(c->d_work);
   double* rrec = reinterpret_cast<double*>(w);
   double* w0 = reinterpret_cast<double*>(w + al(rec_b));
   void* wt = w + al(rec_b) + w0_b;
   double* part = reinterpret_cast<double*>(w + al(rec_b) + w0_b + al(wt_b));
   char* r32 = w + al(rec_b) + w0_b + al(wt_b) + al(part_b);
+  float* pmax = reinterpret_cast<float*>(r32 + al(r32_b));
+  uint4* w16 = reinterpret_cast<uint4*>(r32 + al(r32_b) + al(pm_b));
+  int* wexp = reinterpret_cast<int*>(r32 + al(r32_b) + al(pm_b) + al(w16_b));
   k_adj_rotate<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(m.d_rec64, m.ntri, R, rrec);
   HGX_CHECK_LAUNCH(c);
   k_adj_q0<<<1, 256, 0, c->stream>>>(g, W, w0);
@@ -427,20 +759,33 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
     it.borig = ro;
   }
   it.part = part;
+  it.w16 = w16;
+  it.wexp = wexp;
   it.ntri = m.ntri;
   it.nblocks = (int)m.nblocks;
   it.ncol = c->ncol;
-  it.nz = c->nbranch * c->nrow;
+  it.nz = nz;
   it.nyt = nyt;
-  const double units = 3.0 * it.nyt * it.nblocks;
-  if (units > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
+  it.nzo = nzo;
+  const int64_t units = adjoint_product_units(it, f16);
+  if (units > 2000000000) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
+  if (f16) {
+    adjoint_split_weights(c->stream, 1, static_cast<const float*>(wt), 0, nz, c->ncol, pmax, 0, w16, 0, wexp, 0);
+    HGX_CHECK_LAUNCH(c);
+  }
   {
     LaunchTimer lt(c, true);
-    adjoint_product_launch(c->stream, nullptr, it, (int)units, (int)units, cq);
+    adjoint_product_launch(c->stream, nullptr, it, (int)units, (int)units, cq, f16);
   }
   HGX_CHECK_LAUNCH(c);
   k_adjoint_product_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nyt, m.ntri, rrec, c->d_qx, R, w0,
                                                                          m.d_src, grad);
   HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}
+
+extern "C" int hgx_tune_adjoint(int operands) {
+  if (operands != HGX_PRODUCT_FP32 && operands != HGX_PRODUCT_F16X2) return HGX_ERR_INVALID_ARG;
+  g_adjoint_f16 = operands == HGX_PRODUCT_F16X2;
   return HGX_OK;
 }

@@ -592,6 +592,7 @@ struct GratingPlan {
   int nchunks = 1, nyt = 0, ntt = 0, upi_fwd = 0, upi_adj = 0;
   int npb = 0, ngb = 0;  // pixel blocks, gradient blocks per item
   int pmode = 0;         // product-kernel variant of the forward pass, fixed at creation
+  bool adj_f16 = true;   // split-fp16 adjoint, fixed at creation
   bool cq = false;
   GState* d_state = nullptr;
   double* d_rec64 = nullptr;
@@ -601,6 +602,9 @@ struct GratingPlan {
   float* d_bmax = nullptr;
   double* d_fbuf = nullptr;
   float* d_wt = nullptr;
+  float* d_pmax = nullptr;
+  uint4* d_w16 = nullptr;
+  int* d_wexp = nullptr;
   double* d_apart = nullptr;
   double* d_gpart = nullptr;
   double* d_cpart = nullptr;
@@ -625,8 +629,8 @@ void free_plan(GratingPlan& p) {
   if (p.step_exec) (void)hipGraphExecDestroy(p.step_exec);
   if (p.eval_graph) (void)hipGraphDestroy(p.eval_graph);
   if (p.step_graph) (void)hipGraphDestroy(p.step_graph);
-  void* ptrs[] = {p.d_state, p.d_rec64, p.d_drec, p.d_rec32, p.d_borig, p.d_bmax, p.d_fbuf, p.d_wt,
-                  p.d_apart, p.d_gpart, p.d_cpart, p.d_pitems, p.d_punits, p.d_aitems};
+  void* ptrs[] = {p.d_state, p.d_rec64, p.d_drec, p.d_rec32, p.d_borig, p.d_bmax, p.d_fbuf, p.d_wt, p.d_pmax,
+                  p.d_w16, p.d_wexp, p.d_apart, p.d_gpart, p.d_cpart, p.d_pitems, p.d_punits, p.d_aitems};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p = GratingPlan();
@@ -648,7 +652,12 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
                                        want_I ? Iout : nullptr);
   if (want_I) return;
   AdjItem none{};
-  adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq);
+  if (p.adj_f16) {
+    const int nz = c->nbranch * c->nrow;
+    adjoint_split_weights(s, gsize, p.d_wt, 2 * c->nq, nz, c->ncol, p.d_pmax, adjoint_pmax_count(nz, c->ncol), p.d_w16,
+                          6 * (int64_t)ceil_div(nz, 8) * c->ncol, p.d_wexp, c->ncol);
+  }
+  adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq, p.adj_f16);
   dim3 ggrad(p.ngb, gsize);
   k_grating_grad<<<ggrad, 256, 0, s>>>(gm, p.d_apart, p.nyt, p.d_rec64, p.d_drec, c->d_qx, p.d_gpart);
   k_grating_reduce<<<ceil_div(gsize, 64), 64, 0, s>>>(p.d_state, cand0, gsize, p.d_cpart, p.npb, p.d_gpart, p.ngb);
@@ -750,12 +759,23 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   p.nyt = (int)ceil_div(c->ncol, 32);
   p.ntt = (int)(gm.nslots / 32);
   p.upi_fwd = nzt * nyt_f * p.nchunks;
-  p.upi_adj = 3 * gm.nblocks * p.nyt;
+  p.adj_f16 = adjoint_product_f16();
+  const int nzo = (int)ceil_div(nz, 8);
+  {
+    AdjItem shape{};
+    shape.nblocks = gm.nblocks;
+    shape.ncol = c->ncol;
+    shape.nyt = p.nyt;
+    const int64_t upi = adjoint_product_units(shape, p.adj_f16);
+    if (upi > 2000000000) return set_error(c, HGX_ERR_INVALID_ARG, "grating_plan_create: problem too large");
+    p.upi_adj = (int)upi;
+  }
   p.npb = (int)ceil_div(c->npix, 256);
   p.ngb = (int)ceil_div(gm.nslots, 256);
   // group size: per-candidate device memory within a 24 GiB budget
   const double per = (double)gm.nslots * (8 * 8 + 18 * 8 + 8 * 4) + (double)gm.nblocks * 40 +
                      (double)p.nchunks * c->nq * 16 + (double)c->nq * 8 + 6.0 * p.nyt * gm.nslots * 16 +
+                     (p.adj_f16 ? 96.0 * nzo * c->ncol + 4.0 * adjoint_pmax_count(nz, c->ncol) + 4.0 * c->ncol : 0.0) +
                      (double)(p.npb * 2 + p.ngb * 3) * 8;
   p.G = (int)std::max(1.0, std::min((double)ncand, 24.0 * (1 << 30) / per));
   p.ngroups = (ncand + p.G - 1) / p.G;
@@ -775,6 +795,11 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   if (hipMalloc(&p.d_bmax, sizeof(float) * 2 * gm.nblocks * G) != hipSuccess) return fail("block bounds");
   if (hipMalloc(&p.d_fbuf, sizeof(double) * 2 * p.nchunks * c->nq * G) != hipSuccess) return fail("sums");
   if (hipMalloc(&p.d_wt, sizeof(float) * 2 * c->nq * G) != hipSuccess) return fail("weights");
+  if (p.adj_f16) {
+    if (hipMalloc(&p.d_pmax, sizeof(float) * adjoint_pmax_count(nz, c->ncol) * G) != hipSuccess) return fail("weight maxima");
+    if (hipMalloc(&p.d_w16, sizeof(uint4) * 6 * (size_t)nzo * c->ncol * G) != hipSuccess) return fail("split weights");
+    if (hipMalloc(&p.d_wexp, sizeof(int) * (size_t)c->ncol * G) != hipSuccess) return fail("weight scales");
+  }
   if (hipMalloc(&p.d_apart, sizeof(double) * 2 * 6 * p.nyt * gm.nslots * G) != hipSuccess) return fail("adjoint");
   if (hipMalloc(&p.d_gpart, sizeof(double) * 3 * p.ngb * G) != hipSuccess) return fail("gradient partials");
   if (hipMalloc(&p.d_cpart, sizeof(double) * 2 * p.npb * G) != hipSuccess) return fail("chi2 partials");
@@ -810,6 +835,9 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
     a.ncol = c->ncol;
     a.nz = nz;
     a.nyt = p.nyt;
+    a.nzo = nzo;
+    a.w16 = p.adj_f16 ? p.d_w16 + (int64_t)i * 6 * nzo * c->ncol : nullptr;
+    a.wexp = p.adj_f16 ? p.d_wexp + (int64_t)i * c->ncol : nullptr;
   }
   // forward units, item-major so a short last group runs a prefix
   std::vector<ProductUnit> units;

@@ -123,9 +123,12 @@ struct AdjItem {
   const float* rec32;   // [nblocks * kBlockTri][8] rotated FP32 records (A', c' - o_b)
   const double* borig;  // [nblocks][4] rotated block origins
   double* part;         // [6 forms][nyt][ntri] complex: B11, B1y, B1yy, Bz1, Bzy, Bzz per y-tile
+  const uint4* w16;     // split-fp16 weights: [3 forms][2 halves][nzo][ncol] of 8 fp16 (one z octet, one column)
+  const int* wexp;      // [ncol] power-of-two column scales of w16
   int64_t ntri;
   int nblocks;
   int ncol, nz, nyt;
+  int nzo;              // ceil(nz / 8)
 };
 
 constexpr int kProductTile = 32;
@@ -314,8 +317,17 @@ void product_launch_units(hipStream_t s, const ProductItem* d_items, const Produ
 void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, int mode);
 // Launches the product-grid adjoint over items (d_items == NULL: the single item by value);
 // upi workgroup units per item (3 f-passes x triangle blocks x y-tiles).
+// Split-fp16 form of the adjoint weights for nitems items (strides in elements): per-column maxima
+// into pmax ([nitems][ceil(nz / 256)][ncol] floats), then w16 and wexp as AdjItem describes.
+void adjoint_split_weights(hipStream_t s, int nitems, const float* wt, int64_t wt_stride, int nz, int ncol,
+                           float* pmax, int64_t pmax_stride, uint4* w16, int64_t w16_stride, int* wexp,
+                           int64_t wexp_stride);
+int64_t adjoint_pmax_count(int nz, int ncol);  // floats of pmax per item
+// work units per item of the product-grid adjoint for the current operand setting
+int64_t adjoint_product_units(const AdjItem& it, bool f16);
+bool adjoint_product_f16();  // process-wide operand setting (hgx_tune_adjoint)
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
-                            bool cq);
+                            bool cq, bool f16);
 // Tuning: triangle blocks per work unit and the kernel variant (kProduct* below).
 int product_blocks_per_unit();
 int product_mode();

@@ -368,6 +368,12 @@ int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
  * autotuner; applies to single calls and to batches created afterwards. No reference counterpart. */
 int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows);
 
+/* Operands of the FP32C product-grid record adjoint (hgx_ff_numeric_adjoint and the grating fit):
+ * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16) or HGX_PRODUCT_FP32
+ * (v_mfma_f32_32x32x2_f32). Process-wide; read by each adjoint call and at grating plan creation.
+ * No reference counterpart. */
+int hgx_tune_adjoint(int operands);
+
 #ifdef __cplusplus
 }
 #endif

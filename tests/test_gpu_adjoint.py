@@ -68,14 +68,16 @@ def test_adjoint_fp64_detector_grid(oracle_mod):
         c.close()
 
 
+@pytest.mark.parametrize("operands", ["f16x2", "fp32"])
 @pytest.mark.parametrize("ncol,nrow,dwba,rotated", [
     (40, 23, None, False),
     (37, 33, 3e-6 + 2e-8j, True),
+    (130, 70, 0j, True),  # real branches; columns past two 64-column groups, rows past a 16-row step
 ])
-def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated):
+def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated, operands):
     import torch
 
-    from hipgisaxs_amd.api import Context
+    from hipgisaxs_amd.api import Context, tune_adjoint
 
     O = oracle_mod
     rng = np.random.default_rng(ncol + nrow)
@@ -99,11 +101,13 @@ def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated):
         Wd = torch.from_numpy(np.stack([W.real, W.imag], -1).ravel()).cuda()
         ref = _oracle_grad(O, tri, g, W, R)
         got64 = c.ff_numeric_adjoint(mid, Wd, "fp64", rot=R).cpu().numpy()
+        tune_adjoint(operands)
         got32 = c.ff_numeric_adjoint(mid, Wd, "fp32c", rot=R).cpu().numpy()
         e64, e32 = rel_l2(got64, ref), rel_l2(got32, ref)
-        print(f"{ncol}x{nrow} dwba={dwba} rot={rotated}: fp64 {e64:.2e} fp32c {e32:.2e}")
+        print(f"{ncol}x{nrow} dwba={dwba} rot={rotated} {operands}: fp64 {e64:.2e} fp32c {e32:.2e}")
         assert np.all(got32[5] == 0.0)
         assert e64 <= 1e-10
         assert e32 <= 1e-5
     finally:
+        tune_adjoint("f16x2")
         c.close()
