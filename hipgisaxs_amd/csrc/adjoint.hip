@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __
 constexpr int kAdjZMax = 256;   // z rows per partial column maximum
 constexpr int kAdjNY = 2;       // 32-column tiles per wave
 constexpr int kAdjYG = kAdjTile * kAdjNY;
+constexpr int kAdjZTab = 256;   // z values per per-z table (one per thread)
 
 __global__ __launch_bounds__(256) void k_adj_wmax(const float* __restrict__ wt, int64_t wt_stride, int nz, int ncol,
                                                   float* __restrict__ pmax, int64_t pmax_stride) {
@@ -384,17 +385,25 @@ typedef float fx2 __attribute__((ext_vector_type(2)));
 
 __device__ inline h16x2 a16pk(float a, float b) { return __builtin_convertvector((fx2){a, b}, h16x2); }
 
-// 8 FP32 values to their high and low fp16 halves
-__device__ inline void adj_split8(const float (&x)[8], h16x8& hi, h16x8& lo) {
+__device__ inline h16x8 a16cat(h16x2 a, h16x2 b, h16x2 c, h16x2 d) {
+  typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+  const h16x4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+  const h16x4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
+  return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// 8 FP32 values to their high and low fp16 halves (x - x_h is exact in FP32). neg1 holds -1.0 as a
+// value the compiler cannot see, so x + (-1) x_h stays one mixed-precision FMA (v_fma_mix_f32)
+// instead of a conversion and a subtraction.
+__device__ inline void adj_split8(const float (&x)[8], float neg1, h16x8& hi, h16x8& lo) {
+  h16x2 hh[4], ll[4];
 #pragma unroll
-  for (int e = 0; e < 8; e += 2) {
-    const h16x2 hh = a16pk(x[e], x[e + 1]);
-    const h16x2 ll = a16pk(x[e] - (float)hh[0], x[e + 1] - (float)hh[1]);
-    hi[e] = hh[0];
-    hi[e + 1] = hh[1];
-    lo[e] = ll[0];
-    lo[e + 1] = ll[1];
+  for (int e = 0; e < 4; ++e) {
+    hh[e] = a16pk(x[2 * e], x[2 * e + 1]);
+    ll[e] = a16pk(__builtin_fmaf((float)hh[e][0], neg1, x[2 * e]), __builtin_fmaf((float)hh[e][1], neg1, x[2 * e + 1]));
   }
+  hi = a16cat(hh[0], hh[1], hh[2], hh[3]);
+  lo = a16cat(ll[0], ll[1], ll[2], ll[3]);
 }
 
 __global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt, int64_t wt_stride, int nz, int ncol,
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     h16x8 hi, lo;
-    adj_split8(f[k], hi, lo);
+    adj_split8(f[k], -1.0f, hi, lo);
     o[((int64_t)(2 * k) * nzo + zo) * ncol + y] = __builtin_bit_cast(uint4, hi);
     o[((int64_t)(2 * k + 1) * nzo + zo) * ncol + y] = __builtin_bit_cast(uint4, lo);
   }
@@ -441,9 +450,10 @@ __device__ inline f32x16a mfma16a(h16x8 a, h16x8 b, f32x16a c) {
 template <bool CQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adjoint_product16(
     const AdjItem* __restrict__ items, AdjItem single, int upi) {
-  // weights of one step: [buffer][form * 2 + half][z octet h][column]; per-z data of one step
+  // weights of one step: [buffer][form * 2 + half][z octet h][column]; per-z data of 256 z
   __shared__ uint4 s_w[2][6][2][kAdjYG];
-  __shared__ float s_zb[2][16], s_zu[2][16], s_zfr[2][16], s_zfi[2][16], s_zab[2][16], s_zaz[2][16];
+  __shared__ float s_zb[2][kAdjZTab], s_zu[2][kAdjZTab], s_zfr[2][kAdjZTab], s_zfi[2][kAdjZTab];
+  __shared__ float s_zab[2][CQ ? kAdjZTab : 1], s_zaz[2][CQ ? kAdjZTab : 1];
   __shared__ float s_red[256];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
@@ -489,9 +499,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
   const int sa = s_red[0] > -1e29f ? max(-100, min(100, 14 - (int)ceilf(s_red[0] + 0.5f))) : 0;
 
   const int nks = (nz + 15) / 16;
-  // one step's staged data: 3 uint4 of weights per thread, one z entry for threads < 16
-  uint4 wv[3];
+  float neg1;
+  asm volatile("v_mov_b32 %0, -1.0" : "=v"(neg1));
+  // Per-z data of 256 z (16 steps) at a time, one entry per thread, double-buffered: base phase in
+  // turns (FP64-reduced), qz / 2pi, f(z) 2^sa, and the attenuation terms (complex qz).
   float zb = 0.f, zu = 0.f, zfr = 0.f, zfi = 0.f, zab = 0.f, zaz = 0.f;
+  auto ztable = [&](int chunk) {
+    const int z = chunk * kAdjZTab + tid;
+    zb = zu = zfr = zfi = zab = zaz = 0.0f;
+    if (z < nz) {
+      const double qr = it.qz[2 * (int64_t)z * ncol], qi = CQ ? it.qz[2 * (int64_t)z * ncol + 1] : 0.0;
+      const double tb = qr * oz * kInvTwoPi;
+      zb = (float)(tb - rint(tb));
+      zu = (float)(qr * kInvTwoPi);
+      double fr = 1.0, fi = 0.0;
+      if (fp == 1) {
+        fr = qr;
+        fi = qi;
+      } else if (fp == 2) {
+        fr = qr * qr - qi * qi;
+        fi = 2.0 * qr * qi;
+      }
+      zfr = (float)__builtin_ldexp(fr, sa);
+      zfi = (float)__builtin_ldexp(fi, sa);
+      zab = (float)(-qi * oz * 1.4426950408889634);
+      zaz = (float)(qi * 1.4426950408889634);
+    }
+  };
+  auto zstage = [&](int buf) {
+    s_zb[buf][tid] = zb;
+    s_zu[buf][tid] = zu;
+    s_zfr[buf][tid] = zfr;
+    s_zfi[buf][tid] = zfi;
+    if (CQ) {
+      s_zab[buf][tid] = zab;
+      s_zaz[buf][tid] = zaz;
+    }
+  };
+  // one step's weights: 3 uint4 per thread
+  uint4 wv[3];
   auto fetch = [&](int ks) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -500,26 +546,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
       wv[q] = make_uint4(0, 0, 0, 0);
       if (zo < nzo && y < ncol) wv[q] = it.w16[((int64_t)fh * nzo + zo) * ncol + y];
     }
-    if (tid < 16) {
-      const int z = 16 * ks + tid;
-      zb = zu = zfr = zfi = zab = zaz = 0.0f;
-      if (z < nz) {
-        const double qr = it.qz[2 * (int64_t)z * ncol], qi = CQ ? it.qz[2 * (int64_t)z * ncol + 1] : 0.0;
-        const double tb = qr * oz * kInvTwoPi;
-        zb = (float)(tb - rint(tb));
-        zu = (float)(qr * kInvTwoPi);
-        double fr = 1.0, fi = 0.0;
-        for (int k = 0; k < fp; ++k) {
-          const double nr = fr * qr - fi * qi;
-          fi = fr * qi + fi * qr;
-          fr = nr;
-        }
-        zfr = (float)__builtin_ldexp(fr, sa);
-        zfi = (float)__builtin_ldexp(fi, sa);
-        zab = (float)(-qi * oz * 1.4426950408889634);
-        zaz = (float)(qi * 1.4426950408889634);
-      }
-    }
   };
   auto stage = [&](int buf) {
 #pragma unroll
@@ -527,15 +553,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
       const int i = tid + 256 * q, fh = i / (2 * kAdjYG), hh = (i / kAdjYG) & 1, col = i % kAdjYG;
       s_w[buf][fh][hh][col] = wv[q];
     }
-    if (tid < 16) {
-      s_zb[buf][tid] = zb;
-      s_zu[buf][tid] = zu;
-      s_zfr[buf][tid] = zfr;
-      s_zfi[buf][tid] = zfi;
-      s_zab[buf][tid] = zab;
-      s_zaz[buf][tid] = zaz;
-    }
   };
+  ztable(0);
+  zstage(0);
   fetch(0);
   stage(0);
   __syncthreads();
@@ -547,31 +567,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
     for (int j = 0; j < kAdjNY; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) P[k][j][e] = 0.0f;
+  const int nzt = (nz + kAdjZTab - 1) / kAdjZTab;
   for (int ks = 0; ks < nks; ++ks) {
     const int cur = ks & 1;
     const bool more = ks + 1 < nks;
+    const int chunk = ks / (kAdjZTab / 16), zc = chunk & 1;
+    const bool newz = (ks % (kAdjZTab / 16)) == 0 && chunk + 1 < nzt;
     if (more) fetch(ks + 1);  // in flight during this step's work
+    if (newz) ztable(chunk + 1);
     // A operands: triangle t0 + r at z = 16 ks + 8 h + e
     float ar[3][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int zl = 8 * h + e;
-      const float ph = __builtin_fmaf(s_zu[cur][zl], dcz, s_zb[cur][zl]);
+      const int zl = (ks % (kAdjZTab / 16)) * 16 + 8 * h + e;
+      const float ph = __builtin_fmaf(s_zu[zc][zl], dcz, s_zb[zc][zl]);
       float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
       if (CQ) {
-        const float att = __builtin_amdgcn_exp2f(__builtin_fmaf(-s_zaz[cur][zl], dcz, s_zab[cur][zl]));
+        const float att = __builtin_amdgcn_exp2f(__builtin_fmaf(-s_zaz[zc][zl], dcz, s_zab[zc][zl]));
         c *= att;
         sn *= att;
       }
-      const float fr = s_zfr[cur][zl], fi = s_zfi[cur][zl];
-      const float re = fr * c - fi * sn, im = fr * sn + fi * c;
+      // f(z) = qz^p is real on real branches
+      const float fr = s_zfr[zc][zl], fi = CQ ? s_zfi[zc][zl] : 0.0f;
+      const float re = CQ ? fr * c - fi * sn : fr * c, im = CQ ? fr * sn + fi * c : fr * sn;
       ar[0][e] = re;
       ar[1][e] = im;
       ar[2][e] = re + im;
     }
     h16x8 AH[3], AL[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) adj_split8(ar[k], AH[k], AL[k]);
+    for (int k = 0; k < 3; ++k) adj_split8(ar[k], neg1, AH[k], AL[k]);
 #pragma unroll
     for (int j = 0; j < kAdjNY; ++j)
 #pragma unroll
@@ -582,9 +607,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
         P[k][j] = mfma16a(AL[k], BH, P[k][j]);
         P[k][j] = mfma16a(AH[k], BL, P[k][j]);
       }
-    // the other buffer's last reads were in step ks - 1, which every wave finished before the
-    // barrier that ended it
+    // the other buffers' last reads were in the previous step (weights) or chunk (per-z data), which
+    // every wave finished before the barrier that ended it
     if (more) stage(cur ^ 1);
+    if (newz) zstage(zc ^ 1);
     __syncthreads();
   }
   // X[t][y] for this lane's column and 16 triangle rows of each tile; contract with g(y) Ey_t(y) over
