@@ -151,7 +151,7 @@ _SIGS = {
     "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product_tile": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "hgx_tune_adjoint": (C.c_int, [C.c_int]),
+    "hgx_tune_adjoint": (C.c_int, [C.c_int, C.c_int]),
     "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_grating_set_theta": (C.c_int, [_vp, C.c_int, _vp]),
@@ -171,6 +171,10 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hipgisaxs_amd.build`")
+    # PyTorch ships its own HIP runtime; it must be the process's only one, so it is loaded before
+    # libhgx.so, whose libamdhip64.so.7 dependency then binds to it (loading libhgx.so first would
+    # bring in a second runtime and the device calls of both would fail)
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

@@ -334,11 +334,12 @@ def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "
         raise HgxError("tune_product: invalid row tiles / accumulation / sharing")
 
 
-def tune_adjoint(operands: str = "f16x2") -> None:
-    """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default) or "fp32"."""
+def tune_adjoint(operands: str = "f16x2", col_tiles: int = 2) -> None:
+    """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default; 2 or 4
+    32-column tiles per wave) or "fp32"."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
-    if operands not in ops or N.load().hgx_tune_adjoint(ops[operands]) != N.HGX_OK:
-        raise HgxError("tune_adjoint: invalid operands")
+    if operands not in ops or N.load().hgx_tune_adjoint(ops[operands], int(col_tiles)) != N.HGX_OK:
+        raise HgxError("tune_adjoint: invalid operands / column tiles")
 
 
 class FitPlan:

@@ -354,8 +354,7 @@ __global__ __launch_bounds__(256) void k_adjoint_product_finish(const double* __
 // so workgroups running together share the same weights in L2.
 
 constexpr int kAdjZMax = 256;   // z rows per partial column maximum
-constexpr int kAdjNY = 2;       // 32-column tiles per wave
-constexpr int kAdjYG = kAdjTile * kAdjNY;
+// NY: 32-column tiles per wave (2 at 2 waves per SIMD, or 4 at 1 with the accumulators in AGPRs)
 constexpr int kAdjZTab = 256;   // z values per per-z table (one per thread)
 
 __global__ __launch_bounds__(256) void k_adj_wmax(const float* __restrict__ wt, int64_t wt_stride, int nz, int ncol,
@@ -447,9 +446,11 @@ __device__ inline f32x16a mfma16a(h16x8 a, h16x8 b, f32x16a c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
-template <bool CQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adjoint_product16(
+template <bool CQ, int NY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 : 2, 8))) void k_adjoint_product16(
     const AdjItem* __restrict__ items, AdjItem single, int upi) {
+  constexpr int kAdjNY = NY;
+  constexpr int kAdjYG = kAdjTile * NY;
   // weights of one step: [buffer][form * 2 + half][z octet h][column]; per-z data of 256 z
   __shared__ uint4 s_w[2][6][2][kAdjYG];
   __shared__ float s_zb[2][kAdjZTab], s_zu[2][kAdjZTab], s_zfr[2][kAdjZTab], s_zfi[2][kAdjZTab];
@@ -536,11 +537,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
       s_zaz[buf][tid] = zaz;
     }
   };
-  // one step's weights: 3 uint4 per thread
-  uint4 wv[3];
+  // one step's weights: 6 forms-halves x 2 z octets x kAdjYG columns, kWPT uint4 per thread
+  constexpr int kWPT = 12 * kAdjYG / 256;
+  uint4 wv[kWPT];
   auto fetch = [&](int ks) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < kWPT; ++q) {
       const int i = tid + 256 * q, fh = i / (2 * kAdjYG), hh = (i / kAdjYG) & 1, col = i % kAdjYG;
       const int zo = 2 * ks + hh, y = y0 + col;
       wv[q] = make_uint4(0, 0, 0, 0);
@@ -549,7 +551,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < kWPT; ++q) {
       const int i = tid + 256 * q, fh = i / (2 * kAdjYG), hh = (i / kAdjYG) & 1, col = i % kAdjYG;
       s_w[buf][fh][hh][col] = wv[q];
     }
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
   const int form0 = fp == 0 ? 0 : (fp == 1 ? 3 : 5);
 #pragma unroll
   for (int j = 0; j < kAdjNY; ++j) {
-    const int ytile = 2 * yg + j;
+    const int ytile = kAdjNY * yg + j;
     if (ytile >= it.nyt) continue;  // wave-uniform
     const int yv = y0 + kAdjTile * j + r;
     const bool yok = yv < ncol;
@@ -657,12 +659,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
   }
 }
 
-static bool g_adjoint_f16 = true;
+static int g_adjoint_variant = 2;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
 
-bool adjoint_product_f16() { return g_adjoint_f16; }
+int adjoint_product_variant() { return g_adjoint_variant; }
 
-int64_t adjoint_product_units(const AdjItem& it, bool f16) {
-  if (f16) return 3 * (int64_t)it.nblocks * ceil_div(it.ncol, kAdjYG);
+int64_t adjoint_product_units(const AdjItem& it, int variant) {
+  if (variant > 0) return 3 * (int64_t)it.nblocks * ceil_div(it.ncol, kAdjTile * variant);
   return 3 * (int64_t)it.nblocks * it.nyt;
 }
 
@@ -679,11 +681,16 @@ void adjoint_split_weights(hipStream_t s, int nitems, const float* wt, int64_t w
 }
 
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
-                            bool cq, bool f16) {
+                            bool cq, int variant) {
   if (nunits <= 0) return;
-  if (f16) {
-    if (cq) k_adjoint_product16<true><<<nunits, 256, 0, s>>>(d_items, single, upi);
-    else k_adjoint_product16<false><<<nunits, 256, 0, s>>>(d_items, single, upi);
+  if (variant == 4) {
+    if (cq) k_adjoint_product16<true, 4><<<nunits, 256, 0, s>>>(d_items, single, upi);
+    else k_adjoint_product16<false, 4><<<nunits, 256, 0, s>>>(d_items, single, upi);
+    return;
+  }
+  if (variant == 2) {
+    if (cq) k_adjoint_product16<true, 2><<<nunits, 256, 0, s>>>(d_items, single, upi);
+    else k_adjoint_product16<false, 2><<<nunits, 256, 0, s>>>(d_items, single, upi);
     return;
   }
   if (cq) k_adjoint_product32<true><<<nunits, 256, 0, s>>>(d_items, single, upi);
@@ -726,7 +733,8 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
                                                    : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
   // split-fp16 weights (FP32C with the fp16 matrix cores): column maxima, 12 B per q-point, column scales
-  const bool f16 = precision == HGX_PREC_FP32C && adjoint_product_f16();
+  const int variant = precision == HGX_PREC_FP32C ? adjoint_product_variant() : 0;
+  const bool f16 = variant > 0;
   const int nz = c->nbranch * c->nrow;
   const int nzo = (int)ceil_div(nz, 8);
   const size_t pm_b = f16 ? (size_t)adjoint_pmax_count(nz, c->ncol) * sizeof(float) : 0;
@@ -793,7 +801,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   it.nz = nz;
   it.nyt = nyt;
   it.nzo = nzo;
-  const int64_t units = adjoint_product_units(it, f16);
+  const int64_t units = adjoint_product_units(it, variant);
   if (units > 2000000000) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: problem too large");
   if (f16) {
     adjoint_split_weights(c->stream, 1, static_cast<const float*>(wt), 0, nz, c->ncol, pmax, 0, w16, 0, wexp, 0);
@@ -801,7 +809,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   }
   {
     LaunchTimer lt(c, true);
-    adjoint_product_launch(c->stream, nullptr, it, (int)units, (int)units, cq, f16);
+    adjoint_product_launch(c->stream, nullptr, it, (int)units, (int)units, cq, variant);
   }
   HGX_CHECK_LAUNCH(c);
   k_adjoint_product_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nyt, m.ntri, rrec, c->d_qx, R, w0,
@@ -810,8 +818,9 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   return HGX_OK;
 }
 
-extern "C" int hgx_tune_adjoint(int operands) {
+extern "C" int hgx_tune_adjoint(int operands, int col_tiles) {
   if (operands != HGX_PRODUCT_FP32 && operands != HGX_PRODUCT_F16X2) return HGX_ERR_INVALID_ARG;
-  g_adjoint_f16 = operands == HGX_PRODUCT_F16X2;
+  if (operands == HGX_PRODUCT_F16X2 && col_tiles != 2 && col_tiles != 4) return HGX_ERR_INVALID_ARG;
+  g_adjoint_variant = operands == HGX_PRODUCT_F16X2 ? col_tiles : 0;
   return HGX_OK;
 }

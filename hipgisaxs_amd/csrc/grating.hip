@@ -592,7 +592,8 @@ struct GratingPlan {
   int nchunks = 1, nyt = 0, ntt = 0, upi_fwd = 0, upi_adj = 0;
   int npb = 0, ngb = 0;  // pixel blocks, gradient blocks per item
   int pmode = 0;         // product-kernel variant of the forward pass, fixed at creation
-  bool adj_f16 = true;   // split-fp16 adjoint, fixed at creation
+  int adj_variant = 2;   // adjoint kernel variant (0 FP32, 2/4 split fp16), fixed at creation
+  bool adj_f16 = true;
   bool cq = false;
   GState* d_state = nullptr;
   double* d_rec64 = nullptr;
@@ -657,7 +658,7 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
     adjoint_split_weights(s, gsize, p.d_wt, 2 * c->nq, nz, c->ncol, p.d_pmax, adjoint_pmax_count(nz, c->ncol), p.d_w16,
                           6 * (int64_t)ceil_div(nz, 8) * c->ncol, p.d_wexp, c->ncol);
   }
-  adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq, p.adj_f16);
+  adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq, p.adj_variant);
   dim3 ggrad(p.ngb, gsize);
   k_grating_grad<<<ggrad, 256, 0, s>>>(gm, p.d_apart, p.nyt, p.d_rec64, p.d_drec, c->d_qx, p.d_gpart);
   k_grating_reduce<<<ceil_div(gsize, 64), 64, 0, s>>>(p.d_state, cand0, gsize, p.d_cpart, p.npb, p.d_gpart, p.ngb);
@@ -759,14 +760,15 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   p.nyt = (int)ceil_div(c->ncol, 32);
   p.ntt = (int)(gm.nslots / 32);
   p.upi_fwd = nzt * nyt_f * p.nchunks;
-  p.adj_f16 = adjoint_product_f16();
+  p.adj_variant = adjoint_product_variant();
+  p.adj_f16 = p.adj_variant > 0;
   const int nzo = (int)ceil_div(nz, 8);
   {
     AdjItem shape{};
     shape.nblocks = gm.nblocks;
     shape.ncol = c->ncol;
     shape.nyt = p.nyt;
-    const int64_t upi = adjoint_product_units(shape, p.adj_f16);
+    const int64_t upi = adjoint_product_units(shape, p.adj_variant);
     if (upi > 2000000000) return set_error(c, HGX_ERR_INVALID_ARG, "grating_plan_create: problem too large");
     p.upi_adj = (int)upi;
   }

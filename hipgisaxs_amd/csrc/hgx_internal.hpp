@@ -324,10 +324,11 @@ void adjoint_split_weights(hipStream_t s, int nitems, const float* wt, int64_t w
                            int64_t wexp_stride);
 int64_t adjoint_pmax_count(int nz, int ncol);  // floats of pmax per item
 // work units per item of the product-grid adjoint for the current operand setting
-int64_t adjoint_product_units(const AdjItem& it, bool f16);
-bool adjoint_product_f16();  // process-wide operand setting (hgx_tune_adjoint)
+// adjoint kernel variant: 0 FP32 operands, 2 or 4 split fp16 with that many column tiles per wave
+int64_t adjoint_product_units(const AdjItem& it, int variant);
+int adjoint_product_variant();  // process-wide setting (hgx_tune_adjoint)
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
-                            bool cq, bool f16);
+                            bool cq, int variant);
 // Tuning: triangle blocks per work unit and the kernel variant (kProduct* below).
 int product_blocks_per_unit();
 int product_mode();

@@ -369,10 +369,10 @@ int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
 int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows);
 
 /* Operands of the FP32C product-grid record adjoint (hgx_ff_numeric_adjoint and the grating fit):
- * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16) or HGX_PRODUCT_FP32
- * (v_mfma_f32_32x32x2_f32). Process-wide; read by each adjoint call and at grating plan creation.
- * No reference counterpart. */
-int hgx_tune_adjoint(int operands);
+ * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16) with col_tiles = 2 (default) or 4
+ * 32-column tiles per wave, or HGX_PRODUCT_FP32 (v_mfma_f32_32x32x2_f32; col_tiles ignored).
+ * Process-wide; read by each adjoint call and at grating plan creation. No reference counterpart. */
+int hgx_tune_adjoint(int operands, int col_tiles);
 
 #ifdef __cplusplus
 }

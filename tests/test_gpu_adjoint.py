@@ -68,7 +68,7 @@ def test_adjoint_fp64_detector_grid(oracle_mod):
         c.close()
 
 
-@pytest.mark.parametrize("operands", ["f16x2", "fp32"])
+@pytest.mark.parametrize("operands", ["f16x2", "f16x2/4", "fp32"])
 @pytest.mark.parametrize("ncol,nrow,dwba,rotated", [
     (40, 23, None, False),
     (37, 33, 3e-6 + 2e-8j, True),
@@ -101,7 +101,7 @@ def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated, opera
         Wd = torch.from_numpy(np.stack([W.real, W.imag], -1).ravel()).cuda()
         ref = _oracle_grad(O, tri, g, W, R)
         got64 = c.ff_numeric_adjoint(mid, Wd, "fp64", rot=R).cpu().numpy()
-        tune_adjoint(operands)
+        tune_adjoint(operands.split("/")[0], int(operands.split("/")[1]) if "/" in operands else 2)
         got32 = c.ff_numeric_adjoint(mid, Wd, "fp32c", rot=R).cpu().numpy()
         e64, e32 = rel_l2(got64, ref), rel_l2(got32, ref)
         print(f"{ncol}x{nrow} dwba={dwba} rot={rotated} {operands}: fp64 {e64:.2e} fp32c {e32:.2e}")
