@@ -161,12 +161,33 @@ def bench_pyramid_fp64(reps=5):
     ff = ctx.ff_numeric(mid, "approx", "fp64")
     I = ctx.dwba_intensity(ff, fc=fcd)
     torch.cuda.synchronize()
+    ctx.profile_begin()
     t0 = time.perf_counter()
     for _ in range(reps):
         ctx.ff_numeric(mid, "approx", "fp64", out=ff)
         ctx.dwba_intensity(ff, fc=fcd, out=I)
     torch.cuda.synchronize()
     t_gpu = (time.perf_counter() - t0) / reps
+    k_approx = ctx.profile_end()
+    # the exact (Green's theorem) triangle form factor of the same mesh and grid, FP64
+    ffe = ctx.ff_numeric(mid, "exact", "fp64")
+    torch.cuda.synchronize()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ff_numeric(mid, "exact", "fp64", out=ffe)
+    torch.cuda.synchronize()
+    t_exact = (time.perf_counter() - t0) / reps
+    k_exact = ctx.profile_end()
+    # parity on every 8th row (the oracle's exact method is slow), with the same degenerate-case
+    # threshold as the GPU default (the reference GPU path's 1e-13)
+    rows = np.arange(0, 4 * 256, 8)
+    gs = O.Grid(256, len(rows), np.repeat(g.qx[:256][None], len(rows), 0).ravel(),
+                np.repeat(g.qy[:256][None], len(rows), 0).ravel(), g.qz.reshape(4 * 256, 256)[rows].ravel(),
+                g.alpha_f)
+    E_ref = O.ff_exact(tri, gs, tiny=1e-13)
+    E_gpu = ctx.as_complex(ffe).cpu().numpy().reshape(4 * 256, 256)[rows].ravel()
+    rel_exact = float(np.linalg.norm(E_gpu - E_ref) / np.linalg.norm(E_ref))
     t0 = time.perf_counter()
     F_ref = O.ff_approx(O.tri9_to_rec8(tri), g)
     I_ref = O.dwba_intensity(g.npix, 4, fc, None, F_ref)
@@ -174,10 +195,21 @@ def bench_pyramid_fp64(reps=5):
     got = I.cpu().numpy()
     pairs = 4 * 256 * 256 * tri.shape[0]
     ctx.close()
+    ka_ms = k_approx["main_ms"] / max(k_approx["main_launches"], 1)
+    ke_ms = k_exact["main_ms"] / max(k_exact["main_launches"], 1)
+    # FP64 VALU roofline of the form-factor kernels: algorithmic flop per pair (approximate 33, exact
+    # about 200; transcendentals not counted) over the kernel's own time, against 78.6 TFLOP/s
+    roof = lambda fpp, ms: {"bound": "fp64-valu", "achieved": fpp * pairs / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                            "unit": "TFLOP/s", "frac": fpp * pairs / (ms * 1e-3) / PEAK_FP64, "flop_per_pair": fpp,
+                            "avg_launch_ms": ms}
     return {"gpu_ms": 1e3 * t_gpu, "cpu_ms": 1e3 * t_cpu, "gpu_pairs_per_s": pairs / t_gpu,
             "cpu_pairs_per_s": pairs / t_cpu, "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
             "relL2_intensity_gpu_vs_oracle": float(np.linalg.norm(got - I_ref) / np.linalg.norm(I_ref)),
-            "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA"}
+            "roofline_approx_kernel": roof(FLOP_PER_PAIR, ka_ms),
+            "exact_method": {"gpu_ms": 1e3 * t_exact, "gpu_pairs_per_s": pairs / t_exact,
+                             "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms)},
+            "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA; "
+                        "exact-triangle FF of the same mesh and grid"}
 
 
 def bench_lattice(reps=5):
@@ -217,6 +249,14 @@ def bench_lattice(reps=5):
         res[prec] = I
     d = torch.linalg.vector_norm(res["fp32c"] - res["fp64"]) / torch.linalg.vector_norm(res["fp64"])
     out["relL2_fp32c_vs_fp64"] = float(d)
+    # VALU roofline: per (q, node, grain) sample the sphere form factor (about 55 flop), per (q, grain)
+    # the lattice factor (about 155 flop) and the DWBA term (about 15 flop per branch); transcendentals
+    # not counted
+    flop_step = 4 * 1024 * 1024 * 8 * (v.size * 55 + 155 + 15)
+    for prec, peak in (("fp32c", PEAK_FP32), ("fp64", PEAK_FP64)):
+        a = flop_step / (out[prec]["kernel_ms"] * 1e-3)
+        out[prec]["roofline"] = {"bound": "valu", "achieved": a / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                                 "frac": a / peak}
     out["speedup_fp32c_vs_fp64"] = out["fp32c"]["q_pt_samples_per_s"] / out["fp64"]["q_pt_samples_per_s"]
     out["workload"] = "spheres N(5,0.5) 21 nodes, bcc a=20 10x10x5, 8 azimuths, DW 0.5, 1024^2 DWBA, fused"
     ctx.close()
@@ -314,7 +354,11 @@ def bench_grating(ncand=16, iters=2):
     ctx.close()
     nslots = (256 // 8 * 208 // 8 + 2 * 208 // 16) * 128
     pairs = 4 * npix * nslots  # forward (q-point x triangle slot) pairs per candidate
+    mfma_flop = (48 + 54) * pairs * ncand  # forward (48 flop) + adjoint (54 flop) of matrix work per pair
     return {"grad_evals_per_s": ncand / t_iter, "ms_per_iteration": 1e3 * t_iter, "candidates": ncand,
+            "roofline": {"bound": "mfma", "achieved": mfma_flop / t_iter / 1e12, "peak": PEAK_FP16 / 1e12,
+                         "unit": "TFLOP/s", "frac": mfma_flop / t_iter / PEAK_FP16,
+                         "note": "whole L-BFGS iteration (forward, pixel pass, adjoint, update) over its wall time"},
             "ms_forward_per_candidate": 1e3 * t_fwd, "grad_over_fwd": t_iter / ncand / t_fwd,
             "forward_pairs_per_candidate": pairs,
             "fwd_plus_adjoint_pair_rate": ncand * pairs / t_iter,

@@ -23,7 +23,7 @@ ERRORS = {
 PREC_FP64, PREC_FP32C = 0, 1
 MESH_APPROX8, MESH_EXACT9 = 0, 1
 FF_APPROX, FF_EXACT = 0, 1
-FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA = 1, 2, 4
+FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOLD = 1, 2, 4, 8
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
 SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}

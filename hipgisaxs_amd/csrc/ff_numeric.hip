@@ -384,7 +384,11 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
       part = reinterpret_cast<double*>(c->d_work);
     }
     double sign = (flags & HGX_FLAG_EXACT_REFERENCE_SIGN) ? 1.0 : -1.0;
-    const double tiny = 2.2250738585072014e-308;  // DBL_MIN, the reference CPU threshold
+    // degenerate-case threshold on q.n and q.e: the reference GPU path's 1e-13 (ff_tri_gpu.cu:80,100)
+    // by default; its CPU path's DBL_MIN (ff_tri_cpu.cpp:87,107) on request. With DBL_MIN a case
+    // decision can flip on the last bit of a nearly degenerate projection, and the general case then
+    // divides by it: on a finely meshed solid most results become rounding noise.
+    const double tiny = (flags & HGX_FLAG_EXACT_CPU_THRESHOLD) ? 2.2250738585072014e-308 : 1e-13;
     dim3 grid(p.nqt, p.nchunks);
     {
       LaunchTimer lt(c, true);

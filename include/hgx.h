@@ -52,6 +52,8 @@ extern "C" {
 #define HGX_FLAG_EXACT_REFERENCE_SIGN 1 /* exact method with the reference's exp(-iq.r) sign */
 #define HGX_FLAG_ACC_KAHAN 2            /* FP32C: per-pair Kahan instead of FP32-block/FP64-carry */
 #define HGX_FLAG_NO_MFMA 4              /* FP32C approx: per-pair VALU kernels even on a product grid */
+#define HGX_FLAG_EXACT_CPU_THRESHOLD 8  /* exact method: the reference CPU's DBL_MIN degenerate-case
+                                           threshold instead of its GPU path's 1e-13 */
 
 /* flags for hgx_sf_lattice */
 #define HGX_SF_COMPAT_LB2 1 /* reproduce the reference's lb[2]*lb[2] term (src/sf/sf.cpp:212) */

@@ -147,12 +147,17 @@ def test_exact_fp64_box_and_pyramid(gpu_ctx, oracle_mod):
     for tri in (meshes.box(4.0, 6.0, 8.0), meshes.pyramid(side_levels=2, base_nx=8, base_ny=4)):
         mid = gpu_ctx.upload_mesh(tri, "exact9")
         R = _rot(0.2, 0.1, -0.5)
-        ref = O.ff_exact(tri, g, R)
+        # degenerate-case threshold: the reference GPU path's 1e-13 by default, its CPU path's DBL_MIN
+        # with HGX_FLAG_EXACT_CPU_THRESHOLD (these small meshes and grids are well conditioned for both)
+        ref = O.ff_exact(tri, g, R, tiny=1e-13)
         got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64", rot=R))
         assert rel_l2(got, ref) <= 1e-10
-        ref_m = O.ff_exact(tri, g, R, sign=-1)
+        ref_m = O.ff_exact(tri, g, R, sign=-1, tiny=1e-13)
         got_m = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64", rot=R, flags=1))
         assert rel_l2(got_m, ref_m) <= 1e-10
+        ref_c = O.ff_exact(tri, g, R)
+        got_c = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64", rot=R, flags=8))
+        assert rel_l2(got_c, ref_c) <= 1e-10
         gpu_ctx.release_mesh(mid)
 
 
@@ -185,3 +190,30 @@ def test_invalid_arguments_return_codes(gpu_ctx):
     bad[0, 4] = np.nan
     with pytest.raises(HgxError):
         gpu_ctx.upload_mesh(bad, "exact9")
+
+
+def test_exact_fp64_fine_pyramid_full_grid(gpu_ctx, oracle_mod):
+    """The 2048-triangle pyramid on the 256^2 q-space grid with four DWBA branches (the plain-method
+    workload), exact method: GPU against the oracle with the same (reference GPU path) threshold, on
+    every 8th row."""
+    import math
+
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    tri = meshes.pyramid()
+    g = O.qz_extended(O.qgrid_explicit(256, 256, -2.0, 0.0, 2.0, 2.0, ai, k0), k0, ai)
+    _upload_grid(gpu_ctx, g)
+    mid = gpu_ctx.upload_mesh(tri, "exact9")
+    got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "exact", "fp64")).reshape(4 * 256, 256)
+    rows = np.arange(0, 4 * 256, 8)
+    gs = O.Grid(256, len(rows), np.repeat(g.qx[:256][None], len(rows), 0).ravel(),
+                np.repeat(g.qy[:256][None], len(rows), 0).ravel(), g.qz.reshape(4 * 256, 256)[rows].ravel(),
+                g.alpha_f)
+    ref = O.ff_exact(tri, gs, tiny=1e-13)
+    e = rel_l2(got[rows].ravel(), ref)
+    print(f"exact 2048-triangle pyramid, 256^2 x 4 branches: relL2 {e:.2e}")
+    assert e <= 1e-10
+    gpu_ctx.release_mesh(mid)

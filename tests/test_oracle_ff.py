@@ -149,3 +149,28 @@ def test_as_written_variant_equals_oracle(oracle_mod):
     a = O.ff_approx(rec, g, R)
     b = O.ff_approx_as_written(rec, g, R)
     assert np.array_equal(a, b)
+
+
+def test_exact_degenerate_case_threshold(oracle_mod):
+    """The exact method's degenerate-case threshold on q.n and q.e: with the reference GPU path's 1e-13
+    the fine pyramid mesh's form factor is insensitive to a 1e-13 relative change of the vertices (a
+    well-conditioned evaluation); with the reference CPU path's DBL_MIN the same change moves it by
+    percent (case decisions flip on the last bit of nearly degenerate projections). The GPU kernel
+    defaults to the former."""
+    import math
+
+    from hipgisaxs_amd import meshes
+
+    O = oracle_mod
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    tri = meshes.pyramid()
+    g = O.qz_extended(O.qgrid_explicit(256, 256, -2.0, 0.0, 2.0, 2.0, ai, k0), k0, ai)
+    rows = np.arange(0, 4 * 256, 32)
+    gs = O.Grid(256, len(rows), np.repeat(g.qx[:256][None], len(rows), 0).ravel(),
+                np.repeat(g.qy[:256][None], len(rows), 0).ravel(), g.qz.reshape(4 * 256, 256)[rows].ravel(),
+                g.alpha_f)
+    a, b = O.ff_exact(tri, gs, tiny=1e-13), O.ff_exact(tri * (1 + 1e-13), gs, tiny=1e-13)
+    c, d = O.ff_exact(tri, gs), O.ff_exact(tri * (1 + 1e-13), gs)
+    assert rel_l2(b, a) < 1e-9
+    assert rel_l2(d, c) > 1e-3
