@@ -26,6 +26,7 @@ FF_APPROX, FF_EXACT = 0, 1
 FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOLD = 1, 2, 4, 8
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
+SMEAR_COMPAT = 1
 SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}
 SHAPE_DIMS = {"sphere": 1, "cylinder": 2, "box": 3, "cube": 1, "prism3": 2, "prism6": 2, "pyramid": 4}
 MAX_NODES = 40
@@ -152,6 +153,9 @@ _SIGS = {
     "hgx_tune_product": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product_tile": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hgx_tune_adjoint": (C.c_int, [C.c_int, C.c_int]),
+    "hgx_caxpy": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_double, _vp, C.c_int, _vp]),
+    "hgx_axpy": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_int, _vp]),
+    "hgx_gaussian_smear": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),
     "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_grating_set_theta": (C.c_int, [_vp, C.c_int, _vp]),

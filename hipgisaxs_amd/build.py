@@ -72,18 +72,19 @@ def build_peaks() -> Path:
     return PEAKS_BIN
 
 
-HIG_SRC = PKG / "host" / "hig_gpu.cpp"
+HIG_SRCS = [PKG / "host" / n for n in ("hig_gpu.cpp", "hig_input.cpp", "hig_sim.cpp")]
 HIG_LIB = PKG / "libhgx_hig.so"
 SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
 SHIM_LIB = ROOT / "tests" / "cpp" / "libhig_shim.so"
 
 
 def build_host_adapters() -> Path:
-    """The reference-interface C++ classes (host code on the C ABI) and the test shim that calls them."""
+    """The reference-interface C++ classes and the .hig driver (host code on the C ABI), and the test
+    shim that calls the classes."""
     cxx = shutil.which("g++") or "g++"
     common = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'host'}"]
     tmp = HIG_LIB.with_suffix(".so.tmp")
-    _run([cxx, *common, str(HIG_SRC), "-o", str(tmp), f"-L{PKG}", "-lhgx", "-Wl,-rpath,$ORIGIN"])
+    _run([cxx, *common, *map(str, HIG_SRCS), "-o", str(tmp), f"-L{PKG}", "-lhgx", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HIG_LIB)
     tmp = SHIM_LIB.with_suffix(".so.tmp")
     _run([cxx, *common, str(SHIM_SRC), "-o", str(tmp), f"-L{PKG}", "-lhgx_hig", "-lhgx",

@@ -1,0 +1,167 @@
+"""Python binding of the .hig simulation driver (libhgx_hig.so, C ABI in include/hgx_hig.h).
+
+``HigSimulation(path)`` reads a HipGISAXS input file and runs its scan of incidence and in-plane
+rotation angles on the device, as ``hipgisaxs <input.hig>`` does (src/sim/hipgisaxs_sim.cpp:37-70);
+images are numpy arrays [nrow, ncol] with row 0 at the highest q_z. There is no CPU fallback: a run
+without a HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import _native as N
+from .api import HgxError
+
+LIB_PATH = Path(__file__).resolve().parent / "libhgx_hig.so"
+
+NO_FUSE, SF_COMPAT_LB2, SMEAR_COMPAT, NO_SMEAR, MESH_NO_TRANSLATION = 1, 2, 4, 8, 16
+
+_vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+_SIGS = {
+    "hgx_hig_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "hgx_hig_destroy": (C.c_int, [_vp]),
+    "hgx_hig_load": (C.c_int, [_vp, C.c_char_p]),
+    "hgx_hig_load_text": (C.c_int, [_vp, C.c_char_p, C.c_char_p]),
+    "hgx_hig_options": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint]),
+    "hgx_hig_init": (C.c_int, [_vp]),
+    "hgx_hig_info": (C.c_int, [_vp, _ip, _ip, _ip, _dp]),
+    "hgx_hig_config": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "hgx_hig_run": (C.c_int, [_vp, C.c_int, _vp]),
+    "hgx_hig_compute": (C.c_int, [_vp, _vp]),
+    "hgx_hig_grains": (C.c_int, [_vp, C.c_char_p, C.c_int, _vp, _vp, _vp, _ip]),
+    "hgx_hig_last_counts": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp]),
+    "hgx_hig_save": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
+    "hgx_hig_read_mesh": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_int64), C.c_int64, _vp]),
+    "hgx_hig_last_error": (C.c_char_p, [_vp]),
+}
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libhgx_hig.so after libhgx.so (and through it PyTorch's HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    N.load()
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hipgisaxs_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+def read_mesh(path: str):
+    """A custom shape's mesh file as (kind, records): "exact9" [ntri, 9] for .obj, "approx8" [ntri, 8]
+    for .dat."""
+    lib = load()
+    kind, n = C.c_int(), C.c_int64()
+    rc = lib.hgx_hig_read_mesh(str(path).encode(), C.byref(kind), C.byref(n), 0, None)
+    if rc != 0:
+        raise HgxError(f"hgx_hig_read_mesh: {N.ERRORS.get(rc, rc)}: cannot read {path}")
+    w = 9 if kind.value == N.MESH_EXACT9 else 8
+    rec = np.zeros((n.value, w))
+    rc = lib.hgx_hig_read_mesh(str(path).encode(), C.byref(kind), C.byref(n), n.value, rec.ctypes.data_as(_vp))
+    if rc != 0:
+        raise HgxError(f"hgx_hig_read_mesh: {N.ERRORS.get(rc, rc)}")
+    return ("exact9" if w == 9 else "approx8"), rec
+
+
+class HigSimulation:
+    """A .hig input bound to a device (HipGISAXS::construct_input + init)."""
+
+    def __init__(self, path: str | None = None, text: str | None = None, base_dir: str = ".", device: int = 0,
+                 precision: str = "fp64", flags: int = 0, seed: int = 1):
+        self.lib = load()
+        h = _vp()
+        self._check(self.lib.hgx_hig_create(device, C.byref(h)), "create", h=None)
+        self.h = h
+        self.options(precision, flags, seed)
+        if (path is None) == (text is None):
+            raise ValueError("give exactly one of path and text")
+        if path is not None:
+            self._check(self.lib.hgx_hig_load(self.h, str(path).encode()), "load")
+        else:
+            self._check(self.lib.hgx_hig_load_text(self.h, text.encode(), str(base_dir).encode()), "load")
+
+    def _check(self, rc, what, h=0):
+        if rc != 0:
+            msg = self.lib.hgx_hig_last_error(self.h if h == 0 else h)
+            raise HgxError(f"hgx_hig_{what}: {N.ERRORS.get(rc, rc)}: {(msg or b'').decode()}")
+
+    def options(self, precision="fp64", flags=0, seed=1):
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        self._check(self.lib.hgx_hig_options(self.h, p, int(flags), int(seed)), "options")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hgx_hig_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        nc, nr, ncfg, k0 = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.hgx_hig_info(self.h, C.byref(nc), C.byref(nr), C.byref(ncfg), C.byref(k0)), "info")
+        return {"ncol": nc.value, "nrow": nr.value, "nconfig": ncfg.value, "k0": k0.value}
+
+    def configurations(self) -> list[tuple[float, float, float]]:
+        out = []
+        for i in range(self.info()["nconfig"]):
+            a, p, t = C.c_double(), C.c_double(), C.c_double()
+            self._check(self.lib.hgx_hig_config(self.h, i, C.byref(a), C.byref(p), C.byref(t)), "config")
+            out.append((a.value, p.value, t.value))
+        return out
+
+    def _image(self):
+        i = self.info()
+        return np.zeros((i["nrow"], i["ncol"]))
+
+    def run(self, config: int = 0) -> np.ndarray:
+        img = self._image()
+        self._check(self.lib.hgx_hig_run(self.h, int(config), img.ctypes.data_as(_vp)), "run")
+        return img
+
+    def compute(self) -> np.ndarray:
+        img = self._image()
+        self._check(self.lib.hgx_hig_compute(self.h, img.ctypes.data_as(_vp)), "compute")
+        return img
+
+    def run_all(self) -> list[np.ndarray]:
+        return [self.run(i) for i in range(self.info()["nconfig"])]
+
+    def grains(self, key: str):
+        n = C.c_int()
+        self._check(self.lib.hgx_hig_grains(self.h, key.encode(), 0, None, None, None, C.byref(n)), "grains")
+        rot, cen, w = np.zeros((n.value, 3, 3)), np.zeros((n.value, 3)), np.zeros(n.value)
+        self._check(self.lib.hgx_hig_grains(self.h, key.encode(), n.value, rot.ctypes.data_as(_vp),
+                                            cen.ctypes.data_as(_vp), w.ctypes.data_as(_vp), C.byref(n)), "grains")
+        return rot, cen, w
+
+    def last_counts(self) -> dict:
+        c = (C.c_int64 * 4)()
+        ms = C.c_double()
+        self._check(self.lib.hgx_hig_last_counts(self.h, c, C.byref(ms)), "last_counts")
+        return {"fused_grains": c[0], "grains": c[1], "form_factors": c[2], "structure_factors": c[3],
+                "ms": ms.value}
+
+    @staticmethod
+    def save(image: np.ndarray, path: str) -> None:
+        lib = load()
+        img = np.ascontiguousarray(image, dtype=np.float64)
+        if lib.hgx_hig_save(img.ctypes.data_as(_vp), img.shape[1], img.shape[0], str(path).encode()) != 0:
+            raise HgxError(f"hgx_hig_save: cannot write {path}")

@@ -1,0 +1,819 @@
+// hig_sim.cpp -- the .hig simulation driver on the hgx C ABI (see hig_sim.hpp).
+#include "hig_sim.hpp"
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+
+namespace hig {
+
+namespace {
+
+const double PI_ = 3.141592653589793;  // include/common/constants.hpp:29
+const double SQRT_2PI_ = 2.506628;     // include/common/constants.hpp:30 (as truncated there)
+
+// HipGISAXS::gaussian and ::cauchy (include/sim/hipgisaxs_main.hpp:151-172)
+double gaussian_weight(double x, double mean, double sigma) {
+  if (sigma < DBL_MIN) return (x - mean < DBL_MIN) ? 1.0 : 0.0;
+  return (1.0 / (sigma * SQRT_2PI_) * std::exp(-(x - mean) * (x - mean) / (2 * sigma * sigma)));
+}
+
+double cauchy_weight(double x, double l, double s) {
+  if (s > DBL_MIN) return 1.0 / (PI_ * s * (1.0 + ((x - l) / s) * ((x - l) / s)));
+  return (x - l > DBL_MIN) ? 0.0 : 1.0;
+}
+
+// Lattice::construct_vectors (src/model/structure.cpp:86-182): unit-scaled vectors of a lattice
+// type and hkl; fco uses the class's ca_ = 1 and gamma_ = 0 (the file's caratio and abangle are
+// never passed to it). The basis vector t is left out, as the CPU structure factor does.
+bool lattice_vectors(const HigStructure& s, vec3& a, vec3& b, vec3& c, std::string& err) {
+  if (s.abc_set) {
+    a = s.a;
+    b = s.b;
+    c = s.c;
+    return true;
+  }
+  const std::string& t = s.lattice_type;
+  const std::string& hkl = s.hkl;
+  if (t == "cubic" && hkl == "100") {
+    a = {1, 0, 0}; b = {0, 1, 0}; c = {0, 0, 1};
+  } else if (t == "cubic" && hkl == "110") {
+    a = {1, 0, 0}; b = {0., 1.4142, 0}; c = {0., 0.7071, 0.7071};
+  } else if (t == "bcc" && hkl == "100") {
+    a = {1., 0., 0.}; b = {0., 1., 0.}; c = {0.5, 0.5, 0.5};
+  } else if (t == "bcc" && hkl == "110") {
+    a = {1.0, 0.0, 0}; b = {0.5, 0.7071, 0}; c = {0.0, 0.7071, 0.7071};
+  } else if (t == "fcc" && hkl == "100") {
+    a = {0.0, 0.5, 0.5}; b = {0.5, 0.0, 0.5}; c = {0.5, 0.5, 0.0};
+  } else if (t == "fcc" && hkl == "111") {
+    a = {1., 0., 0}; b = {0.5, 0.866, -0.}; c = {1., 0.5773, 0.866};
+  } else if (t == "fco") {
+    const double ca = 1.0, gamma = 0.0;
+    a = {1, 0, 0}; b = {0.5, std::tan(gamma) / 2.0, 0}; c = {0, 0, ca};
+  } else if (t == "hcp") {
+    a = {1, 0, 0}; b = {0.5, 0.866, 0}; c = {0.5, 0.2887, 0.8165};
+  } else if (t == "hex") {
+    a = {1, 0, 0}; b = {0.5, std::sqrt(3.0) / 2.0, 0}; c = {0, 0, 1};
+  } else {
+    err = "invalid lattice type and hkl combination '" + t + "', '" + hkl + "'";
+    return false;
+  }
+  return true;
+}
+
+bool same9(const double* a, const double* b) { return std::memcmp(a, b, 9 * sizeof(double)) == 0; }
+
+// angle count of a scanned range, as run_all_gisaxs counts it (src/sim/hipgisaxs_main.cpp:298-312)
+int scan_count(double mn, double mx, double step, bool alphai) {
+  if (alphai) {
+    if (mx < mn) mx = mn;
+    if (mn == mx || step == 0) return 1;
+  } else if (step == 0) {
+    return 1;
+  }
+  int n = (int)((mx - mn) / step + 1);
+  return n < 1 ? 1 : n;
+}
+
+}  // namespace
+
+// ----- GlibcRandom ---------------------------------------------------------------------------------
+
+void GlibcRandom::reseed(unsigned seed) {
+  int32_t init[34];
+  init[0] = (int32_t)(seed == 0 ? 1 : seed);
+  for (int i = 1; i < 31; ++i) {
+    // 16807 * r mod (2^31 - 1) by Schrage's decomposition (no 32-bit overflow)
+    int32_t hi = init[i - 1] / 127773, lo = init[i - 1] % 127773;
+    int32_t w = 16807 * lo - 2836 * hi;
+    if (w < 0) w += 2147483647;
+    init[i] = w;
+  }
+  for (int i = 31; i < 34; ++i) init[i] = init[i - 31];
+  for (int i = 0; i < 34; ++i) r_[i] = (uint32_t)init[i];
+  i_ = 0;
+  for (int k = 0; k < 310; ++k) next();
+}
+
+int GlibcRandom::next() {
+  // ring slot i_ holds r[n-34]; r[n] = r[n-31] + r[n-3] replaces it
+  uint32_t v = r_[(i_ + 3) % 34] + r_[(i_ + 31) % 34];
+  r_[i_] = v;
+  i_ = (i_ + 1) % 34;
+  return (int)(v >> 1);
+}
+
+// ----- HipGISAXS -----------------------------------------------------------------------------------
+
+HipGISAXS::HipGISAXS(int device) : device_(device) {}
+
+HipGISAXS::~HipGISAXS() { release_device_objects(); }
+
+bool HipGISAXS::device() {
+  if (gpu_ && gpu_->ok()) return true;
+  gpu_.reset(new GpuContext(device_));
+  if (!gpu_->ok()) {
+    gpu_.reset();
+    return fail("could not create the device context on device " + std::to_string(device_));
+  }
+  return true;
+}
+
+void HipGISAXS::release_device_objects() {
+  if (gpu_ && gpu_->ok()) {
+    for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
+    for (int i = 0; i < 8; ++i)
+      if (buf_[i]) hgx_device_free(gpu_->get(), buf_[i]);
+  }
+  meshes_.clear();
+  for (int i = 0; i < 8; ++i) {
+    buf_[i] = nullptr;
+    buf_bytes_[i] = 0;
+  }
+}
+
+bool HipGISAXS::fail(const std::string& msg) {
+  err_ = msg;
+  return false;
+}
+
+bool HipGISAXS::hgx(int rc, const char* what) {
+  if (rc == HGX_OK) return true;
+  err_ = std::string(what) + " failed (" + std::to_string(rc) + "): " + (gpu_ ? hgx_last_error(gpu_->get()) : "");
+  return false;
+}
+
+double* HipGISAXS::buffer(int slot, size_t bytes) {
+  if (bytes <= buf_bytes_[slot]) return static_cast<double*>(buf_[slot]);
+  if (buf_[slot]) hgx_device_free(gpu_->get(), buf_[slot]);
+  buf_[slot] = nullptr;
+  buf_bytes_[slot] = 0;
+  if (hgx_device_alloc(gpu_->get(), bytes, &buf_[slot]) != HGX_OK) return nullptr;
+  buf_bytes_[slot] = bytes;
+  return static_cast<double*>(buf_[slot]);
+}
+
+void HipGISAXS::options(int precision, int flags, unsigned seed) {
+  precision_ = precision;
+  flags_ = flags;
+  seed_ = seed;
+  rng_.reseed(seed_);
+}
+
+bool HipGISAXS::construct_input(const std::string& path) {
+  loaded_ = inited_ = false;
+  if (gpu_ && gpu_->ok())
+    for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
+  meshes_.clear();
+  if (!in_.construct_input(path)) return fail(in_.error());
+  loaded_ = true;
+  rng_.reseed(seed_);
+  return true;
+}
+
+bool HipGISAXS::construct_input_text(const std::string& text, const std::string& base_dir) {
+  loaded_ = inited_ = false;
+  if (gpu_ && gpu_->ok())
+    for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
+  meshes_.clear();
+  if (!in_.construct_input_text(text, base_dir)) return fail(in_.error());
+  loaded_ = true;
+  rng_.reseed(seed_);
+  return true;
+}
+
+bool HipGISAXS::prepare() {
+  if (!loaded_) return fail("no input has been read");
+  // photon energy -> wavelength -> k0 (src/sim/hipgisaxs_main.cpp:139-158)
+  double photon = in_.scattering.photon, lambda;
+  if (in_.scattering.unit == "ev") {
+    photon = photon / 1000;
+    lambda = 1.23984 / photon;
+  } else if (in_.scattering.unit == "kev") {
+    lambda = 1.23984 / photon;
+  } else {
+    return fail("photon energy is not given in 'ev' or 'kev'");
+  }
+  if (!(lambda > 0) || !std::isfinite(lambda)) return fail("photon energy must be positive");
+  k0_ = 2 * PI_ / lambda;
+  const HigCompute& cp = in_.compute;
+  if (cp.region != "qspace")
+    return fail("output region '" + cp.region + "' is not supported (the reference implements qspace only)");
+  ncol_ = (unsigned)cp.resolution[0];
+  nrow_ = (unsigned)cp.resolution[1];
+  if (ncol_ < 2 || nrow_ < 2) return fail("resolution must be at least [2 2]");
+  npix_ = (int64_t)ncol_ * nrow_;
+  return true;
+}
+
+bool HipGISAXS::init() {
+  if (inited_) return true;
+  if (!prepare() || !device()) return false;
+  grid_ok_ = false;  // a new input: rebuild the grid
+  if (!ensure_grid(in_.scattering.alphai_min * PI_ / 180, 0.0, 0.0)) return false;
+  inited_ = true;
+  return true;
+}
+
+// The q-grid of the output region is built at the smallest incidence angle (HipGISAXS::init,
+// src/sim/hipgisaxs_main.cpp:171-176); a GISAXS run extends its q_z for the run's incidence angle and
+// the particle layer (layer_qgrid_qz, :1548-1556). alphai < 0 requests the plain (SAXS) grid.
+bool HipGISAXS::ensure_grid(double alphai, double dnl_re, double dnl_im) {
+  const bool want_ext = alphai >= 0 && in_.scattering.expt == "gisaxs";
+  if (grid_ok_ && !want_ext && !grid_ext_) return true;
+  if (grid_ok_ && want_ext && grid_ext_ && grid_ai_ == alphai && grid_dn_[0] == dnl_re && grid_dn_[1] == dnl_im)
+    return true;
+  grid_ok_ = false;
+  const HigCompute& cp = in_.compute;
+  double ai_min = in_.scattering.alphai_min * PI_ / 180;
+  if (!hgx(hgx_qgrid_qspace(gpu_->get(), (int)ncol_, (int)nrow_, cp.minpoint[0], cp.minpoint[1], cp.maxpoint[0],
+                            cp.maxpoint[1], ai_min, k0_),
+           "hgx_qgrid_qspace"))
+    return false;
+  grid_ext_ = false;
+  if (want_ext && !hgx(hgx_qz_extended(gpu_->get(), k0_, alphai, dnl_re, dnl_im), "hgx_qz_extended")) return false;
+  grid_ok_ = true;
+  if (!want_ext) return true;
+  grid_ext_ = true;
+  grid_ai_ = alphai;
+  grid_dn_[0] = dnl_re;
+  grid_dn_[1] = dnl_im;
+  return true;
+}
+
+std::vector<std::array<double, 3>> HipGISAXS::configurations() const {
+  const HigScattering& sc = in_.scattering;
+  double amax = sc.alphai_max < sc.alphai_min ? sc.alphai_min : sc.alphai_max;
+  int na = scan_count(sc.alphai_min, amax, sc.alphai_step, true);
+  int np = scan_count(sc.phi_min, sc.phi_max, sc.phi_step, false);
+  int nt = scan_count(sc.tilt_min, sc.tilt_max, sc.tilt_step, false);
+  std::vector<std::array<double, 3>> out;
+  // angles advance by accumulation, as the reference's loops do (:350-351, :383, :410)
+  double a = sc.alphai_min;
+  for (int i = 0; i < na; ++i, a += sc.alphai_step) {
+    double p = sc.phi_min;
+    for (int j = 0; j < np; ++j, p += sc.phi_step) {
+      double t = sc.tilt_min;
+      for (int k = 0; k < nt; ++k, t += sc.tilt_step) out.push_back({a, p, t});
+    }
+  }
+  return out;
+}
+
+bool HipGISAXS::compute_gisaxs(std::vector<double>& img) {
+  if (!loaded_) return fail("no input");
+  if (configurations().size() != 1)
+    return fail("currently you can simulate only for single alpha_i, phi and tilt angles");
+  const HigScattering& sc = in_.scattering;
+  return run_gisaxs(sc.alphai_min, sc.phi_min, sc.tilt_min, img);
+}
+
+// spatial_distribution and orientation_distribution (src/sim/hipgisaxs_main.cpp:1702-2086) and the
+// grain rotation and centre of run_gisaxs (:925-958).
+bool HipGISAXS::make_grains(const HigStructure& s, GlibcRandom& rng, double phi, std::vector<HigGrain>& out) {
+  out.clear();
+  if (s.distribution != "regular")
+    return fail("structure '" + s.key + "': ensemble distribution '" + s.distribution +
+                "' is not supported (regular only; the reference's random placement reseeds rand() from the clock)");
+  const int nd[3] = {(int)s.maxgrains[0], (int)s.maxgrains[1], (int)s.maxgrains[2]};
+  const int ng = nd[0] * nd[1] * nd[2];
+  if (ng < 1 || ng > 1000000) return fail("structure '" + s.key + "': grain count out of range");
+  // grain positions: cell_ of the reference is never set (zero), so the spacing alone separates grains
+  const double tz = 0.0;
+  std::vector<double> dx(nd[0]), dy(nd[1]), dz(nd[2]);
+  {
+    double val = 0, step = s.spacing[0], mx = s.spacing[0] * (nd[0] - 1);
+    for (int i = 0; i < nd[0]; ++i, val += step) dx[i] = val - mx / 2;
+    val = 0, step = s.spacing[1], mx = s.spacing[1] * (nd[1] - 1);
+    for (int i = 0; i < nd[1]; ++i, val += step) dy[i] = val - mx / 2;
+    val = 0, step = s.spacing[2];
+    for (int i = 0; i < nd[2]; ++i, val += step) dz[i] = val + tz;
+  }
+  // orientations: nn[k][g] (radians) and weights
+  std::vector<double> nn(3 * (size_t)ng, 0.0), wght(3 * (size_t)ng, 1.0);
+  const std::string& st = s.orient_stat;
+  HigRotation rot[3];
+  int axis[3];
+  for (int k = 0; k < 3; ++k) {
+    rot[k] = s.rot[k];
+    if (rot[k].axis == 'n') {  // Structure::rotation_rotN returns (0, 0, 0) for axis 'n'
+      rot[k].axis = 'x';
+      rot[k].amin = rot[k].amax = 0.0;
+    }
+    axis[k] = rot[k].axis - 'x';
+  }
+  bool gauss = false;
+  if (st == "single") {
+    for (int k = 0; k < 3; ++k)
+      for (int g = 0; g < ng; ++g) nn[k * ng + g] = rot[k].amin * PI_ / 180;
+  } else if (st == "random") {
+    for (size_t x = 0; x < nn.size(); ++x) nn[x] = (double(rng.next()) / GlibcRandom::kRandMax) * 2 * PI_;
+  } else if (st == "range") {
+    for (int k = 0; k < 3; ++k) {
+      double d = std::fabs(rot[k].amax - rot[k].amin);
+      for (int g = 0; g < ng; ++g)
+        nn[k * ng + g] = (rot[k].amin + (double(rng.next()) / GlibcRandom::kRandMax) * d) * PI_ / 180;
+    }
+  } else if (st == "gaussian" || st == "normal" || st == "cauchy") {
+    gauss = true;
+    const bool cauchy = st == "cauchy";
+    for (int k = 0; k < 3; ++k) {
+      double d = std::fabs(rot[k].amax - rot[k].amin);
+      d = cauchy ? (ng > 1 ? d / (ng - 1) : 0.0) : d;  // gaussian divides by ng below, cauchy by ng - 1
+      if (d > DBL_MIN) {
+        if (!cauchy) d /= ng;
+        for (int g = 0; g < ng; ++g) {
+          double t = rot[k].amin + g * d;
+          nn[k * ng + g] = t * PI_ / 180;
+          wght[k * ng + g] = cauchy ? cauchy_weight(t, rot[k].location, rot[k].scale)
+                                    : gaussian_weight(t, rot[k].mean, rot[k].sd);
+        }
+      } else {
+        for (int g = 0; g < ng; ++g) {
+          nn[k * ng + g] = rot[k].amin * PI_ / 180;
+          wght[k * ng + g] = 1;
+        }
+      }
+    }
+  } else {
+    return fail("structure '" + s.key + "': orientation statistic '" + st + "' is not supported");
+  }
+  vec3 tv = s.transvec;  // minus the reference's never-initialised single_layer_thickness_, taken as 0
+  RotMatrix_t rphi(2, phi);
+  out.resize(ng);
+  const int nyz = nd[1] * nd[2];
+  for (int g = 0; g < ng; ++g) {
+    RotMatrix_t r1(axis[0], nn[g]), r2(axis[1], nn[ng + g]), r3(axis[2], nn[2 * ng + g]);
+    HigGrain& G = out[g];
+    G.rot = rphi * r3 * r2 * r1;
+    // d1[nyz x + yz] = dx[x], d2[nyz x + nz y + z] = dy[y], d3[nz xy + z] = dz[z]
+    int x = g / nyz, y = (g % nyz) / nd[2], z = g % nd[2];
+    vector3_t dd(dx[x], dy[y], dz[z]);
+    vector3_t c = G.rot * dd;
+    G.center[0] = c[0] + tv[0];
+    G.center[1] = c[1] + tv[1];
+    G.center[2] = c[2] + tv[2];
+    G.weight = gauss ? wght[g] * wght[ng + g] * wght[2 * ng + g] : 1.0;
+  }
+  return true;
+}
+
+bool HipGISAXS::grains(const std::string& key, std::vector<HigGrain>& out) {
+  if (!loaded_) return fail("no input");
+  if (!in_.structures.count(key)) return fail("no structure '" + key + "'");
+  GlibcRandom rng = rng_;  // the sampler's state is left as it is
+  double phi = in_.scattering.phi_min * PI_ / 180;
+  for (const auto& s : in_.structures) {
+    if (!make_grains(s.second, rng, phi, out)) return false;
+    if (s.first == key) return true;
+  }
+  return false;
+}
+
+// Distribution nodes of an analytic shape's parameters in the order hgx_ff_analytic takes them
+// (the parameter switches of src/ff/ff_ana_<shape>.cpp; the cube reads its edge from "height",
+// ff_ana_cube.cpp:54-56; a box "edge" sets all three sides unless a side is given).
+bool HipGISAXS::shape_params(const HigShape& sh, hgx_shape_params& p, std::vector<double> store[8]) {
+  std::memset(&p, 0, sizeof(p));
+  std::vector<std::vector<std::string>> dims;
+  switch (sh.shape) {
+    case HGX_SHAPE_SPHERE: dims = {{"radius"}}; break;
+    case HGX_SHAPE_CYLINDER: dims = {{"radius"}, {"height"}}; break;
+    case HGX_SHAPE_BOX: dims = {{"xsize", "edge"}, {"ysize", "edge"}, {"height", "edge"}}; break;
+    case HGX_SHAPE_CUBE: dims = {{"height"}}; break;
+    case HGX_SHAPE_PRISM3:
+    case HGX_SHAPE_PRISM6: dims = {{"edge"}, {"height"}}; break;
+    case HGX_SHAPE_PYRAMID: dims = {{"xsize"}, {"ysize"}, {"height"}, {"baseangle"}}; break;
+    default: return fail("shape '" + sh.key + "' (" + sh.name + ") has no analytic form factor");
+  }
+  for (size_t d = 0; d < dims.size(); ++d) {
+    const HigShapeParam* prm = nullptr;
+    for (const auto& name : dims[d]) {
+      auto it = sh.params.find(name);
+      if (it != sh.params.end()) {
+        prm = &it->second;
+        break;
+      }
+    }
+    if (!prm) return fail("shape '" + sh.key + "' (" + sh.name + ") needs a '" + dims[d][0] + "' parameter");
+    if (prm->stat == 2 && !(prm->p2 > 0))
+      return fail("shape '" + sh.key + "': gaussian parameter '" + prm->type + "' needs p2 (deviation) > 0");
+    int cap = 4 * prm->nvalues + 8;
+    store[2 * d].resize(cap);
+    store[2 * d + 1].resize(cap);
+    int n = hgx_param_distribution(prm->min, prm->max, prm->nvalues, prm->stat, prm->p1, prm->p2,
+                                   store[2 * d].data(), store[2 * d + 1].data(), cap);
+    if (n < 1) return fail("shape '" + sh.key + "': bad distribution for parameter '" + prm->type + "'");
+    if (n > HGX_MAX_NODES)
+      return fail("shape '" + sh.key + "': parameter '" + prm->type + "' has more than " +
+                  std::to_string(HGX_MAX_NODES) + " nodes");
+    p.n[d] = n;
+    p.val[d] = store[2 * d].data();
+    p.wt[d] = store[2 * d + 1].data();
+  }
+  return true;
+}
+
+int HipGISAXS::mesh_of(const HigShape& sh) {
+  auto it = meshes_.find(sh.key);
+  if (it != meshes_.end()) return it->second;
+  std::vector<double> rec;
+  std::string err;
+  int kind;
+  if (sh.file.size() > 4 && sh.file.compare(sh.file.size() - 4, 4, ".obj") == 0) {
+    if (!read_obj_mesh(sh.file, rec, err)) return fail(err), -1;
+    kind = HGX_MESH_EXACT9;
+  } else if (sh.file.size() > 4 && sh.file.compare(sh.file.size() - 4, 4, ".dat") == 0) {
+    if (!read_raw_mesh(sh.file, rec, err)) return fail(err), -1;
+    kind = HGX_MESH_APPROX8;
+  } else {
+    return fail("mesh file '" + sh.file + "': only .obj and .dat meshes are supported"), -1;
+  }
+  int id = -1;
+  int64_t n = (int64_t)(rec.size() / (kind == HGX_MESH_EXACT9 ? 9 : 8));
+  if (!hgx(hgx_mesh_upload(gpu_->get(), kind, n, rec.data(), &id), "hgx_mesh_upload")) return -1;
+  meshes_[sh.key] = id;
+  return id;
+}
+
+// FormFactor::compute_form_factor (src/ff/ff.cpp:32-61): analytic shapes with the location phase,
+// mesh shapes by the approximate (centroid) method of NumericFormFactor::compute
+// (src/ff/ff_num.cpp:187-284) under the run's precision.
+bool HipGISAXS::form_factor(const HigShape& sh, const RotMatrix_t& rot, const vec3& loc, double* d_out) {
+  ++counts_[2];
+  if (sh.shape == -1) {
+    int id = mesh_of(sh);
+    if (id < 0) return false;
+    const double* tv = (flags_ & HGX_HIG_MESH_NO_TRANSLATION) ? nullptr : loc.data();
+    return hgx(hgx_ff_numeric(gpu_->get(), id, HGX_FF_APPROX, precision_, 0, rot.data(), tv, d_out), "hgx_ff_numeric");
+  }
+  hgx_shape_params p;
+  std::vector<double> store[8];
+  if (!shape_params(sh, p, store)) return false;
+  return hgx(hgx_ff_analytic(gpu_->get(), sh.shape, &p, HGX_PREC_FP64, rot.data(), loc.data(), d_out),
+             "hgx_ff_analytic");
+}
+
+// One structure's grain sum into d_I (written, not accumulated): the body of run_gisaxs's structure
+// loop for the nGnE correlation (src/sim/hipgisaxs_main.cpp:775-1222).
+bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains,
+                                    double* d_I) {
+  auto uit = in_.unitcells.find(s.unitcell_key);
+  if (uit == in_.unitcells.end()) return fail("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
+  const HigUnitcell& uc = uit->second;
+  for (const auto& e : uc.elements)
+    if (!in_.shapes.count(e.first)) return fail("unitcell '" + uc.key + "': no shape '" + e.first + "'");
+  const bool gisaxs = in_.scattering.expt == "gisaxs";
+  // The particle layer: the reference reads Structure::layer_order(), which no code path sets, so it
+  // is always 0 -- the vacuum (structure.cpp:358, hipgisaxs_main.cpp:780).
+  const int order = 0;
+  const double dnl[2] = {0.0, 0.0};
+  if (!ensure_grid(gisaxs ? alphai : -1.0, dnl[0], dnl[1])) return false;
+  const int nbranch = gisaxs ? 4 : 1;
+  const int64_t nq = npix_ * nbranch;
+  double* d_fc = nullptr;
+  if (gisaxs) {
+    // MultiLayer::init (src/model/multilayer.cpp:29-59): layers of order >= 0 top to bottom, then the
+    // substrate (order -1, or an empty layer when the input has none)
+    std::vector<double> de, be, th;
+    for (const auto& l : in_.layers) {
+      if (l.first < 0) continue;
+      if (l.first == 0 && l.second.key == "vacuum" && l.second.delta == 0 && l.second.beta == 0 &&
+          l.second.thickness == 0)
+        continue;  // the implicit vacuum: a zero-thickness vacuum layer leaves the coefficients unchanged
+      de.push_back(l.second.delta);
+      be.push_back(l.second.beta);
+      th.push_back(l.second.thickness);
+    }
+    auto sub = in_.layers.find(-1);
+    de.push_back(sub != in_.layers.end() ? sub->second.delta : 0.0);
+    be.push_back(sub != in_.layers.end() ? sub->second.beta : 0.0);
+    th.push_back(sub != in_.layers.end() ? sub->second.thickness : 0.0);
+    d_fc = buffer(2, (size_t)nq * 16);
+    if (!d_fc) return fail("device allocation failed");
+    if (!hgx(hgx_fresnel(gpu_->get(), (int)de.size(), de.data(), be.data(), th.data(), order, k0_, alphai, d_fc),
+             "hgx_fresnel"))
+      return false;
+  }
+  vec3 a, b, c;
+  std::string err;
+  if (!lattice_vectors(s, a, b, c, err)) return fail("structure '" + s.key + "': " + err);
+  // StructureFactor::compute_structure_factor scales the vectors (src/sf/sf.cpp:165-171)
+  double la[3], lb[3], lc[3];
+  for (int i = 0; i < 3; ++i) {
+    la[i] = a[i] * s.scaling[0];
+    lb[i] = b[i] * s.scaling[1];
+    lc[i] = c[i] * s.scaling[2];
+  }
+  const double repet[3] = {s.repetition[0], s.repetition[1], s.repetition[2]};
+  const int sf_flags = (flags_ & HGX_HIG_SF_COMPAT_LB2) ? HGX_SF_COMPAT_LB2 : 0;
+  const int ng = (int)grains.size();
+
+  // the fused grain-ensemble kernel: one analytic sphere or cylinder at the cell origin, unrotated
+  // against the grain, every grain centred alike
+  bool fuse = !(flags_ & HGX_HIG_NO_FUSE) && uc.elements.size() == 1 && uc.elements.begin()->second.size() == 1;
+  const HigShape* sh0 = fuse ? &in_.shapes.at(uc.elements.begin()->first) : nullptr;
+  if (fuse) {
+    const vec3& l0 = uc.elements.begin()->second[0];
+    fuse = (sh0->shape == HGX_SHAPE_SPHERE || sh0->shape == HGX_SHAPE_CYLINDER) && l0[0] == 0 && l0[1] == 0 &&
+           l0[2] == 0 && sh0->xrot == 0 && sh0->yrot == 0 && sh0->zrot == 0;
+    for (int g = 1; fuse && g < ng; ++g)
+      fuse = std::memcmp(grains[g].center, grains[0].center, sizeof(grains[0].center)) == 0;
+  }
+  if (fuse) {
+    hgx_lattice_model m;
+    std::memset(&m, 0, sizeof(m));
+    std::vector<double> store[8];
+    if (!shape_params(*sh0, m.params, store)) return false;
+    m.shape = sh0->shape;
+    for (int i = 0; i < 3; ++i) {
+      m.la[i] = la[i];
+      m.lb[i] = lb[i];
+      m.lc[i] = lc[i];
+      m.repet[i] = repet[i];
+      m.center[i] = grains[0].center[i];
+    }
+    m.dw_sigma = 0.0;
+    // contrast (1 - n^2)_layer - (1 - n^2)_shape, one_minus_n2 = 2 delta + 2 i beta
+    m.dn2_re = dnl[0] - 2 * sh0->delta;
+    m.dn2_im = dnl[1] - 2 * sh0->beta;
+    m.sf_flags = sf_flags;
+    m.ngrains = ng;
+    std::vector<double> rots(9 * (size_t)ng), wts(ng);
+    for (int g = 0; g < ng; ++g) {
+      std::memcpy(&rots[9 * (size_t)g], grains[g].rot.data(), 9 * sizeof(double));
+      wts[g] = grains[g].weight;
+    }
+    m.rot = rots.data();
+    m.weight = wts.data();
+    if (!hgx(hgx_lattice_intensity(gpu_->get(), &m, precision_, d_fc, 0, d_I), "hgx_lattice_intensity")) return false;
+    counts_[0] += ng;
+    counts_[2] += ng;
+    counts_[3] += ng;
+    // the model arrays are read on the host during the call; synchronise before they go out of scope
+    return hgx(hgx_synchronize(gpu_->get()), "hgx_synchronize");
+  }
+
+  // per grain: form factor (reused while the grain rotation repeats), structure factor, DWBA sum
+  double* d_ff = buffer(3, (size_t)nq * 16);
+  double* d_eff = buffer(4, (size_t)nq * 16);
+  double* d_sf = buffer(5, (size_t)nq * 16);
+  if (!d_ff || !d_eff || !d_sf) return fail("device allocation failed");
+  for (int g = 0; g < ng; ++g) {
+    const HigGrain& G = grains[g];
+    bool new_rot = g == 0 || !same9(G.rot.data(), grains[g - 1].rot.data());
+    if (new_rot) {
+      bool first = true;
+      for (const auto& e : uc.elements) {
+        const HigShape& sh = in_.shapes.at(e.first);
+        const double dn2r = dnl[0] - 2 * sh.delta, dn2i = dnl[1] - 2 * sh.beta;
+        // shape rotation rot * Rx * Ry * Rz (src/sim/hipgisaxs_main.cpp:986)
+        RotMatrix_t srot = G.rot * RotMatrix_t(0, sh.xrot) * RotMatrix_t(1, sh.yrot) * RotMatrix_t(2, sh.zrot);
+        for (const auto& loc : e.second) {
+          if (!form_factor(sh, srot, loc, d_eff)) return false;
+          if (!hgx(hgx_caxpy(gpu_->get(), nq, dn2r, dn2i, d_eff, first ? 0 : 1, d_ff), "hgx_caxpy")) return false;
+          first = false;
+        }
+      }
+    }
+    bool new_sf = new_rot || std::memcmp(G.center, grains[g - 1].center, sizeof(G.center)) != 0;
+    if (new_sf) {
+      ++counts_[3];
+      if (!hgx(hgx_sf_lattice(gpu_->get(), la, lb, lc, repet, G.center, 0.0, sf_flags, G.rot.data(), d_sf),
+               "hgx_sf_lattice"))
+        return false;
+    }
+    // the reference assigns instead of accumulating in its SAXS branch (:1102-1105), which keeps the
+    // last grain only; both experiments accumulate here
+    if (!hgx(hgx_dwba_intensity(gpu_->get(), d_fc, d_sf, d_ff, G.weight, g > 0 ? 1 : 0, d_I), "hgx_dwba_intensity"))
+      return false;
+  }
+  counts_[1] += ng;
+  return true;
+}
+
+bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, std::vector<double>& img) {
+  (void)tilt_deg;  // the reference's run_gisaxs reads the tilt angle nowhere
+  if (!loaded_) return fail("no input");
+  if (!inited_ && !init()) return false;
+  auto t0 = std::chrono::steady_clock::now();
+  for (auto& c : counts_) c = 0;
+  const HigCompute& cp = in_.compute;
+  if (cp.nslices > 1) return fail("the slicing scheme (nslices > 1) is not implemented by the reference either");
+  if (cp.structcorr != "nGnE")
+    return fail("structcorrelation '" + cp.structcorr + "' is not supported (nGnE only)");
+  for (const auto& s : in_.structures) {
+    const HigStructure& st = s.second;
+    if (!st.grain_type.empty())
+      return fail("structure '" + st.key + "': " + st.grain_type + " structures are not supported");
+    if (st.scaling_dist || st.repetition_dist)
+      return fail("structure '" + st.key + "': scaling and repetition distributions are not supported");
+  }
+  const double alphai = alpha_i_deg * PI_ / 180;
+  const double phi = phi_deg * PI_ / 180;
+  img.assign((size_t)npix_, 0.0);
+  if (in_.structures.empty()) return true;
+  // iratio normalisation (src/sim/hipgisaxs_main.cpp:1268-1282)
+  std::vector<double> iratio;
+  double isum = 0.0;
+  for (const auto& s : in_.structures) {
+    iratio.push_back(s.second.iratio);
+    isum += s.second.iratio;
+  }
+  if (isum != 1.0)
+    for (double& r : iratio) r /= isum;
+  double* d_total = buffer(0, (size_t)npix_ * 8);
+  double* d_Is = buffer(1, (size_t)npix_ * 8);
+  if (!d_total || !d_Is) return fail("device allocation failed");
+  std::vector<HigGrain> grains;
+  size_t si = 0;
+  for (const auto& s : in_.structures) {
+    if (!make_grains(s.second, rng_, phi, grains)) return false;
+    if (!structure_intensity(s.second, alphai, grains, d_Is)) return false;
+    // intensity = sum_s struct_intensity_s * iratio_s (:1351-1364)
+    if (!hgx(hgx_axpy(gpu_->get(), npix_, iratio[si], d_Is, si > 0 ? 1 : 0, d_total), "hgx_axpy")) return false;
+    ++si;
+  }
+  const double sigma = in_.scattering.smearing;
+  if (sigma > DBL_MIN && !(flags_ & HGX_HIG_NO_SMEAR)) {
+    int sf = (flags_ & HGX_HIG_SMEAR_COMPAT) ? HGX_SMEAR_COMPAT : 0;
+    if (!hgx(hgx_gaussian_smear(gpu_->get(), (int)ncol_, (int)nrow_, sigma, sf, d_total), "hgx_gaussian_smear"))
+      return false;
+  }
+  if (!hgx(hgx_memcpy_d2h(gpu_->get(), img.data(), d_total, (size_t)npix_ * 8), "hgx_memcpy_d2h")) return false;
+  last_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
+
+void HipGISAXS::last_counts(int64_t counts[4]) const {
+  for (int i = 0; i < 4; ++i) counts[i] = counts_[i];
+}
+
+bool HipGISAXS::save_gisaxs(const double* data, unsigned ncol, unsigned nrow, const std::string& path) {
+  std::ofstream f(path);
+  if (!f) return false;
+  // the reference streams with the default 6 significant digits; 17 keep every bit
+  f << std::setprecision(17);
+  for (unsigned z = 0; z < nrow; ++z) {
+    for (unsigned y = 0; y < ncol; ++y) f << data[(size_t)ncol * z + y] << "\t";
+    f << "\n";
+  }
+  return (bool)f;
+}
+
+}  // namespace hig
+
+// ----- C ABI ---------------------------------------------------------------------------------------
+
+struct hgx_hig {
+  hig::HipGISAXS sim;
+  std::string err;
+  explicit hgx_hig(int device) : sim(device) {}
+};
+
+namespace {
+
+int hig_fail(hgx_hig* h, int code) {
+  h->err = h->sim.error();
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hgx_hig_create(int device, hgx_hig** out) {
+  if (!out) return HGX_ERR_INVALID_ARG;
+  *out = nullptr;
+  hgx_hig* h = new (std::nothrow) hgx_hig(device);
+  if (!h) return HGX_ERR_OOM;
+  if (!h->sim.error().empty()) {
+    delete h;
+    return HGX_ERR_DEVICE;
+  }
+  *out = h;
+  return HGX_OK;
+}
+
+int hgx_hig_destroy(hgx_hig* h) {
+  delete h;
+  return HGX_OK;
+}
+
+int hgx_hig_load(hgx_hig* h, const char* path) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!path) return HGX_ERR_INVALID_ARG;
+  return h->sim.construct_input(path) ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+}
+
+int hgx_hig_load_text(hgx_hig* h, const char* text, const char* base_dir) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!text) return HGX_ERR_INVALID_ARG;
+  return h->sim.construct_input_text(text, base_dir ? base_dir : ".") ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+}
+
+int hgx_hig_options(hgx_hig* h, int precision, int flags, unsigned seed) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C) return HGX_ERR_INVALID_ARG;
+  h->sim.options(precision, flags, seed);
+  return HGX_OK;
+}
+
+int hgx_hig_init(hgx_hig* h) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  return h->sim.init() ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+}
+
+int hgx_hig_info(hgx_hig* h, int* ncol, int* nrow, int* nconfig, double* k0) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!h->sim.prepare()) return hig_fail(h, HGX_ERR_INVALID_ARG);
+  if (ncol) *ncol = (int)h->sim.ncols();
+  if (nrow) *nrow = (int)h->sim.nrows();
+  if (nconfig) *nconfig = (int)h->sim.configurations().size();
+  if (k0) *k0 = h->sim.k0();
+  return HGX_OK;
+}
+
+int hgx_hig_config(hgx_hig* h, int config, double* alpha_i, double* phi, double* tilt) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  auto cf = h->sim.configurations();
+  if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
+  if (alpha_i) *alpha_i = cf[config][0];
+  if (phi) *phi = cf[config][1];
+  if (tilt) *tilt = cf[config][2];
+  return HGX_OK;
+}
+
+int hgx_hig_run(hgx_hig* h, int config, double* image) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!image) return HGX_ERR_INVALID_ARG;
+  auto cf = h->sim.configurations();
+  if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
+  std::vector<double> img;
+  if (!h->sim.run_gisaxs(cf[config][0], cf[config][1], cf[config][2], img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+  std::memcpy(image, img.data(), img.size() * sizeof(double));
+  return HGX_OK;
+}
+
+int hgx_hig_compute(hgx_hig* h, double* image) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!image) return HGX_ERR_INVALID_ARG;
+  std::vector<double> img;
+  if (!h->sim.compute_gisaxs(img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+  std::memcpy(image, img.data(), img.size() * sizeof(double));
+  return HGX_OK;
+}
+
+int hgx_hig_grains(hgx_hig* h, const char* key, int cap, double* rot, double* center, double* weight, int* n) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (!key || !n) return HGX_ERR_INVALID_ARG;
+  std::vector<hig::HigGrain> g;
+  if (!h->sim.grains(key, g)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+  *n = (int)g.size();
+  for (int i = 0; i < (int)g.size() && i < cap; ++i) {
+    if (rot) std::memcpy(rot + 9 * (size_t)i, g[i].rot.data(), 9 * sizeof(double));
+    if (center) std::memcpy(center + 3 * (size_t)i, g[i].center, 3 * sizeof(double));
+    if (weight) weight[i] = g[i].weight;
+  }
+  return HGX_OK;
+}
+
+int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  if (counts) h->sim.last_counts(counts);
+  if (ms) *ms = h->sim.last_run_ms();
+  return HGX_OK;
+}
+
+int hgx_hig_save(const double* image, int ncol, int nrow, const char* path) {
+  if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
+  return hig::HipGISAXS::save_gisaxs(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+}
+
+int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records) {
+  if (!path || !kind || !ntri) return HGX_ERR_INVALID_ARG;
+  std::string p(path), err;
+  std::vector<double> rec;
+  bool ok;
+  if (p.size() > 4 && p.compare(p.size() - 4, 4, ".obj") == 0) {
+    *kind = HGX_MESH_EXACT9;
+    ok = hig::read_obj_mesh(p, rec, err);
+  } else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".dat") == 0) {
+    *kind = HGX_MESH_APPROX8;
+    ok = hig::read_raw_mesh(p, rec, err);
+  } else {
+    return HGX_ERR_UNSUPPORTED;
+  }
+  if (!ok) return HGX_ERR_INVALID_ARG;
+  const size_t w = *kind == HGX_MESH_EXACT9 ? 9 : 8;
+  *ntri = (int64_t)(rec.size() / w);
+  if (records && cap >= *ntri) std::memcpy(records, rec.data(), rec.size() * sizeof(double));
+  return HGX_OK;
+}
+
+const char* hgx_hig_last_error(const hgx_hig* h) { return h ? h->err.c_str() : "bad handle"; }
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// hig_sim.hpp -- the simulation driver of a .hig input on the hgx C ABI: HipGISAXS::init,
+// compute_gisaxs, run_gisaxs and run_all_gisaxs (src/sim/hipgisaxs_main.cpp:106-216, 283-626,
+// 630-1436) with the same loop structure (structures -> grains -> unit-cell elements ->
+// locations), reference orientation sampling and output layout, every array on the device.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hgx.h"
+#include "hgx_hig.h"
+#include "hig_gpu.hpp"
+#include "hig_input.hpp"
+
+namespace hig {
+
+// The random() generator of glibc (additive feedback, degree 31: r[i] = r[i-3] + r[i-31] mod 2^32
+// after an LCG seeding of r[0..30] and 310 discarded outputs; rand() returns r >> 1). A process
+// that never calls srand() draws from seed 1; the reference samples grain orientations with rand()
+// (src/sim/hipgisaxs_main.cpp:1917-1936) and seeds nothing on that path.
+class GlibcRandom {
+ public:
+  explicit GlibcRandom(unsigned seed = 1) { reseed(seed); }
+  void reseed(unsigned seed);
+  int next();  // in [0, 2^31 - 1]
+  static constexpr double kRandMax = 2147483647.0;
+
+ private:
+  uint32_t r_[34];
+  int i_ = 0;  // ring position of r[i - 34]
+};
+
+struct HigGrain {
+  RotMatrix_t rot;      // rot_phi * r3 * r2 * r1
+  double center[3];     // rot * dd + transvec
+  double weight;        // gauss_weight * scaling weight
+};
+
+// Parsing, configurations and grain sampling run on the host alone; the device context is created
+// by the first call that needs it (init, the runs).
+class HipGISAXS {
+ public:
+  explicit HipGISAXS(int device = 0);
+  ~HipGISAXS();
+  HipGISAXS(const HipGISAXS&) = delete;
+  HipGISAXS& operator=(const HipGISAXS&) = delete;
+
+  bool construct_input(const std::string& path);
+  bool construct_input_text(const std::string& text, const std::string& base_dir = ".");
+  // precision HGX_PREC_FP64 / HGX_PREC_FP32C, HGX_HIG_* flags, orientation sampling seed (1 = a
+  // fresh reference process)
+  void options(int precision, int flags, unsigned seed);
+  // photon energy -> k0 and the image size (host only)
+  bool prepare();
+  // prepare, then the q-grid of the output region at the smallest incidence angle on the device
+  bool init();
+  // One image [nrow][ncol] for incidence alpha_i and in-plane rotation phi (degrees).
+  bool run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, std::vector<double>& img);
+  // The single-configuration run (fails when the input scans alpha_i, phi or tilt).
+  bool compute_gisaxs(std::vector<double>& img);
+  // The (alpha_i, phi, tilt) scan of run_all_gisaxs, in its loop order.
+  std::vector<std::array<double, 3>> configurations() const;
+  // The grains the next run would use for structure `key` (does not advance the sampler).
+  bool grains(const std::string& key, std::vector<HigGrain>& out);
+  static bool save_gisaxs(const double* data, unsigned ncol, unsigned nrow, const std::string& path);
+
+  unsigned nrows() const { return nrow_; }
+  unsigned ncols() const { return ncol_; }
+  double k0() const { return k0_; }
+  const std::string& error() const { return err_; }
+  const HigInput& input() const { return in_; }
+  // work counts of the last run: grains through the fused kernel, grains through per-grain
+  // calls, form-factor and structure-factor evaluations
+  void last_counts(int64_t counts[4]) const;
+  double last_run_ms() const { return last_ms_; }
+
+ private:
+  bool fail(const std::string& msg);
+  bool hgx(int rc, const char* what);
+  bool device();  // creates the device context on first use
+  void release_device_objects();
+  bool ensure_grid(double alphai_rad, double dnl_re, double dnl_im);
+  bool make_grains(const HigStructure& s, GlibcRandom& rng, double phi_rad, std::vector<HigGrain>& out);
+  bool structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains, double* d_I);
+  bool form_factor(const HigShape& shape, const RotMatrix_t& rot, const vec3& loc, double* d_out);
+  bool shape_params(const HigShape& shape, hgx_shape_params& p, std::vector<double> store[8]);
+  int mesh_of(const HigShape& shape);
+  double* buffer(int slot, size_t bytes);
+
+  int device_ = 0;
+  std::unique_ptr<GpuContext> gpu_;
+  HigInput in_;
+  bool loaded_ = false, inited_ = false;
+  int precision_ = HGX_PREC_FP64;
+  int flags_ = 0;
+  unsigned seed_ = 1;
+  GlibcRandom rng_;
+  double k0_ = 0.0;
+  unsigned nrow_ = 0, ncol_ = 0;
+  int64_t npix_ = 0;
+  // the grid currently on the device: real (nbranch 1) or extended for (alphai, dnl)
+  bool grid_ok_ = false, grid_ext_ = false;
+  double grid_ai_ = 0.0, grid_dn_[2] = {0.0, 0.0};
+  void* buf_[8] = {};
+  size_t buf_bytes_[8] = {};
+  std::map<std::string, int> meshes_;
+  int64_t counts_[4] = {0, 0, 0, 0};
+  double last_ms_ = 0.0;
+  std::string err_;
+};
+
+}  // namespace hig

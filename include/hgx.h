@@ -334,6 +334,27 @@ int hgx_grating_get(hgx_ctx* ctx, int plan, double* theta, double* chi2, double*
 /* Model image of candidate cand at its accepted theta into I [npix] (device). */
 int hgx_grating_intensity(hgx_ctx* ctx, int plan, int cand, double* I);
 
+/* ---- accumulation and image post-processing ------------------------------------------------ */
+
+/* y[0..n) (+)= a x[0..n) over complex arrays (interleaved, device; accumulate = 0 overwrites y): the
+ * element-and-location sum of the form factor, ff += dn2 * F (inline in the reference,
+ * src/sim/hipgisaxs_main.cpp:1015). */
+int hgx_caxpy(hgx_ctx* ctx, int64_t n, double a_re, double a_im, const double* x, int accumulate, double* y);
+/* y[0..n) (+)= a x[0..n) over real arrays (device): the iratio-weighted sum of structure intensities
+ * (src/sim/hipgisaxs_main.cpp:1351-1364). */
+int hgx_axpy(hgx_ctx* ctx, int64_t n, double a, const double* x, int accumulate, double* y);
+
+/* Smearing window of the reference: [i - h, i + h) with the kernel weight indexed from the window
+ * start, which shifts the kernel at the image edges (include/numerics/convolutions.hpp:141-147). */
+#define HGX_SMEAR_COMPAT 1
+/* Separable Gaussian smearing of the image I [nrow][ncol] (device, in place), rows then columns, each
+ * output the weight-normalised sum over the window of half-width h = ceil(6 sigma) pixels; replaces
+ * Convolutions::convolution_gaussian_2d (include/numerics/convolutions.hpp:128-175), applied by
+ * run_gisaxs when the smearing sigma exceeds DBL_MIN (src/sim/hipgisaxs_main.cpp:1409-1431). By
+ * default the weight follows the distance |k - i| over [i - h, i + h]; HGX_SMEAR_COMPAT keeps the
+ * reference's window. Synchronises the stream. */
+int hgx_gaussian_smear(hgx_ctx* ctx, int ncol, int nrow, double sigma, int flags, double* I);
+
 /* ---- host helpers ------------------------------------------------------------------------- */
 
 /* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values of a shape

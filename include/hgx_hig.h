@@ -1,0 +1,75 @@
+/* hgx_hig.h -- C ABI of the .hig simulation driver (libhgx_hig.so, host code on the hgx C ABI).
+ *
+ * Replaces the reference's HipGISAXS driver object (include/sim/hipgisaxs_main.hpp, used by
+ * src/sim/hipgisaxs_sim.cpp:37-70): construct_input, init, compute_gisaxs / run_all_gisaxs and
+ * save_gisaxs. Images are host arrays of nrow * ncol doubles, row-major, row 0 holding the highest
+ * q_z (src/model/qgrid.cpp:75). Every function returns HGX_OK (0) or a negative HGX_ERR_* code
+ * (include/hgx.h) and never exits; hgx_hig_last_error returns the message of the last failure.
+ */
+#ifndef HGX_HIG_H_
+#define HGX_HIG_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hgx_hig hgx_hig;
+
+/* flags of hgx_hig_options */
+#define HGX_HIG_NO_FUSE 1           /* per-grain form factor, structure factor and DWBA calls only */
+#define HGX_HIG_SF_COMPAT_LB2 2     /* the reference CPU structure factor's lb[2]*lb[2] term */
+#define HGX_HIG_SMEAR_COMPAT 4      /* the reference's smearing window (HGX_SMEAR_COMPAT) */
+#define HGX_HIG_NO_SMEAR 8          /* skip the smearing whatever the input's sigma */
+#define HGX_HIG_MESH_NO_TRANSLATION 16 /* ignore unit-cell locations of mesh shapes, as the reference
+                                          does (src/ff/ff.cpp:44) */
+
+/* A driver on HIP device `device`. Reading inputs, hgx_hig_info, hgx_hig_config and hgx_hig_grains
+ * run on the host; the device context (one hgx context) is created by the first call that needs it. */
+int hgx_hig_create(int device, hgx_hig** out);
+int hgx_hig_destroy(hgx_hig* h);
+/* HipGISAXS::construct_input (src/sim/hipgisaxs_main.cpp, HiGInput::construct_input_config
+ * src/config/hig_input.cpp:62-106): reads a .hig file, or .hig text with base_dir for relative mesh
+ * file names. Resets the orientation sampler to the seed of hgx_hig_options. */
+int hgx_hig_load(hgx_hig* h, const char* path);
+int hgx_hig_load_text(hgx_hig* h, const char* text, const char* base_dir);
+/* precision: HGX_PREC_FP64 (default) or HGX_PREC_FP32C; flags: HGX_HIG_*; seed of the glibc rand()
+ * sequence the orientation statistics "range" and "random" draw from (1, the default, is the
+ * sequence of a reference process, which never seeds it on that path). Resets the sampler. */
+int hgx_hig_options(hgx_hig* h, int precision, int flags, unsigned seed);
+/* HipGISAXS::init (src/sim/hipgisaxs_main.cpp:106-216): k0 from the photon energy and the q-grid of
+ * the output region (q-space) at the smallest incidence angle. Called by the runs when needed. */
+int hgx_hig_init(hgx_hig* h);
+/* Image size, number of (alpha_i, phi, tilt) configurations run_all_gisaxs scans, k0 (1/nm).
+ * Host only (no device work). */
+int hgx_hig_info(hgx_hig* h, int* ncol, int* nrow, int* nconfig, double* k0);
+/* Angles (degrees) of configuration `config` in run_all_gisaxs's loop order (alpha_i outermost). */
+int hgx_hig_config(hgx_hig* h, int config, double* alpha_i, double* phi, double* tilt);
+/* run_gisaxs for configuration `config`: image [nrow * ncol] (host). Draws grain orientations from
+ * the sampler, as each reference run does. */
+int hgx_hig_run(hgx_hig* h, int config, double* image);
+/* compute_gisaxs: the single configuration of a non-scanning input. */
+int hgx_hig_compute(hgx_hig* h, double* image);
+/* The grains the next run gives structure `key` (host [cap][9] rotations applied to q, [cap][3]
+ * centres, [cap] weights; any may be NULL) without advancing the sampler; *n = grain count. */
+int hgx_hig_grains(hgx_hig* h, const char* key, int cap, double* rot, double* center, double* weight, int* n);
+/* Work of the last run: counts[0] grains through the fused lattice kernel, [1] grains through
+ * per-grain calls, [2] form-factor evaluations, [3] structure-factor evaluations; *ms its wall time. */
+int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms);
+/* HipGISAXS::save_gisaxs (src/sim/hipgisaxs_main.cpp:2250-2260): nrow lines of ncol tab-terminated
+ * values. */
+int hgx_hig_save(const double* image, int ncol, int nrow, const char* path);
+/* Reads a mesh file of a custom shape (host only): Wavefront .obj into HGX_MESH_EXACT9 records
+ * (ObjectShapeReader, src/file/objectshape_reader.cpp:52-300; polygons are split into triangle fans
+ * where the reference drops every non-triangle face) or raw .dat into HGX_MESH_APPROX8 records
+ * (RawShapeReader, src/file/rawshape_reader.cpp:39-62). Writes *kind and *ntri, and the records
+ * into `records` when cap >= *ntri (pass cap = 0 to query the size). */
+int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records);
+const char* hgx_hig_last_error(const hgx_hig* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGX_HIG_H_ */

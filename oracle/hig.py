@@ -1,0 +1,492 @@
+"""CPU oracle of the .hig simulation loop (test infrastructure only).
+
+A restatement, in Python over the FP64 oracle kernels of this package, of what the reference does
+with a .hig input: HiGInput's reading of the file (src/config/hig_input.cpp:62-2832, defaults from
+src/model/{structure,shape,layer,inst_scattering,compute_params}.cpp), HipGISAXS::init
+(src/sim/hipgisaxs_main.cpp:106-216), run_gisaxs (:692-1436) with spatial_distribution /
+orientation_distribution (:1702-2086) and the smearing of include/numerics/convolutions.hpp:128-175.
+Grain orientations drawn with rand() come from the C library's own rand() (the reference never
+seeds it on this path, so a fresh process draws the seed-1 sequence). Only tests import this.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+
+import numpy as np
+
+from . import (dwba_intensity, ff_analytic, ff_approx, fresnel, param_distribution, qgrid_qspace, qz_extended,
+               sf_lattice, tri9_to_rec8)
+
+PI_ = 3.141592653589793
+SQRT_2PI_ = 2.506628
+DBL_MIN = 2.2250738585072014e-308
+
+# ----- reading ---------------------------------------------------------------------------------
+
+_TOK = re.compile(r"""\s*(?:(\#[^\n]*)|("[^"]*"|'[^']*')|([{}\[\]=,])|([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)|
+                      ([A-Za-z_][A-Za-z0-9_:.\-]*))""", re.X)
+
+
+def _tokens(text):
+    pos, out = 0, []
+    text = text.rstrip()
+    while pos < len(text):
+        m = _TOK.match(text, pos)
+        if not m or m.end() == pos:
+            raise ValueError(f"cannot read .hig text at offset {pos}: {text[pos:pos + 20]!r}")
+        pos = m.end()
+        com, s, p, num, word = m.groups()
+        if com is not None:
+            continue
+        if s is not None:
+            out.append(("str", s[1:-1]))
+        elif p is not None:
+            out.append(("p", p))
+        elif num is not None:
+            out.append(("num", float(num)))
+        elif word is not None:
+            out.append(("str", word))
+    return out
+
+
+def parse_tree(text):
+    """The file as nested lists of (key, value) pairs (objects), Python lists (arrays), floats and
+    strings."""
+    toks = _tokens(text)
+    i = 0
+
+    def value():
+        nonlocal i
+        kind, v = toks[i]
+        if kind == "p" and v == "{":
+            i += 1
+            return assigns("}")
+        if kind == "p" and v == "[":
+            i += 1
+            arr = []
+            while toks[i] != ("p", "]"):
+                arr.append(value())
+                if toks[i] == ("p", ","):
+                    i += 1
+            i += 1
+            return arr
+        if kind in ("num", "str"):
+            i += 1
+            return v
+        raise ValueError(f"unexpected token {v!r}")
+
+    def assigns(close):
+        nonlocal i
+        out = []
+        while True:
+            if close is None and i >= len(toks):
+                return out
+            if close is not None and toks[i] == ("p", close):
+                i += 1
+                return out
+            kind, key = toks[i]
+            if kind != "str" or toks[i + 1] != ("p", "="):
+                raise ValueError(f"expected 'key =' at {key!r}")
+            i += 2
+            out.append((key, value()))
+            if i < len(toks) and toks[i] == ("p", ","):
+                i += 1
+
+    return assigns(None)
+
+
+def _obj(pairs):
+    return pairs if isinstance(pairs, list) and all(isinstance(p, tuple) for p in pairs) else None
+
+
+def read_model(text):
+    """The typed model the simulation reads, with the reference's defaults."""
+    top = dict(parse_tree(text))["hipGisaxsInput"]
+    m = {"shapes": {}, "unitcells": {}, "layers": {}, "structures": {},
+         "scattering": {"expt": "gisaxs", "alphai": [0.1, 0.1, 0.1], "inplanerot": [0.0, 0.0, 0.0],
+                        "tilt": [0.0, 0.0, 0.0], "photon": 10000.0, "unit": "ev", "smearing": 1.0},
+         "compute": {"minpoint": [-1.0, -1.0], "maxpoint": [-1.0, -1.0], "resolution": [1, 1],
+                     "region": "qspace", "nslices": 0, "structcorrelation": "nGnE"}}
+    for key, val in top:
+        if key == "shape":
+            sh = {"xrot": 0.0, "yrot": 0.0, "zrot": 0.0, "delta": 0.0, "beta": 0.0, "params": {}}
+            for k, v in val:
+                if k in ("key", "name"):
+                    sh[k] = v
+                elif k in ("xrot", "yrot", "zrot"):
+                    sh[k] = v * PI_ / 180.
+                elif k == "refindex":
+                    sh.update(dict(v))
+                elif k == "param":
+                    p = {"stat": "none", "min": 0.0, "max": 0.0, "p1": 0.0, "p2": 0.0, "nvalues": 1}
+                    p.update(dict(v))
+                    sh["params"][p["type"]] = p
+            m["shapes"][sh["key"]] = sh
+        elif key == "unitcell":
+            d = dict(val)
+            els = {}
+            for k, v in val:
+                if k == "element":
+                    e = dict(v)
+                    els[e["shape:key"]] = [list(map(float, loc)) for loc in e["locations"]]
+            m["unitcells"][d["key"]] = els
+        elif key == "layer":
+            d = {"order": 0, "thickness": 0.0, "delta": 0.0, "beta": 0.0}
+            for k, v in val:
+                if k == "refindex":
+                    d.update(dict(v))
+                else:
+                    d[k] = v
+            m["layers"][int(d["order"])] = d
+        elif key == "structure":
+            st = {"iratio": 1.0, "lattice": "cubic", "hkl": "100", "abc": None, "repetition": [1.0, 1.0, 1.0],
+                  "scaling": [1.0, 1.0, 1.0], "transvec": [0.0, 0.0, 0.0], "maxgrains": [1, 1, 1],
+                  "spacing": [0.0, 0.0, 0.0], "distribution": "regular", "stat": "single",
+                  "rot": [{"axis": "x", "angles": [0.0, 0.0]}, {"axis": "y", "angles": [0.0, 0.0]},
+                          {"axis": "z", "angles": [0.0, 0.0]}]}
+            for k, v in val:
+                if k in ("key", "iratio"):
+                    st[k] = v
+                elif k == "grain":
+                    for gk, gv in v:
+                        if gk == "unitcell:key":
+                            st["unitcell"] = gv
+                        elif gk == "lattice":
+                            ld = dict(gv)
+                            st["lattice"] = ld.get("type", "cubic")
+                            st["hkl"] = str(ld.get("hkl", "100"))
+                            if "a" in ld:
+                                st["abc"] = (ld["a"], ld["b"], ld["c"])
+                        elif gk == "scaling":
+                            st["scaling"] = [gv] * 3 if isinstance(gv, float) else list(gv)
+                        elif gk in ("repetition", "transvec"):
+                            st[gk] = list(gv)
+                elif k == "ensemble":
+                    for ek, ev in v:
+                        if ek in ("maxgrains", "spacing"):
+                            st[ek] = list(ev)
+                        elif ek == "distribution":
+                            st[ek] = ev
+                        elif ek == "orientations":
+                            for ok, ov in ev:
+                                if ok == "stat":
+                                    st["stat"] = ov
+                                else:
+                                    r = dict(ov)
+                                    ang = list(r.get("angles", [0.0, 0.0]))
+                                    st["rot"][int(ok[3]) - 1] = {
+                                        "axis": r.get("axis", "x"), "angles": ang if len(ang) == 2 else ang * 2,
+                                        "mean": r.get("anglemean", 0.0), "sd": r.get("anglesd", 0.0),
+                                        "location": r.get("anglelocation", 0.0), "scale": r.get("anglescale", 0.0)}
+            m["structures"][st["key"]] = st
+        elif key == "instrumentation":
+            for ik, iv in val:
+                if ik != "scattering":
+                    continue
+                for sk, sv in iv:
+                    sc = m["scattering"]
+                    if sk in ("alphai", "inplanerot", "tilt"):
+                        for rk, rv in sv:
+                            sc[sk][{"min": 0, "max": 1, "step": 2}[rk]] = rv
+                            if sk == "alphai" and rk == "step":
+                                sc["inplanerot"][2] = rv  # the reference's switch falls through
+                    elif sk == "photon":
+                        d = dict(sv)
+                        sc["photon"], sc["unit"] = d.get("value", 10000.0), d.get("unit", "ev")
+                    elif sk in ("expt", "smearing"):
+                        sc[sk] = sv
+        elif key == "computation":
+            for ck, cv in val:
+                c = m["compute"]
+                if ck == "outputregion":
+                    for ok, ov in cv:
+                        c["region" if ok == "type" else ok] = ov
+                elif ck == "resolution":
+                    c["resolution"] = [int(cv[0]), int(cv[1])]
+                elif ck in ("nslices", "structcorrelation"):
+                    c[ck] = cv
+    return m
+
+
+# ----- simulation ------------------------------------------------------------------------------
+
+_LIBC = None
+
+
+def _libc():
+    global _LIBC
+    if _LIBC is None:
+        _LIBC = ctypes.CDLL("libc.so.6")
+        _LIBC.rand.restype = ctypes.c_int
+    return _LIBC
+
+
+def libc_rand(seed):
+    """The C library's rand() sequence after srand(seed), drawn on demand."""
+    lib = _libc()
+    lib.srand(ctypes.c_uint(seed))
+    while True:
+        yield lib.rand()
+
+
+def rotmat(axis, a):
+    c, s = math.cos(a), math.sin(a)
+    if axis == 0:
+        return np.array([[1., 0., 0.], [0., c, -s], [0., s, c]])
+    if axis == 1:
+        return np.array([[c, 0., -s], [0., 1., 0.], [s, 0., c]])
+    return np.array([[c, -s, 0.], [s, c, 0.], [0., 0., 1.]])
+
+
+def _mul(a, b):
+    """Row-by-column product with the reference's summation order (k = 0, 1, 2)."""
+    out = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            s = 0.0
+            for k in range(3):
+                s += a[i, k] * b[k, j]
+            out[i, j] = s
+    return out
+
+
+def grains(st, phi, rand):
+    """(rot, center, weight) per grain of a structure; rand() draws from the iterator `rand`."""
+    nd = [int(x) for x in st["maxgrains"]]
+    ng = nd[0] * nd[1] * nd[2]
+    sp = st["spacing"]
+    dx, dy, dz = [], [], []
+    val = 0.0
+    for i in range(nd[0]):
+        dx.append(val - sp[0] * (nd[0] - 1) / 2)
+        val += sp[0]
+    val = 0.0
+    for i in range(nd[1]):
+        dy.append(val - sp[1] * (nd[1] - 1) / 2)
+        val += sp[1]
+    val = 0.0
+    for i in range(nd[2]):
+        dz.append(val)
+        val += sp[2]
+    rots = []
+    for r in st["rot"]:
+        if r["axis"] == "n":
+            rots.append({"axis": "x", "angles": [0.0, 0.0], "mean": 0, "sd": 0, "location": 0, "scale": 0})
+        else:
+            rots.append(dict({"mean": 0, "sd": 0, "location": 0, "scale": 0}, **r))
+    nn = np.zeros((3, ng))
+    w = np.ones((3, ng))
+    stat = st["stat"]
+    weighted = False
+    if stat == "single":
+        for k in range(3):
+            nn[k, :] = rots[k]["angles"][0] * PI_ / 180
+    elif stat == "random":
+        for k in range(3):
+            for g in range(ng):
+                nn[k, g] = (next(rand) / 2147483647.0) * 2 * PI_
+    elif stat == "range":
+        for k in range(3):
+            lo, hi = rots[k]["angles"]
+            d = abs(hi - lo)
+            for g in range(ng):
+                nn[k, g] = (lo + (next(rand) / 2147483647.0) * d) * PI_ / 180
+    elif stat in ("gaussian", "normal"):
+        weighted = True
+        for k in range(3):
+            lo, hi = rots[k]["angles"]
+            d = abs(hi - lo)
+            if d > DBL_MIN:
+                d /= ng
+                for g in range(ng):
+                    t = lo + g * d
+                    nn[k, g] = t * PI_ / 180
+                    sd, mean = rots[k]["sd"], rots[k]["mean"]
+                    w[k, g] = ((1.0 if t - mean < DBL_MIN else 0.0) if sd < DBL_MIN else
+                               1.0 / (sd * SQRT_2PI_) * math.exp(-(t - mean) * (t - mean) / (2 * sd * sd)))
+            else:
+                nn[k, :] = lo * PI_ / 180
+    else:
+        raise ValueError(stat)
+    out = []
+    rphi = rotmat(2, phi)
+    nyz = nd[1] * nd[2]
+    for g in range(ng):
+        R = rphi
+        for k in (2, 1, 0):
+            R = _mul(R, rotmat("xyz".index(rots[k]["axis"]), nn[k, g]))
+        x, y, z = g // nyz, (g % nyz) // nd[2], g % nd[2]
+        dd = np.array([dx[x], dy[y], dz[z]])
+        c = np.array([R[i, 0] * dd[0] + R[i, 1] * dd[1] + R[i, 2] * dd[2] for i in range(3)])
+        out.append((R, c + np.asarray(st["transvec"], float), float(w[0, g] * w[1, g] * w[2, g]) if weighted else 1.0))
+    return out
+
+
+_LATTICES = {
+    ("cubic", "100"): ([1, 0, 0], [0, 1, 0], [0, 0, 1]),
+    ("cubic", "110"): ([1, 0, 0], [0., 1.4142, 0], [0., 0.7071, 0.7071]),
+    ("bcc", "100"): ([1., 0., 0.], [0., 1., 0.], [0.5, 0.5, 0.5]),
+    ("bcc", "110"): ([1.0, 0.0, 0], [0.5, 0.7071, 0], [0.0, 0.7071, 0.7071]),
+    ("fcc", "100"): ([0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]),
+    ("fcc", "111"): ([1., 0., 0], [0.5, 0.866, -0.], [1., 0.5773, 0.866]),
+    ("hcp", None): ([1, 0, 0], [0.5, 0.866, 0], [0.5, 0.2887, 0.8165]),
+    ("hex", None): ([1, 0, 0], [0.5, math.sqrt(3.0) / 2.0, 0], [0, 0, 1]),
+    ("fco", None): ([1, 0, 0], [0.5, 0.0, 0], [0, 0, 1.0]),
+}
+
+_DIMS = {"sphere": [["radius"]], "cylinder": [["radius"], ["height"]],
+         "box": [["xsize", "edge"], ["ysize", "edge"], ["height", "edge"]], "cube": [["height"]],
+         "prism3": [["edge"], ["height"]], "prism6": [["edge"], ["height"]],
+         "pyramid": [["xsize"], ["ysize"], ["height"], ["baseangle"]]}
+_STAT = {"none": 0, "single": 0, "uniform": 1, "gaussian": 2}
+
+
+def shape_dims(sh):
+    dims = []
+    for names in _DIMS[sh["name"]]:
+        p = next(sh["params"][n] for n in names if n in sh["params"])
+        dims.append(param_distribution(p["min"], p["max"], int(p["nvalues"]), _STAT[p["stat"]], p["p1"], p["p2"]))
+    return dims
+
+
+def read_obj(path):
+    """Triangles [n, 9] of a Wavefront OBJ file: faces split into fans (v0, vk, vk+1), indices from 1
+    or negative from the last vertex read."""
+    verts, tris = [], []
+    for line in open(path):
+        f = line.split()
+        if not f:
+            continue
+        if f[0] == "v":
+            verts.append([float(x) for x in f[1:4]])
+        elif f[0] == "f":
+            idx = []
+            for t in f[1:]:
+                i = int(t.split("/")[0])
+                idx.append(i - 1 if i > 0 else len(verts) + i)
+            for k in range(1, len(idx) - 1):
+                tris.append(verts[idx[0]] + verts[idx[k]] + verts[idx[k + 1]])
+    return np.array(tris, float)
+
+
+def read_raw(path):
+    """Approximate-method records [n, 8] of a raw shape file (7 numbers per triangle, pad 0)."""
+    v = np.array(open(path).read().split(), float).reshape(-1, 7)
+    return np.hstack([v, np.zeros((v.shape[0], 1))])
+
+
+def mesh_ff(path, g, rot, loc, translate=True):
+    """Approximate (centroid-rule) form factor of a mesh file shape, times exp(i (R q).t)."""
+    rec = tri9_to_rec8(read_obj(path)) if path.endswith(".obj") else read_raw(path)
+    F = ff_approx(rec, g, rot=rot)
+    if translate:
+        mq = g.q_rotated(rot)
+        F = F * np.exp(1j * (mq[0] * loc[0] + mq[1] * loc[1] + mq[2] * loc[2]))
+    return F
+
+
+def smear(img, sigma, compat=False):
+    """Convolutions::convolution_gaussian_2d on img [nrow][ncol] (compat: the reference's window and
+    weight indexing; otherwise distance weights over the symmetric window)."""
+    h = int(math.ceil(6 * sigma))
+    g = np.array([1.0 / (sigma * math.sqrt(2 * PI_) * math.exp((x * x) / (2 * sigma * sigma)))
+                  for x in range(-h, h + 1)])
+
+    def one_d(line):
+        n = line.size
+        out = np.zeros(n)
+        for i in range(n):
+            beg = max(0, i - h)
+            end = min(n, i + h) if compat else min(n - 1, i + h) + 1
+            s = nrm = 0.0
+            for k in range(beg, end):
+                wk = g[k - beg] if compat else g[k - i + h]
+                s += line[k] * wk
+                nrm += wk
+            out[i] = s / nrm if nrm > DBL_MIN else 0.0
+        return out
+
+    tmp = np.array([one_d(r) for r in img])
+    return np.array([one_d(c) for c in tmp.T]).T
+
+
+def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True):
+    """Image [nrow][ncol] of configuration `config` (alpha_i outermost, as run_all_gisaxs loops) for
+    .hig text; grain orientations drawn from rand() after srand(seed) (earlier configurations draw
+    first)."""
+    m = read_model(text)
+    sc, cp = m["scattering"], m["compute"]
+    keV = sc["photon"] / 1000 if sc["unit"] == "ev" else sc["photon"]
+    k0 = 2 * PI_ / (1.23984 / keV)
+    ncol, nrow = cp["resolution"]
+    npix = ncol * nrow
+    a_min, a_max, a_step = sc["alphai"]
+    a_max = max(a_max, a_min)
+    na = 1 if (a_min == a_max or a_step == 0) else int((a_max - a_min) / a_step + 1)
+    p_min, p_max, p_step = sc["inplanerot"]
+    nph = 1 if p_step == 0 else int((p_max - p_min) / p_step + 1)
+    t_min, t_max, t_step = sc["tilt"]
+    nt = 1 if t_step == 0 else int((t_max - t_min) / t_step + 1)
+    configs = []
+    a = a_min
+    for _ in range(na):
+        p = p_min
+        for _ in range(nph):
+            for _ in range(nt):
+                configs.append((a, p))
+            p += p_step
+        a += a_step
+    draws = libc_rand(seed)
+    ai_min = a_min * PI_ / 180
+    base = qgrid_qspace(ncol, nrow, cp["minpoint"][0], cp["minpoint"][1], cp["maxpoint"][0], cp["maxpoint"][1],
+                        ai_min, k0)
+    gis = sc["expt"] == "gisaxs"
+    for ci, (a_deg, p_deg) in enumerate(configs):
+        alphai, phi = a_deg * PI_ / 180, p_deg * PI_ / 180
+        g = qz_extended(base, k0, alphai, 0j) if gis else base
+        nb = 4 if gis else 1
+        layers = [m["layers"][o] for o in sorted(m["layers"]) if o >= 0]
+        sub = m["layers"].get(-1, {"delta": 0.0, "beta": 0.0, "thickness": 0.0})
+        stack = layers + [sub]
+        fc = fresnel(g, [l["delta"] for l in stack], [l["beta"] for l in stack], [l["thickness"] for l in stack],
+                     0, k0, alphai) if gis else None
+        keys = sorted(m["structures"])
+        ir = np.array([m["structures"][k]["iratio"] for k in keys], float)
+        if ir.sum() != 1.0:
+            ir = ir / ir.sum()
+        total = np.zeros(npix)
+        for si, key in enumerate(keys):
+            st = m["structures"][key]
+            gr = grains(st, phi, draws)
+            if ci != config:
+                continue
+            if st["abc"] is not None:
+                va, vb, vc = st["abc"]
+            else:
+                va, vb, vc = _LATTICES.get((st["lattice"], st["hkl"]), _LATTICES.get((st["lattice"], None)))
+            s = st["scaling"]
+            la, lb, lc = (np.asarray(va, float) * s[0], np.asarray(vb, float) * s[1], np.asarray(vc, float) * s[2])
+            els = m["unitcells"][st["unitcell"]]
+            I = np.zeros(npix)
+            for R, center, w in gr:
+                ff = np.zeros(g.nq, complex)
+                for skey in sorted(els):
+                    sh = m["shapes"][skey]
+                    dn2 = 0j - complex(2 * sh["delta"], 2 * sh["beta"])
+                    srot = _mul(_mul(_mul(R, rotmat(0, sh["xrot"])), rotmat(1, sh["yrot"])), rotmat(2, sh["zrot"]))
+                    for loc in els[skey]:
+                        if sh["name"].endswith((".obj", ".dat")):
+                            ff += dn2 * mesh_ff(f"{base_dir}/{sh['name']}", g, srot, loc, mesh_translate)
+                        else:
+                            ff += dn2 * ff_analytic(sh["name"], shape_dims(sh), g, rot=srot, transvec=loc)
+                sf = sf_lattice(la, lb, lc, st["repetition"], center, g, 0.0, sf_flags, rot=R)
+                I = dwba_intensity(npix, nb, fc, sf, ff, w, I)
+            total += ir[si] * I
+        if ci == config:
+            img = total.reshape(nrow, ncol)
+            if smear_mode != "off" and sc["smearing"] > DBL_MIN:
+                img = smear(img, sc["smearing"], compat=smear_mode == "compat")
+            return img
+    raise IndexError(config)

@@ -10,8 +10,8 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def _declared_functions():
-    text = (ROOT / "include" / "hgx.h").read_text()
+def _declared_functions(header="hgx.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(hgx_[a-z0-9_]+)\s*\(", text)))
 
@@ -31,6 +31,16 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # the Python binding declares exactly the header's functions
     assert sorted(_native.exported_symbols()) == _declared_functions()
+
+
+def test_driver_library_exports_every_declared_symbol():
+    from hipgisaxs_amd import hig
+
+    lib = hig.load()
+    names = _declared_functions("hgx_hig.h")
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(hig.exported_symbols()) == names
 
 
 def test_version_string():

@@ -1,0 +1,164 @@
+"""The .hig driver on the device (libhgx_hig.so over libhgx.so) against the CPU oracle of the
+simulation loop (oracle/hig.py over the FP64 oracle kernels): whole images of the committed .hig
+fixtures, every path of the driver (fused grain-ensemble kernel, per-grain calls, mesh shapes, SAXS,
+scans of incidence and in-plane angles, the reference-compatible switches)."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle.hig as OH
+from hipgisaxs_amd import hig as H
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden" / "hig"
+TOL64 = 1e-10   # FP64 driver vs FP64 oracle, relative L2 over the image
+TOL32 = 1e-5    # FP32C paths
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _say(*a):
+    print("[hig]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _text(name):
+    return (GOLD / f"{name}.hig").read_text()
+
+
+@pytest.mark.parametrize("flags", [0, H.NO_FUSE], ids=["default", "per_grain"])
+@pytest.mark.parametrize("name", ["pyramid_dwba", "spheres_bcc", "cylinders_hcp", "box_grating", "saxs_cubes",
+                                  "mesh_tetra"])
+def test_fixture_image_matches_oracle(name, flags):
+    _say(name, "flags", flags)
+    text = _text(name)
+    sim = H.HigSimulation(text=text, base_dir=str(GOLD), flags=flags)
+    img = sim.compute()
+    counts = sim.last_counts()
+    ref = OH.run(text, base_dir=str(GOLD))
+    assert img.shape == ref.shape
+    assert np.all(np.isfinite(img)) and np.linalg.norm(ref) > 0
+    r = _rel(img, ref)
+    _say(name, "relL2", r, counts)
+    assert r <= TOL64
+    fusable = name in ("spheres_bcc", "cylinders_hcp")
+    assert (counts["fused_grains"] > 0) == (fusable and flags == 0)
+
+
+def test_scan_of_two_structures_matches_oracle():
+    """Two structures (iratio 3:1), a two-element cell with three locations, shape rotations, a gaussian
+    size distribution, gaussian-weighted orientations with grain spacing, a film layer, and a scan of
+    2 x 2 (alpha_i, phi): each run draws its orientations after the previous ones, as the reference's
+    run_all_gisaxs does."""
+    text = _text("two_structures")
+    sim = H.HigSimulation(text=text)
+    imgs = sim.run_all()
+    assert len(imgs) == 4
+    for c, img in enumerate(imgs):
+        ref = OH.run(text, config=c)
+        r = _rel(img, ref)
+        _say("two_structures config", c, "relL2", r)
+        assert r <= TOL64
+    # configurations differ (angles and grain draws), so the images must too
+    assert _rel(imgs[0], imgs[1]) > 1e-3 and _rel(imgs[0], imgs[2]) > 1e-3
+
+
+@pytest.mark.parametrize("name,flags,kw", [
+    ("spheres_bcc", H.SMEAR_COMPAT, {"smear_mode": "compat"}),
+    ("box_grating", H.SF_COMPAT_LB2, {"sf_flags": 1}),
+    ("mesh_tetra", H.MESH_NO_TRANSLATION, {"mesh_translate": False}),
+    ("cylinders_hcp", H.NO_SMEAR, {"smear_mode": "off"}),
+])
+def test_reference_compatible_switches(name, flags, kw):
+    _say(name, "flags", flags)
+    text = _text(name)
+    if flags == H.SF_COMPAT_LB2:  # the lb[2] term needs a lattice vector b out of the plane
+        text = text.replace("b = [0 1 0]", "b = [0 1 0.3]")
+    img = H.HigSimulation(text=text, base_dir=str(GOLD), flags=flags).compute()
+    ref = OH.run(text, base_dir=str(GOLD), **kw)
+    r = _rel(img, ref)
+    _say(name, "relL2", r)
+    assert r <= TOL64
+    # each switch changes the image against the default
+    dflt = OH.run(text, base_dir=str(GOLD))
+    assert _rel(ref, dflt) > 1e-6
+
+
+@pytest.mark.parametrize("name", ["spheres_bcc", "cylinders_hcp", "mesh_tetra"])
+def test_fp32c_paths(name):
+    _say(name, "fp32c")
+    text = _text(name)
+    img = H.HigSimulation(text=text, base_dir=str(GOLD), precision="fp32c").compute()
+    ref = OH.run(text, base_dir=str(GOLD))
+    r = _rel(img, ref)
+    _say(name, "fp32c relL2", r)
+    assert r <= TOL32
+
+
+def test_seed_changes_orientations_only_where_drawn():
+    text = _text("spheres_bcc")
+    a = H.HigSimulation(text=text, seed=1).compute()
+    b = H.HigSimulation(text=text, seed=2).compute()
+    assert _rel(a, b) > 1e-6
+    ref = OH.run(text, seed=2)
+    assert _rel(b, ref) <= TOL64
+    # a single-orientation input draws nothing
+    p1 = H.HigSimulation(text=_text("pyramid_dwba"), seed=1).compute()
+    p2 = H.HigSimulation(text=_text("pyramid_dwba"), seed=99).compute()
+    assert np.array_equal(p1, p2)
+
+
+def _full_spheres(maxgrains):
+    return _text("spheres_bcc").replace("resolution = [60 30]", "resolution = [600 300]").replace(
+        "maxgrains = [4 4 1]", f"maxgrains = [{maxgrains} {maxgrains} 1]")
+
+
+def test_full_detector_spheres_matches_oracle():
+    """The detector of the reference's spheres example (600 x 300) with 25 grains, against the oracle."""
+    text = _full_spheres(5)
+    sim = H.HigSimulation(text=text)
+    img = sim.compute()
+    _say("full detector 25 grains", sim.last_counts())
+    ref = OH.run(text)
+    r = _rel(img, ref)
+    _say("relL2", r)
+    assert r <= TOL64
+
+
+def test_full_spheres_example_paths_agree():
+    """All 625 grains of the reference's spheres example at its 600 x 300 detector: the fused kernel and
+    the per-grain calls agree (two independent device paths over the same sampled grains), and FP32C
+    stays within its bound of FP64."""
+    text = _full_spheres(25)
+    fused = H.HigSimulation(text=text)
+    a = fused.compute()
+    ca = fused.last_counts()
+    per = H.HigSimulation(text=text, flags=H.NO_FUSE)
+    b = per.compute()
+    cb = per.last_counts()
+    _say("fused", ca, "per-grain", cb)
+    assert ca["fused_grains"] == 625 and cb["grains"] == 625
+    r = _rel(a, b)
+    _say("fused vs per-grain relL2", r)
+    assert r <= 1e-12
+    c = H.HigSimulation(text=text, precision="fp32c").compute()
+    r32 = _rel(c, a)
+    _say("fp32c vs fp64 relL2", r32)
+    assert r32 <= TOL32
+
+
+def test_errors_are_reported_not_fatal():
+    text = _text("spheres_bcc").replace('unitcell:key = "cell"', 'unitcell:key = "nope"')
+    sim = H.HigSimulation(text=text)
+    with pytest.raises(Exception, match="nope"):
+        sim.compute()
+    # the driver stays usable after a failed run
+    ok = H.HigSimulation(text=_text("spheres_bcc")).compute()
+    assert np.all(np.isfinite(ok))
