@@ -19,6 +19,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -369,6 +370,69 @@ def bench_grating(ncand=16, iters=2):
                         "FP32C forward + adjoint + L-BFGS per hipGraph step"}
 
 
+HIG_GOLD = Path(__file__).resolve().parent / "tests" / "golden" / "hig"
+# the reference's example inputs (examples/01-03, 07) at their own detector sizes and grain counts,
+# rebuilt from the committed fixtures of the same models (which use smaller detectors and fewer grains)
+HIG_EXAMPLES = {
+    "01-pyramid": ("pyramid_dwba", {"resolution = [60 50]": "resolution = [300 250]"}),
+    "02-spheres-bcc": ("spheres_bcc", {"resolution = [60 30]": "resolution = [600 300]",
+                                       "maxgrains = [4 4 1]": "maxgrains = [{g} {g} 1]"}),
+    "03-cylinders-hcp": ("cylinders_hcp", {"resolution = [48 32]": "resolution = [600 300]",
+                                           "maxgrains = [3 3 1]": "maxgrains = [{g} {g} 1]"}),
+    "07-gratings": ("box_grating", {"resolution = [50 30]": "resolution = [600 300]",
+                                    "maxgrains = [3 3 1]": "maxgrains = [{g} {g} 1]"}),
+}
+
+
+def _hig_text(name, grains):
+    fixture, subs = HIG_EXAMPLES[name]
+    text = (HIG_GOLD / f"{fixture}.hig").read_text()
+    for a, b in subs.items():
+        assert a in text, (fixture, a)
+        text = text.replace(a, b.format(g=grains))
+    return text
+
+
+def bench_hig_examples(reps=3, cpu=True):
+    """The .hig driver on the reference's four example models at full size (600 x 300 or 300 x 250
+    detector, 625 grains for the lattice examples): wall time per image of compute_gisaxs, FP64, and
+    the CPU oracle of the same loop timed on 2 x 2 grains (extrapolated per grain)."""
+    from hipgisaxs_amd import hig as H
+    import oracle.hig as OH
+
+    out = {}
+    for name in HIG_EXAMPLES:
+        text = _hig_text(name, 25)
+        res = {}
+        for prec in ("fp64", "fp32c"):
+            sim = H.HigSimulation(text=text, precision=prec)
+            sim.run(0)  # grid, buffers, first launches
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                img = sim.run(0)
+            wall = (time.perf_counter() - t0) / reps
+            c = sim.last_counts()
+            res[prec] = {"ms_per_image": 1e3 * wall, "images_per_s": 1.0 / wall,
+                         "fused_grains": c["fused_grains"], "per_grain_calls": c["grains"]}
+            sim.close()
+        ng = 625 if name != "01-pyramid" else 1
+        res["grains"] = ng
+        res["pixels"] = int(img.size)
+        if cpu:
+            small = _hig_text(name, 2)
+            t0 = time.perf_counter()
+            OH.run(small)
+            t_small = time.perf_counter() - t0
+            per_grain = t_small / (4 if ng > 1 else 1)
+            res["cpu_baseline"] = {"kind": "port", "sample": "2 x 2 grains" if ng > 1 else "1 grain", "s": t_small,
+                                   "ms_per_image_extrapolated": 1e3 * per_grain * ng,
+                                   "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+            res["speedup_fp64_vs_cpu"] = res["cpu_baseline"]["ms_per_image_extrapolated"] / res["fp64"]["ms_per_image"]
+        out[name] = res
+        print(f"[bench] hig {name} {res['fp64']['ms_per_image']:.2f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -608,6 +672,8 @@ def main():
         print(f"[bench] fit done", file=sys.stderr, flush=True)
         result["grating_fit"] = bench_grating()
         print(f"[bench] grating done", file=sys.stderr, flush=True)
+        result["hig_examples"] = bench_hig_examples(cpu=not args.no_cpu)
+        print(f"[bench] hig examples done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb, (s, rows, ref) = cpu_baseline(shapes)
         got = Context.as_complex(s["out32"]).cpu().numpy()[: rows * s["side"]]
