@@ -27,6 +27,7 @@ FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOL
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
 SMEAR_COMPAT = 1
+METRICS = {"chi2": 0, "l2": 1, "sqrt_unit_l2": 2, "sqrt_c_l2": 3}
 SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}
 SHAPE_DIMS = {"sphere": 1, "cylinder": 2, "box": 3, "cube": 1, "prism3": 2, "prism6": 2, "pyramid": 4}
 MAX_NODES = 40
@@ -147,6 +148,10 @@ _SIGS = {
     "hgx_fit_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_chi2_fwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _vp]),
     "hgx_chi2_bwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _dp]),
+    "hgx_fit_plan_metric": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hgx_chi2_fwd_many": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int, _dp]),
+    "hgx_fit_lbfgs": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int), _dp, _dp, C.c_int, C.c_double, _dp,
+                                C.POINTER(C.c_int)]),
     "hgx_param_distribution": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                          _dp, _dp, C.c_int]),
     "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),

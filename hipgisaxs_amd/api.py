@@ -382,6 +382,33 @@ class FitPlan:
                         "chi2_bwd")
         return chi2.value, g
 
+    def metric(self, name: str) -> None:
+        """Objective: "chi2" (default), "l2", "sqrt_unit_l2" or "sqrt_c_l2" (the reference's distance
+        functors, include/analyzer/distance_functions.hpp)."""
+        self.ctx._check(self.ctx.lib.hgx_fit_plan_metric(self.ctx.h, self.id, N.METRICS[name]), "fit_plan_metric")
+
+    def fwd_many(self, thetas) -> np.ndarray:
+        """Objective of each row of thetas [n, 6] (one synchronisation)."""
+        th = _f64(thetas).reshape(-1, 6)
+        out = np.zeros(th.shape[0])
+        self.ctx._check(self.ctx.lib.hgx_chi2_fwd_many(self.ctx.h, self.id, th.shape[0],
+                                                       th.ctypes.data_as(C.POINTER(C.c_double)), N.PREC_FP64,
+                                                       out.ctypes.data_as(C.POINTER(C.c_double))), "chi2_fwd_many")
+        return out
+
+    def lbfgs(self, theta, free=(1, 1, 1, 1, 1, 1), lo=None, hi=None, iterations=100, tol=1e-9):
+        """Bounded L-BFGS with the adjoint gradient; returns (theta, objective, iterations)."""
+        th = _f64(theta).ravel().copy()
+        fr = np.ascontiguousarray(free, dtype=np.int32)
+        lo = _f64(np.full(6, -np.inf) if lo is None else lo).ravel()
+        hi = _f64(np.full(6, np.inf) if hi is None else hi).ravel()
+        f, it = C.c_double(), C.c_int()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self.ctx._check(self.ctx.lib.hgx_fit_lbfgs(self.ctx.h, self.id, dp(th), fr.ctypes.data_as(C.POINTER(C.c_int)),
+                                                   dp(lo), dp(hi), int(iterations), float(tol), C.byref(f),
+                                                   C.byref(it)), "fit_lbfgs")
+        return th, f.value, it.value
+
     def release(self):
         self.ctx._check(self.ctx.lib.hgx_fit_plan_release(self.ctx.h, self.id), "fit_plan_release")
 

@@ -12,7 +12,17 @@
 // with respect to the six parameters (no stored intermediates), forms
 // dchi2/dtheta_j = sum_p 2 mask_p (I_p - O_p)/max(O_p,1) dI_p/dtheta_j, and the per-pixel terms
 // are reduced per block in LDS and then over blocks in a fixed order (deterministic).
+//
+// Other objectives (hgx_fit_plan_metric): the reference's distance functors
+// (include/analyzer/distance_functions.hpp) -- the masked squared difference
+// (AbsoluteDifferenceSquare, :125-143) and the two square-root normalised forms
+// (SqrtUnitVectorNormL2DistanceSquare :423-456, SqrtCNormL2DistanceSquare :460-499). The normalised
+// forms depend on image-wide sums, so they take a statistics pass (six masked sums) before the
+// gradient pass, whose per-pixel seed dD/dI_p is formed from those sums (derivation in the kernel).
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <vector>
 
 #include "hgx_internal.hpp"
 
@@ -32,7 +42,14 @@ struct FitArgs {
   double rot0[9];
   int compat_lb2;
   double th[6];  // radius, height, a, tilt, sigma, scale
+  int metric;    // HGX_METRIC_*
+  double k[3];   // constants of the normalised metrics' per-pixel seed (from the statistics pass)
 };
+
+// kernel modes
+constexpr int kModeValue = 0;  // objective terms (chi^2, L2) in slot 0
+constexpr int kModeGrad = 1;   // objective terms in slot 0 and gradient terms in slots 1..6
+constexpr int kModeStats = 2;  // six masked sums of the normalised metrics in slots 0..5
 
 struct QViewF {
   const double* __restrict__ qx;
@@ -72,12 +89,13 @@ __device__ inline void sinc_expi(double u, cd* G, cd* dG) {
   *dG = cd{(c2 - G->r) / u, (s2 - G->i) / u};
 }
 
-template <bool GRAD>
+template <int MODE>
 __global__ __launch_bounds__(kFitWG) void k_fit(QViewF g, const double* __restrict__ fc, const double* __restrict__ obs,
                                                 const double* __restrict__ mask, FitArgs a, double* __restrict__ Imod,
                                                 double* __restrict__ part) {
   __shared__ double red[kNOut][kFitWG / 64];
   const int64_t p = (int64_t)blockIdx.x * kFitWG + threadIdx.x;
+  constexpr bool GRAD = MODE == kModeGrad;
   double out[kNOut];
 #pragma unroll
   for (int k = 0; k < kNOut; ++k) out[k] = 0.0;
@@ -186,13 +204,44 @@ __global__ __launch_bounds__(kFitWG) void k_fit(QViewF g, const double* __restri
     if (Imod) Imod[p] = I;
     const double o = obs ? obs[p] : 0.0;
     const double mk = mask ? mask[p] : 1.0;
-    const double wgt = mk / fmax(o, 1.0);
-    out[0] = wgt * (I - o) * (I - o);
-    if (GRAD) {
-      const double gI = 2.0 * wgt * (I - o);
+    // square roots of model and data for the normalised metrics (a negative value contributes 0, as
+    // the reference's NaN guards make it)
+    const double sI = I > 0.0 ? sqrt(I) : 0.0, sO = o > 0.0 ? sqrt(o) : 0.0;
+    if (MODE == kModeStats) {
+      // A1 = sum m I, A2 = sum m sqrt(IO), A3 = sum m O, B1 = sum m^2 I, B2 = sum m^2 sqrt(IO),
+      // B3 = sum m^2 O (norm_l2 and vec_dot weight by the mask, the squared distance by its square)
+      out[0] = mk * sI * sI;
+      out[1] = mk * sI * sO;
+      out[2] = mk * sO * sO;
+      out[3] = mk * mk * sI * sI;
+      out[4] = mk * mk * sI * sO;
+      out[5] = mk * mk * sO * sO;
+    } else {
+      double gI = 0.0;  // dD/dI_p
+      if (a.metric == HGX_METRIC_CHI2) {
+        const double wgt = mk / fmax(o, 1.0);
+        out[0] = wgt * (I - o) * (I - o);
+        gI = 2.0 * wgt * (I - o);
+      } else if (a.metric == HGX_METRIC_L2) {
+        const double d = mk * (I - o);
+        out[0] = d * d;
+        gI = 2.0 * mk * d;
+      } else if (sI > 0.0) {
+        // s = sqrt(I), r = sqrt(O), dD/dI = (dD/ds) / (2 s).
+        // sqrt-unit-L2: D = sum m^2 (r/Nr - s/Ns)^2, Ns^2 = A1, Nr^2 = A3, L = B1/Ns - B2/Nr:
+        //   dD/dI_p = m^2/Ns^2 - m^2 r/(Nr Ns s) - m L/Ns^3          (k = 1/Ns^2, 1/(Nr Ns), L/Ns^3)
+        // sqrt-c-L2: D = sum m^2 (r - c s)^2, c = A2/A1, K = B2 - c B1:
+        //   dD/dI_p = -m^2 c (r/s - c) - (K/A1) m (r/s - 2c)         (k = c, K/A1)
+        const double ratio = sO / sI;
+        gI = a.metric == HGX_METRIC_SQRT_UNIT_L2
+                 ? mk * mk * a.k[0] - mk * mk * a.k[1] * ratio - mk * a.k[2]
+                 : -mk * mk * a.k[0] * (ratio - a.k[0]) - a.k[1] * mk * (ratio - 2.0 * a.k[0]);
+      }
+      if (GRAD) {
 #pragma unroll
-      for (int j = 0; j < 5; ++j) out[1 + j] = gI * 2.0 * sc * (amp.r * dA[j].r + amp.i * dA[j].i);
-      out[6] = gI * A2;
+        for (int j = 0; j < 5; ++j) out[1 + j] = gI * 2.0 * sc * (amp.r * dA[j].r + amp.i * dA[j].i);
+        out[6] = gI * A2;
+      }
     }
   }
   // block reduction: wave shuffles, then the 4 wave sums through LDS, in a fixed order
@@ -237,43 +286,49 @@ __global__ __launch_bounds__(256) void k_fit_finish(const double* __restrict__ p
 
 static std::vector<FitPlan>& plans(Context* c) { return c->fit_plans; }
 
-static int run_fit(Context* c, int plan, const double theta[6], int precision, bool grad, double* chi2,
-                   double* gradient, double* Imod) {
+static int check_plan(Context* c, int plan) {
   auto& P = plans(c);
   if (plan < 0 || plan >= (int)P.size() || !P[plan].live) return set_error(c, HGX_ERR_BAD_HANDLE, "fit: unknown plan %d", plan);
-  if (!theta || !chi2) return set_error(c, HGX_ERR_INVALID_ARG, "fit: null theta or chi2");
-  if (precision != HGX_PREC_FP64) return set_error(c, HGX_ERR_UNSUPPORTED, "fit: FP64 only");
   if (!c->qz_real) return set_error(c, HGX_ERR_UNSUPPORTED, "fit: needs real q (non-absorbing layer)");
-  const FitPlan& fp = P[plan];
+  return HGX_OK;
+}
+
+static int pack_fit(Context* c, const FitPlan& fp, const double theta[6], FitArgs* a) {
   const hgx_fit_model& m = fp.model;
   if (!(theta[0] > 0) || (m.shape == HGX_SHAPE_CYLINDER && !(theta[1] > 0)) || !(theta[2] > 0))
     return set_error(c, HGX_ERR_INVALID_ARG, "fit: radius, height and lattice constant must be positive");
-  FitArgs a;
-  a.shape = m.shape;
-  a.nb = c->nbranch;
+  a->shape = m.shape;
+  a->nb = c->nbranch;
   for (int k = 0; k < 3; ++k) {
-    a.ua[k] = m.ua[k];
-    a.ub[k] = m.ub[k];
-    a.uc[k] = m.uc[k];
-    a.N[k] = m.repet[k] < 1 ? 1 : m.repet[k];
-    a.center[k] = m.center[k];
+    a->ua[k] = m.ua[k];
+    a->ub[k] = m.ub[k];
+    a->uc[k] = m.uc[k];
+    a->N[k] = m.repet[k] < 1 ? 1 : m.repet[k];
+    a->center[k] = m.center[k];
   }
-  for (int k = 0; k < 9; ++k) a.rot0[k] = m.rot0[k];
-  a.dn2r = m.dn2_re;
-  a.dn2i = m.dn2_im;
-  a.compat_lb2 = (m.sf_flags & HGX_SF_COMPAT_LB2) != 0;
-  for (int k = 0; k < 6; ++k) a.th[k] = theta[k];
+  for (int k = 0; k < 9; ++k) a->rot0[k] = m.rot0[k];
+  a->dn2r = m.dn2_re;
+  a->dn2i = m.dn2_im;
+  a->compat_lb2 = (m.sf_flags & HGX_SF_COMPAT_LB2) != 0;
+  for (int k = 0; k < 6; ++k) a->th[k] = theta[k];
+  a->metric = fp.metric;
+  a->k[0] = a->k[1] = a->k[2] = 0.0;
+  return HGX_OK;
+}
+
+// One pass of k_fit in `mode` plus the fixed-order block reduction into res[kNOut] (device).
+static int launch_fit(Context* c, const FitPlan& fp, const FitArgs& a, int mode, double* Imod, double* part,
+                      double* res) {
   unsigned nblk = ceil_div(c->npix, kFitWG);
-  if (int rc = ensure_work(c, ((size_t)nblk * kNOut + kNOut) * sizeof(double))) return rc;
-  double* part = reinterpret_cast<double*>(c->d_work);
-  double* res = part + (size_t)nblk * kNOut;
   QViewF g{c->d_qx, c->d_qy, c->d_qz, c->npix};
   {
     LaunchTimer lt(c, true);
-    if (grad)
-      k_fit<true><<<nblk, kFitWG, 0, c->stream>>>(g, fp.fc, fp.obs, fp.mask, a, Imod, part);
+    if (mode == kModeGrad)
+      k_fit<kModeGrad><<<nblk, kFitWG, 0, c->stream>>>(g, fp.fc, fp.obs, fp.mask, a, Imod, part);
+    else if (mode == kModeStats)
+      k_fit<kModeStats><<<nblk, kFitWG, 0, c->stream>>>(g, fp.fc, fp.obs, fp.mask, a, Imod, part);
     else
-      k_fit<false><<<nblk, kFitWG, 0, c->stream>>>(g, fp.fc, fp.obs, fp.mask, a, Imod, part);
+      k_fit<kModeValue><<<nblk, kFitWG, 0, c->stream>>>(g, fp.fc, fp.obs, fp.mask, a, Imod, part);
   }
   HGX_CHECK_LAUNCH(c);
   {
@@ -281,15 +336,221 @@ static int run_fit(Context* c, int plan, const double theta[6], int precision, b
     k_fit_finish<<<1, 256, 0, c->stream>>>(part, (int)nblk, res);
   }
   HGX_CHECK_LAUNCH(c);
-  double host[kNOut];
-  HGX_CHECK_HIP(c, hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, c->stream));
+  return HGX_OK;
+}
+
+// The normalised metrics from the six sums of the statistics pass: the distance D and the seed
+// constants k[0..2] (see k_fit). A zero model image (A1 = 0) is taken as u = 0 (sqrt-unit) and c = 0.
+static double metric_from_sums(int metric, const double* S, double k[3]) {
+  const double A1 = S[0], A2 = S[1], A3 = S[2], B1 = S[3], B2 = S[4], B3 = S[5];
+  k[0] = k[1] = k[2] = 0.0;
+  if (metric == HGX_METRIC_SQRT_UNIT_L2) {
+    const double Nr = std::sqrt(A3), Ns = std::sqrt(A1);
+    double D = Nr > 0 ? B3 / A3 : 0.0;
+    if (Ns > 0) {
+      D += B1 / A1 - (Nr > 0 ? 2.0 * B2 / (Nr * Ns) : 0.0);
+      const double L = B1 / Ns - (Nr > 0 ? B2 / Nr : 0.0);
+      k[0] = 1.0 / A1;
+      k[1] = Nr > 0 ? 1.0 / (Nr * Ns) : 0.0;
+      k[2] = L / (A1 * Ns);
+    }
+    return D;
+  }
+  // sqrt-c-L2
+  const double cc = A1 > 0 ? A2 / A1 : 0.0;
+  const double K = B2 - cc * B1;
+  k[0] = cc;
+  k[1] = A1 > 0 ? K / A1 : 0.0;
+  return B3 - 2.0 * cc * B2 + cc * cc * B1;
+}
+
+static bool normalised(int metric) {
+  return metric == HGX_METRIC_SQRT_UNIT_L2 || metric == HGX_METRIC_SQRT_C_L2;
+}
+
+static int run_fit(Context* c, int plan, const double theta[6], int precision, bool grad, double* chi2,
+                   double* gradient, double* Imod) {
+  if (int rc = check_plan(c, plan)) return rc;
+  if (!theta || !chi2) return set_error(c, HGX_ERR_INVALID_ARG, "fit: null theta or chi2");
+  if (precision != HGX_PREC_FP64) return set_error(c, HGX_ERR_UNSUPPORTED, "fit: FP64 only");
+  const FitPlan& fp = plans(c)[plan];
+  FitArgs a;
+  if (int rc = pack_fit(c, fp, theta, &a)) return rc;
+  unsigned nblk = ceil_div(c->npix, kFitWG);
+  if (int rc = ensure_work(c, ((size_t)nblk * kNOut + 2 * kNOut) * sizeof(double))) return rc;
+  double* part = reinterpret_cast<double*>(c->d_work);
+  double* res = part + (size_t)nblk * kNOut;
+  double host[2 * kNOut];
+  const bool norm = normalised(fp.metric);
+  if (norm) {
+    if (int rc = launch_fit(c, fp, a, kModeStats, Imod, part, res)) return rc;
+    HGX_CHECK_HIP(c, hipMemcpyAsync(host, res, kNOut * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    *chi2 = metric_from_sums(fp.metric, host, a.k);
+    if (!grad) return HGX_OK;
+    if (int rc = launch_fit(c, fp, a, kModeGrad, nullptr, part, res + kNOut)) return rc;
+  } else {
+    if (int rc = launch_fit(c, fp, a, grad ? kModeGrad : kModeValue, Imod, part, res + kNOut)) return rc;
+  }
+  HGX_CHECK_HIP(c, hipMemcpyAsync(host + kNOut, res + kNOut, kNOut * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
   HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-  *chi2 = host[0];
+  if (!norm) *chi2 = host[kNOut];
   if (grad && gradient) {
     // kernel order: R, H, a, tilt, sigma, scale
-    for (int k = 0; k < 6; ++k) gradient[k] = host[1 + k];
-    if (m.shape == HGX_SHAPE_SPHERE) gradient[1] = 0.0;
+    for (int k = 0; k < 6; ++k) gradient[k] = host[kNOut + 1 + k];
+    if (fp.model.shape == HGX_SHAPE_SPHERE) gradient[1] = 0.0;
   }
+  return HGX_OK;
+}
+
+// Objective of n parameter vectors queued back to back, one synchronisation at the end.
+static int run_fit_many(Context* c, int plan, int n, const double* thetas, double* out) {
+  if (int rc = check_plan(c, plan)) return rc;
+  const FitPlan& fp = plans(c)[plan];
+  unsigned nblk = ceil_div(c->npix, kFitWG);
+  if (int rc = ensure_work(c, ((size_t)nblk * kNOut + (size_t)n * kNOut) * sizeof(double))) return rc;
+  double* part = reinterpret_cast<double*>(c->d_work);
+  double* res = part + (size_t)nblk * kNOut;
+  const bool norm = normalised(fp.metric);
+  for (int i = 0; i < n; ++i) {
+    FitArgs a;
+    if (int rc = pack_fit(c, fp, thetas + 6 * (size_t)i, &a)) return rc;
+    // the next pass reuses `part` only after this finish has read it (same stream)
+    if (int rc = launch_fit(c, fp, a, norm ? kModeStats : kModeValue, nullptr, part, res + (size_t)i * kNOut))
+      return rc;
+  }
+  std::vector<double> host((size_t)n * kNOut);
+  HGX_CHECK_HIP(c, hipMemcpyAsync(host.data(), res, host.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    double k[3];
+    out[i] = norm ? metric_from_sums(fp.metric, &host[(size_t)i * kNOut], k) : host[(size_t)i * kNOut];
+  }
+  return HGX_OK;
+}
+
+// Projected L-BFGS (memory 6) over the free parameters within [lo, hi]: two-loop recursion on the
+// projected gradient, backtracking (Armijo, c1 = 1e-4) along the projected path, each trial point
+// one value-and-gradient pass. Stops when the projected gradient's largest component times the
+// parameter range falls below tol * |f|, or when a line search cannot decrease f.
+static int run_lbfgs(Context* c, int plan, double theta[6], const int freep[6], const double lo[6],
+                     const double hi[6], int iters, double tol, double* f_out, int* done) {
+  const int M = 6;
+  double x[6], g[6], f;
+  for (int j = 0; j < 6; ++j) x[j] = std::min(std::max(theta[j], lo[j]), hi[j]);
+  if (int rc = run_fit(c, plan, x, HGX_PREC_FP64, true, &f, g, nullptr)) return rc;
+  std::vector<std::array<double, 6>> S, Y;
+  std::vector<double> rho;
+  auto proj_grad = [&](const double* xx, const double* gg, double* pg) {
+    for (int j = 0; j < 6; ++j) {
+      pg[j] = freep[j] ? gg[j] : 0.0;
+      if ((xx[j] <= lo[j] && pg[j] > 0) || (xx[j] >= hi[j] && pg[j] < 0)) pg[j] = 0.0;
+    }
+  };
+  int it = 0;
+  for (; it < iters; ++it) {
+    double pg[6];
+    proj_grad(x, g, pg);
+    double gmax = 0.0;
+    for (int j = 0; j < 6; ++j) gmax = std::max(gmax, std::fabs(pg[j]) * std::max(hi[j] - lo[j], 1e-300));
+    if (gmax <= tol * std::max(std::fabs(f), 1e-300)) break;
+    // two-loop recursion
+    double d[6], al[M];
+    for (int j = 0; j < 6; ++j) d[j] = -pg[j];
+    for (int k = (int)S.size() - 1; k >= 0; --k) {
+      double sd = 0;
+      for (int j = 0; j < 6; ++j) sd += S[k][j] * d[j];
+      al[k] = rho[k] * sd;
+      for (int j = 0; j < 6; ++j) d[j] -= al[k] * Y[k][j];
+    }
+    if (!S.empty()) {
+      const auto& s = S.back();
+      const auto& y = Y.back();
+      double sy = 0, yy = 0;
+      for (int j = 0; j < 6; ++j) {
+        sy += s[j] * y[j];
+        yy += y[j] * y[j];
+      }
+      for (int j = 0; j < 6; ++j) d[j] *= sy / yy;
+    } else {
+      // first step: at most 1 % of each free parameter's range
+      double dm = 0;
+      for (int j = 0; j < 6; ++j)
+        if (freep[j]) dm = std::max(dm, std::fabs(d[j]) / (0.01 * (hi[j] - lo[j])));
+      if (dm > 0)
+        for (int j = 0; j < 6; ++j) d[j] /= dm;
+    }
+    for (size_t k = 0; k < S.size(); ++k) {
+      double yd = 0;
+      for (int j = 0; j < 6; ++j) yd += Y[k][j] * d[j];
+      double be = rho[k] * yd;
+      for (int j = 0; j < 6; ++j) d[j] += S[k][j] * (al[k] - be);
+    }
+    for (int j = 0; j < 6; ++j)
+      if (!freep[j]) d[j] = 0.0;
+    double gd = 0;
+    for (int j = 0; j < 6; ++j) gd += pg[j] * d[j];
+    if (!(gd < 0)) {  // not a descent direction: restart from the projected steepest descent
+      S.clear();
+      Y.clear();
+      rho.clear();
+      double dm = 0;
+      for (int j = 0; j < 6; ++j) {
+        d[j] = -pg[j];
+        if (freep[j]) dm = std::max(dm, std::fabs(d[j]) / (0.01 * (hi[j] - lo[j])));
+      }
+      if (dm > 0)
+        for (int j = 0; j < 6; ++j) d[j] /= dm;
+      gd = 0;
+      for (int j = 0; j < 6; ++j) gd += pg[j] * d[j];
+    }
+    double step = 1.0, xn[6], gn[6], fn = f;
+    bool ok = false;
+    for (int ls = 0; ls < 30; ++ls) {
+      double dec = 0;
+      for (int j = 0; j < 6; ++j) {
+        xn[j] = std::min(std::max(x[j] + step * d[j], lo[j]), hi[j]);
+        dec += pg[j] * (xn[j] - x[j]);
+      }
+      if (int rc = run_fit(c, plan, xn, HGX_PREC_FP64, true, &fn, gn, nullptr)) {
+        if (rc != HGX_ERR_INVALID_ARG) return rc;  // a trial outside the model's domain: shorten the step
+        step *= 0.5;
+        continue;
+      }
+      if (std::isfinite(fn) && fn <= f + 1e-4 * dec) {
+        ok = true;
+        break;
+      }
+      step *= 0.5;
+    }
+    if (!ok) break;
+    std::array<double, 6> s, y;
+    double sy = 0;
+    for (int j = 0; j < 6; ++j) {
+      s[j] = xn[j] - x[j];
+      y[j] = freep[j] ? gn[j] - g[j] : 0.0;
+      sy += s[j] * y[j];
+    }
+    if (sy > 1e-300) {
+      if ((int)S.size() == M) {
+        S.erase(S.begin());
+        Y.erase(Y.begin());
+        rho.erase(rho.begin());
+      }
+      S.push_back(s);
+      Y.push_back(y);
+      rho.push_back(1.0 / sy);
+    }
+    for (int j = 0; j < 6; ++j) {
+      x[j] = xn[j];
+      g[j] = gn[j];
+    }
+    f = fn;
+  }
+  for (int j = 0; j < 6; ++j) theta[j] = x[j];
+  if (f_out) *f_out = f;
+  if (done) *done = it;
   return HGX_OK;
 }
 
@@ -345,4 +606,36 @@ extern "C" int hgx_chi2_bwd(hgx_ctx* h, int plan, const double theta[6], int pre
   if (!grad) return set_error(c, HGX_ERR_INVALID_ARG, "chi2_bwd: null gradient");
   if (int rc = set_device(c)) return rc;
   return run_fit(c, plan, theta, precision, true, chi2, grad, nullptr);
+}
+
+extern "C" int hgx_fit_plan_metric(hgx_ctx* h, int plan, int metric) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  auto& P = plans(c);
+  if (plan < 0 || plan >= (int)P.size() || !P[plan].live) return set_error(c, HGX_ERR_BAD_HANDLE, "fit: unknown plan");
+  if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_SQRT_C_L2)
+    return set_error(c, HGX_ERR_INVALID_ARG, "fit_plan_metric: unknown metric %d", metric);
+  P[plan].metric = metric;
+  return HGX_OK;
+}
+
+extern "C" int hgx_chi2_fwd_many(hgx_ctx* h, int plan, int n, const double* thetas, int precision, double* out) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (n < 1 || !thetas || !out) return set_error(c, HGX_ERR_INVALID_ARG, "chi2_fwd_many: bad arguments");
+  if (precision != HGX_PREC_FP64) return set_error(c, HGX_ERR_UNSUPPORTED, "fit: FP64 only");
+  if (int rc = set_device(c)) return rc;
+  return run_fit_many(c, plan, n, thetas, out);
+}
+
+extern "C" int hgx_fit_lbfgs(hgx_ctx* h, int plan, double theta[6], const int free_params[6], const double lo[6],
+                             const double hi[6], int iterations, double tol, double* objective, int* iterations_done) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!theta || !free_params || !lo || !hi || iterations < 0 || !(tol >= 0))
+    return set_error(c, HGX_ERR_INVALID_ARG, "fit_lbfgs: bad arguments");
+  for (int j = 0; j < 6; ++j)
+    if (!(lo[j] <= hi[j])) return set_error(c, HGX_ERR_INVALID_ARG, "fit_lbfgs: lo > hi for parameter %d", j);
+  if (int rc = set_device(c)) return rc;
+  return run_lbfgs(c, plan, theta, free_params, lo, hi, iterations, tol, objective, iterations_done);
 }

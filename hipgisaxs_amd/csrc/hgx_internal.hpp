@@ -43,6 +43,7 @@ struct Mesh {
 // A chi^2 plan: model constants and the device inputs it reads (caller-owned).
 struct FitPlan {
   bool live = false;
+  int metric = 0;  // HGX_METRIC_*
   hgx_fit_model model;
   const double* fc = nullptr;
   const double* obs = nullptr;

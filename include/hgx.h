@@ -287,12 +287,36 @@ typedef struct hgx_fit_model {
 int hgx_fit_plan_create(hgx_ctx* ctx, const hgx_fit_model* model, const double* fc, const double* I_obs,
                         const double* mask, int* plan);
 int hgx_fit_plan_release(hgx_ctx* ctx, int plan);
-/* chi2 = sum_p mask_p (I_p(theta) - O_p)^2 / max(O_p, 1); I_model (device [npix]) may be NULL.
+/* Objective of the plan (HGX_METRIC_CHI2 unless hgx_fit_plan_metric chose another):
+ * chi2 = sum_p mask_p (I_p(theta) - O_p)^2 / max(O_p, 1); I_model (device [npix]) may be NULL.
  * Synchronises the stream (the scalar is returned to the host). */
 int hgx_chi2_fwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, double* chi2, double* I_model);
-/* chi2 and dchi2/dtheta (6 components, the height component is 0 for spheres) by the hand-written
- * adjoint: one pass over the pixels that recomputes the amplitudes instead of storing them. */
+/* The objective and its gradient (6 components, the height component is 0 for spheres) by the hand-written
+ * adjoint: one pass over the pixels that recomputes the amplitudes instead of storing them (the
+ * normalised metrics add one statistics pass before it). */
 int hgx_chi2_bwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, double* chi2, double grad[6]);
+
+/* Objectives of a plan: the default chi^2, and the reference's distance functors
+ * (include/analyzer/distance_functions.hpp, HiG fitting keyword "distancemetric") with O = I_obs, I the
+ * model image and m the mask: */
+#define HGX_METRIC_CHI2 0         /* sum m (I - O)^2 / max(O, 1) */
+#define HGX_METRIC_L2 1           /* AbsoluteDifferenceSquare (:125-143): sum (m |O - I|)^2 */
+#define HGX_METRIC_SQRT_UNIT_L2 2 /* "sqrt_unit_l2", SqrtUnitVectorNormL2DistanceSquare (:423-456):
+                                     sum (m |sqrt O/||sqrt O|| - sqrt I/||sqrt I|| |)^2, ||x||^2 = sum m x^2 */
+#define HGX_METRIC_SQRT_C_L2 3    /* "sqrt_c_l2", SqrtCNormL2DistanceSquare (:460-499):
+                                     sum (m |sqrt O - c sqrt I|)^2, c = <sqrt I, sqrt O>_m / ||sqrt I||^2 */
+int hgx_fit_plan_metric(hgx_ctx* ctx, int plan, int metric);
+/* Objective values of n parameter vectors (host [n][6]) into out[n] (host): the population evaluation
+ * of the reference's particle-swarm fit (src/analyzer/hipgisaxs_fit_pso.cpp:521-543), queued back to back
+ * with one synchronisation. FP64. */
+int hgx_chi2_fwd_many(hgx_ctx* ctx, int plan, int n, const double* thetas, int precision, double* out);
+/* Bounded L-BFGS (memory 6) on the plan's objective with the adjoint gradient, in place of the
+ * reference's TAO LMVM with central finite differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224):
+ * theta (in: start, out: result) moves only in the components with free_params[j] != 0, kept within
+ * [lo, hi]; stops after `iterations` or when max_j |projected dD/dtheta_j| (hi_j - lo_j) <= tol |D|.
+ * objective and iterations_done may be NULL. Host loop, one value-and-gradient pass per trial point. */
+int hgx_fit_lbfgs(hgx_ctx* ctx, int plan, double theta[6], const int free_params[6], const double lo[6],
+                  const double hi[6], int iterations, double tol, double* objective, int* iterations_done);
 
 /* ---- batched trapezoid-grating fit --------------------------------------------------------- */
 

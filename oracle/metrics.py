@@ -1,0 +1,53 @@
+"""Distance functors of the reference's fitting (include/analyzer/distance_functions.hpp) restated in
+numpy over an observed image O, a model image I and a mask m (test infrastructure only).
+
+norm_l2(x) = sqrt(sum m x^2) and vec_dot(a, b) = sum m |a b| with NaN terms dropped (:40-57); each
+distance sums (m |difference|)^2 with non-finite terms dropped."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _nan0(x):
+    return np.where(np.isnan(x), 0.0, x)
+
+
+def norm_l2(x, m):
+    return float(np.sqrt(np.sum(m * _nan0(x * x))))
+
+
+def vec_dot(a, b, m):
+    return float(np.sum(m * _nan0(np.abs(a * b))))
+
+
+def _sumsq(t):
+    t = np.where(np.isfinite(t), t, 0.0)
+    return float(np.sum(t * t))
+
+
+def chi2(O, I, m):
+    """The build's default objective: sum m (I - O)^2 / max(O, 1)."""
+    return float(np.sum(m * (I - O) ** 2 / np.maximum(O, 1.0)))
+
+
+def l2(O, I, m):
+    """AbsoluteDifferenceSquare (:125-143)."""
+    return _sumsq(m * np.abs(O - I))
+
+
+def sqrt_unit_l2(O, I, m):
+    """SqrtUnitVectorNormL2DistanceSquare (:423-456)."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sr, sd = np.sqrt(O), np.sqrt(I)
+        return _sumsq(m * np.abs(sr / norm_l2(sr, m) - sd / norm_l2(sd, m)))
+
+
+def sqrt_c_l2(O, I, m):
+    """SqrtCNormL2DistanceSquare (:460-499)."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        sr, sd = np.sqrt(O), np.sqrt(I)
+        c = vec_dot(sd, sr, m) / norm_l2(sd, m) ** 2
+        return _sumsq(m * np.abs(sr - c * sd))
+
+
+METRICS = {"chi2": chi2, "l2": l2, "sqrt_unit_l2": sqrt_unit_l2, "sqrt_c_l2": sqrt_c_l2}

@@ -106,3 +106,86 @@ def test_chi2_forward_and_adjoint(gpu_ctx, oracle_mod, setup, shape):
     ref_I = _oracle_I(O, g, fc, theta_true if shape == "cylinder" else theta, shape)
     assert rel_l2(Im.cpu().numpy(), ref_I) <= 1e-10
     plan.release()
+
+
+@pytest.mark.parametrize("metric", ["chi2", "l2", "sqrt_unit_l2", "sqrt_c_l2"])
+def test_distance_metrics_and_their_adjoints(gpu_ctx, oracle_mod, setup, metric):
+    """The reference's distance functors as objectives: value against the oracle's restatement on the
+    oracle image, gradient against central differences of that oracle objective."""
+    from hipgisaxs_amd.api import FitPlan
+    from oracle import metrics as M
+
+    O = oracle_mod
+    g, fc, obs, mask, theta_true = setup
+    gpu_ctx.qgrid_set(g.ncol, g.nrow, g.qx, g.qy, g.qz, g.alpha_f)
+    plan = FitPlan(gpu_ctx, "cylinder", UA, UB, UC, REP, _dev(obs), fc=_dev(fc), mask=_dev(mask), dn2=DN2)
+    plan.metric(metric)
+    theta = theta_true * np.array([1.02, 0.98, 1.005, 1.0, 1.1, 0.9]) + np.array([0, 0, 0, 0.01, 0, 0])
+    f_or = lambda th: M.METRICS[metric](obs, _oracle_I(O, g, fc, th), mask)
+    ref = f_or(theta)
+    val = plan.fwd(theta)
+    print(metric, "value", val, ref)
+    assert abs(val - ref) <= 1e-10 * abs(ref)
+    v2, grad = plan.bwd(theta)
+    assert abs(v2 - ref) <= 1e-10 * abs(ref)
+    fd = np.zeros(6)
+    for j in range(6):
+        h = 1e-6 * max(abs(theta[j]), 1e-2)
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (f_or(tp) - f_or(tm)) / (2 * h)
+    print(metric, "grad", grad, "fd", fd)
+    scale = np.abs(fd).max()
+    for j in range(6):
+        assert abs(grad[j] - fd[j]) <= 1e-5 * max(abs(fd[j]), 1e-6 * scale), (metric, j, grad[j], fd[j])
+    if metric in ("sqrt_unit_l2", "sqrt_c_l2"):  # scale-invariant objectives: no gradient along the scale
+        assert abs(grad[5]) <= 1e-9 * scale
+    plan.release()
+
+
+def test_population_evaluation_matches_single_calls(gpu_ctx, setup):
+    from hipgisaxs_amd.api import FitPlan
+
+    g, fc, obs, mask, theta_true = setup
+    gpu_ctx.qgrid_set(g.ncol, g.nrow, g.qx, g.qy, g.qz, g.alpha_f)
+    plan = FitPlan(gpu_ctx, "cylinder", UA, UB, UC, REP, _dev(obs), fc=_dev(fc), mask=_dev(mask), dn2=DN2)
+    rng = np.random.default_rng(3)
+    pop = theta_true * (1 + 0.05 * rng.standard_normal((9, 6)))
+    pop[:, 3] = 0.02 + 0.01 * rng.standard_normal(9)
+    for metric in ("chi2", "sqrt_c_l2"):
+        plan.metric(metric)
+        many = plan.fwd_many(pop)
+        single = np.array([plan.fwd(t) for t in pop])
+        assert np.array_equal(many, single), (metric, many, single)
+    plan.release()
+
+
+@pytest.mark.parametrize("metric,free", [("chi2", (1, 1, 1, 1, 1, 1)), ("sqrt_c_l2", (1, 1, 1, 1, 1, 0))])
+def test_lbfgs_fit_recovers_the_parameters(gpu_ctx, setup, metric, free):
+    """Bounded L-BFGS with the adjoint gradient from a start 3-8 % off: the objective ends at or below
+    its value at the true parameters (the noisy data's optimum lies near them) and the parameters come
+    back to within the noise."""
+    from hipgisaxs_amd.api import FitPlan
+
+    g, fc, obs, mask, theta_true = setup
+    gpu_ctx.qgrid_set(g.ncol, g.nrow, g.qx, g.qy, g.qz, g.alpha_f)
+    plan = FitPlan(gpu_ctx, "cylinder", UA, UB, UC, REP, _dev(obs), fc=_dev(fc), mask=_dev(mask), dn2=DN2)
+    plan.metric(metric)
+    start = theta_true * np.array([1.05, 0.95, 1.03, 1.0, 1.08, 1.0 if not free[5] else 0.92])
+    start[3] = 0.0
+    lo = np.array([2.0, 6.0, 10.0, -0.1, 0.0, 0.1])
+    hi = np.array([6.0, 20.0, 20.0, 0.1, 2.0, 10.0])
+    f_true = plan.fwd(theta_true)
+    f0 = plan.fwd(start)
+    th, f, it = plan.lbfgs(start, free=free, lo=lo, hi=hi, iterations=200, tol=1e-10)
+    print(metric, "f0", f0, "f", f, "f_true", f_true, "iterations", it, "theta", th)
+    assert f <= f_true * (1 + 1e-6)
+    assert f < 0.01 * f0
+    rel = np.abs(th - theta_true) / np.abs(theta_true)
+    assert rel[0] < 0.01 and rel[1] < 0.02 and rel[2] < 0.01, rel
+    assert np.all(th >= lo) and np.all(th <= hi)
+    # the result is a stationary point of the free parameters
+    _, grad = plan.bwd(th)
+    assert np.all(np.abs(grad * (hi - lo) * np.array(free)) <= 1e-4 * abs(f) + 1e-12)
+    plan.release()

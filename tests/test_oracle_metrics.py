@@ -1,0 +1,47 @@
+"""Pins of the oracle's restatement of the reference's distance functors
+(include/analyzer/distance_functions.hpp): closed forms and invariances that a dropped square root,
+a wrong normalisation or a mask applied once instead of twice would break."""
+import numpy as np
+import pytest
+
+from oracle import metrics as M
+
+
+@pytest.fixture
+def images():
+    rng = np.random.default_rng(11)
+    I = rng.gamma(2.0, 3.0, 500)
+    O = rng.poisson(I).astype(float)
+    m = (rng.uniform(size=500) > 0.1).astype(float)
+    return O, I, m
+
+
+def test_scaled_data_are_at_distance_zero(images):
+    """sqrt_unit_l2 and sqrt_c_l2 are invariant to the scale of either image: O = k I gives 0."""
+    _, I, m = images
+    for k in (1.0, 3.7, 1e-4):
+        assert M.sqrt_unit_l2(k * I, I, m) == pytest.approx(0.0, abs=1e-28)
+        assert M.sqrt_c_l2(k * I, I, m) == pytest.approx(0.0, abs=1e-20 * k)
+        assert M.sqrt_c_l2(I, k * I, m) == pytest.approx(0.0, abs=1e-20)
+
+
+def test_closed_forms(images):
+    O, I, m = images
+    assert M.l2(O, I, m) == pytest.approx(float(np.sum(m * (O - I) ** 2)), rel=1e-14)
+    sO, sI = np.sqrt(O), np.sqrt(I)
+    # sqrt_c_l2: c is the masked least-squares scale of sqrt I onto sqrt O, so D is the residual of
+    # that projection: sum m (sqrt O)^2 - <sqrt I, sqrt O>^2 / ||sqrt I||^2 for a 0/1 mask
+    proj = np.sum(m * sO * sO) - np.sum(m * sO * sI) ** 2 / np.sum(m * sI * sI)
+    assert M.sqrt_c_l2(O, I, m) == pytest.approx(proj, rel=1e-12)
+    # sqrt_unit_l2 of unit vectors: 2 - 2 cos(angle) for a 0/1 mask
+    cos = np.sum(m * sO * sI) / np.sqrt(np.sum(m * sO * sO) * np.sum(m * sI * sI))
+    assert M.sqrt_unit_l2(O, I, m) == pytest.approx(2 - 2 * cos, rel=1e-12)
+
+
+def test_mask_weights_twice_in_the_distance():
+    """A mask value w weights the norms once and the squared difference twice (w^2), as the reference
+    multiplies the absolute difference by the mask before squaring."""
+    O, I = np.array([4.0, 9.0]), np.array([1.0, 1.0])
+    w = np.array([1.0, 0.5])
+    assert M.l2(O, I, w) == pytest.approx(9.0 + 0.25 * 64.0)
+    assert M.l2(O, I, np.ones(2)) == pytest.approx(9.0 + 64.0)
