@@ -295,9 +295,28 @@ def bench_fit(evals=20):
     for _ in range(evals):
         chi2, grad = plan.bwd(theta)
     t_bwd = (time.perf_counter() - t0) / evals
+    # a particle-swarm generation (64 parameter vectors) through the population entry point
+    pop = theta_true * (1 + 0.02 * rng.standard_normal((64, 6)))
+    pop[:, 3] = 0.01 * rng.standard_normal(64)
+    plan.fwd_many(pop)
+    t0 = time.perf_counter()
+    plan.fwd_many(pop)
+    t_pop = time.perf_counter() - t0
+    # bounded L-BFGS from a start inside the true parameters' basin, 100 iterations
+    lo = np.array([2.0, 6.0, 10.0, -0.1, 0.0, 0.1])
+    hi = np.array([6.0, 20.0, 20.0, 0.1, 2.0, 10.0])
+    start = theta_true * np.array([1.01, 0.99, 1.003, 1.0, 1.05, 0.97])
+    start[3] = 0.005
+    f_true = plan.fwd(theta_true)
+    t0 = time.perf_counter()
+    th, f_fit, its = plan.lbfgs(start, lo=lo, hi=hi, iterations=100, tol=1e-10)
+    t_fit = time.perf_counter() - t0
     ctx.close()
     return {"chi2_grad_evals_per_s": 1.0 / t_bwd, "ms_fwd": 1e3 * t_fwd, "ms_grad": 1e3 * t_bwd,
             "grad_over_fwd": t_bwd / t_fwd, "fd_equivalent_fwds": 13, "chi2": chi2,
+            "population_64_ms": 1e3 * t_pop, "population_evals_per_s": 64 / t_pop,
+            "lbfgs": {"iterations": its, "s": t_fit, "objective": f_fit, "objective_at_truth": f_true,
+                      "theta": [float(x) for x in th]},
             "workload": "cylinders R4 H12, hcp a=15 20x20x1, 512^2 DWBA, 6 parameters, FP64 adjoint"}
 
 

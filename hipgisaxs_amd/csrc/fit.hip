@@ -430,32 +430,45 @@ static int run_fit_many(Context* c, int plan, int n, const double* thetas, doubl
   return HGX_OK;
 }
 
-// Projected L-BFGS (memory 6) over the free parameters within [lo, hi]: two-loop recursion on the
-// projected gradient, backtracking (Armijo, c1 = 1e-4) along the projected path, each trial point
-// one value-and-gradient pass. Stops when the projected gradient's largest component times the
-// parameter range falls below tol * |f|, or when a line search cannot decrease f.
+// Projected L-BFGS (memory 6) over the free parameters within [lo, hi], in coordinates scaled by
+// each parameter's range (so a tilt in radians and a lattice constant in nm are comparable): two-loop
+// recursion on the projected gradient, backtracking (Armijo, c1 = 1e-4) along the projected path,
+// each trial point one value-and-gradient pass. Stops when the projected gradient's largest scaled
+// component falls below tol * |f|, or when a line search cannot decrease f.
 static int run_lbfgs(Context* c, int plan, double theta[6], const int freep[6], const double lo[6],
                      const double hi[6], int iters, double tol, double* f_out, int* done) {
   const int M = 6;
-  double x[6], g[6], f;
-  for (int j = 0; j < 6; ++j) x[j] = std::min(std::max(theta[j], lo[j]), hi[j]);
+  double sc[6], x[6], g[6], f;
+  for (int j = 0; j < 6; ++j) {
+    x[j] = std::min(std::max(theta[j], lo[j]), hi[j]);
+    const double r = hi[j] - lo[j];
+    sc[j] = (std::isfinite(r) && r > 0) ? r : std::max(std::fabs(x[j]), 1.0);
+  }
   if (int rc = run_fit(c, plan, x, HGX_PREC_FP64, true, &f, g, nullptr)) return rc;
-  std::vector<std::array<double, 6>> S, Y;
+  std::vector<std::array<double, 6>> S, Y;  // scaled coordinates
   std::vector<double> rho;
+  // projected gradient in scaled coordinates: d f / d(x_j / sc_j)
   auto proj_grad = [&](const double* xx, const double* gg, double* pg) {
     for (int j = 0; j < 6; ++j) {
-      pg[j] = freep[j] ? gg[j] : 0.0;
+      pg[j] = freep[j] ? gg[j] * sc[j] : 0.0;
       if ((xx[j] <= lo[j] && pg[j] > 0) || (xx[j] >= hi[j] && pg[j] < 0)) pg[j] = 0.0;
     }
+  };
+  auto clip = [&](double* d, const double* pg, double frac) {  // largest free move = frac (scaled)
+    double dm = 0;
+    for (int j = 0; j < 6; ++j)
+      if (freep[j]) dm = std::max(dm, std::fabs(d[j]) / frac);
+    if (dm > 1)
+      for (int j = 0; j < 6; ++j) d[j] /= dm;
   };
   int it = 0;
   for (; it < iters; ++it) {
     double pg[6];
     proj_grad(x, g, pg);
     double gmax = 0.0;
-    for (int j = 0; j < 6; ++j) gmax = std::max(gmax, std::fabs(pg[j]) * std::max(hi[j] - lo[j], 1e-300));
+    for (int j = 0; j < 6; ++j) gmax = std::max(gmax, std::fabs(pg[j]));
     if (gmax <= tol * std::max(std::fabs(f), 1e-300)) break;
-    // two-loop recursion
+    // two-loop recursion (scaled coordinates)
     double d[6], al[M];
     for (int j = 0; j < 6; ++j) d[j] = -pg[j];
     for (int k = (int)S.size() - 1; k >= 0; --k) {
@@ -474,12 +487,10 @@ static int run_lbfgs(Context* c, int plan, double theta[6], const int freep[6], 
       }
       for (int j = 0; j < 6; ++j) d[j] *= sy / yy;
     } else {
-      // first step: at most 1 % of each free parameter's range
-      double dm = 0;
-      for (int j = 0; j < 6; ++j)
-        if (freep[j]) dm = std::max(dm, std::fabs(d[j]) / (0.01 * (hi[j] - lo[j])));
-      if (dm > 0)
-        for (int j = 0; j < 6; ++j) d[j] /= dm;
+      double gm = 0;
+      for (int j = 0; j < 6; ++j) gm = std::max(gm, std::fabs(d[j]));
+      if (gm > 0)
+        for (int j = 0; j < 6; ++j) d[j] *= 0.01 / gm;  // first step: 1 % of the range
     }
     for (size_t k = 0; k < S.size(); ++k) {
       double yd = 0;
@@ -495,41 +506,36 @@ static int run_lbfgs(Context* c, int plan, double theta[6], const int freep[6], 
       S.clear();
       Y.clear();
       rho.clear();
-      double dm = 0;
+      double gm = 0;
       for (int j = 0; j < 6; ++j) {
         d[j] = -pg[j];
-        if (freep[j]) dm = std::max(dm, std::fabs(d[j]) / (0.01 * (hi[j] - lo[j])));
+        gm = std::max(gm, std::fabs(d[j]));
       }
-      if (dm > 0)
-        for (int j = 0; j < 6; ++j) d[j] /= dm;
-      gd = 0;
-      for (int j = 0; j < 6; ++j) gd += pg[j] * d[j];
+      for (int j = 0; j < 6; ++j) d[j] *= gm > 0 ? 0.01 / gm : 0.0;
     }
+    clip(d, pg, 0.25);  // no trial moves a parameter by more than a quarter of its range
     double step = 1.0, xn[6], gn[6], fn = f;
     bool ok = false;
     for (int ls = 0; ls < 30; ++ls) {
       double dec = 0;
       for (int j = 0; j < 6; ++j) {
-        xn[j] = std::min(std::max(x[j] + step * d[j], lo[j]), hi[j]);
-        dec += pg[j] * (xn[j] - x[j]);
+        xn[j] = std::min(std::max(x[j] + step * d[j] * sc[j], lo[j]), hi[j]);
+        dec += pg[j] * (xn[j] - x[j]) / sc[j];
       }
-      if (int rc = run_fit(c, plan, xn, HGX_PREC_FP64, true, &fn, gn, nullptr)) {
-        if (rc != HGX_ERR_INVALID_ARG) return rc;  // a trial outside the model's domain: shorten the step
-        step *= 0.5;
-        continue;
-      }
-      if (std::isfinite(fn) && fn <= f + 1e-4 * dec) {
+      int rc = run_fit(c, plan, xn, HGX_PREC_FP64, true, &fn, gn, nullptr);
+      if (rc == HGX_OK && std::isfinite(fn) && fn <= f + 1e-4 * dec) {
         ok = true;
         break;
       }
+      if (rc != HGX_OK && rc != HGX_ERR_INVALID_ARG) return rc;  // INVALID_ARG: outside the model's domain
       step *= 0.5;
     }
     if (!ok) break;
     std::array<double, 6> s, y;
     double sy = 0;
     for (int j = 0; j < 6; ++j) {
-      s[j] = xn[j] - x[j];
-      y[j] = freep[j] ? gn[j] - g[j] : 0.0;
+      s[j] = (xn[j] - x[j]) / sc[j];
+      y[j] = freep[j] ? (gn[j] - g[j]) * sc[j] : 0.0;
       sy += s[j] * y[j];
     }
     if (sy > 1e-300) {

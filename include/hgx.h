@@ -313,8 +313,9 @@ int hgx_chi2_fwd_many(hgx_ctx* ctx, int plan, int n, const double* thetas, int p
 /* Bounded L-BFGS (memory 6) on the plan's objective with the adjoint gradient, in place of the
  * reference's TAO LMVM with central finite differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224):
  * theta (in: start, out: result) moves only in the components with free_params[j] != 0, kept within
- * [lo, hi]; stops after `iterations` or when max_j |projected dD/dtheta_j| (hi_j - lo_j) <= tol |D|.
- * objective and iterations_done may be NULL. Host loop, one value-and-gradient pass per trial point. */
+ * [lo, hi] (steps measured in units of hi_j - lo_j, or of max(|theta_j|, 1) for an unbounded one); stops
+ * after `iterations` or when max_j |projected dD/dtheta_j| (hi_j - lo_j) <= tol |D|. objective and
+ * iterations_done may be NULL. Host loop, one value-and-gradient pass per trial point. */
 int hgx_fit_lbfgs(hgx_ctx* ctx, int plan, double theta[6], const int free_params[6], const double lo[6],
                   const double hi[6], int iterations, double tol, double* objective, int* iterations_done);
 
