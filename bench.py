@@ -434,6 +434,23 @@ def bench_hig_examples(reps=3, cpu=True):
             res[prec] = {"ms_per_image": 1e3 * wall, "images_per_s": 1.0 / wall,
                          "fused_grains": c["fused_grains"], "per_grain_calls": c["grains"]}
             sim.close()
+            # kernel times of one more run with HIP events around every launch
+            sim = H.HigSimulation(text=text, precision=prec, flags=H.PROFILE)
+            sim.run(0)
+            sim.run(0)
+            res[prec]["kernels"] = sim.kernel_ms()
+            sim.close()
+        ng_ = 625 if name != "01-pyramid" else 1
+        if name.startswith(("02", "03")):
+            # fused lattice kernel, VALU roofline with the lattice leg's flop model: per (q, node, grain)
+            # the particle form factor (sphere about 55 flop; cylinder 60), per (q, grain) the lattice
+            # factor (155) and the DWBA term (15 per branch); transcendentals not counted
+            ffl = 55 if name.startswith("02") else 60
+            flop = 4 * img.size * ng_ * (ffl + 155 + 15)
+            for prec, peak in (("fp64", PEAK_FP64), ("fp32c", PEAK_FP32)):
+                a_ = flop / (res[prec]["kernels"]["main_ms"] * 1e-3)
+                res[prec]["roofline"] = {"bound": "valu", "achieved": a_ / 1e12, "peak": peak / 1e12,
+                                         "unit": "TFLOP/s", "frac": a_ / peak}
         ng = 625 if name != "01-pyramid" else 1
         res["grains"] = ng
         res["pixels"] = int(img.size)
@@ -477,6 +494,8 @@ def main():
     ap.add_argument("--kahan", action="store_true", help="per-pair Kahan accumulation in FP32C")
     ap.add_argument("--no-mfma", action="store_true",
                     help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
+    ap.add_argument("--hig-only", action="store_true",
+                    help="only the .hig driver leg (the reference's example models), as one JSON object")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -487,6 +506,9 @@ def main():
     import torch
     from hipgisaxs_amd.api import Batch, Context
 
+    if args.hig_only:
+        print(json.dumps({"hig_examples": bench_hig_examples(cpu=not args.no_cpu)}))
+        return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)

@@ -66,6 +66,35 @@ class LatticeModel(C.Structure):
     ]
 
 
+class EnsembleElement(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int),
+        ("params", ShapeParams),
+        ("rot", C.c_double * 9),
+        ("nloc", C.c_int),
+        ("loc", _dp),
+        ("dn2_re", C.c_double),
+        ("dn2_im", C.c_double),
+    ]
+
+
+class EnsembleModel(C.Structure):
+    _fields_ = [
+        ("nelem", C.c_int),
+        ("elem", C.POINTER(EnsembleElement)),
+        ("la", C.c_double * 3),
+        ("lb", C.c_double * 3),
+        ("lc", C.c_double * 3),
+        ("repet", C.c_double * 3),
+        ("dw_sigma", C.c_double),
+        ("sf_flags", C.c_int),
+        ("ngrains", C.c_int),
+        ("rot", _dp),
+        ("center", _dp),
+        ("weight", _dp),
+    ]
+
+
 class FitModel(C.Structure):
     _fields_ = [
         ("shape", C.c_int),
@@ -139,6 +168,7 @@ _SIGS = {
     "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),
     "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "hgx_lattice_intensity": (C.c_int, [_vp, C.POINTER(LatticeModel), C.c_int, _vp, C.c_int, _vp]),
+    "hgx_ensemble_intensity": (C.c_int, [_vp, C.POINTER(EnsembleModel), _vp, C.c_int, _vp]),
     "hgx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(BatchItem), C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -17,7 +17,7 @@ from .api import HgxError
 
 LIB_PATH = Path(__file__).resolve().parent / "libhgx_hig.so"
 
-NO_FUSE, SF_COMPAT_LB2, SMEAR_COMPAT, NO_SMEAR, MESH_NO_TRANSLATION = 1, 2, 4, 8, 16
+NO_FUSE, SF_COMPAT_LB2, SMEAR_COMPAT, NO_SMEAR, MESH_NO_TRANSLATION, PROFILE = 1, 2, 4, 8, 16, 32
 
 _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 _SIGS = {
@@ -33,6 +33,7 @@ _SIGS = {
     "hgx_hig_compute": (C.c_int, [_vp, _vp]),
     "hgx_hig_grains": (C.c_int, [_vp, C.c_char_p, C.c_int, _vp, _vp, _vp, _ip]),
     "hgx_hig_last_counts": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp]),
+    "hgx_hig_kernel_ms": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)]),
     "hgx_hig_save": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
     "hgx_hig_read_mesh": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_int64), C.c_int64, _vp]),
     "hgx_hig_last_error": (C.c_char_p, [_vp]),
@@ -158,6 +159,13 @@ class HigSimulation:
         self._check(self.lib.hgx_hig_last_counts(self.h, c, C.byref(ms)), "last_counts")
         return {"fused_grains": c[0], "grains": c[1], "form_factors": c[2], "structure_factors": c[3],
                 "ms": ms.value}
+
+    def kernel_ms(self) -> dict:
+        """Kernel times of the last run (needs flags with PROFILE)."""
+        m, a = C.c_double(), C.c_double()
+        nm, na = C.c_int64(), C.c_int64()
+        self._check(self.lib.hgx_hig_kernel_ms(self.h, C.byref(m), C.byref(nm), C.byref(a), C.byref(na)), "kernel_ms")
+        return {"main_ms": m.value, "main_launches": nm.value, "aux_ms": a.value, "aux_launches": na.value}
 
     @staticmethod
     def save(image: np.ndarray, path: str) -> None:

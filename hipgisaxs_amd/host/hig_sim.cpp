@@ -557,6 +557,58 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     return hgx(hgx_synchronize(gpu_->get()), "hgx_synchronize");
   }
 
+  // any analytic cell: the fused ensemble kernel (FP64) over all grains, elements and locations
+  bool analytic = !(flags_ & HGX_HIG_NO_FUSE);
+  for (const auto& e : uc.elements) analytic = analytic && in_.shapes.at(e.first).shape >= 0;
+  if (analytic) {
+    const int ne = (int)uc.elements.size();
+    std::vector<hgx_ensemble_element> el(ne);
+    std::vector<std::vector<double>> stores(8 * (size_t)ne), locs(ne);
+    int e = 0;
+    for (const auto& it : uc.elements) {
+      const HigShape& sh = in_.shapes.at(it.first);
+      hgx_ensemble_element& E = el[e];
+      std::memset(&E, 0, sizeof(E));
+      if (!shape_params(sh, E.params, &stores[8 * (size_t)e])) return false;
+      E.shape = sh.shape;
+      RotMatrix_t re = RotMatrix_t(0, sh.xrot) * RotMatrix_t(1, sh.yrot) * RotMatrix_t(2, sh.zrot);
+      std::memcpy(E.rot, re.data(), sizeof(E.rot));
+      for (const auto& l : it.second) locs[e].insert(locs[e].end(), l.begin(), l.end());
+      E.nloc = (int)it.second.size();
+      E.loc = locs[e].data();
+      E.dn2_re = dnl[0] - 2 * sh.delta;
+      E.dn2_im = dnl[1] - 2 * sh.beta;
+      counts_[2] += (int64_t)ng * E.nloc;
+      ++e;
+    }
+    hgx_ensemble_model m;
+    std::memset(&m, 0, sizeof(m));
+    m.nelem = ne;
+    m.elem = el.data();
+    for (int i = 0; i < 3; ++i) {
+      m.la[i] = la[i];
+      m.lb[i] = lb[i];
+      m.lc[i] = lc[i];
+      m.repet[i] = repet[i];
+    }
+    m.dw_sigma = 0.0;
+    m.sf_flags = sf_flags;
+    m.ngrains = ng;
+    std::vector<double> rots(9 * (size_t)ng), cen(3 * (size_t)ng), wts(ng);
+    for (int g = 0; g < ng; ++g) {
+      std::memcpy(&rots[9 * (size_t)g], grains[g].rot.data(), 9 * sizeof(double));
+      std::memcpy(&cen[3 * (size_t)g], grains[g].center, 3 * sizeof(double));
+      wts[g] = grains[g].weight;
+    }
+    m.rot = rots.data();
+    m.center = cen.data();
+    m.weight = wts.data();
+    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, d_fc, 0, d_I), "hgx_ensemble_intensity")) return false;
+    counts_[0] += ng;
+    counts_[3] += ng;
+    return true;
+  }
+
   // per grain: form factor (reused while the grain rotation repeats), structure factor, DWBA sum
   double* d_ff = buffer(3, (size_t)nq * 16);
   double* d_eff = buffer(4, (size_t)nq * 16);
@@ -628,6 +680,8 @@ bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, 
   double* d_total = buffer(0, (size_t)npix_ * 8);
   double* d_Is = buffer(1, (size_t)npix_ * 8);
   if (!d_total || !d_Is) return fail("device allocation failed");
+  const bool prof = (flags_ & HGX_HIG_PROFILE) != 0;
+  if (prof && !hgx(hgx_profile_begin(gpu_->get()), "hgx_profile_begin")) return false;
   std::vector<HigGrain> grains;
   size_t si = 0;
   for (const auto& s : in_.structures) {
@@ -643,6 +697,8 @@ bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, 
     if (!hgx(hgx_gaussian_smear(gpu_->get(), (int)ncol_, (int)nrow_, sigma, sf, d_total), "hgx_gaussian_smear"))
       return false;
   }
+  if (prof && !hgx(hgx_profile_end(gpu_->get(), &kms_[0], &kn_[0], &kms_[1], &kn_[1]), "hgx_profile_end"))
+    return false;
   if (!hgx(hgx_memcpy_d2h(gpu_->get(), img.data(), d_total, (size_t)npix_ * 8), "hgx_memcpy_d2h")) return false;
   last_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return true;
@@ -785,6 +841,18 @@ int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms) {
   if (!h) return HGX_ERR_BAD_HANDLE;
   if (counts) h->sim.last_counts(counts);
   if (ms) *ms = h->sim.last_run_ms();
+  return HGX_OK;
+}
+
+int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches) {
+  if (!h) return HGX_ERR_BAD_HANDLE;
+  double m, a;
+  int64_t nm, na;
+  h->sim.last_kernel_times(&m, &nm, &a, &na);
+  if (main_ms) *main_ms = m;
+  if (main_launches) *main_launches = nm;
+  if (aux_ms) *aux_ms = a;
+  if (aux_launches) *aux_launches = na;
   return HGX_OK;
 }
 

@@ -77,6 +77,13 @@ class HipGISAXS {
   // calls, form-factor and structure-factor evaluations
   void last_counts(int64_t counts[4]) const;
   double last_run_ms() const { return last_ms_; }
+  // kernel times of the last run (HGX_HIG_PROFILE): main ms, main launches, aux ms, aux launches
+  void last_kernel_times(double* main_ms, int64_t* main_n, double* aux_ms, int64_t* aux_n) const {
+    *main_ms = kms_[0];
+    *aux_ms = kms_[1];
+    *main_n = kn_[0];
+    *aux_n = kn_[1];
+  }
 
  private:
   bool fail(const std::string& msg);
@@ -110,6 +117,8 @@ class HipGISAXS {
   std::map<std::string, int> meshes_;
   int64_t counts_[4] = {0, 0, 0, 0};
   double last_ms_ = 0.0;
+  double kms_[2] = {0.0, 0.0};
+  int64_t kn_[2] = {0, 0};
   std::string err_;
 };
 

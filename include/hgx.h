@@ -265,6 +265,41 @@ typedef struct hgx_lattice_model {
 int hgx_lattice_intensity(hgx_ctx* ctx, const hgx_lattice_model* model, int precision, const double* fc,
                           int accumulate, double* I);
 
+/* One analytic element of a unit cell: a shape with its distribution nodes, its rotation inside the
+ * grain (the shape's xrot/yrot/zrot product, src/sim/hipgisaxs_main.cpp:986), its locations and its
+ * contrast (1-n^2)_layer - (1-n^2)_shape. */
+typedef struct hgx_ensemble_element {
+  int shape;               /* HGX_SHAPE_* */
+  hgx_shape_params params; /* host node arrays */
+  double rot[9];           /* R_e: the element is evaluated at (R_g R_e) q */
+  int nloc;                /* >= 1 */
+  const double* loc;       /* host [nloc][3] locations t_l */
+  double dn2_re, dn2_im;
+} hgx_ensemble_element;
+
+/* A lattice of unit cells of analytic elements seen by several grains, each with its own rotation,
+ * centre and weight. */
+typedef struct hgx_ensemble_model {
+  int nelem;                       /* 1..64 */
+  const hgx_ensemble_element* elem;
+  double la[3], lb[3], lc[3];      /* lattice vectors already multiplied by the grain scaling */
+  double repet[3];                 /* repetitions (values < 1 become 1) */
+  double dw_sigma;                 /* Debye-Waller sigma (0 = off) */
+  int sf_flags;                    /* HGX_SF_* */
+  int ngrains;                     /* >= 1 */
+  const double* rot;               /* host [ngrains][9]: R_g, for the structure factor */
+  const double* center;            /* host [ngrains][3] grain centres, or NULL for the origin */
+  const double* weight;            /* host [ngrains] weights, or NULL for 1 */
+} hgx_ensemble_model;
+
+/* I[p] (+)= sum_g |w_g sum_b fc_b S_g(R_g q_b) sum_e dn2_e F_e(R_g R_e q_b) sum_l exp(i (R_g R_e q_b).t_l)|^2:
+ * the grain, element and location loops, structure factor and DWBA sum of run_gisaxs
+ * (src/sim/hipgisaxs_main.cpp:925-1100) for any analytic unit cell in one pass per pixel, FP64. fc
+ * (device [2*nq]) may be NULL (Born). Equals hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice +
+ * hgx_dwba_intensity per grain to round-off. Synchronises the stream. */
+int hgx_ensemble_intensity(hgx_ctx* ctx, const hgx_ensemble_model* model, const double* fc, int accumulate,
+                           double* I);
+
 /* ---- chi^2 and its gradient (forward / backward pair) ------------------------------------- */
 
 /* One-grain model fitted by hgx_chi2_*: I(p) = s |sum_b fc_b dn2 F(m_b) S(m_b)|^2 with

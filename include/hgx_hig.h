@@ -24,6 +24,7 @@ typedef struct hgx_hig hgx_hig;
 #define HGX_HIG_NO_SMEAR 8          /* skip the smearing whatever the input's sigma */
 #define HGX_HIG_MESH_NO_TRANSLATION 16 /* ignore unit-cell locations of mesh shapes, as the reference
                                           does (src/ff/ff.cpp:44) */
+#define HGX_HIG_PROFILE 32          /* time every kernel of a run with HIP events (hgx_hig_kernel_ms) */
 
 /* A driver on HIP device `device`. Reading inputs, hgx_hig_info, hgx_hig_config and hgx_hig_grains
  * run on the host; the device context (one hgx context) is created by the first call that needs it. */
@@ -57,6 +58,9 @@ int hgx_hig_grains(hgx_hig* h, const char* key, int cap, double* rot, double* ce
 /* Work of the last run: counts[0] grains through the fused lattice kernel, [1] grains through
  * per-grain calls, [2] form-factor evaluations, [3] structure-factor evaluations; *ms its wall time. */
 int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms);
+/* With HGX_HIG_PROFILE: summed durations (ms) and launch counts of the last run's kernels, "main" the
+ * form-factor / structure-factor / lattice kernels, "aux" the rest (hgx_profile_end of include/hgx.h). */
+int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches);
 /* HipGISAXS::save_gisaxs (src/sim/hipgisaxs_main.cpp:2250-2260): nrow lines of ncol tab-terminated
  * values. */
 int hgx_hig_save(const double* image, int ncol, int nrow, const char* path);

@@ -48,7 +48,7 @@ def test_fixture_image_matches_oracle(name, flags):
     r = _rel(img, ref)
     _say(name, "relL2", r, counts)
     assert r <= TOL64
-    fusable = name in ("spheres_bcc", "cylinders_hcp")
+    fusable = name != "mesh_tetra"  # analytic cells run fused (lattice or ensemble kernel)
     assert (counts["fused_grains"] > 0) == (fusable and flags == 0)
 
 
@@ -152,6 +152,27 @@ def test_full_spheres_example_paths_agree():
     r32 = _rel(c, a)
     _say("fp32c vs fp64 relL2", r32)
     assert r32 <= TOL32
+
+
+def test_ensemble_kernel_matches_per_grain_calls_at_full_size():
+    """All 625 grains of the reference's gratings example (boxes, 600 x 300) and the two-structure scan
+    with rotated multi-element cells: the fused ensemble kernel against the per-grain calls."""
+    box = (GOLD / "box_grating.hig").read_text().replace("resolution = [50 30]", "resolution = [600 300]").replace(
+        "maxgrains = [3 3 1]", "maxgrains = [25 25 1]")
+    a = H.HigSimulation(text=box)
+    ia = a.compute()
+    b = H.HigSimulation(text=box, flags=H.NO_FUSE)
+    ib = b.compute()
+    _say("gratings 625 grains: ensemble", a.last_counts(), "per grain", b.last_counts())
+    assert a.last_counts()["fused_grains"] == 625
+    r = _rel(ia, ib)
+    _say("ensemble vs per-grain relL2", r)
+    assert r <= 1e-12
+    text = _text("two_structures")
+    fa = H.HigSimulation(text=text).run_all()
+    fb = H.HigSimulation(text=text, flags=H.NO_FUSE).run_all()
+    for x, y in zip(fa, fb):
+        assert _rel(x, y) <= 1e-12
 
 
 def test_errors_are_reported_not_fatal():
