@@ -244,16 +244,22 @@ __device__ inline cd cdiv(cd a, cd b) {
 }
 __device__ inline cd mul_i(cd a) { return cd{-a.i, a.r}; }      // i * a
 __device__ inline cd mul_mi(cd a) { return cd{a.i, -a.r}; }     // -i * a
+// The complex functions below take a shortcut for an exactly real argument (a particle in a
+// non-absorbing layer makes every q real): exp(-0) = cosh(0) = 1 and sinh(+-0) = +-0, sin(+-0) = +-0
+// exactly, so the shortcut returns the same bits as the general formula while skipping its
+// exponentials.
 __device__ inline cd cexpi_d(cd t) {                             // exp(i t)
   double s, c;
   sincos(t.r, &s, &c);
+  if (t.i == 0.0) return cd{c, s};
   double m = exp(-t.i);
   return cd{m * c, m * s};
 }
 __device__ inline cd cexp_d(cd z) {
+  double m = exp(z.r);
+  if (z.i == 0.0) return cd{m, m * z.i};
   double s, c;
   sincos(z.i, &s, &c);
-  double m = exp(z.r);
   return cd{m * c, m * s};
 }
 __device__ inline cd csqrt_d(cd z) {
@@ -270,14 +276,16 @@ __device__ inline cd csqrt_d(cd z) {
 __device__ inline cd csin_d(cd z) {  // sin(a+ib) = sin a cosh b + i cos a sinh b
   double s, c;
   sincos(z.r, &s, &c);
+  if (z.i == 0.0) return cd{s, c * z.i};
   return cd{s * cosh(z.i), c * sinh(z.i)};
 }
 __device__ inline cd ccos_d(cd z) {  // cos(a+ib) = cos a cosh b - i sin a sinh b
   double s, c;
   sincos(z.r, &s, &c);
+  if (z.i == 0.0) return cd{c, -s * z.i};
   return cd{c * cosh(z.i), -s * sinh(z.i)};
 }
-__device__ inline double cabs_d(cd z) { return hypot(z.r, z.i); }
+__device__ inline double cabs_d(cd z) { return z.i == 0.0 ? fabs(z.r) : hypot(z.r, z.i); }
 
 // Rotation of a q-point: mq = R (qx, qy, qz) with complex qz (matrix.hpp:123-130).
 struct Rot {
