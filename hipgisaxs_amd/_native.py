@@ -168,7 +168,7 @@ _SIGS = {
     "hgx_fresnel": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp]),
     "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "hgx_lattice_intensity": (C.c_int, [_vp, C.POINTER(LatticeModel), C.c_int, _vp, C.c_int, _vp]),
-    "hgx_ensemble_intensity": (C.c_int, [_vp, C.POINTER(EnsembleModel), _vp, C.c_int, _vp]),
+    "hgx_ensemble_intensity": (C.c_int, [_vp, C.POINTER(EnsembleModel), C.c_int, _vp, C.c_int, _vp]),
     "hgx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(BatchItem), C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -9,6 +9,11 @@
 // the per-shape and structure-factor kernels (analytic_shapes.hpp, lattice_sf.hpp), so it matches
 // hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice + hgx_dwba_intensity to round-off.
 //
+// FP32C (real q; sphere, cylinder, box and cube cells): q rotated, lattice, location and sinc phases
+// formed and reduced to turns in FP64, the form-factor node sums and Dirichlet kernels in FP32 with
+// the turn-based hardware sin/cos (Taylor series at small arguments), amplitudes summed and squared
+// in FP64 -- the scheme of the FP32C lattice kernel (lattice.hip).
+//
 // Work split: blockIdx.x tiles the pixels (256 per workgroup), blockIdx.y a contiguous range of grains;
 // each (x, y) writes its partial grain sum, and a second kernel adds the y partials in a fixed order
 // (deterministic). The model (nodes, matrices, locations, grain data) sits in the context workspace.
@@ -17,6 +22,7 @@
 #include <vector>
 
 #include "analytic_shapes.hpp"
+#include "fp32_math.hpp"
 #include "lattice_sf.hpp"
 
 namespace hgx {
@@ -113,6 +119,162 @@ __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const doub
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
 }
 
+// ----- FP32C -------------------------------------------------------------------------------------
+
+// x (radians, FP64) reduced to turns in (-1/2, 1/2]
+__device__ inline float turns_of(double x) {
+  double u = x * (1.0 / (2.0 * kPiD));
+  return (float)(u - rint(u));
+}
+
+// sin(u)/u for u = 0.5 * m * L, the phase reduced in FP64
+__device__ inline float sinc32(double m, double L) {
+  double u = 0.5 * m * L;
+  float uf = (float)u;
+  if (fabsf(uf) < 0.1f) {
+    float u2 = uf * uf;
+    return 1.0f - u2 * (1.0f / 6.0f) * (1.0f - u2 * (1.0f / 20.0f));
+  }
+  return sin_turn(turns_of(u)) / uf;
+}
+
+template <int SHAPE>
+__device__ cf ff32(const double m[3], const Nodes& P);
+
+// sphere (src/ff/ff_ana_sphere.cpp:33-45): sum_k w_k 4 pi R^3 (sin x - x cos x)/x^3 e^{i m_z R}
+template <>
+__device__ cf ff32<HGX_SHAPE_SPHERE>(const double m[3], const Nodes& P) {
+  const double q64 = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  if (q64 == 0.0) return cf{0.0f, 0.0f};
+  const float qv = (float)q64;
+  cf acc = cf{0.0f, 0.0f};
+  for (int i = 0; i < P.n[0]; ++i) {
+    const double R = P.val[0][i];
+    const float x = qv * (float)R;
+    float t;
+    if (x < 0.6f) {
+      float x2 = x * x;
+      t = x2 * x * ((1.0f / 3.0f) - x2 * ((1.0f / 30.0f) - x2 * ((1.0f / 840.0f) - x2 * (1.0f / 45360.0f))));
+    } else {
+      const float tr = turns_of(q64 * R);
+      t = sin_turn(tr) - x * cos_turn(tr);
+    }
+    acc = acc + ((float)(P.wt[0][i] * 4.0 * kPiD) * t) * expi_turn(turns_of(m[2] * R));
+  }
+  return (1.0f / (qv * qv * qv)) * acc;
+}
+
+// cylinder (src/ff/ff_ana_cylinder.cpp:34-42), node double sum factorised
+template <>
+__device__ cf ff32<HGX_SHAPE_CYLINDER>(const double m[3], const Nodes& P) {
+  const float qp = (float)sqrt(m[0] * m[0] + m[1] * m[1]);
+  float br = 0.0f;
+  for (int i = 0; i < P.n[0]; ++i) {
+    const float R = (float)P.val[0][i];
+    if (qp * R * qp * R > 1.0e-13f) br += (float)(P.wt[0][i] * 2.0 * kPiD) * R * R * j1_over_x(qp * R);
+  }
+  cf hz = cf{0.0f, 0.0f};
+  for (int k = 0; k < P.n[1]; ++k) {
+    const double H = P.val[1][k];
+    hz = hz + ((float)(P.wt[1][k] * H) * sinc32(m[2], H)) * expi_turn(turns_of(0.5 * H * m[2]));
+  }
+  return br * hz;
+}
+
+// box (src/ff/ff_ana_box.cpp:35-41, :111-121), node triple sum factorised per axis
+template <>
+__device__ cf ff32<HGX_SHAPE_BOX>(const double m[3], const Nodes& P) {
+  float sx = 0.0f, sy = 0.0f;
+  for (int i = 0; i < P.n[0]; ++i) sx += (float)(P.wt[0][i] * P.val[0][i]) * sinc32(m[0], P.val[0][i]);
+  for (int i = 0; i < P.n[1]; ++i) sy += (float)(P.wt[1][i] * P.val[1][i]) * sinc32(m[1], P.val[1][i]);
+  cf hz = cf{0.0f, 0.0f};
+  for (int k = 0; k < P.n[2]; ++k) {
+    const double H = P.val[2][k];
+    hz = hz + ((float)(P.wt[2][k] * H) * sinc32(m[2], H)) * expi_turn(turns_of(0.5 * H * m[2]));
+  }
+  return (sx * sy) * hz;
+}
+
+// cube (src/ff/ff_ana_cube.cpp:35-42, :109-113)
+template <>
+__device__ cf ff32<HGX_SHAPE_CUBE>(const double m[3], const Nodes& P) {
+  cf acc = cf{0.0f, 0.0f};
+  for (int i = 0; i < P.n[0]; ++i) {
+    const double L = P.val[0][i];
+    const float s = sinc32(m[0], L) * sinc32(m[1], L) * sinc32(m[2], L);
+    acc = acc + ((float)(P.wt[0][i] * L * L * L) * s) * expi_turn(turns_of(0.5 * L * m[2]));
+  }
+  return acc;
+}
+
+// S(m) in FP32 from FP64 lattice phases (the FP32C lattice kernel's form)
+__device__ inline cf sf32e(const double m[3], const SFArgs& s) {
+  const double xa = m[0] * s.la[0] + m[1] * s.la[1] + m[2] * s.la[2];
+  const double xb = m[0] * s.lb[0] + m[1] * s.lb[1] + (s.compat_lb2 ? s.lb[2] * s.lb[2] : m[2] * s.lb[2]);
+  const double xc = m[0] * s.lc[0] + m[1] * s.lc[1] + m[2] * s.lc[2];
+  float d = dirichlet_turn(turns_of(xa), (float)s.N[0]) * dirichlet_turn(turns_of(xb), (float)s.N[1]) *
+            dirichlet_turn(turns_of(xc), (float)s.N[2]);
+  const cf ph = expi_turn(turns_of(m[0] * s.center[0] + m[1] * s.center[1] + m[2] * s.center[2]));
+  if (s.dw != 0.0) d *= __expf((float)(-s.dw * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2])));
+  return d * ph;
+}
+
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const double* __restrict__ fc,
+                                                    const EnsElem* __restrict__ E, int nelem,
+                                                    const double* __restrict__ loc, const double* __restrict__ M,
+                                                    const EnsGrain* __restrict__ G, int ngrains, int per_y,
+                                                    SFArgs sfb, double* __restrict__ part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
+  const double qx = g.qx[p], qy = g.qy[p];
+  double qzb[4];
+  cd fcb[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= nb) break;
+    const int64_t q = p + b * g.npix;
+    qzb[b] = g.qz[2 * q];
+    fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+  }
+  double acc = 0.0;
+  for (int o = g0; o < g1; ++o) {
+    SFArgs sf = sfb;
+    sf.center[0] = G[o].center[0];
+    sf.center[1] = G[o].center[1];
+    sf.center[2] = G[o].center[2];
+    cd amp = cd{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= nb) break;
+      double mg[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) mg[j] = G[o].rot[3 * j] * qx + G[o].rot[3 * j + 1] * qy + G[o].rot[3 * j + 2] * qzb[b];
+      const cf S = sf32e(mg, sf);
+      cd ff = cd{0.0, 0.0};
+      for (int e = 0; e < nelem; ++e) {
+        const double* Me = M + 9 * ((int64_t)o * nelem + e);
+        double mq[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) mq[j] = Me[3 * j] * qx + Me[3 * j + 1] * qy + Me[3 * j + 2] * qzb[b];
+        const cf F = ff32<SHAPE>(mq, E[e].P);
+        cf ph = cf{0.0f, 0.0f};
+        for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
+          const double* t = loc + 3 * l;
+          ph = ph + expi_turn(turns_of(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+        }
+        const cf Fp = F * ph;
+        ff = ff + cd{E[e].dn2r, E[e].dn2i} * cd{(double)Fp.r, (double)Fp.i};
+      }
+      amp = amp + fcb[b] * (cd{(double)S.r, (double)S.i} * ff);
+    }
+    amp = G[o].w * amp;
+    acc += amp.r * amp.r + amp.i * amp.i;
+  }
+  part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_ensemble_sum(int64_t npix, int ny, const double* __restrict__ part,
                                                       int accumulate, double* __restrict__ I) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -128,14 +290,16 @@ using namespace hgx;
 
 static const int kEnsDims[7] = {1, 2, 3, 1, 2, 2, 4};
 
-extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, const double* fc, int accumulate,
-                                      double* I) {
+extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, int precision, const double* fc,
+                                      int accumulate, double* I) {
   Context* c = reinterpret_cast<Context*>(h);
   if (!c) return HGX_ERR_BAD_HANDLE;
   if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ensemble_intensity: no q-grid");
   if (!m || !I || m->nelem < 1 || !m->elem || m->ngrains < 1 || !m->rot)
     return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: need >= 1 element, >= 1 grain and an output");
   if (m->nelem > 64) return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: at most 64 elements");
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
+    return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: bad precision");
   const int ne = m->nelem, ng = m->ngrains;
   std::vector<EnsElem> E(ne);
   std::vector<double> loc;
@@ -168,6 +332,16 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, c
     E[e].dn2r = el.dn2_re;
     E[e].dn2i = el.dn2_im;
     loc.insert(loc.end(), el.loc, el.loc + 3 * el.nloc);
+  }
+  int common = E[0].shape;
+  for (int e = 1; e < ne; ++e)
+    if (E[e].shape != common) common = -1;
+  if (precision == HGX_PREC_FP32C) {
+    if (!c->qz_real) return set_error(c, HGX_ERR_UNSUPPORTED, "ensemble_intensity: FP32C needs real q");
+    if (common != HGX_SHAPE_SPHERE && common != HGX_SHAPE_CYLINDER && common != HGX_SHAPE_BOX &&
+        common != HGX_SHAPE_CUBE)
+      return set_error(c, HGX_ERR_UNSUPPORTED,
+                       "ensemble_intensity: FP32C takes cells of one shape among sphere, cylinder, box, cube");
   }
   // M[g][e] = R_g R_e with the summation order of the reference's RotMatrix_t product
   std::vector<double> Mm(9 * (size_t)ng * ne);
@@ -220,10 +394,20 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, c
   HGX_CHECK_HIP(c, hipMemcpyAsync(dM, Mm.data(), Mm.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HGX_CHECK_HIP(c, hipMemcpyAsync(dG, G.data(), ng * sizeof(EnsGrain), hipMemcpyHostToDevice, c->stream));
   QViewE g{c->d_qx, c->d_qy, c->d_qz, c->npix};
-  int common = E[0].shape;
-  for (int e = 1; e < ne; ++e)
-    if (E[e].shape != common) common = -1;
-  {
+  if (precision == HGX_PREC_FP32C) {
+    LaunchTimer lt(c, true);
+    const dim3 grid(nbx, ny);
+#define HGX_ENS32_LAUNCH(S) \
+  k_ensemble32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
+    switch (common) {
+      case HGX_SHAPE_SPHERE: HGX_ENS32_LAUNCH(HGX_SHAPE_SPHERE); break;
+      case HGX_SHAPE_CYLINDER: HGX_ENS32_LAUNCH(HGX_SHAPE_CYLINDER); break;
+      case HGX_SHAPE_BOX: HGX_ENS32_LAUNCH(HGX_SHAPE_BOX); break;
+      default: HGX_ENS32_LAUNCH(HGX_SHAPE_CUBE); break;
+    }
+#undef HGX_ENS32_LAUNCH
+    HGX_CHECK_LAUNCH(c);
+  } else {
     LaunchTimer lt(c, true);
     const dim3 grid(nbx, ny);
 #define HGX_ENS_LAUNCH(S) \

@@ -603,7 +603,16 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     m.rot = rots.data();
     m.center = cen.data();
     m.weight = wts.data();
-    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, d_fc, 0, d_I), "hgx_ensemble_intensity")) return false;
+    // FP32C where the ensemble kernel has it (one shape among sphere, cylinder, box, cube), else FP64
+    int prec = HGX_PREC_FP64;
+    if (precision_ == HGX_PREC_FP32C) {
+      bool one = true;
+      for (int k = 1; k < ne; ++k) one = one && el[k].shape == el[0].shape;
+      const int s0 = el[0].shape;
+      if (one && (s0 == HGX_SHAPE_SPHERE || s0 == HGX_SHAPE_CYLINDER || s0 == HGX_SHAPE_BOX || s0 == HGX_SHAPE_CUBE))
+        prec = HGX_PREC_FP32C;
+    }
+    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, prec, d_fc, 0, d_I), "hgx_ensemble_intensity")) return false;
     counts_[0] += ng;
     counts_[3] += ng;
     return true;

@@ -294,11 +294,13 @@ typedef struct hgx_ensemble_model {
 
 /* I[p] (+)= sum_g |w_g sum_b fc_b S_g(R_g q_b) sum_e dn2_e F_e(R_g R_e q_b) sum_l exp(i (R_g R_e q_b).t_l)|^2:
  * the grain, element and location loops, structure factor and DWBA sum of run_gisaxs
- * (src/sim/hipgisaxs_main.cpp:925-1100) for any analytic unit cell in one pass per pixel, FP64. fc
- * (device [2*nq]) may be NULL (Born). Equals hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice +
- * hgx_dwba_intensity per grain to round-off. Synchronises the stream. */
-int hgx_ensemble_intensity(hgx_ctx* ctx, const hgx_ensemble_model* model, const double* fc, int accumulate,
-                           double* I);
+ * (src/sim/hipgisaxs_main.cpp:925-1100) for any analytic unit cell in one pass per pixel. fc (device
+ * [2*nq]) may be NULL (Born). HGX_PREC_FP64 equals hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice +
+ * hgx_dwba_intensity per grain to round-off; HGX_PREC_FP32C (real q, cells whose elements are all
+ * spheres, all cylinders, all boxes or all cubes) evaluates the form factors and Dirichlet kernels in
+ * FP32 on phases reduced in FP64 and sums amplitudes in FP64. Synchronises the stream. */
+int hgx_ensemble_intensity(hgx_ctx* ctx, const hgx_ensemble_model* model, int precision, const double* fc,
+                           int accumulate, double* I);
 
 /* ---- chi^2 and its gradient (forward / backward pair) ------------------------------------- */
 

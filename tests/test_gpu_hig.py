@@ -91,7 +91,7 @@ def test_reference_compatible_switches(name, flags, kw):
     assert _rel(ref, dflt) > 1e-6
 
 
-@pytest.mark.parametrize("name", ["spheres_bcc", "cylinders_hcp", "mesh_tetra"])
+@pytest.mark.parametrize("name", ["spheres_bcc", "cylinders_hcp", "mesh_tetra", "box_grating", "saxs_cubes"])
 def test_fp32c_paths(name):
     _say(name, "fp32c")
     text = _text(name)
@@ -168,6 +168,10 @@ def test_ensemble_kernel_matches_per_grain_calls_at_full_size():
     r = _rel(ia, ib)
     _say("ensemble vs per-grain relL2", r)
     assert r <= 1e-12
+    ic = H.HigSimulation(text=box, precision="fp32c").compute()
+    r32 = _rel(ic, ia)
+    _say("gratings fp32c vs fp64 relL2", r32)
+    assert r32 <= TOL32
     text = _text("two_structures")
     fa = H.HigSimulation(text=text).run_all()
     fb = H.HigSimulation(text=text, flags=H.NO_FUSE).run_all()
