@@ -163,10 +163,6 @@ class HigInput {
   std::string base_dir_ = ".";
 };
 
-}  // namespace hig
-
-namespace hig {
-
 // Mesh files of custom shapes (ObjectShapeReader, src/file/objectshape_reader.cpp:52-300): Wavefront
 // OBJ vertices and faces ("f i", "f i/j", "f i//k", "f i/j/k", negative indices counting back from
 // the last vertex) into 9 doubles per triangle (v1, v2, v3). Polygons of n > 3 vertices are split

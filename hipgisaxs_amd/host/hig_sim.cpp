@@ -318,7 +318,9 @@ bool HipGISAXS::make_grains(const HigStructure& s, GlibcRandom& rng, double phi,
         nn[k * ng + g] = (rot[k].amin + (double(rng.next()) / GlibcRandom::kRandMax) * d) * PI_ / 180;
     }
   } else if (st == "gaussian" || st == "normal" || st == "cauchy") {
-    gauss = true;
+    // run_gisaxs multiplies the grain weights in for "gaussian" and "cauchy" only; "normal" samples the
+    // same angles and weights but leaves the grains unweighted (src/sim/hipgisaxs_main.cpp:934-938)
+    gauss = st != "normal";
     const bool cauchy = st == "cauchy";
     for (int k = 0; k < 3; ++k) {
       double d = std::fabs(rot[k].amax - rot[k].amin);

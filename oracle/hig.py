@@ -293,19 +293,28 @@ def grains(st, phi, rand):
             d = abs(hi - lo)
             for g in range(ng):
                 nn[k, g] = (lo + (next(rand) / 2147483647.0) * d) * PI_ / 180
-    elif stat in ("gaussian", "normal"):
-        weighted = True
+    elif stat in ("gaussian", "normal", "cauchy"):
+        # run_gisaxs weights the grains for "gaussian" and "cauchy" only (hipgisaxs_main.cpp:934-938)
+        weighted = stat != "normal"
         for k in range(3):
             lo, hi = rots[k]["angles"]
             d = abs(hi - lo)
+            if stat == "cauchy":
+                d = d / (ng - 1) if ng > 1 else 0.0
             if d > DBL_MIN:
-                d /= ng
+                if stat != "cauchy":
+                    d /= ng
                 for g in range(ng):
                     t = lo + g * d
                     nn[k, g] = t * PI_ / 180
-                    sd, mean = rots[k]["sd"], rots[k]["mean"]
-                    w[k, g] = ((1.0 if t - mean < DBL_MIN else 0.0) if sd < DBL_MIN else
-                               1.0 / (sd * SQRT_2PI_) * math.exp(-(t - mean) * (t - mean) / (2 * sd * sd)))
+                    if stat == "cauchy":
+                        s, l = rots[k]["scale"], rots[k]["location"]
+                        w[k, g] = (1.0 / (PI_ * s * (1.0 + ((t - l) / s) * ((t - l) / s))) if s > DBL_MIN
+                                   else (0.0 if t - l > DBL_MIN else 1.0))
+                    else:
+                        sd, mean = rots[k]["sd"], rots[k]["mean"]
+                        w[k, g] = ((1.0 if t - mean < DBL_MIN else 0.0) if sd < DBL_MIN else
+                                   1.0 / (sd * SQRT_2PI_) * math.exp(-(t - mean) * (t - mean) / (2 * sd * sd)))
             else:
                 nn[k, :] = lo * PI_ / 180
     else:

@@ -22,8 +22,10 @@ hipGisaxsInput = {{
   structure = {{ key = "a", grain = {{ unitcell:key = "u" }},
                 ensemble = {{ maxgrains = [7 5 3], spacing = [{sp}],
                              orientations = {{ stat = "{stat}",
-                               rot1 = {{ axis = "z", angles = [0 360], anglemean = 100, anglesd = 40 }},
-                               rot2 = {{ axis = "x", angles = [0 10], anglemean = 2, anglesd = 3 }},
+                               rot1 = {{ axis = "z", angles = [0 360], anglemean = 100, anglesd = 40,
+                                        anglelocation = 90, anglescale = 30 }},
+                               rot2 = {{ axis = "x", angles = [0 10], anglemean = 2, anglesd = 3,
+                                        anglelocation = 4, anglescale = 2 }},
                                rot3 = {{ axis = "{ax3}", angles = [5 20], anglemean = 9, anglesd = 4 }} }} }} }},
   structure = {{ key = "b", grain = {{ unitcell:key = "u", transvec = [1 2 3] }},
                 ensemble = {{ maxgrains = [3 2 1], orientations = {{ stat = "{stat}",
@@ -32,6 +34,15 @@ hipGisaxsInput = {{
   computation = {{ outputregion = {{ type = "qspace", minpoint = [-1 0], maxpoint = [1 1] }}, resolution = [8 6] }}
 }}
 """
+
+
+def test_normal_orientations_are_unweighted():
+    """"normal" draws the angles and weights of "gaussian" but run_gisaxs weights only "gaussian" and
+    "cauchy" grains."""
+    g = H.HigSimulation(text=SAMPLER.format(stat="gaussian", ax3="z", sp="0 0 0", phi=0)).grains("a")
+    n = H.HigSimulation(text=SAMPLER.format(stat="normal", ax3="z", sp="0 0 0", phi=0)).grains("a")
+    np.testing.assert_array_equal(g[0], n[0])
+    assert np.all(n[2] == 1.0) and not np.all(g[2] == 1.0)
 
 
 def _grains_oracle(text, key, seed):
@@ -46,7 +57,8 @@ def _grains_oracle(text, key, seed):
 
 
 @pytest.mark.parametrize("stat,ax3,sp", [("range", "y", "0 0 0"), ("random", "z", "0 0 0"),
-                                         ("gaussian", "n", "40 30 20"), ("single", "x", "10 0 5")])
+                                         ("gaussian", "n", "40 30 20"), ("normal", "z", "0 0 0"),
+                                         ("cauchy", "y", "5 5 0"), ("single", "x", "10 0 5")])
 @pytest.mark.parametrize("seed", [1, 2, 987654])
 def test_grains_match_oracle_and_libc_rand(stat, ax3, sp, seed):
     """Orientation sampling, grain centres and weights (spatial_distribution / orientation_distribution,
