@@ -278,6 +278,53 @@ def lattice_intensity(ctx: Context, shape: str, dims, la, lb, lc, repet, rots, w
     return o
 
 
+def ensemble_intensity(ctx: Context, elements, la, lb, lc, repet, rots, centers=None, weights=None, dw_sigma=0.0,
+                       sf_flags=0, precision="fp64", fc=None, accumulate=False, out=None):
+    """hgx_ensemble_intensity: a lattice of unit cells of analytic elements under several grains.
+    elements: dicts with shape, dims [(values, weights), ...], rot (3x3, default identity), locations
+    [[x, y, z], ...] and dn2 (complex)."""
+    keep = []
+    el = (N.EnsembleElement * len(elements))()
+    for e, d in zip(el, elements):
+        e.shape = N.SHAPES[d["shape"]]
+        _fill_shape_params(e.params, d["dims"], keep)
+        e.params.tau, e.params.eta = float(d.get("tau", 0.0)), float(d.get("eta", 0.0))
+        e.rot[:] = [float(x) for x in np.asarray(d.get("rot", np.eye(3)), dtype=float).ravel()]
+        L = _f64(d.get("locations", [[0.0, 0.0, 0.0]])).reshape(-1, 3)
+        keep.append(L)
+        e.nloc = L.shape[0]
+        e.loc = L.ctypes.data_as(C.POINTER(C.c_double))
+        e.dn2_re, e.dn2_im = complex(d.get("dn2", 1.0)).real, complex(d.get("dn2", 1.0)).imag
+    m = N.EnsembleModel()
+    m.nelem = len(elements)
+    m.elem = el
+    for name, v in (("la", la), ("lb", lb), ("lc", lc), ("repet", repet)):
+        getattr(m, name)[:] = [float(x) for x in v]
+    m.dw_sigma = dw_sigma
+    m.sf_flags = sf_flags
+    R = _f64(rots).reshape(-1, 9)
+    keep.append(R)
+    m.ngrains = R.shape[0]
+    m.rot = R.ctypes.data_as(C.POINTER(C.c_double))
+    for name, arr, w in (("center", centers, 3), ("weight", weights, 1)):
+        if arr is not None:
+            A = _f64(arr).ravel()
+            keep.append(A)
+            setattr(m, name, A.ctypes.data_as(C.POINTER(C.c_double)))
+    o = ctx._out(ctx.info()["npix"], out)
+    p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+    ctx._check(ctx.lib.hgx_ensemble_intensity(ctx.h, C.byref(m), p, None if fc is None else C.c_void_p(fc.data_ptr()),
+                                              int(accumulate), C.c_void_p(o.data_ptr())), "ensemble_intensity")
+    return o
+
+
+def gaussian_smear(ctx: Context, image, ncol: int, nrow: int, sigma: float, compat: bool = False):
+    """hgx_gaussian_smear in place on a device image [nrow * ncol]."""
+    ctx._check(ctx.lib.hgx_gaussian_smear(ctx.h, int(ncol), int(nrow), float(sigma), N.SMEAR_COMPAT if compat else 0,
+                                          C.c_void_p(image.data_ptr())), "gaussian_smear")
+    return image
+
+
 class Batch:
     """hgx_batch_create / hgx_batch_run: items are (grid_context_or_None, mesh_id, rot or None, out tensor)."""
 
