@@ -193,6 +193,7 @@ _SIGS = {
     "hgx_gaussian_smear": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),
     "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_grating_plan_metric": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_grating_set_theta": (C.c_int, [_vp, C.c_int, _vp]),
     "hgx_grating_eval": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_grating_lbfgs": (C.c_int, [_vp, C.c_int, C.c_int]),

@@ -497,6 +497,11 @@ class GratingPlan:
                                                    C.byref(pid)), "grating_plan_create")
         self.id = pid.value
 
+    def metric(self, name: str) -> None:
+        """Objective: "chi2" (default), "l2", "sqrt_unit_l2" or "sqrt_c_l2"."""
+        self.ctx._check(self.ctx.lib.hgx_grating_plan_metric(self.ctx.h, self.id, N.METRICS[name]),
+                        "grating_plan_metric")
+
     def set_theta(self, theta):
         t = _f64(theta).reshape(self.ncand, 4)
         self.ctx._check(self.ctx.lib.hgx_grating_set_theta(self.ctx.h, self.id, _ptr(t)), "grating_set_theta")

@@ -217,26 +217,8 @@ __global__ __launch_bounds__(kFitWG) void k_fit(QViewF g, const double* __restri
       out[4] = mk * mk * sI * sO;
       out[5] = mk * mk * sO * sO;
     } else {
-      double gI = 0.0;  // dD/dI_p
-      if (a.metric == HGX_METRIC_CHI2) {
-        const double wgt = mk / fmax(o, 1.0);
-        out[0] = wgt * (I - o) * (I - o);
-        gI = 2.0 * wgt * (I - o);
-      } else if (a.metric == HGX_METRIC_L2) {
-        const double d = mk * (I - o);
-        out[0] = d * d;
-        gI = 2.0 * mk * d;
-      } else if (sI > 0.0) {
-        // s = sqrt(I), r = sqrt(O), dD/dI = (dD/ds) / (2 s).
-        // sqrt-unit-L2: D = sum m^2 (r/Nr - s/Ns)^2, Ns^2 = A1, Nr^2 = A3, L = B1/Ns - B2/Nr:
-        //   dD/dI_p = m^2/Ns^2 - m^2 r/(Nr Ns s) - m L/Ns^3          (k = 1/Ns^2, 1/(Nr Ns), L/Ns^3)
-        // sqrt-c-L2: D = sum m^2 (r - c s)^2, c = A2/A1, K = B2 - c B1:
-        //   dD/dI_p = -m^2 c (r/s - c) - (K/A1) m (r/s - 2c)         (k = c, K/A1)
-        const double ratio = sO / sI;
-        gI = a.metric == HGX_METRIC_SQRT_UNIT_L2
-                 ? mk * mk * a.k[0] - mk * mk * a.k[1] * ratio - mk * a.k[2]
-                 : -mk * mk * a.k[0] * (ratio - a.k[0]) - a.k[1] * mk * (ratio - 2.0 * a.k[0]);
-      }
+      // dD/dI_p and the pixel's share of D (metric_from_sums documents the normalised forms)
+      const double gI = metric_seed(a.metric, I, o, mk, a.k, &out[0]);
       if (GRAD) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) out[1 + j] = gI * 2.0 * sc * (amp.r * dA[j].r + amp.i * dA[j].i);
@@ -337,31 +319,6 @@ static int launch_fit(Context* c, const FitPlan& fp, const FitArgs& a, int mode,
   }
   HGX_CHECK_LAUNCH(c);
   return HGX_OK;
-}
-
-// The normalised metrics from the six sums of the statistics pass: the distance D and the seed
-// constants k[0..2] (see k_fit). A zero model image (A1 = 0) is taken as u = 0 (sqrt-unit) and c = 0.
-static double metric_from_sums(int metric, const double* S, double k[3]) {
-  const double A1 = S[0], A2 = S[1], A3 = S[2], B1 = S[3], B2 = S[4], B3 = S[5];
-  k[0] = k[1] = k[2] = 0.0;
-  if (metric == HGX_METRIC_SQRT_UNIT_L2) {
-    const double Nr = std::sqrt(A3), Ns = std::sqrt(A1);
-    double D = Nr > 0 ? B3 / A3 : 0.0;
-    if (Ns > 0) {
-      D += B1 / A1 - (Nr > 0 ? 2.0 * B2 / (Nr * Ns) : 0.0);
-      const double L = B1 / Ns - (Nr > 0 ? B2 / Nr : 0.0);
-      k[0] = 1.0 / A1;
-      k[1] = Nr > 0 ? 1.0 / (Nr * Ns) : 0.0;
-      k[2] = L / (A1 * Ns);
-    }
-    return D;
-  }
-  // sqrt-c-L2
-  const double cc = A1 > 0 ? A2 / A1 : 0.0;
-  const double K = B2 - cc * B1;
-  k[0] = cc;
-  k[1] = A1 > 0 ? K / A1 : 0.0;
-  return B3 - 2.0 * cc * B2 + cc * cc * B1;
 }
 
 static bool normalised(int metric) {

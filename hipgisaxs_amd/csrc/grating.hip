@@ -317,13 +317,19 @@ __device__ inline double g_dirichlet(double x, int N) {
   return sin(0.5 * N * x) / sh;
 }
 
-// One thread per pixel, blockIdx.y = item. Reads the raw sums (chunked or not), writes Wt = -i W / |q|^2
-// (FP32) for the adjoint, per-block partial sums of chi^2 and dchi^2/ds, and optionally I.
+// One thread per pixel, blockIdx.y = item. Reads the raw sums (chunked or not) and forms the pixel's
+// model intensity. STATS: the six masked sums of the normalised metrics as per-block partials (spart).
+// Otherwise: Wt = -i W / |q|^2 (FP32) for the adjoint, with W the derivative of the objective through
+// the pixel's amplitude (seed dD/dI from metric_seed, constants per candidate in mconst), per-block
+// partial sums of the objective and dD/ds, and optionally I.
+template <bool STATS>
 __global__ __launch_bounds__(256) void k_grating_pixel(GPixArgs a, const GState* __restrict__ st, int cand0,
                                                        const double* __restrict__ fbuf, int nchunks, int64_t nq,
+                                                       int metric, const double* __restrict__ mconst,
                                                        float* __restrict__ wt, double* __restrict__ cpart,
                                                        double* __restrict__ Iout) {
-  __shared__ double sh[2][256];
+  constexpr int NS = STATS ? 6 : 2;
+  __shared__ double sh[NS][256];
   const int item = blockIdx.y;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const GState& S = st[cand0 + item];
@@ -331,7 +337,9 @@ __global__ __launch_bounds__(256) void k_grating_pixel(GPixArgs a, const GState*
   // radius of the line about its centre, for the small-|q| exclusion
   const double wb = wt_t + 2.0 * h * cos(al) / sin(al);
   const double Rl = sqrt(0.25 * a.L * a.L + 0.25 * fmax(wb, wt_t) * fmax(wb, wt_t) + 0.25 * h * h);
-  double c2 = 0.0, cs = 0.0;
+  double acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
   if (p < a.npix) {
     const double* fb = fbuf + (int64_t)item * nchunks * nq * 2;
     cd Fb[4] = {};
@@ -355,40 +363,72 @@ __global__ __launch_bounds__(256) void k_grating_pixel(GPixArgs a, const GState*
     A = dn2 * A;
     const double sf = g_dirichlet(qy * a.pitch, a.periods);
     const double I = sc * sf * sf * norm2(A);
-    if (Iout) Iout[(int64_t)item * a.npix + p] = I;
     const double O = a.obs[p];
     const double m = (a.mask ? a.mask[p] : 1.0) * (near0 ? 0.0 : 1.0);
-    const double wgt = m / fmax(O, 1.0);
-    const double res = I - O;
-    c2 = wgt * res * res;
-    cs = sc != 0.0 ? 2.0 * wgt * res * I / sc : 0.0;
-    // W_b = 4 wgt res s SF^2 conj(A) dn2 fc_b; Wt_b = -i W_b / |q_b|^2
-    const cd base = (4.0 * wgt * res * sc * sf * sf) * (conj(A) * dn2);
-    for (int b = 0; b < a.nbranch; ++b) {
-      const int64_t q = b * a.npix + p;
-      const cd fcb = a.fc ? cd{a.fc[2 * q], a.fc[2 * q + 1]} : cd{1.0, 0.0};
-      const cd W = base * fcb;
-      const double s2 = q2b[b] > 0.0 ? 1.0 / q2b[b] : 0.0;
-      float* o = wt + ((int64_t)item * nq + q) * 2;
-      o[0] = (float)(W.i * s2);
-      o[1] = (float)(-W.r * s2);
+    if (STATS) {
+      const double sI = I > 0.0 ? sqrt(I) : 0.0, sO = O > 0.0 ? sqrt(O) : 0.0;
+      acc[0] = m * sI * sI;
+      acc[1] = m * sI * sO;
+      acc[2] = m * sO * sO;
+      acc[3] = m * m * sI * sI;
+      acc[4] = m * m * sI * sO;
+      acc[5] = m * m * sO * sO;
+    } else {
+      if (Iout) Iout[(int64_t)item * a.npix + p] = I;
+      double k[3] = {0.0, 0.0, 0.0};
+      if (mconst) {
+        k[0] = mconst[4 * item];
+        k[1] = mconst[4 * item + 1];
+        k[2] = mconst[4 * item + 2];
+      }
+      double term;
+      const double gI = metric_seed(metric, I, O, m, k, &term);
+      acc[0] = term;
+      acc[1] = sc != 0.0 ? gI * I / sc : 0.0;
+      // W_b = 2 gI s SF^2 conj(A) dn2 fc_b; Wt_b = -i W_b / |q_b|^2
+      const cd base = (2.0 * gI * sc * sf * sf) * (conj(A) * dn2);
+      for (int b = 0; b < a.nbranch; ++b) {
+        const int64_t q = b * a.npix + p;
+        const cd fcb = a.fc ? cd{a.fc[2 * q], a.fc[2 * q + 1]} : cd{1.0, 0.0};
+        const cd W = base * fcb;
+        const double s2 = q2b[b] > 0.0 ? 1.0 / q2b[b] : 0.0;
+        float* o = wt + ((int64_t)item * nq + q) * 2;
+        o[0] = (float)(W.i * s2);
+        o[1] = (float)(-W.r * s2);
+      }
     }
   }
-  sh[0][threadIdx.x] = c2;
-  sh[1][threadIdx.x] = cs;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) sh[k][threadIdx.x] = acc[k];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-    }
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < NS; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double* o = cpart + ((int64_t)item * gridDim.x + blockIdx.x) * 2;
-    o[0] = sh[0][0];
-    o[1] = sh[1][0];
+  if (threadIdx.x < NS) {
+    double* o = cpart + ((int64_t)item * gridDim.x + blockIdx.x) * NS;
+    o[threadIdx.x] = sh[threadIdx.x][0];
   }
+}
+
+// Per candidate: the six sums of the statistics pass (fixed block order) -> the normalised metric's
+// distance and seed constants, mconst[item] = (k0, k1, k2, D).
+__global__ void k_grating_mconst(int gsize, int metric, const double* __restrict__ spart, int nblk,
+                                 double* __restrict__ mconst) {
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= gsize) return;
+  double S[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nblk; ++b)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S[k] += spart[((int64_t)item * nblk + b) * 6 + k];
+  double kk[3];
+  const double D = metric_from_sums(metric, S, kk);
+  mconst[4 * item] = kk[0];
+  mconst[4 * item + 1] = kk[1];
+  mconst[4 * item + 2] = kk[2];
+  mconst[4 * item + 3] = D;
 }
 
 // ----- gradient contraction --------------------------------------------------------------------
@@ -431,7 +471,8 @@ __global__ __launch_bounds__(256) void k_grating_grad(GMesh gm, const double* __
 
 // chi^2 and gradient of each item at its trial point (fixed-order sums of the block partials).
 __global__ void k_grating_reduce(GState* __restrict__ st, int cand0, int gsize, const double* __restrict__ cpart,
-                                 int ncb, const double* __restrict__ gpart, int ngb) {
+                                 int ncb, const double* __restrict__ gpart, int ngb,
+                                 const double* __restrict__ mconst) {
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= gsize) return;
   double c2 = 0.0, cs = 0.0, g[3] = {0, 0, 0};
@@ -443,7 +484,7 @@ __global__ void k_grating_reduce(GState* __restrict__ st, int cand0, int gsize, 
 #pragma unroll
     for (int p = 0; p < 3; ++p) g[p] += gpart[((int64_t)item * ngb + b) * 3 + p];
   GState& S = st[cand0 + item];
-  S.ft = c2;
+  S.ft = mconst ? mconst[4 * item + 3] : c2;
   S.gt[0] = g[0];
   S.gt[1] = g[1];
   S.gt[2] = g[2];
@@ -609,6 +650,9 @@ struct GratingPlan {
   double* d_apart = nullptr;
   double* d_gpart = nullptr;
   double* d_cpart = nullptr;
+  double* d_spart = nullptr;   // statistics-pass partials [G][npb][6]
+  double* d_mconst = nullptr;  // per-candidate metric constants [G][4]
+  int metric = HGX_METRIC_CHI2;
   ProductItem* d_pitems = nullptr;
   ProductUnit* d_punits = nullptr;
   AdjItem* d_aitems = nullptr;
@@ -631,7 +675,8 @@ void free_plan(GratingPlan& p) {
   if (p.eval_graph) (void)hipGraphDestroy(p.eval_graph);
   if (p.step_graph) (void)hipGraphDestroy(p.step_graph);
   void* ptrs[] = {p.d_state, p.d_rec64, p.d_drec, p.d_rec32, p.d_borig, p.d_bmax, p.d_fbuf, p.d_wt, p.d_pmax,
-                  p.d_w16, p.d_wexp, p.d_apart, p.d_gpart, p.d_cpart, p.d_pitems, p.d_punits, p.d_aitems};
+                  p.d_w16, p.d_wexp, p.d_apart, p.d_gpart, p.d_cpart, p.d_spart, p.d_mconst, p.d_pitems,
+                  p.d_punits, p.d_aitems};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p = GratingPlan();
@@ -649,8 +694,15 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   k_grating_scales<<<gsize, 256, 0, s>>>(gm.nblocks, p.d_bmax, p.d_pitems);
   product_launch_units(s, p.d_pitems, p.d_punits, gsize * p.upi_fwd, p.cq, p.pmode);
   dim3 gpix(p.npb, gsize);
-  k_grating_pixel<<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.d_wt, p.d_cpart,
-                                       want_I ? Iout : nullptr);
+  const bool norm = p.metric == HGX_METRIC_SQRT_UNIT_L2 || p.metric == HGX_METRIC_SQRT_C_L2;
+  if (norm && !want_I) {
+    k_grating_pixel<true><<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.metric,
+                                               nullptr, nullptr, p.d_spart, nullptr);
+    k_grating_mconst<<<ceil_div(gsize, 64), 64, 0, s>>>(gsize, p.metric, p.d_spart, p.npb, p.d_mconst);
+  }
+  k_grating_pixel<false><<<gpix, 256, 0, s>>>(p.pa, p.d_state, cand0, p.d_fbuf, p.nchunks, c->nq, p.metric,
+                                              norm && !want_I ? p.d_mconst : nullptr, p.d_wt, p.d_cpart,
+                                              want_I ? Iout : nullptr);
   if (want_I) return;
   AdjItem none{};
   if (p.adj_f16) {
@@ -661,7 +713,8 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq, p.adj_variant);
   dim3 ggrad(p.ngb, gsize);
   k_grating_grad<<<ggrad, 256, 0, s>>>(gm, p.d_apart, p.nyt, p.d_rec64, p.d_drec, c->d_qx, p.d_gpart);
-  k_grating_reduce<<<ceil_div(gsize, 64), 64, 0, s>>>(p.d_state, cand0, gsize, p.d_cpart, p.npb, p.d_gpart, p.ngb);
+  k_grating_reduce<<<ceil_div(gsize, 64), 64, 0, s>>>(p.d_state, cand0, gsize, p.d_cpart, p.npb, p.d_gpart, p.ngb,
+                                                       norm ? p.d_mconst : nullptr);
 }
 
 void all_groups(const GratingPlan& p, Context* c, hipStream_t s) {
@@ -778,7 +831,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   const double per = (double)gm.nslots * (8 * 8 + 18 * 8 + 8 * 4) + (double)gm.nblocks * 40 +
                      (double)p.nchunks * c->nq * 16 + (double)c->nq * 8 + 6.0 * p.nyt * gm.nslots * 16 +
                      (p.adj_f16 ? 96.0 * nzo * c->ncol + 4.0 * adjoint_pmax_count(nz, c->ncol) + 4.0 * c->ncol : 0.0) +
-                     (double)(p.npb * 2 + p.ngb * 3) * 8;
+                     (double)(p.npb * 8 + p.ngb * 3 + 4) * 8;
   p.G = (int)std::max(1.0, std::min((double)ncand, 24.0 * (1 << 30) / per));
   p.ngroups = (ncand + p.G - 1) / p.G;
   if ((double)p.G * p.upi_fwd > 2.0e9 || (double)p.G * p.upi_adj > 2.0e9)
@@ -805,6 +858,8 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   if (hipMalloc(&p.d_apart, sizeof(double) * 2 * 6 * p.nyt * gm.nslots * G) != hipSuccess) return fail("adjoint");
   if (hipMalloc(&p.d_gpart, sizeof(double) * 3 * p.ngb * G) != hipSuccess) return fail("gradient partials");
   if (hipMalloc(&p.d_cpart, sizeof(double) * 2 * p.npb * G) != hipSuccess) return fail("chi2 partials");
+  if (hipMalloc(&p.d_spart, sizeof(double) * 6 * p.npb * G) != hipSuccess) return fail("metric partials");
+  if (hipMalloc(&p.d_mconst, sizeof(double) * 4 * G) != hipSuccess) return fail("metric constants");
   std::vector<ProductItem> pit(G);
   std::vector<AdjItem> ait(G);
   for (int i = 0; i < G; ++i) {
@@ -883,6 +938,27 @@ extern "C" int hgx_grating_plan_release(hgx_ctx* h, int plan) {
   free_plan(*p);
   delete p;
   gplans(c)[plan] = nullptr;
+  return HGX_OK;
+}
+
+extern "C" int hgx_grating_plan_metric(hgx_ctx* h, int plan, int metric) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  GratingPlan* p = get_plan(c, plan);
+  if (!p) return set_error(c, HGX_ERR_BAD_HANDLE, "grating_plan_metric: unknown plan %d", plan);
+  if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_SQRT_C_L2)
+    return set_error(c, HGX_ERR_INVALID_ARG, "grating_plan_metric: unknown metric %d", metric);
+  if (metric == p->metric) return HGX_OK;
+  if (int rc = set_device(c)) return rc;
+  // the captured graphs hold the old metric's launches: drop them (captured again on next use)
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  if (p->eval_exec) (void)hipGraphExecDestroy(p->eval_exec);
+  if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+  if (p->eval_graph) (void)hipGraphDestroy(p->eval_graph);
+  if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
+  p->eval_exec = p->step_exec = nullptr;
+  p->eval_graph = p->step_graph = nullptr;
+  p->metric = metric;
   return HGX_OK;
 }
 

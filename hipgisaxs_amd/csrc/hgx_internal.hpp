@@ -314,6 +314,60 @@ inline Vec3 make_vec(const double* v) {
   return o;
 }
 
+// The square-root normalised distance functors from six masked sums of the model image I and the
+// data O: S = (sum m I, sum m sqrt(IO), sum m O, sum m^2 I, sum m^2 sqrt(IO), sum m^2 O) (norm_l2 and
+// vec_dot weight by the mask, the squared distance by its square; include/analyzer/
+// distance_functions.hpp:40-57, 423-499). Returns the distance D and the constants k[0..2] of the
+// per-pixel seed dD/dI, with s = sqrt I, r = sqrt O:
+//   sqrt-unit-L2: D = B3/A3 - 2 B2/(Nr Ns) + B1/A1 (Ns^2 = A1, Nr^2 = A3), L = B1/Ns - B2/Nr,
+//                 dD/dI = m^2 k0 - m^2 k1 r/s - m k2 with k = (1/A1, 1/(Nr Ns), L/Ns^3);
+//   sqrt-c-L2:    c = A2/A1, K = B2 - c B1, D = B3 - 2 c B2 + c^2 B1,
+//                 dD/dI = -m^2 k0 (r/s - k0) - m k1 (r/s - 2 k0) with k = (c, K/A1).
+// A zero model image (A1 = 0) is taken as u = 0 (sqrt-unit) and c = 0.
+__host__ __device__ inline double metric_from_sums(int metric, const double* S, double k[3]) {
+  const double A1 = S[0], A2 = S[1], A3 = S[2], B1 = S[3], B2 = S[4], B3 = S[5];
+  k[0] = k[1] = k[2] = 0.0;
+  if (metric == HGX_METRIC_SQRT_UNIT_L2) {
+    const double Nr = sqrt(A3), Ns = sqrt(A1);
+    double D = Nr > 0 ? B3 / A3 : 0.0;
+    if (Ns > 0) {
+      D += B1 / A1 - (Nr > 0 ? 2.0 * B2 / (Nr * Ns) : 0.0);
+      const double L = B1 / Ns - (Nr > 0 ? B2 / Nr : 0.0);
+      k[0] = 1.0 / A1;
+      k[1] = Nr > 0 ? 1.0 / (Nr * Ns) : 0.0;
+      k[2] = L / (A1 * Ns);
+    }
+    return D;
+  }
+  const double cc = A1 > 0 ? A2 / A1 : 0.0;
+  const double K = B2 - cc * B1;
+  k[0] = cc;
+  k[1] = A1 > 0 ? K / A1 : 0.0;
+  return B3 - 2.0 * cc * B2 + cc * cc * B1;
+}
+
+// dD/dI of one pixel for every metric (chi^2, L2 and the two normalised forms with the constants
+// above); term is the pixel's own share of D (0 for the normalised forms, whose D comes from the sums).
+__host__ __device__ inline double metric_seed(int metric, double I, double O, double m, const double k[3],
+                                              double* term) {
+  if (metric == HGX_METRIC_CHI2) {
+    const double w = m / fmax(O, 1.0);
+    *term = w * (I - O) * (I - O);
+    return 2.0 * w * (I - O);
+  }
+  if (metric == HGX_METRIC_L2) {
+    const double d = m * (I - O);
+    *term = d * d;
+    return 2.0 * m * d;
+  }
+  *term = 0.0;
+  const double sI = I > 0.0 ? sqrt(I) : 0.0, sO = O > 0.0 ? sqrt(O) : 0.0;
+  if (!(sI > 0.0)) return 0.0;
+  const double ratio = sO / sI;
+  return metric == HGX_METRIC_SQRT_UNIT_L2 ? m * m * k[0] - m * m * k[1] * ratio - m * k[2]
+                                           : -m * m * k[0] * (ratio - k[0]) - k[1] * m * (ratio - 2.0 * k[0]);
+}
+
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 // Rotates an item's block-relative FP32 records and FP64 block origins by R^T.

@@ -382,6 +382,11 @@ typedef struct hgx_grating_model {
 int hgx_grating_plan_create(hgx_ctx* ctx, const hgx_grating_model* model, int ncand, const double* fc,
                             const double* I_obs, const double* mask, int* plan);
 int hgx_grating_plan_release(hgx_ctx* ctx, int plan);
+/* Objective of the plan's candidates: HGX_METRIC_CHI2 (default), HGX_METRIC_L2, HGX_METRIC_SQRT_UNIT_L2 or
+ * HGX_METRIC_SQRT_C_L2 (definitions at hgx_fit_plan_metric); the normalised forms add a statistics pass
+ * and a per-candidate reduction on the device before the adjoint. The optimiser and hgx_grating_get
+ * then work on that objective (reported where chi^2 is named). */
+int hgx_grating_plan_metric(hgx_ctx* ctx, int plan, int metric);
 /* Sets every candidate's theta (host [ncand][4]) and resets its optimiser state. */
 int hgx_grating_set_theta(hgx_ctx* ctx, int plan, const double* theta);
 /* chi^2 and its gradient (adjoint, one pass) for every candidate at its theta; use_graph replays the

@@ -256,3 +256,57 @@ def test_grating_full_size_image_matches_fp64_path():
         assert e <= 1e-5
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("metric", ["l2", "sqrt_unit_l2", "sqrt_c_l2"])
+def test_grating_distance_metrics(oracle_mod, metric):
+    """The reference's distance functors as the batched fit's objective: values against the oracle's
+    functors on the host FP64 model image (with the plan's small-|q| exclusion as the mask), gradients
+    against central differences (h = 1e-5 relative) of that FP64 host objective, and graph replay."""
+    import torch
+
+    from hipgisaxs_amd.api import GratingPlan
+    from oracle import metrics as M
+
+    O = oracle_mod
+    c, g, fc, fc_d = _setup(O, 40, 32, 0j)
+    try:
+        host0 = Host(O, g, fc, np.zeros(g.npix))
+        I1 = host0.parts(THETA_TRUE)[5]
+        scale = 1e4 / I1.max()
+        obs = scale * I1 * (1 + 0.05 * np.sin(np.arange(g.npix)))  # data not exactly of the model's form
+        host = Host(O, g, fc, obs)
+        th0 = THETA_TRUE * [1.04, 0.99, 0.96, scale * 1.1]
+
+        def f_host(th):
+            parts = host.parts(th)
+            return M.METRICS[metric](obs, parts[5], (parts[6] > 0).astype(float))
+
+        plan = GratingPlan(c, 1, torch.from_numpy(obs).cuda(), fc=fc_d, dn2=DN2, **MODEL)
+        plan.metric(metric)
+        plan.set_theta(th0[None])
+        plan.eval()
+        _, val, grad = plan.get()
+        ref = f_host(th0)
+        print(metric, "value", val[0], "host", ref)
+        assert abs(val[0] - ref) <= 1e-4 * abs(ref)
+        fd = np.zeros(4)
+        for j in range(4):
+            h = 1e-5 * th0[j]
+            tp, tm = th0.copy(), th0.copy()
+            tp[j] += h
+            tm[j] -= h
+            fd[j] = (f_host(tp) - f_host(tm)) / (2 * h)
+        print(metric, "adjoint", grad[0], "host central differences", fd)
+        big = np.abs(fd * th0).max()
+        for j in range(4):
+            assert abs(fd[j] - grad[0][j]) * th0[j] <= 2e-3 * max(abs(fd[j]) * th0[j], 1e-3 * big), j
+        if metric != "l2":  # scale-invariant objectives
+            assert abs(grad[0][3]) * th0[3] <= 1e-6 * big
+        plan.eval(use_graph=True)
+        plan.eval(use_graph=True)
+        _, val_g, grad_g = plan.get()
+        assert np.allclose(val_g, val, rtol=1e-12) and np.allclose(grad_g, grad, rtol=1e-12)
+        plan.release()
+    finally:
+        c.close()
