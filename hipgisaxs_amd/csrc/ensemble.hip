@@ -301,6 +301,8 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, i
   if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
     return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: bad precision");
   const int ne = m->nelem, ng = m->ngrains;
+  if ((int64_t)ng * ne > ((int64_t)1 << 24))
+    return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: grains x elements above 2^24");
   std::vector<EnsElem> E(ne);
   std::vector<double> loc;
   for (int e = 0; e < ne; ++e) {

@@ -278,9 +278,10 @@ bool HipGISAXS::make_grains(const HigStructure& s, GlibcRandom& rng, double phi,
   if (s.distribution != "regular")
     return fail("structure '" + s.key + "': ensemble distribution '" + s.distribution +
                 "' is not supported (regular only; the reference's random placement reseeds rand() from the clock)");
+  const double ngd = s.maxgrains[0] * s.maxgrains[1] * s.maxgrains[2];
+  if (!(ngd >= 1) || ngd > 1000000) return fail("structure '" + s.key + "': grain count out of range (1 to 1e6)");
   const int nd[3] = {(int)s.maxgrains[0], (int)s.maxgrains[1], (int)s.maxgrains[2]};
   const int ng = nd[0] * nd[1] * nd[2];
-  if (ng < 1 || ng > 1000000) return fail("structure '" + s.key + "': grain count out of range");
   // grain positions: cell_ of the reference is never set (zero), so the spacing alone separates grains
   const double tz = 0.0;
   std::vector<double> dx(nd[0]), dy(nd[1]), dz(nd[2]);
