@@ -83,11 +83,13 @@ int scan_count(double mn, double mx, double step, bool alphai) {
 // ----- GlibcRandom ---------------------------------------------------------------------------------
 
 void GlibcRandom::reseed(unsigned seed) {
+  // r[0] is the seed as a signed 32-bit word, and the LCG runs on signed 32-bit words (so a seed of 2^31
+  // or more starts from its negative value, as the C library's does)
   int32_t init[34];
   init[0] = (int32_t)(seed == 0 ? 1 : seed);
   for (int i = 1; i < 31; ++i) {
     // 16807 * r mod (2^31 - 1) by Schrage's decomposition (no 32-bit overflow)
-    int32_t hi = init[i - 1] / 127773, lo = init[i - 1] % 127773;
+    const int32_t hi = init[i - 1] / 127773, lo = init[i - 1] % 127773;
     int32_t w = 16807 * lo - 2836 * hi;
     if (w < 0) w += 2147483647;
     init[i] = w;

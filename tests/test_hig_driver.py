@@ -59,7 +59,7 @@ def _grains_oracle(text, key, seed):
 @pytest.mark.parametrize("stat,ax3,sp", [("range", "y", "0 0 0"), ("random", "z", "0 0 0"),
                                          ("gaussian", "n", "40 30 20"), ("normal", "z", "0 0 0"),
                                          ("cauchy", "y", "5 5 0"), ("single", "x", "10 0 5")])
-@pytest.mark.parametrize("seed", [1, 2, 987654])
+@pytest.mark.parametrize("seed", [1, 2, 987654, 3000000000])
 def test_grains_match_oracle_and_libc_rand(stat, ax3, sp, seed):
     """Orientation sampling, grain centres and weights (spatial_distribution / orientation_distribution,
     src/sim/hipgisaxs_main.cpp:1702-2086) equal the oracle's, whose rand() is the C library's own."""
