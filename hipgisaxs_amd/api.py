@@ -325,6 +325,18 @@ def gaussian_smear(ctx: Context, image, ncol: int, nrow: int, sigma: float, comp
     return image
 
 
+def image_distance(ctx: Context, I, O, mask=None, metric="sqrt_unit_l2") -> float:
+    """hgx_image_distance of device images I and O (float64 tensors of one size), optional mask."""
+    if I.numel() != O.numel() or (mask is not None and mask.numel() != I.numel()):
+        raise ValueError("image_distance: I, O and mask must have one size")
+    m = N.METRICS[metric] if isinstance(metric, str) else int(metric)
+    out = C.c_double()
+    ctx._check(ctx.lib.hgx_image_distance(ctx.h, I.numel(), m, C.c_void_p(I.data_ptr()), C.c_void_p(O.data_ptr()),
+                                          None if mask is None else C.c_void_p(mask.data_ptr()), C.byref(out)),
+               "image_distance")
+    return out.value
+
+
 class Batch:
     """hgx_batch_create / hgx_batch_run: items are (grid_context_or_None, mesh_id, rot or None, out tensor)."""
 

@@ -72,7 +72,7 @@ def build_peaks() -> Path:
     return PEAKS_BIN
 
 
-HIG_SRCS = [PKG / "host" / n for n in ("hig_gpu.cpp", "hig_input.cpp", "hig_sim.cpp")]
+HIG_SRCS = [PKG / "host" / n for n in ("hig_gpu.cpp", "hig_input.cpp", "hig_sim.cpp", "hig_fit.cpp")]
 HIG_LIB = PKG / "libhgx_hig.so"
 SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
 SHIM_LIB = ROOT / "tests" / "cpp" / "libhig_shim.so"

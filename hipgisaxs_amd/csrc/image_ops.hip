@@ -1,7 +1,9 @@
-// image_ops.hip -- element-wise accumulation of form factors and intensities, and the separable
-// Gaussian smearing of a detector image (the host loop around the hot path, run on the device so
-// the image never leaves HBM between grains, structures and the final smearing).
+// image_ops.hip -- element-wise accumulation of form factors and intensities, the separable
+// Gaussian smearing of a detector image, and the distance of an image to reference data (the host
+// loop around the hot path, run on the device so the image never leaves HBM between grains,
+// structures, the final smearing and a fit's objective).
 #include <cmath>
+#include <algorithm>
 #include <cfloat>
 #include <vector>
 
@@ -93,6 +95,144 @@ extern "C" int hgx_axpy(hgx_ctx* h, int64_t n, double a, const double* x, int ac
   return HGX_OK;
 }
 
+// ----- distance of a model image to reference data ------------------------------------------------
+
+namespace hgx {
+
+// Pixel transform of the normalised functors: square root or cube root (ROOT 2 or 3).
+template <int ROOT>
+__device__ inline double root_of(double x) {
+  return ROOT == 2 ? sqrt(x) : cbrt(x);
+}
+
+__device__ inline double nan0(double x) { return isnan(x) ? 0.0 : x; }
+
+constexpr int kDistWG = 256;
+
+// Sums [nb][K] of one block: the K values of each thread reduced over the block in a fixed tree.
+template <int K>
+__device__ inline void block_store(double (&v)[K], double* __restrict__ out) {
+  __shared__ double red[K][kDistWG];
+#pragma unroll
+  for (int k = 0; k < K; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = kDistWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[(int64_t)blockIdx.x * K + k] = red[k][0];
+  }
+}
+
+// Pass 1 of the normalised functors: per block, norm_l2^2 of the transformed model image, vec_dot of
+// the transformed images and norm_l2^2 of the transformed data, NaN terms dropped
+// (include/analyzer/distance_functions.hpp:40-57).
+template <int ROOT>
+__global__ __launch_bounds__(kDistWG) void k_distance_stats(int64_t n, const double* __restrict__ I,
+                                                            const double* __restrict__ O,
+                                                            const double* __restrict__ m,
+                                                            double* __restrict__ part) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t p = (int64_t)blockIdx.x * kDistWG + threadIdx.x; p < n; p += (int64_t)gridDim.x * kDistWG) {
+    const double w = m ? m[p] : 1.0;
+    const double u = root_of<ROOT>(I[p]), r = root_of<ROOT>(O[p]);
+    v[0] += w * nan0(u * u);
+    v[1] += w * nan0(fabs(u * r));
+    v[2] += w * nan0(r * r);
+  }
+  block_store<3>(v, part);
+}
+
+// Sums the [nb][K] block partials in a fixed order; for the normalised functors turns them into the
+// constants of pass 2: (dat_norm, ref_norm, c = dot / dat_norm^2).
+template <int K>
+__global__ __launch_bounds__(kDistWG) void k_distance_finish(int nb, bool consts, const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kDistWG) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += part[(int64_t)b * K + k];
+  }
+  __shared__ double red[K][kDistWG];
+#pragma unroll
+  for (int k = 0; k < K; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = kDistWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (consts) {
+      const double ns = sqrt(red[0][0]), nr = sqrt(red[K - 1][0]);
+      out[0] = ns;
+      out[1] = nr;
+      out[2] = red[1][0] / (ns * ns);
+    } else {
+      out[0] = red[0][0];
+    }
+  }
+}
+
+// Pass 2: the per-pixel terms of each functor summed per block. chi^2: m (I - O)^2 / max(O, 1);
+// AbsoluteDifferenceSquare (:125-143): (m |O - I|)^2; unit-vector forms (:423-456, :543-575):
+// (m |r/|r| - u/|u||)^2; c-norm forms (:460-499, :579-616): (m |r - c u|)^2, with u and r the
+// transformed model and data. The square-root forms and L2 drop non-finite terms, the cube-root forms
+// keep them, as the reference's loops do.
+template <int METRIC>
+__global__ __launch_bounds__(kDistWG) void k_distance_terms(int64_t n, const double* __restrict__ I,
+                                                            const double* __restrict__ O,
+                                                            const double* __restrict__ m,
+                                                            const double* __restrict__ cst,
+                                                            double* __restrict__ part) {
+  double v[1] = {0.0};
+  double ns = 1.0, nr = 1.0, cc = 0.0;
+  if (METRIC >= HGX_METRIC_SQRT_UNIT_L2) {
+    ns = cst[0];
+    nr = cst[1];
+    cc = cst[2];
+  }
+  for (int64_t p = (int64_t)blockIdx.x * kDistWG + threadIdx.x; p < n; p += (int64_t)gridDim.x * kDistWG) {
+    const double w = m ? m[p] : 1.0;
+    const double a = I[p], b = O[p];
+    if (METRIC == HGX_METRIC_CHI2) {
+      v[0] += w * (a - b) * (a - b) / fmax(b, 1.0);
+    } else if (METRIC == HGX_METRIC_L2) {
+      double t = w * fabs(b - a);
+      t = isfinite(t) ? t : 0.0;
+      v[0] += t * t;
+    } else if (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_CBRT_UNIT_L2) {
+      constexpr int R = METRIC == HGX_METRIC_SQRT_UNIT_L2 ? 2 : 3;
+      double t = w * fabs(root_of<R>(b) / nr - root_of<R>(a) / ns);
+      if (R == 2) t = isfinite(t) ? t : 0.0;
+      v[0] += t * t;
+    } else {
+      constexpr int R = METRIC == HGX_METRIC_SQRT_C_L2 ? 2 : 3;
+      double t = w * fabs(root_of<R>(b) - cc * root_of<R>(a));
+      if (R == 2) t = isfinite(t) ? t : 0.0;
+      v[0] += t * t;
+    }
+  }
+  block_store<1>(v, part);
+}
+
+template <int METRIC>
+void launch_terms(hipStream_t s, unsigned nb, int64_t n, const double* I, const double* O, const double* m,
+                  const double* cst, double* part) {
+  k_distance_terms<METRIC><<<nb, kDistWG, 0, s>>>(n, I, O, m, cst, part);
+}
+
+}  // namespace hgx
+
 extern "C" int hgx_gaussian_smear(hgx_ctx* h, int ncol, int nrow, double sigma, int flags, double* I) {
   Context* c = reinterpret_cast<Context*>(h);
   if (!c) return HGX_ERR_BAD_HANDLE;
@@ -131,5 +271,50 @@ extern "C" int hgx_gaussian_smear(hgx_ctx* h, int ncol, int nrow, double sigma, 
   }
   // the host table must outlive the asynchronous copy
   HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return HGX_OK;
+}
+
+extern "C" int hgx_image_distance(hgx_ctx* h, int64_t n, int metric, const double* I, const double* O,
+                                  const double* mask, double* distance) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (n < 1 || !I || !O || !distance) return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: bad arguments");
+  if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2)
+    return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: unknown metric %d", metric);
+  if (int rc = set_device(c)) return rc;
+  // a fixed grid per size keeps the summation order, and so the value, reproducible
+  const unsigned nb = (unsigned)std::min<int64_t>((n + kDistWG - 1) / kDistWG, 1024);
+  const size_t part_b = ((size_t)nb * 3 * sizeof(double) + 255) / 256 * 256;
+  if (int rc = ensure_work(c, part_b + 256)) return rc;
+  double* part = static_cast<double*>(c->d_work);
+  double* cst = reinterpret_cast<double*>(static_cast<char*>(c->d_work) + part_b);
+  hipStream_t s = c->stream;
+  if (metric >= HGX_METRIC_SQRT_UNIT_L2) {
+    const bool cube = metric == HGX_METRIC_CBRT_UNIT_L2 || metric == HGX_METRIC_CBRT_C_L2;
+    {
+      LaunchTimer lt(c, false);
+      if (cube) k_distance_stats<3><<<nb, kDistWG, 0, s>>>(n, I, O, mask, part);
+      else k_distance_stats<2><<<nb, kDistWG, 0, s>>>(n, I, O, mask, part);
+      HGX_CHECK_LAUNCH(c);
+    }
+    k_distance_finish<3><<<1, kDistWG, 0, s>>>((int)nb, true, part, cst);
+    HGX_CHECK_LAUNCH(c);
+  }
+  {
+    LaunchTimer lt(c, false);
+    switch (metric) {
+      case HGX_METRIC_CHI2: launch_terms<HGX_METRIC_CHI2>(s, nb, n, I, O, mask, cst, part); break;
+      case HGX_METRIC_L2: launch_terms<HGX_METRIC_L2>(s, nb, n, I, O, mask, cst, part); break;
+      case HGX_METRIC_SQRT_UNIT_L2: launch_terms<HGX_METRIC_SQRT_UNIT_L2>(s, nb, n, I, O, mask, cst, part); break;
+      case HGX_METRIC_SQRT_C_L2: launch_terms<HGX_METRIC_SQRT_C_L2>(s, nb, n, I, O, mask, cst, part); break;
+      case HGX_METRIC_CBRT_UNIT_L2: launch_terms<HGX_METRIC_CBRT_UNIT_L2>(s, nb, n, I, O, mask, cst, part); break;
+      default: launch_terms<HGX_METRIC_CBRT_C_L2>(s, nb, n, I, O, mask, cst, part); break;
+    }
+    HGX_CHECK_LAUNCH(c);
+  }
+  k_distance_finish<1><<<1, kDistWG, 0, s>>>((int)nb, false, part, cst + 3);
+  HGX_CHECK_LAUNCH(c);
+  HGX_CHECK_HIP(c, hipMemcpyAsync(distance, cst + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(s));
   return HGX_OK;
 }

@@ -36,6 +36,14 @@ _SIGS = {
     "hgx_hig_kernel_ms": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)]),
     "hgx_hig_save": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
     "hgx_hig_read_mesh": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_int64), C.c_int64, _vp]),
+    "hgx_hig_update_param": (C.c_int, [_vp, C.c_char_p, C.c_double]),
+    "hgx_hig_fit_info": (C.c_int, [_vp, _ip, _ip]),
+    "hgx_hig_fit_param": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_size_t, _dp]),
+    "hgx_hig_fit_algorithm": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_size_t, _ip, _dp]),
+    "hgx_hig_fit_init": (C.c_int, [_vp]),
+    "hgx_hig_fit_objective": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_double, _dp]),
+    "hgx_hig_fit_run": (C.c_int, [_vp, C.c_int, C.c_ulonglong, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "hgx_hig_fit_history": (C.c_int, [_vp, C.c_longlong, _dp, C.POINTER(C.c_longlong)]),
     "hgx_hig_last_error": (C.c_char_p, [_vp]),
 }
 _lib = None
@@ -166,6 +174,70 @@ class HigSimulation:
         nm, na = C.c_int64(), C.c_int64()
         self._check(self.lib.hgx_hig_kernel_ms(self.h, C.byref(m), C.byref(nm), C.byref(a), C.byref(na)), "kernel_ms")
         return {"main_ms": m.value, "main_launches": nm.value, "aux_ms": a.value, "aux_launches": na.value}
+
+    # ---- fitting (the analyzer's HipGISAXSObjectiveFunction and fit drivers) ----------------------
+
+    def update_param(self, variable: str, value: float) -> None:
+        """HiGInput::update_params for one fit variable path (host only)."""
+        self._check(self.lib.hgx_hig_update_param(self.h, variable.encode(), float(value)), "update_param")
+
+    def fit_params(self) -> list[dict]:
+        """The fitparam blocks in fit-vector order (by key): key, min, max, step (-1: none), init."""
+        n, na = C.c_int(), C.c_int()
+        self._check(self.lib.hgx_hig_fit_info(self.h, C.byref(n), C.byref(na)), "fit_info")
+        out = []
+        for i in range(n.value):
+            key = C.create_string_buffer(256)
+            r = (C.c_double * 4)()
+            self._check(self.lib.hgx_hig_fit_param(self.h, i, key, 256, r), "fit_param")
+            out.append({"key": key.value.decode(), "min": r[0], "max": r[1], "step": r[2], "init": r[3]})
+        return out
+
+    def fit_algorithms(self) -> list[dict]:
+        """The algorithm blocks in file order: name, metric (HGX_METRIC_* id, -1 unsupported), tolerance,
+        regularization."""
+        n, na = C.c_int(), C.c_int()
+        self._check(self.lib.hgx_hig_fit_info(self.h, C.byref(n), C.byref(na)), "fit_info")
+        out = []
+        for i in range(na.value):
+            name = C.create_string_buffer(64)
+            m = C.c_int()
+            tr = (C.c_double * 2)()
+            self._check(self.lib.hgx_hig_fit_algorithm(self.h, i, name, 64, C.byref(m), tr), "fit_algorithm")
+            out.append({"name": name.value.decode(), "metric": m.value, "tolerance": tr[0], "regularization": tr[1]})
+        return out
+
+    def fit_init(self) -> None:
+        self._check(self.lib.hgx_hig_fit_init(self.h), "fit_init")
+
+    def objective(self, x, metric="sqrt_unit_l2", reg: float = 0.0) -> float:
+        """HipGISAXSObjectiveFunction::operator(): distance of the image at fit vector x to the reference data."""
+        xv = np.ascontiguousarray(x, dtype=np.float64)
+        m = N.METRICS[metric] if isinstance(metric, str) else int(metric)
+        v = C.c_double()
+        self._check(self.lib.hgx_hig_fit_objective(self.h, xv.size, xv.ctypes.data_as(_dp), m, float(reg),
+                                                   C.byref(v)), "fit_objective")
+        return v.value
+
+    def fit(self, algorithm: int = 0, seed: int = 0) -> dict:
+        """Runs algorithm block `algorithm`; returns the best x, its objective and the evaluation count."""
+        n = len(self.fit_params())
+        x = np.zeros(n)
+        v = C.c_double()
+        ne = C.c_longlong()
+        self._check(self.lib.hgx_hig_fit_run(self.h, int(algorithm), int(seed), x.ctypes.data_as(_dp), C.byref(v),
+                                             C.byref(ne)), "fit_run")
+        return {"x": x, "value": v.value, "evals": ne.value}
+
+    def fit_history(self) -> np.ndarray:
+        """Every objective evaluation since the last fit or fit_init: rows (x..., value)."""
+        nr = C.c_longlong()
+        self._check(self.lib.hgx_hig_fit_history(self.h, 0, None, C.byref(nr)), "fit_history")
+        w = len(self.fit_params()) + 1
+        rows = np.zeros((nr.value, w))
+        self._check(self.lib.hgx_hig_fit_history(self.h, nr.value, rows.ctypes.data_as(_dp), C.byref(nr)),
+                    "fit_history")
+        return rows
 
     @staticmethod
     def save(image: np.ndarray, path: str) -> None:

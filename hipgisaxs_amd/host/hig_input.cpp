@@ -289,7 +289,7 @@ bool HigInput::read_root(const HigNode& root) {
     else if (k == "structure") ok = read_structure(f.second);
     else if (k == "instrumentation") ok = read_instrumentation(f.second);
     else if (k == "computation") ok = read_computation(f.second);
-    else if (k == "fitting") ok = true;  // the fitting blocks drive the analyzer, not a simulation
+    else if (k == "fitting") ok = read_fitting(f.second);
     else return fail(f.second, "unknown keyword '" + k + "'");
     if (!ok) return false;
   }
@@ -700,6 +700,119 @@ bool HigInput::read_computation(const HigNode& n) {
       if (!string(v, unused)) return false;
     } else {
       return fail(v, "unknown keyword '" + k + "' in computation");
+    }
+  }
+  return true;
+}
+
+// fitting = { fitparam = {...} ... referencedata = {...} algorithm = {...} ... }: parameters are kept
+// by key (the reference's maps order the fit vector by key); a duplicate key or a parameter without a
+// key or variable is an error (src/config/hig_input.cpp:307-321).
+bool HigInput::read_fitting(const HigNode& n) {
+  if (n.kind != HigNode::kObject) return fail(n, "fitting must be an object");
+  for (const auto& f : n.fields) {
+    const std::string& k = f.first;
+    const HigNode& v = f.second;
+    if (v.kind != HigNode::kObject) return fail(v, k + " must be an object");
+    if (k == "fitparam") {
+      HigFitParam p;
+      for (const auto& g : v.fields) {
+        if (g.first == "key") {
+          if (!string(g.second, p.key)) return false;
+        } else if (g.first == "variable") {
+          if (!string(g.second, p.variable)) return false;
+        } else if (g.first == "init") {
+          if (!number(g.second, p.init)) return false;
+        } else if (g.first == "range") {
+          if (g.second.kind != HigNode::kObject) return fail(g.second, "range must be an object");
+          for (const auto& r : g.second.fields) {
+            double* dst = r.first == "min" ? &p.min : r.first == "max" ? &p.max : r.first == "step" ? &p.step : nullptr;
+            if (!dst) return fail(r.second, "unknown keyword '" + r.first + "' in range");
+            if (!number(r.second, *dst)) return false;
+          }
+        } else {
+          return fail(g.second, "unknown keyword '" + g.first + "' in fitparam");
+        }
+      }
+      if (p.key.empty() || p.variable.empty()) return fail(v, "incomplete fit parameter definition");
+      if (fit_params.count(p.key)) return fail(v, "duplicate key '" + p.key + "' in fit parameters");
+      fit_params[p.key] = p;
+    } else if (k == "referencedata") {
+      HigFitReference& r = reference;
+      for (const auto& g : v.fields) {
+        if (g.first == "path" || g.first == "mask") {
+          std::string s;
+          if (!string(g.second, s)) return false;
+          if (!s.empty() && s[0] != '/') s = base_dir_ + "/" + s;
+          (g.first == "path" ? r.path : r.mask) = s;
+        } else if (g.first == "fitregion") {
+          if (g.second.kind != HigNode::kObject) return fail(g.second, "fitregion must be an object");
+          for (const auto& q : g.second.fields) {
+            if (q.first == "type") {
+              if (!string(q.second, r.region)) return false;
+            } else if (q.first == "regmin") {
+              if (!vector(q.second, 2, r.regmin)) return false;
+            } else if (q.first == "regmax") {
+              if (!vector(q.second, 2, r.regmax)) return false;
+            } else {
+              return fail(q.second, "unknown keyword '" + q.first + "' in fitregion");
+            }
+          }
+        } else if (g.first == "npoints") {
+          if (g.second.kind != HigNode::kObject) return fail(g.second, "npoints must be an object");
+          for (const auto& q : g.second.fields) {
+            double x;
+            if (q.first != "parallel" && q.first != "perpendicular")
+              return fail(q.second, "unknown keyword '" + q.first + "' in npoints");
+            if (!number(q.second, x)) return false;
+            r.npoints[q.first == "parallel" ? 0 : 1] = (int)x;
+          }
+        } else {
+          return fail(g.second, "unknown keyword '" + g.first + "' in referencedata");
+        }
+      }
+      reference_set = true;
+    } else if (k == "algorithm") {
+      HigFitAlgorithm a;
+      for (const auto& g : v.fields) {
+        double x;
+        if (g.first == "algoname") {
+          if (!string(g.second, a.name)) return false;
+        } else if (g.first == "algoorder") {
+          if (!number(g.second, x)) return false;
+          a.order = (int)x;
+        } else if (g.first == "tolerance") {
+          if (!number(g.second, a.tolerance)) return false;
+        } else if (g.first == "regularization") {
+          if (!number(g.second, a.regularization)) return false;
+        } else if (g.first == "distancemetric") {
+          if (!string(g.second, a.metric)) return false;
+        } else if (g.first == "restart") {
+          std::string b;
+          if (!string(g.second, b)) return false;
+          a.restart = b == "true" || b == "True" || b == "TRUE" || b == "yes" || b == "Yes" || b == "YES";
+        } else if (g.first == "algoparam") {
+          if (g.second.kind != HigNode::kObject) return fail(g.second, "algoparam must be an object");
+          std::string type;
+          double value = 0.0;
+          for (const auto& q : g.second.fields) {
+            if (q.first == "type") {
+              if (!string(q.second, type)) return false;
+            } else if (q.first == "pvalue") {
+              if (!number(q.second, value)) return false;
+            } else {
+              return fail(q.second, "unknown keyword '" + q.first + "' in algoparam");
+            }
+          }
+          if (type.empty()) return fail(g.second, "algoparam without a type");
+          a.params[type] = value;
+        } else {
+          return fail(g.second, "unknown keyword '" + g.first + "' in algorithm");
+        }
+      }
+      algorithms.push_back(a);
+    } else {
+      return fail(v, "unknown keyword '" + k + "' in fitting");
     }
   }
   return true;

@@ -128,6 +128,29 @@ struct HigDetector {
   std::string origin = "bottom-left";
 };
 
+// The fitting block (src/config/hig_input.cpp:302-340, 849-890, 1444-1477, 1524-1571, 1755-1777;
+// include/model/fitting_params.hpp, include/config/temp_helpers.hpp).
+struct HigFitParam {
+  std::string key, variable;  // variable: a path such as shape['s']:param['radius']:p1
+  double min = 0.0, max = 0.0, step = -1.0, init = 0.0;
+};
+
+struct HigFitReference {
+  std::string path, mask;
+  std::string region = "qspace";
+  double regmin[2] = {0.0, 0.0}, regmax[2] = {0.0, 0.0};
+  int npoints[2] = {0, 0};  // parallel, perpendicular
+};
+
+struct HigFitAlgorithm {
+  std::string name;             // lmvm, pounders, pso, bruteforce, none_pounders
+  int order = 0;
+  double tolerance = 0.0, regularization = 0.0;
+  std::string metric;           // distancemetric
+  bool restart = false;
+  std::map<std::string, double> params;  // algoparam type -> pvalue
+};
+
 class HigInput {
  public:
   // Reads a .hig file; mesh file names of custom shapes are resolved against its directory.
@@ -143,6 +166,15 @@ class HigInput {
   HigScattering scattering;
   HigCompute compute;
   HigDetector detector;
+  // fitting: parameters by key (the order of the fit vector), reference data, algorithms in file order
+  std::map<std::string, HigFitParam> fit_params;
+  HigFitReference reference;
+  bool reference_set = false;
+  std::vector<HigFitAlgorithm> algorithms;
+
+  // HiGInput::update_params for one parameter (src/config/hig_input.cpp:2712-2832): sets the model
+  // value that `variable` names. False, with the reason in error(), for a malformed or unknown path.
+  bool update_param(const std::string& variable, double value);
 
  private:
   bool fail(const HigNode& at, const std::string& msg);
@@ -155,6 +187,7 @@ class HigInput {
   bool read_ensemble(const HigNode& n, HigStructure& s);
   bool read_instrumentation(const HigNode& n);
   bool read_computation(const HigNode& n);
+  bool read_fitting(const HigNode& n);
   bool number(const HigNode& n, double& out);
   bool vector(const HigNode& n, size_t count, double* out);
   bool string(const HigNode& n, std::string& out);
@@ -168,6 +201,10 @@ class HigInput {
 // the last vertex) into 9 doubles per triangle (v1, v2, v3). Polygons of n > 3 vertices are split
 // into the fan (v0, v_k, v_k+1); the reference keeps triangles only and drops the rest (*polygons
 // counts the faces split).
+// Lattice::construct_vectors: the unit-scaled vectors of the structure's lattice (its a, b, c when
+// the file gives them).
+bool lattice_vectors(const HigStructure& s, vec3& a, vec3& b, vec3& c, std::string& err);
+
 bool read_obj_mesh(const std::string& path, std::vector<double>& tri9, std::string& err, int* polygons = nullptr);
 // Raw shape files (RawShapeReader, src/file/rawshape_reader.cpp:39-62): 7 numbers per triangle
 // (area, nx, ny, nz, cx, cy, cz) into 8-double approximate-method records (pad 0).

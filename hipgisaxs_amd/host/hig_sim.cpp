@@ -26,6 +26,22 @@ double cauchy_weight(double x, double l, double s) {
   return (x - l > DBL_MIN) ? 0.0 : 1.0;
 }
 
+bool same9(const double* a, const double* b) { return std::memcmp(a, b, 9 * sizeof(double)) == 0; }
+
+// angle count of a scanned range, as run_all_gisaxs counts it (src/sim/hipgisaxs_main.cpp:298-312)
+int scan_count(double mn, double mx, double step, bool alphai) {
+  if (alphai) {
+    if (mx < mn) mx = mn;
+    if (mn == mx || step == 0) return 1;
+  } else if (step == 0) {
+    return 1;
+  }
+  int n = (int)((mx - mn) / step + 1);
+  return n < 1 ? 1 : n;
+}
+
+}  // namespace
+
 // Lattice::construct_vectors (src/model/structure.cpp:86-182): unit-scaled vectors of a lattice
 // type and hkl; fco uses the class's ca_ = 1 and gamma_ = 0 (the file's caratio and abangle are
 // never passed to it). The basis vector t is left out, as the CPU structure factor does.
@@ -63,22 +79,6 @@ bool lattice_vectors(const HigStructure& s, vec3& a, vec3& b, vec3& c, std::stri
   }
   return true;
 }
-
-bool same9(const double* a, const double* b) { return std::memcmp(a, b, 9 * sizeof(double)) == 0; }
-
-// angle count of a scanned range, as run_all_gisaxs counts it (src/sim/hipgisaxs_main.cpp:298-312)
-int scan_count(double mn, double mx, double step, bool alphai) {
-  if (alphai) {
-    if (mx < mn) mx = mn;
-    if (mn == mx || step == 0) return 1;
-  } else if (step == 0) {
-    return 1;
-  }
-  int n = (int)((mx - mn) / step + 1);
-  return n < 1 ? 1 : n;
-}
-
-}  // namespace
 
 // ----- GlibcRandom ---------------------------------------------------------------------------------
 
@@ -166,7 +166,8 @@ void HipGISAXS::options(int precision, int flags, unsigned seed) {
 }
 
 bool HipGISAXS::construct_input(const std::string& path) {
-  loaded_ = inited_ = false;
+  loaded_ = inited_ = fit_ready_ = false;
+  history_.clear();
   if (gpu_ && gpu_->ok())
     for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
   meshes_.clear();
@@ -177,7 +178,8 @@ bool HipGISAXS::construct_input(const std::string& path) {
 }
 
 bool HipGISAXS::construct_input_text(const std::string& text, const std::string& base_dir) {
-  loaded_ = inited_ = false;
+  loaded_ = inited_ = fit_ready_ = false;
+  history_.clear();
   if (gpu_ && gpu_->ok())
     for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
   meshes_.clear();
@@ -663,9 +665,22 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
 
 bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, std::vector<double>& img) {
   (void)tilt_deg;  // the reference's run_gisaxs reads the tilt angle nowhere
+  auto t0 = std::chrono::steady_clock::now();
+  const double* d_total = nullptr;
+  if (!run_device(alpha_i_deg, phi_deg, &d_total)) return false;
+  img.assign((size_t)npix_, 0.0);
+  if (d_total && !hgx(hgx_memcpy_d2h(gpu_->get(), img.data(), d_total, (size_t)npix_ * 8), "hgx_memcpy_d2h"))
+    return false;
+  last_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
+
+// The image of one configuration left in device buffer 0; *d_img is null for an input without
+// structures (a zero image).
+bool HipGISAXS::run_device(double alpha_i_deg, double phi_deg, const double** d_img) {
+  *d_img = nullptr;
   if (!loaded_) return fail("no input");
   if (!inited_ && !init()) return false;
-  auto t0 = std::chrono::steady_clock::now();
   for (auto& c : counts_) c = 0;
   const HigCompute& cp = in_.compute;
   if (cp.nslices > 1) return fail("the slicing scheme (nslices > 1) is not implemented by the reference either");
@@ -680,7 +695,6 @@ bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, 
   }
   const double alphai = alpha_i_deg * PI_ / 180;
   const double phi = phi_deg * PI_ / 180;
-  img.assign((size_t)npix_, 0.0);
   if (in_.structures.empty()) return true;
   // iratio normalisation (src/sim/hipgisaxs_main.cpp:1268-1282)
   std::vector<double> iratio;
@@ -713,8 +727,7 @@ bool HipGISAXS::run_gisaxs(double alpha_i_deg, double phi_deg, double tilt_deg, 
   }
   if (prof && !hgx(hgx_profile_end(gpu_->get(), &kms_[0], &kn_[0], &kms_[1], &kn_[1]), "hgx_profile_end"))
     return false;
-  if (!hgx(hgx_memcpy_d2h(gpu_->get(), img.data(), d_total, (size_t)npix_ * 8), "hgx_memcpy_d2h")) return false;
-  last_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *d_img = d_total;
   return true;
 }
 
@@ -737,12 +750,6 @@ bool HipGISAXS::save_gisaxs(const double* data, unsigned ncol, unsigned nrow, co
 }  // namespace hig
 
 // ----- C ABI ---------------------------------------------------------------------------------------
-
-struct hgx_hig {
-  hig::HipGISAXS sim;
-  std::string err;
-  explicit hgx_hig(int device) : sim(device) {}
-};
 
 namespace {
 

@@ -68,6 +68,24 @@ class HipGISAXS {
   bool grains(const std::string& key, std::vector<HigGrain>& out);
   static bool save_gisaxs(const double* data, unsigned ncol, unsigned nrow, const std::string& path);
 
+  // ---- fitting: the HipGISAXSObjectiveFunction of the reference's analyzer
+  // (src/analyzer/objective_func_hipgisaxs.cpp:47-72, 110-170, 245-332) on the device.
+  // Sets the model value a fit variable path names (HiGInput::update_params); a change of an
+  // instrument value re-derives k0 and the q-grid at the next run.
+  bool update_param(const std::string& variable, double value);
+  // Reads the input's reference data (ASCII rows, ImageData::read) and mask (whitespace-separated
+  // weights; none: all 1), overrides the output region with the reference region when the sizes
+  // differ (override_qregion), and keeps both on the device.
+  bool fit_init();
+  bool fit_ready() const { return fit_ready_; }
+  // The fit vector's keys (sorted, as the reference's maps order them), limits, steps and initial values.
+  std::vector<std::string> fit_keys() const;
+  // The objective of x [nkeys]: update every parameter, compute_gisaxs on the device, the distance
+  // under `metric` (HGX_METRIC_*) to the reference data, plus (reg/2) |x - range midpoints|^2; a
+  // non-finite distance becomes 1e8. Appends (x, value) to the evaluation history.
+  bool fit_objective(const double* x, int metric, double reg, double* value);
+  std::vector<std::vector<double>>& fit_history() { return history_; }
+
   unsigned nrows() const { return nrow_; }
   unsigned ncols() const { return ncol_; }
   double k0() const { return k0_; }
@@ -91,6 +109,7 @@ class HipGISAXS {
   bool device();  // creates the device context on first use
   void release_device_objects();
   bool ensure_grid(double alphai_rad, double dnl_re, double dnl_im);
+  bool run_device(double alpha_i_deg, double phi_deg, const double** d_img);
   bool make_grains(const HigStructure& s, GlibcRandom& rng, double phi_rad, std::vector<HigGrain>& out);
   bool structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains, double* d_I);
   bool form_factor(const HigShape& shape, const RotMatrix_t& rot, const vec3& loc, double* d_out);
@@ -115,6 +134,9 @@ class HipGISAXS {
   void* buf_[8] = {};
   size_t buf_bytes_[8] = {};
   std::map<std::string, int> meshes_;
+  bool fit_ready_ = false;
+  int64_t fit_npix_ = 0;
+  std::vector<std::vector<double>> history_;
   int64_t counts_[4] = {0, 0, 0, 0};
   double last_ms_ = 0.0;
   double kms_[2] = {0.0, 0.0};
@@ -123,3 +145,10 @@ class HipGISAXS {
 };
 
 }  // namespace hig
+
+// The handle behind the C ABI of include/hgx_hig.h.
+struct hgx_hig {
+  hig::HipGISAXS sim;
+  std::string err;
+  explicit hgx_hig(int device) : sim(device) {}
+};

@@ -342,6 +342,10 @@ int hgx_chi2_bwd(hgx_ctx* ctx, int plan, const double theta[6], int precision, d
                                      sum (m |sqrt O/||sqrt O|| - sqrt I/||sqrt I|| |)^2, ||x||^2 = sum m x^2 */
 #define HGX_METRIC_SQRT_C_L2 3    /* "sqrt_c_l2", SqrtCNormL2DistanceSquare (:460-499):
                                      sum (m |sqrt O - c sqrt I|)^2, c = <sqrt I, sqrt O>_m / ||sqrt I||^2 */
+#define HGX_METRIC_CBRT_UNIT_L2 4 /* "cbrt_unit_l2", CbrtUnitVectorNormL2DistanceSquare (:543-575): the
+                                     unit-vector form over cube roots (hgx_image_distance only) */
+#define HGX_METRIC_CBRT_C_L2 5    /* "cbrt_c_l2", CbrtCNormL2DistanceSquare (:579-616): the c-norm form
+                                     over cube roots (hgx_image_distance only) */
 int hgx_fit_plan_metric(hgx_ctx* ctx, int plan, int metric);
 /* Objective values of n parameter vectors (host [n][6]) into out[n] (host): the population evaluation
  * of the reference's particle-swarm fit (src/analyzer/hipgisaxs_fit_pso.cpp:521-543), queued back to back
@@ -421,6 +425,15 @@ int hgx_axpy(hgx_ctx* ctx, int64_t n, double a, const double* x, int accumulate,
  * default the weight follows the distance |k - i| over [i - h, i + h]; HGX_SMEAR_COMPAT keeps the
  * reference's window. Synchronises the stream. */
 int hgx_gaussian_smear(hgx_ctx* ctx, int ncol, int nrow, double sigma, int flags, double* I);
+
+/* Distance of a model image I to reference data O, both device [n], with an optional device mask
+ * [n] of pixel weights (NULL: all 1), under one of the HGX_METRIC_* functors: the scalar that the
+ * reference's DistanceMeasure objects compute in HipGISAXSObjectiveFunction::operator()
+ * (src/analyzer/objective_func_hipgisaxs.cpp:284-287; include/analyzer/distance_functions.hpp). Two
+ * passes over the images for the normalised forms (the norms, then the terms), one for chi^2 and L2;
+ * block partials summed in a fixed order. Writes the host scalar *distance; synchronises the stream. */
+int hgx_image_distance(hgx_ctx* ctx, int64_t n, int metric, const double* I, const double* O,
+                       const double* mask, double* distance);
 
 /* ---- host helpers ------------------------------------------------------------------------- */
 

@@ -70,6 +70,40 @@ int hgx_hig_save(const double* image, int ncol, int nrow, const char* path);
  * (RawShapeReader, src/file/rawshape_reader.cpp:39-62). Writes *kind and *ntri, and the records
  * into `records` when cap >= *ntri (pass cap = 0 to query the size). */
 int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records);
+/* ---- fitting (the analyzer's objective and fit drivers on the device) ------------------------- */
+
+/* HiGInput::update_params for one value (src/config/hig_input.cpp:2712-2832): the model value a fit
+ * variable path names, e.g. shape['s']:param['radius']:p1, layer['film']:refindex:delta,
+ * structure['st']:grain:lattice:a:x, instrumentation:scattering:alphai:min. Host only. */
+int hgx_hig_update_param(hgx_hig* h, const char* variable, double value);
+/* Number of fit parameters (fitparam blocks, ordered by key as the reference's maps order them) and
+ * of algorithm blocks. Host only. */
+int hgx_hig_fit_info(hgx_hig* h, int* nparams, int* nalgorithms);
+/* Fit parameter i: its key (NUL-terminated, truncated to key_cap) and range4 = (min, max, step, init);
+ * a step of -1 means none was given. Host only. */
+int hgx_hig_fit_param(hgx_hig* h, int i, char* key, size_t key_cap, double* range4);
+/* Algorithm block i: its algoname, the HGX_METRIC_* of its distancemetric (-1 when not supported) and
+ * tol_reg = (tolerance, regularization). Host only. */
+int hgx_hig_fit_algorithm(hgx_hig* h, int i, char* name, size_t name_cap, int* metric, double* tol_reg);
+/* HipGISAXSObjectiveFunction::set_reference_data (src/analyzer/objective_func_hipgisaxs.cpp:110-170):
+ * reads the reference data (ASCII rows) and mask, takes the reference region and size as the output
+ * region when the sizes differ (HipGISAXS::override_qregion), and keeps both on the device. */
+int hgx_hig_fit_init(hgx_hig* h);
+/* HipGISAXSObjectiveFunction::operator() (:245-332): sets the n fit parameters to x, computes the
+ * image on the device and its distance under `metric` (HGX_METRIC_*) to the reference data, plus
+ * (reg / 2) |x - range midpoints|^2; a non-finite distance gives 1e8. Runs hgx_hig_fit_init first
+ * when it has not run. */
+int hgx_hig_fit_objective(hgx_hig* h, int n, const double* x, int metric, double reg, double* value);
+/* Runs algorithm block `algorithm`: pso (ParticleSwarmOptimization, base flavour, generator seeded
+ * with `seed`), bruteforce, lmvm (bounded L-BFGS on central differences) or none_pounders (the
+ * objective at the initial values). x_best [nparams] and *value: the best point and its objective;
+ * *evals: objective calls. */
+int hgx_hig_fit_run(hgx_hig* h, int algorithm, unsigned long long seed, double* x_best, double* value,
+                    long long* evals);
+/* Every objective evaluation since the last hgx_hig_fit_run or hgx_hig_fit_init, in call order: rows
+ * [nrows][nparams + 1] of (x, value), at most cap_rows written (rows may be NULL to query *nrows). */
+int hgx_hig_fit_history(hgx_hig* h, long long cap_rows, double* rows, long long* nrows);
+
 const char* hgx_hig_last_error(const hgx_hig* h);
 
 #ifdef __cplusplus

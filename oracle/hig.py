@@ -421,11 +421,11 @@ def smear(img, sigma, compat=False):
     return np.array([one_d(c) for c in tmp.T]).T
 
 
-def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True):
+def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True, model=None):
     """Image [nrow][ncol] of configuration `config` (alpha_i outermost, as run_all_gisaxs loops) for
-    .hig text; grain orientations drawn from rand() after srand(seed) (earlier configurations draw
-    first)."""
-    m = read_model(text)
+    .hig text (or an already read and edited `model`); grain orientations drawn from rand() after
+    srand(seed) (earlier configurations draw first)."""
+    m = read_model(text) if model is None else model
     sc, cp = m["scattering"], m["compute"]
     keV = sc["photon"] / 1000 if sc["unit"] == "ev" else sc["photon"]
     k0 = 2 * PI_ / (1.23984 / keV)
@@ -499,3 +499,213 @@ def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", me
                 img = smear(img, sc["smearing"], compat=smear_mode == "compat")
             return img
     raise IndexError(config)
+
+
+# ----- fitting ---------------------------------------------------------------------------------
+# The analyzer's side of a .hig input: the fitting block (src/config/hig_input.cpp:302-340,
+# 849-890, 1444-1477, 1524-1571, 1755-1777), HiGInput::update_params (:2712-2832) over the model
+# above, the objective of HipGISAXSObjectiveFunction (src/analyzer/objective_func_hipgisaxs.cpp:
+# 245-332) and the particle-swarm and brute-force drivers (src/analyzer/hipgisaxs_fit_pso.cpp,
+# hipgisaxs_fit_pso_particle.cpp, hipgisaxs_fit_bruteforce.cpp).
+
+
+def read_fitting(text):
+    """fitparams sorted by key as (key, variable, min, max, step, init), the reference data and the
+    algorithm blocks in file order."""
+    top = dict(parse_tree(text))["hipGisaxsInput"]
+    params, ref, algos = {}, {}, []
+    for key, val in top:
+        if key != "fitting":
+            continue
+        for k, v in val:
+            d = dict(v)
+            if k == "fitparam":
+                r = dict(d.get("range", []))
+                params[d["key"]] = (d["key"], d["variable"], r.get("min", 0.0), r.get("max", 0.0),
+                                    r.get("step", -1.0), d.get("init", 0.0))
+            elif k == "referencedata":
+                reg = dict(d.get("fitregion", []))
+                ref = {"path": d.get("path", ""), "mask": d.get("mask", ""), "region": reg.get("type", "qspace"),
+                       "regmin": reg.get("regmin"), "regmax": reg.get("regmax")}
+            elif k == "algorithm":
+                a = {"name": d.get("algoname", ""), "metric": d.get("distancemetric", ""),
+                     "tolerance": d.get("tolerance", 0.0), "regularization": d.get("regularization", 0.0),
+                     "params": {}}
+                for ak, av in v:
+                    if ak == "algoparam":
+                        ad = dict(av)
+                        a["params"][ad["type"]] = ad.get("pvalue", 0.0)
+                algos.append(a)
+    return [params[k] for k in sorted(params)], ref, algos
+
+
+def _split(path):
+    kw, _, rest = path.partition(":")
+    mk = re.fullmatch(r"([^\[]*)\['(.*)'\]", kw)
+    return (mk.group(1), mk.group(2), rest) if mk else (kw, "", rest)
+
+
+_AX = {"x": 0, "0": 0, "y": 1, "1": 1, "z": 2, "2": 2}
+
+
+def update_param(m, variable, v):
+    """HiGInput::update_params for one path, on the model of read_model (Shape, Layer, Structure and
+    ScatteringParams update_param: src/model/shape.cpp:145-247, layer.cpp:41-86, structure.cpp:519-820,
+    inst_scattering.cpp:41-167). Raises KeyError on an unknown path."""
+    name, key, rest = _split(variable)
+    n1, key1, r1 = _split(rest)
+    k2, _, r2 = r1.partition(":")
+    k3, _, r3 = r2.partition(":")
+    if name == "shape":
+        sh = m["shapes"][key]
+        if n1 in ("xrot", "yrot", "zrot"):
+            sh[n1] = v  # stored as given (radians)
+        elif n1 == "param":
+            p = sh["params"][key1]
+            if k2 not in ("min", "max", "p1", "p2", "nvalues"):
+                raise KeyError(variable)
+            p[k2] = int(v) if k2 == "nvalues" else v
+        elif n1 != "originvec":  # the shape origin enters no simulation step
+            raise KeyError(variable)
+    elif name == "layer":
+        lay = next(l for l in m["layers"].values() if l.get("key") == key)
+        if n1 == "thickness":
+            lay["thickness"] = v
+        elif n1 == "refindex" and k2 in ("delta", "beta"):
+            lay[k2] = v
+        else:
+            raise KeyError(variable)
+    elif name == "structure":
+        st = m["structures"][key]
+        if n1 == "iratio":
+            st["iratio"] = v
+        elif n1 == "grain" and k2 == "lattice" and k3 in ("a", "b", "c"):
+            if st["abc"] is None:
+                vs = _LATTICES.get((st["lattice"], st["hkl"]), _LATTICES.get((st["lattice"], None)))
+                st["abc"] = tuple(list(map(float, x)) for x in vs)
+            else:
+                st["abc"] = tuple(list(map(float, x)) for x in st["abc"])
+            st["abc"]["abc".index(k3)][_AX[r3.partition(":")[0]]] = v
+        elif n1 == "grain" and k2 == "scaling" and k3 == "":
+            st["scaling"] = [v, v, v]
+        elif n1 == "grain" and k2 == "scaling":
+            st["scaling"]["abc".index(k3)] = v
+        elif n1 == "grain" and k2 in ("transvec", "repetition"):
+            st[k2][_AX[k3]] = v
+        elif n1 == "ensemble" and k2 in ("spacing", "maxgrains"):
+            st[k2][_AX[k3]] = v
+        else:
+            raise KeyError(variable)
+    elif name == "instrumentation" and n1 == "scattering":
+        sc = m["scattering"]
+        if k2 in ("alphai", "inplanerot", "tilt"):
+            sc[k2][{"min": 0, "max": 1, "step": 2}[k3]] = v
+        elif k2 == "photon" and k3 == "value":
+            sc["photon"] = v
+        else:
+            raise KeyError(variable)
+    else:
+        raise KeyError(variable)
+
+
+def read_ascii_image(path):
+    """ImageData::read (src/analyzer/ImageData.cpp:79-104): one row per non-empty line."""
+    rows = [list(map(float, ln.split())) for ln in open(path) if ln.strip()]
+    return np.array(rows, dtype=np.float64)
+
+
+def objective(text, params, x, metric, O, mask=None, reg=0.0, base_dir=".", seed=1):
+    """HipGISAXSObjectiveFunction::operator(): the image of the edited model, its distance to O under
+    `metric` (oracle.metrics), plus (reg / 2) |x - range midpoints|^2; non-finite -> 1e8."""
+    from .metrics import METRICS
+    m = read_model(text)
+    pn = 0.0
+    for p, xi in zip(params, x):
+        update_param(m, p[1], xi)
+        mean = (p[3] - p[2]) / 2.0 + p[2]
+        pn += (xi - mean) ** 2
+    img = run(None, model=m, base_dir=base_dir, seed=seed)
+    mk = np.ones_like(O) if mask is None else mask
+    d = METRICS[metric](O.ravel(), img.ravel(), mk.ravel())
+    return d + reg / 2 * pn if math.isfinite(d) else 1e8
+
+
+class MT19937_64:
+    """std::mt19937_64 (the 64-bit Mersenne twister of C++11, [rand.predef]), the generator under
+    woo::MTRandomNumberGenerator (include/woo/random/woo_mtrandom.hpp:25-33)."""
+    _M = (1 << 64) - 1
+
+    def __init__(self, seed=5489):
+        mt = [seed & self._M]
+        for i in range(1, 312):
+            mt.append((6364136223846793005 * (mt[-1] ^ (mt[-1] >> 62)) + i) & self._M)
+        self.mt, self.i = mt, 312
+
+    def __call__(self):
+        if self.i >= 312:
+            mt = self.mt
+            for k in range(312):
+                y = (mt[k] & 0xFFFFFFFF80000000) | (mt[(k + 1) % 312] & 0x7FFFFFFF)
+                mt[k] = mt[(k + 156) % 312] ^ (y >> 1) ^ (0xB5026F5AA96619E9 if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self._M
+
+    def uniform(self):
+        """MTRandomNumberGenerator::rand: (g() - min) / (max - min) in double."""
+        return float(self()) / float(self._M)
+
+
+def pso(f, lo, hi, omega, phi1, phi2, nparticles, ngenerations, seed, tune_omega=False):
+    """ParticleSwarmOptimization, base flavour, on objective f: returns (best x, best f, evaluated
+    rows (x..., f) in order)."""
+    d = len(lo)
+    xmin = [min(a, b) for a, b in zip(lo, hi)]
+    xmax = [max(a, b) for a, b in zip(lo, hi)]
+    g = MT19937_64(seed)
+    x, v = [], []
+    for _ in range(nparticles):  # PSOParticle::init_random_uniform: value then velocity per parameter
+        xi, vi = [], []
+        for j in range(d):
+            xi.append(xmin[j] + g.uniform() * (xmax[j] - xmin[j]))
+            vi.append(-xmax[j] + g.uniform() * (xmax[j] + xmax[j]))
+        x.append(xi)
+        v.append(vi)
+    pb, pf = [list(xi) for xi in x], [float("inf")] * nparticles
+    best, fbest, rows = [0.0] * d, float("inf"), []
+    for _ in range(ngenerations):
+        for i in range(nparticles):
+            fi = f(x[i])
+            rows.append(list(x[i]) + [fi])
+            if pf[i] > fi:
+                pf[i], pb[i] = fi, list(x[i])
+            if fbest > fi:
+                fbest, best = fi, list(x[i])
+        for i in range(nparticles):  # PSOParticle::update_particle
+            for j in range(d):
+                r1, r2 = g.uniform(), g.uniform()
+                nv = omega * v[i][j] + phi1 * r1 * (pb[i][j] - x[i][j]) + phi2 * r2 * (best[j] - x[i][j])
+                v[i][j] = min(max(nv, -xmax[j]), xmax[j])
+                x[i][j] = min(max(x[i][j] + v[i][j], xmin[j]), xmax[j])
+        if tune_omega:
+            omega /= 2.0
+    return best, fbest, rows
+
+
+def bruteforce_points(lo, hi, step):
+    """BruteForceOptimization::loop_over_param: p = min, min + step, ... <= max with the step
+    accumulated (a negative step means 1), the first parameter outermost."""
+    pts = [[]]
+    for a, b, s in zip(lo, hi, step):
+        s = 1.0 if s < 0 else s
+        vals, p = [], a
+        while p <= b:
+            vals.append(p)
+            p += s
+        pts = [q + [val] for q in pts for val in vals]
+    return pts

@@ -50,4 +50,21 @@ def sqrt_c_l2(O, I, m):
         return _sumsq(m * np.abs(sr - c * sd))
 
 
-METRICS = {"chi2": chi2, "l2": l2, "sqrt_unit_l2": sqrt_unit_l2, "sqrt_c_l2": sqrt_c_l2}
+def cbrt_unit_l2(O, I, m):
+    """CbrtUnitVectorNormL2DistanceSquare (:543-575): the unit-vector form over cube roots, every term
+    kept (the loop drops nothing)."""
+    cr, cd = np.cbrt(O), np.cbrt(I)
+    t = m * (cd / norm_l2(cd, m) - cr / norm_l2(cr, m))
+    return float(np.sum(t * t))
+
+
+def cbrt_c_l2(O, I, m):
+    """CbrtCNormL2DistanceSquare (:579-616): the c-norm form over cube roots, every term kept."""
+    cr, cd = np.cbrt(O), np.cbrt(I)
+    c = vec_dot(cd, cr, m) / norm_l2(cd, m) ** 2
+    t = m * np.abs(cr - c * cd)
+    return float(np.sum(t * t))
+
+
+METRICS = {"chi2": chi2, "l2": l2, "sqrt_unit_l2": sqrt_unit_l2, "sqrt_c_l2": sqrt_c_l2,
+           "cbrt_unit_l2": cbrt_unit_l2, "cbrt_c_l2": cbrt_c_l2}

@@ -1,0 +1,177 @@
+"""Fitting a .hig model on the device against the oracle: the distance functors of hgx_image_distance,
+the objective of HipGISAXSObjectiveFunction (update_params, compute_gisaxs, distance, regularization),
+the particle-swarm and brute-force drivers step by step, LMVM convergence, and the override of the
+output region by the reference data's region."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.hig as OH
+from oracle import metrics as OM
+from hipgisaxs_amd import _native as N
+from hipgisaxs_amd import api
+from hipgisaxs_amd import hig as H
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden" / "hig"
+FIT = (GOLD / "fit_cylinders.hig").read_text()
+TRUTH = [5.0, 50.0]
+
+
+def _say(*a):
+    print("[fit]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fitdir(tmp_path_factory):
+    """The fit input beside reference data that the oracle computes at the true parameters."""
+    d = tmp_path_factory.mktemp("fit")
+    params, _, _ = OH.read_fitting(FIT)
+    m = OH.read_model(FIT)
+    for p, v in zip(params, TRUTH):
+        OH.update_param(m, p[1], v)
+    ref = OH.run(None, model=m)
+    H.HigSimulation.save(ref, str(d / "fit_cylinders_ref.out"))
+    (d / "fit.hig").write_text(FIT)
+    return d
+
+
+def _sim(d, text=FIT):
+    return H.HigSimulation(text=text, base_dir=str(d))
+
+
+def _oracle_f(d, x, metric, reg=0.0, text=FIT):
+    params, _, _ = OH.read_fitting(text)
+    O = OH.read_ascii_image(str(d / "fit_cylinders_ref.out"))
+    return OH.objective(text, params, x, metric, O, reg=reg)
+
+
+def test_image_distance_matches_oracle_functors():
+    _say("image_distance")
+    ctx = api.Context()
+    rng = np.random.default_rng(5)
+    for n in (1, 257, 100003):
+        I = rng.gamma(2.0, 3.0, n)
+        O = rng.poisson(I).astype(float) + 0.5
+        m = (rng.uniform(size=n) > 0.2).astype(float) * rng.choice([1.0, 2.0], n)
+        dI, dO, dm = (torch.tensor(a, dtype=torch.float64, device="cuda") for a in (I, O, m))
+        for name, fn in OM.METRICS.items():
+            got = api.image_distance(ctx, dI, dO, dm, metric=name)
+            want = fn(O, I, m)
+            _say(n, name, got, want)
+            # one pixel: the unit-vector forms are zero up to rounding
+            assert got == pytest.approx(want, rel=1e-11, abs=1e-20)
+        # no mask: all weights 1
+        assert api.image_distance(ctx, dI, dO, None, "sqrt_c_l2") == pytest.approx(OM.sqrt_c_l2(O, I, np.ones(n)),
+                                                                                   rel=1e-11)
+
+
+@pytest.mark.parametrize("metric", ["sqrt_unit_l2", "sqrt_c_l2", "cbrt_unit_l2", "cbrt_c_l2", "chi2", "l2"])
+def test_objective_matches_oracle(fitdir, metric):
+    sim = _sim(fitdir)
+    for x, reg in (([4.2, 48.5], 0.0), ([6.5, 58.0], 0.3), ([5.0, 50.0], 0.0)):
+        got = sim.objective(x, metric, reg)
+        want = _oracle_f(fitdir, x, metric, reg)
+        _say(metric, x, reg, got, want)
+        if x == TRUTH and reg == 0.0:
+            assert abs(got) <= 1e-12 * max(1.0, abs(_oracle_f(fitdir, [4.2, 48.5], metric)))
+        else:
+            assert got == pytest.approx(want, rel=1e-9)
+
+
+def test_pso_follows_the_oracle_swarm(fitdir):
+    """The swarm's positions generation by generation equal the oracle's with the same mt19937_64 seed;
+    the objective values agree to rounding, so every best update takes the same branch."""
+    sim = _sim(fitdir)
+    out = sim.fit(algorithm=0, seed=7)
+    rows = sim.fit_history()
+    params, _, algos = OH.read_fitting(FIT)
+    a = algos[0]["params"]
+    best, fb, ref_rows = OH.pso(lambda x: _oracle_f(fitdir, x, "sqrt_unit_l2"), [p[2] for p in params],
+                                [p[3] for p in params], a["pso_omega"], a["pso_phi1"], a["pso_phi2"],
+                                int(a["pso_num_particles"]), int(a["pso_num_generations"]), seed=7)
+    ref_rows = np.array(ref_rows)
+    _say("pso", out, "oracle best", best, fb)
+    assert out["evals"] == len(ref_rows) == 24
+    np.testing.assert_allclose(rows[:, :2], ref_rows[:, :2], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rows[:, 2], ref_rows[:, 2], rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(out["x"], best, rtol=0, atol=1e-12)
+    assert out["value"] == pytest.approx(fb, rel=1e-9)
+    # another seed moves the swarm differently
+    assert not np.allclose(sim.fit(algorithm=0, seed=8)["x"], out["x"])
+
+
+def test_bruteforce_visits_the_grid_and_finds_the_truth(fitdir):
+    sim = _sim(fitdir)
+    out = sim.fit(algorithm=1)
+    rows = sim.fit_history()
+    params, _, _ = OH.read_fitting(FIT)
+    pts = np.array(OH.bruteforce_points([p[2] for p in params], [p[3] for p in params], [p[4] for p in params]))
+    _say("bruteforce", out)
+    assert rows.shape[0] == len(pts) == 99
+    np.testing.assert_array_equal(rows[:, :2], pts)
+    np.testing.assert_allclose(out["x"], TRUTH, rtol=0, atol=1e-12)
+    assert out["value"] <= 1e-12
+    # spot-check three grid values against the oracle objective
+    for k in (0, 17, 98):
+        assert rows[k, 2] == pytest.approx(_oracle_f(fitdir, list(pts[k]), "sqrt_c_l2"), rel=1e-9)
+
+
+def test_lmvm_converges_to_the_truth(fitdir):
+    """From initial values inside the truth's basin (the lattice scaling has side minima a few units
+    away), LMVM on central differences over the fitparam steps: with the brute-force grid's coarse steps
+    it descends by more than 20x; with fine steps it ends at the truth."""
+    sim = _sim(fitdir)
+    f0 = sim.objective([4.2, 48.5], "sqrt_unit_l2")
+    out = sim.fit(algorithm=2)
+    _say("lmvm coarse steps", out, "f0", f0)
+    assert out["value"] < 0.05 * f0
+    assert 3 <= out["x"][0] <= 8 and 40 <= out["x"][1] <= 60
+    fine = FIT.replace("step = 0.5 }", "step = 0.02 }").replace("step = 2.5 }", "step = 0.1 }")
+    sim = _sim(fitdir, fine)
+    out = sim.fit(algorithm=2)
+    _say("lmvm fine steps", out)
+    assert out["value"] < 1e-6 * f0
+    np.testing.assert_allclose(out["x"], TRUTH, rtol=0, atol=0.02)
+
+
+def test_plain_objective_algorithm(fitdir):
+    sim = _sim(fitdir)
+    out = sim.fit(algorithm=3)
+    assert out["evals"] == 1
+    np.testing.assert_array_equal(out["x"], [4.2, 48.5])
+    assert out["value"] == pytest.approx(_oracle_f(fitdir, [4.2, 48.5], "cbrt_unit_l2"), rel=1e-9)
+
+
+def test_reference_region_overrides_the_output_region(tmp_path):
+    """Reference data of another size: the simulation takes the reference's region and size
+    (override_qregion) and the objective equals the oracle's on that region."""
+    text = FIT.replace("regmin = [-1 0], regmax = [1 1]", "regmin = [-0.6 0.1], regmax = [0.8 0.9]")
+    params, _, _ = OH.read_fitting(text)
+    m = OH.read_model(text)
+    m["compute"]["minpoint"], m["compute"]["maxpoint"] = [-0.6, 0.1], [0.8, 0.9]
+    m["compute"]["resolution"] = [30, 20]
+    for p, v in zip(params, TRUTH):
+        OH.update_param(m, p[1], v)
+    ref = OH.run(None, model=m)
+    H.HigSimulation.save(ref, str(tmp_path / "fit_cylinders_ref.out"))
+    sim = H.HigSimulation(text=text, base_dir=str(tmp_path))
+    sim.fit_init()
+    assert (sim.info()["ncol"], sim.info()["nrow"]) == (30, 20)
+    x = [6.0, 44.0]
+    got = sim.objective(x, "sqrt_c_l2")
+    m2 = OH.read_model(text)
+    m2["compute"] = m["compute"]
+    for p, v in zip(params, x):
+        OH.update_param(m2, p[1], v)
+    want = OM.sqrt_c_l2(ref.ravel(), OH.run(None, model=m2).ravel(), np.ones(ref.size))
+    _say("override", got, want)
+    assert got == pytest.approx(want, rel=1e-9)
+    assert sim.objective(TRUTH, "sqrt_c_l2") <= 1e-12

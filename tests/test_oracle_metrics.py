@@ -45,3 +45,21 @@ def test_mask_weights_twice_in_the_distance():
     w = np.array([1.0, 0.5])
     assert M.l2(O, I, w) == pytest.approx(9.0 + 0.25 * 64.0)
     assert M.l2(O, I, np.ones(2)) == pytest.approx(9.0 + 64.0)
+
+
+def test_cube_root_forms(images):
+    """cbrt_unit_l2 and cbrt_c_l2: scale invariance (O = k I gives 0), the same closed forms over cube
+    roots, and a negative value kept with its sign (cbrt, not a root of |x|)."""
+    O, I, m = images
+    for k in (1.0, 5.3, 1e-3):
+        assert M.cbrt_unit_l2(k * I, I, m) == pytest.approx(0.0, abs=1e-26)
+        assert M.cbrt_c_l2(k * I, I, m) == pytest.approx(0.0, abs=1e-20)
+    cO, cI = np.cbrt(O), np.cbrt(I)
+    proj = np.sum(m * cO * cO) - np.sum(m * cO * cI) ** 2 / np.sum(m * cI * cI)
+    assert M.cbrt_c_l2(O, I, m) == pytest.approx(proj, rel=1e-12)
+    cos = np.sum(m * cO * cI) / np.sqrt(np.sum(m * cO * cO) * np.sum(m * cI * cI))
+    assert M.cbrt_unit_l2(O, I, m) == pytest.approx(2 - 2 * cos, rel=1e-12)
+    # cube roots differ from square roots: the two unit forms disagree on generic data
+    assert abs(M.cbrt_unit_l2(O, I, m) - M.sqrt_unit_l2(O, I, m)) > 1e-6
+    # (-8, 1) against (8, 1): cube roots (-2, 1) and (2, 1), unit vectors apart by 4 * (2/sqrt 5)^2
+    assert M.cbrt_unit_l2(np.array([-8.0, 1.0]), np.array([8.0, 1.0]), np.ones(2)) == pytest.approx(16 / 5)
