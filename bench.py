@@ -469,6 +469,47 @@ def bench_hig_examples(reps=3, cpu=True):
     return out
 
 
+def bench_hig_fit(cpu=True):
+    """Fitting a .hig model (tests/golden/hig/fit_cylinders.hig at a 600 x 300 detector): objective
+    evaluations per second of the particle-swarm driver (16 particles x 4 generations, each evaluation
+    = update_params + compute_gisaxs + the sqrt_unit_l2 distance on the device), and the CPU oracle's
+    objective timed on one evaluation of the same model."""
+    import tempfile
+
+    from hipgisaxs_amd import hig as H
+    import oracle.hig as OH
+
+    text = (Path(ROOT) / "tests" / "golden" / "hig" / "fit_cylinders.hig").read_text()
+    text = text.replace("resolution = [40 24]", "resolution = [600 300]")
+    text = text.replace("pvalue = 6 }", "pvalue = 16 }")
+    with tempfile.TemporaryDirectory() as d:
+        sim = H.HigSimulation(text=text, base_dir=d)
+        for var, v in (("shape['cyl']:param['radius']:min", 5.0), ("structure['hcp']:grain:scaling", 50.0)):
+            sim.update_param(var, v)
+        H.HigSimulation.save(sim.compute(), str(Path(d) / "fit_cylinders_ref.out"))
+        sim.close()
+        sim = H.HigSimulation(text=text, base_dir=d)
+        sim.fit_init()
+        sim.objective([4.2, 48.5])  # first launches
+        t0 = time.perf_counter()
+        out = sim.fit(algorithm=0, seed=1)
+        wall = time.perf_counter() - t0
+        res = {"pixels": 600 * 300, "pso_evals": out["evals"], "pso_s": wall, "evals_per_s": out["evals"] / wall,
+               "ms_per_eval": 1e3 * wall / out["evals"], "best": [float(x) for x in out["x"]]}
+        sim.close()
+        if cpu:
+            params, _, _ = OH.read_fitting(text)
+            O = OH.read_ascii_image(str(Path(d) / "fit_cylinders_ref.out"))
+            t0 = time.perf_counter()
+            OH.objective(text, params, [4.2, 48.5], "sqrt_unit_l2", O)
+            tc = time.perf_counter() - t0
+            res["cpu_baseline"] = {"kind": "port", "sample": "one objective evaluation", "s": tc,
+                                   "evals_per_s": 1.0 / tc,
+                                   "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    print(f"[bench] hig fit {res['ms_per_eval']:.3f} ms per evaluation", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,7 +548,8 @@ def main():
     from hipgisaxs_amd.api import Batch, Context
 
     if args.hig_only:
-        print(json.dumps({"hig_examples": bench_hig_examples(cpu=not args.no_cpu)}))
+        print(json.dumps({"hig_examples": bench_hig_examples(cpu=not args.no_cpu),
+                          "hig_fit": bench_hig_fit(cpu=not args.no_cpu)}))
         return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
@@ -714,6 +756,7 @@ def main():
         result["grating_fit"] = bench_grating()
         print(f"[bench] grating done", file=sys.stderr, flush=True)
         result["hig_examples"] = bench_hig_examples(cpu=not args.no_cpu)
+        result["hig_fit"] = bench_hig_fit(cpu=not args.no_cpu)
         print(f"[bench] hig examples done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb, (s, rows, ref) = cpu_baseline(shapes)
