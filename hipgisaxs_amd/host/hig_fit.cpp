@@ -358,10 +358,11 @@ bool HipGISAXS::fit_objective(const double* x, int metric, double reg, double* v
   if (!run_device(sc.alphai_min, sc.phi_min, &d_img)) return false;
   double D = 0.0;
   if (!d_img) {  // no structures: a zero image
-    d_img = buffer(0, (size_t)npix_ * 8);
-    if (!d_img || !hgx(hgx_axpy(gpu_->get(), npix_, 0.0, buf_[6] ? (double*)buf_[6] : nullptr, 0, (double*)d_img),
-                       "hgx_axpy"))
+    double* d_zero = buffer(0, (size_t)npix_ * 8);
+    const std::vector<double> zeros((size_t)npix_, 0.0);
+    if (!d_zero || !hgx(hgx_memcpy_h2d(gpu_->get(), d_zero, zeros.data(), (size_t)npix_ * 8), "hgx_memcpy_h2d"))
       return false;
+    d_img = d_zero;
   }
   if (!hgx(hgx_image_distance(gpu_->get(), npix_, metric, d_img, static_cast<const double*>(buf_[6]),
                               static_cast<const double*>(buf_[7]), &D),
