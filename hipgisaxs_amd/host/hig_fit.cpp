@@ -524,8 +524,8 @@ bool run_bruteforce(Problem& P, std::vector<double>& best, double& fbest, std::s
   return loop(0);
 }
 
-// FitLMVMAlgo (hipgisaxs_fit_lmvm.cpp:43-180, 190-224): bounded limited-memory BFGS from the initial
-// values, gradients by central differences over +-step/2 (step, or 0.04 where the step is not
+// One solve of FitLMVMAlgo (hipgisaxs_fit_lmvm.cpp:43-180, 190-224): bounded limited-memory BFGS
+// from P.x0, gradients by central differences over +-step/2 (step, or 0.04 where the step is not
 // positive), divided by the same step (the reference divides by 0.04 whatever the step). Projected
 // search directions with a backtracking Armijo line search; stops when the projected gradient or
 // the relative decrease falls below the tolerance, or after 200 iterations.
@@ -707,7 +707,18 @@ bool fit_run(HipGISAXS& sim, int algo_index, uint64_t seed, std::vector<double>&
     P.reg = 0.0;
     ok = run_bruteforce(P, best, fbest, err);
   } else if (a.name == "lmvm") {
-    ok = run_lmvm(P, a, best, fbest, err);
+    // FitLMVMAlgo::run's regularization continuation (hipgisaxs_fit_lmvm.cpp:77-172): up to 10 solves,
+    // the factor divided by 5 after each, each solve starting from the previous solution; a zero
+    // factor gives one solve
+    double reg = a.regularization;
+    ok = true;
+    for (int r = 0; r < 10 && ok; ++r) {
+      P.reg = reg;
+      ok = run_lmvm(P, a, best, fbest, err);
+      if (ok) P.x0 = best;
+      reg /= 5.0;
+      if (reg <= DBL_MIN) break;
+    }
   } else if (a.name == "none_pounders") {  // the objective at the initial values, no fit
     best = P.x0;
     P.reg = 0.0;

@@ -142,6 +142,26 @@ def test_lmvm_converges_to_the_truth(fitdir):
     np.testing.assert_allclose(out["x"], TRUTH, rtol=0, atol=0.02)
 
 
+def test_lmvm_regularization_continuation_and_fp32c_objective(fitdir):
+    """A regularized LMVM runs the reference's continuation (factor / 5 per solve, warm starts) and ends
+    at the truth; the FP32C simulation gives the objective within the FP32C image bound."""
+    fine = FIT.replace("step = 0.5 }", "step = 0.02 }").replace("step = 2.5 }", "step = 0.1 }")
+    fine = fine.replace("tolerance = 1e-10 }", "tolerance = 1e-10, regularization = 0.01 }")
+    sim = _sim(fitdir, fine)
+    assert sim.fit_algorithms()[2]["regularization"] == 0.01
+    out = sim.fit(algorithm=2)
+    _say("lmvm regularized", out)
+    np.testing.assert_allclose(out["x"], TRUTH, rtol=0, atol=0.02)
+    # the last solve's factor is 0.01 / 5^9: the value is the distance plus that small penalty
+    assert out["value"] == pytest.approx(sim.objective(out["x"], "sqrt_unit_l2", 0.01 / 5 ** 9), rel=1e-9)
+    x = [4.2, 48.5]
+    f64 = _sim(fitdir).objective(x, "sqrt_c_l2")
+    s32 = H.HigSimulation(text=FIT, base_dir=str(fitdir), precision="fp32c")
+    f32 = s32.objective(x, "sqrt_c_l2")
+    _say("fp32c objective", f32, f64)
+    assert f32 == pytest.approx(f64, rel=1e-5)
+
+
 def test_plain_objective_algorithm(fitdir):
     sim = _sim(fitdir)
     out = sim.fit(algorithm=3)
