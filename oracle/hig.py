@@ -7,6 +7,12 @@ src/model/{structure,shape,layer,inst_scattering,compute_params}.cpp), HipGISAXS
 orientation_distribution (:1702-2086) and the smearing of include/numerics/convolutions.hpp:128-175.
 Grain orientations drawn with rand() come from the C library's own rand() (the reference never
 seeds it on this path, so a fresh process draws the seed-1 sequence). Only tests import this.
+
+The fitting half (read_fitting, update_param, objective, MT19937_64, pso, bruteforce_points) restates
+the analyzer. Pins: the generator by the C++ standard's 10000th-output value, the brute-force grid by
+its float accumulation (0.1 added three times exceeds 0.3), the distance functors in oracle.metrics,
+update_param by the grains it produces. Parity unpinned: the particle swarm's draw order and update
+(the reference seeds it with the clock and holds no vectors), beyond convergence on a quadratic.
 """
 from __future__ import annotations
 
