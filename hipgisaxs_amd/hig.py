@@ -35,6 +35,7 @@ _SIGS = {
     "hgx_hig_last_counts": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp]),
     "hgx_hig_kernel_ms": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)]),
     "hgx_hig_save": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
+    "hgx_hig_save_tiff": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
     "hgx_hig_read_mesh": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_int64), C.c_int64, _vp]),
     "hgx_hig_update_param": (C.c_int, [_vp, C.c_char_p, C.c_double]),
     "hgx_hig_fit_info": (C.c_int, [_vp, _ip, _ip]),
@@ -238,6 +239,14 @@ class HigSimulation:
         self._check(self.lib.hgx_hig_fit_history(self.h, nr.value, rows.ctypes.data_as(_dp), C.byref(nr)),
                     "fit_history")
         return rows
+
+    @staticmethod
+    def save_tiff(image: np.ndarray, path: str) -> None:
+        """The run's image file: log10 of the shifted image normalised to [0, 1], 16-bit grayscale TIFF."""
+        lib = load()
+        img = np.ascontiguousarray(image, dtype=np.float64)
+        if lib.hgx_hig_save_tiff(img.ctypes.data_as(_vp), img.shape[1], img.shape[0], str(path).encode()) != 0:
+            raise HgxError(f"hgx_hig_save_tiff: cannot write {path} (non-finite pixels or bad size)")
 
     @staticmethod
     def save(image: np.ndarray, path: str) -> None:

@@ -1,6 +1,7 @@
 // hig_sim.cpp -- the .hig simulation driver on the hgx C ABI (see hig_sim.hpp).
 #include "hig_sim.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -747,6 +748,59 @@ bool HipGISAXS::save_gisaxs(const double* data, unsigned ncol, unsigned nrow, co
   return (bool)f;
 }
 
+bool HipGISAXS::save_tiff(const double* data, unsigned ncol, unsigned nrow, const std::string& path) {
+  const size_t n = (size_t)ncol * nrow;
+  if (n == 0 || ncol > 65535 || nrow > 65535) return false;
+  std::vector<double> v(data, data + n);
+  for (double x : v)
+    if (!std::isfinite(x)) return false;
+  // translate_pixels_to_positive, mat_log10_2d, normalize_pixels (image.cpp:269-312, utilities.cpp:52-75)
+  const double mn = *std::min_element(v.begin(), v.end());
+  if (mn < 0)
+    for (double& x : v) x -= mn;
+  for (double& x : v) x = x == 0.0 ? 0.0 : std::log10(x);
+  const double lo = *std::min_element(v.begin(), v.end()), hi = *std::max_element(v.begin(), v.end());
+  std::vector<uint16_t> px(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double u = lo == hi ? (lo < 0 ? 0.0 : 1.0) : (v[i] - lo) / (hi - lo);
+    px[i] = (uint16_t)std::lround(u * 65535.0);
+  }
+  // little-endian baseline TIFF: header, pixel strip, one IFD of 10 entries
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return false;
+  auto u16 = [&](uint16_t x) { f.put((char)(x & 0xff)).put((char)(x >> 8)); };
+  auto u32 = [&](uint32_t x) {
+    for (int k = 0; k < 4; ++k) f.put((char)((x >> (8 * k)) & 0xff));
+  };
+  const uint32_t strip = 8, bytes = (uint32_t)(n * 2), ifd = strip + bytes + (bytes & 1);
+  f.write("II", 2);
+  u16(42);
+  u32(ifd);
+  for (uint16_t x : px) u16(x);
+  if (bytes & 1) f.put(0);
+  struct Entry {
+    uint16_t tag, type;
+    uint32_t count, value;
+  };
+  const Entry e[10] = {{256, 4, 1, ncol},       {257, 4, 1, nrow},  {258, 3, 1, 16}, {259, 3, 1, 1},
+                       {262, 3, 1, 1},          {273, 4, 1, strip}, {277, 3, 1, 1},  {278, 4, 1, nrow},
+                       {279, 4, 1, bytes},      {284, 3, 1, 1}};
+  u16(10);
+  for (const Entry& x : e) {
+    u16(x.tag);
+    u16(x.type);
+    u32(x.count);
+    if (x.type == 3) {  // SHORT values sit left-justified in the 4-byte field
+      u16((uint16_t)x.value);
+      u16(0);
+    } else {
+      u32(x.value);
+    }
+  }
+  u32(0);  // no further IFD
+  return (bool)f;
+}
+
 }  // namespace hig
 
 // ----- C ABI ---------------------------------------------------------------------------------------
@@ -880,6 +934,11 @@ int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, doubl
 int hgx_hig_save(const double* image, int ncol, int nrow, const char* path) {
   if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
   return hig::HipGISAXS::save_gisaxs(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+}
+
+int hgx_hig_save_tiff(const double* image, int ncol, int nrow, const char* path) {
+  if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
+  return hig::HipGISAXS::save_tiff(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
 }
 
 int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records) {

@@ -67,6 +67,11 @@ class HipGISAXS {
   // The grains the next run would use for structure `key` (does not advance the sampler).
   bool grains(const std::string& key, std::vector<HigGrain>& out);
   static bool save_gisaxs(const double* data, unsigned ncol, unsigned nrow, const std::string& path);
+  // The image file of run_all_gisaxs (Image::construct_image + Image::save, src/image/image.cpp:125-226,
+  // 502-512): pixels translated to non-negative, log10 (zero stays zero), normalised to [0, 1]; written
+  // as an uncompressed 16-bit grayscale TIFF (the reference maps [0, 1] through its colour palette into
+  // an 8-bit RGB TIFF). Non-finite pixels are an error.
+  static bool save_tiff(const double* data, unsigned ncol, unsigned nrow, const std::string& path);
 
   // ---- fitting: the HipGISAXSObjectiveFunction of the reference's analyzer
   // (src/analyzer/objective_func_hipgisaxs.cpp:47-72, 110-170, 245-332) on the device.

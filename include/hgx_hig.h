@@ -64,6 +64,11 @@ int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, doubl
 /* HipGISAXS::save_gisaxs (src/sim/hipgisaxs_main.cpp:2250-2260): nrow lines of ncol tab-terminated
  * values. */
 int hgx_hig_save(const double* image, int ncol, int nrow, const char* path);
+/* The image file of run_all_gisaxs (Image::construct_image + Image::save, src/image/image.cpp:125-226,
+ * 502-512): pixels shifted to non-negative, log10 (zeros kept), normalised to [0, 1], written as an
+ * uncompressed 16-bit grayscale TIFF (the reference's colour palette is not applied). Host only;
+ * non-finite pixels are an error. */
+int hgx_hig_save_tiff(const double* image, int ncol, int nrow, const char* path);
 /* Reads a mesh file of a custom shape (host only): Wavefront .obj into HGX_MESH_EXACT9 records
  * (ObjectShapeReader, src/file/objectshape_reader.cpp:52-300; polygons are split into triangle fans
  * where the reference drops every non-triangle face) or raw .dat into HGX_MESH_APPROX8 records

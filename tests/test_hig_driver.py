@@ -218,3 +218,39 @@ def test_oracle_smear_pins():
     assert one_fixed[0, 15] == pytest.approx(np.dot(line[15 - h:15 + h + 1], g) / g.sum(), rel=1e-13)
     one_compat = OH.smear(line[None, :], sig, compat=True)
     assert one_compat[0, 15] == pytest.approx(np.dot(line[15 - h:15 + h], gl) / gl.sum(), rel=1e-13)
+
+
+def _read_tiff16(path):
+    """Minimal reader of the driver's baseline TIFF: (width, height, uint16 pixels, tags)."""
+    import struct
+    b = Path(path).read_bytes()
+    assert b[:4] == b"II*\x00"
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    n = struct.unpack_from("<H", b, ifd)[0]
+    tags = {}
+    for k in range(n):
+        tag, typ, cnt, val = struct.unpack_from("<HHII", b, ifd + 2 + 12 * k)
+        tags[tag] = val & 0xFFFF if typ == 3 else val
+    w, h = tags[256], tags[257]
+    px = np.frombuffer(b, dtype="<u2", count=w * h, offset=tags[273]).reshape(h, w)
+    return w, h, px, tags
+
+
+def test_tiff_image_follows_the_reference_transform(tmp_path):
+    """Image::construct_image: shift to non-negative, log10 with zeros kept, min-max normalisation; here
+    16-bit grayscale. Closed form on (0, 1, 10, 100, 1000, 10): logs (0, 0, 1, 2, 3, 1) -> (0, 0, 1/3,
+    2/3, 1, 1/3)."""
+    img = np.array([[0.0, 1.0, 10.0], [100.0, 1000.0, 10.0]])
+    H.HigSimulation.save_tiff(img, tmp_path / "a.tif")
+    w, h, px, tags = _read_tiff16(tmp_path / "a.tif")
+    assert (w, h, tags[258], tags[259], tags[262], tags[277]) == (3, 2, 16, 1, 1, 1)
+    third = round(65535 / 3)
+    np.testing.assert_array_equal(px, [[0, 0, third], [round(2 * 65535 / 3), 65535, third]])
+    # a negative minimum is shifted away first: (-1, 0, 9) -> (0, 1, 10) -> logs (0, 0, 1)
+    H.HigSimulation.save_tiff(np.array([[-1.0, 0.0, 9.0]]), tmp_path / "b.tif")
+    np.testing.assert_array_equal(_read_tiff16(tmp_path / "b.tif")[2], [[0, 0, 65535]])
+    # a constant image maps to 1 (0 when its log is negative); non-finite pixels are refused
+    H.HigSimulation.save_tiff(np.full((2, 2), 5.0), tmp_path / "c.tif")
+    assert np.all(_read_tiff16(tmp_path / "c.tif")[2] == 65535)
+    with pytest.raises(Exception, match="non-finite"):
+        H.HigSimulation.save_tiff(np.array([[1.0, np.nan]]), tmp_path / "d.tif")
