@@ -176,6 +176,23 @@ class HigSimulation:
         self._check(self.lib.hgx_hig_kernel_ms(self.h, C.byref(m), C.byref(nm), C.byref(a), C.byref(na)), "kernel_ms")
         return {"main_ms": m.value, "main_launches": nm.value, "aux_ms": a.value, "aux_launches": na.value}
 
+    def run_all_gisaxs(self, out_dir: str) -> list[str]:
+        """run_all_gisaxs with its files (src/sim/hipgisaxs_main.cpp:436-472): for every configuration
+        img_ai=<a>_rot=<phi>_tilt=<t>.tif and gisaxs_ai=..._rot=..._tilt=....out, the angles printed as a
+        C++ stream prints them (6 significant digits), into out_dir (created). Returns the written paths."""
+        import os
+        cfg = self.configurations()
+        os.makedirs(out_dir, exist_ok=True)
+        written = []
+        for c, (a, p, t) in enumerate(cfg):
+            img = self.run(c)
+            tag = f"ai={a:g}_rot={p:g}_tilt={t:g}"
+            tif, out = Path(out_dir) / f"img_{tag}.tif", Path(out_dir) / f"gisaxs_{tag}.out"
+            self.save_tiff(img, str(tif))
+            self.save(img, str(out))
+            written += [str(tif), str(out)]
+        return written
+
     # ---- fitting (the analyzer's HipGISAXSObjectiveFunction and fit drivers) ----------------------
 
     def update_param(self, variable: str, value: float) -> None:

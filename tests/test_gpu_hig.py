@@ -187,3 +187,18 @@ def test_errors_are_reported_not_fatal():
     # the driver stays usable after a failed run
     ok = H.HigSimulation(text=_text("spheres_bcc")).compute()
     assert np.all(np.isfinite(ok))
+
+
+def test_run_all_writes_the_reference_files(tmp_path):
+    """run_all_gisaxs's files: one .out (the image, 17 digits) and one .tif per configuration, named by
+    the angles as the reference streams them."""
+    text = _text("two_structures")
+    sim = H.HigSimulation(text=text)
+    paths = sim.run_all_gisaxs(str(tmp_path / "run"))
+    names = sorted(Path(p).name for p in paths)
+    _say("run_all files", names)
+    assert len(names) == 8 and all(n.startswith(("img_ai=", "gisaxs_ai=")) for n in names)
+    outs = sorted(p for p in paths if p.endswith(".out"))
+    for c, p in enumerate(sorted(outs, key=lambda q: paths.index(q))):
+        img = np.loadtxt(p)
+        assert _rel(img, OH.run(text, config=c)) <= TOL64
