@@ -126,6 +126,7 @@ bool HipGISAXS::device() {
 }
 
 void HipGISAXS::release_device_objects() {
+  fc_key_.clear();
   if (gpu_ && gpu_->ok()) {
     for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
     for (int i = 0; i < 8; ++i)
@@ -239,6 +240,7 @@ bool HipGISAXS::ensure_grid(double alphai, double dnl_re, double dnl_im) {
            "hgx_qgrid_qspace"))
     return false;
   grid_ext_ = false;
+  ++grid_gen_;  // coefficients computed on an earlier grid are stale
   if (want_ext && !hgx(hgx_qz_extended(gpu_->get(), k0_, alphai, dnl_re, dnl_im), "hgx_qz_extended")) return false;
   grid_ok_ = true;
   if (!want_ext) return true;
@@ -502,9 +504,19 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     th.push_back(sub != in_.layers.end() ? sub->second.thickness : 0.0);
     d_fc = buffer(2, (size_t)nq * 16);
     if (!d_fc) return fail("device allocation failed");
-    if (!hgx(hgx_fresnel(gpu_->get(), (int)de.size(), de.data(), be.data(), th.data(), order, k0_, alphai, d_fc),
-             "hgx_fresnel"))
-      return false;
+    // the coefficients depend on the stack, k0, the incidence angle and the grid's rows: reuse them while
+    // none of these changed (structures of one run, successive fit evaluations)
+    std::vector<double> key = de;
+    key.insert(key.end(), be.begin(), be.end());
+    key.insert(key.end(), th.begin(), th.end());
+    key.insert(key.end(), {k0_, alphai, (double)nq, (double)grid_gen_, (double)(uintptr_t)d_fc});
+    if (key != fc_key_) {
+      fc_key_.clear();
+      if (!hgx(hgx_fresnel(gpu_->get(), (int)de.size(), de.data(), be.data(), th.data(), order, k0_, alphai, d_fc),
+               "hgx_fresnel"))
+        return false;
+      fc_key_ = key;
+    }
   }
   vec3 a, b, c;
   std::string err;

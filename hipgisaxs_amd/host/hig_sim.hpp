@@ -135,6 +135,8 @@ class HipGISAXS {
   int64_t npix_ = 0;
   // the grid currently on the device: real (nbranch 1) or extended for (alphai, dnl)
   bool grid_ok_ = false, grid_ext_ = false;
+  uint64_t grid_gen_ = 0;          // bumped at every grid rebuild
+  std::vector<double> fc_key_;     // inputs of the Fresnel coefficients now in buffer 2
   double grid_ai_ = 0.0, grid_dn_[2] = {0.0, 0.0};
   void* buf_[8] = {};
   size_t buf_bytes_[8] = {};
