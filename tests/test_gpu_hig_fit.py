@@ -195,3 +195,25 @@ def test_reference_region_overrides_the_output_region(tmp_path):
     _say("override", got, want)
     assert got == pytest.approx(want, rel=1e-9)
     assert sim.objective(TRUTH, "sqrt_c_l2") <= 1e-12
+
+
+def test_mask_file_weights_the_distance(fitdir, tmp_path):
+    """A referencedata mask (whitespace-separated weights, read_mask_data) weights every functor as the
+    oracle's mask does; a zero mask over the lower half changes the distance."""
+    import shutil
+    shutil.copy(fitdir / "fit_cylinders_ref.out", tmp_path / "fit_cylinders_ref.out")
+    mask = np.ones((24, 40))
+    mask[12:, :] = 0
+    mask[:3, :5] = 2
+    np.savetxt(tmp_path / "mask.txt", mask, fmt="%d")
+    text = FIT.replace('mask = ""', 'mask = "mask.txt"')
+    sim = H.HigSimulation(text=text, base_dir=str(tmp_path))
+    params, _, _ = OH.read_fitting(text)
+    O = OH.read_ascii_image(str(tmp_path / "fit_cylinders_ref.out"))
+    x = [6.0, 47.5]
+    for metric in ("sqrt_unit_l2", "cbrt_c_l2", "chi2"):
+        got = sim.objective(x, metric)
+        want = OH.objective(text, params, x, metric, O, mask=mask)
+        _say("mask", metric, got, want)
+        assert got == pytest.approx(want, rel=1e-9)
+        assert abs(got - _oracle_f(fitdir, x, metric)) > 1e-6 * abs(got)

@@ -153,3 +153,11 @@ def test_bruteforce_grid_accumulates_its_step():
     pts = OH.bruteforce_points([3, 40], [8, 60], [0.5, 2.5])
     assert len(pts) == 11 * 9 and pts[:2] == [[3, 40], [3, 42.5]]
     assert OH.bruteforce_points([1], [3], [-1]) == [[1], [2], [3]]  # no step: 1
+
+
+def test_mask_of_the_wrong_size_is_refused(tmp_path):
+    (tmp_path / "fit_cylinders_ref.out").write_text("\n".join(" ".join(["1.0"] * 40) for _ in range(24)) + "\n")
+    (tmp_path / "mask.txt").write_text(" ".join(["1"] * 100))
+    text = FIT.replace('mask = ""', 'mask = "mask.txt"')
+    with pytest.raises(HgxError, match="mask and reference data dimension sizes do not match"):
+        H.HigSimulation(text=text, base_dir=str(tmp_path)).fit_init()
