@@ -596,7 +596,7 @@ def main():
         pcfg = (pcfg[0], 4, "f16x2", args.row_tiles, not args.flat, not args.no_share)
     if args.blocks_per_unit is not None:
         pcfg = (args.blocks_per_unit,) + pcfg[1:]
-    tune_product(*pcfg)
+    tune_product(main_ctx, *pcfg)
     if args.mode == "batch" and args.tune and not args.no_tune:
         from hipgisaxs_amd.autotune import autotune_batch, autotune_product
 
@@ -732,7 +732,7 @@ def main():
         # the one-level accumulation variant (FP32 sums over a whole work unit): faster, less accurate
         from hipgisaxs_amd.autotune import PRODUCT_ONE_LEVEL
 
-        tune_product(*PRODUCT_ONE_LEVEL)
+        tune_product(main_ctx, *PRODUCT_ONE_LEVEL)
         b1 = Batch(main_ctx, items["fp32c"], precision="fp32c", flags=flags)
         wall_1, k1_ms, n1, _ = timed(lambda g: b1.run(use_graph=g), args.steps, 1, True)
         b1.release()
@@ -743,7 +743,7 @@ def main():
         result["fp32c_one_level"] = {"config": "x".join(map(str, PRODUCT_ONE_LEVEL)),
                                      "value": pairs * args.steps / wall_1, "ms_per_step": 1e3 * wall_1 / args.steps,
                                      "relL2_vs_fp64_gpu": math.sqrt(num1 / den1)}
-        tune_product(*pcfg)
+        tune_product(main_ctx, *pcfg)
         step32(False)  # the headline outputs again, for the oracle comparison below
         torch.cuda.synchronize()
     if not args.no_extra:

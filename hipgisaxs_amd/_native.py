@@ -27,6 +27,7 @@ FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOL
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
 SMEAR_COMPAT = 1
+HGX_DIST_REFERENCE = 1
 METRICS = {"chi2": 0, "l2": 1, "sqrt_unit_l2": 2, "sqrt_c_l2": 3, "cbrt_unit_l2": 4, "cbrt_c_l2": 5}
 SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}
 SHAPE_DIMS = {"sphere": 1, "cylinder": 2, "box": 3, "cube": 1, "prism3": 2, "prism6": 2, "pyramid": 4}
@@ -173,7 +174,7 @@ _SIGS = {
     "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "hgx_batch_release": (C.c_int, [_vp, C.c_int]),
-    "hgx_tune_batch": (C.c_int, [C.c_int, C.c_int]),
+    "hgx_tune_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_fit_plan_create": (C.c_int, [_vp, C.POINTER(FitModel), _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_fit_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_chi2_fwd": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double), _vp]),
@@ -183,11 +184,11 @@ _SIGS = {
     "hgx_fit_lbfgs": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int), _dp, _dp, C.c_int, C.c_double, _dp,
                                 C.POINTER(C.c_int)]),
     "hgx_param_distribution": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
-                                         _dp, _dp, C.c_int]),
-    "hgx_tune_numeric": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "hgx_tune_product": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "hgx_tune_product_tile": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "hgx_tune_adjoint": (C.c_int, [C.c_int, C.c_int]),
+                                         C.c_int, _dp, _dp, C.c_int]),
+    "hgx_tune_numeric": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "hgx_tune_product": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "hgx_tune_product_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "hgx_tune_adjoint": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_caxpy": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_double, _vp, C.c_int, _vp]),
     "hgx_axpy": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_int, _vp]),
     "hgx_gaussian_smear": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),

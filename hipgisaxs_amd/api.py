@@ -367,18 +367,19 @@ class Batch:
         self.ctx._check(self.ctx.lib.hgx_batch_release(self.ctx.h, self.id), "batch_release")
 
 
-def tune_batch(qpt: int, blocks_per_unit: int) -> None:
-    if N.load().hgx_tune_batch(qpt, blocks_per_unit) != N.HGX_OK:
+def tune_batch(ctx: "Context", qpt: int, blocks_per_unit: int) -> None:
+    """Work-unit shape of the context's later batches of per-pair kernels."""
+    if N.load().hgx_tune_batch(ctx.h, qpt, blocks_per_unit) != N.HGX_OK:
         raise HgxError("tune_batch: invalid knob values")
 
 
-# the library's own product-kernel setting (hgx_tune_product / hgx_tune_product_tile at load):
+# a new context's product-kernel setting (hgx_tune_product / hgx_tune_product_tile):
 # (blocks per unit, real products per complex product, operands, row tiles, two-level, shared rows)
 PRODUCT_DEFAULT = (64, 4, "f16x2", 2, True, True)
 
 
-def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "fp32", row_tiles: int = 2,
-                 two_level: bool = True, shared_rows: bool = True) -> None:
+def tune_product(ctx: "Context", blocks_per_unit: int, complex_mults: int = 3, operands: str = "fp32",
+                 row_tiles: int = 2, two_level: bool = True, shared_rows: bool = True) -> None:
     """Work-unit size (128-triangle blocks) of the product-grid matrix-core kernel, 3 or 4 real
     products per complex product, and the operand precision: "fp32" or "f16x2" (each operand split
     into two fp16 halves; 4 real products only). For "f16x2" also the 32-row tiles per wave (1, 2,
@@ -386,18 +387,19 @@ def tune_product(blocks_per_unit: int, complex_mults: int = 3, operands: str = "
     once per workgroup and shared (2 or 4 row tiles)."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
     lib = N.load()
-    if operands not in ops or lib.hgx_tune_product(blocks_per_unit, complex_mults, ops[operands]) != N.HGX_OK:
+    if operands not in ops or lib.hgx_tune_product(ctx.h, blocks_per_unit, complex_mults,
+                                                   ops[operands]) != N.HGX_OK:
         raise HgxError("tune_product: invalid knob values")
-    if operands == "f16x2" and lib.hgx_tune_product_tile(int(row_tiles), int(bool(two_level)),
+    if operands == "f16x2" and lib.hgx_tune_product_tile(ctx.h, int(row_tiles), int(bool(two_level)),
                                                          int(bool(shared_rows))) != N.HGX_OK:
         raise HgxError("tune_product: invalid row tiles / accumulation / sharing")
 
 
-def tune_adjoint(operands: str = "f16x2", col_tiles: int = 2) -> None:
+def tune_adjoint(ctx: "Context", operands: str = "f16x2", col_tiles: int = 2) -> None:
     """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default; 2 or 4
     32-column tiles per wave) or "fp32"."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
-    if operands not in ops or N.load().hgx_tune_adjoint(ops[operands], int(col_tiles)) != N.HGX_OK:
+    if operands not in ops or N.load().hgx_tune_adjoint(ctx.h, ops[operands], int(col_tiles)) != N.HGX_OK:
         raise HgxError("tune_adjoint: invalid operands / column tiles")
 
 
@@ -472,21 +474,25 @@ class FitPlan:
         self.ctx._check(self.ctx.lib.hgx_fit_plan_release(self.ctx.h, self.id), "fit_plan_release")
 
 
-def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0):
-    """AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284) through the C ABI."""
+def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0, reference=False):
+    """AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284) through the C ABI: exactly
+    ``nvalues`` nodes with weights normalised to 1, or (``reference=True``) the reference's
+    accumulating do-while with unnormalised weights."""
     lib = N.load()
     cap = max(4 * nvalues + 8, 16)
     v = np.empty(cap)
     w = np.empty(cap)
-    n = lib.hgx_param_distribution(pmin, pmax, nvalues, stat, mean, sd, v.ctypes.data_as(C.POINTER(C.c_double)),
+    n = lib.hgx_param_distribution(pmin, pmax, nvalues, stat, mean, sd, N.HGX_DIST_REFERENCE if reference else 0,
+                                   v.ctypes.data_as(C.POINTER(C.c_double)),
                                    w.ctypes.data_as(C.POINTER(C.c_double)), cap)
     if n < 0:
         raise HgxError(f"param_distribution: {N.ERRORS.get(n, n)}")
     return v[:n].copy(), w[:n].copy()
 
 
-def tune_numeric(qpt64: int, qpt32: int, target_wg: int) -> None:
-    rc = N.load().hgx_tune_numeric(qpt64, qpt32, target_wg)
+def tune_numeric(ctx: "Context", qpt64: int, qpt32: int, target_wg: int) -> None:
+    """Launch shape of the context's per-pair numeric kernels."""
+    rc = N.load().hgx_tune_numeric(ctx.h, qpt64, qpt32, target_wg)
     if rc != N.HGX_OK:
         raise HgxError("tune_numeric: invalid knob values")
 

@@ -41,27 +41,28 @@ def _time_batch(ctx: Context, items, precision, flags, reps):
     return (prof["main_ms"] + prof["aux_ms"]) / reps
 
 
-def _search(setter, default, candidates, run, margin):
+def _search(ctx, setter, default, candidates, run, margin):
     times = {}
     for cand in candidates:
-        setter(*cand)
+        setter(ctx, *cand)
         times[cand] = run()
     best = min(times, key=times.get)
     if default in times and times[best] > (1.0 - margin) * times[default]:
         best = default
-    setter(*best)
+    setter(ctx, *best)
     return best, times
 
 
 def autotune_batch(ctx: Context, items, precision: str = "fp32c", flags: int = 0,
                    candidates=VALU_CANDIDATES, reps: int = 1, margin: float = 0.02):
-    """Per-pair kernels: returns ((qpt, blocks_per_unit), {candidate: ms}), leaves the best active."""
-    return _search(tune_batch, VALU_DEFAULT, candidates,
+    """Per-pair kernels: returns ((qpt, blocks_per_unit), {candidate: ms}), leaves the best active
+    in the context."""
+    return _search(ctx, tune_batch, VALU_DEFAULT, candidates,
                    lambda: _time_batch(ctx, items, precision, flags, reps), margin)
 
 
 def autotune_product(ctx: Context, items, flags: int = 0, candidates=PRODUCT_CANDIDATES, reps: int = 1,
                      margin: float = 0.02):
     """Product-grid kernel (FP32C): returns (best candidate tuple as taken by tune_product, {candidate: ms})."""
-    return _search(tune_product, PRODUCT_DEFAULT, candidates,
+    return _search(ctx, tune_product, PRODUCT_DEFAULT, candidates,
                    lambda: _time_batch(ctx, items, "fp32c", flags, reps), margin)

@@ -659,9 +659,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
   }
 }
 
-static int g_adjoint_variant = 2;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
-
-int adjoint_product_variant() { return g_adjoint_variant; }
+int adjoint_product_variant(const Context* c) { return c->tune.adjoint_variant; }
 
 int64_t adjoint_product_units(const AdjItem& it, int variant) {
   if (variant > 0) return 3 * (int64_t)it.nblocks * ceil_div(it.ncol, kAdjTile * variant);
@@ -733,7 +731,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
                                                    : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
   // split-fp16 weights (FP32C with the fp16 matrix cores): column maxima, 12 B per q-point, column scales
-  const int variant = precision == HGX_PREC_FP32C ? adjoint_product_variant() : 0;
+  const int variant = precision == HGX_PREC_FP32C ? adjoint_product_variant(c) : 0;
   const bool f16 = variant > 0;
   const int nz = c->nbranch * c->nrow;
   const int nzo = (int)ceil_div(nz, 8);
@@ -818,9 +816,11 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   return HGX_OK;
 }
 
-extern "C" int hgx_tune_adjoint(int operands, int col_tiles) {
+extern "C" int hgx_tune_adjoint(hgx_ctx* h, int operands, int col_tiles) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
   if (operands != HGX_PRODUCT_FP32 && operands != HGX_PRODUCT_F16X2) return HGX_ERR_INVALID_ARG;
   if (operands == HGX_PRODUCT_F16X2 && col_tiles != 2 && col_tiles != 4) return HGX_ERR_INVALID_ARG;
-  g_adjoint_variant = operands == HGX_PRODUCT_F16X2 ? col_tiles : 0;
+  c->tune.adjoint_variant = operands == HGX_PRODUCT_F16X2 ? col_tiles : 0;
   return HGX_OK;
 }

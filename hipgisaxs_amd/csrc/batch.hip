@@ -115,9 +115,6 @@ using namespace hgx;
 
 namespace {
 
-int g_batch_qpt = 4;
-int g_batch_bpu = 16;  // triangle blocks per work unit
-
 using Batch = BatchState;
 
 std::vector<Batch>& batches(Context* c) { return c->batches; }
@@ -173,10 +170,12 @@ void free_batch_state(BatchState& b) {
 }
 }  // namespace hgx
 
-extern "C" int hgx_tune_batch(int qpt, int blocks_per_unit) {
+extern "C" int hgx_tune_batch(hgx_ctx* h, int qpt, int blocks_per_unit) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
   if ((qpt != 1 && qpt != 2 && qpt != 4 && qpt != 8) || blocks_per_unit < 1) return HGX_ERR_INVALID_ARG;
-  g_batch_qpt = qpt;
-  g_batch_bpu = blocks_per_unit;
+  c->tune.batch_qpt = qpt;
+  c->tune.batch_bpu = blocks_per_unit;
   return HGX_OK;
 }
 
@@ -190,9 +189,9 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   Batch b;
   b.precision = precision;
   b.flags = flags;
-  b.qpt = g_batch_qpt;
-  b.pmode = product_mode();
-  const int64_t bpu = g_batch_bpu;
+  b.qpt = c->tune.batch_qpt;
+  b.pmode = product_mode(c);
+  const int64_t bpu = c->tune.batch_bpu;
   std::vector<BatchItemDev> it(n);
   std::vector<ProductItem> pit(n);
   std::vector<char> is_prod(n, 0);
@@ -207,6 +206,8 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
       return set_error(c, HGX_ERR_BAD_HANDLE, "batch_create: item %d: unknown mesh %d", i, x.mesh_id);
     if (!x.ff) return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: item %d has no output", i);
     const Mesh& m = c->meshes[x.mesh_id];
+    b.grids.emplace_back(gc, gc->grid_gen);
+    b.meshes.emplace_back(x.mesh_id, m.serial);
     BatchItemDev& d = it[i];
     d.g = QView{gc->d_qx, gc->d_qy, gc->d_qz, gc->npix, gc->nq};
     d.R = make_rot(x.rot);
@@ -227,7 +228,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     if (prod) {
       // product grid: complex matrix products on the matrix cores (product_mfma.hip)
       is_prod[i] = 1;
-      d.bchunk = std::min<int64_t>(product_blocks_per_unit(), m.nblocks);
+      d.bchunk = std::min<int64_t>(product_blocks_per_unit(c), m.nblocks);
       ProductItem& pi = pit[i];
       pi.qx = gc->d_qx;
       pi.qy = gc->d_qy;
@@ -376,6 +377,15 @@ extern "C" int hgx_batch_run(hgx_ctx* h, int batch_id, int use_graph) {
     return set_error(c, HGX_ERR_BAD_HANDLE, "batch_run: unknown batch %d", batch_id);
   if (int rc = set_device(c)) return rc;
   Batch& b = B[batch_id];
+  // the work tables hold the device addresses of the grids and meshes of creation time
+  for (const auto& g : b.grids)
+    if (g.first->grid_gen != g.second)
+      return set_error(c, HGX_ERR_INVALID_ARG, "batch_run: a q-grid of batch %d was replaced or extended after "
+                       "the batch was created; create the batch again", batch_id);
+  for (const auto& m : b.meshes)
+    if (m.first >= (int)c->meshes.size() || !c->meshes[m.first].live || c->meshes[m.first].serial != m.second)
+      return set_error(c, HGX_ERR_INVALID_ARG, "batch_run: mesh %d of batch %d was released after the batch was "
+                       "created", m.first, batch_id);
   if (!use_graph) {
     {
       LaunchTimer lt(c, true);

@@ -82,16 +82,20 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   return HGX_OK;
 }
 
-// AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values by the
-// reference's accumulating do-while, weights none/uniform/gaussian (unnormalised pdf). Host-only
-// arithmetic; returns the node count or a negative error code.
+// AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284). Default: exactly nvalues
+// nodes on [pmin, pmax] with weights normalised to sum 1. HGX_DIST_REFERENCE: the reference's
+// accumulating do-while (nvalues +- 1 nodes) and unnormalised pdf weights. Host-only arithmetic;
+// returns the node count or a negative error code.
 extern "C" int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd,
-                                      double* vals, double* wts, int cap) {
+                                      int flags, double* vals, double* wts, int cap) {
   if (nvalues < 1 || !vals || !wts) return HGX_ERR_INVALID_ARG;
+  if (flags & ~HGX_DIST_REFERENCE) return HGX_ERR_INVALID_ARG;
+  if (stat < 0 || stat > 2) return HGX_ERR_INVALID_ARG;
+  const bool reference = (flags & HGX_DIST_REFERENCE) != 0;
   double pmx = pmax < pmin ? pmin : pmax;
   int n = 0;
   double node[4096];
-  if (nvalues > 1) {
+  if (nvalues > 1 && reference) {
     double step = std::fabs(pmx - pmin) / (nvalues - 1);
     double curr = pmin;
     do {
@@ -99,17 +103,30 @@ extern "C" int hgx_param_distribution(double pmin, double pmax, int nvalues, int
       node[n++] = curr;
       curr += step;
     } while (curr < pmax);
+  } else if (nvalues > 1) {
+    if (nvalues > 4096) return HGX_ERR_INVALID_ARG;
+    for (int k = 0; k < nvalues; ++k) node[n++] = pmin + (pmx - pmin) * k / (nvalues - 1);
   } else {
     node[n++] = pmin;
   }
-  if (stat < 0 || stat > 2) return HGX_ERR_INVALID_ARG;
   double m = mean;
   if (stat == 2 && !std::isfinite(m)) m = (node[0] + node[n - 1]) / 2;
+  double wsum = 0.0;
+  if (stat == 2 && !reference) {
+    for (int i = 0; i < n; ++i)
+      wsum += std::exp(-1.0 * std::pow((node[i] - m), 2) / (2 * std::pow(sd, 2))) / (std::sqrt(2 * kPi) * sd);
+    if (!(wsum > 0)) return HGX_ERR_INVALID_ARG;
+  }
   for (int i = 0; i < n && i < cap; ++i) {
     vals[i] = node[i];
-    if (stat == 0) wts[i] = i == 0 ? 1.0 : 0.0;
-    else if (stat == 1) wts[i] = 1.0;
-    else wts[i] = std::exp(-1.0 * std::pow((node[i] - m), 2) / (2 * std::pow(sd, 2))) / (std::sqrt(2 * kPi) * sd);
+    if (stat == 0) {
+      wts[i] = i == 0 ? 1.0 : 0.0;
+    } else if (stat == 1) {
+      wts[i] = reference ? 1.0 : 1.0 / n;
+    } else {
+      double pdf = std::exp(-1.0 * std::pow((node[i] - m), 2) / (2 * std::pow(sd, 2))) / (std::sqrt(2 * kPi) * sd);
+      wts[i] = reference ? pdf : pdf / wsum;
+    }
   }
   return n;
 }

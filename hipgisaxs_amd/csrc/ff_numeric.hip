@@ -242,14 +242,14 @@ static void launch_approx32(hipStream_t s, const Plan& p, QView g, Rot R, const 
 using namespace hgx;
 
 // Tuning knobs for the numeric kernels (set by the autotuner; defaults are measured choices).
-static int g_qpt64 = 2, g_qpt32 = 4, g_target_wg = 2048;
-
-extern "C" int hgx_tune_numeric(int qpt64, int qpt32, int target_wg) {
+extern "C" int hgx_tune_numeric(hgx_ctx* h, int qpt64, int qpt32, int target_wg) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
   auto ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8; };
   if (!ok(qpt64) || !ok(qpt32) || target_wg < 1) return HGX_ERR_INVALID_ARG;
-  g_qpt64 = qpt64;
-  g_qpt32 = qpt32;
-  g_target_wg = target_wg;
+  c->tune.qpt64 = qpt64;
+  c->tune.qpt32 = qpt32;
+  c->tune.target_wg = target_wg;
   return HGX_OK;
 }
 
@@ -292,14 +292,14 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     it.snmax = (float)m.snmax;
     it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
                       m.radius);
-    const int pmode = product_mode();
+    const int pmode = product_mode(c);
     const int64_t tiles = (int64_t)ceil_div(it.nz, product_zrows(pmode)) * ceil_div(it.ncol, product_ycols(pmode));
     // enough waves to fill the chip several times over, without splitting below one block
     const int64_t upw = product_is_shared(pmode) ? 1 : kProductWaves;  // work units per workgroup
-    int64_t want = std::max<int64_t>(1, (int64_t)(upw * g_target_wg + tiles - 1) / tiles);
+    int64_t want = std::max<int64_t>(1, (int64_t)(upw * c->tune.target_wg + tiles - 1) / tiles);
     want = std::min<int64_t>(want, m.nblocks);
-    // chunks accumulate in FP32: at most product_blocks_per_unit() blocks each
-    it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit());
+    // chunks accumulate in FP32: at most product_blocks_per_unit(c) blocks each
+    it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit(c));
     it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
     if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
     size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
@@ -336,9 +336,9 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
   if (method == HGX_FF_APPROX) {
     Plan p;
     if (precision == HGX_PREC_FP64)
-      p = make_plan(c->nq, m.ntri, kBlockTri, g_qpt64, g_target_wg);
+      p = make_plan(c->nq, m.ntri, kBlockTri, c->tune.qpt64, c->tune.target_wg);
     else
-      p = make_plan(c->nq, m.nblocks, 1, g_qpt32, g_target_wg);
+      p = make_plan(c->nq, m.nblocks, 1, c->tune.qpt32, c->tune.target_wg);
     double* part = nullptr;
     if (p.nchunks > 1) {
       if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;
@@ -377,7 +377,7 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
       return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: exact method needs a vertex (exact9) mesh");
     if (precision != HGX_PREC_FP64)
       return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric: exact method is FP64 only");
-    Plan p = make_plan(c->nq, m.ntri, 64, 1, g_target_wg);
+    Plan p = make_plan(c->nq, m.ntri, 64, 1, c->tune.target_wg);
     double* part = nullptr;
     if (p.nchunks > 1) {
       if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;

@@ -653,6 +653,7 @@ struct GratingPlan {
   double* d_spart = nullptr;   // statistics-pass partials [G][npb][6]
   double* d_mconst = nullptr;  // per-candidate metric constants [G][4]
   int metric = HGX_METRIC_CHI2;
+  uint64_t grid_gen = 0;  // the context's grid generation the device tables were built for
   ProductItem* d_pitems = nullptr;
   ProductUnit* d_punits = nullptr;
   AdjItem* d_aitems = nullptr;
@@ -749,6 +750,14 @@ GratingPlan* get_plan(Context* c, int id) {
   return P[id];
 }
 
+// A plan's tables and captured graphs hold the grid's device addresses and sizes of creation time.
+int check_grid(Context* c, const GratingPlan& p, const char* who) {
+  if (c->grid_gen != p.grid_gen)
+    return set_error(c, HGX_ERR_INVALID_ARG, "%s: the q-grid was replaced or extended after the plan was "
+                     "created; create the plan again", who);
+  return HGX_OK;
+}
+
 }  // namespace
 
 namespace hgx {
@@ -806,14 +815,14 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   pa.L = m.length;
   p.cq = !c->qz_real;
   const int nz = c->nbranch * c->nrow;
-  p.pmode = product_mode();
+  p.pmode = product_mode(c);
   const int nzt = (int)ceil_div(nz, product_zrows(p.pmode)), nyt_f = (int)ceil_div(c->ncol, product_ycols(p.pmode));
   const int64_t bchunk = std::min<int64_t>(kGChunkBlocks, gm.nblocks);
   p.nchunks = (int)((gm.nblocks + bchunk - 1) / bchunk);
   p.nyt = (int)ceil_div(c->ncol, 32);
   p.ntt = (int)(gm.nslots / 32);
   p.upi_fwd = nzt * nyt_f * p.nchunks;
-  p.adj_variant = adjoint_product_variant();
+  p.adj_variant = adjoint_product_variant(c);
   p.adj_f16 = p.adj_variant > 0;
   const int nzo = (int)ceil_div(nz, 8);
   {
@@ -915,6 +924,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
     free_plan(p);
     return set_error(c, HGX_ERR_DEVICE, "grating_plan_create: copy to the device failed: %s", hipGetErrorString(e));
   }
+  p.grid_gen = c->grid_gen;
   p.live = true;
   auto& P = gplans(c);
   for (size_t i = 0; i < P.size(); ++i)
@@ -987,6 +997,7 @@ extern "C" int hgx_grating_eval(hgx_ctx* h, int plan, int use_graph) {
   if (!c) return HGX_ERR_BAD_HANDLE;
   GratingPlan* p = get_plan(c, plan);
   if (!p) return set_error(c, HGX_ERR_BAD_HANDLE, "grating_eval: unknown plan %d", plan);
+  if (int rc = check_grid(c, *p, "grating_eval")) return rc;
   if (int rc = set_device(c)) return rc;
   if (!use_graph) {
     LaunchTimer lt(c, true);
@@ -1008,6 +1019,7 @@ extern "C" int hgx_grating_lbfgs(hgx_ctx* h, int plan, int iterations) {
   if (!c) return HGX_ERR_BAD_HANDLE;
   GratingPlan* p = get_plan(c, plan);
   if (!p || iterations < 0) return set_error(c, HGX_ERR_INVALID_ARG, "grating_lbfgs: bad plan or count");
+  if (int rc = check_grid(c, *p, "grating_lbfgs")) return rc;
   if (int rc = set_device(c)) return rc;
   if (!p->step_exec)
     if (int rc = capture(c, *p, true)) return rc;
@@ -1043,6 +1055,7 @@ extern "C" int hgx_grating_intensity(hgx_ctx* h, int plan, int cand, double* I) 
   GratingPlan* p = get_plan(c, plan);
   if (!p || !I || cand < 0 || cand >= p->ncand)
     return set_error(c, HGX_ERR_INVALID_ARG, "grating_intensity: bad plan, candidate or output");
+  if (int rc = check_grid(c, *p, "grating_intensity")) return rc;
   if (int rc = set_device(c)) return rc;
   k_grating_begin_eval<<<ceil_div(p->ncand, 64), 64, 0, c->stream>>>(p->d_state, p->ncand);
   group_eval(*p, c, c->stream, cand, 1, true, I);

@@ -1,5 +1,6 @@
 // hgx_context.hip -- context lifetime, q-grid construction on the device, mesh upload.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -72,7 +73,13 @@ static void free_mesh(Mesh& m) {
   m = Mesh();
 }
 
+uint64_t next_serial() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
+
 static void free_qgrid(Context* c) {
+  c->grid_gen = next_serial();
   if (c->d_qx) (void)hipFree(c->d_qx);
   if (c->d_qy) (void)hipFree(c->d_qy);
   if (c->d_qz) (void)hipFree(c->d_qz);
@@ -464,6 +471,7 @@ int hgx_qz_extended(hgx_ctx* h, double k0, double alpha_i, double dnl_re, double
   HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   HGX_CHECK_HIP(c, hipFree(c->d_qz));
   c->d_qz = nq;
+  c->grid_gen = next_serial();
   c->nbranch = 4;
   c->nq = 4 * c->npix;
   // kzi and kzf are real exactly when the layer is lossless and every kzf0^2 >= k0^2 dnl
@@ -697,6 +705,7 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
     return set_error(c, HGX_ERR_DEVICE, "mesh_upload: copy to the device failed: %s", hipGetErrorString(e));
   }
   m.live = true;
+  m.serial = next_serial();
   int id = -1;
   for (size_t i = 0; i < c->meshes.size(); ++i)
     if (!c->meshes[i].live) {

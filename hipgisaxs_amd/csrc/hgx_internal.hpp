@@ -21,8 +21,13 @@ constexpr int kSeriesOrder = 8;  // small-|q| moment series: terms k = 0..kSerie
 // number of moment components: sum_k 3 * (k+1)(k+2)/2
 constexpr int kMomentCount = 3 * 165;
 
+// Process-wide serial numbers: each q-grid generation and each uploaded mesh takes a new one, so
+// work prepared against a grid or a mesh (batches, grating plans) can tell that it was replaced.
+uint64_t next_serial();
+
 struct Mesh {
   bool live = false;
+  uint64_t serial = 0;    // next_serial() at upload
   int kind = 0;
   int64_t ntri = 0;       // triangles (after dropping degenerate ones)
   int64_t ntri_in = 0;    // triangles as uploaded
@@ -70,6 +75,9 @@ struct BatchState {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   double pairs = 0.0;
+  // what the work tables point into, with the serial each had at creation (checked before a run)
+  std::vector<std::pair<const struct Context*, uint64_t>> grids;
+  std::vector<std::pair<int, uint64_t>> meshes;
 };
 
 void free_batch_state(BatchState& b);
@@ -161,8 +169,21 @@ inline int product_zrows(int mode) {
 inline int product_ycols(int mode) { return product_is_shared(mode) ? kProductWaves * kProductTile : kProductTile; }
 
 
+// Launch-shape settings of the numeric kernels, per context (hgx_tune_*; the autotuner sets them).
+// Batches and grating plans copy what they need when they are created.
+struct Tuning {
+  int qpt64 = 2, qpt32 = 4, target_wg = 2048;  // per-pair kernels: q-points per thread, workgroup target
+  int batch_qpt = 4, batch_bpu = 16;            // batched per-pair kernels: q-points, blocks per unit
+  int product_bpu = 64;                         // product-grid kernels: triangle blocks per unit
+  int product_operands = HGX_PRODUCT_F16X2;
+  int product_mults = 4;
+  int product_mz = 2, product_two = 1, product_shared = 1;
+  int adjoint_variant = 2;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
+};
+
 struct Context {
   int device = 0;
+  Tuning tune;
   std::vector<FitPlan> fit_plans;
   std::vector<BatchState> batches;
   std::vector<GratingPlan*> gratings;
@@ -179,6 +200,7 @@ struct Context {
   double* d_qy = nullptr;   // [npix]
   double* d_qz = nullptr;   // [2*nq] interleaved complex
   double* d_alpha = nullptr;// [nrow]
+  uint64_t grid_gen = 0;    // next_serial() whenever the grid arrays are replaced or freed
   // meshes
   std::vector<Mesh> meshes;
   // scratch for partial sums
@@ -389,11 +411,11 @@ int64_t adjoint_pmax_count(int nz, int ncol);  // floats of pmax per item
 // work units per item of the product-grid adjoint for the current operand setting
 // adjoint kernel variant: 0 FP32 operands, 2 or 4 split fp16 with that many column tiles per wave
 int64_t adjoint_product_units(const AdjItem& it, int variant);
-int adjoint_product_variant();  // process-wide setting (hgx_tune_adjoint)
+int adjoint_product_variant(const Context* c);  // the context's setting (hgx_tune_adjoint)
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq, int variant);
 // Tuning: triangle blocks per work unit and the kernel variant (kProduct* below).
-int product_blocks_per_unit();
-int product_mode();
+int product_blocks_per_unit(const Context* c);
+int product_mode(const Context* c);
 
 }  // namespace hgx

@@ -186,8 +186,8 @@ __global__ __launch_bounds__(kDistWG) void k_distance_finish(int nb, bool consts
 // Pass 2: the per-pixel terms of each functor summed per block. chi^2: m (I - O)^2 / max(O, 1);
 // AbsoluteDifferenceSquare (:125-143): (m |O - I|)^2; unit-vector forms (:423-456, :543-575):
 // (m |r/|r| - u/|u||)^2; c-norm forms (:460-499, :579-616): (m |r - c u|)^2, with u and r the
-// transformed model and data. The square-root forms and L2 drop non-finite terms, the cube-root forms
-// keep them, as the reference's loops do.
+// transformed model and data. The square-root forms drop non-finite terms; L2, chi^2 and the cube-root
+// forms keep them (a NaN pixel makes the distance NaN), as the reference's loops do.
 template <int METRIC>
 __global__ __launch_bounds__(kDistWG) void k_distance_terms(int64_t n, const double* __restrict__ I,
                                                             const double* __restrict__ O,
@@ -207,8 +207,7 @@ __global__ __launch_bounds__(kDistWG) void k_distance_terms(int64_t n, const dou
     if (METRIC == HGX_METRIC_CHI2) {
       v[0] += w * (a - b) * (a - b) / fmax(b, 1.0);
     } else if (METRIC == HGX_METRIC_L2) {
-      double t = w * fabs(b - a);
-      t = isfinite(t) ? t : 0.0;
+      const double t = w * fabs(b - a);
       v[0] += t * t;
     } else if (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_CBRT_UNIT_L2) {
       constexpr int R = METRIC == HGX_METRIC_SQRT_UNIT_L2 ? 2 : 3;

@@ -28,19 +28,13 @@ namespace hgx {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-static int g_product_bpu = 64;
-static int g_product_operands = HGX_PRODUCT_F16X2;
-static int g_product_mults = 4;
-static int g_product_mz = 2;
-static int g_product_two = 1;
-static int g_product_shared = 1;
-
-int product_blocks_per_unit() { return g_product_bpu; }
-int product_mode() {
-  if (g_product_operands == HGX_PRODUCT_FP32) return g_product_mults == 3 ? kProductF32Gauss : kProductF32x4;
-  if (g_product_shared) return g_product_mz == 4 ? (g_product_two ? kProductF16x2S4 : kProductF16x2S4Flat) : kProductF16x2S2;
-  if (g_product_mz == 4) return g_product_two ? kProductF16x2Z4 : kProductF16x2Z4Flat;
-  return g_product_mz == 2 ? kProductF16x2Z2 : kProductF16x2;
+int product_blocks_per_unit(const Context* c) { return c->tune.product_bpu; }
+int product_mode(const Context* c) {
+  const Tuning& t = c->tune;
+  if (t.product_operands == HGX_PRODUCT_FP32) return t.product_mults == 3 ? kProductF32Gauss : kProductF32x4;
+  if (t.product_shared) return t.product_mz == 4 ? (t.product_two ? kProductF16x2S4 : kProductF16x2S4Flat) : kProductF16x2S2;
+  if (t.product_mz == 4) return t.product_two ? kProductF16x2Z4 : kProductF16x2Z4Flat;
+  return t.product_mz == 2 ? kProductF16x2Z2 : kProductF16x2;
 }
 
 __global__ __launch_bounds__(256) void k_rotate_records(const float* __restrict__ rec, const double* __restrict__ borig,
@@ -809,22 +803,26 @@ void product_launch_single(hipStream_t s, const ProductItem& item, bool cq, int 
 
 using namespace hgx;
 
-extern "C" int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands) {
+extern "C" int hgx_tune_product(hgx_ctx* h, int blocks_per_unit, int complex_mults, int operands) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
   if (blocks_per_unit < 1 || (complex_mults != 3 && complex_mults != 4)) return HGX_ERR_INVALID_ARG;
   if (operands != HGX_PRODUCT_FP32 && operands != HGX_PRODUCT_F16X2) return HGX_ERR_INVALID_ARG;
   if (operands == HGX_PRODUCT_F16X2 && complex_mults != 4) return HGX_ERR_INVALID_ARG;
-  g_product_bpu = blocks_per_unit;
-  g_product_operands = operands;
-  g_product_mults = complex_mults;
+  c->tune.product_bpu = blocks_per_unit;
+  c->tune.product_operands = operands;
+  c->tune.product_mults = complex_mults;
   return HGX_OK;
 }
 
-extern "C" int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows) {
+extern "C" int hgx_tune_product_tile(hgx_ctx* h, int row_tiles, int two_level, int shared_rows) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
   if (!(row_tiles == 1 || row_tiles == 2 || row_tiles == 4)) return HGX_ERR_INVALID_ARG;
   if (!two_level && row_tiles != 4) return HGX_ERR_INVALID_ARG;
   if (shared_rows && row_tiles == 1) return HGX_ERR_INVALID_ARG;
-  g_product_mz = row_tiles;
-  g_product_two = two_level ? 1 : 0;
-  g_product_shared = shared_rows ? 1 : 0;
+  c->tune.product_mz = row_tiles;
+  c->tune.product_two = two_level ? 1 : 0;
+  c->tune.product_shared = shared_rows ? 1 : 0;
   return HGX_OK;
 }

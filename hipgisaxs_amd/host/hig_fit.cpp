@@ -184,6 +184,10 @@ bool HigInput::update_param(const std::string& variable, double v) {
       if (k2 == "spacing" || k2 == "maxgrains") {
         const int ax = axis_of(k3);
         if (ax < 0) return bad(k3);
+        if (k2 == "maxgrains" && !(v >= 1 && v <= 1e6 && v == std::floor(v))) {
+          err_ = "maxgrains must be a positive integer up to 1e6 (" + variable + ")";
+          return false;
+        }
         (k2 == "spacing" ? st.spacing : st.maxgrains)[ax] = v;
       } else if (k2 == "orientations" || k2 == "distribution") {
         return true;  // ignored by the reference
@@ -491,6 +495,10 @@ bool run_bruteforce(Problem& P, std::vector<double>& best, double& fbest, std::s
   double count = 1.0;
   for (int j = 0; j < d; ++j) {
     const double st = P.step[j] < 0 ? 1.0 : P.step[j];
+    if (!(P.lo[j] <= P.hi[j])) {  // the grid would hold no point
+      err = "brute force needs min <= max for every fit parameter";
+      return false;
+    }
     if (!(st > 0)) {
       err = "brute force needs positive steps";
       return false;
@@ -758,90 +766,108 @@ void copy_str(const std::string& s, char* out, size_t cap) {
 extern "C" {
 
 int hgx_hig_update_param(hgx_hig* h, const char* variable, double value) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!variable) return fit_fail(h, "update_param: null variable");
-  if (!h->sim.update_param(variable, value)) return fit_fail(h, h->sim.error());
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!variable) return fit_fail(h, "update_param: null variable");
+    if (!h->sim.update_param(variable, value)) return fit_fail(h, h->sim.error());
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_info(hgx_hig* h, int* nparams, int* nalgorithms) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!nparams || !nalgorithms) return fit_fail(h, "fit_info: null output");
-  *nparams = (int)h->sim.input().fit_params.size();
-  *nalgorithms = (int)h->sim.input().algorithms.size();
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!nparams || !nalgorithms) return fit_fail(h, "fit_info: null output");
+    *nparams = (int)h->sim.input().fit_params.size();
+    *nalgorithms = (int)h->sim.input().algorithms.size();
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_param(hgx_hig* h, int i, char* key, size_t key_cap, double* range4) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  const auto& fp = h->sim.input().fit_params;
-  if (i < 0 || i >= (int)fp.size() || !range4) return fit_fail(h, "fit_param: bad index or output");
-  auto it = fp.begin();
-  std::advance(it, i);
-  copy_str(it->second.key, key, key_cap);
-  range4[0] = it->second.min;
-  range4[1] = it->second.max;
-  range4[2] = it->second.step;
-  range4[3] = it->second.init;
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    const auto& fp = h->sim.input().fit_params;
+    if (i < 0 || i >= (int)fp.size() || !range4) return fit_fail(h, "fit_param: bad index or output");
+    auto it = fp.begin();
+    std::advance(it, i);
+    copy_str(it->second.key, key, key_cap);
+    range4[0] = it->second.min;
+    range4[1] = it->second.max;
+    range4[2] = it->second.step;
+    range4[3] = it->second.init;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_algorithm(hgx_hig* h, int i, char* name, size_t name_cap, int* metric, double* tol_reg) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  const auto& al = h->sim.input().algorithms;
-  if (i < 0 || i >= (int)al.size() || !metric || !tol_reg) return fit_fail(h, "fit_algorithm: bad index or output");
-  copy_str(al[i].name, name, name_cap);
-  *metric = hig::metric_of(al[i].metric);
-  tol_reg[0] = al[i].tolerance;
-  tol_reg[1] = al[i].regularization;
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    const auto& al = h->sim.input().algorithms;
+    if (i < 0 || i >= (int)al.size() || !metric || !tol_reg) return fit_fail(h, "fit_algorithm: bad index or output");
+    copy_str(al[i].name, name, name_cap);
+    *metric = hig::metric_of(al[i].metric);
+    tol_reg[0] = al[i].tolerance;
+    tol_reg[1] = al[i].regularization;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_init(hgx_hig* h) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_objective(hgx_hig* h, int n, const double* x, int metric, double reg, double* value) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!x || !value || n != (int)h->sim.input().fit_params.size())
-    return fit_fail(h, "fit_objective: x must hold one value per fit parameter");
-  if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2) return fit_fail(h, "fit_objective: unknown metric");
-  if (!h->sim.fit_ready() && !h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
-  if (!h->sim.fit_objective(x, metric, reg, value)) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!x || !value || n != (int)h->sim.input().fit_params.size())
+      return fit_fail(h, "fit_objective: x must hold one value per fit parameter");
+    if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2) return fit_fail(h, "fit_objective: unknown metric");
+    if (!h->sim.fit_ready() && !h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
+    if (!h->sim.fit_objective(x, metric, reg, value)) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_run(hgx_hig* h, int algorithm, unsigned long long seed, double* x_best, double* value,
                     long long* evals) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!x_best || !value || !evals) return fit_fail(h, "fit_run: null output");
-  std::vector<double> best;
-  double f = 0.0;
-  int64_t ne = 0;
-  std::string err;
-  const bool ok = hig::fit_run(h->sim, algorithm, (uint64_t)seed, best, f, ne, err);
-  *evals = (long long)ne;
-  if (!ok) return fit_fail(h, err, HGX_ERR_INVALID_ARG);
-  std::copy(best.begin(), best.end(), x_best);
-  *value = f;
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!x_best || !value || !evals) return fit_fail(h, "fit_run: null output");
+    std::vector<double> best;
+    double f = 0.0;
+    int64_t ne = 0;
+    std::string err;
+    const bool ok = hig::fit_run(h->sim, algorithm, (uint64_t)seed, best, f, ne, err);
+    *evals = (long long)ne;
+    if (!ok) return fit_fail(h, err, HGX_ERR_INVALID_ARG);
+    if (best.size() != h->sim.input().fit_params.size())
+      return fit_fail(h, "fit_run: the algorithm evaluated no point", HGX_ERR_INVALID_ARG);
+    std::copy(best.begin(), best.end(), x_best);
+    *value = f;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_fit_history(hgx_hig* h, long long cap_rows, double* rows, long long* nrows) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!nrows) return fit_fail(h, "fit_history: null output");
-  auto& hist = h->sim.fit_history();
-  *nrows = (long long)hist.size();
-  if (!rows) return HGX_OK;
-  const size_t w = h->sim.input().fit_params.size() + 1;
-  const long long n = std::min<long long>(cap_rows, (long long)hist.size());
-  for (long long r = 0; r < n; ++r) {
-    if (hist[r].size() != w) return fit_fail(h, "fit_history: rows of another parameter count", HGX_ERR_INVALID_ARG);
-    std::copy(hist[r].begin(), hist[r].end(), rows + r * w);
-  }
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!nrows) return fit_fail(h, "fit_history: null output");
+    auto& hist = h->sim.fit_history();
+    *nrows = (long long)hist.size();
+    if (!rows) return HGX_OK;
+    const size_t w = h->sim.input().fit_params.size() + 1;
+    const long long n = std::min<long long>(cap_rows, (long long)hist.size());
+    for (long long r = 0; r < n; ++r) {
+      if (hist[r].size() != w) return fit_fail(h, "fit_history: rows of another parameter count", HGX_ERR_INVALID_ARG);
+      std::copy(hist[r].begin(), hist[r].end(), rows + r * w);
+    }
+    return HGX_OK;
+  });
 }
 
 }  // extern "C"

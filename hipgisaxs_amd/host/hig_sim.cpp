@@ -285,8 +285,11 @@ bool HipGISAXS::make_grains(const HigStructure& s, GlibcRandom& rng, double phi,
   if (s.distribution != "regular")
     return fail("structure '" + s.key + "': ensemble distribution '" + s.distribution +
                 "' is not supported (regular only; the reference's random placement reseeds rand() from the clock)");
+  for (double m : s.maxgrains)
+    if (!(m >= 1 && m <= 1e6 && m == std::floor(m)))
+      return fail("structure '" + s.key + "': maxgrains must be positive integers up to 1e6");
   const double ngd = s.maxgrains[0] * s.maxgrains[1] * s.maxgrains[2];
-  if (!(ngd >= 1) || ngd > 1000000) return fail("structure '" + s.key + "': grain count out of range (1 to 1e6)");
+  if (ngd > 1000000) return fail("structure '" + s.key + "': grain count out of range (1 to 1e6)");
   const int nd[3] = {(int)s.maxgrains[0], (int)s.maxgrains[1], (int)s.maxgrains[2]};
   const int ng = nd[0] * nd[1] * nd[2];
   // grain positions: cell_ of the reference is never set (zero), so the spacing alone separates grains
@@ -414,8 +417,9 @@ bool HipGISAXS::shape_params(const HigShape& sh, hgx_shape_params& p, std::vecto
     int cap = 4 * prm->nvalues + 8;
     store[2 * d].resize(cap);
     store[2 * d + 1].resize(cap);
+    // the reference's node grid and weights, so that an input gives the reference's image
     int n = hgx_param_distribution(prm->min, prm->max, prm->nvalues, prm->stat, prm->p1, prm->p2,
-                                   store[2 * d].data(), store[2 * d + 1].data(), cap);
+                                   HGX_DIST_REFERENCE, store[2 * d].data(), store[2 * d + 1].data(), cap);
     if (n < 1) return fail("shape '" + sh.key + "': bad distribution for parameter '" + prm->type + "'");
     if (n > HGX_MAX_NODES)
       return fail("shape '" + sh.key + "': parameter '" + prm->type + "' has more than " +
@@ -829,149 +833,181 @@ int hig_fail(hgx_hig* h, int code) {
 extern "C" {
 
 int hgx_hig_create(int device, hgx_hig** out) {
-  if (!out) return HGX_ERR_INVALID_ARG;
-  *out = nullptr;
-  hgx_hig* h = new (std::nothrow) hgx_hig(device);
-  if (!h) return HGX_ERR_OOM;
-  if (!h->sim.error().empty()) {
-    delete h;
-    return HGX_ERR_DEVICE;
-  }
-  *out = h;
-  return HGX_OK;
+  return hgx_hig_guard(nullptr, [&]() -> int {
+    if (!out) return HGX_ERR_INVALID_ARG;
+    *out = nullptr;
+    hgx_hig* h = new (std::nothrow) hgx_hig(device);
+    if (!h) return HGX_ERR_OOM;
+    if (!h->sim.error().empty()) {
+      delete h;
+      return HGX_ERR_DEVICE;
+    }
+    *out = h;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_destroy(hgx_hig* h) {
-  delete h;
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    delete h;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_load(hgx_hig* h, const char* path) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!path) return HGX_ERR_INVALID_ARG;
-  return h->sim.construct_input(path) ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!path) return HGX_ERR_INVALID_ARG;
+    return h->sim.construct_input(path) ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  });
 }
 
 int hgx_hig_load_text(hgx_hig* h, const char* text, const char* base_dir) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!text) return HGX_ERR_INVALID_ARG;
-  return h->sim.construct_input_text(text, base_dir ? base_dir : ".") ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!text) return HGX_ERR_INVALID_ARG;
+    return h->sim.construct_input_text(text, base_dir ? base_dir : ".") ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  });
 }
 
 int hgx_hig_options(hgx_hig* h, int precision, int flags, unsigned seed) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C) return HGX_ERR_INVALID_ARG;
-  h->sim.options(precision, flags, seed);
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C) return HGX_ERR_INVALID_ARG;
+    h->sim.options(precision, flags, seed);
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_init(hgx_hig* h) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  return h->sim.init() ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    return h->sim.init() ? HGX_OK : hig_fail(h, HGX_ERR_INVALID_ARG);
+  });
 }
 
 int hgx_hig_info(hgx_hig* h, int* ncol, int* nrow, int* nconfig, double* k0) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!h->sim.prepare()) return hig_fail(h, HGX_ERR_INVALID_ARG);
-  if (ncol) *ncol = (int)h->sim.ncols();
-  if (nrow) *nrow = (int)h->sim.nrows();
-  if (nconfig) *nconfig = (int)h->sim.configurations().size();
-  if (k0) *k0 = h->sim.k0();
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!h->sim.prepare()) return hig_fail(h, HGX_ERR_INVALID_ARG);
+    if (ncol) *ncol = (int)h->sim.ncols();
+    if (nrow) *nrow = (int)h->sim.nrows();
+    if (nconfig) *nconfig = (int)h->sim.configurations().size();
+    if (k0) *k0 = h->sim.k0();
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_config(hgx_hig* h, int config, double* alpha_i, double* phi, double* tilt) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  auto cf = h->sim.configurations();
-  if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
-  if (alpha_i) *alpha_i = cf[config][0];
-  if (phi) *phi = cf[config][1];
-  if (tilt) *tilt = cf[config][2];
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    auto cf = h->sim.configurations();
+    if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
+    if (alpha_i) *alpha_i = cf[config][0];
+    if (phi) *phi = cf[config][1];
+    if (tilt) *tilt = cf[config][2];
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_run(hgx_hig* h, int config, double* image) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!image) return HGX_ERR_INVALID_ARG;
-  auto cf = h->sim.configurations();
-  if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
-  std::vector<double> img;
-  if (!h->sim.run_gisaxs(cf[config][0], cf[config][1], cf[config][2], img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
-  std::memcpy(image, img.data(), img.size() * sizeof(double));
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!image) return HGX_ERR_INVALID_ARG;
+    auto cf = h->sim.configurations();
+    if (config < 0 || config >= (int)cf.size()) return HGX_ERR_INVALID_ARG;
+    std::vector<double> img;
+    if (!h->sim.run_gisaxs(cf[config][0], cf[config][1], cf[config][2], img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+    std::memcpy(image, img.data(), img.size() * sizeof(double));
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_compute(hgx_hig* h, double* image) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!image) return HGX_ERR_INVALID_ARG;
-  std::vector<double> img;
-  if (!h->sim.compute_gisaxs(img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
-  std::memcpy(image, img.data(), img.size() * sizeof(double));
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!image) return HGX_ERR_INVALID_ARG;
+    std::vector<double> img;
+    if (!h->sim.compute_gisaxs(img)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+    std::memcpy(image, img.data(), img.size() * sizeof(double));
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_grains(hgx_hig* h, const char* key, int cap, double* rot, double* center, double* weight, int* n) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (!key || !n) return HGX_ERR_INVALID_ARG;
-  std::vector<hig::HigGrain> g;
-  if (!h->sim.grains(key, g)) return hig_fail(h, HGX_ERR_INVALID_ARG);
-  *n = (int)g.size();
-  for (int i = 0; i < (int)g.size() && i < cap; ++i) {
-    if (rot) std::memcpy(rot + 9 * (size_t)i, g[i].rot.data(), 9 * sizeof(double));
-    if (center) std::memcpy(center + 3 * (size_t)i, g[i].center, 3 * sizeof(double));
-    if (weight) weight[i] = g[i].weight;
-  }
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!key || !n) return HGX_ERR_INVALID_ARG;
+    std::vector<hig::HigGrain> g;
+    if (!h->sim.grains(key, g)) return hig_fail(h, HGX_ERR_INVALID_ARG);
+    *n = (int)g.size();
+    for (int i = 0; i < (int)g.size() && i < cap; ++i) {
+      if (rot) std::memcpy(rot + 9 * (size_t)i, g[i].rot.data(), 9 * sizeof(double));
+      if (center) std::memcpy(center + 3 * (size_t)i, g[i].center, 3 * sizeof(double));
+      if (weight) weight[i] = g[i].weight;
+    }
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  if (counts) h->sim.last_counts(counts);
-  if (ms) *ms = h->sim.last_run_ms();
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (counts) h->sim.last_counts(counts);
+    if (ms) *ms = h->sim.last_run_ms();
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches) {
-  if (!h) return HGX_ERR_BAD_HANDLE;
-  double m, a;
-  int64_t nm, na;
-  h->sim.last_kernel_times(&m, &nm, &a, &na);
-  if (main_ms) *main_ms = m;
-  if (main_launches) *main_launches = nm;
-  if (aux_ms) *aux_ms = a;
-  if (aux_launches) *aux_launches = na;
-  return HGX_OK;
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    double m, a;
+    int64_t nm, na;
+    h->sim.last_kernel_times(&m, &nm, &a, &na);
+    if (main_ms) *main_ms = m;
+    if (main_launches) *main_launches = nm;
+    if (aux_ms) *aux_ms = a;
+    if (aux_launches) *aux_launches = na;
+    return HGX_OK;
+  });
 }
 
 int hgx_hig_save(const double* image, int ncol, int nrow, const char* path) {
-  if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
-  return hig::HipGISAXS::save_gisaxs(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+  return hgx_hig_guard(nullptr, [&]() -> int {
+    if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
+    return hig::HipGISAXS::save_gisaxs(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+  });
 }
 
 int hgx_hig_save_tiff(const double* image, int ncol, int nrow, const char* path) {
-  if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
-  return hig::HipGISAXS::save_tiff(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+  return hgx_hig_guard(nullptr, [&]() -> int {
+    if (!image || !path || ncol < 1 || nrow < 1) return HGX_ERR_INVALID_ARG;
+    return hig::HipGISAXS::save_tiff(image, (unsigned)ncol, (unsigned)nrow, path) ? HGX_OK : HGX_ERR_INVALID_ARG;
+  });
 }
 
 int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records) {
-  if (!path || !kind || !ntri) return HGX_ERR_INVALID_ARG;
-  std::string p(path), err;
-  std::vector<double> rec;
-  bool ok;
-  if (p.size() > 4 && p.compare(p.size() - 4, 4, ".obj") == 0) {
-    *kind = HGX_MESH_EXACT9;
-    ok = hig::read_obj_mesh(p, rec, err);
-  } else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".dat") == 0) {
-    *kind = HGX_MESH_APPROX8;
-    ok = hig::read_raw_mesh(p, rec, err);
-  } else {
-    return HGX_ERR_UNSUPPORTED;
-  }
-  if (!ok) return HGX_ERR_INVALID_ARG;
-  const size_t w = *kind == HGX_MESH_EXACT9 ? 9 : 8;
-  *ntri = (int64_t)(rec.size() / w);
-  if (records && cap >= *ntri) std::memcpy(records, rec.data(), rec.size() * sizeof(double));
-  return HGX_OK;
+  return hgx_hig_guard(nullptr, [&]() -> int {
+    if (!path || !kind || !ntri) return HGX_ERR_INVALID_ARG;
+    std::string p(path), err;
+    std::vector<double> rec;
+    bool ok;
+    if (p.size() > 4 && p.compare(p.size() - 4, 4, ".obj") == 0) {
+      *kind = HGX_MESH_EXACT9;
+      ok = hig::read_obj_mesh(p, rec, err);
+    } else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".dat") == 0) {
+      *kind = HGX_MESH_APPROX8;
+      ok = hig::read_raw_mesh(p, rec, err);
+    } else {
+      return HGX_ERR_UNSUPPORTED;
+    }
+    if (!ok) return HGX_ERR_INVALID_ARG;
+    const size_t w = *kind == HGX_MESH_EXACT9 ? 9 : 8;
+    *ntri = (int64_t)(rec.size() / w);
+    if (records && cap >= *ntri) std::memcpy(records, rec.data(), rec.size() * sizeof(double));
+    return HGX_OK;
+  });
 }
 
 const char* hgx_hig_last_error(const hgx_hig* h) { return h ? h->err.c_str() : "bad handle"; }

@@ -6,6 +6,8 @@
 
 #include <array>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <map>
 #include <memory>
 #include <string>
@@ -159,3 +161,21 @@ struct hgx_hig {
   std::string err;
   explicit hgx_hig(int device) : sim(device) {}
 };
+
+// Runs the body of a C entry point and turns any C++ exception into an error code, so that none
+// crosses the extern "C" boundary.
+template <class F>
+int hgx_hig_guard(hgx_hig* h, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    if (h) h->err = "out of host memory";
+    return HGX_ERR_OOM;
+  } catch (const std::exception& e) {
+    if (h) h->err = std::string("internal error: ") + e.what();
+    return HGX_ERR_INVALID_ARG;
+  } catch (...) {
+    if (h) h->err = "internal error";
+    return HGX_ERR_INVALID_ARG;
+  }
+}

@@ -229,17 +229,19 @@ typedef struct hgx_batch_item {
 
 /* Prepares a batch: cuts it into work units (item, q-tile, run of triangle blocks) of similar cost,
  * orders them longest first and allocates the partial-sum buffer once. No reference counterpart: the
- * reference evaluates one shape per call (src/ff/gpu/ff_tri_gpu.cu:395-521). The grids and meshes the
- * items name must stay unchanged while the batch lives. */
+ * reference evaluates one shape per call (src/ff/gpu/ff_tri_gpu.cu:395-521). The contexts whose grids
+ * the items name must outlive the batch; a grid replaced (any hgx_qgrid_* call) or extended
+ * (hgx_qz_extended) after creation, or a mesh released, makes hgx_batch_run refuse with
+ * HGX_ERR_INVALID_ARG before any launch: create the batch again. */
 int hgx_batch_create(hgx_ctx* ctx, int nitems, const hgx_batch_item* items, int precision, int flags, int* batch);
 /* Runs the batch: one launch over all work units plus one finishing launch. use_graph = 1 replays them
  * from a hipGraph captured on the first such call (no allocation, no host work per run). */
 int hgx_batch_run(hgx_ctx* ctx, int batch, int use_graph);
 int hgx_batch_info(hgx_ctx* ctx, int batch, int64_t* units, double* pairs);
 int hgx_batch_release(hgx_ctx* ctx, int batch);
-/* Work-unit shape of later batches: q-points per thread (1, 2, 4, 8) and triangle blocks (128
- * triangles each) per unit. Process-wide; set by the autotuner. */
-int hgx_tune_batch(int qpt, int blocks_per_unit);
+/* Work-unit shape of the context's later batches: q-points per thread (1, 2, 4, 8) and triangle
+ * blocks (128 triangles each) per unit. Set by the autotuner. */
+int hgx_tune_batch(hgx_ctx* ctx, int qpt, int blocks_per_unit);
 
 /* ---- fused grain-ensemble intensity ------------------------------------------------------- */
 
@@ -380,7 +382,10 @@ typedef struct hgx_grating_model {
  * (I_p - O_p)^2 / max(O_p, 1), leaving out pixels with |q_b| R < 0.1 on any branch (R: the line's
  * radius). The context's q-grid must be a product grid (FP32C on the matrix cores). fc [2*nq] or NULL
  * (Born), I_obs [npix] and mask [npix] (or NULL) are device arrays read by pointer. ncand candidates
- * are evaluated together; device memory is bounded by running them in groups. No reference
+ * are evaluated together; device memory is bounded by running them in groups. The plan keeps the
+ * grid of its creation: after a grid is replaced (hgx_qgrid_*) or extended (hgx_qz_extended),
+ * hgx_grating_eval, hgx_grating_lbfgs and hgx_grating_intensity refuse with HGX_ERR_INVALID_ARG
+ * before any launch; create the plan again. No reference
  * counterpart: the reference fits with TAO/PSO and finite-difference gradients
  * (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224, src/analyzer/hipgisaxs_fit_pso.cpp). */
 int hgx_grating_plan_create(hgx_ctx* ctx, const hgx_grating_model* model, int ncand, const double* fc,
@@ -437,45 +442,51 @@ int hgx_image_distance(hgx_ctx* ctx, int64_t n, int metric, const double* I, con
 
 /* ---- host helpers ------------------------------------------------------------------------- */
 
-/* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values of a shape
- * parameter by the reference's accumulating do-while over [pmin, pmax] with nvalues points (which
- * can yield nvalues +- 1 nodes), weights for stat 0 (none: one weight 1), 1 (uniform: 1 each),
- * 2 (gaussian: unnormalised pdf, mean replaced by the node midpoint when not finite). Writes at
- * most cap nodes; returns the node count or a negative error code. Host arithmetic only. */
-int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd, double* vals,
-                           double* wts, int cap);
+/* AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284): node values and weights of a
+ * shape parameter over [pmin, pmax] with nvalues points, weights for stat 0 (none: one weight 1),
+ * 1 (uniform), 2 (gaussian pdf of mean, sd; mean replaced by the node midpoint when not finite).
+ * Default (flags 0): exactly nvalues evenly spaced nodes, weights normalised to sum 1.
+ * HGX_DIST_REFERENCE: the reference's float-accumulating do-while (ff_ana.cpp:249-254, which can
+ * yield nvalues +- 1 nodes) and unnormalised weights (1 each, or the raw pdf, :271-272), as the
+ * .hig driver uses to reproduce the reference's images. Writes at most cap nodes; returns the node
+ * count or a negative error code. Host arithmetic only. */
+#define HGX_DIST_REFERENCE 1
+int hgx_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd, int flags,
+                           double* vals, double* wts, int cap);
 
-/* Launch-shape knobs of the numeric form-factor kernels (q-points per thread for the FP64 and
- * FP32C kernels, each 1, 2, 4 or 8, and the workgroup count the triangle split aims for).
- * Process-wide; set by the autotuner. No reference counterpart (the reference hard-codes its
- * tiles, include/common/parameters.hpp:37-48). */
-int hgx_tune_numeric(int qpt64, int qpt32, int target_wg);
+/* Launch-shape knobs of the context's numeric form-factor kernels (q-points per thread for the FP64
+ * and FP32C kernels, each 1, 2, 4 or 8, and the workgroup count the triangle split aims for). Set
+ * by the autotuner. No reference counterpart (the reference hard-codes its tiles,
+ * include/common/parameters.hpp:37-48). Every hgx_tune_* setting belongs to its context: contexts
+ * do not share them, and batches and grating plans keep the setting of their creation. */
+int hgx_tune_numeric(hgx_ctx* ctx, int qpt64, int qpt32, int target_wg);
 
 /* Operand precision of the product-grid kernel. */
 #define HGX_PRODUCT_FP32 0  /* FP32 operands on v_mfma_f32_32x32x2_f32 */
 #define HGX_PRODUCT_F16X2 1 /* each FP32 operand split into two fp16 halves on v_mfma_f32_32x32x16_f16 */
 
-/* Work-unit shape of the product-grid kernel (single calls and later batches): triangle blocks (128
- * triangles each) per unit, 3 (Gauss) or 4 real products per complex product (HGX_PRODUCT_F16X2
- * takes 4 only), and the operand precision. Default (64, 4, HGX_PRODUCT_F16X2). Process-wide; set by
- * the autotuner. The grating fit's forward pass keeps FP32 operands whatever is set. No reference
+/* Work-unit shape of the context's product-grid kernel (single calls, later batches and later
+ * grating plans, whose forward pass uses the same kernel): triangle blocks (128 triangles each) per
+ * unit, 3 (Gauss) or 4 real products per complex product (HGX_PRODUCT_F16X2 takes 4 only), and the
+ * operand precision. Default (64, 4, HGX_PRODUCT_F16X2). Set by the autotuner. No reference
  * counterpart. */
-int hgx_tune_product(int blocks_per_unit, int complex_mults, int operands);
+int hgx_tune_product(hgx_ctx* ctx, int blocks_per_unit, int complex_mults, int operands);
 
 /* Tile of the split-fp16 product-grid kernel. row_tiles: 32-row tiles per wave (1, 2 or 4); each
  * wave builds its column's operands once for all its rows. two_level = 1 adds each 128-triangle
  * block's sums to the work unit's sums after the block; 0 accumulates in place over the unit
  * (row_tiles 4 only: fewer registers, about 10x larger FP32 rounding error). shared_rows = 1 builds
  * the row operands once per workgroup of four waves (adjacent column tiles) and shares them through
- * LDS (row_tiles 2 or 4). Default (2, 1, 1). Process-wide; set by the
- * autotuner; applies to single calls and to batches created afterwards. No reference counterpart. */
-int hgx_tune_product_tile(int row_tiles, int two_level, int shared_rows);
+ * LDS (row_tiles 2 or 4). Default (2, 1, 1). Set by the autotuner; applies to the
+ * context's single calls and to batches and grating plans created afterwards. No reference
+ * counterpart. */
+int hgx_tune_product_tile(hgx_ctx* ctx, int row_tiles, int two_level, int shared_rows);
 
 /* Operands of the FP32C product-grid record adjoint (hgx_ff_numeric_adjoint and the grating fit):
  * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16) with col_tiles = 2 (default) or 4
  * 32-column tiles per wave, or HGX_PRODUCT_FP32 (v_mfma_f32_32x32x2_f32; col_tiles ignored).
- * Process-wide; read by each adjoint call and at grating plan creation. No reference counterpart. */
-int hgx_tune_adjoint(int operands, int col_tiles);
+ * Read by each adjoint call of the context and at grating plan creation. No reference counterpart. */
+int hgx_tune_adjoint(hgx_ctx* ctx, int operands, int col_tiles);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ def load():
             "hgxo_ff_exact": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i, _d, _vp]),
             "hgxo_ff_approx_adjoint": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_approx_as_written": (None, [_i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
-            "hgxo_param_distribution": (_i, [_d, _d, _i, _i, _d, _d, _vp, _vp, _i]),
+            "hgxo_param_distribution": (_i, [_d, _d, _i, _i, _d, _d, _i, _vp, _vp, _i]),
             "hgxo_ff_sphere": (None, [_i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_cylinder": (None, [_i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
             "hgxo_ff_box": (None, [_i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
@@ -183,10 +183,13 @@ def tri9_to_rec8(tri9) -> np.ndarray:
     return out[:n].copy()
 
 
-def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0):
+def param_distribution(pmin, pmax, nvalues, stat=0, mean=float("nan"), sd=1.0, reference=False):
+    """Node values and weights of a shape parameter: exactly ``nvalues`` nodes with normalised
+    weights, or (``reference=True``) the reference's accumulating do-while with unnormalised pdf."""
     cap = 4 * nvalues + 8
     v, w = np.empty(cap), np.empty(cap)
-    n = load().hgxo_param_distribution(pmin, pmax, nvalues, stat, mean, sd, _p(v), _p(w), cap)
+    n = load().hgxo_param_distribution(pmin, pmax, nvalues, stat, mean, sd, 1 if reference else 0, _p(v), _p(w),
+                                       cap)
     if n < 0:
         raise ValueError("param_distribution rejected its arguments")
     return v[:n].copy(), w[:n].copy()

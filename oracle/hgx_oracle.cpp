@@ -434,37 +434,49 @@ void hgxo_ff_exact(int64_t ntri, const double* tri, int64_t npix, int64_t nq, co
 
 // ---------------------------------------------------------------------------------------------
 // a10: AnalyticFormFactor::param_distribution (src/ff/ff_ana.cpp:235-284).
-// Nodes by the reference's float-accumulating do-while (:249-254), which can yield nvalues +- 1
-// nodes; weights: stat 0 (none) -> single weight 1, 1 (uniform) -> 1 each, 2 (gaussian) ->
-// unnormalised pdf exp(-(x-mean)^2 / 2sd^2) / (sqrt(2 pi) sd) (:271-272); a non-finite mean is
-// replaced by the midpoint of the nodes (:266-268). Returns the node count (or -1), writing at
-// most cap nodes.
-// Pinned by: tests (node grids against numpy for exact-step cases; pdf against scipy.stats).
-int hgxo_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd,
+// flags & 1 (reference form): nodes by the reference's float-accumulating do-while (:249-254),
+// which can yield nvalues +- 1 nodes; weights stat 0 (none) -> single weight 1, 1 (uniform) -> 1
+// each, 2 (gaussian) -> unnormalised pdf exp(-(x-mean)^2 / 2sd^2) / (sqrt(2 pi) sd) (:271-272); a
+// non-finite mean is replaced by the midpoint of the nodes (:266-268).
+// flags 0 (default): exactly nvalues nodes pmin + k (pmax - pmin) / (nvalues - 1) and the same
+// weights normalised to sum 1 (stat 1: 1/n each; stat 0 keeps the single weight 1).
+// Returns the node count (or -1), writing at most cap nodes.
+// Pinned by: tests (node grids against numpy linspace, the do-while count on known cases; pdf against
+// scipy.stats; normalisation).
+int hgxo_param_distribution(double pmin, double pmax, int nvalues, int stat, double mean, double sd, int flags,
                             double* vals, double* wts, int cap) {
   if (nvalues < 1) return -1;
   std::vector<double> dim, w;
   double pmx = pmax;
   if (pmx < pmin) pmx = pmin;
-  if (nvalues > 1) {
+  const bool reference = (flags & 1) != 0;
+  if (nvalues > 1 && reference) {
     double step = std::fabs(pmx - pmin) / (nvalues - 1);
     double curr = pmin;
     do {
       dim.push_back(curr);
       curr += step;
     } while (curr < pmax);
+  } else if (nvalues > 1) {
+    for (int k = 0; k < nvalues; ++k) dim.push_back(pmin + (pmx - pmin) * k / (nvalues - 1));
   } else {
     dim.push_back(pmin);
   }
   if (stat == 0) {
     w.push_back(1.0);
   } else if (stat == 1) {
-    w.assign(dim.size(), 1.0);
+    w.assign(dim.size(), reference ? 1.0 : 1.0 / dim.size());
   } else if (stat == 2) {
     double m = mean;
     if (!std::isfinite(m)) m = (dim[0] + dim[dim.size() - 1]) / 2;
     for (size_t i = 0; i < dim.size(); ++i)
       w.push_back(std::exp(-1.0 * std::pow((dim[i] - m), 2) / (2 * std::pow(sd, 2))) / (std::sqrt(2 * kPi) * sd));
+    if (!reference) {
+      double sum = 0.0;
+      for (double x : w) sum += x;
+      if (!(sum > 0)) return -1;
+      for (double& x : w) x /= sum;
+    }
   } else {
     return -1;
   }

@@ -362,7 +362,8 @@ def shape_dims(sh):
     dims = []
     for names in _DIMS[sh["name"]]:
         p = next(sh["params"][n] for n in names if n in sh["params"])
-        dims.append(param_distribution(p["min"], p["max"], int(p["nvalues"]), _STAT[p["stat"]], p["p1"], p["p2"]))
+        dims.append(param_distribution(p["min"], p["max"], int(p["nvalues"]), _STAT[p["stat"]], p["p1"], p["p2"],
+                                       reference=True))
     return dims
 
 

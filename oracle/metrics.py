@@ -1,8 +1,8 @@
 """Distance functors of the reference's fitting (include/analyzer/distance_functions.hpp) restated in
 numpy over an observed image O, a model image I and a mask m (test infrastructure only).
 
-norm_l2(x) = sqrt(sum m x^2) and vec_dot(a, b) = sum m |a b| with NaN terms dropped (:40-57); each
-distance sums (m |difference|)^2 with non-finite terms dropped."""
+norm_l2(x) = sqrt(sum m x^2) and vec_dot(a, b) = sum m |a b| with NaN terms dropped (:40-57); the
+square-root distances sum (m |difference|)^2 with non-finite terms dropped, L2 (:125-143) keeps them."""
 from __future__ import annotations
 
 import numpy as np
@@ -31,8 +31,9 @@ def chi2(O, I, m):
 
 
 def l2(O, I, m):
-    """AbsoluteDifferenceSquare (:125-143)."""
-    return _sumsq(m * np.abs(O - I))
+    """AbsoluteDifferenceSquare (:125-143): every term kept, so a NaN pixel gives NaN."""
+    t = m * np.abs(O - I)
+    return float(np.sum(t * t))
 
 
 def sqrt_unit_l2(O, I, m):

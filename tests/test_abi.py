@@ -72,8 +72,10 @@ def test_param_distribution_matches_oracle(oracle_mod):
     cases = [(3.0, 7.0, 21, 2, 5.0, 0.5), (1.0, 2.0, 7, 1, np.nan, 1.0), (4.0, 4.0, 1, 0, np.nan, 1.0),
              (0.1, 0.7, 13, 2, np.nan, 0.2), (10.0, 30.0, 5, 2, 20.0, 5.0)]
     for c in cases:
-        v1, w1 = param_distribution(*c)
-        v2, w2 = oracle_mod.param_distribution(*c)
-        assert v1.shape == v2.shape
-        assert np.array_equal(v1, v2)
-        assert np.allclose(w1, w2, rtol=0, atol=0) or np.array_equal(w1, w2)
+        for ref in (False, True):
+            v1, w1 = param_distribution(*c, reference=ref)
+            v2, w2 = oracle_mod.param_distribution(*c, reference=ref)
+            assert v1.shape == v2.shape
+            assert np.array_equal(v1, v2)
+            assert np.allclose(w1, w2, rtol=1e-15, atol=0)
+    assert param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)[0].size == 21

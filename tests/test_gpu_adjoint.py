@@ -101,7 +101,7 @@ def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated, opera
         Wd = torch.from_numpy(np.stack([W.real, W.imag], -1).ravel()).cuda()
         ref = _oracle_grad(O, tri, g, W, R)
         got64 = c.ff_numeric_adjoint(mid, Wd, "fp64", rot=R).cpu().numpy()
-        tune_adjoint(operands.split("/")[0], int(operands.split("/")[1]) if "/" in operands else 2)
+        tune_adjoint(c, operands.split("/")[0], int(operands.split("/")[1]) if "/" in operands else 2)
         got32 = c.ff_numeric_adjoint(mid, Wd, "fp32c", rot=R).cpu().numpy()
         e64, e32 = rel_l2(got64, ref), rel_l2(got32, ref)
         print(f"{ncol}x{nrow} dwba={dwba} rot={rotated} {operands}: fp64 {e64:.2e} fp32c {e32:.2e}")
@@ -109,5 +109,5 @@ def test_adjoint_fp32c_product_grid(oracle_mod, ncol, nrow, dwba, rotated, opera
         assert e64 <= 1e-10
         assert e32 <= 1e-5
     finally:
-        tune_adjoint("f16x2")
+        tune_adjoint(c, "f16x2")
         c.close()

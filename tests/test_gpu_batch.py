@@ -33,7 +33,7 @@ def test_batch_matches_oracle_and_single_calls(gpu_ctx, oracle_mod, qpt, bpu):
         tri = meshes.convex_polyhedron(rng, rng.uniform(3, 12, 3), int(rng.integers(100, 900)))
         shapes.append((tri, sides[k % 3], _rz(0.3 * k)))
     try:
-        tune_batch(qpt, bpu)
+        tune_batch(gpu_ctx, qpt, bpu)
         mids = [gpu_ctx.upload_mesh(t, "exact9") for t, _, _ in shapes]
         for prec, tol in (("fp64", 1e-10), ("fp32c", 1e-5)):
             outs = [torch.full((2 * s * s,), float("nan"), dtype=torch.float64, device="cuda") for _, s, _ in shapes]
@@ -54,6 +54,40 @@ def test_batch_matches_oracle_and_single_calls(gpu_ctx, oracle_mod, qpt, bpu):
         for m in mids:
             gpu_ctx.release_mesh(m)
     finally:
-        tune_batch(4, 16)
+        tune_batch(gpu_ctx, 4, 16)
         for c in grids.values():
             c.close()
+
+
+def test_batch_refuses_a_replaced_grid_or_released_mesh(gpu_ctx):
+    """A batch's work tables hold the device addresses of its grids and meshes: after a grid is
+    replaced or extended, or a mesh released, hgx_batch_run refuses before any launch."""
+    import torch
+
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Batch, HgxError
+
+    rng = np.random.default_rng(3)
+    gpu_ctx.qgrid_explicit(16, 16, -2.0, 0.0, 2.0, 2.0, 0.0, 1e3)
+    m0 = gpu_ctx.upload_mesh(meshes.convex_polyhedron(rng, rng.uniform(3, 12, 3), 200), "exact9")
+    m1 = gpu_ctx.upload_mesh(meshes.convex_polyhedron(rng, rng.uniform(3, 12, 3), 300), "exact9")
+    outs = [torch.empty(2 * 256, dtype=torch.float64, device="cuda") for _ in range(2)]
+    try:
+        b = Batch(gpu_ctx, [(None, m0, None, outs[0]), (None, m1, None, outs[1])], precision="fp64")
+        b.run()
+        torch.cuda.synchronize()
+        gpu_ctx.qz_extended(1e3, 0.0, 0j)
+        for g in (False, True):
+            with pytest.raises(HgxError, match="create the batch again"):
+                b.run(use_graph=g)
+        b.release()
+        outs = [torch.empty(2 * 4 * 256, dtype=torch.float64, device="cuda") for _ in range(2)]
+        b = Batch(gpu_ctx, [(None, m0, None, outs[0]), (None, m1, None, outs[1])], precision="fp64")
+        b.run()
+        torch.cuda.synchronize()
+        gpu_ctx.release_mesh(m1)
+        with pytest.raises(HgxError, match="was released"):
+            b.run()
+        b.release()
+    finally:
+        gpu_ctx.release_mesh(m0)

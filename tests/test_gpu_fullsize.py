@@ -30,17 +30,17 @@ def test_numeric_fp32c_full_size_grid():
         f64 = c.as_complex(c.ff_numeric(mid, "approx", "fp64")).cpu().numpy()
         fva = c.as_complex(c.ff_numeric(mid, "approx", "fp32c", flags=4)).cpu().numpy()
         try:
-            tune_product(16, 3, "fp32")
+            tune_product(c, 16, 3, "fp32")
             f32 = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
             f16 = {}
             for variant in ((16, 4, "f16x2", 1, True, False), (64, 4, "f16x2", 2, True, False),
                             (64, 4, "f16x2", 4, True, False), (64, 4, "f16x2", 4, False, False),
                             (64, 4, "f16x2", 2, True, True), (64, 4, "f16x2", 4, False, True),
                             (64, 4, "f16x2", 4, True, True)):
-                tune_product(*variant)
+                tune_product(c, *variant)
                 f16[variant] = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         finally:
-            tune_product(*PRODUCT_DEFAULT)
+            tune_product(c, *PRODUCT_DEFAULT)
         e_mfma, e_valu = rel_l2(f32, f64), rel_l2(fva, f64)
         e_16 = {"x".join(map(str, k)): rel_l2(v, f64) for k, v in f16.items()}
         print(f"1024^2 x 50k: mfma {e_mfma:.2e} valu {e_valu:.2e} split-fp16 mfma",

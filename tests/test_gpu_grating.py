@@ -310,3 +310,28 @@ def test_grating_distance_metrics(oracle_mod, metric):
         plan.release()
     finally:
         c.close()
+
+
+def test_grating_plan_refuses_a_replaced_grid(oracle_mod):
+    """A plan keeps the device addresses of its grid: after the grid is replaced (or extended) eval,
+    lbfgs and intensity refuse before any launch, with a message that says what to do."""
+    import torch
+
+    from hipgisaxs_amd.api import GratingPlan, HgxError
+
+    O = oracle_mod
+    c, g, fc, fc_d = _setup(O, 32, 24, 0j)
+    try:
+        obs_d = torch.ones(g.npix, dtype=torch.float64, device="cuda")
+        plan = GratingPlan(c, 1, obs_d, fc=fc_d, dn2=DN2, **MODEL)
+        plan.set_theta(THETA_TRUE[None, :])
+        plan.eval(use_graph=False)
+        torch.cuda.synchronize()
+        c.qgrid_explicit(32, 24, -1.0, 0.0, 1.0, 1.5, AI, K0)  # same size, new arrays
+        for call in (lambda: plan.eval(use_graph=False), lambda: plan.eval(use_graph=True),
+                     lambda: plan.lbfgs(1), lambda: plan.intensity(0)):
+            with pytest.raises(HgxError, match="create the plan again"):
+                call()
+        plan.release()
+    finally:
+        c.close()

@@ -71,6 +71,15 @@ def test_image_distance_matches_oracle_functors():
         # no mask: all weights 1
         assert api.image_distance(ctx, dI, dO, None, "sqrt_c_l2") == pytest.approx(OM.sqrt_c_l2(O, I, np.ones(n)),
                                                                                    rel=1e-11)
+    # a NaN model pixel: L2 and chi^2 keep the term (NaN, so the objective takes its 1e8 penalty), the
+    # square-root forms drop it, as the oracle's functors
+    I = np.array([1.5, np.nan, 2.0, 4.0])
+    O = np.array([1.0, 2.0, 3.0, 5.0])
+    dI, dO = (torch.tensor(a, dtype=torch.float64, device="cuda") for a in (I, O))
+    assert np.isnan(api.image_distance(ctx, dI, dO, None, "l2"))
+    assert np.isnan(api.image_distance(ctx, dI, dO, None, "chi2"))
+    assert api.image_distance(ctx, dI, dO, None, "sqrt_c_l2") == pytest.approx(OM.sqrt_c_l2(O, I, np.ones(4)),
+                                                                               rel=1e-11)
 
 
 @pytest.mark.parametrize("metric", ["sqrt_unit_l2", "sqrt_c_l2", "cbrt_unit_l2", "cbrt_c_l2", "chi2", "l2"])

@@ -129,13 +129,13 @@ def test_chunked_reduction_matches_single_pass(gpu_ctx, oracle_mod, pyramid_case
     ref = O.ff_approx(O.tri9_to_rec8(tri), g)
     try:
         for qpt64, qpt32, twg in ((1, 1, 1), (2, 4, 100000), (8, 8, 64)):
-            tune_numeric(qpt64, qpt32, twg)
+            tune_numeric(gpu_ctx, qpt64, qpt32, twg)
             a = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64"))
             b = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp32c"))
             assert rel_l2(a, ref) <= 1e-10
             assert rel_l2(b, ref) <= 1e-5
     finally:
-        tune_numeric(2, 4, 2048)
+        tune_numeric(gpu_ctx, 2, 4, 2048)
 
 
 def test_exact_fp64_box_and_pyramid(gpu_ctx, oracle_mod):

@@ -80,7 +80,7 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
     R = _rot(0.3, -0.7, 1.1) if rotated else None
     c = Context(0)
     try:
-        tune_product(*variant)
+        tune_product(c, *variant)
         g = _grid(c, O, ncol, nrow, dwba)
         assert c.is_product()
         mid = c.upload_mesh(tri, "exact9")
@@ -106,7 +106,7 @@ def test_product_fp32c_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotate
         assert e_valu <= 1e-5
         c.release_mesh(mid)
     finally:
-        tune_product(*PRODUCT_DEFAULT)
+        tune_product(c, *PRODUCT_DEFAULT)
         c.close()
 
 
@@ -123,10 +123,10 @@ def test_product_chunked_matches_single_chunk(oracle_mod, operands):
     try:
         g = _grid(c, O, 96, 64)
         mid = c.upload_mesh(tri, "exact9")
-        tune_product(16, 4 if operands == "f16x2" else 3, operands)
-        tune_numeric(2, 4, 1)  # few waves wanted: one chunk
+        tune_product(c, 16, 4 if operands == "f16x2" else 3, operands)
+        tune_numeric(c, 2, 4, 1)  # few waves wanted: one chunk
         one = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
-        tune_numeric(2, 4, 1 << 16)  # many waves wanted: one block per chunk
+        tune_numeric(c, 2, 4, 1 << 16)  # many waves wanted: one block per chunk
         many = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
         ref = O.ff_approx(O.tri9_to_rec8(tri), g)
         e1, em, d = rel_l2(one, ref), rel_l2(many, ref), rel_l2(one, many)
@@ -135,8 +135,8 @@ def test_product_chunked_matches_single_chunk(oracle_mod, operands):
         # the two differ only in how many blocks one FP32 accumulation spans
         assert d <= 5e-6
     finally:
-        tune_numeric(2, 4, 2048)
-        tune_product(*PRODUCT_DEFAULT)
+        tune_numeric(c, 2, 4, 2048)
+        tune_product(c, *PRODUCT_DEFAULT)
         c.close()
 
 
@@ -165,7 +165,7 @@ def test_product_batch_mixed_with_valu_items(oracle_mod, operands):
             grid, ref = (gp, ref_p) if k % 2 == 0 else (gd, ref_d)
             shapes.append((tri, R, grid, ref))
         mids = [main.upload_mesh(t, "exact9") for t, _, _, _ in shapes]
-        tune_product(16, 4 if operands == "f16x2" else 3, operands)  # read when the batch is made
+        tune_product(main, 16, 4 if operands == "f16x2" else 3, operands)  # read when the batch is made
         outs = [torch.full((2 * gr.info()["nq"],), float("nan"), dtype=torch.float64, device="cuda")
                 for _, _, gr, _ in shapes]
         b = Batch(main, [(gr, m, R, o) for (t, R, gr, _), m, o in zip(shapes, mids, outs)], precision="fp32c")
@@ -179,7 +179,7 @@ def test_product_batch_mixed_with_valu_items(oracle_mod, operands):
                 assert rel_l2(main.as_complex(o).cpu().numpy(), want) <= 1e-5, use_graph
         b.release()
     finally:
-        tune_product(*PRODUCT_DEFAULT)
+        tune_product(main, *PRODUCT_DEFAULT)
         for c in (main, gp, gd):
             c.close()
 
@@ -218,12 +218,12 @@ def test_product_f16x2_scaling_edges(oracle_mod, case):
         ok = qn * info["radius"] >= 1e-6
         errs = {}
         for variant in PRODUCT_VARIANTS[1:]:
-            tune_product(*variant)
+            tune_product(c, *variant)
             got = c.as_complex(c.ff_numeric(mid, "approx", "fp32c")).cpu().numpy()
             assert np.all(np.isfinite(got))
             errs["x".join(map(str, variant))] = rel_l2(got[ok], ref[ok])
         print(case, {k: f"{v:.2e}" for k, v in errs.items()})
         assert max(errs.values()) <= 1e-5
     finally:
-        tune_product(*PRODUCT_DEFAULT)
+        tune_product(c, *PRODUCT_DEFAULT)
         c.close()

@@ -161,3 +161,17 @@ def test_mask_of_the_wrong_size_is_refused(tmp_path):
     text = FIT.replace('mask = ""', 'mask = "mask.txt"')
     with pytest.raises(HgxError, match="mask and reference data dimension sizes do not match"):
         H.HigSimulation(text=text, base_dir=str(tmp_path)).fit_init()
+
+
+@pytest.mark.parametrize("bad", [-2.0, 0.0, 1.5, 2e6])
+def test_maxgrains_update_out_of_range_is_refused(bad):
+    """maxgrains takes the parser's rule (an integer from 1 to 1e6 per axis) in update_param too: a
+    negative pair of axes used to pass the product check and abort the process in make_grains."""
+    sim = H.HigSimulation(text=SAMPLER)
+    with pytest.raises(HgxError, match="maxgrains"):
+        sim.update_param("structure['a']:ensemble:maxgrains:x", bad)
+    with pytest.raises(HgxError, match="maxgrains"):
+        sim.update_param("structure['a']:ensemble:maxgrains:y", -1.0)
+    # the model is unchanged and still samples its grains
+    rot, cen, w = sim.grains("a")
+    assert rot.shape[0] == 4

@@ -125,14 +125,35 @@ def test_translation_phase(oracle_mod):
 
 
 def test_param_distribution_reference_grid(oracle_mod):
-    """Node values follow the reference's accumulating do-while; gaussian weights are the pdf."""
+    """Reference form: node values follow the accumulating do-while; gaussian weights are the pdf."""
     O = oracle_mod
-    v, w = O.param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)
+    v, w = O.param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5, reference=True)
     # the accumulating do-while can drop or add the last node (here it stops at 20 nodes)
     assert 20 <= v.size <= 22
     assert np.allclose(v, 3.0 + 0.2 * np.arange(v.size), atol=1e-12)
     from scipy.stats import norm
 
     assert np.allclose(w, norm.pdf(v, 5.0, 0.5), rtol=1e-13)
+    v, w = O.param_distribution(2.0, 2.0, 1, 0, reference=True)
+    assert v.tolist() == [2.0] and w.tolist() == [1.0]
+
+
+def test_param_distribution_exact_grid(oracle_mod):
+    """Default form: exactly nvalues nodes (numpy linspace) with weights normalised to 1; the C2
+    size distribution N(5, 0.5) is 21 nodes on [3, 7], symmetric about 5."""
+    O = oracle_mod
+    from scipy.stats import norm
+
+    v, w = O.param_distribution(3.0, 7.0, 21, 2, 5.0, 0.5)
+    assert v.size == 21
+    assert np.allclose(v, np.linspace(3.0, 7.0, 21), rtol=0, atol=1e-14)
+    ref = norm.pdf(v, 5.0, 0.5)
+    assert np.allclose(w, ref / ref.sum(), rtol=1e-13)
+    assert abs(w.sum() - 1) < 1e-14 and np.allclose(w, w[::-1], rtol=1e-13)
+    assert abs((w * v).sum() - 5.0) < 1e-12  # symmetric grid: the weighted mean is the mean
+    for n in (2, 3, 7, 13, 40):
+        v, w = O.param_distribution(0.1, 0.7, n, 1)
+        assert v.size == n and np.allclose(v, np.linspace(0.1, 0.7, n), atol=1e-15)
+        assert np.allclose(w, 1.0 / n, rtol=1e-15)
     v, w = O.param_distribution(2.0, 2.0, 1, 0)
     assert v.tolist() == [2.0] and w.tolist() == [1.0]

@@ -63,3 +63,18 @@ def test_cube_root_forms(images):
     assert abs(M.cbrt_unit_l2(O, I, m) - M.sqrt_unit_l2(O, I, m)) > 1e-6
     # (-8, 1) against (8, 1): cube roots (-2, 1) and (2, 1), unit vectors apart by 4 * (2/sqrt 5)^2
     assert M.cbrt_unit_l2(np.array([-8.0, 1.0]), np.array([8.0, 1.0]), np.ones(2)) == pytest.approx(16 / 5)
+
+
+def test_l2_keeps_non_finite_terms_and_the_objective_penalises_them():
+    """AbsoluteDifferenceSquare (distance_functions.hpp:125-143) sums every term, so a NaN model pixel
+    makes the distance NaN, and the fit objective then takes the 1e8 penalty
+    (objective_func_hipgisaxs.cpp:297-312); the square-root forms drop the pixel instead."""
+    O = np.array([1.0, 2.0, 3.0])
+    I = np.array([1.5, np.nan, 2.0])
+    m = np.ones(3)
+    assert np.isnan(M.l2(O, I, m))
+    assert np.isfinite(M.sqrt_c_l2(O, I, m))
+    import math
+
+    d = M.l2(O, I, m)
+    assert (d if math.isfinite(d) else 1e8) == 1e8

@@ -169,3 +169,133 @@ def test_dwba_reduces_to_born_and_is_linear(oracle_mod):
     amp = lambda f: np.sqrt(O.dwba_intensity(npix, 4, f, sf, ff, 1.0))
     A = (fc2.reshape(4, -1) * sf.reshape(4, -1) * ff.reshape(4, -1)).sum(0)
     assert np.allclose(amp(fc2), np.abs(A), rtol=1e-13)
+
+
+@pytest.mark.parametrize("delta,beta", [(9.4e-6, 3.37e-8), (3.0e-6, 2.5e-7), (4.88e-6, 0.0)])
+def test_qz_extended_in_a_layer_is_snell_and_decays(oracle_mod, delta, beta):
+    """Particle in a layer of index n = 1 - delta + i beta (dnl = 1 - n^2, formed here from n, not
+    from 2 delta + 2 i beta). The branches must hold the in-layer incoming and outgoing vertical
+    wavevectors: kzi from b2 - b0, kzf from b0 - b1, with
+      * b0 + b3 = 0, b1 + b2 = 0 (the four sign pairs of src/model/qgrid.cpp:121-124);
+      * dispersion in the layer (Snell: the in-plane component k0 cos(alpha) is conserved),
+        kzi^2 + (k0 cos ai)^2 = k0^2 n^2 and kzf^2 + (k0 cos af)^2 = k0^2 n^2 with
+        sin af = qz / k0 - sin ai;
+      * the incoming wave travels down (Re kzi < 0) and decays downward in an absorbing layer
+        (Im kzi < 0: |e^{i kzi z}| -> 0 as z -> -inf); the outgoing wave keeps the direction of the
+        vacuum one, sgn(Re kzf) = sgn(qz - k0 sin ai), and decays along it (Im kzf Re kzf >= 0).
+    A dropped conjugate, a sign of dnl or a swapped branch fails one of these."""
+    O = oracle_mod
+    k0, ai = 50.677, np.radians(0.2)
+    g = O.qgrid_explicit(9, 11, -1.0, -0.1, 1.0, 2.0, ai, k0)  # qz rows on both sides of k0 sin ai
+    n = 1 - delta + 1j * beta
+    dnl = 1 - n * n
+    e = O.qz_extended(g, k0, ai, dnl).qz.reshape(4, -1)
+    assert np.allclose(e[0] + e[3], 0, atol=1e-12) and np.allclose(e[1] + e[2], 0, atol=1e-12)
+    kzi = (e[2] - e[0]) / 2
+    kzf = (e[0] - e[1]) / 2
+    qz = g.qz.real
+    kzf0 = qz - k0 * np.sin(ai)
+    cos_af2 = 1 - (qz / k0 - np.sin(ai)) ** 2
+    scale = k0 * k0
+    assert np.allclose(kzi ** 2 + k0 ** 2 * np.cos(ai) ** 2, k0 ** 2 * n * n, rtol=0, atol=1e-11 * scale)
+    assert np.allclose(kzf ** 2 + k0 ** 2 * cos_af2, k0 ** 2 * n * n, rtol=0, atol=1e-11 * scale)
+    assert np.all(kzi.real < 0)
+    sgn = np.where(kzf0 < 0, -1, 1)
+    prop = np.abs(kzf.real) > 1e-9 * k0  # without absorption, exit angles below the layer's critical
+    assert np.all(np.sign(kzf.real[prop]) == sgn[prop])  # angle give an evanescent kzf (Re = 0)
+    if beta == 0:  # (its Im sign then sits on the square root's branch cut: signed zero decides)
+        assert np.any(~prop) and np.any(prop)
+    else:
+        assert np.all(kzi.imag < 0)
+        assert np.all(kzf.imag * kzf.real > 0)
+    # the vacuum limit: n = 1 gives back the real branches
+    e0 = O.qz_extended(g, k0, ai, 0j).qz.reshape(4, -1)
+    assert np.allclose(e0[0], qz, atol=1e-12)
+
+
+def _stack_amplitudes(k0, alpha, n2s, z_if):
+    """Field amplitudes of a layer stack solved from the boundary conditions, independently of any
+    recursion. Media j = 0 (vacuum) .. M-1 (substrate) with 1 - n_j^2 = n2s[j]; interface j (between
+    media j and j + 1) at depth z_if[j]. In medium j the field is T_j e^{-i k_j z} + R_j e^{+i k_j z}
+    with k_j = -k0 sqrt(sin^2 alpha - n2s[j]) (the convention of src/model/multilayer.cpp:80-100);
+    E and dE/dz are continuous at each interface, the substrate has no R, and T_0 = 1."""
+    M = len(n2s)
+    k = -k0 * np.sqrt(np.sin(alpha) ** 2 - np.asarray(n2s, complex))
+    # unknowns: R_0, (T_j, R_j) for 0 < j < M-1, T_{M-1}
+    idx = {}
+    cols = 0
+    for j in range(M):
+        if j > 0:
+            idx[("T", j)] = cols
+            cols += 1
+        if j < M - 1:
+            idx[("R", j)] = cols
+            cols += 1
+    A = np.zeros((2 * (M - 1), cols), complex)
+    b = np.zeros(2 * (M - 1), complex)
+    for i, z in enumerate(z_if):
+        for side, j in ((1.0, i), (-1.0, i + 1)):
+            for kind, ph, dsign in (("T", np.exp(-1j * k[j] * z), -1.0), ("R", np.exp(1j * k[j] * z), 1.0)):
+                val_e = side * ph
+                val_d = side * dsign * 1j * k[j] * ph
+                if (kind, j) in idx:
+                    A[2 * i, idx[(kind, j)]] += val_e
+                    A[2 * i + 1, idx[(kind, j)]] += val_d
+                elif kind == "T" and j == 0:  # T_0 = 1 moves to the right-hand side
+                    b[2 * i] -= val_e
+                    b[2 * i + 1] -= val_d
+    x = np.linalg.solve(A, b)
+    T = np.array([1.0 + 0j] + [x[idx[("T", j)]] for j in range(1, M)])
+    R = np.array([x[idx[("R", j)]] for j in range(M - 1)] + [0j])
+    return T, R
+
+
+@pytest.mark.parametrize("beta1", [0.0, 4e-8])
+@pytest.mark.parametrize("order", [0, 1])
+def test_fresnel_film_amplitudes_solve_the_boundary_conditions(oracle_mod, order, beta1):
+    """Particle inside a film (order 1) and above it (order 0): T_order, R_order of the oracle equal
+    the amplitudes obtained by solving the continuity conditions of E and dE/dz at both interfaces
+    (T_0 = 1, no upward wave in the substrate), at angles below and above both critical angles,
+    with and without absorption in the film."""
+    O = oracle_mod
+    k0, t = 50.677, 25.0
+    d1, d2, b2 = 6.0e-6, 4.88e-6, 7.37e-8
+    ai = np.radians(0.23)
+    af = np.radians(np.array([0.05, 0.12, 0.18, 0.21, 0.35, 0.6]))
+    if beta1 == 0:
+        # without absorption, below the film's critical angle its k is imaginary and sits on the
+        # square root's branch cut, where the names T and R of the two film waves swap with the
+        # sign of a zero imaginary part; the physics is the same, the labels are not
+        af = af[af > np.sqrt(2 * d1) * 1.01]
+    g = O.Grid(1, af.size, np.zeros(af.size), np.zeros(af.size), np.zeros(af.size) + 0j, af)
+    fc = O.fresnel(g, [d1, d2], [beta1, b2], [t, 0.0], order, k0, ai).reshape(4, -1)
+    n2s = [0.0, 2 * d1 + 2j * beta1, 2 * d2 + 2j * b2]
+    z_if = [0.0, -t]
+    Ti, Ri = (v[order] for v in _stack_amplitudes(k0, ai, n2s, z_if))
+    for c, a in enumerate(af):
+        Tf, Rf = (v[order] for v in _stack_amplitudes(k0, a, n2s, z_if))
+        ref = [Ti * np.conj(Tf), Ti * np.conj(Rf), Ri * np.conj(Tf), Ri * np.conj(Rf)]
+        assert np.allclose(fc[:, c], ref, rtol=1e-10, atol=1e-13), (order, c)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_fresnel_two_films_without_absorption(oracle_mod, order):
+    """Two films on a substrate, real wavevectors in every medium (no absorption, angles above the
+    critical angles): the recursion's amplitudes at orders 1 and 2 equal the boundary-condition
+    solution. (With absorption in a film that is not the last one the reference's recursion takes
+    conj(e^{-i(k_j + k_{j+1}) z}) for e^{+i(k_j + k_{j+1}) z} (src/model/multilayer.cpp:92-95); the two
+    agree only for real k, which is the case pinned here.)"""
+    O = oracle_mod
+    k0 = 50.677
+    ds, ts = [2.0e-6, 5.0e-6, 3.0e-6], [12.0, 31.0, 0.0]
+    ai = np.radians(0.4)
+    af = np.radians(np.array([0.3, 0.45, 0.8]))
+    g = O.Grid(1, af.size, np.zeros(af.size), np.zeros(af.size), np.zeros(af.size) + 0j, af)
+    fc = O.fresnel(g, ds, [0.0] * 3, ts, order, k0, ai).reshape(4, -1)
+    n2s = [0.0] + [2 * d + 0j for d in ds]
+    z_if = [0.0, -ts[0], -ts[0] - ts[1]]
+    Ti, Ri = (v[order] for v in _stack_amplitudes(k0, ai, n2s, z_if))
+    for c, a in enumerate(af):
+        Tf, Rf = (v[order] for v in _stack_amplitudes(k0, a, n2s, z_if))
+        ref = [Ti * np.conj(Tf), Ti * np.conj(Rf), Ri * np.conj(Tf), Ri * np.conj(Rf)]
+        assert np.allclose(fc[:, c], ref, rtol=1e-10, atol=1e-13), (order, c)
