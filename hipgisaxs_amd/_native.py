@@ -174,6 +174,7 @@ _SIGS = {
     "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "hgx_batch_release": (C.c_int, [_vp, C.c_int]),
+    "hgx_ff_numeric_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "hgx_tune_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_fit_plan_create": (C.c_int, [_vp, C.POINTER(FitModel), _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_fit_plan_release": (C.c_int, [_vp, C.c_int]),

@@ -200,14 +200,16 @@ struct Plan {
   int64_t per_chunk;  // blocks (FP32C) or triangles (FP64 / exact) per chunk
 };
 
-// Enough workgroups to fill 256 CUs several times over, without splitting below one block.
-static Plan make_plan(int64_t nq, int64_t units, int64_t min_units, int qpt, int target_wg) {
+// Enough workgroups to fill 256 CUs several times over, without splitting below one block, and
+// at most max_chunks chunks.
+static Plan make_plan(int64_t nq, int64_t units, int64_t min_units, int qpt, int target_wg,
+                      int64_t max_chunks = INT64_MAX) {
   Plan p;
   p.qpt = qpt;
   p.nqt = ceil_div(nq, (int64_t)kWG * qpt);
   int64_t want = std::max<int64_t>(1, (target_wg + p.nqt - 1) / p.nqt);
   int64_t maxc = std::max<int64_t>(1, units / std::max<int64_t>(1, min_units));
-  want = std::min(want, maxc);
+  want = std::min(std::min(want, maxc), std::max<int64_t>(1, max_chunks));
   p.per_chunk = (units + want - 1) / want;
   p.nchunks = (int)((units + p.per_chunk - 1) / p.per_chunk);
   return p;
@@ -253,10 +255,54 @@ extern "C" int hgx_tune_numeric(hgx_ctx* h, int qpt64, int qpt32, int target_wg)
   return HGX_OK;
 }
 
-extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision, int flags, const double rot[9],
-                              const double transvec[3], double* ff) {
-  Context* c = reinterpret_cast<Context*>(h);
-  if (!c) return HGX_ERR_BAD_HANDLE;
+namespace {
+
+// Device memory of one numeric form-factor call (partial sums of triangle chunks, rotated records):
+// the context's workspace, grown on demand (fixed == nullptr), or a fixed buffer of cap bytes that
+// the call must fit in, choosing fewer chunks where needed (the allocation-free entry point).
+struct Work {
+  char* fixed = nullptr;
+  size_t cap = 0;
+};
+
+int take_work(Context* c, const Work& w, size_t bytes, char** out) {
+  if (!w.fixed) {
+    if (int rc = ensure_work(c, bytes)) return rc;
+    *out = static_cast<char*>(c->d_work);
+    return HGX_OK;
+  }
+  if (bytes > w.cap)
+    return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_noalloc: needs %zu bytes of scratch, the maximum is %zu",
+                     bytes, w.cap);
+  *out = w.fixed;
+  return HGX_OK;
+}
+
+// Fewest chunk count reductions: with a fixed buffer, at most as many chunks of nq complex partial
+// sums as fit beside `reserved` bytes (one chunk needs none: it writes the output directly).
+int64_t chunk_limit(const Work& w, int64_t nq, size_t reserved) {
+  if (!w.fixed) return INT64_MAX;
+  if (reserved >= w.cap) return 1;
+  const int64_t n = (int64_t)((w.cap - reserved) / ((size_t)nq * 2 * sizeof(double)));
+  return n >= 2 ? n : 1;
+}
+
+// Launch timing (hgx_profile_begin) only for calls that may allocate: event creation is not done by
+// the allocation-free entry point.
+struct OptTimer {
+  alignas(LaunchTimer) unsigned char buf[sizeof(LaunchTimer)];
+  LaunchTimer* t = nullptr;
+  OptTimer(Context* c, bool main, bool on) {
+    if (on) t = new (buf) LaunchTimer(c, main);
+  }
+  ~OptTimer() {
+    if (t) t->~LaunchTimer();
+  }
+};
+
+int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags, const double rot[9],
+                   const double transvec[3], double* ff, const Work& work) {
+  const bool timed = work.fixed == nullptr;
   if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ff_numeric: no q-grid");
   if (mesh_id < 0 || mesh_id >= (int)c->meshes.size() || !c->meshes[mesh_id].live)
     return set_error(c, HGX_ERR_BAD_HANDLE, "ff_numeric: unknown mesh %d", mesh_id);
@@ -300,12 +346,18 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     want = std::min<int64_t>(want, m.nblocks);
     // chunks accumulate in FP32: at most product_blocks_per_unit(c) blocks each
     it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit(c));
+    size_t rec_bytes = identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
+    {
+      // a fixed buffer may hold fewer chunks: longer chunks then (their FP32 sums span more blocks)
+      const int64_t lim = chunk_limit(work, c->nq, rec_bytes);
+      if ((m.nblocks + it.bchunk - 1) / it.bchunk > lim) it.bchunk = (m.nblocks + lim - 1) / lim;
+    }
     it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
     if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
     size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
-    size_t rec_bytes = identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
-    if (int rc = ensure_work(c, part_bytes + rec_bytes)) return rc;
-    char* w = static_cast<char*>(c->d_work);
+    char* w = nullptr;
+    if (part_bytes + rec_bytes > 0)
+      if (int rc = take_work(c, work, part_bytes + rec_bytes, &w)) return rc;
     it.part = it.nchunks > 1 ? reinterpret_cast<double*>(w) : nullptr;
     it.out = ff;
     if (identity) {
@@ -315,19 +367,19 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
       float* rr = reinterpret_cast<float*>(w + part_bytes);
       double* ro = reinterpret_cast<double*>(w + part_bytes + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
       {
-        LaunchTimer lt(c, false);
+        OptTimer lt(c, false, timed);
         if (int rc = product_rotate_mesh(c, m, R, rr, ro, c->stream)) return rc;
       }
       it.rec = rr;
       it.borig = ro;
     }
     {
-      LaunchTimer lt(c, true);
+      OptTimer lt(c, true, timed);
       product_launch_single(c->stream, it, cq, pmode);
     }
     HGX_CHECK_LAUNCH(c);
     {
-      LaunchTimer lt(c, false);
+      OptTimer lt(c, false, timed);
       k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
@@ -336,16 +388,17 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
   if (method == HGX_FF_APPROX) {
     Plan p;
     if (precision == HGX_PREC_FP64)
-      p = make_plan(c->nq, m.ntri, kBlockTri, c->tune.qpt64, c->tune.target_wg);
+      p = make_plan(c->nq, m.ntri, kBlockTri, c->tune.qpt64, c->tune.target_wg, chunk_limit(work, c->nq, 0));
     else
-      p = make_plan(c->nq, m.nblocks, 1, c->tune.qpt32, c->tune.target_wg);
+      p = make_plan(c->nq, m.nblocks, 1, c->tune.qpt32, c->tune.target_wg, chunk_limit(work, c->nq, 0));
     double* part = nullptr;
     if (p.nchunks > 1) {
-      if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;
-      part = reinterpret_cast<double*>(c->d_work);
+      char* w = nullptr;
+      if (int rc = take_work(c, work, (size_t)p.nchunks * c->nq * 2 * sizeof(double), &w)) return rc;
+      part = reinterpret_cast<double*>(w);
     }
     {
-    LaunchTimer lt(c, true);
+    OptTimer lt(c, true, timed);
     if (precision == HGX_PREC_FP64) {
       switch (p.qpt) {
         case 1: launch_approx64<1>(c->stream, p, g, R, m, cq, part, ff, e); break;
@@ -366,7 +419,7 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     }
     HGX_CHECK_LAUNCH(c);
     {
-      LaunchTimer lt(c, false);
+      OptTimer lt(c, false, timed);
       k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
@@ -377,11 +430,12 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
       return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: exact method needs a vertex (exact9) mesh");
     if (precision != HGX_PREC_FP64)
       return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric: exact method is FP64 only");
-    Plan p = make_plan(c->nq, m.ntri, 64, 1, c->tune.target_wg);
+    Plan p = make_plan(c->nq, m.ntri, 64, 1, c->tune.target_wg, chunk_limit(work, c->nq, 0));
     double* part = nullptr;
     if (p.nchunks > 1) {
-      if (int rc = ensure_work(c, (size_t)p.nchunks * c->nq * 2 * sizeof(double))) return rc;
-      part = reinterpret_cast<double*>(c->d_work);
+      char* w = nullptr;
+      if (int rc = take_work(c, work, (size_t)p.nchunks * c->nq * 2 * sizeof(double), &w)) return rc;
+      part = reinterpret_cast<double*>(w);
     }
     double sign = (flags & HGX_FLAG_EXACT_REFERENCE_SIGN) ? 1.0 : -1.0;
     // degenerate-case threshold on q.n and q.e: the reference GPU path's 1e-13 (ff_tri_gpu.cu:80,100)
@@ -391,16 +445,34 @@ extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision
     const double tiny = (flags & HGX_FLAG_EXACT_CPU_THRESHOLD) ? 2.2250738585072014e-308 : 1e-13;
     dim3 grid(p.nqt, p.nchunks);
     {
-      LaunchTimer lt(c, true);
+      OptTimer lt(c, true, timed);
       k_ff_exact64<<<grid, kWG, 0, c->stream>>>(g, R, sign, m.d_tri9, m.ntri, p.per_chunk, p.nchunks, tiny, part, ff, e);
     }
     HGX_CHECK_LAUNCH(c);
     if (p.nchunks > 1) {
-      LaunchTimer lt(c, false);
+      OptTimer lt(c, false, timed);
       k_ff_exact_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
     return HGX_OK;
   }
   return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: bad method %d", method);
+}
+
+}  // namespace
+
+extern "C" int hgx_ff_numeric(hgx_ctx* h, int mesh_id, int method, int precision, int flags, const double rot[9],
+                              const double transvec[3], double* ff) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  return ff_numeric_run(c, mesh_id, method, precision, flags, rot, transvec, ff, Work{});
+}
+
+extern "C" int hgx_ff_numeric_noalloc(hgx_ctx* h, int mesh_id, int method, int precision, int flags,
+                                      const double rot[9], const double transvec[3], double* ff) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!c->d_scratch)
+    return set_error(c, HGX_ERR_DEVICE, "ff_numeric_noalloc: the scratch array of the device was not resolved");
+  return ff_numeric_run(c, mesh_id, method, precision, flags, rot, transvec, ff, Work{c->d_scratch, kScratchBytes});
 }

@@ -1,6 +1,7 @@
 // hgx_context.hip -- context lifetime, q-grid construction on the device, mesh upload.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -71,6 +72,32 @@ static void free_mesh(Mesh& m) {
   if (m.d_moments) (void)hipFree(m.d_moments);
   if (m.d_src) (void)hipFree(m.d_src);
   m = Mesh();
+}
+
+// Scratch memory of the allocation-free entry point (hgx_ff_numeric_noalloc): a static array that
+// each device holds until the process ends. Host code never uses the name `scratch` as an address:
+// its device address is resolved once per device, with that device current, and kept.
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+static char* device_scratch(int device) {
+  static std::mutex mu;
+  static std::vector<char*> ptrs;
+  static std::vector<char> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptrs.size() <= device) {
+    ptrs.resize(device + 1, nullptr);
+    done.resize(device + 1, 0);
+  }
+  if (!done[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) {
+      ptrs[device] = static_cast<char*>(p);
+      done[device] = 1;  // a failed lookup is tried again by the next context on the device
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  return ptrs[device];
 }
 
 uint64_t next_serial() {
@@ -253,6 +280,7 @@ int hgx_create(int device, hgx_ctx** out) {
     return HGX_ERR_DEVICE;
   }
   c->stream = c->own_stream;
+  c->d_scratch = device_scratch(device);  // with the device current
   *out = reinterpret_cast<hgx_ctx*>(c);
   return HGX_OK;
 }

@@ -21,6 +21,9 @@ constexpr int kSeriesOrder = 8;  // small-|q| moment series: terms k = 0..kSerie
 // number of moment components: sum_k 3 * (k+1)(k+2)/2
 constexpr int kMomentCount = 3 * 165;
 
+// Size of the per-device scratch array of the allocation-free entry point (hgx_ff_numeric_noalloc).
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+
 // Process-wide serial numbers: each q-grid generation and each uploaded mesh takes a new one, so
 // work prepared against a grid or a mesh (batches, grating plans) can tell that it was replaced.
 uint64_t next_serial();
@@ -206,6 +209,9 @@ struct Context {
   // scratch for partial sums
   void* d_work = nullptr;
   size_t work_bytes = 0;
+  // device address of the device's static scratch array (kScratchBytes), shared by every context on
+  // the device; null when it could not be resolved
+  char* d_scratch = nullptr;
   std::string last_error;
   // optional HIP-event timing of kernel launches (hgx_profile_begin / hgx_profile_end)
   bool profiling = false;

@@ -173,6 +173,18 @@ int hgx_mesh_info(const hgx_ctx* ctx, int mesh_id, int64_t* ntri, double* volume
 int hgx_ff_numeric(hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, const double rot[9],
                    const double transvec[3], double* ff);
 
+/* hgx_ff_numeric without any device allocation: the same kernels and arguments, with their partial
+ * sums and rotated records in a fixed per-device scratch array of 32 MiB instead of the context's
+ * growing workspace (fewer triangle chunks where the array is short; FP32C results may differ from
+ * hgx_ff_numeric's within the FP32C bar). No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops). Refuses with
+ * HGX_ERR_INVALID_ARG a call whose rotated records alone exceed the array (FP32C on a product grid
+ * with a rotation, meshes of more than about a million triangles). Every context on a device shares
+ * that device's array, and it keeps the last call's contents: calls that overlap on one device (two
+ * streams, two contexts, two host threads) race. No hipMalloc, no hipFree, no profiling events. */
+int hgx_ff_numeric_noalloc(hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, const double rot[9],
+                           const double transvec[3], double* ff);
+
 /* Record adjoint of the approximate numeric form factor. For L = Re sum_q W(q) F(q), F as computed by
  * hgx_ff_numeric(HGX_FF_APPROX) for the same mesh and rotation and W complex [2*nq] on the device,
  * writes grad[6 * ntri_uploaded] (device) = (dL/d(s n)_t, dL/dc_t) per triangle in the order the mesh
