@@ -41,11 +41,62 @@ def flop_per_pair_product(mults, operands="fp32"):
 PEAK_FP16 = 2.5e15  # MI355X dense fp16 matrix-core peak
 
 
-# HBM bytes per launch of the product kernel on this workload, per work-unit shape, from rocprofv3
-# --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB device copy) in KiB per
-# dispatch: k_ff_product32 (16 blocks per unit, Gauss form) and k_ff_product16s (the default).
-PRODUCT_TRAFFIC_BYTES = {(16, 3, "fp32"): (94933.4 * 2 + 2218240.0) * 1024,
-                         (64, 4, "f16x2", 2, True, True): (99335.0 * 2 + 694528.0) * 1024}
+# HBM bytes per launch of the product kernel on this workload and of the finishing kernel that
+# follows it, per work-unit shape, from rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
+# (scales from a 256 MiB device copy) in KiB per dispatch (profiles/k_ff_product16s_64polyhedra_batch.md).
+PRODUCT_TRAFFIC_BYTES = {(64, 4, "f16x2", 2, True, True): ((99408.8 * 2 + 694528.0) * 1024,
+                                                           (617116.5 * 2 + 306688.0) * 1024)}
+
+
+def ALGO_BYTES(shapes):
+    """Algorithmic HBM bytes of one step of the product kernel: every triangle record once (32 B, FP32,
+    padded to 128-triangle blocks) and every complex FP64 output once (16 B per q-point)."""
+    return float(sum(32 * 128 * math.ceil(s["ntri"] / 128) + 16 * s["side"] ** 2 for s in shapes))
+
+
+def intensity(z):
+    """|F|^2 of a complex image stored as interleaved float64."""
+    v = z.view(-1, 2)
+    return v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]
+
+
+class pixel_errors:
+    """Per-pixel accuracy of FP32C intensity images I against FP64 ones I64, over many images:
+    r = |I - I64| / max(I64, 1e-10 max I64) (the floor keeps exact nodes of F, where I64 -> 0, from
+    dividing by rounding; ten decades below the peak), its maximum, median and the fractions of pixels
+    above 1e-6, 1e-5 and 1e-4; and relL2 of log10 of the floored intensities (the error a fit on log
+    intensities sees)."""
+
+    FLOOR = 1e-10
+
+    def __init__(self):
+        self.max = 0.0
+        self.n = 0
+        self.over = {"1e-6": 0, "1e-5": 0, "1e-4": 0}
+        self.med = []
+        self.lnum = self.lden = 0.0
+
+    def add(self, I, I64):
+        import torch
+
+        fl = self.FLOOR * float(I64.max())
+        den = torch.clamp(I64, min=fl)
+        r = (I - I64).abs() / den
+        self.max = max(self.max, float(r.max()))
+        self.n += r.numel()
+        for k in self.over:
+            self.over[k] += int((r > float(k)).sum())
+        self.med.append(float(r.median()))
+        l32 = torch.log10(torch.clamp(I, min=fl))
+        l64 = torch.log10(den)
+        self.lnum += float(torch.linalg.vector_norm(l32 - l64)) ** 2
+        self.lden += float(torch.linalg.vector_norm(l64)) ** 2
+
+    def result(self):
+        return {"max_rel": self.max, "median_rel_of_images": float(np.median(self.med)) if self.med else None,
+                "frac_rel_over": {k: v / max(self.n, 1) for k, v in self.over.items()},
+                "relL2_log10I": math.sqrt(self.lnum / max(self.lden, 1e-300)), "floor_rel_to_peak": self.FLOOR,
+                "pixels": self.n}
 
 
 def make_workload(nshapes=64, seed=SEED + 3):
@@ -180,6 +231,13 @@ def bench_pyramid_fp64(reps=5):
     torch.cuda.synchronize()
     t_exact = (time.perf_counter() - t0) / reps
     k_exact = ctx.profile_end()
+    # the FP32C path on the same inputs: its intensity against the FP64 one, per pixel
+    ff32 = ctx.ff_numeric(mid, "approx", "fp32c")
+    I32 = ctx.dwba_intensity(ff32, fc=fcd)
+    torch.cuda.synchronize()
+    pix = pixel_errors()
+    pix.add(I32, I)
+    rel32 = float(torch.linalg.vector_norm(I32 - I) / torch.linalg.vector_norm(I))
     # parity on every 8th row (the oracle's exact method is slow), with the same degenerate-case
     # threshold as the GPU default (the reference GPU path's 1e-13)
     rows = np.arange(0, 4 * 256, 8)
@@ -198,17 +256,21 @@ def bench_pyramid_fp64(reps=5):
     ctx.close()
     ka_ms = k_approx["main_ms"] / max(k_approx["main_launches"], 1)
     ke_ms = k_exact["main_ms"] / max(k_exact["main_launches"], 1)
-    # FP64 VALU roofline of the form-factor kernels: algorithmic flop per pair (approximate 33, exact
-    # about 200; transcendentals not counted) over the kernel's own time, against 78.6 TFLOP/s
-    roof = lambda fpp, ms: {"bound": "fp64-valu", "achieved": fpp * pairs / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
-                            "unit": "TFLOP/s", "frac": fpp * pairs / (ms * 1e-3) / PEAK_FP64, "flop_per_pair": fpp,
-                            "avg_launch_ms": ms}
+    # FP64 roofline of the form-factor kernels: algorithmic flop per pair (approximate on this product
+    # grid: 16, two complex multiply-adds on the FP64 matrix cores; exact about 200 on the VALU;
+    # transcendentals not counted) over the kernel's own time, against 78.6 TFLOP/s (the FP64 peak of
+    # both the matrix cores and the vector units)
+    roof = lambda fpp, ms, bound: {"bound": bound, "achieved": fpp * pairs / (ms * 1e-3) / 1e12,
+                                   "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                                   "frac": fpp * pairs / (ms * 1e-3) / PEAK_FP64, "flop_per_pair": fpp,
+                                   "avg_launch_ms": ms}
     return {"gpu_ms": 1e3 * t_gpu, "cpu_ms": 1e3 * t_cpu, "gpu_pairs_per_s": pairs / t_gpu,
             "cpu_pairs_per_s": pairs / t_cpu, "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
             "relL2_intensity_gpu_vs_oracle": float(np.linalg.norm(got - I_ref) / np.linalg.norm(I_ref)),
-            "roofline_approx_kernel": roof(FLOP_PER_PAIR, ka_ms),
+            "roofline_approx_kernel": dict(roof(16, ka_ms, "fp64-mfma"), kernel="k_ff_product64"),
             "exact_method": {"gpu_ms": 1e3 * t_exact, "gpu_pairs_per_s": pairs / t_exact,
-                             "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms)},
+                             "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms, "fp64-valu")},
+            "fp32c": {"relL2_intensity_vs_fp64": rel32, "pixel_error_vs_fp64": pix.result()},
             "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA; "
                         "exact-triangle FF of the same mesh and grid"}
 
@@ -250,6 +312,9 @@ def bench_lattice(reps=5):
         res[prec] = I
     d = torch.linalg.vector_norm(res["fp32c"] - res["fp64"]) / torch.linalg.vector_norm(res["fp64"])
     out["relL2_fp32c_vs_fp64"] = float(d)
+    pix = pixel_errors()
+    pix.add(res["fp32c"], res["fp64"])
+    out["pixel_error_fp32c_vs_fp64"] = pix.result()
     # VALU roofline: per (q, node, grain) sample the sphere form factor (about 55 flop), per (q, grain)
     # the lattice factor (about 155 flop) and the DWBA term (about 15 flop per branch); transcendentals
     # not counted
@@ -643,13 +708,27 @@ def main():
     print(f"[bench] fp32c {args.steps} steps: {wall32:.3f}s", file=sys.stderr, flush=True)
     # roofline: the same kernels launched directly, so the hot kernel's own events are recorded
     _, k32_ms, n32, aux32 = timed(step32, args.steps, 0, False)
+    # plain FP64 on the same tiling (the FP64 product-grid kernel on product grids), then the
+    # per-pair FP64 kernels (HGX_FLAG_NO_MFMA) as a second baseline
     step64, b64 = make_step("fp64")
     wall64, k64_ms, n64, aux64 = timed(step64, args.fp64_steps, 1 if args.warmup else 0, use_graph)
     print(f"[bench] fp64 {args.fp64_steps} steps: {wall64:.3f}s", file=sys.stderr, flush=True)
+    wall64p = k64p_ms = None
+    if product and args.mode == "batch" and not args.no_extra:
+        bp = Batch(main_ctx, items["fp64"], precision="fp64", flags=4)
+        wall64p, k64p_ms, _, _ = timed(lambda g: bp.run(use_graph=g), 1, 1, True)
+        bp.release()
+        step64(use_graph)  # the same-tiling FP64 outputs again, as the reference for the accuracy below
+        torch.cuda.synchronize()
+        print(f"[bench] fp64 per pair 1 step: {wall64p:.3f}s", file=sys.stderr, flush=True)
+    if b64 is not None:
+        b64.release()
 
-    # accuracy of the FP32C outputs against the plain FP64 GPU outputs (all shapes)
+    # accuracy of the FP32C outputs against the plain FP64 GPU outputs (all shapes): relL2 of the
+    # amplitudes, and per pixel of the intensity I = |F|^2
     num = den = 0.0
     worst = 0.0
+    pix = pixel_errors()
     for s in shapes:
         a, b = s["out32"], s["out64"]
         d = float(torch.linalg.vector_norm(a - b)) ** 2
@@ -657,6 +736,7 @@ def main():
         num += d
         den += n
         worst = max(worst, math.sqrt(d / max(n, 1e-300)))
+        pix.add(intensity(a), intensity(b))
     rel_fp64 = math.sqrt(num / den)
 
     value32 = pairs * args.steps / wall32
@@ -664,8 +744,13 @@ def main():
     avg_launch_ms = k32_ms / max(n32, 1)
     pairs_per_launch = pairs * args.steps / max(n32, 1)
     bpu_p, mults, operands = pcfg[:3]
-    fpp = flop_per_pair_product(mults, operands) if product else FLOP_PER_PAIR
+    # algorithmic work: on product grids two complex multiply-adds per (q, triangle) pair (16 flop,
+    # the factorized form's own count; the split-fp16 emulation and the table work are overhead), per
+    # pair the direct form's 33 flop otherwise
+    fpp = 16 if product else FLOP_PER_PAIR
+    fpp_exec = flop_per_pair_product(mults, operands) if product else FLOP_PER_PAIR
     achieved = fpp * pairs_per_launch / (avg_launch_ms * 1e-3)
+    achieved_exec = fpp_exec * pairs_per_launch / (avg_launch_ms * 1e-3)
     split16 = product and operands == "f16x2"
     peak = PEAK_FP16 if split16 else PEAK_FP32
     result = {
@@ -693,10 +778,13 @@ def main():
             "tuning": tuned,
         },
         "fp64_plain": {"value": value64, "ms_per_step": 1e3 * wall64 / args.fp64_steps,
-                       "kernel_ms_per_step": k64_ms / args.fp64_steps},
+                       "kernel_ms_per_step": k64_ms / args.fp64_steps,
+                       "kernel": "k_ff_product64 (same tiling, FP64 matrix cores)" if product else
+                                 ("k_batch64" if args.mode == "batch" else "k_ff_approx64")},
         "speedup_fp32c_vs_fp64": value32 / value64,
         "relL2_fp32c_vs_fp64_gpu": rel_fp64,
         "relL2_fp32c_vs_fp64_gpu_worst_shape": worst,
+        "pixel_error_fp32c_vs_fp64_gpu": pix.result(),
         "roofline": {
             "bound": "mfma" if product else "valu",
             "kernel": (("k_ff_product16s" if len(pcfg) > 5 and pcfg[5] else "k_ff_product16") if split16
@@ -708,19 +796,26 @@ def main():
             "frac": achieved / peak,
             "traffic": None,
             "flop_per_pair": fpp,
+            "executed": {"flop_per_pair": fpp_exec, "tflops": achieved_exec / 1e12, "frac": achieved_exec / peak},
             "avg_launch_ms": avg_launch_ms,
             "launches": n32,
             "aux_ms_per_step": aux32 / args.steps,
         },
     }
+    if wall64p is not None:
+        result["fp64_per_pair"] = {"value": pairs / wall64p, "ms_per_step": 1e3 * wall64p,
+                                   "kernel": "k_batch64 (one q-point per lane, no matrix cores)"}
+        result["speedup_fp32c_vs_fp64_per_pair"] = value32 * wall64p / pairs
     if product:
         rate = pairs_per_launch / (avg_launch_ms * 1e-3)
-        # the same pairs counted as complex arithmetic (8 flop per complex multiply-add) and as the
-        # direct per-pair form (33 flop): rates of equivalent work, not of matrix-core issue
-        result["roofline"]["complex_form_tflops"] = 16 * rate / 1e12
+        # the same pairs counted in the direct per-pair form (33 flop): a rate of equivalent work
         result["roofline"]["direct_form_equivalent_tflops"] = FLOP_PER_PAIR * rate / 1e12
         if len(shapes) == 64 and not args.kahan and pcfg in PRODUCT_TRAFFIC_BYTES:
-            result["roofline"]["traffic"] = PRODUCT_TRAFFIC_BYTES[pcfg]
+            kb, fb = PRODUCT_TRAFFIC_BYTES[pcfg]
+            result["roofline"]["traffic"] = kb
+            # the whole step: the finishing kernel reads every chunk's FP64 partials once more
+            result["roofline"]["traffic_step"] = kb + fb if fb is not None else None
+            result["roofline"]["algorithmic_bytes"] = ALGO_BYTES(shapes)
     if b32 is not None:
         result["config"]["work_units"] = b32.info()["units"]
     if product and not args.no_extra:

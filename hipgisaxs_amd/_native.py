@@ -188,6 +188,7 @@ _SIGS = {
                                          C.c_int, _dp, _dp, C.c_int]),
     "hgx_tune_numeric": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "hgx_tune_product": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "hgx_tune_product64": (C.c_int, [_vp, C.c_int]),
     "hgx_tune_product_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "hgx_tune_adjoint": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_caxpy": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_double, _vp, C.c_int, _vp]),

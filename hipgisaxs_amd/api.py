@@ -395,6 +395,12 @@ def tune_product(ctx: "Context", blocks_per_unit: int, complex_mults: int = 3, o
         raise HgxError("tune_product: invalid row tiles / accumulation / sharing")
 
 
+def tune_product64(ctx: "Context", complex_mults: int = 4) -> None:
+    """Real products per complex product (4 or 3, Gauss) of the plain FP64 product-grid kernel."""
+    if N.load().hgx_tune_product64(ctx.h, int(complex_mults)) != N.HGX_OK:
+        raise HgxError("tune_product64: complex_mults must be 3 or 4")
+
+
 def tune_adjoint(ctx: "Context", operands: str = "f16x2", col_tiles: int = 2) -> None:
     """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default; 2 or 4
     32-column tiles per wave) or "fp32"."""

@@ -190,7 +190,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   b.precision = precision;
   b.flags = flags;
   b.qpt = c->tune.batch_qpt;
-  b.pmode = product_mode(c);
+  b.pmode = precision == HGX_PREC_FP64 ? product_mode64(c) : product_mode(c);
   const int64_t bpu = c->tune.batch_bpu;
   std::vector<BatchItemDev> it(n);
   std::vector<ProductItem> pit(n);
@@ -224,7 +224,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     d.nblocks = m.nblocks;
     d.ntri = m.ntri;
     d.bchunk = std::min<int64_t>(bpu, m.nblocks);
-    const bool prod = precision == HGX_PREC_FP32C && gc->product && !(flags & HGX_FLAG_NO_MFMA);
+    const bool prod = gc->product && !(flags & HGX_FLAG_NO_MFMA);
     if (prod) {
       // product grid: complex matrix products on the matrix cores (product_mfma.hip)
       is_prod[i] = 1;
@@ -234,6 +234,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
       pi.qy = gc->d_qy;
       pi.qz = gc->d_qz;
       pi.nblocks = m.nblocks;
+      pi.ntri = m.ntri;
       pi.bchunk = d.bchunk;
       pi.nq = gc->nq;
       pi.ncol = gc->ncol;
@@ -246,9 +247,11 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
       if (identity) {
         pi.rec = m.d_rec32;
         pi.borig = m.d_borig;
+        pi.rec64 = m.d_rec64;
       } else {
         rot_off[i] = rot_total;
-        rot_total += (int64_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double));
+        rot_total += precision == HGX_PREC_FP64 ? (int64_t)m.ntri * 8 * sizeof(double)
+                                                : (int64_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double));
       }
     }
     d.nchunks = (int)((m.nblocks + d.bchunk - 1) / d.bchunk);
@@ -331,6 +334,15 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
       if (rot_off[i] < 0) continue;
       const Mesh& m = c->meshes[items[i].mesh_id];
       char* base = static_cast<char*>(b.d_prot) + rot_off[i];
+      if (precision == HGX_PREC_FP64) {
+        double* r64 = reinterpret_cast<double*>(base);
+        if (int rc = product_rotate_records64(c, m, make_rot(items[i].rot), r64, c->stream)) {
+          free_batch_state(b);
+          return rc;
+        }
+        pit[i].rec64 = r64;
+        continue;
+      }
       float* rr = reinterpret_cast<float*>(base);
       double* ro = reinterpret_cast<double*>(base + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
       if (int rc = product_rotate_mesh(c, m, make_rot(items[i].rot), rr, ro, c->stream)) {

@@ -322,9 +322,11 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
   e.radius = m.radius;
   e.use_series = 0;
   const bool cq = !c->qz_real;  // a rotation keeps real q real
-  if (method == HGX_FF_APPROX && precision == HGX_PREC_FP32C && c->product && !(flags & HGX_FLAG_NO_MFMA)) {
-    // product grid: complex matrix products on the matrix cores (product_mfma.hip)
-    e.use_series = 1;
+  if (method == HGX_FF_APPROX && c->product && !(flags & HGX_FLAG_NO_MFMA)) {
+    // product grid: complex matrix products on the matrix cores (product_mfma.hip; FP64:
+    // product_fp64.hip)
+    const bool f64 = precision == HGX_PREC_FP64;
+    e.use_series = f64 ? 0 : 1;
     const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
                                              R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
     ProductItem it{};
@@ -338,7 +340,8 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     it.snmax = (float)m.snmax;
     it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
                       m.radius);
-    const int pmode = product_mode(c);
+    it.ntri = m.ntri;
+    const int pmode = f64 ? product_mode64(c) : product_mode(c);
     const int64_t tiles = (int64_t)ceil_div(it.nz, product_zrows(pmode)) * ceil_div(it.ncol, product_ycols(pmode));
     // enough waves to fill the chip several times over, without splitting below one block
     const int64_t upw = product_is_shared(pmode) ? 1 : kProductWaves;  // work units per workgroup
@@ -346,7 +349,9 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     want = std::min<int64_t>(want, m.nblocks);
     // chunks accumulate in FP32: at most product_blocks_per_unit(c) blocks each
     it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit(c));
-    size_t rec_bytes = identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
+    size_t rec_bytes = identity ? 0
+                       : f64    ? (size_t)m.ntri * 8 * sizeof(double)
+                                : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
     {
       // a fixed buffer may hold fewer chunks: longer chunks then (their FP32 sums span more blocks)
       const int64_t lim = chunk_limit(work, c->nq, rec_bytes);
@@ -363,6 +368,14 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     if (identity) {
       it.rec = m.d_rec32;
       it.borig = m.d_borig;
+      it.rec64 = m.d_rec64;
+    } else if (f64) {
+      double* rr = reinterpret_cast<double*>(w + part_bytes);
+      {
+        OptTimer lt(c, false, timed);
+        if (int rc = product_rotate_records64(c, m, R, rr, c->stream)) return rc;
+      }
+      it.rec64 = rr;
     } else {
       float* rr = reinterpret_cast<float*>(w + part_bytes);
       double* ro = reinterpret_cast<double*>(w + part_bytes + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));

@@ -104,6 +104,8 @@ struct ProductItem {
   const double* qz;
   const float* rec;     // [nblocks * kBlockTri][8] rotated s*n (3), c - o_b (3), 0, 0
   const double* borig;  // [nblocks][4] rotated block origins
+  const double* rec64;  // [ntri][8] rotated FP64 records s*n (3), c (3), 0, 0 (FP64 kernel only)
+  int64_t ntri;         // triangles of rec64
   int64_t nblocks;
   int64_t bchunk;       // blocks per chunk
   int64_t nq;
@@ -159,17 +161,26 @@ constexpr int kProductF16x2Z4Flat = 5;
 constexpr int kProductF16x2S2 = 6;
 constexpr int kProductF16x2S4Flat = 7;
 constexpr int kProductF16x2S4 = 8;
-inline bool product_is_f16(int mode) { return mode >= kProductF16x2; }
+// plain FP64 on the FP64 matrix cores (product_fp64.hip): 64 x 64 per workgroup, four or three
+// (Gauss) real products per complex product
+constexpr int kProductF64x4 = 9;
+constexpr int kProductF64Gauss = 10;
+inline bool product_is_f64(int mode) { return mode == kProductF64x4 || mode == kProductF64Gauss; }
+inline bool product_is_f16(int mode) { return mode >= kProductF16x2 && mode <= kProductF16x2S4; }
+// one work unit per workgroup
 inline bool product_is_shared(int mode) { return mode >= kProductF16x2S2; }
 // rows (z) and columns (y) of one work unit's tile for a kernel variant
 inline int product_zrows(int mode) {
-  return (mode == kProductF16x2Z2 || mode == kProductF16x2S2)           ? 2 * kProductTile
+  return product_is_f64(mode) ? 2 * kProductTile
+         : (mode == kProductF16x2Z2 || mode == kProductF16x2S2) ? 2 * kProductTile
          : (mode == kProductF16x2Z4 || mode == kProductF16x2Z4Flat || mode == kProductF16x2S4Flat ||
             mode == kProductF16x2S4)
              ? 4 * kProductTile
                                                                                  : kProductTile;
 }
-inline int product_ycols(int mode) { return product_is_shared(mode) ? kProductWaves * kProductTile : kProductTile; }
+inline int product_ycols(int mode) {
+  return product_is_f64(mode) ? 2 * kProductTile : product_is_shared(mode) ? kProductWaves * kProductTile : kProductTile;
+}
 
 
 // Launch-shape settings of the numeric kernels, per context (hgx_tune_*; the autotuner sets them).
@@ -182,6 +193,7 @@ struct Tuning {
   int product_mults = 4;
   int product_mz = 2, product_two = 1, product_shared = 1;
   int adjoint_variant = 2;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
+  int product64_mults = 4;  // FP64 product-grid kernel: 4 or 3 (Gauss) real products per complex product
 };
 
 struct Context {
@@ -420,8 +432,15 @@ int64_t adjoint_product_units(const AdjItem& it, int variant);
 int adjoint_product_variant(const Context* c);  // the context's setting (hgx_tune_adjoint)
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq, int variant);
-// Tuning: triangle blocks per work unit and the kernel variant (kProduct* below).
+// Tuning: triangle blocks per work unit and the kernel variant (kProduct* above); the FP64 variant.
 int product_blocks_per_unit(const Context* c);
 int product_mode(const Context* c);
+inline int product_mode64(const Context* c);
+// FP64 product-grid kernel (product_fp64.hip): rotated FP64 records (R^T applied) and the launch
+int product_rotate_records64(Context* c, const Mesh& m, const Rot& R, double* rec_out, hipStream_t s);
+void product64_launch(hipStream_t s, const ProductItem* items, const ProductUnit* units, int nunits,
+                      const ProductItem& single, int nzt, int nyt, bool cq, bool gauss);
+
+inline int product_mode64(const Context* c) { return c->tune.product64_mults == 3 ? kProductF64Gauss : kProductF64x4; }
 
 }  // namespace hgx

@@ -775,7 +775,9 @@ static void launch16_mode(hipStream_t s, const ProductItem* items, const Product
 static void launch_any(hipStream_t s, const ProductItem* items, const ProductUnit* units, int nunits,
                        const ProductItem& single, int nzt, int nyt, bool cq, int mode) {
   if (nunits <= 0) return;
-  if (product_is_f16(mode)) {
+  if (product_is_f64(mode)) {
+    product64_launch(s, items, units, nunits, single, nzt, nyt, cq, mode == kProductF64Gauss);
+  } else if (product_is_f16(mode)) {
     if (cq) launch16_mode<true>(s, items, units, nunits, single, nzt, nyt, mode);
     else launch16_mode<false>(s, items, units, nunits, single, nzt, nyt, mode);
   } else if (cq) {
@@ -812,6 +814,14 @@ extern "C" int hgx_tune_product(hgx_ctx* h, int blocks_per_unit, int complex_mul
   c->tune.product_bpu = blocks_per_unit;
   c->tune.product_operands = operands;
   c->tune.product_mults = complex_mults;
+  return HGX_OK;
+}
+
+extern "C" int hgx_tune_product64(hgx_ctx* h, int complex_mults) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (complex_mults != 3 && complex_mults != 4) return HGX_ERR_INVALID_ARG;
+  c->tune.product64_mults = complex_mults;
   return HGX_OK;
 }
 

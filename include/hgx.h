@@ -51,7 +51,7 @@ extern "C" {
 /* flags for hgx_ff_numeric */
 #define HGX_FLAG_EXACT_REFERENCE_SIGN 1 /* exact method with the reference's exp(-iq.r) sign */
 #define HGX_FLAG_ACC_KAHAN 2            /* FP32C: per-pair Kahan instead of FP32-block/FP64-carry */
-#define HGX_FLAG_NO_MFMA 4              /* FP32C approx: per-pair VALU kernels even on a product grid */
+#define HGX_FLAG_NO_MFMA 4              /* approx: per-pair VALU kernels even on a product grid */
 #define HGX_FLAG_EXACT_CPU_THRESHOLD 8  /* exact method: the reference CPU's DBL_MIN degenerate-case
                                            threshold instead of its GPU path's 1e-13 */
 
@@ -483,6 +483,11 @@ int hgx_tune_numeric(hgx_ctx* ctx, int qpt64, int qpt32, int target_wg);
  * operand precision. Default (64, 4, HGX_PRODUCT_F16X2). Set by the autotuner. No reference
  * counterpart. */
 int hgx_tune_product(hgx_ctx* ctx, int blocks_per_unit, int complex_mults, int operands);
+
+/* Real products per complex product (4, default, or 3: Gauss's form) of the context's plain FP64
+ * product-grid kernel (v_mfma_f64_16x16x4_f64; HGX_PREC_FP64 approximate form factors on product
+ * grids, single calls and later batches). No reference counterpart. */
+int hgx_tune_product64(hgx_ctx* ctx, int complex_mults);
 
 /* Tile of the split-fp16 product-grid kernel. row_tiles: 32-row tiles per wave (1, 2 or 4); each
  * wave builds its column's operands once for all its rows. two_level = 1 adds each 128-triangle

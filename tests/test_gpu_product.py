@@ -227,3 +227,85 @@ def test_product_f16x2_scaling_edges(oracle_mod, case):
     finally:
         tune_product(c, *PRODUCT_DEFAULT)
         c.close()
+
+
+@pytest.mark.parametrize("ncol,nrow,ntri,dwba,rotated,chunks", [
+    (40, 23, 900, None, False, False),
+    (64, 64, 3000, None, True, True),
+    (33, 31, 60, None, True, False),           # fewer triangles than one step or block
+    (48, 20, 1500, 2e-6 + 1e-8j, True, True),  # complex branches (absorbing layer), nz = 4 * 20
+    (37, 45, 700, 0j, False, False),           # real branches
+    (130, 70, 20000, None, True, True),        # several 64 x 64 tiles each way, long chunks
+])
+@pytest.mark.parametrize("mults", [4, 3])
+def test_product_fp64_matches_oracle(oracle_mod, ncol, nrow, ntri, dwba, rotated, chunks, mults):
+    """The plain FP64 product-grid kernel (v_mfma_f64_16x16x4_f64, four or three real products per
+    complex product) against the oracle at the FP64 bar, and against the per-pair FP64 kernel."""
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Context, tune_numeric, tune_product, tune_product64
+
+    O = oracle_mod
+    rng = np.random.default_rng(ncol * 1000 + nrow + 7)
+    tri = meshes.convex_polyhedron(rng, rng.uniform(4, 12, 3), ntri)
+    R = _rot(0.3, -0.7, 1.1) if rotated else None
+    c = Context(0)
+    try:
+        tune_product64(c, mults)
+        if chunks:  # one block per chunk: many partials reduced by the finishing kernel
+            tune_product(c, 1, 4, "f16x2", 2, True, True)
+            tune_numeric(c, 2, 4, 1 << 16)
+        g = _grid(c, O, ncol, nrow, dwba)
+        assert c.is_product()
+        mid = c.upload_mesh(tri, "exact9")
+        ref = O.ff_approx(O.tri9_to_rec8(tri), g, R)
+        got = c.as_complex(c.ff_numeric(mid, "approx", "fp64", rot=R)).cpu().numpy()
+        pair = c.as_complex(c.ff_numeric(mid, "approx", "fp64", rot=R, flags=4)).cpu().numpy()
+        # rounding-level |q| (odd-sized grids): the direct sum is ill-conditioned in both codes
+        info = c.mesh_info(mid)
+        qn = np.sqrt((np.abs(g.q_rotated(R)) ** 2).sum(0))
+        ok = qn * info["radius"] >= 1e-6
+        e, e_pair = rel_l2(got[ok], ref[ok]), rel_l2(got[ok], pair[ok])
+        print(f"fp64 product {ncol}x{nrow} ntri={ntri} dwba={dwba} rot={rotated} mults={mults}: "
+              f"vs oracle {e:.2e}, vs per-pair {e_pair:.2e}")
+        assert np.all(np.isfinite(got[ok]))
+        assert e <= 1e-10 and e_pair <= 1e-10
+        c.release_mesh(mid)
+    finally:
+        c.close()
+
+
+def test_product_fp64_batch_matches_single_calls(oracle_mod):
+    """FP64 batch items on product grids run the FP64 product kernel: equal to single calls and to
+    the oracle, with rotations, on grids of several contexts, graph replay included."""
+    import torch
+
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Batch, Context
+
+    O = oracle_mod
+    rng = np.random.default_rng(19)
+    main = Context(0)
+    grids = {s: Context(0) for s in (24, 70)}
+    try:
+        for s, gc in grids.items():
+            gc.qgrid_explicit(s, s, -2.0, 0.0, 2.0, 2.0, AI, K0)
+        shapes = [(meshes.convex_polyhedron(rng, rng.uniform(3, 12, 3), int(n)), s, _rot(0.2 * k, 0.1, -0.3 * k))
+                  for k, (n, s) in enumerate(((400, 24), (2500, 70), (9000, 70), (150, 24)))]
+        mids = [main.upload_mesh(t, "exact9") for t, _, _ in shapes]
+        outs = [torch.empty(2 * s * s, dtype=torch.float64, device="cuda") for _, s, _ in shapes]
+        b = Batch(main, [(grids[s], m, R, o) for (t, s, R), m, o in zip(shapes, mids, outs)], precision="fp64")
+        for use_graph in (False, True, True):
+            for o in outs:
+                o.fill_(float("nan"))
+            b.run(use_graph=use_graph)
+            torch.cuda.synchronize()
+            for (t, s, R), o in zip(shapes, outs):
+                g = O.qgrid_explicit(s, s, -2.0, 0.0, 2.0, 2.0, AI, K0)
+                ref = O.ff_approx(O.tri9_to_rec8(t), g, R)
+                got = main.as_complex(o).cpu().numpy()
+                assert rel_l2(got, ref) <= 1e-10, (s, use_graph)
+        b.release()
+    finally:
+        main.close()
+        for gc in grids.values():
+            gc.close()
