@@ -315,10 +315,10 @@ def bench_lattice(reps=5):
     pix = pixel_errors()
     pix.add(res["fp32c"], res["fp64"])
     out["pixel_error_fp32c_vs_fp64"] = pix.result()
-    # VALU roofline: per (q, node, grain) sample the sphere form factor (about 55 flop), per (q, grain)
-    # the lattice factor (about 155 flop) and the DWBA term (about 15 flop per branch); transcendentals
-    # not counted
-    flop_step = 4 * 1024 * 1024 * 8 * (v.size * 55 + 155 + 15)
+    # VALU roofline on the work the kernel must do: the 8 azimuths rotate about z, so the sphere form
+    # factor (about 55 flop per (q, node)) is evaluated once per q-point and branch; per (q, grain) the
+    # lattice factor (about 155 flop) and the DWBA term (about 15 flop); transcendentals not counted
+    flop_step = 4 * 1024 * 1024 * (v.size * 55 + 8 * (155 + 15))
     for prec, peak in (("fp32c", PEAK_FP32), ("fp64", PEAK_FP64)):
         a = flop_step / (out[prec]["kernel_ms"] * 1e-3)
         out[prec]["roofline"] = {"bound": "valu", "achieved": a / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
@@ -507,11 +507,12 @@ def bench_hig_examples(reps=3, cpu=True):
             sim.close()
         ng_ = 625 if name != "01-pyramid" else 1
         if name.startswith(("02", "03")):
-            # fused lattice kernel, VALU roofline with the lattice leg's flop model: per (q, node, grain)
-            # the particle form factor (sphere about 55 flop; cylinder 60), per (q, grain) the lattice
-            # factor (155) and the DWBA term (15 per branch); transcendentals not counted
+            # fused ensemble kernel, VALU roofline with the lattice leg's flop model: the grains rotate
+            # about z, so the particle form factor (sphere about 55 flop; cylinder 60) is evaluated once
+            # per (q, branch); per (q, grain) the lattice factor (155) and the DWBA term (15 per
+            # branch); transcendentals not counted
             ffl = 55 if name.startswith("02") else 60
-            flop = 4 * img.size * ng_ * (ffl + 155 + 15)
+            flop = 4 * img.size * (ffl + ng_ * (155 + 15))
             for prec, peak in (("fp64", PEAK_FP64), ("fp32c", PEAK_FP32)):
                 a_ = flop / (res[prec]["kernels"]["main_ms"] * 1e-3)
                 res[prec]["roofline"] = {"bound": "valu", "achieved": a_ / 1e12, "peak": peak / 1e12,

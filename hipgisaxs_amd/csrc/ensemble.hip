@@ -67,9 +67,31 @@ __device__ inline void rotate_m(const double* __restrict__ m, double qx, double 
   }
 }
 
-// SHAPE >= 0: every element has that shape (one form-factor function inlined, fewer registers);
-// SHAPE = -1: mixed cells, the shape chosen per element.
+// The unit cell's form-factor sum for grain o at one q-point:
+//     sum_e dn2_e F_e(M_oe q) sum_l exp(i (M_oe q).t_l).
 template <int SHAPE>
+__device__ inline cd cell64(int o, double qx, double qy, cd qz, const EnsElem* __restrict__ E, int nelem,
+                            const double* __restrict__ loc, const double* __restrict__ M) {
+  cd ff = cd{0.0, 0.0};
+  for (int e = 0; e < nelem; ++e) {
+    cd mq[3];
+    rotate_m(M + 9 * ((int64_t)o * nelem + e), qx, qy, qz, mq);
+    const cd F = SHAPE >= 0 ? shape_ff<(SHAPE >= 0 ? SHAPE : 0)>(mq, E[e].P) : ff_any(E[e].shape, mq, E[e].P);
+    const cd dn2 = cd{E[e].dn2r, E[e].dn2i};
+    for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
+      const double* t = loc + 3 * l;
+      ff = ff + dn2 * (F * cexpi_d(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+    }
+  }
+  return ff;
+}
+
+// SHAPE >= 0: every element has that shape (one form-factor function inlined, fewer registers);
+// SHAPE = -1: mixed cells, the shape chosen per element. HOIST: the cell's form-factor sum is the same
+// for every grain (the host checked it: spheres and cylinders whose F depends on q only through q.q
+// and the third row of R_g R_e, which every grain shares, with locations that every grain sees
+// alike), so it is evaluated once per branch and the grain loop runs the structure factor alone.
+template <int SHAPE, bool HOIST>
 __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const double* __restrict__ fc,
                                                     const EnsElem* __restrict__ E, int nelem,
                                                     const double* __restrict__ loc, const double* __restrict__ M,
@@ -79,13 +101,14 @@ __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const doub
   if (p >= g.npix) return;
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
-  cd qzb[4], fcb[4];
+  cd qzb[4], fcb[4], ffh[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     if (b >= nb) break;
     const int64_t q = p + b * g.npix;
     qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
     fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+    if (HOIST) ffh[b] = fcb[b] * cell64<SHAPE>(g0, qx, qy, qzb[b], E, nelem, loc, M);
   }
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
@@ -100,18 +123,22 @@ __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const doub
       cd mg[3];
       rotate_m(G[o].rot, qx, qy, qzb[b], mg);
       const cd S = sf_value(mg, sf);
-      cd ff = cd{0.0, 0.0};
-      for (int e = 0; e < nelem; ++e) {
-        cd mq[3];
-        rotate_m(M + 9 * ((int64_t)o * nelem + e), qx, qy, qzb[b], mq);
-        const cd F = SHAPE >= 0 ? shape_ff<(SHAPE >= 0 ? SHAPE : 0)>(mq, E[e].P) : ff_any(E[e].shape, mq, E[e].P);
-        const cd dn2 = cd{E[e].dn2r, E[e].dn2i};
-        for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
-          const double* t = loc + 3 * l;
-          ff = ff + dn2 * (F * cexpi_d(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+      if (HOIST) {
+        amp = amp + ffh[b] * S;
+      } else {
+        cd ff = cd{0.0, 0.0};
+        for (int e = 0; e < nelem; ++e) {
+          cd mq[3];
+          rotate_m(M + 9 * ((int64_t)o * nelem + e), qx, qy, qzb[b], mq);
+          const cd F = SHAPE >= 0 ? shape_ff<(SHAPE >= 0 ? SHAPE : 0)>(mq, E[e].P) : ff_any(E[e].shape, mq, E[e].P);
+          const cd dn2 = cd{E[e].dn2r, E[e].dn2i};
+          for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
+            const double* t = loc + 3 * l;
+            ff = ff + dn2 * (F * cexpi_d(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+          }
         }
+        amp = amp + fcb[b] * S * ff;
       }
-      amp = amp + fcb[b] * S * ff;
     }
     amp = G[o].w * amp;
     acc += amp.r * amp.r + amp.i * amp.i;
@@ -219,7 +246,29 @@ __device__ inline cf sf32e(const double m[3], const SFArgs& s) {
   return d * ph;
 }
 
+// FP32C form-factor sum of the unit cell for grain o (FP64 result of FP32 terms).
 template <int SHAPE>
+__device__ inline cd cell32(int o, double qx, double qy, double qz, const EnsElem* __restrict__ E, int nelem,
+                            const double* __restrict__ loc, const double* __restrict__ M) {
+  cd ff = cd{0.0, 0.0};
+  for (int e = 0; e < nelem; ++e) {
+    const double* Me = M + 9 * ((int64_t)o * nelem + e);
+    double mq[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) mq[j] = Me[3 * j] * qx + Me[3 * j + 1] * qy + Me[3 * j + 2] * qz;
+    const cf F = ff32<SHAPE>(mq, E[e].P);
+    cf ph = cf{0.0f, 0.0f};
+    for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
+      const double* t = loc + 3 * l;
+      ph = ph + expi_turn(turns_of(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+    }
+    const cf Fp = F * ph;
+    ff = ff + cd{E[e].dn2r, E[e].dn2i} * cd{(double)Fp.r, (double)Fp.i};
+  }
+  return ff;
+}
+
+template <int SHAPE, bool HOIST>
 __global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const double* __restrict__ fc,
                                                     const EnsElem* __restrict__ E, int nelem,
                                                     const double* __restrict__ loc, const double* __restrict__ M,
@@ -230,13 +279,14 @@ __global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const doub
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
   double qzb[4];
-  cd fcb[4];
+  cd fcb[4], ffh[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     if (b >= nb) break;
     const int64_t q = p + b * g.npix;
     qzb[b] = g.qz[2 * q];
     fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+    if (HOIST) ffh[b] = fcb[b] * cell32<SHAPE>(g0, qx, qy, qzb[b], E, nelem, loc, M);
   }
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
@@ -252,22 +302,12 @@ __global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const doub
 #pragma unroll
       for (int j = 0; j < 3; ++j) mg[j] = G[o].rot[3 * j] * qx + G[o].rot[3 * j + 1] * qy + G[o].rot[3 * j + 2] * qzb[b];
       const cf S = sf32e(mg, sf);
-      cd ff = cd{0.0, 0.0};
-      for (int e = 0; e < nelem; ++e) {
-        const double* Me = M + 9 * ((int64_t)o * nelem + e);
-        double mq[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) mq[j] = Me[3 * j] * qx + Me[3 * j + 1] * qy + Me[3 * j + 2] * qzb[b];
-        const cf F = ff32<SHAPE>(mq, E[e].P);
-        cf ph = cf{0.0f, 0.0f};
-        for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
-          const double* t = loc + 3 * l;
-          ph = ph + expi_turn(turns_of(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
-        }
-        const cf Fp = F * ph;
-        ff = ff + cd{E[e].dn2r, E[e].dn2i} * cd{(double)Fp.r, (double)Fp.i};
+      const cd Sd = cd{(double)S.r, (double)S.i};
+      if (HOIST) {
+        amp = amp + ffh[b] * Sd;
+      } else {
+        amp = amp + fcb[b] * (Sd * cell32<SHAPE>(o, qx, qy, qzb[b], E, nelem, loc, M));
       }
-      amp = amp + fcb[b] * (cd{(double)S.r, (double)S.i} * ff);
     }
     amp = G[o].w * amp;
     acc += amp.r * amp.r + amp.i * amp.i;
@@ -290,10 +330,40 @@ using namespace hgx;
 
 static const int kEnsDims[7] = {1, 2, 3, 1, 2, 2, 4};
 
-extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, int precision, const double* fc,
-                                      int accumulate, double* I) {
-  Context* c = reinterpret_cast<Context*>(h);
-  if (!c) return HGX_ERR_BAD_HANDLE;
+namespace hgx {
+
+// Whether the cell's form-factor sum is the same for every grain: every element a sphere or a
+// cylinder (F depends on q only through q.q and (M q)_z, so on the third row of M = R_g R_e), that row
+// equal for every grain, and every location seen alike (M_ge^T t_l equal for every grain, so the
+// phases (M q).t_l are). Rotations about z of cells at the origin, the reference's orientation
+// distributions of spheres and cylinders, meet it.
+static bool cell_is_grain_invariant(const hgx_ensemble_model* m, const std::vector<double>& Mm) {
+  const int ne = m->nelem, ng = m->ngrains;
+  auto close = [](double a, double b, double scale) { return std::fabs(a - b) <= 1e-14 * scale; };
+  for (int e = 0; e < ne; ++e) {
+    const int sh = m->elem[e].shape;
+    if (sh != HGX_SHAPE_SPHERE && sh != HGX_SHAPE_CYLINDER) return false;
+    const double* M0 = &Mm[9 * (size_t)e];
+    for (int o = 1; o < ng; ++o) {
+      const double* Mo = &Mm[9 * ((size_t)o * ne + e)];
+      for (int k = 6; k < 9; ++k)
+        if (!close(Mo[k], M0[k], 1.0)) return false;
+      for (int l = 0; l < m->elem[e].nloc; ++l) {
+        const double* t = m->elem[e].loc + 3 * l;
+        const double tn = std::fabs(t[0]) + std::fabs(t[1]) + std::fabs(t[2]);
+        for (int j = 0; j < 3; ++j) {
+          const double u0 = M0[j] * t[0] + M0[3 + j] * t[1] + M0[6 + j] * t[2];
+          const double uo = Mo[j] * t[0] + Mo[3 + j] * t[1] + Mo[6 + j] * t[2];
+          if (!close(uo, u0, tn)) return false;
+        }
+      }
+    }
+  }
+  return true;
+}
+
+static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, const double* fc, int accumulate,
+                        double* I) {
   if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ensemble_intensity: no q-grid");
   if (!m || !I || m->nelem < 1 || !m->elem || m->ngrains < 1 || !m->rot)
     return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: need >= 1 element, >= 1 grain and an output");
@@ -364,6 +434,7 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, i
         }
     }
   }
+  const bool hoist = cell_is_grain_invariant(m, Mm);
   SFArgs sfb;
   for (int k = 0; k < 3; ++k) {
     sfb.la[k] = m->la[k];
@@ -391,16 +462,22 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, i
   double* dloc = reinterpret_cast<double*>(w + b_part + b_E);
   double* dM = reinterpret_cast<double*>(w + b_part + b_E + b_loc);
   EnsGrain* dG = reinterpret_cast<EnsGrain*>(w + b_part + b_E + b_loc + b_M);
-  HGX_CHECK_HIP(c, hipMemcpyAsync(dE, E.data(), ne * sizeof(EnsElem), hipMemcpyHostToDevice, c->stream));
-  HGX_CHECK_HIP(c, hipMemcpyAsync(dloc, loc.data(), loc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HGX_CHECK_HIP(c, hipMemcpyAsync(dM, Mm.data(), Mm.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HGX_CHECK_HIP(c, hipMemcpyAsync(dG, G.data(), ng * sizeof(EnsGrain), hipMemcpyHostToDevice, c->stream));
+  {
+    const void* src[4] = {E.data(), loc.data(), Mm.data(), G.data()};
+    const size_t nb[4] = {ne * sizeof(EnsElem), loc.size() * sizeof(double), Mm.size() * sizeof(double),
+                          ng * sizeof(EnsGrain)};
+    if (int rc = stage_to_device(c, 4, src, nb, reinterpret_cast<char*>(dE))) return rc;
+  }
   QViewE g{c->d_qx, c->d_qy, c->d_qz, c->npix};
   if (precision == HGX_PREC_FP32C) {
     LaunchTimer lt(c, true);
     const dim3 grid(nbx, ny);
-#define HGX_ENS32_LAUNCH(S) \
-  k_ensemble32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
+#define HGX_ENS32_LAUNCH(S)                                                                                    \
+  if (hoist)                                                                                                   \
+    k_ensemble32<S, true><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, \
+                                                       part);                                                  \
+  else                                                                                                         \
+    k_ensemble32<S, false><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
     switch (common) {
       case HGX_SHAPE_SPHERE: HGX_ENS32_LAUNCH(HGX_SHAPE_SPHERE); break;
       case HGX_SHAPE_CYLINDER: HGX_ENS32_LAUNCH(HGX_SHAPE_CYLINDER); break;
@@ -412,8 +489,12 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, i
   } else {
     LaunchTimer lt(c, true);
     const dim3 grid(nbx, ny);
-#define HGX_ENS_LAUNCH(S) \
-  k_ensemble64<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
+#define HGX_ENS_LAUNCH(S)                                                                                      \
+  if (hoist)                                                                                                   \
+    k_ensemble64<S, true><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, \
+                                                       part);                                                  \
+  else                                                                                                         \
+    k_ensemble64<S, false><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
     switch (common) {
       case HGX_SHAPE_SPHERE: HGX_ENS_LAUNCH(HGX_SHAPE_SPHERE); break;
       case HGX_SHAPE_CYLINDER: HGX_ENS_LAUNCH(HGX_SHAPE_CYLINDER); break;
@@ -432,7 +513,55 @@ extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, i
     k_ensemble_sum<<<nbx, 256, 0, c->stream>>>(c->npix, ny, part, accumulate, I);
     HGX_CHECK_LAUNCH(c);
   }
-  // the host tables must outlive the asynchronous copies
-  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return HGX_OK;
+}
+
+}  // namespace hgx
+
+extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, int precision, const double* fc,
+                                      int accumulate, double* I) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  return ensemble_run(c, m, precision, fc, accumulate, I);
+}
+
+// A lattice of one particle shape at the cell origin: the ensemble path with one element (identity
+// rotation, one location at the origin) and every grain at the model's centre.
+extern "C" int hgx_lattice_intensity(hgx_ctx* h, const hgx_lattice_model* m, int precision, const double* fc,
+                                     int accumulate, double* I) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!m || !I || !m->rot || m->ngrains < 1)
+    return set_error(c, HGX_ERR_INVALID_ARG, "lattice_intensity: need a model with >= 1 grain and an output");
+  if (m->shape != HGX_SHAPE_SPHERE && m->shape != HGX_SHAPE_CYLINDER)
+    return set_error(c, HGX_ERR_UNSUPPORTED, "lattice: shape %d not supported (sphere, cylinder)", m->shape);
+  if (m->ngrains > (1 << 24)) return set_error(c, HGX_ERR_INVALID_ARG, "lattice_intensity: too many grains");
+  static const double origin[3] = {0.0, 0.0, 0.0};
+  hgx_ensemble_element el{};
+  el.shape = m->shape;
+  el.params = m->params;
+  for (int k = 0; k < 9; ++k) el.rot[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  el.nloc = 1;
+  el.loc = origin;
+  el.dn2_re = m->dn2_re;
+  el.dn2_im = m->dn2_im;
+  std::vector<double> centers(3 * (size_t)m->ngrains);
+  for (int o = 0; o < m->ngrains; ++o)
+    for (int k = 0; k < 3; ++k) centers[3 * (size_t)o + k] = m->center[k];
+  hgx_ensemble_model em{};
+  em.nelem = 1;
+  em.elem = &el;
+  for (int k = 0; k < 3; ++k) {
+    em.la[k] = m->la[k];
+    em.lb[k] = m->lb[k];
+    em.lc[k] = m->lc[k];
+    em.repet[k] = m->repet[k];
+  }
+  em.dw_sigma = m->dw_sigma;
+  em.sf_flags = m->sf_flags;
+  em.ngrains = m->ngrains;
+  em.rot = m->rot;
+  em.center = centers.data();
+  em.weight = m->weight;
+  return ensemble_run(c, &em, precision, fc, accumulate, I);
 }

@@ -45,6 +45,36 @@ int ensure_work(Context* c, size_t bytes) {
   return HGX_OK;
 }
 
+int stage_to_device(Context* c, int n, const void* const* src, const size_t* bytes, char* dst) {
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += (bytes[i] + 255) / 256 * 256;
+  if (total == 0) return HGX_OK;
+  if (!c->stage_done) HGX_CHECK_HIP(c, hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming));
+  // the previous staged copy must have read the buffer before it is rewritten
+  HGX_CHECK_HIP(c, hipEventSynchronize(c->stage_done));
+  if (total > c->stage_bytes) {
+    if (c->h_stage) HGX_CHECK_HIP(c, hipHostFree(c->h_stage));
+    c->h_stage = nullptr;
+    c->stage_bytes = 0;
+    const size_t sz = std::max<size_t>(total, (size_t)1 << 16);
+    if (hipHostMalloc(&c->h_stage, sz, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      c->h_stage = nullptr;
+      return set_error(c, HGX_ERR_OOM, "pinned staging allocation of %zu bytes failed", sz);
+    }
+    c->stage_bytes = sz;
+  }
+  char* h = static_cast<char*>(c->h_stage);
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bytes[i]) std::memcpy(h + off, src[i], bytes[i]);
+    off += (bytes[i] + 255) / 256 * 256;
+  }
+  HGX_CHECK_HIP(c, hipMemcpyAsync(dst, h, total, hipMemcpyHostToDevice, c->stream));
+  HGX_CHECK_HIP(c, hipEventRecord(c->stage_done, c->stream));
+  return HGX_OK;
+}
+
 LaunchTimer::LaunchTimer(Context* ctx, bool is_main) : c(ctx), main(is_main) {
   if (!c->profiling) return;
   if (!c->ev_pool.empty()) {
@@ -301,6 +331,8 @@ int hgx_destroy(hgx_ctx* h) {
     }
   if (c->d_work) (void)hipFree(c->d_work);
   if (c->d_flag) (void)hipFree(c->d_flag);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->stage_done) (void)hipEventDestroy(c->stage_done);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return HGX_OK;

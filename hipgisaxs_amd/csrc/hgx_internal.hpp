@@ -224,6 +224,10 @@ struct Context {
   // device address of the device's static scratch array (kScratchBytes), shared by every context on
   // the device; null when it could not be resolved
   char* d_scratch = nullptr;
+  // pinned host staging of small per-call tables (stage_to_device), reused once its last copy is done
+  void* h_stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
   std::string last_error;
   // optional HIP-event timing of kernel launches (hgx_profile_begin / hgx_profile_end)
   bool profiling = false;
@@ -256,6 +260,10 @@ int set_error(Context* c, int code, const char* fmt, ...);
 #define HGX_CHECK_LAUNCH(ctx) HGX_CHECK_HIP(ctx, hipGetLastError())
 
 int ensure_work(Context* c, size_t bytes);
+// Copies n host tables (sizes in bytes) into device memory dst, each at a 256-byte aligned offset,
+// through the context's pinned staging buffer and one asynchronous copy on the context's stream; the
+// caller's tables may be released on return (no stream synchronisation).
+int stage_to_device(Context* c, int n, const void* const* src, const size_t* bytes, char* dst);
 int set_device(Context* c);
 
 // ----- device helpers ------------------------------------------------------------------------

@@ -275,7 +275,8 @@ typedef struct hgx_lattice_model {
 /* I[p] (+)= sum_g |w_g dn2 sum_b fc_b F(R_g q_b) S(R_g q_b)|^2 over the grid's branches b: the grain
  * loop, form factor, structure factor and DWBA sum of HipGISAXS::run_gisaxs
  * (src/sim/hipgisaxs_main.cpp:925-1100) fused into one pass per pixel. fc (device, [2*nq]) may be
- * NULL for the Born approximation. FP32C requires real q. */
+ * NULL for the Born approximation. FP32C requires real q. Runs as hgx_ensemble_intensity with one
+ * element at the cell origin (so rotations about z share one form-factor evaluation). */
 int hgx_lattice_intensity(hgx_ctx* ctx, const hgx_lattice_model* model, int precision, const double* fc,
                           int accumulate, double* I);
 
@@ -312,7 +313,11 @@ typedef struct hgx_ensemble_model {
  * [2*nq]) may be NULL (Born). HGX_PREC_FP64 equals hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice +
  * hgx_dwba_intensity per grain to round-off; HGX_PREC_FP32C (real q, cells whose elements are all
  * spheres, all cylinders, all boxes or all cubes) evaluates the form factors and Dirichlet kernels in
- * FP32 on phases reduced in FP64 and sums amplitudes in FP64. Synchronises the stream. */
+ * FP32 on phases reduced in FP64 and sums amplitudes in FP64. When every grain sees the cell alike
+ * (spheres and cylinders whose grain rotations share their third row, e.g. rotations about z, at
+ * locations every grain sees alike) the cell's form factor is evaluated once per q-point and branch
+ * and only the structure factor per grain. Host tables are staged through pinned memory: no stream
+ * synchronisation. */
 int hgx_ensemble_intensity(hgx_ctx* ctx, const hgx_ensemble_model* model, int precision, const double* fc,
                            int accumulate, double* I);
 
