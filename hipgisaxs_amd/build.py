@@ -72,23 +72,31 @@ def build_peaks() -> Path:
     return PEAKS_BIN
 
 
-HIG_SRCS = [PKG / "host" / n for n in ("hig_gpu.cpp", "hig_input.cpp", "hig_sim.cpp", "hig_fit.cpp")]
+ADAPTER_SRCS = [PKG / "host" / "hgx_adapter.cpp"]
+ADAPTER_LIB = PKG / "libhgx_adapter.so"
+HIG_SRCS = [PKG / "host" / n for n in ("hig_input.cpp", "hig_sim.cpp", "hig_fit.cpp")]
 HIG_LIB = PKG / "libhgx_hig.so"
 SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
 SHIM_LIB = ROOT / "tests" / "cpp" / "libhig_shim.so"
 
 
 def build_host_adapters() -> Path:
-    """The reference-interface C++ classes and the .hig driver (host code on the C ABI), and the test
-    shim that calls the classes."""
+    """The plain-type library behind the reference-interface C++ classes (libhgx_adapter.so), the .hig
+    driver (libhgx_hig.so; host code on the C ABI), and the test shim that compiles the classes of
+    host/hig_gpu.hpp against stand-ins of the reference's type headers (tests/cpp/refapi) and calls
+    them as the reference does."""
     cxx = shutil.which("g++") or "g++"
     common = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'host'}"]
+    tmp = ADAPTER_LIB.with_suffix(".so.tmp")
+    _run([cxx, *common, *map(str, ADAPTER_SRCS), "-o", str(tmp), f"-L{PKG}", "-lhgx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, ADAPTER_LIB)
     tmp = HIG_LIB.with_suffix(".so.tmp")
-    _run([cxx, *common, *map(str, HIG_SRCS), "-o", str(tmp), f"-L{PKG}", "-lhgx", "-Wl,-rpath,$ORIGIN"])
+    _run([cxx, *common, *map(str, HIG_SRCS), "-o", str(tmp), f"-L{PKG}", "-lhgx_adapter", "-lhgx",
+          "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HIG_LIB)
     tmp = SHIM_LIB.with_suffix(".so.tmp")
-    _run([cxx, *common, str(SHIM_SRC), "-o", str(tmp), f"-L{PKG}", "-lhgx_hig", "-lhgx",
-          "-Wl,-rpath,$ORIGIN/../../hipgisaxs_amd"])
+    _run([cxx, *common, f"-I{ROOT / 'tests' / 'cpp' / 'refapi'}", str(SHIM_SRC), "-o", str(tmp), f"-L{PKG}",
+          "-lhgx_adapter", "-lhgx", "-Wl,-rpath,$ORIGIN/../../hipgisaxs_amd"])
     os.replace(tmp, SHIM_LIB)
     return HIG_LIB
 

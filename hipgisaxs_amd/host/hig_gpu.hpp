@@ -1,189 +1,296 @@
-// hig_gpu.hpp -- drop-in C++ classes with the reference's GPU-class interfaces, implemented on the
-// hgx C ABI (include/hgx.h). A HipGISAXS build links libhgx_hig.so + libhgx.so in place of its CUDA
-// objects; the callers (FormFactor::compute_form_factor, StructureFactor, run_gisaxs) keep their
-// shape. Every method returns the reference's success/failure value (counts or bool), prints the
-// reason to std::cerr like the reference does, and never exits the process.
+// hig_gpu.hpp -- drop-in replacements of the reference's GPU classes NumericFormFactorG,
+// AnalyticFormFactorG and StructureFactorG, with the reference's names and signatures
+// (include/ff/gpu/ff_num_gpu.cuh:31-157, include/ff/gpu/ff_ana_gpu.cuh:28-96,
+// include/sf/gpu/sf_gpu.cuh:30-70), implemented on libhgx_adapter.so (hgx_adapter.hpp) and the hgx
+// C ABI.
+//
+// This header is included inside the reference's own translation units, in place of the three CUDA
+// headers. It defines none of the reference's types: real_t, complex_t, cucomplex_t, triangle_t,
+// vector3_t, RotMatrix_t, Lattice and QGrid come from the reference's headers included below, and
+// the classes are header-only, so they compile against exactly the definitions the calling code
+// sees. What crosses into the library is plain doubles. The reference is built in double precision
+// (DOUBLEP): real_t must be double and cucomplex_t two doubles.
+//
+// Ownership and error conventions are the reference's: compute_approx_triangle and
+// compute_exact_triangle allocate ff with new[] for the caller to delete[] and return the triangle
+// count (0 on error); the others return bool; errors go to std::cerr; nothing exits the process.
+// AnalyticFormFactorG::init and StructureFactorG::init read the q-grid from the QGrid singleton, as
+// the reference's GPU classes do (src/ff/gpu/ff_ana_gpu.cu:57-98, src/sf/gpu/sf_gpu.cu:45-116).
 #pragma once
 
 #include <complex>
+#include <iostream>
+#include <new>
 #include <string>
 #include <vector>
 
-#include "hgx.h"
+#include <common/globals.hpp>    // vector3_t (include/common/globals.hpp:68-135)
+#include <common/typedefs.hpp>   // real_t, complex_t, cucomplex_t, triangle_t (include/common/typedefs.hpp:38-106)
+#include <model/qgrid.hpp>       // QGrid singleton (include/model/qgrid.hpp)
+#include <model/structure.hpp>   // Lattice (include/model/structure.hpp:86-147)
+#include <numerics/matrix.hpp>   // RotMatrix_t (include/numerics/matrix.hpp:30-140)
+
+#include "hgx_adapter.hpp"
 
 namespace hig {
 
-typedef double real_t;                  // DOUBLEP build (include/common/typedefs.hpp:47-65)
-typedef std::complex<real_t> complex_t; // layout-identical to cuDoubleComplex
-typedef std::vector<real_t> real_vec_t;
-typedef std::vector<complex_t> complex_vec_t;
+static_assert(sizeof(real_t) == sizeof(double), "the hgx adapters take the reference's DOUBLEP build");
+static_assert(sizeof(cucomplex_t) == 2 * sizeof(double), "cucomplex_t must be two doubles");
+static_assert(sizeof(complex_t) == 2 * sizeof(double), "complex_t must be two doubles");
+static_assert(sizeof(triangle_t) == 9 * sizeof(double), "triangle_t must be nine doubles");
 
-// include/common/typedefs.hpp:102-106
-typedef struct Triangle {
-  real_t v1[3];
-  real_t v2[3];
-  real_t v3[3];
-} triangle_t;
+namespace hgx_detail {
 
-class vector3_t {
- public:
-  real_t vec_[3];
-  vector3_t() : vec_{0, 0, 0} {}
-  vector3_t(real_t a, real_t b, real_t c) : vec_{a, b, c} {}
-  real_t& operator[](int i) { return vec_[i]; }
-  real_t operator[](int i) const { return vec_[i]; }
-};
+// The nine entries of a RotMatrix_t through its public interface (its data_ is private): column j is
+// the rotation of the unit vector e_j.
+inline void rot_entries(RotMatrix_t r, double m[9]) {
+  for (int j = 0; j < 3; ++j) {
+    std::vector<complex_t> col = r.rotate(j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, complex_t(j == 2 ? 1.0 : 0.0, 0.0));
+    for (int i = 0; i < 3; ++i) m[3 * i + j] = col[i].real();
+  }
+}
 
-// Row-major rotation, include/numerics/matrix.hpp:30-140: RotMatrix_t(axis, angle) and products.
-class RotMatrix_t {
- public:
-  RotMatrix_t();
-  RotMatrix_t(int axis, real_t angle);
-  RotMatrix_t operator*(const RotMatrix_t& rhs) const;
-  vector3_t operator*(const vector3_t& rhs) const;
-  const real_t* data() const { return data_; }
-  real_t data_[9];
-};
+// The q-grid of the QGrid singleton: qx, qy per pixel (nqy), qz extended (nqz).
+inline void singleton_grid(unsigned nqy, unsigned nqz, std::vector<double>& qx, std::vector<double>& qy,
+                           std::vector<std::complex<double>>& qz) {
+  qx.resize(nqy);
+  qy.resize(nqy);
+  qz.resize(nqz);
+  for (unsigned i = 0; i < nqy; ++i) {
+    qx[i] = QGrid::instance().qx(i);
+    qy[i] = QGrid::instance().qy(i);
+  }
+  for (unsigned i = 0; i < nqz; ++i) {
+    const complex_t z = QGrid::instance().qz_extended(i);
+    qz[i] = std::complex<double>(z.real(), z.imag());
+  }
+}
 
-// One hgx context plus the q-grid cache shared by the adapters below.
-class GpuContext {
- public:
-  explicit GpuContext(int device = 0);
-  ~GpuContext();
-  GpuContext(const GpuContext&) = delete;
-  GpuContext& operator=(const GpuContext&) = delete;
-  hgx_ctx* get() const { return ctx_; }
-  bool ok() const { return ctx_ != nullptr; }
-  // Uploads (qx, qy [nqy], qz [nqz]) unless identical to the last upload; nqz must be nqy or 4*nqy.
-  bool set_grid(unsigned nqy, const real_t* qx, const real_t* qy, unsigned nqz, const complex_t* qz,
-                const real_t* alpha_f = nullptr, unsigned nrow = 1);
-  // Device scratch of at least bytes (grows).
-  void* scratch(size_t bytes, int slot = 0);
-  bool fail(const char* where, int rc) const;
-  double last_kernel_ms() const { return last_ms_; }
-  void begin_timing();
-  void end_timing();
+inline std::vector<double> to_vec(const std::vector<real_t>& v) { return std::vector<double>(v.begin(), v.end()); }
 
- private:
-  hgx_ctx* ctx_ = nullptr;
-  unsigned long long grid_hash_ = 0;
-  unsigned nqy_ = 0, nqz_ = 0;
-  void* scratch_[4] = {nullptr, nullptr, nullptr, nullptr};
-  size_t scratch_bytes_[4] = {0, 0, 0, 0};
-  double last_ms_ = 0.0;
-};
+}  // namespace hgx_detail
 
-// Replaces NumericFormFactorG (include/ff/gpu/ff_num_gpu.cuh:31-157). The block-size constructor
-// arguments of the reference are accepted and ignored (the launch shape is chosen by hgx).
+// include/ff/gpu/ff_num_gpu.cuh:31-157. The block shapes of the constructors are accepted and
+// ignored (hgx chooses its launch shapes). precision() (HGX_PREC_FP64 by default, or HGX_PREC_FP32C)
+// is an extension.
 class NumericFormFactorG {
  public:
-  NumericFormFactorG();
-  explicit NumericFormFactorG(int block_cuda);
-  NumericFormFactorG(int block_cuda_t, int block_cuda_y, int block_cuda_z);
-  ~NumericFormFactorG();
+  explicit NumericFormFactorG(int /*block_cuda*/) {}
+  NumericFormFactorG(int /*block_cuda_t*/, int /*block_cuda_y*/, int /*block_cuda_z*/) {}
+  NumericFormFactorG() {}
+  ~NumericFormFactorG() {}
 
-  // ff_tri_gpu.cu:395-521: shape_def = 8 reals per triangle (s, nx, ny, nz, cx, cy, cz, pad);
-  // allocates ff = new complex_t[nqz] (the caller delete[]s it); returns the triangle count, 0 on
-  // error; kernel_time in ms.
-  unsigned int compute_approx_triangle(std::vector<real_t>& shape_def, complex_t*& ff, int nqy, real_t* qx,
-                                       real_t* qy, int nqz, complex_t* qz, RotMatrix_t& rot, real_t& kernel_time);
-  // ff_tri_cpu.cpp:159-195 (exact method); same ownership and return convention.
-  unsigned int compute_exact_triangle(triangle_t* shape_def, int num_triangles, complex_t*& ff, int nqy,
-                                      real_t* qx, real_t* qy, int nqz, complex_t* qz, RotMatrix_t& rot,
-                                      real_t& kernel_time);
-  // Extension: HGX_PREC_FP64 (default, plain) or HGX_PREC_FP32C.
+  // src/ff/gpu/ff_tri_gpu.cu:395-521; shape_def: 8 reals per triangle (s, nx, ny, nz, cx, cy, cz, pad)
+  unsigned int compute_approx_triangle(std::vector<real_t>& shape_def, cucomplex_t*& ff, int nqy, real_t* qx,
+                                       real_t* qy, int nqz, cucomplex_t* qz, RotMatrix_t& rot, real_t& kernel_time) {
+    return run(HGX_MESH_APPROX8, shape_def.data(), (int)(shape_def.size() / 8), HGX_FF_APPROX, precision_, ff, nqy,
+               qx, qy, nqz, qz, rot, kernel_time);
+  }
+  // src/ff/gpu/ff_tri_gpu.cu:319-390 (exact method, FP64)
+  unsigned int compute_exact_triangle(triangle_t* shape_def, int num_triangles, cucomplex_t*& ff, int nqy,
+                                      real_t* qx, real_t* qy, int nqz, cucomplex_t* qz, RotMatrix_t& rot,
+                                      real_t& kernel_time) {
+    return run(HGX_MESH_EXACT9, reinterpret_cast<const real_t*>(shape_def), num_triangles, HGX_FF_EXACT,
+               HGX_PREC_FP64, ff, nqy, qx, qy, nqz, qz, rot, kernel_time);
+  }
+
   void precision(int p) { precision_ = p; }
 
  private:
-  unsigned int run(int kind, const real_t* rec, int ntri, int method, complex_t*& ff, int nqy, real_t* qx,
-                   real_t* qy, int nqz, complex_t* qz, RotMatrix_t& rot, real_t& kernel_time);
-  GpuContext gpu_;
+  unsigned int run(int kind, const real_t* rec, int ntri, int method, int prec, cucomplex_t*& ff, int nqy,
+                   real_t* qx, real_t* qy, int nqz, cucomplex_t* qz, RotMatrix_t& rot, real_t& kernel_time) {
+    ff = nullptr;
+    if (ntri < 1 || nqz < 1) return 0;
+    double m[9];
+    hgx_detail::rot_entries(rot, m);
+    ff = new (std::nothrow) cucomplex_t[nqz];
+    if (!ff) {
+      std::cerr << "Memory allocation failed for ff. Requested size: " << nqz << std::endl;
+      return 0;
+    }
+    double ms = 0.0;
+    unsigned n = core_.run(kind, reinterpret_cast<const double*>(rec), ntri, method, prec,
+                           reinterpret_cast<double*>(ff), nqy, qx, qy, nqz, reinterpret_cast<const double*>(qz), m,
+                           &ms);
+    if (n == 0) {
+      delete[] ff;
+      ff = nullptr;
+      return 0;
+    }
+    kernel_time = ms;
+    return n;
+  }
+  hgx_adapter::NumericFF core_;
   int precision_ = HGX_PREC_FP64;
-  int mesh_id_ = -1;
-  unsigned long long mesh_hash_ = 0;
 };
 
-// Replaces AnalyticFormFactorG (include/ff/gpu/ff_ana_gpu.cuh:28-96). init() takes the q-grid
-// arrays explicitly (the reference reads the QGrid singleton, src/ff/gpu/ff_ana_gpu.cu:57-98).
-// Weights are applied for every shape (the reference's CPU ignores them for cylinder and prisms:
-// pass unit weights to reproduce); tau/eta are the prism tilt angles.
+// include/ff/gpu/ff_ana_gpu.cuh:28-96. Weights are applied for every shape (the reference's CPU
+// ignores them for cylinders and prisms: pass unit weights to reproduce that). compat() (bits
+// HGX_SHAPE_COMPAT_*) is an extension.
 class AnalyticFormFactorG {
  public:
-  AnalyticFormFactorG();
-  ~AnalyticFormFactorG();
-  bool init(unsigned int nqy, const real_t* qx, const real_t* qy, unsigned int nqz, const complex_t* qz);
-  bool clear() { return true; }
-  bool destroy() { return true; }
+  AnalyticFormFactorG(unsigned int nqy, unsigned int nqz) { init(nqy, nqz); }
+  AnalyticFormFactorG() {}
+  ~AnalyticFormFactorG() {}
 
-  bool compute_box(const real_t tau, const real_t eta, const real_vec_t& x, const real_vec_t& distr_x,
-                   const real_vec_t& y, const real_vec_t& distr_y, const real_vec_t& z, const real_vec_t& distr_z,
-                   const RotMatrix_t& rot, const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_cube(const real_t tau, const real_t eta, const real_vec_t& x, const real_vec_t& distr_x,
-                    const RotMatrix_t& rot, const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_cylinder(const real_t tau, const real_t eta, const real_vec_t& r, const real_vec_t& distr_r,
-                        const real_vec_t& h, const real_vec_t& distr_h, const RotMatrix_t& rot,
-                        const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_sphere(const real_vec_t& r, const real_vec_t& distr_r, const real_vec_t& h, const real_vec_t& distr_h,
-                      const RotMatrix_t& rot, const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_prism3(const real_t tau, const real_t eta, const real_vec_t& l, const real_vec_t& distr_l,
-                      const real_vec_t& h, const real_vec_t& distr_h, const RotMatrix_t& rot,
-                      const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_prism6(const real_t tau, const real_t eta, const real_vec_t& l, const real_vec_t& distr_l,
-                      const real_vec_t& h, const real_vec_t& distr_h, const RotMatrix_t& rot,
-                      const real_vec_t& transvec, complex_vec_t& ff);
-  bool compute_pyramid(const real_t tau, const real_t eta, const real_vec_t& x, const real_vec_t& distr_x,
-                       const real_vec_t& y, const real_vec_t& distr_y, const real_vec_t& h, const real_vec_t& distr_h,
-                       const real_vec_t& b, const real_vec_t& distr_b, const RotMatrix_t& rot,
-                       const real_vec_t& transvec, complex_vec_t& ff);
-  // Extension: HGX_SHAPE_COMPAT_* bits (e.g. the reference's pyramid expression).
+  // src/ff/gpu/ff_ana_gpu.cu:57-98: the grid from the QGrid singleton (call site src/ff/ff_ana.cpp:43)
+  bool init(unsigned int nqy, unsigned int nqz) {
+    std::vector<double> qx, qy;
+    std::vector<std::complex<double>> qz;
+    hgx_detail::singleton_grid(nqy, nqz, qx, qy, qz);
+    return core_.init(nqy, qx.data(), qy.data(), nqz, qz.data());
+  }
+  bool clear() {
+    core_.clear();
+    return true;
+  }
+  bool destroy() { return clear(); }
+  void grid_size(unsigned int nqy, unsigned int nqz) { init(nqy, nqz); }
+
+  bool compute_box(const real_t tau, const real_t eta, const std::vector<real_t>& x, const std::vector<real_t>& dx,
+                   const std::vector<real_t>& y, const std::vector<real_t>& dy, const std::vector<real_t>& z,
+                   const std::vector<real_t>& dz, const RotMatrix_t& rot, const std::vector<real_t>& t,
+                   std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&x, &y, &z, nullptr};
+    const std::vector<real_t>* w[4] = {&dx, &dy, &dz, nullptr};
+    return run(HGX_SHAPE_BOX, 3, v, w, tau, eta, rot, t, ff);
+  }
+  bool compute_cube(const real_t tau, const real_t eta, const std::vector<real_t>& x, const std::vector<real_t>& dx,
+                    const RotMatrix_t& rot, const std::vector<real_t>& t, std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&x, nullptr, nullptr, nullptr};
+    const std::vector<real_t>* w[4] = {&dx, nullptr, nullptr, nullptr};
+    return run(HGX_SHAPE_CUBE, 1, v, w, tau, eta, rot, t, ff);
+  }
+  bool compute_cylinder(const real_t tau, const real_t eta, const std::vector<real_t>& r,
+                        const std::vector<real_t>& dr, const std::vector<real_t>& h, const std::vector<real_t>& dh,
+                        const RotMatrix_t& rot, const std::vector<real_t>& t, std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&r, &h, nullptr, nullptr};
+    const std::vector<real_t>* w[4] = {&dr, &dh, nullptr, nullptr};
+    return run(HGX_SHAPE_CYLINDER, 2, v, w, tau, eta, rot, t, ff);
+  }
+  // the height parameter is parsed but not used by the reference's sphere (src/ff/ff_ana_sphere.cpp:98-110)
+  bool compute_sphere(const std::vector<real_t>& r, const std::vector<real_t>& dr, const std::vector<real_t>& /*h*/,
+                      const std::vector<real_t>& /*dh*/, const RotMatrix_t& rot, const std::vector<real_t>& t,
+                      std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&r, nullptr, nullptr, nullptr};
+    const std::vector<real_t>* w[4] = {&dr, nullptr, nullptr, nullptr};
+    return run(HGX_SHAPE_SPHERE, 1, v, w, 0.0, 0.0, rot, t, ff);
+  }
+  bool compute_prism3(const real_t tau, const real_t eta, const std::vector<real_t>& l, const std::vector<real_t>& dl,
+                      const std::vector<real_t>& h, const std::vector<real_t>& dh, const RotMatrix_t& rot,
+                      const std::vector<real_t>& t, std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&l, &h, nullptr, nullptr};
+    const std::vector<real_t>* w[4] = {&dl, &dh, nullptr, nullptr};
+    return run(HGX_SHAPE_PRISM3, 2, v, w, tau, eta, rot, t, ff);
+  }
+  bool compute_prism6(const real_t tau, const real_t eta, const std::vector<real_t>& l, const std::vector<real_t>& dl,
+                      const std::vector<real_t>& h, const std::vector<real_t>& dh, const RotMatrix_t& rot,
+                      const std::vector<real_t>& t, std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&l, &h, nullptr, nullptr};
+    const std::vector<real_t>* w[4] = {&dl, &dh, nullptr, nullptr};
+    return run(HGX_SHAPE_PRISM6, 2, v, w, tau, eta, rot, t, ff);
+  }
+  bool compute_pyramid(const real_t tau, const real_t eta, const std::vector<real_t>& x,
+                       const std::vector<real_t>& dx, const std::vector<real_t>& y, const std::vector<real_t>& dy,
+                       const std::vector<real_t>& h, const std::vector<real_t>& dh, const std::vector<real_t>& b,
+                       const std::vector<real_t>& db, const RotMatrix_t& rot, const std::vector<real_t>& t,
+                       std::vector<complex_t>& ff) {
+    const std::vector<real_t>* v[4] = {&x, &y, &h, &b};
+    const std::vector<real_t>* w[4] = {&dx, &dy, &dh, &db};
+    return run(HGX_SHAPE_PYRAMID, 4, v, w, tau, eta, rot, t, ff);
+  }
+
   void compat(int flags) { compat_ = flags; }
 
  private:
-  bool run(int shape, int ndim, const real_vec_t* vals[4], const real_vec_t* wts[4], real_t tau, real_t eta,
-           const RotMatrix_t& rot, const real_vec_t& transvec, complex_vec_t& ff);
-  GpuContext gpu_;
-  unsigned nqz_ = 0;
+  bool run(int shape, int ndim, const std::vector<real_t>* vals[4], const std::vector<real_t>* wts[4], real_t tau,
+           real_t eta, const RotMatrix_t& rot, const std::vector<real_t>& transvec, std::vector<complex_t>& ff) {
+    std::vector<double> v[4], w[4];
+    const std::vector<double>* pv[4] = {nullptr, nullptr, nullptr, nullptr};
+    const std::vector<double>* pw[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int d = 0; d < ndim; ++d) {
+      v[d] = hgx_detail::to_vec(*vals[d]);
+      w[d] = hgx_detail::to_vec(*wts[d]);
+      pv[d] = &v[d];
+      pw[d] = &w[d];
+    }
+    double m[9];
+    hgx_detail::rot_entries(rot, m);
+    std::vector<std::complex<double>> out;
+    if (!core_.run(shape, ndim, pv, pw, tau, eta, compat_, m, hgx_detail::to_vec(transvec), out)) return false;
+    ff.assign(out.size(), complex_t(0, 0));
+    for (size_t i = 0; i < out.size(); ++i) ff[i] = complex_t(out[i].real(), out[i].imag());
+    return true;
+  }
+  hgx_adapter::AnalyticFF core_;
   int compat_ = 0;
 };
 
-// Replaces StructureFactorG (include/sf/gpu/sf_gpu.cuh:30-70). The reference's compute() takes a
-// Lattice*; here the three lattice vectors a, b, c (Lattice::construct_vectors,
-// src/model/structure.cpp:86-182) are passed, then scaled by `scaling` (src/sf/sf.cpp:169-171).
+// include/sf/gpu/sf_gpu.cuh:30-70. compute() takes the Lattice the reference passes and scales its
+// vectors by `scaling` (src/sf/gpu/sf_gpu.cu:205-207). The lattice sum is the reference's CPU formula
+// (src/sf/sf.cpp:165-245, no basis term 1 + exp(i q.t); every built-in lattice but fco has t = 0).
+// dw_sigma() (a Debye-Waller factor) and flags() (HGX_SF_*) are extensions.
 class StructureFactorG {
  public:
-  StructureFactorG();
-  ~StructureFactorG();
-  bool init(unsigned int nqy, const real_t* qx, const real_t* qy, unsigned int nqz, const complex_t* qz);
-  bool compute(std::string expt, vector3_t center, const vector3_t& a, const vector3_t& b, const vector3_t& c,
-               vector3_t repet, vector3_t scaling, RotMatrix_t rot, real_t dw_sigma = 0.0, int flags = 0);
-  // Copies the last result into sf (caller-allocated, nqz elements), as sf.cpp:297-305 expects.
-  void get_sf(complex_t*& sf);
+  StructureFactorG(unsigned int nx, unsigned int ny, unsigned int nz) { init(nx, ny, nz); }
+  StructureFactorG() {}
+  ~StructureFactorG() { destroy(); }
+
+  // src/sf/gpu/sf_gpu.cu:45-116: the grid from the QGrid singleton (call sites src/sf/sf.cpp:58, 300)
+  bool init(unsigned int /*nx*/, unsigned int ny, unsigned int nz) {
+    if (core_.ready()) {
+      std::cerr << "warning: gpu sf already initialized" << std::endl;
+      return false;
+    }
+    std::vector<double> qx, qy;
+    std::vector<std::complex<double>> qz;
+    hgx_detail::singleton_grid(ny, nz, qx, qy, qz);
+    nqz_ = nz;
+    return core_.init(ny, qx.data(), qy.data(), nz, qz.data());
+  }
+  bool clear() { return destroy(); }
+  bool destroy() {
+    core_.destroy();
+    nqz_ = 0;
+    return true;
+  }
+  void grid_size(unsigned int nx, unsigned int ny, unsigned int nz) {
+    destroy();
+    init(nx, ny, nz);
+  }
+
+  // src/sf/gpu/sf_gpu.cu:182-241 (call site src/sf/sf.cpp:301-306)
+  bool compute(std::string expt, vector3_t center, Lattice* lattice, vector3_t repet, vector3_t scaling,
+               RotMatrix_t rot) {
+    if (!lattice) return false;
+    vector3_t a = lattice->a(), b = lattice->b(), c = lattice->c();
+    double la[3], lb[3], lc[3], rep[3], ce[3], m[9];
+    for (int k = 0; k < 3; ++k) {
+      la[k] = a[k] * scaling[0];
+      lb[k] = b[k] * scaling[1];
+      lc[k] = c[k] * scaling[2];
+      rep[k] = repet[k];
+      ce[k] = center[k];
+    }
+    hgx_detail::rot_entries(rot, m);
+    return core_.compute(expt, ce, la, lb, lc, rep, m, dw_sigma_, flags_);
+  }
+  // src/sf/gpu/sf_gpu.cu:118-140: copies the last result into the caller's nqz-element buffer
+  void get_sf(complex_t*& sf) {
+    if (!sf || nqz_ == 0) return;
+    std::vector<std::complex<double>> out(nqz_);
+    if (!core_.get_sf(out.data())) return;
+    for (unsigned i = 0; i < nqz_; ++i) sf[i] = complex_t(out[i].real(), out[i].imag());
+  }
+
+  void dw_sigma(real_t s) { dw_sigma_ = s; }
+  void flags(int f) { flags_ = f; }
 
  private:
-  GpuContext gpu_;
+  hgx_adapter::StructureF core_;
   unsigned nqz_ = 0;
+  double dw_sigma_ = 0.0;
+  int flags_ = 0;
 };
-
-// Replaces MultiLayer::propagation_coeffs (src/model/multilayer.cpp:110-155) on explicit rows.
-class MultiLayerG {
- public:
-  struct Layer {
-    real_t delta, beta, thickness;
-  };
-  // layers below the vacuum, top to bottom, substrate last (MultiLayer::init order)
-  bool init(const std::vector<Layer>& layers);
-  // fc (4 * nrow * ncol) for the rows' alpha_f; order = the particle's layer (0 = vacuum).
-  bool propagation_coeffs(complex_vec_t& coeff, real_t k0, real_t alpha_i, int order, const real_vec_t& alpha_f,
-                          unsigned ncol);
-
- private:
-  GpuContext gpu_;
-  std::vector<Layer> layers_;
-};
-
-// The DWBA sum of run_gisaxs (src/sim/hipgisaxs_main.cpp:1084-1100) on host arrays:
-// I[p] += |weight * sum_b fc_b sf_b ff_b|^2 (nbranch = nq / npix).
-bool dwba_intensity(GpuContext& gpu, unsigned npix, unsigned nq, const complex_t* fc, const complex_t* sf,
-                    const complex_t* ff, real_t weight, real_t* I);
 
 }  // namespace hig

@@ -11,6 +11,35 @@
 
 namespace hig {
 
+// ----- Rot3 (include/numerics/matrix.hpp:35-110) ---------------------------------------------------
+
+Rot3::Rot3() : m{1, 0, 0, 0, 1, 0, 0, 0, 1} {}
+
+Rot3::Rot3(int axis, double a) {
+  const double c = std::cos(a), s = std::sin(a);
+  const double x[9] = {1, 0, 0, 0, c, -s, 0, s, c};
+  const double y[9] = {c, 0, -s, 0, 1, 0, s, 0, c};
+  const double z[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+  const double* r = axis == 0 ? x : axis == 1 ? y : z;
+  std::memcpy(m, r, sizeof(m));
+}
+
+Rot3 Rot3::operator*(const Rot3& rhs) const {
+  Rot3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += m[3 * i + k] * rhs.m[3 * k + j];
+      r.m[3 * i + j] = s;
+    }
+  return r;
+}
+
+std::array<double, 3> Rot3::operator*(const std::array<double, 3>& v) const {
+  return {m[0] * v[0] + m[1] * v[1] + m[2] * v[2], m[3] * v[0] + m[4] * v[1] + m[5] * v[2],
+          m[6] * v[0] + m[7] * v[1] + m[8] * v[2]};
+}
+
 namespace {
 
 const double PI_ = 3.141592653589793;  // include/common/constants.hpp:29
@@ -117,7 +146,7 @@ HipGISAXS::~HipGISAXS() { release_device_objects(); }
 
 bool HipGISAXS::device() {
   if (gpu_ && gpu_->ok()) return true;
-  gpu_.reset(new GpuContext(device_));
+  gpu_.reset(new hgx_adapter::GpuContext(device_));
   if (!gpu_->ok()) {
     gpu_.reset();
     return fail("could not create the device context on device " + std::to_string(device_));
@@ -355,17 +384,17 @@ bool HipGISAXS::make_grains(const HigStructure& s, GlibcRandom& rng, double phi,
     return fail("structure '" + s.key + "': orientation statistic '" + st + "' is not supported");
   }
   vec3 tv = s.transvec;  // minus the reference's never-initialised single_layer_thickness_, taken as 0
-  RotMatrix_t rphi(2, phi);
+  Rot3 rphi(2, phi);
   out.resize(ng);
   const int nyz = nd[1] * nd[2];
   for (int g = 0; g < ng; ++g) {
-    RotMatrix_t r1(axis[0], nn[g]), r2(axis[1], nn[ng + g]), r3(axis[2], nn[2 * ng + g]);
+    Rot3 r1(axis[0], nn[g]), r2(axis[1], nn[ng + g]), r3(axis[2], nn[2 * ng + g]);
     HigGrain& G = out[g];
     G.rot = rphi * r3 * r2 * r1;
     // d1[nyz x + yz] = dx[x], d2[nyz x + nz y + z] = dy[y], d3[nz xy + z] = dz[z]
     int x = g / nyz, y = (g % nyz) / nd[2], z = g % nd[2];
-    vector3_t dd(dx[x], dy[y], dz[z]);
-    vector3_t c = G.rot * dd;
+    std::array<double, 3> dd{dx[x], dy[y], dz[z]};
+    std::array<double, 3> c = G.rot * dd;
     G.center[0] = c[0] + tv[0];
     G.center[1] = c[1] + tv[1];
     G.center[2] = c[2] + tv[2];
@@ -456,7 +485,7 @@ int HipGISAXS::mesh_of(const HigShape& sh) {
 // FormFactor::compute_form_factor (src/ff/ff.cpp:32-61): analytic shapes with the location phase,
 // mesh shapes by the approximate (centroid) method of NumericFormFactor::compute
 // (src/ff/ff_num.cpp:187-284) under the run's precision.
-bool HipGISAXS::form_factor(const HigShape& sh, const RotMatrix_t& rot, const vec3& loc, double* d_out) {
+bool HipGISAXS::form_factor(const HigShape& sh, const Rot3& rot, const vec3& loc, double* d_out) {
   ++counts_[2];
   if (sh.shape == -1) {
     int id = mesh_of(sh);
@@ -595,7 +624,7 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
       std::memset(&E, 0, sizeof(E));
       if (!shape_params(sh, E.params, &stores[8 * (size_t)e])) return false;
       E.shape = sh.shape;
-      RotMatrix_t re = RotMatrix_t(0, sh.xrot) * RotMatrix_t(1, sh.yrot) * RotMatrix_t(2, sh.zrot);
+      Rot3 re = Rot3(0, sh.xrot) * Rot3(1, sh.yrot) * Rot3(2, sh.zrot);
       std::memcpy(E.rot, re.data(), sizeof(E.rot));
       for (const auto& l : it.second) locs[e].insert(locs[e].end(), l.begin(), l.end());
       E.nloc = (int)it.second.size();
@@ -656,7 +685,7 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
         const HigShape& sh = in_.shapes.at(e.first);
         const double dn2r = dnl[0] - 2 * sh.delta, dn2i = dnl[1] - 2 * sh.beta;
         // shape rotation rot * Rx * Ry * Rz (src/sim/hipgisaxs_main.cpp:986)
-        RotMatrix_t srot = G.rot * RotMatrix_t(0, sh.xrot) * RotMatrix_t(1, sh.yrot) * RotMatrix_t(2, sh.zrot);
+        Rot3 srot = G.rot * Rot3(0, sh.xrot) * Rot3(1, sh.yrot) * Rot3(2, sh.zrot);
         for (const auto& loc : e.second) {
           if (!form_factor(sh, srot, loc, d_eff)) return false;
           if (!hgx(hgx_caxpy(gpu_->get(), nq, dn2r, dn2i, d_eff, first ? 0 : 1, d_ff), "hgx_caxpy")) return false;

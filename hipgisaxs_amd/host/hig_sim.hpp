@@ -15,10 +15,22 @@
 
 #include "hgx.h"
 #include "hgx_hig.h"
-#include "hig_gpu.hpp"
+#include "hgx_adapter.hpp"
 #include "hig_input.hpp"
 
 namespace hig {
+
+// The driver's rotations (row-major), with the reference's constructions and products
+// (include/numerics/matrix.hpp:35-110): about axis 0, 1 or 2 by an angle, matrix products summed in
+// the reference's order, rotation of a vector.
+struct Rot3 {
+  double m[9];
+  Rot3();
+  Rot3(int axis, double angle);
+  Rot3 operator*(const Rot3& rhs) const;
+  std::array<double, 3> operator*(const std::array<double, 3>& v) const;
+  const double* data() const { return m; }
+};
 
 // The random() generator of glibc (additive feedback, degree 31: r[i] = r[i-3] + r[i-31] mod 2^32
 // after an LCG seeding of r[0..30] and 310 discarded outputs; rand() returns r >> 1). A process
@@ -37,7 +49,7 @@ class GlibcRandom {
 };
 
 struct HigGrain {
-  RotMatrix_t rot;      // rot_phi * r3 * r2 * r1
+  Rot3 rot;             // rot_phi * r3 * r2 * r1
   double center[3];     // rot * dd + transvec
   double weight;        // gauss_weight * scaling weight
 };
@@ -119,13 +131,13 @@ class HipGISAXS {
   bool run_device(double alpha_i_deg, double phi_deg, const double** d_img);
   bool make_grains(const HigStructure& s, GlibcRandom& rng, double phi_rad, std::vector<HigGrain>& out);
   bool structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains, double* d_I);
-  bool form_factor(const HigShape& shape, const RotMatrix_t& rot, const vec3& loc, double* d_out);
+  bool form_factor(const HigShape& shape, const Rot3& rot, const vec3& loc, double* d_out);
   bool shape_params(const HigShape& shape, hgx_shape_params& p, std::vector<double> store[8]);
   int mesh_of(const HigShape& shape);
   double* buffer(int slot, size_t bytes);
 
   int device_ = 0;
-  std::unique_ptr<GpuContext> gpu_;
+  std::unique_ptr<hgx_adapter::GpuContext> gpu_;
   HigInput in_;
   bool loaded_ = false, inited_ = false;
   int precision_ = HGX_PREC_FP64;

@@ -1,10 +1,15 @@
-// Reference-style callers of the drop-in C++ classes (hipgisaxs_amd/host/hig_gpu.hpp), exported
-// with C linkage so the GPU tests can drive them from pytest. Each function calls the adapter the way
-// the reference's own code calls its GPU class:
-//   numeric FF   -> src/ff/ff_num.cpp:261-280 (compute_approx_triangle, copy p_ff out, delete[] it)
-//   analytic FF  -> src/ff/ff_ana_sphere.cpp:88-91 / ff_ana_pyramid.cpp:121-127
-//   SF           -> src/sf/sf.cpp:288-309 (init, compute, get_sf into a new[] buffer)
-//   Fresnel      -> src/sim/hipgisaxs_main.cpp:781-786 (propagation_coeffs)
+// Reference-style callers of the drop-in GPU classes (hipgisaxs_amd/host/hig_gpu.hpp), compiled the
+// way a reference translation unit compiles them: against the reference's type headers (here the
+// stand-ins of tests/cpp/refapi, which carry the reference's names), with the reference's own call
+// lines, and exported with C linkage so the GPU tests can drive them from pytest:
+//   numeric FF   -> src/ff/ff_num.cpp:261-280 (approx) and :150-160 (exact): compute_*_triangle into
+//                   a callee-allocated cucomplex_t array, read p_ff[i].x / .y, delete[] it
+//   analytic FF  -> src/ff/ff_ana.cpp:43 (gff_.init(nqy_, nqz_)), src/ff/ff_ana_sphere.cpp and
+//                   ff_ana_pyramid.cpp (gff_.compute_<shape>)
+//   SF           -> src/sf/sf.cpp:58, 296-307 (gsf_.init(ny_, ny_, nz_), compute, get_sf, destroy)
+//   Fresnel      -> src/sim/hipgisaxs_main.cpp:781-786 (propagation_coeffs; the adapter library's
+//                   MultiLayer)
+// The q-grid enters through the QGrid singleton, as QGrid::create and create_qz_extended fill it.
 #include <cstring>
 #include <vector>
 
@@ -12,97 +17,160 @@
 
 using namespace hig;
 
+static void fill_qgrid(int nqy, const double* qx, const double* qy, int nqz, const double* qz) {
+  QGrid& g = QGrid::instance();
+  g.qx_.assign(qx, qx + nqy);
+  g.qy_.assign(qy, qy + nqy);
+  g.qz_extended_.resize(nqz);
+  for (int i = 0; i < nqz; ++i) g.qz_extended_[i] = complex_t(qz[2 * i], qz[2 * i + 1]);
+}
+
 static RotMatrix_t rot_of(const double* r) {
-  RotMatrix_t R;
-  if (r) std::memcpy(R.data_, r, 9 * sizeof(double));
-  return R;
+  if (!r) return RotMatrix_t();
+  vector3_t a(r[0], r[1], r[2]), b(r[3], r[4], r[5]), c(r[6], r[7], r[8]);
+  return RotMatrix_t(a, b, c);
 }
 
 extern "C" {
 
-int shim_numeric_approx(int nqy, double* qx, double* qy, int nqz, double* qz, int ntri, const double* rec8,
+int shim_numeric_approx(int nqy, double* qx_in, double* qy_in, int nqz, double* qz_in, int ntri, const double* rec8,
                         const double* rot, int precision, double* out) {
-  NumericFormFactorG gff(2, 4, 4);  // the block shape FormFactor eff(2, 4, 4) passes (hipgisaxs_main.cpp:995)
-  gff.precision(precision);
+  fill_qgrid(nqy, qx_in, qy_in, nqz, qz_in);
+  NumericFormFactorG gff_(2, 4, 4);  // the block shape FormFactor eff(2, 4, 4) passes (hipgisaxs_main.cpp:995)
+  gff_.precision(precision);
   std::vector<real_t> shape_def(rec8, rec8 + 8 * (size_t)ntri);
-  complex_t* p_ff = nullptr;
-  real_t kernel_time = 0.0;
-  RotMatrix_t R = rot_of(rot);
-  unsigned int n = gff.compute_approx_triangle(shape_def, p_ff, nqy, qx, qy, nqz, reinterpret_cast<complex_t*>(qz), R,
-                                               kernel_time);
-  if (n == 0 || !p_ff) return -1;
+  RotMatrix_t rot_ = rot_of(rot);
+  // src/ff/ff_num.cpp:224-280 with FF_NUM_GPU
+  real_t* qx = new real_t[nqy];
+  real_t* qy = new real_t[nqy];
+  for (int i = 0; i < nqy; i++) qx[i] = QGrid::instance().qx(i);
+  for (int i = 0; i < nqy; i++) qy[i] = QGrid::instance().qy(i);
+  cucomplex_t* qz = new cucomplex_t[nqz];
   for (int i = 0; i < nqz; i++) {
-    out[2 * i] = p_ff[i].real();
-    out[2 * i + 1] = p_ff[i].imag();
+    qz[i].x = QGrid::instance().qz_extended(i).real();
+    qz[i].y = QGrid::instance().qz_extended(i).imag();
   }
-  delete[] p_ff;
-  return (int)n;
+  std::vector<complex_t> ff;
+  cucomplex_t* p_ff = NULL;
+  real_t kernel_time = 0.;
+  unsigned int ret_numtriangles = 0;
+  ret_numtriangles = gff_.compute_approx_triangle(shape_def, p_ff, nqy, qx, qy, nqz, qz, rot_, kernel_time);
+  if (ret_numtriangles == 0 || p_ff == NULL) return -1;
+  for (int i = 0; i < nqz; i++) ff.push_back(complex_t(p_ff[i].x, p_ff[i].y));
+  if (p_ff != NULL) delete[] p_ff;
+  delete[] qz;
+  delete[] qy;
+  delete[] qx;
+  std::memcpy(out, ff.data(), sizeof(complex_t) * nqz);
+  return (int)ret_numtriangles;
 }
 
-int shim_numeric_exact(int nqy, double* qx, double* qy, int nqz, double* qz, int ntri, const double* tri9,
-                       const double* rot, double* out) {
-  NumericFormFactorG gff;
-  std::vector<triangle_t> tris(ntri);
-  std::memcpy(tris.data(), tri9, sizeof(triangle_t) * ntri);
-  complex_t* p_ff = nullptr;
-  real_t kernel_time = 0.0;
-  RotMatrix_t R = rot_of(rot);
-  unsigned int n = gff.compute_exact_triangle(tris.data(), ntri, p_ff, nqy, qx, qy, nqz, reinterpret_cast<complex_t*>(qz),
-                                              R, kernel_time);
-  if (n == 0 || !p_ff) return -1;
-  std::memcpy(out, p_ff, sizeof(complex_t) * nqz);
+int shim_numeric_exact(int nqy, double* qx_in, double* qy_in, int nqz, double* qz_in, int num_triangles,
+                       const double* tri9, const double* rot, double* out) {
+  fill_qgrid(nqy, qx_in, qy_in, nqz, qz_in);
+  NumericFormFactorG gff_;
+  triangle_t* triangles = new triangle_t[num_triangles];
+  std::memcpy(triangles, tri9, sizeof(triangle_t) * num_triangles);
+  RotMatrix_t rot_ = rot_of(rot);
+  real_t* qx = new real_t[nqy];
+  real_t* qy = new real_t[nqy];
+  for (int i = 0; i < nqy; i++) qx[i] = QGrid::instance().qx(i);
+  for (int i = 0; i < nqy; i++) qy[i] = QGrid::instance().qy(i);
+  cucomplex_t* qz = new cucomplex_t[nqz];
+  for (int i = 0; i < nqz; i++) {
+    qz[i].x = QGrid::instance().qz_extended(i).real();
+    qz[i].y = QGrid::instance().qz_extended(i).imag();
+  }
+  // src/ff/ff_num.cpp:150-160
+  std::vector<complex_t> ff;
+  real_t compute_time = 0.;
+  cucomplex_t* p_ff = NULL;
+  if (num_triangles != (int)gff_.compute_exact_triangle(triangles, num_triangles, p_ff, nqy, qx, qy, nqz, qz, rot_,
+                                                          compute_time))
+    return -1;
+  for (int i = 0; i < nqz; i++) ff.push_back(complex_t(p_ff[i].x, p_ff[i].y));
   delete[] p_ff;
-  return (int)n;
+  delete[] qx;
+  delete[] qy;
+  delete[] qz;
+  delete[] triangles;
+  std::memcpy(out, ff.data(), sizeof(complex_t) * nqz);
+  return num_triangles;
 }
 
 // shape: 0 sphere (r), 1 pyramid (x, y, h, b); dims packed as counts + values + weights
-int shim_analytic(int shape, int nqy, double* qx, double* qy, int nqz, double* qz, const int* counts,
+int shim_analytic(int shape, int nqy_in, double* qx_in, double* qy_in, int nqz_in, double* qz_in, const int* counts,
                   const double* vals, const double* wts, const double* rot, const double* transvec, int compat,
                   double* out) {
-  AnalyticFormFactorG gff;
-  if (!gff.init(nqy, qx, qy, nqz, reinterpret_cast<complex_t*>(qz))) return -1;
-  gff.compat(compat);
+  fill_qgrid(nqy_in, qx_in, qy_in, nqz_in, qz_in);
+  AnalyticFormFactorG gff_;
+  // src/ff/ff_ana.cpp:35-43
+  unsigned int nqy_ = QGrid::instance().nqy();
+  unsigned int nqz_ = QGrid::instance().nqz_extended();
+  std::vector<complex_t> ff;
+  ff.clear();
+  if (!gff_.init(nqy_, nqz_)) return -1;
+  gff_.compat(compat);
   int nd = shape == 0 ? 1 : 4;
   std::vector<real_vec_t> v(4), w(4);
   for (int d = 0, off = 0; d < nd; off += counts[d], ++d) {
     v[d].assign(vals + off, vals + off + counts[d]);
     w[d].assign(wts + off, wts + off + counts[d]);
   }
-  real_vec_t tv(transvec, transvec + 3);
-  complex_vec_t ff;
-  RotMatrix_t R = rot_of(rot);
-  bool ok = shape == 0 ? gff.compute_sphere(v[0], w[0], real_vec_t(), real_vec_t(), R, tv, ff)
-                       : gff.compute_pyramid(0.0, 0.0, v[0], w[0], v[1], w[1], v[2], w[2], v[3], w[3], R, tv, ff);
+  std::vector<real_t> transvec_v(transvec, transvec + 3);
+  RotMatrix_t rot_ = rot_of(rot);
+  real_t tau = 0.0, eta = 0.0;
+  std::vector<real_t> r = v[0], distr_r = w[0], h, distr_h;
+  bool ok = shape == 0 ? gff_.compute_sphere(r, distr_r, h, distr_h, rot_, transvec_v, ff)
+                       : gff_.compute_pyramid(tau, eta, v[0], w[0], v[1], w[1], v[2], w[2], v[3], w[3], rot_,
+                                              transvec_v, ff);
+  gff_.clear();
   if (!ok) return -1;
   std::memcpy(out, ff.data(), sizeof(complex_t) * ff.size());
   return 0;
 }
 
-int shim_sf(int nqy, double* qx, double* qy, int nqz, double* qz, const double* abc, const double* repet,
-            const double* scaling, const double* center, const double* rot, double dw_sigma, double* out) {
-  StructureFactorG gsf;
-  if (!gsf.init(nqy, qx, qy, nqz, reinterpret_cast<complex_t*>(qz))) return -1;
-  vector3_t a(abc[0], abc[1], abc[2]), b(abc[3], abc[4], abc[5]), c(abc[6], abc[7], abc[8]);
-  if (!gsf.compute("gisaxs", vector3_t(center[0], center[1], center[2]), a, b, c,
-                   vector3_t(repet[0], repet[1], repet[2]), vector3_t(scaling[0], scaling[1], scaling[2]),
-                   rot_of(rot), dw_sigma))
+int shim_sf(int nqy, double* qx, double* qy, int nqz, double* qz, const double* abc, const double* rep,
+            const double* scl, const double* cen, const double* rot, double dw_sigma, double* out) {
+  fill_qgrid(nqy, qx, qy, nqz, qz);
+  Lattice lat;
+  lat.a(vector3_t(abc[0], abc[1], abc[2]));
+  lat.b(vector3_t(abc[3], abc[4], abc[5]));
+  lat.c(vector3_t(abc[6], abc[7], abc[8]));
+  Lattice* lattice = &lat;
+  std::string expt = "gisaxs";
+  vector3_t center(cen[0], cen[1], cen[2]), repet(rep[0], rep[1], rep[2]), scaling(scl[0], scl[1], scl[2]);
+  RotMatrix_t rot_ = rot_of(rot);
+  // src/sf/sf.cpp:288-307 (StructureFactor::compute_structure_factor_gpu)
+  StructureFactorG gsf_;
+  gsf_.dw_sigma(dw_sigma);
+  unsigned int ny_ = QGrid::instance().nqy();
+  unsigned int nz_ = QGrid::instance().nqz_extended();
+  complex_t* sf_ = new (std::nothrow) complex_t[nz_];
+  if (sf_ == NULL) return -1;
+  gsf_.init(ny_, ny_, nz_);
+  bool ret = gsf_.compute(expt, center, lattice, repet, scaling, rot_);
+  gsf_.get_sf(sf_);
+  gsf_.destroy();
+  if (!ret) {
+    delete[] sf_;
     return -1;
-  complex_t* sf = new complex_t[nqz];
-  gsf.get_sf(sf);
-  std::memcpy(out, sf, sizeof(complex_t) * nqz);
-  delete[] sf;
+  }
+  std::memcpy(out, sf_, sizeof(complex_t) * nz_);
+  delete[] sf_;
   return 0;
 }
 
 int shim_fresnel(int nl, const double* delta, const double* beta, const double* thickness, int order, double k0,
                  double alpha_i, int nrow, const double* alpha_f, int ncol, double* out) {
-  MultiLayerG ml;
-  std::vector<MultiLayerG::Layer> layers;
+  hgx_adapter::MultiLayer ml;
+  std::vector<hgx_adapter::MultiLayer::Layer> layers;
   for (int i = 0; i < nl; ++i) layers.push_back({delta[i], beta[i], thickness[i]});
   if (!ml.init(layers)) return -1;
-  complex_vec_t fc;
-  if (!ml.propagation_coeffs(fc, k0, alpha_i, order, real_vec_t(alpha_f, alpha_f + nrow), (unsigned)ncol)) return -1;
-  std::memcpy(out, fc.data(), sizeof(complex_t) * fc.size());
+  std::vector<std::complex<double>> fc;
+  if (!ml.propagation_coeffs(fc, k0, alpha_i, order, std::vector<double>(alpha_f, alpha_f + nrow), (unsigned)ncol))
+    return -1;
+  std::memcpy(out, fc.data(), sizeof(std::complex<double>) * fc.size());
   return 0;
 }
 

@@ -28,15 +28,36 @@ def _c2r(z):
 
 
 def test_adapter_library_exports_reference_classes():
+    """The drop-in classes are header-only and take every reference type from the reference's own
+    headers: the adapter library exports plain-type code only (nothing in namespace hig, so it links
+    into a reference build without clashing with its symbols), the header defines none of the
+    reference's types, and the shim (compiled against stand-ins of those headers) carries the classes
+    with the reference's method names."""
+    import re
     import subprocess
 
-    syms = subprocess.run(["nm", "-DC", str(ROOT / "hipgisaxs_amd" / "libhgx_hig.so")], capture_output=True,
-                          text=True).stdout
-    for s in ("hig::NumericFormFactorG::compute_approx_triangle", "hig::NumericFormFactorG::compute_exact_triangle",
-              "hig::AnalyticFormFactorG::compute_sphere", "hig::AnalyticFormFactorG::compute_pyramid",
-              "hig::StructureFactorG::compute", "hig::StructureFactorG::get_sf",
-              "hig::MultiLayerG::propagation_coeffs", "hig::dwba_intensity"):
+    adapter = ROOT / "hipgisaxs_amd" / "libhgx_adapter.so"
+    syms = subprocess.run(["nm", "-DC", "--defined-only", str(adapter)], capture_output=True, text=True).stdout
+    for s in ("hgx_adapter::NumericFF::run", "hgx_adapter::AnalyticFF::run", "hgx_adapter::StructureF::compute",
+              "hgx_adapter::MultiLayer::propagation_coeffs", "hgx_adapter::dwba_intensity"):
         assert s in syms, s
+    assert not re.search(r"\bhig::", syms)
+    header = (ROOT / "hipgisaxs_amd" / "host" / "hig_gpu.hpp").read_text()
+    code = "\n".join(l.split("//")[0] for l in header.splitlines())
+    for t in ("real_t", "complex_t", "cucomplex_t", "triangle_t", "vector3_t", "RotMatrix_t", "Lattice", "QGrid"):
+        assert not re.search(rf"(typedef[^;]*\b{t}\s*;|class\s+{t}\b|struct\s+{t}\b)", code), t
+    for inc in ("common/typedefs.hpp", "common/globals.hpp", "numerics/matrix.hpp", "model/structure.hpp",
+                "model/qgrid.hpp"):
+        assert f"#include <{inc}>" in header, inc
+    # the shim compiled (the build would have failed otherwise) with the reference's own call lines
+    shim = (ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp").read_text()
+    for line in ("gsf_.init(ny_, ny_, nz_);",                                       # src/sf/sf.cpp:300
+                 "bool ret = gsf_.compute(expt, center, lattice, repet, scaling, rot_);",  # :301
+                 "gsf_.get_sf(sf_);", "gsf_.destroy();",                              # :305-306
+                 "gff_.init(nqy_, nqz_)",                                           # src/ff/ff_ana.cpp:43
+                 "gff_.compute_approx_triangle(shape_def, p_ff, nqy, qx, qy, nqz, qz, rot_, kernel_time)",  # ff_num.cpp:269
+                 "gff_.compute_exact_triangle(triangles, num_triangles, p_ff, nqy, qx, qy, nqz, qz, rot_,"):
+        assert line in shim, line
     assert SHIM.exists()
 
 
