@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "analytic_shapes.hpp"
+#include "analytic32.hpp"
 #include "fp32_math.hpp"
 #include "lattice_sf.hpp"
 
@@ -148,92 +149,6 @@ __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const doub
 
 // ----- FP32C -------------------------------------------------------------------------------------
 
-// x (radians, FP64) reduced to turns in (-1/2, 1/2]
-__device__ inline float turns_of(double x) {
-  double u = x * (1.0 / (2.0 * kPiD));
-  return (float)(u - rint(u));
-}
-
-// sin(u)/u for u = 0.5 * m * L, the phase reduced in FP64
-__device__ inline float sinc32(double m, double L) {
-  double u = 0.5 * m * L;
-  float uf = (float)u;
-  if (fabsf(uf) < 0.1f) {
-    float u2 = uf * uf;
-    return 1.0f - u2 * (1.0f / 6.0f) * (1.0f - u2 * (1.0f / 20.0f));
-  }
-  return sin_turn(turns_of(u)) / uf;
-}
-
-template <int SHAPE>
-__device__ cf ff32(const double m[3], const Nodes& P);
-
-// sphere (src/ff/ff_ana_sphere.cpp:33-45): sum_k w_k 4 pi R^3 (sin x - x cos x)/x^3 e^{i m_z R}
-template <>
-__device__ cf ff32<HGX_SHAPE_SPHERE>(const double m[3], const Nodes& P) {
-  const double q64 = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-  if (q64 == 0.0) return cf{0.0f, 0.0f};
-  const float qv = (float)q64;
-  cf acc = cf{0.0f, 0.0f};
-  for (int i = 0; i < P.n[0]; ++i) {
-    const double R = P.val[0][i];
-    const float x = qv * (float)R;
-    float t;
-    if (x < 0.6f) {
-      float x2 = x * x;
-      t = x2 * x * ((1.0f / 3.0f) - x2 * ((1.0f / 30.0f) - x2 * ((1.0f / 840.0f) - x2 * (1.0f / 45360.0f))));
-    } else {
-      const float tr = turns_of(q64 * R);
-      t = sin_turn(tr) - x * cos_turn(tr);
-    }
-    acc = acc + ((float)(P.wt[0][i] * 4.0 * kPiD) * t) * expi_turn(turns_of(m[2] * R));
-  }
-  return (1.0f / (qv * qv * qv)) * acc;
-}
-
-// cylinder (src/ff/ff_ana_cylinder.cpp:34-42), node double sum factorised
-template <>
-__device__ cf ff32<HGX_SHAPE_CYLINDER>(const double m[3], const Nodes& P) {
-  const float qp = (float)sqrt(m[0] * m[0] + m[1] * m[1]);
-  float br = 0.0f;
-  for (int i = 0; i < P.n[0]; ++i) {
-    const float R = (float)P.val[0][i];
-    if (qp * R * qp * R > 1.0e-13f) br += (float)(P.wt[0][i] * 2.0 * kPiD) * R * R * j1_over_x(qp * R);
-  }
-  cf hz = cf{0.0f, 0.0f};
-  for (int k = 0; k < P.n[1]; ++k) {
-    const double H = P.val[1][k];
-    hz = hz + ((float)(P.wt[1][k] * H) * sinc32(m[2], H)) * expi_turn(turns_of(0.5 * H * m[2]));
-  }
-  return br * hz;
-}
-
-// box (src/ff/ff_ana_box.cpp:35-41, :111-121), node triple sum factorised per axis
-template <>
-__device__ cf ff32<HGX_SHAPE_BOX>(const double m[3], const Nodes& P) {
-  float sx = 0.0f, sy = 0.0f;
-  for (int i = 0; i < P.n[0]; ++i) sx += (float)(P.wt[0][i] * P.val[0][i]) * sinc32(m[0], P.val[0][i]);
-  for (int i = 0; i < P.n[1]; ++i) sy += (float)(P.wt[1][i] * P.val[1][i]) * sinc32(m[1], P.val[1][i]);
-  cf hz = cf{0.0f, 0.0f};
-  for (int k = 0; k < P.n[2]; ++k) {
-    const double H = P.val[2][k];
-    hz = hz + ((float)(P.wt[2][k] * H) * sinc32(m[2], H)) * expi_turn(turns_of(0.5 * H * m[2]));
-  }
-  return (sx * sy) * hz;
-}
-
-// cube (src/ff/ff_ana_cube.cpp:35-42, :109-113)
-template <>
-__device__ cf ff32<HGX_SHAPE_CUBE>(const double m[3], const Nodes& P) {
-  cf acc = cf{0.0f, 0.0f};
-  for (int i = 0; i < P.n[0]; ++i) {
-    const double L = P.val[0][i];
-    const float s = sinc32(m[0], L) * sinc32(m[1], L) * sinc32(m[2], L);
-    acc = acc + ((float)(P.wt[0][i] * L * L * L) * s) * expi_turn(turns_of(0.5 * L * m[2]));
-  }
-  return acc;
-}
-
 // S(m) in FP32 from FP64 lattice phases (the FP32C lattice kernel's form)
 __device__ inline cf sf32e(const double m[3], const SFArgs& s) {
   const double xa = m[0] * s.la[0] + m[1] * s.la[1] + m[2] * s.la[2];
@@ -256,14 +171,13 @@ __device__ inline cd cell32(int o, double qx, double qy, double qz, const EnsEle
     double mq[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) mq[j] = Me[3 * j] * qx + Me[3 * j + 1] * qy + Me[3 * j + 2] * qz;
-    const cf F = ff32<SHAPE>(mq, E[e].P);
+    const cd F = SHAPE >= 0 ? ff_fp32c<(SHAPE >= 0 ? SHAPE : 0)>(mq, E[e].P) : ff_fp32c_any(E[e].shape, mq, E[e].P);
     cf ph = cf{0.0f, 0.0f};
     for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
       const double* t = loc + 3 * l;
       ph = ph + expi_turn(turns_of(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
     }
-    const cf Fp = F * ph;
-    ff = ff + cd{E[e].dn2r, E[e].dn2i} * cd{(double)Fp.r, (double)Fp.i};
+    ff = ff + cd{E[e].dn2r, E[e].dn2i} * (F * cd{(double)ph.r, (double)ph.i});
   }
   return ff;
 }
@@ -408,13 +322,8 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   int common = E[0].shape;
   for (int e = 1; e < ne; ++e)
     if (E[e].shape != common) common = -1;
-  if (precision == HGX_PREC_FP32C) {
-    if (!c->qz_real) return set_error(c, HGX_ERR_UNSUPPORTED, "ensemble_intensity: FP32C needs real q");
-    if (common != HGX_SHAPE_SPHERE && common != HGX_SHAPE_CYLINDER && common != HGX_SHAPE_BOX &&
-        common != HGX_SHAPE_CUBE)
-      return set_error(c, HGX_ERR_UNSUPPORTED,
-                       "ensemble_intensity: FP32C takes cells of one shape among sphere, cylinder, box, cube");
-  }
+  if (precision == HGX_PREC_FP32C && !c->qz_real)
+    return set_error(c, HGX_ERR_UNSUPPORTED, "ensemble_intensity: FP32C needs real q");
   // M[g][e] = R_g R_e with the summation order of the reference's RotMatrix_t product
   std::vector<double> Mm(9 * (size_t)ng * ne);
   std::vector<EnsGrain> G(ng);
@@ -482,7 +391,11 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
       case HGX_SHAPE_SPHERE: HGX_ENS32_LAUNCH(HGX_SHAPE_SPHERE); break;
       case HGX_SHAPE_CYLINDER: HGX_ENS32_LAUNCH(HGX_SHAPE_CYLINDER); break;
       case HGX_SHAPE_BOX: HGX_ENS32_LAUNCH(HGX_SHAPE_BOX); break;
-      default: HGX_ENS32_LAUNCH(HGX_SHAPE_CUBE); break;
+      case HGX_SHAPE_CUBE: HGX_ENS32_LAUNCH(HGX_SHAPE_CUBE); break;
+      case HGX_SHAPE_PRISM3: HGX_ENS32_LAUNCH(HGX_SHAPE_PRISM3); break;
+      case HGX_SHAPE_PRISM6: HGX_ENS32_LAUNCH(HGX_SHAPE_PRISM6); break;
+      case HGX_SHAPE_PYRAMID: HGX_ENS32_LAUNCH(HGX_SHAPE_PYRAMID); break;
+      default: HGX_ENS32_LAUNCH(-1); break;
     }
 #undef HGX_ENS32_LAUNCH
     HGX_CHECK_LAUNCH(c);

@@ -1,9 +1,11 @@
-// ff_analytic.hip -- analytic form factors (FP64 for either precision argument), one thread per
-// q-point, node loops in registers.
+// ff_analytic.hip -- analytic form factors, one thread per q-point, node loops in registers: plain
+// FP64 (analytic_shapes.hpp) or FP32C on real q (analytic32.hpp: FP64-reduced phases, FP32 sines and
+// node sums, the FP64 formula where the prism and pyramid closed forms cancel).
 // Formulas are the CPU ones of the reference (src/ff/ff_ana_<shape>.cpp), which the reference's
 // GPU kernels do not all follow (src/ff/gpu/ff_ana_prism_gpu.cu:39-57, ff_ana_prism6_gpu.cu:47).
 #include <cmath>
 
+#include "analytic32.hpp"
 #include "analytic_shapes.hpp"
 
 namespace hgx {
@@ -25,6 +27,25 @@ __global__ __launch_bounds__(256) void k_ff_analytic(QViewA g, Rot R, Nodes P, V
   rotate_q(R, g.qx[p], g.qy[p], cd{g.qz[2 * q], g.qz[2 * q + 1]}, mq);
   cd F = shape_ff<SHAPE>(mq, P);
   if (has_tv) F = F * cexpi_d(mq[0] * tv.v[0] + mq[1] * tv.v[1] + mq[2] * tv.v[2]);
+  out[2 * q] = F.r;
+  out[2 * q + 1] = F.i;
+}
+
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ff_analytic32(QViewA g, Rot R, Nodes P, Vec3 tv, int has_tv,
+                                                       double* __restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= g.nq) return;
+  int64_t p = q % g.npix;
+  const double qx = g.qx[p], qy = g.qy[p], qz = g.qz[2 * q];
+  double m[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) m[j] = R.m[3 * j] * qx + R.m[3 * j + 1] * qy + R.m[3 * j + 2] * qz;
+  cd F = ff_fp32c<SHAPE>(m, P);
+  if (has_tv) {
+    const cf e = expi_turn(turns_of(m[0] * tv.v[0] + m[1] * tv.v[1] + m[2] * tv.v[2]));
+    F = F * cd{(double)e.r, (double)e.i};
+  }
   out[2 * q] = F.r;
   out[2 * q + 1] = F.i;
 }
@@ -68,6 +89,24 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   Vec3 tv = make_vec(transvec);
   int has_tv = transvec != nullptr;
   unsigned nb = ceil_div(c->nq, 256);
+  if (precision == HGX_PREC_FP32C) {
+    if (!c->qz_real)
+      return set_error(c, HGX_ERR_UNSUPPORTED, "ff_analytic: FP32C needs real q (a particle in a non-absorbing layer)");
+    LaunchTimer lt(c, true);
+#define HGX_A32(S) k_ff_analytic32<S><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff)
+    switch (shape) {
+      case HGX_SHAPE_SPHERE: HGX_A32(HGX_SHAPE_SPHERE); break;
+      case HGX_SHAPE_CYLINDER: HGX_A32(HGX_SHAPE_CYLINDER); break;
+      case HGX_SHAPE_BOX: HGX_A32(HGX_SHAPE_BOX); break;
+      case HGX_SHAPE_CUBE: HGX_A32(HGX_SHAPE_CUBE); break;
+      case HGX_SHAPE_PRISM3: HGX_A32(HGX_SHAPE_PRISM3); break;
+      case HGX_SHAPE_PRISM6: HGX_A32(HGX_SHAPE_PRISM6); break;
+      default: HGX_A32(HGX_SHAPE_PYRAMID); break;
+    }
+#undef HGX_A32
+    HGX_CHECK_LAUNCH(c);
+    return HGX_OK;
+  }
   LaunchTimer lt(c, true);
   switch (shape) {
     case HGX_SHAPE_SPHERE: k_ff_analytic<HGX_SHAPE_SPHERE><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff); break;

@@ -496,7 +496,7 @@ bool HipGISAXS::form_factor(const HigShape& sh, const Rot3& rot, const vec3& loc
   hgx_shape_params p;
   std::vector<double> store[8];
   if (!shape_params(sh, p, store)) return false;
-  return hgx(hgx_ff_analytic(gpu_->get(), sh.shape, &p, HGX_PREC_FP64, rot.data(), loc.data(), d_out),
+  return hgx(hgx_ff_analytic(gpu_->get(), sh.shape, &p, precision_, rot.data(), loc.data(), d_out),
              "hgx_ff_analytic");
 }
 
@@ -656,16 +656,9 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     m.rot = rots.data();
     m.center = cen.data();
     m.weight = wts.data();
-    // FP32C where the ensemble kernel has it (one shape among sphere, cylinder, box, cube), else FP64
-    int prec = HGX_PREC_FP64;
-    if (precision_ == HGX_PREC_FP32C) {
-      bool one = true;
-      for (int k = 1; k < ne; ++k) one = one && el[k].shape == el[0].shape;
-      const int s0 = el[0].shape;
-      if (one && (s0 == HGX_SHAPE_SPHERE || s0 == HGX_SHAPE_CYLINDER || s0 == HGX_SHAPE_BOX || s0 == HGX_SHAPE_CUBE))
-        prec = HGX_PREC_FP32C;
-    }
-    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, prec, d_fc, 0, d_I), "hgx_ensemble_intensity")) return false;
+    // the particles sit in the vacuum layer (dnl = 0 above), so q is real and FP32C takes any cell
+    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, precision_, d_fc, 0, d_I), "hgx_ensemble_intensity"))
+      return false;
     counts_[0] += ng;
     counts_[3] += ng;
     return true;

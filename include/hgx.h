@@ -197,8 +197,10 @@ int hgx_ff_numeric_adjoint(hgx_ctx* ctx, int mesh_id, int precision, int flags, 
 
 /* Analytic form factor, replaces AnalyticFormFactorG::compute_<shape>
  * (include/ff/gpu/ff_ana_gpu.cuh:54-94) with the CPU formulas of src/ff/ff_ana_<shape>.cpp.
- * Evaluated in FP64 for either precision value (HGX_PREC_FP32C is accepted for interface symmetry);
- * the FP32C analytic path is the fused grain-ensemble kernel, hgx_lattice_intensity. */
+ * HGX_PREC_FP64: plain FP64. HGX_PREC_FP32C (real q only, else HGX_ERR_UNSUPPORTED): phases formed
+ * and reduced to turns in FP64, FP32 sines and node sums, and, per q-point, the FP64 formula where
+ * the prism and pyramid closed forms cancel (near qx^2 = 3 qy^2, 3 qy^2 = qx^2, qx qy = 0 and small
+ * |q|); results in FP64. */
 int hgx_ff_analytic(hgx_ctx* ctx, int shape, const hgx_shape_params* params, int precision,
                     const double rot[9], const double transvec[3], double* ff);
 
@@ -311,9 +313,9 @@ typedef struct hgx_ensemble_model {
  * the grain, element and location loops, structure factor and DWBA sum of run_gisaxs
  * (src/sim/hipgisaxs_main.cpp:925-1100) for any analytic unit cell in one pass per pixel. fc (device
  * [2*nq]) may be NULL (Born). HGX_PREC_FP64 equals hgx_ff_analytic + hgx_caxpy + hgx_sf_lattice +
- * hgx_dwba_intensity per grain to round-off; HGX_PREC_FP32C (real q, cells whose elements are all
- * spheres, all cylinders, all boxes or all cubes) evaluates the form factors and Dirichlet kernels in
- * FP32 on phases reduced in FP64 and sums amplitudes in FP64. When every grain sees the cell alike
+ * hgx_dwba_intensity per grain to round-off; HGX_PREC_FP32C (real q, any cell) evaluates the form
+ * factors as hgx_ff_analytic's FP32C and the Dirichlet kernels in FP32 on phases reduced in FP64 and
+ * sums amplitudes in FP64. When every grain sees the cell alike
  * (spheres and cylinders whose grain rotations share their third row, e.g. rotations about z, at
  * locations every grain sees alike) the cell's form factor is evaluated once per q-point and branch
  * and only the structure factor per grain. Host tables are staged through pinned memory: no stream

@@ -74,6 +74,40 @@ def test_analytic_fp64_matches_oracle(gpu_ctx, oracle_mod, det_grid, shape):
     assert rel_l2(got, ref) <= 1e-10
 
 
+@pytest.mark.parametrize("shape", sorted(SHAPES))
+def test_analytic_fp32c_matches_oracle(gpu_ctx, oracle_mod, det_grid, shape):
+    """FP32C analytic form factors (real q) against the FP64 oracle: relL2 of F and, per pixel, the
+    relative error of |F|^2 (floored at 1e-10 of the peak); the prism and pyramid points near their
+    cancelling lines take the FP64 formula."""
+    O = oracle_mod
+    _upload(gpu_ctx, det_grid)
+    R = _rot(0.3, 0.05)
+    tv = np.array([1.0, -2.0, 0.5])
+    ref = O.ff_analytic(shape, SHAPES[shape], det_grid, R, tv, tau=0.1, eta=0.2)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_analytic(shape, SHAPES[shape], "fp32c", rot=R, transvec=tv, tau=0.1, eta=0.2))
+    assert np.all(np.isfinite(got))
+    e = rel_l2(got, ref)
+    I, Ir = np.abs(got) ** 2, np.abs(ref) ** 2
+    r = np.abs(I - Ir) / np.maximum(Ir, 1e-10 * Ir.max())
+    bright = Ir >= 1e-6 * Ir.max()
+    print(f"{shape} fp32c relL2 {e:.2e} max pixel {r.max():.2e} (within 6 decades of the peak "
+          f"{r[bright].max():.2e})")
+    assert e <= 1e-5
+    # an FP32 amplitude error of eps |F_peak| is a relative intensity error of about eps sqrt(I_peak / I):
+    # 1e-4 within six decades of the peak, 1e-2 down to ten
+    assert r[bright].max() <= 1e-4
+    assert r.max() <= 1e-2
+
+
+def test_analytic_fp32c_refuses_complex_q(gpu_ctx, oracle_mod):
+    from hipgisaxs_amd.api import HgxError
+
+    gpu_ctx.qgrid_qspace(16, 12, -1.0, 0.0, 1.0, 1.0, AI, K0)
+    gpu_ctx.qz_extended(K0, AI, 2e-6 + 1e-8j)
+    with pytest.raises(HgxError, match="real q"):
+        gpu_ctx.ff_analytic("sphere", SHAPES["sphere"], "fp32c")
+
+
 def test_pyramid_reference_compat_matches_oracle(gpu_ctx, oracle_mod, det_grid):
     _upload(gpu_ctx, det_grid)
     d = SHAPES["pyramid"]

@@ -87,6 +87,9 @@ def test_mixed_cell_matches_oracle(gpu_ctx, oracle_mod, absorbing):
     ("cube", [([9.0, 11.0], [0.6, 0.4])]),
     ("sphere", [([4.0, 5.0, 6.0], [0.25, 0.5, 0.25])]),
     ("cylinder", [([4.0], [1.0]), ([12.0, 14.0], [0.5, 0.5])]),
+    ("prism3", [([6.0, 7.0], [0.5, 0.5]), ([5.0], [1.0])]),
+    ("prism6", [([6.0], [1.0]), ([5.0, 5.5], [0.5, 0.5])]),
+    ("pyramid", [([16.0], [1.0]), ([16.0, 17.0], [0.5, 0.5]), ([4.0], [1.0]), ([54.73], [1.0])]),
 ])
 def test_fp32c_cells(gpu_ctx, oracle_mod, shape, dims):
     from hipgisaxs_amd.api import ensemble_intensity
@@ -111,10 +114,28 @@ def test_fp32c_refuses_what_it_does_not_cover(gpu_ctx, oracle_mod):
     with pytest.raises(HgxError, match="real q"):
         ensemble_intensity(gpu_ctx, ELEMENTS[:1], (20, 0, 0), (0, 20, 0), (0, 0, 20), (2, 2, 2), np.eye(3)[None],
                            precision="fp32c")
-    _grid(O, gpu_ctx, False)
-    with pytest.raises(HgxError, match="one shape"):
-        ensemble_intensity(gpu_ctx, ELEMENTS, (20, 0, 0), (0, 20, 0), (0, 0, 20), (2, 2, 2), np.eye(3)[None],
-                           precision="fp32c")
+
+
+
+def test_fp32c_mixed_cell_matches_oracle(gpu_ctx, oracle_mod):
+    """A cell of a sphere, a box and a pyramid with rotations, locations and grain centres in FP32C
+    (the shape chosen per element; pyramid points near qx qy = 0 in FP64)."""
+    from hipgisaxs_amd.api import ensemble_intensity
+
+    O = oracle_mod
+    g, fc = _grid(O, gpu_ctx, False)
+    la, lb, lc, rep = (25.0, 0, 0), (0, 25.0, 0), (12.5, 12.5, 20.0), (4, 3, 2)
+    rng = np.random.default_rng(9)
+    rots = [_rz(a) @ _rx(b) for a, b in zip(rng.uniform(0, np.pi, 5), rng.uniform(0, 0.1, 5))]
+    centers = rng.uniform(-20, 20, (5, 3))
+    weights = rng.uniform(0.5, 1.5, 5)
+    els = [dict(e, dn2=complex(e["dn2"]).real) for e in ELEMENTS]
+    got = ensemble_intensity(gpu_ctx, els, la, lb, lc, rep, np.stack(rots), centers=centers, weights=weights,
+                             fc=_dev(fc), precision="fp32c").cpu().numpy()
+    ref = _oracle(O, g, fc, els, la, lb, lc, rep, rots, centers, weights)
+    r = rel_l2(got, ref)
+    print("mixed fp32c relL2", r)
+    assert r <= 1e-5
 
 
 @pytest.mark.parametrize("compat", [False, True])
