@@ -26,6 +26,7 @@ FF_APPROX, FF_EXACT = 0, 1
 FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOLD = 1, 2, 4, 8
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
+SF_FP32C = 2
 SMEAR_COMPAT = 1
 HGX_DIST_REFERENCE = 1
 METRICS = {"chi2": 0, "l2": 1, "sqrt_unit_l2": 2, "sqrt_c_l2": 3, "cbrt_unit_l2": 4, "cbrt_c_l2": 5}

@@ -244,6 +244,18 @@ __device__ inline cd ff_fp32c(const double m[3], const Nodes& P) {
   return cd{(double)F.r, (double)F.i};
 }
 
+// S(m) in FP32 from FP64 lattice phases (the FP32C lattice kernel's form)
+__device__ inline cf sf32e(const double m[3], const SFArgs& s) {
+  const double xa = m[0] * s.la[0] + m[1] * s.la[1] + m[2] * s.la[2];
+  const double xb = m[0] * s.lb[0] + m[1] * s.lb[1] + (s.compat_lb2 ? s.lb[2] * s.lb[2] : m[2] * s.lb[2]);
+  const double xc = m[0] * s.lc[0] + m[1] * s.lc[1] + m[2] * s.lc[2];
+  float d = dirichlet_turn(turns_of(xa), (float)s.N[0]) * dirichlet_turn(turns_of(xb), (float)s.N[1]) *
+            dirichlet_turn(turns_of(xc), (float)s.N[2]);
+  const cf ph = expi_turn(turns_of(m[0] * s.center[0] + m[1] * s.center[1] + m[2] * s.center[2]));
+  if (s.dw != 0.0) d *= __expf((float)(-s.dw * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2])));
+  return d * ph;
+}
+
 // the same with the shape chosen at run time (cells of mixed shapes)
 __device__ inline cd ff_fp32c_any(int shape, const double m[3], const Nodes& P) {
   switch (shape) {

@@ -149,18 +149,6 @@ __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const doub
 
 // ----- FP32C -------------------------------------------------------------------------------------
 
-// S(m) in FP32 from FP64 lattice phases (the FP32C lattice kernel's form)
-__device__ inline cf sf32e(const double m[3], const SFArgs& s) {
-  const double xa = m[0] * s.la[0] + m[1] * s.la[1] + m[2] * s.la[2];
-  const double xb = m[0] * s.lb[0] + m[1] * s.lb[1] + (s.compat_lb2 ? s.lb[2] * s.lb[2] : m[2] * s.lb[2]);
-  const double xc = m[0] * s.lc[0] + m[1] * s.lc[1] + m[2] * s.lc[2];
-  float d = dirichlet_turn(turns_of(xa), (float)s.N[0]) * dirichlet_turn(turns_of(xb), (float)s.N[1]) *
-            dirichlet_turn(turns_of(xc), (float)s.N[2]);
-  const cf ph = expi_turn(turns_of(m[0] * s.center[0] + m[1] * s.center[1] + m[2] * s.center[2]));
-  if (s.dw != 0.0) d *= __expf((float)(-s.dw * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2])));
-  return d * ph;
-}
-
 // FP32C form-factor sum of the unit cell for grain o (FP64 result of FP32 terms).
 template <int SHAPE>
 __device__ inline cd cell32(int o, double qx, double qy, double qz, const EnsElem* __restrict__ E, int nelem,

@@ -1,6 +1,7 @@
 // sf_dwba.hip -- lattice structure factor, Parratt/Fresnel coefficients and the DWBA combination.
 #include <cmath>
 
+#include "analytic32.hpp"
 #include "lattice_sf.hpp"
 
 namespace hgx {
@@ -18,6 +19,23 @@ __global__ __launch_bounds__(256) void k_sf_lattice(const double* __restrict__ q
   cd s = sf_value(mq, a);
   out[2 * i] = s.r;
   out[2 * i + 1] = s.i;
+}
+
+// FP32C: lattice and centre phases formed and reduced to turns in FP64, Dirichlet kernels in FP32
+// (Taylor sines next to the Bragg peaks), real q only.
+__global__ __launch_bounds__(256) void k_sf_lattice32(const double* __restrict__ qx, const double* __restrict__ qy,
+                                                      const double* __restrict__ qz, int64_t npix, int64_t nq, Rot R,
+                                                      SFArgs a, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nq) return;
+  int64_t j = i % npix;
+  const double x = qx[j], y = qy[j], z = qz[2 * i];
+  double m[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) m[k] = R.m[3 * k] * x + R.m[3 * k + 1] * y + R.m[3 * k + 2] * z;
+  const cf s = sf32e(m, a);
+  out[2 * i] = (double)s.r;
+  out[2 * i + 1] = (double)s.i;
 }
 
 // ----- Parratt recursion (src/model/multilayer.cpp:65-108) --------------------------------------
@@ -130,10 +148,17 @@ extern "C" int hgx_sf_lattice(hgx_ctx* h, const double la[3], const double lb[3]
   }
   a.dw = 0.5 * dw_sigma * dw_sigma;
   a.compat_lb2 = (flags & HGX_SF_COMPAT_LB2) != 0;
+  if (flags & ~(HGX_SF_COMPAT_LB2 | HGX_SF_FP32C)) return set_error(c, HGX_ERR_INVALID_ARG, "sf_lattice: bad flags");
+  if ((flags & HGX_SF_FP32C) && !c->qz_real)
+    return set_error(c, HGX_ERR_UNSUPPORTED, "sf_lattice: FP32C needs real q (a particle in a non-absorbing layer)");
   if (int rc = set_device(c)) return rc;
   LaunchTimer lt(c, true);
-  k_sf_lattice<<<ceil_div(c->nq, 256), 256, 0, c->stream>>>(c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq, make_rot(rot), a,
-                                                            sf);
+  if (flags & HGX_SF_FP32C)
+    k_sf_lattice32<<<ceil_div(c->nq, 256), 256, 0, c->stream>>>(c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq,
+                                                                make_rot(rot), a, sf);
+  else
+    k_sf_lattice<<<ceil_div(c->nq, 256), 256, 0, c->stream>>>(c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq,
+                                                              make_rot(rot), a, sf);
   HGX_CHECK_LAUNCH(c);
   return HGX_OK;
 }

@@ -57,6 +57,7 @@ extern "C" {
 
 /* flags for hgx_sf_lattice */
 #define HGX_SF_COMPAT_LB2 1 /* reproduce the reference's lb[2]*lb[2] term (src/sf/sf.cpp:212) */
+#define HGX_SF_FP32C 2      /* hgx_sf_lattice in FP32C: phases reduced in FP64, FP32 Dirichlet kernels (real q) */
 
 /* analytic shapes */
 #define HGX_SHAPE_SPHERE 0   /* dims: [radius] */

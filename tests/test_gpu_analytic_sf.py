@@ -130,6 +130,11 @@ def test_sf_lattice_matches_oracle(gpu_ctx, oracle_mod, det_grid, lattice, compa
     ref = O.sf_lattice(la, lb, lc, rep, ce, det_grid, dw_sigma=0.5, flags=compat, rot=R)
     got = _gpu(gpu_ctx, gpu_ctx.sf_lattice(la, lb, lc, rep, ce, dw_sigma=0.5, flags=compat, rot=R))
     assert rel_l2(got, ref) <= 1e-10
+    # FP32C (HGX_SF_FP32C = 2): FP64-reduced phases, FP32 Dirichlet kernels
+    g32 = _gpu(gpu_ctx, gpu_ctx.sf_lattice(la, lb, lc, rep, ce, dw_sigma=0.5, flags=compat | 2, rot=R))
+    e32 = rel_l2(g32, ref)
+    print(f"sf {lattice} compat={compat} fp32c relL2 {e32:.2e}")
+    assert e32 <= 1e-5
 
 
 def test_fresnel_matches_oracle(gpu_ctx, oracle_mod, det_grid):
