@@ -231,6 +231,17 @@ def bench_pyramid_fp64(reps=5):
     torch.cuda.synchronize()
     t_exact = (time.perf_counter() - t0) / reps
     k_exact = ctx.profile_end()
+    # the FP32C exact method (FP32 series / divided differences per triangle) on the same inputs
+    ffe32 = ctx.ff_numeric(mid, "exact", "fp32c")
+    torch.cuda.synchronize()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ff_numeric(mid, "exact", "fp32c", out=ffe32)
+    torch.cuda.synchronize()
+    t_exact32 = (time.perf_counter() - t0) / reps
+    k_exact32 = ctx.profile_end()
+    rel_e32 = float(torch.linalg.vector_norm(ffe32 - ffe) / torch.linalg.vector_norm(ffe))
     # the FP32C path on the same inputs: its intensity against the FP64 one, per pixel
     ff32 = ctx.ff_numeric(mid, "approx", "fp32c")
     I32 = ctx.dwba_intensity(ff32, fc=fcd)
@@ -256,6 +267,7 @@ def bench_pyramid_fp64(reps=5):
     ctx.close()
     ka_ms = k_approx["main_ms"] / max(k_approx["main_launches"], 1)
     ke_ms = k_exact["main_ms"] / max(k_exact["main_launches"], 1)
+    ke32_ms = k_exact32["main_ms"] / max(k_exact32["main_launches"], 1)
     # FP64 roofline of the form-factor kernels: algorithmic flop per pair (approximate on this product
     # grid: 16, two complex multiply-adds on the FP64 matrix cores; exact about 200 on the VALU;
     # transcendentals not counted) over the kernel's own time, against 78.6 TFLOP/s (the FP64 peak of
@@ -269,7 +281,16 @@ def bench_pyramid_fp64(reps=5):
             "relL2_intensity_gpu_vs_oracle": float(np.linalg.norm(got - I_ref) / np.linalg.norm(I_ref)),
             "roofline_approx_kernel": dict(roof(16, ka_ms, "fp64-mfma"), kernel="k_ff_product64"),
             "exact_method": {"gpu_ms": 1e3 * t_exact, "gpu_pairs_per_s": pairs / t_exact,
-                             "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms, "fp64-valu")},
+                             "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms, "fp64-valu"),
+                             # FP32 series with 16-18 terms: about 110 flop per pair (phase, q.sn, two
+                             # q.d, p and r, the recursion, the complex combine; sin/cos not counted)
+                             "fp32c": {"gpu_ms": 1e3 * t_exact32, "gpu_pairs_per_s": pairs / t_exact32,
+                                       "speedup_vs_fp64": t_exact / t_exact32, "relL2_vs_fp64": rel_e32,
+                                       "roofline": dict(roof(110, ke32_ms, "fp32-valu"),
+                                                        peak=PEAK_FP32 / 1e12,
+                                                        frac=110 * pairs / (ke32_ms * 1e-3) / PEAK_FP32,
+                                                        achieved=110 * pairs / (ke32_ms * 1e-3) / 1e12,
+                                                        kernel="k_ff_exact32")}},
             "fp32c": {"relL2_intensity_vs_fp64": rel32, "pixel_error_vs_fp64": pix.result()},
             "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA; "
                         "exact-triangle FF of the same mesh and grid"}

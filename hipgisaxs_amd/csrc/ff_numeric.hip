@@ -191,6 +191,242 @@ __global__ __launch_bounds__(kWG) void k_ff_exact_finish(QView g, Rot R, int nch
   out[2 * q + 1] = F.i;
 }
 
+// ----- FP32 compensated exact-triangle kernel -------------------------------------------------
+//
+// The flat-triangle integral is the centroid term of the approximate method times a factor:
+//     int_T exp(i q.r) dS = s exp(i q.c) S(q),  S = -2 g[x1, x2, x3],  g(x) = exp(i x),
+// the second divided difference of exp(i x) at x_j = q.d_j, d_j = v_j - c. Two evaluations, both
+// free of the cancellations the closed form (an edge sum with 1/(q_t^2 q_e^2) factors) has near
+// degenerate directions:
+//   * series: S = 2 sum_n i^n h_n(x) / (n+2)!, h_n the complete homogeneous symmetric polynomial.
+//     The x_j sum to zero, so h_n = p h_{n-2} + r h_{n-3} with p = x1^2 + x1 x2 + x2^2 and
+//     r = -x1 x2 (x1 + x2). With a = max|q| (over the wave) x max|d| (over the 128-triangle block),
+//     the FP32 truncation plus rounding error of S stays below 2e-7 (absolute; |S| <= 1) for
+//     8/10/12/16/18 terms up to a = 0.5/1/2/3/4;
+//   * divided differences, for blocks with a > 4: with x sorted lo <= md <= hi,
+//     g[u, v] = i exp(i (u+v)/2) sinc((v-u)/2) (no cancellation at any spacing) and
+//     S = -2 (g[md, hi] - g[lo, md]) / (hi - lo); a triangle (lane) whose spread hi - lo is below 1
+//     takes the 10-term series instead (then max|x_j| <= 2/3). FP32 error below 3.2e-7 at spread 1,
+//     falling as the spread grows.
+
+// c_n i^n split: even n feed Re S with sign (-1)^(n/2), odd n feed Im S with sign (-1)^((n-1)/2)
+__device__ constexpr float exact_coef(int n) {
+  double f = 1.0;
+  for (int j = 3; j <= n + 2; ++j) f *= j;
+  return (float)(((n / 2) % 2 ? -2.0 : 2.0) / (2.0 * f));
+}
+
+// S for x1 = q.d1, x2 = q.d2 (radians) with NT terms (NT even, >= 4).
+template <int NT>
+__device__ inline void exact_series(float x1, float x2, float& Sr, float& Si) {
+  const float p = __builtin_fmaf(x1, x1, x2 * (x1 + x2));
+  const float r = -(x1 * x2) * (x1 + x2);
+  // h0 = 1, h1 = 0, h2 = p, h3 = r
+  Sr = __builtin_fmaf(exact_coef(2), p, 1.0f);
+  Si = exact_coef(3) * r;
+  float h1 = 0.0f, h2 = p, h3 = r;
+#pragma unroll
+  for (int n = 4; n < NT; n += 2) {
+    const float he = __builtin_fmaf(p, h2, r * h1);  // h_n
+    const float ho = __builtin_fmaf(p, h3, r * h2);  // h_{n+1}
+    Sr = __builtin_fmaf(exact_coef(n), he, Sr);
+    Si = __builtin_fmaf(exact_coef(n + 1), ho, Si);
+    h1 = h3;
+    h2 = he;
+    h3 = ho;
+  }
+}
+
+// sin(h)/h: Taylor to h^8 for h^2 < 1/4 (truncation below 3e-11), else the hardware sine (turns).
+__device__ inline float sinc_small_safe(float h) {
+  const float h2 = h * h;
+  const float t = __builtin_fmaf(
+      h2, __builtin_fmaf(h2, __builtin_fmaf(h2, __builtin_fmaf(h2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f),
+                         -1.0f / 6.0f),
+      1.0f);
+  const float big = __builtin_amdgcn_sinf(h * (float)kInvTwoPi) * __builtin_amdgcn_rcpf(h);
+  return h2 < 0.25f ? t : big;
+}
+
+// g[u, v] = i exp(i (u+v)/2) sinc((v-u)/2)
+__device__ inline void exp_dd1(float u, float v, float& gr, float& gi) {
+  const float m = 0.5f * (u + v) * (float)kInvTwoPi;  // turns
+  const float sc = sinc_small_safe(0.5f * (v - u));
+  gr = -__builtin_amdgcn_sinf(m) * sc;
+  gi = __builtin_amdgcn_cosf(m) * sc;
+}
+
+// S by divided differences (spread >= 1) or the 10-term series.
+__device__ inline void exact_dd(float x1, float x2, float& Sr, float& Si) {
+  const float x3 = -(x1 + x2);
+  const float lo = fminf(fminf(x1, x2), x3), hi = fmaxf(fmaxf(x1, x2), x3);
+  const float md = __builtin_amdgcn_fmed3f(x1, x2, x3);
+  if (hi - lo < 1.0f) {
+    exact_series<10>(x1, x2, Sr, Si);
+    return;
+  }
+  float ar, ai, br, bi;
+  exp_dd1(md, hi, ar, ai);
+  exp_dd1(lo, md, br, bi);
+  const float w = -2.0f * __builtin_amdgcn_rcpf(hi - lo);
+  Sr = (ar - br) * w;
+  Si = (ai - bi) * w;
+}
+
+// One block of 128 triangles in FP32, block-relative: acc_k += sum_t (q_k.s n_t) exp(i q_k.(c_t - o_b)) S_t(q_k).
+// NT: series terms, 0 for divided differences.
+template <int QPT, int NT>
+__device__ inline void exact32_block(const float* __restrict__ rb, const float* __restrict__ db, const float (&mx)[QPT],
+                                     const float (&my)[QPT], const float (&mz)[QPT], const float (&ux)[QPT],
+                                     const float (&uy)[QPT], const float (&uz)[QPT], float (&sr)[QPT],
+                                     float (&si)[QPT]) {
+#pragma unroll 2
+  for (int t = 0; t < kBlockTri; ++t) {
+    const float* r = rb + 8 * t;
+    const float* d = db + 8 * t;
+    const float snx = r[0], sny = r[1], snz = r[2], cx = r[3], cy = r[4], cz = r[5];
+    const float d1x = d[0], d1y = d[1], d1z = d[2], d2x = d[3], d2y = d[4], d2z = d[5];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+      const float ph = ux[k] * cx + uy[k] * cy + uz[k] * cz;
+      const float s = __builtin_amdgcn_sinf(ph);
+      const float c = __builtin_amdgcn_cosf(ph);
+      const float qn = mx[k] * snx + my[k] * sny + mz[k] * snz;
+      const float x1 = mx[k] * d1x + my[k] * d1y + mz[k] * d1z;
+      const float x2 = mx[k] * d2x + my[k] * d2y + mz[k] * d2z;
+      float Sr, Si;
+      if (NT == 0)
+        exact_dd(x1, x2, Sr, Si);
+      else
+        exact_series<NT == 0 ? 4 : NT>(x1, x2, Sr, Si);
+      const float er = qn * c, ei = qn * s;
+      sr[k] = __builtin_fmaf(er, Sr, __builtin_fmaf(-ei, Si, sr[k]));
+      si[k] = __builtin_fmaf(er, Si, __builtin_fmaf(ei, Sr, si[k]));
+    }
+  }
+}
+
+// Largest |q|^2 over the q-points of this thread's wave (the kWG-strided tile layout below), from
+// the unrotated real grid (a rotation keeps |q|).
+template <int QPT>
+__device__ inline float wave_q2max(const QView& g, int64_t qbase) {
+  float q2m = 0.0f;
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    q = q < g.nq ? q : g.nq - 1;
+    const int64_t p = q % g.npix;
+    const double qx = g.qx[p], qy = g.qy[p], qz = g.qz[2 * q];
+    q2m = fmaxf(q2m, (float)(qx * qx + qy * qy + qz * qz));
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) q2m = fmaxf(q2m, __shfl_xor(q2m, off));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q2m)));
+}
+
+// Raw sums P = sum_t (q.s n_t) exp(i q.c_t) S_t(q) over blocks [b0, b1) for q = qsign * (R q_grid)
+// (qsign -1 gives the reference's phase exp(-i q.r)); real q only. FP32 inside a block, FP64
+// across blocks, like the approximate FP32C kernel.
+template <int QPT>
+__global__ __launch_bounds__(kWG) void k_ff_exact32(QView g, Rot R, double qsign, const float* __restrict__ rec,
+                                                    const float* __restrict__ dv, const double* __restrict__ borig,
+                                                    const float* __restrict__ bdmax, int64_t nblocks, int64_t bchunk,
+                                                    int nchunks, double* __restrict__ part, double* __restrict__ out) {
+  const int64_t qbase = (int64_t)blockIdx.x * (kWG * QPT) + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int64_t b0 = (int64_t)chunk * bchunk;
+  const int64_t b1 = min(nblocks, b0 + bchunk);
+  const float q2m = wave_q2max<QPT>(g, qbase);
+  float mx[QPT], my[QPT], mz[QPT], ux[QPT], uy[QPT], uz[QPT];
+  double ar[QPT], ai[QPT];
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    cd mq[3];
+    load_q<false>(g, R, q < g.nq ? q : g.nq - 1, mq);
+    const double qx = qsign * mq[0].r, qy = qsign * mq[1].r, qz = qsign * mq[2].r;
+    mx[k] = (float)qx;
+    my[k] = (float)qy;
+    mz[k] = (float)qz;
+    ux[k] = (float)(qx * kInvTwoPi);
+    uy[k] = (float)(qy * kInvTwoPi);
+    uz[k] = (float)(qz * kInvTwoPi);
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    // a^2 = (max |q|)^2 (max |d|)^2: the evaluation is chosen per block for the whole wave
+    const float bd = bdmax[b];
+    const float a2 = q2m * (bd * bd);
+    float sr[QPT], si[QPT];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) sr[k] = si[k] = 0.0f;
+    const float* rb = rec + (int64_t)8 * kBlockTri * b;
+    const float* db = dv + (int64_t)8 * kBlockTri * b;
+    if (a2 <= 0.25f)
+      exact32_block<QPT, 8>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    else if (a2 <= 1.0f)
+      exact32_block<QPT, 10>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    else if (a2 <= 4.0f)
+      exact32_block<QPT, 12>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    else if (a2 <= 9.0f)
+      exact32_block<QPT, 16>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    else if (a2 <= 16.0f)
+      exact32_block<QPT, 18>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    else
+      exact32_block<QPT, 0>(rb, db, mx, my, mz, ux, uy, uz, sr, si);
+    // rotate by exp(i q.o_b): FP64 phase in turns, reduced in FP64
+    const double ox = borig[4 * b], oy = borig[4 * b + 1], oz = borig[4 * b + 2];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+      int64_t q = qbase + (int64_t)k * kWG;
+      cd mq[3];
+      load_q<false>(g, R, q < g.nq ? q : g.nq - 1, mq);
+      const double tb = qsign * (mq[0].r * ox + mq[1].r * oy + mq[2].r * oz) * kInvTwoPi;
+      const float fr = (float)(tb - rint(tb));
+      const double cb = (double)__builtin_amdgcn_cosf(fr), sb = (double)__builtin_amdgcn_sinf(fr);
+      ar[k] += (double)sr[k] * cb - (double)si[k] * sb;
+      ai[k] += (double)sr[k] * sb + (double)si[k] * cb;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < QPT; ++k) {
+    int64_t q = qbase + (int64_t)k * kWG;
+    if (q >= g.nq) continue;
+    double* dst = nchunks == 1 ? out + 2 * q : part + 2 * ((int64_t)chunk * g.nq + q);
+    dst[0] = ar[k];
+    dst[1] = ai[k];
+  }
+}
+
+// Epilogue of k_ff_exact32: sums the chunks in order and finishes at q = qsign R q_grid (small-|q|
+// series from the flat-triangle moments); the translation phase stays exp(i (R q_grid).t), as in
+// the FP64 exact kernel.
+__global__ __launch_bounds__(kWG) void k_ff_exact32_finish(QView g, Rot R, double qsign, int nchunks,
+                                                           const double* __restrict__ part, double* __restrict__ out,
+                                                           Epi e) {
+  int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (q >= g.nq) return;
+  cd mq[3];
+  load_q<false>(g, R, q, mq);
+  double pr = 0.0, pi = 0.0;
+  if (nchunks == 1) {
+    pr = out[2 * q];
+    pi = out[2 * q + 1];
+  }
+  for (int c = 0; nchunks > 1 && c < nchunks; ++c) {
+    pr += part[2 * ((int64_t)c * g.nq + q)];
+    pi += part[2 * ((int64_t)c * g.nq + q) + 1];
+  }
+  const cd qe[3] = {qsign * mq[0], qsign * mq[1], qsign * mq[2]};
+  Epi e0 = e;
+  e0.has_tv = 0;
+  cd F = finish_q(e0, qe, cd{pr, pi});
+  if (e.has_tv) F = F * cexpi_d(mq[0] * e.tv.v[0] + mq[1] * e.tv.v[1] + mq[2] * e.tv.v[2]);
+  out[2 * q] = F.r;
+  out[2 * q + 1] = F.i;
+}
+
 // ----- host-side launch planning --------------------------------------------------------------
 
 struct Plan {
@@ -441,8 +677,51 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
   if (method == HGX_FF_EXACT) {
     if (m.kind != HGX_MESH_EXACT9 || !m.d_tri9)
       return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: exact method needs a vertex (exact9) mesh");
+    double sign = (flags & HGX_FLAG_EXACT_REFERENCE_SIGN) ? 1.0 : -1.0;
+    // degenerate-case threshold on q.n and q.e: the reference GPU path's 1e-13 (ff_tri_gpu.cu:80,100)
+    // by default; its CPU path's DBL_MIN (ff_tri_cpu.cpp:87,107) on request. With DBL_MIN a case
+    // decision can flip on the last bit of a nearly degenerate projection, and the general case then
+    // divides by it: on a finely meshed solid most results become rounding noise.
+    const double tiny = (flags & HGX_FLAG_EXACT_CPU_THRESHOLD) ? 2.2250738585072014e-308 : 1e-13;
+    if (precision == HGX_PREC_FP32C) {
+      if (cq)
+        return set_error(c, HGX_ERR_UNSUPPORTED,
+                         "ff_numeric: the FP32C exact method takes a real q-grid (FP64 handles complex qz)");
+      if (!m.d_dv32 || !m.d_bdmax || !m.d_xmoments)
+        return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: mesh has no exact-triangle tables");
+      const int qpt = c->tune.qpt32 >= 4 ? 4 : c->tune.qpt32 >= 2 ? 2 : 1;
+      Plan p = make_plan(c->nq, m.nblocks, 1, qpt, c->tune.target_wg, chunk_limit(work, c->nq, 0));
+      double* part = nullptr;
+      if (p.nchunks > 1) {
+        char* w = nullptr;
+        if (int rc = take_work(c, work, (size_t)p.nchunks * c->nq * 2 * sizeof(double), &w)) return rc;
+        part = reinterpret_cast<double*>(w);
+      }
+      e.moments = m.d_xmoments;
+      e.radius = m.xradius;
+      e.use_series = 1;
+      const double qsign = -sign;  // phase exp(i qsign q.r)
+      dim3 grid(p.nqt, p.nchunks);
+      {
+        OptTimer lt(c, true, timed);
+#define HGX_LX32(Q)                                                                                      \
+  k_ff_exact32<Q><<<grid, kWG, 0, c->stream>>>(g, R, qsign, m.d_rec32, m.d_dv32, m.d_borig, m.d_bdmax, m.nblocks, \
+                                               p.per_chunk, p.nchunks, part, ff)
+        if (p.qpt == 4) HGX_LX32(4);
+        else if (p.qpt == 2) HGX_LX32(2);
+        else HGX_LX32(1);
+#undef HGX_LX32
+      }
+      HGX_CHECK_LAUNCH(c);
+      {
+        OptTimer lt(c, false, timed);
+        k_ff_exact32_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, qsign, p.nchunks, part, ff, e);
+        HGX_CHECK_LAUNCH(c);
+      }
+      return HGX_OK;
+    }
     if (precision != HGX_PREC_FP64)
-      return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric: exact method is FP64 only");
+      return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric: bad precision for the exact method");
     Plan p = make_plan(c->nq, m.ntri, 64, 1, c->tune.target_wg, chunk_limit(work, c->nq, 0));
     double* part = nullptr;
     if (p.nchunks > 1) {
@@ -450,12 +729,6 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
       if (int rc = take_work(c, work, (size_t)p.nchunks * c->nq * 2 * sizeof(double), &w)) return rc;
       part = reinterpret_cast<double*>(w);
     }
-    double sign = (flags & HGX_FLAG_EXACT_REFERENCE_SIGN) ? 1.0 : -1.0;
-    // degenerate-case threshold on q.n and q.e: the reference GPU path's 1e-13 (ff_tri_gpu.cu:80,100)
-    // by default; its CPU path's DBL_MIN (ff_tri_cpu.cpp:87,107) on request. With DBL_MIN a case
-    // decision can flip on the last bit of a nearly degenerate projection, and the general case then
-    // divides by it: on a finely meshed solid most results become rounding noise.
-    const double tiny = (flags & HGX_FLAG_EXACT_CPU_THRESHOLD) ? 2.2250738585072014e-308 : 1e-13;
     dim3 grid(p.nqt, p.nchunks);
     {
       OptTimer lt(c, true, timed);

@@ -101,6 +101,9 @@ static void free_mesh(Mesh& m) {
   if (m.d_tri9) (void)hipFree(m.d_tri9);
   if (m.d_moments) (void)hipFree(m.d_moments);
   if (m.d_src) (void)hipFree(m.d_src);
+  if (m.d_dv32) (void)hipFree(m.d_dv32);
+  if (m.d_xmoments) (void)hipFree(m.d_xmoments);
+  if (m.d_bdmax) (void)hipFree(m.d_bdmax);
   m = Mesh();
 }
 
@@ -604,6 +607,60 @@ static void accumulate_moments(const double* sn, const double* d, double* M) {
         }
 }
 
+// Index of monomial (a, b, n - a - b) among those of degree n in the moment layout's order.
+static inline int mono_index(int n, int a, int b) { return (n - a) * (n - a + 1) / 2 + (n - a - b); }
+
+// R = (w . q) P(q) for P homogeneous of degree n - 1, both as coefficients of the monomials of q.
+static void mul_linear(const double* P, int n, const double* w, double* R) {
+  for (int a = n; a >= 0; --a)
+    for (int b = n - a; b >= 0; --b) {
+      const int cc = n - a - b;
+      double v = 0.0;
+      if (a > 0) v += w[0] * P[mono_index(n - 1, a - 1, b)];
+      if (b > 0) v += w[1] * P[mono_index(n - 1, a, b - 1)];
+      if (cc > 0) v += w[2] * P[mono_index(n - 1, a, b)];
+      R[mono_index(n, a, b)] = v;
+    }
+}
+
+// The same moments for flat triangles with vertices o + w_j (exact meshes):
+// M = sum_t s_t n_{t,i} <(r - o)^(a,b,c)>_t, the average over triangle t. It follows from
+// <(q.(r - o))^n>_t = 2 n!/(n+2)! h_n(q.w_1, q.w_2, q.w_3), with h_n the complete homogeneous
+// symmetric polynomial, built degree by degree from h_n(x1..x3) = h_n(x1, x2) + x3 h_{n-1}(x1..x3),
+// h_n(x1, x2) = x1^n + x2 h_{n-1}(x1, x2). The symmetric-tensor entry of a monomial is its
+// coefficient times a! b! c! / n!.
+static void accumulate_moments_exact(const double* sn, const double (*w)[3], double* M) {
+  constexpr int kTop = (kSeriesOrder + 1) * (kSeriesOrder + 2) / 2;
+  double x1[kTop], h2[kTop], h3[kTop], t1[kTop], t2[kTop], t3[kTop];
+  double fact[kSeriesOrder + 3];
+  fact[0] = 1.0;
+  for (int k = 1; k <= kSeriesOrder + 2; ++k) fact[k] = fact[k - 1] * k;
+  x1[0] = h2[0] = h3[0] = 1.0;
+  int idx = 0;
+  for (int n = 0; n <= kSeriesOrder; ++n) {
+    const int nm = (n + 1) * (n + 2) / 2;
+    if (n > 0) {
+      mul_linear(x1, n, w[0], t1);
+      mul_linear(h2, n, w[1], t2);
+      mul_linear(h3, n, w[2], t3);
+      for (int j = 0; j < nm; ++j) {
+        t2[j] += t1[j];
+        t3[j] += t2[j];
+      }
+      std::copy(t1, t1 + nm, x1);
+      std::copy(t2, t2 + nm, h2);
+      std::copy(t3, t3 + nm, h3);
+    }
+    const double scale = 2.0 / fact[n + 2];
+    for (int i = 0; i < 3; ++i)
+      for (int a = n; a >= 0; --a)
+        for (int b = n - a; b >= 0; --b) {
+          const int cc = n - a - b;
+          M[idx++] += sn[i] * scale * fact[a] * fact[b] * fact[cc] * h3[mono_index(n, a, b)];
+        }
+  }
+}
+
 int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, int* mesh_id) {
   Context* c = ctx_of(h);
   if (!c) return HGX_ERR_BAD_HANDLE;
@@ -681,6 +738,8 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   std::vector<float> rec32(8 * m.nblocks * kBlockTri, 0.0f);
   std::vector<double> borig(4 * m.nblocks, 0.0);
   std::vector<double> tri9s(kind == HGX_MESH_EXACT9 ? 9 * ntri : 0);
+  std::vector<float> dv32(kind == HGX_MESH_EXACT9 ? 8 * m.nblocks * kBlockTri : 0, 0.0f);
+  std::vector<float> bdmax(kind == HGX_MESH_EXACT9 ? m.nblocks : 0, 0.0f);
   double vol = 0.0, orig[3] = {0, 0, 0};
   for (int64_t t = 0; t < ntri; ++t) {
     const double* r = &s_n_c[7 * order[t]];
@@ -706,6 +765,19 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
     accumulate_moments(o, d, M.data());
   }
   m.radius = rad;
+  std::vector<double> XM(kind == HGX_MESH_EXACT9 ? kMomentCount : 0, 0.0);
+  if (kind == HGX_MESH_EXACT9) {
+    double xr = 0.0;
+    for (int64_t t = 0; t < ntri; ++t) {
+      double w[3][3];
+      for (int j = 0; j < 3; ++j) {
+        for (int k = 0; k < 3; ++k) w[j][k] = tri9s[9 * t + 3 * j + k] - m.origin[k];
+        xr = std::max(xr, std::sqrt(w[j][0] * w[j][0] + w[j][1] * w[j][1] + w[j][2] * w[j][2]));
+      }
+      accumulate_moments_exact(&rec64[8 * t], w, XM.data());
+    }
+    m.xradius = xr;
+  }
   {
     // A closed surface has sum_t s_t n_t = 0 (divergence theorem); what the sum holds then is
     // rounding, which the series would divide by |q| and so blow up as q -> 0. Zero it for meshes
@@ -719,7 +791,10 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
     }
     m.snmax = amax;
     double n0 = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
-    if (n0 <= 1e-8 * asum) M[0] = M[1] = M[2] = 0.0;
+    if (n0 <= 1e-8 * asum) {
+      M[0] = M[1] = M[2] = 0.0;
+      if (!XM.empty()) XM[0] = XM[1] = XM[2] = 0.0;  // the same sums
+    }
   }
   for (int64_t b = 0; b < m.nblocks; ++b) {
     int64_t t0 = b * kBlockTri, t1 = std::min<int64_t>(ntri, t0 + kBlockTri);
@@ -738,6 +813,23 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
       f[5] = (float)(o[5] - borig[4 * b + 2]);
     }
     // padding triangles of the last block keep s*n = 0 and contribute exactly zero
+    if (kind == HGX_MESH_EXACT9) {
+      double dm = 0.0;
+      for (int64_t t = t0; t < t1; ++t) {
+        const double* v = &tri9s[9 * t];
+        const double* o = &rec64[8 * t];
+        for (int i = 0; i < 3; ++i) {
+          double d2 = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            const double d = v[3 * i + k] - o[3 + k];
+            d2 += d * d;
+            if (i < 2) dv32[8 * t + 3 * i + k] = (float)d;
+          }
+          dm = std::max(dm, std::sqrt(d2));
+        }
+      }
+      bdmax[b] = (float)dm;
+    }
   }
   // 4. device copies
   auto fail = [&](const char* what) {
@@ -751,6 +843,12 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
   if (hipMalloc(&m.d_moments, M.size() * sizeof(double)) != hipSuccess) return fail("moments");
   if (hipMalloc(&m.d_src, src.size() * sizeof(int64_t)) != hipSuccess) return fail("index map");
   if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_tri9, tri9s.size() * sizeof(double)) != hipSuccess) return fail("tri9");
+  if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_dv32, dv32.size() * sizeof(float)) != hipSuccess)
+    return fail("vertex offsets");
+  if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_bdmax, bdmax.size() * sizeof(float)) != hipSuccess)
+    return fail("block extents");
+  if (kind == HGX_MESH_EXACT9 && hipMalloc(&m.d_xmoments, XM.size() * sizeof(double)) != hipSuccess)
+    return fail("triangle moments");
   auto h2d = [](void* dst, const void* src, size_t n) {
     return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
   };
@@ -759,7 +857,10 @@ int hgx_mesh_upload(hgx_ctx* h, int kind, int64_t ntri_in, const double* rec, in
       !h2d(m.d_borig, borig.data(), borig.size() * sizeof(double)) ||
       !h2d(m.d_moments, M.data(), M.size() * sizeof(double)) ||
       !h2d(m.d_src, src.data(), src.size() * sizeof(int64_t)) ||
-      (kind == HGX_MESH_EXACT9 && !h2d(m.d_tri9, tri9s.data(), tri9s.size() * sizeof(double)))) {
+      (kind == HGX_MESH_EXACT9 && !h2d(m.d_tri9, tri9s.data(), tri9s.size() * sizeof(double))) ||
+      (kind == HGX_MESH_EXACT9 && !h2d(m.d_dv32, dv32.data(), dv32.size() * sizeof(float))) ||
+      (kind == HGX_MESH_EXACT9 && !h2d(m.d_bdmax, bdmax.data(), bdmax.size() * sizeof(float))) ||
+      (kind == HGX_MESH_EXACT9 && !h2d(m.d_xmoments, XM.data(), XM.size() * sizeof(double)))) {
     hipError_t e = hipGetLastError();
     free_mesh(m);
     return set_error(c, HGX_ERR_DEVICE, "mesh_upload: copy to the device failed: %s", hipGetErrorString(e));

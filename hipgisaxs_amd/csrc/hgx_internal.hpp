@@ -38,13 +38,17 @@ struct Mesh {
   double volume = 0.0;    // sum_t s_t (n_t . c_t) / 3
   double origin[3] = {0, 0, 0};
   double radius = 0.0;    // max_t |c_t - origin|
+  double xradius = 0.0;   // max |v - origin| over the vertices (exact meshes)
   double snmax = 0.0;     // max_t |s_t n_t|
   // device arrays
   double* d_rec64 = nullptr;   // [ntri][8]: s*nx, s*ny, s*nz, cx, cy, cz, 0, 0 (absolute, FP64)
   float* d_rec32 = nullptr;    // [nblocks*kBlockTri][8]: s*n (3), c - block origin (3), 0, 0 (FP32)
   double* d_borig = nullptr;   // [nblocks][4]: block origin x, y, z, 0
   double* d_tri9 = nullptr;    // [ntri][9] vertices (exact meshes only), same order as d_rec64
+  float* d_dv32 = nullptr;     // [nblocks*kBlockTri][8] vertices 1, 2 minus the centroid (exact meshes), FP32
+  float* d_bdmax = nullptr;    // [nblocks] largest vertex-to-centroid distance in the block (exact meshes)
   double* d_moments = nullptr; // [kMomentCount] moments about origin (see ff_numeric.hip)
+  double* d_xmoments = nullptr; // [kMomentCount] the same series for flat triangles (exact meshes)
   int64_t* d_src = nullptr;    // [ntri] caller's index of each stored (sorted) triangle
 };
 

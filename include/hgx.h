@@ -167,7 +167,14 @@ int hgx_mesh_info(const hgx_ctx* ctx, int mesh_id, int64_t* ntri, double* volume
  *  HGX_FF_APPROX replaces NumericFormFactorC/G::compute_approx_triangle
  *    (src/ff/cpu/ff_tri_cpu.cpp:200-246; src/ff/gpu/ff_tri_gpu.cu:395-521;
  *     signature include/ff/gpu/ff_num_gpu.cuh:55-58);
- *  HGX_FF_EXACT replaces compute_exact_triangle (src/ff/cpu/ff_tri_cpu.cpp:46-195).
+ *  HGX_FF_EXACT replaces compute_exact_triangle (src/ff/cpu/ff_tri_cpu.cpp:46-195;
+ *    src/ff/gpu/ff_tri_gpu.cu:144-205). HGX_PREC_FP32C with HGX_FF_EXACT needs a real q-grid
+ *    (HGX_ERR_UNSUPPORTED otherwise): each flat triangle is its centroid term times the second
+ *    divided difference of exp(i x) at x_j = q.d_j (d_j: vertex minus centroid), in FP32 as the
+ *    series 2 sum_n h_n(i x)/(n+2)! (128-triangle blocks with |q| max|d| <= 4) or from sinc-form
+ *    first differences (beyond), plus an FP64 moment series at small |q|. It has no degenerate-case
+ *    thresholds (HGX_FLAG_EXACT_CPU_THRESHOLD does not apply) and stays accurate on sliver
+ *    triangles, where the closed form of the FP64 path (the reference's algorithm) cancels.
  * transvec (may be NULL) multiplies by exp(i q.t); the reference ignores the unit-cell location
  * for numeric shapes (src/ff/ff.cpp:44), so pass NULL to reproduce it.
  * At q = 0 the result is the mesh volume (the reference divides 0/0 there). */
