@@ -190,6 +190,30 @@ def _det_grid(ctx, side, pixel_mm, alpha_i, k0, dwba=True):
         ctx.qz_extended(k0, alpha_i, 0j)
 
 
+def exact32_flop_per_pair(g, tri, qpt=4):
+    """Flop per pair of the FP32C exact kernel's series on grid g: 39 + 5 (NT - 4) for NT terms
+    (phase, q.sn, q.d1, q.d2: 5 each; p, r: 6; series start: 3; complex combine: 10; each further
+    pair of terms: 10; sin/cos not counted), NT chosen per wave (64 lanes x qpt q-points strided by
+    256) from a^2 = max|q|^2 max|d|^2, here with the mesh-wide max|d| (an upper bound per block)."""
+    npix, nq = len(g.qx), len(g.qz)
+    q2 = np.tile(g.qx, nq // npix) ** 2 + np.tile(g.qy, nq // npix) ** 2 + g.qz.real ** 2
+    t = np.asarray(tri, dtype=np.float64).reshape(-1, 3, 3)
+    bd2 = float((np.linalg.norm(t - t.mean(1)[:, None], axis=2).max()) ** 2)
+    per_wg = 256 * qpt
+    mix = {}
+    for base in range(0, nq, per_wg):
+        for w in range(4):
+            idx = np.array([base + w * 64 + l + k * 256 for k in range(qpt) for l in range(64)])
+            idx = np.minimum(idx, nq - 1)
+            a2 = float(q2[idx].max()) * bd2
+            nt = next((n for lim, n in ((0.25, 8), (1, 10), (4, 12), (9, 16), (16, 18)) if a2 <= lim), 0)
+            mix[nt] = mix.get(nt, 0) + 1
+    if 0 in mix:
+        return None, mix  # divided-difference waves: no single flop model
+    tot = sum(mix.values())
+    return sum((39 + 5 * (n - 4)) * c for n, c in mix.items()) / tot, {str(k): v for k, v in mix.items()}
+
+
 def bench_pyramid_fp64(reps=5):
     """The plain-method configuration at full size: the 2048-triangle square pyramid (base 20 nm, height
     10 nm) on the 256^2 q-space grid qy [-2, 2], qz [0, 2] with its four DWBA branches (10 keV,
@@ -268,6 +292,8 @@ def bench_pyramid_fp64(reps=5):
     ka_ms = k_approx["main_ms"] / max(k_approx["main_launches"], 1)
     ke_ms = k_exact["main_ms"] / max(k_exact["main_launches"], 1)
     ke32_ms = k_exact32["main_ms"] / max(k_exact32["main_launches"], 1)
+    fpp32, nt_mix = exact32_flop_per_pair(g, tri)
+    fpp32 = float("nan") if fpp32 is None else fpp32
     # FP64 roofline of the form-factor kernels: algorithmic flop per pair (approximate on this product
     # grid: 16, two complex multiply-adds on the FP64 matrix cores; exact about 200 on the VALU;
     # transcendentals not counted) over the kernel's own time, against 78.6 TFLOP/s (the FP64 peak of
@@ -282,15 +308,13 @@ def bench_pyramid_fp64(reps=5):
             "roofline_approx_kernel": dict(roof(16, ka_ms, "fp64-mfma"), kernel="k_ff_product64"),
             "exact_method": {"gpu_ms": 1e3 * t_exact, "gpu_pairs_per_s": pairs / t_exact,
                              "relL2_vs_oracle_sample": rel_exact, "roofline": roof(200, ke_ms, "fp64-valu"),
-                             # FP32 series with 16-18 terms: about 110 flop per pair (phase, q.sn, two
-                             # q.d, p and r, the recursion, the complex combine; sin/cos not counted)
                              "fp32c": {"gpu_ms": 1e3 * t_exact32, "gpu_pairs_per_s": pairs / t_exact32,
                                        "speedup_vs_fp64": t_exact / t_exact32, "relL2_vs_fp64": rel_e32,
-                                       "roofline": dict(roof(110, ke32_ms, "fp32-valu"),
+                                       "roofline": dict(roof(fpp32, ke32_ms, "fp32-valu"),
                                                         peak=PEAK_FP32 / 1e12,
-                                                        frac=110 * pairs / (ke32_ms * 1e-3) / PEAK_FP32,
-                                                        achieved=110 * pairs / (ke32_ms * 1e-3) / 1e12,
-                                                        kernel="k_ff_exact32")}},
+                                                        frac=fpp32 * pairs / (ke32_ms * 1e-3) / PEAK_FP32,
+                                                        achieved=fpp32 * pairs / (ke32_ms * 1e-3) / 1e12,
+                                                        kernel="k_ff_exact32", term_mix=nt_mix)}},
             "fp32c": {"relL2_intensity_vs_fp64": rel32, "pixel_error_vs_fp64": pix.result()},
             "workload": "pyramid 2048 triangles, 256^2 DWBA (4 branches), FP64 approximate FF + DWBA; "
                         "exact-triangle FF of the same mesh and grid"}
