@@ -171,6 +171,8 @@ _SIGS = {
     "hgx_dwba_intensity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp]),
     "hgx_lattice_intensity": (C.c_int, [_vp, C.POINTER(LatticeModel), C.c_int, _vp, C.c_int, _vp]),
     "hgx_ensemble_intensity": (C.c_int, [_vp, C.POINTER(EnsembleModel), C.c_int, _vp, C.c_int, _vp]),
+    "hgx_ensemble_intensity_tangent": (C.c_int, [_vp, C.POINTER(EnsembleModel), C.c_int, C.POINTER(EnsembleModel),
+                                                 _vp, _vp, _vp]),
     "hgx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(BatchItem), C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "hgx_batch_run": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_batch_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
@@ -196,6 +198,9 @@ _SIGS = {
     "hgx_axpy": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_int, _vp]),
     "hgx_gaussian_smear": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),
     "hgx_image_distance": (C.c_int, [_vp, C.c_int64, C.c_int, _vp, _vp, _vp, C.POINTER(C.c_double)]),
+    "hgx_image_distance_async": (C.c_int, [_vp, C.c_int64, C.c_int, _vp, _vp, _vp, _vp]),
+    "hgx_image_distance_tangent": (C.c_int, [_vp, C.c_int64, C.c_int, _vp, _vp, _vp, C.c_int, _vp,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hgx_grating_plan_create": (C.c_int, [_vp, C.POINTER(GratingModel), C.c_int, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     "hgx_grating_plan_release": (C.c_int, [_vp, C.c_int]),
     "hgx_grating_plan_metric": (C.c_int, [_vp, C.c_int, C.c_int]),

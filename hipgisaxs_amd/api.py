@@ -278,13 +278,9 @@ def lattice_intensity(ctx: Context, shape: str, dims, la, lb, lc, repet, rots, w
     return o
 
 
-def ensemble_intensity(ctx: Context, elements, la, lb, lc, repet, rots, centers=None, weights=None, dw_sigma=0.0,
-                       sf_flags=0, precision="fp64", fc=None, accumulate=False, out=None):
-    """hgx_ensemble_intensity: a lattice of unit cells of analytic elements under several grains.
-    elements: dicts with shape, dims [(values, weights), ...], rot (3x3, default identity), locations
-    [[x, y, z], ...] and dn2 (complex)."""
-    keep = []
+def _ensemble_model(elements, la, lb, lc, repet, rots, centers, weights, dw_sigma, sf_flags, keep):
     el = (N.EnsembleElement * len(elements))()
+    keep.append(el)
     for e, d in zip(el, elements):
         e.shape = N.SHAPES[d["shape"]]
         _fill_shape_params(e.params, d["dims"], keep)
@@ -311,11 +307,71 @@ def ensemble_intensity(ctx: Context, elements, la, lb, lc, repet, rots, centers=
             A = _f64(arr).ravel()
             keep.append(A)
             setattr(m, name, A.ctypes.data_as(C.POINTER(C.c_double)))
+    return m
+
+
+def ensemble_intensity(ctx: Context, elements, la, lb, lc, repet, rots, centers=None, weights=None, dw_sigma=0.0,
+                       sf_flags=0, precision="fp64", fc=None, accumulate=False, out=None):
+    """hgx_ensemble_intensity: a lattice of unit cells of analytic elements under several grains.
+    elements: dicts with shape, dims [(values, weights), ...], rot (3x3, default identity), locations
+    [[x, y, z], ...] and dn2 (complex)."""
+    keep = []
+    m = _ensemble_model(elements, la, lb, lc, repet, rots, centers, weights, dw_sigma, sf_flags, keep)
     o = ctx._out(ctx.info()["npix"], out)
     p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
     ctx._check(ctx.lib.hgx_ensemble_intensity(ctx.h, C.byref(m), p, None if fc is None else C.c_void_p(fc.data_ptr()),
                                               int(accumulate), C.c_void_p(o.data_ptr())), "ensemble_intensity")
     return o
+
+
+def ensemble_intensity_tangent(ctx: Context, model: dict, tangents, fc=None):
+    """hgx_ensemble_intensity_tangent: (I, dI [ndir][npix]) for a model given as the keyword arguments of
+    ensemble_intensity (elements, la, lb, lc, repet, rots, centers, weights, dw_sigma, sf_flags) and a
+    list of tangent dicts of the same form holding the derivatives of those fields (missing fields:
+    zero; repet and shapes are taken from the model)."""
+    keep = []
+    base = dict(centers=None, weights=None, dw_sigma=0.0, sf_flags=0)
+    base.update(model)
+    m = _ensemble_model(base["elements"], base["la"], base["lb"], base["lc"], base["repet"], base["rots"],
+                        base["centers"], base["weights"], base["dw_sigma"], base["sf_flags"], keep)
+    ng = m.ngrains
+    T = (N.EnsembleModel * len(tangents))()
+    for k, t in enumerate(tangents):
+        els = []
+        for e, te in zip(base["elements"], t.get("elements", [{}] * len(base["elements"]))):
+            dims = te.get("dims", [(np.zeros(len(v)), np.zeros(len(w))) for v, w in e["dims"]])
+            els.append({"shape": e["shape"], "dims": dims, "tau": te.get("tau", 0.0), "eta": te.get("eta", 0.0),
+                        "rot": te.get("rot", np.zeros((3, 3))),
+                        "locations": te.get("locations", np.zeros_like(np.asarray(e.get("locations", [[0, 0, 0]]),
+                                                                                  dtype=float))),
+                        "dn2": te.get("dn2", 0.0)})
+        T[k] = _ensemble_model(els, t.get("la", (0, 0, 0)), t.get("lb", (0, 0, 0)), t.get("lc", (0, 0, 0)),
+                               base["repet"], t.get("rots", np.zeros((ng, 3, 3))), t.get("centers"),
+                               t.get("weights"), t.get("dw_sigma", 0.0), base["sf_flags"], keep)
+    npix = ctx.info()["npix"]
+    I = ctx._out(npix, None)
+    import torch
+
+    dI = torch.empty(len(tangents) * npix, dtype=torch.float64, device=I.device)
+    ctx._check(ctx.lib.hgx_ensemble_intensity_tangent(ctx.h, C.byref(m), len(tangents), T,
+                                                      None if fc is None else C.c_void_p(fc.data_ptr()),
+                                                      C.c_void_p(I.data_ptr()), C.c_void_p(dI.data_ptr())),
+               "ensemble_intensity_tangent")
+    return I, dI.view(len(tangents), npix)
+
+
+def image_distance_tangent(ctx: Context, I, O, mask, dI, metric):
+    """hgx_image_distance_tangent: (D, dD [ndir]) for device images I, O, optional mask and tangents dI
+    [ndir][n]."""
+    n = I.numel()
+    ndir = dI.numel() // n
+    D = C.c_double()
+    dD = (C.c_double * ndir)()
+    m = N.METRICS[metric] if isinstance(metric, str) else int(metric)
+    ctx._check(ctx.lib.hgx_image_distance_tangent(ctx.h, n, m, C.c_void_p(I.data_ptr()), C.c_void_p(O.data_ptr()),
+                                                  None if mask is None else C.c_void_p(mask.data_ptr()), ndir,
+                                                  C.c_void_p(dI.data_ptr()), C.byref(D), dD), "image_distance_tangent")
+    return D.value, np.array(list(dD))
 
 
 def gaussian_smear(ctx: Context, image, ncol: int, nrow: int, sigma: float, compat: bool = False):

@@ -18,6 +18,7 @@
 // each (x, y) writes its partial grain sum, and a second kernel adds the y partials in a fixed order
 // (deterministic). The model (nodes, matrices, locations, grain data) sits in the context workspace.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -215,6 +216,97 @@ __global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const doub
     acc += amp.r * amp.r + amp.i * amp.i;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
+// ----- forward-mode tangents (FP64) -------------------------------------------------------------
+
+// One element with dual nodes, contrast and its location range; grains with dual rotation, centre
+// and weight. Every table of the model enters as value + tangent along one direction.
+struct EnsElemD {
+  NodesT<dd> P;
+  int shape;
+  int loc0, nloc;
+  dd dn2r, dn2i;
+};
+struct EnsGrainD {
+  dd rot[9];
+  dd center[3];
+  dd w;
+};
+
+__device__ inline dz ff_any_d(int shape, const dz mq[3], const NodesT<dd>& P) {
+  switch (shape) {
+    case HGX_SHAPE_SPHERE: return shape_ff<HGX_SHAPE_SPHERE>(mq, P);
+    case HGX_SHAPE_CYLINDER: return shape_ff<HGX_SHAPE_CYLINDER>(mq, P);
+    case HGX_SHAPE_BOX: return shape_ff<HGX_SHAPE_BOX>(mq, P);
+    case HGX_SHAPE_CUBE: return shape_ff<HGX_SHAPE_CUBE>(mq, P);
+    case HGX_SHAPE_PRISM3: return shape_ff<HGX_SHAPE_PRISM3>(mq, P);
+    case HGX_SHAPE_PRISM6: return shape_ff<HGX_SHAPE_PRISM6>(mq, P);
+    default: return shape_ff<HGX_SHAPE_PYRAMID>(mq, P);
+  }
+}
+
+// mq = M q for a dual matrix and a fixed q-point
+__device__ inline void rotate_md(const dd* __restrict__ m, double qx, double qy, cd qz, dz mq[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const dd a = m[3 * j], b = m[3 * j + 1], c = m[3 * j + 2];
+    mq[j].v = cd{a.v * qx + b.v * qy + c.v * qz.r, c.v * qz.i};
+    mq[j].d = cd{a.d * qx + b.d * qy + c.d * qz.r, c.d * qz.i};
+  }
+}
+
+// I[p] and dI[j][p] = d/dt I[p](tables + t tangents_j) for direction j = blockIdx.y: the grain, element,
+// location, structure-factor and branch sums of k_ensemble64 in dual arithmetic, every grain in one
+// thread, dI = sum_g 2 Re(conj(A_g) dA_g) with A_g = w_g sum_b fc_b S_g F_g.
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_tangent(QViewE g, int nb, const double* __restrict__ fc,
+                                                          const EnsElemD* __restrict__ E, int nelem,
+                                                          const dd* __restrict__ loc, int nloc_all,
+                                                          const dd* __restrict__ M, const EnsGrainD* __restrict__ G,
+                                                          int ngrains, const SFArgsT<dd>* __restrict__ SF,
+                                                          double* __restrict__ I, double* __restrict__ dI) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int j = blockIdx.y;
+  const EnsElemD* Ej = E + (int64_t)j * nelem;
+  const dd* locj = loc + (int64_t)j * 3 * nloc_all;
+  const dd* Mj = M + (int64_t)j * 9 * ngrains * nelem;
+  const EnsGrainD* Gj = G + (int64_t)j * ngrains;
+  const double qx = g.qx[p], qy = g.qy[p];
+  double acc = 0.0, dacc = 0.0;
+  for (int o = 0; o < ngrains; ++o) {
+    SFArgsT<dd> sf = SF[j];
+    sf.center[0] = Gj[o].center[0];
+    sf.center[1] = Gj[o].center[1];
+    sf.center[2] = Gj[o].center[2];
+    dz amp = cconst<dz>(0.0);
+    for (int b = 0; b < nb; ++b) {
+      const int64_t q = p + b * g.npix;
+      const cd qz = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+      const cd fcb = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+      dz mg[3];
+      rotate_md(Gj[o].rot, qx, qy, qz, mg);
+      const dz S = sf_value(mg, sf);
+      dz ff = cconst<dz>(0.0);
+      for (int e = 0; e < nelem; ++e) {
+        dz mq[3];
+        rotate_md(Mj + 9 * ((int64_t)o * nelem + e), qx, qy, qz, mq);
+        const dz F = SHAPE >= 0 ? shape_ff<(SHAPE >= 0 ? SHAPE : 0)>(mq, Ej[e].P) : ff_any_d(Ej[e].shape, mq, Ej[e].P);
+        const dz dn2 = mkc(Ej[e].dn2r, Ej[e].dn2i);
+        for (int l = Ej[e].loc0; l < Ej[e].loc0 + Ej[e].nloc; ++l) {
+          const dd* t = locj + 3 * l;
+          ff = ff + dn2 * (F * cexpi_d(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
+        }
+      }
+      amp = amp + fcb * (S * ff);
+    }
+    amp = Gj[o].w * amp;
+    acc += norm2(amp.v);
+    dacc += 2.0 * (amp.v.r * amp.d.r + amp.v.i * amp.d.i);
+  }
+  if (j == 0) I[p] = acc;
+  dI[(int64_t)j * g.npix + p] = dacc;
 }
 
 __global__ __launch_bounds__(256) void k_ensemble_sum(int64_t npix, int ny, const double* __restrict__ part,
@@ -417,7 +509,158 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   return HGX_OK;
 }
 
+// The tangent pass: the value tables as ensemble_run builds them, each direction's tangents beside
+// them (the same counts), dual tables staged in one go, one launch over (pixels, directions).
+static int ensemble_tangent_run(Context* c, const hgx_ensemble_model* m, int ndir, const hgx_ensemble_model* t,
+                                const double* fc, double* I, double* dI) {
+  if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ensemble_tangent: no q-grid");
+  if (!m || !t || !I || !dI || ndir < 1 || ndir > 64 || m->nelem < 1 || m->nelem > 64 || !m->elem ||
+      m->ngrains < 1 || !m->rot)
+    return set_error(c, HGX_ERR_INVALID_ARG,
+                     "ensemble_tangent: need a model (1..64 elements, >= 1 grain), 1..64 tangents and outputs");
+  const int ne = m->nelem, ng = m->ngrains;
+  if ((int64_t)ng * ne * ndir > ((int64_t)1 << 22))
+    return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: grains x elements x directions above 2^22");
+  int nloc_all = 0;
+  for (int e = 0; e < ne; ++e) {
+    const hgx_ensemble_element& el = m->elem[e];
+    if (el.shape < 0 || el.shape > HGX_SHAPE_PYRAMID || el.nloc < 1 || !el.loc)
+      return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: element %d needs a shape and locations", e);
+    for (int d = 0; d < kEnsDims[el.shape]; ++d)
+      if (el.params.n[d] < 1 || el.params.n[d] > HGX_MAX_NODES || !el.params.val[d] || !el.params.wt[d])
+        return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: element %d dimension %d needs 1..%d nodes", e, d,
+                         HGX_MAX_NODES);
+    nloc_all += el.nloc;
+  }
+  // every tangent model has the value model's counts; node and location arrays may be NULL (zero)
+  for (int j = 0; j < ndir; ++j) {
+    const hgx_ensemble_model& tj = t[j];
+    if (tj.nelem != ne || tj.ngrains != ng || !tj.elem)
+      return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: tangent %d has other counts than the model", j);
+    for (int e = 0; e < ne; ++e) {
+      if (tj.elem[e].shape != m->elem[e].shape || tj.elem[e].nloc != m->elem[e].nloc)
+        return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: tangent %d element %d differs in shape or locations",
+                         j, e);
+      for (int d = 0; d < kEnsDims[m->elem[e].shape]; ++d)
+        if ((tj.elem[e].params.val[d] || tj.elem[e].params.wt[d]) && tj.elem[e].params.n[d] != m->elem[e].params.n[d])
+          return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: tangent %d element %d dimension %d node count", j,
+                           e, d);
+    }
+  }
+  std::vector<EnsElemD> E((size_t)ndir * ne);
+  std::vector<dd> loc((size_t)ndir * 3 * nloc_all), M((size_t)ndir * 9 * ng * ne);
+  std::vector<EnsGrainD> G((size_t)ndir * ng);
+  std::vector<SFArgsT<dd>> SF(ndir);
+  int common = m->elem[0].shape;
+  for (int e = 1; e < ne; ++e)
+    if (m->elem[e].shape != common) common = -1;
+  for (int j = 0; j < ndir; ++j) {
+    const hgx_ensemble_model& tj = t[j];
+    int l0 = 0;
+    for (int e = 0; e < ne; ++e) {
+      const hgx_ensemble_element &el = m->elem[e], &te = tj.elem[e];
+      EnsElemD& D = E[(size_t)j * ne + e];
+      std::memset(&D, 0, sizeof(D));
+      for (int d = 0; d < 4; ++d) {
+        if (d >= kEnsDims[el.shape]) continue;
+        D.P.n[d] = el.params.n[d];
+        for (int i = 0; i < el.params.n[d]; ++i) {
+          D.P.val[d][i] = dd{el.params.val[d][i], te.params.val[d] ? te.params.val[d][i] : 0.0};
+          D.P.wt[d][i] = dd{el.params.wt[d][i], te.params.wt[d] ? te.params.wt[d][i] : 0.0};
+        }
+      }
+      const double tt = std::tan(el.params.tau);
+      D.P.tan_tau = dd{tt, (1.0 + tt * tt) * te.params.tau};
+      D.P.sin_eta = dd{std::sin(el.params.eta), std::cos(el.params.eta) * te.params.eta};
+      D.P.cos_eta = dd{std::cos(el.params.eta), -std::sin(el.params.eta) * te.params.eta};
+      D.P.flags = el.params.flags;
+      D.shape = el.shape;
+      D.loc0 = l0;
+      D.nloc = el.nloc;
+      D.dn2r = dd{el.dn2_re, te.dn2_re};
+      D.dn2i = dd{el.dn2_im, te.dn2_im};
+      for (int k = 0; k < 3 * el.nloc; ++k)
+        loc[(size_t)j * 3 * nloc_all + 3 * (size_t)l0 + k] = dd{el.loc[k], te.loc ? te.loc[k] : 0.0};
+      l0 += el.nloc;
+    }
+    for (int o = 0; o < ng; ++o) {
+      const double* Rg = m->rot + 9 * (size_t)o;
+      const double* dRg = tj.rot ? tj.rot + 9 * (size_t)o : nullptr;
+      EnsGrainD& Gd = G[(size_t)j * ng + o];
+      for (int k = 0; k < 9; ++k) Gd.rot[k] = dd{Rg[k], dRg ? dRg[k] : 0.0};
+      for (int k = 0; k < 3; ++k)
+        Gd.center[k] = dd{m->center ? m->center[3 * (size_t)o + k] : 0.0, tj.center ? tj.center[3 * (size_t)o + k] : 0.0};
+      Gd.w = dd{m->weight ? m->weight[o] : 1.0, tj.weight ? tj.weight[o] : 0.0};
+      for (int e = 0; e < ne; ++e) {
+        // M = R_g R_e (the order of ensemble_run), dM = dR_g R_e + R_g dR_e
+        const double* Re = m->elem[e].rot;
+        const double* dRe = tj.elem[e].rot;
+        dd* out = &M[(size_t)j * 9 * ng * ne + 9 * ((size_t)o * ne + e)];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            double v = 0, dv = 0;
+            for (int k = 0; k < 3; ++k) {
+              v += Rg[3 * a + k] * Re[3 * k + b];
+              dv += (dRg ? dRg[3 * a + k] : 0.0) * Re[3 * k + b] + Rg[3 * a + k] * dRe[3 * k + b];
+            }
+            out[3 * a + b] = dd{v, dv};
+          }
+      }
+    }
+    SFArgsT<dd>& sf = SF[j];
+    for (int k = 0; k < 3; ++k) {
+      sf.la[k] = dd{m->la[k], tj.la[k]};
+      sf.lb[k] = dd{m->lb[k], tj.lb[k]};
+      sf.lc[k] = dd{m->lc[k], tj.lc[k]};
+      sf.N[k] = m->repet[k] < 1 ? 1 : m->repet[k];
+      sf.center[k] = dd{0.0, 0.0};
+    }
+    sf.dw = dd{0.5 * m->dw_sigma * m->dw_sigma, m->dw_sigma * tj.dw_sigma};
+    sf.compat_lb2 = (m->sf_flags & HGX_SF_COMPAT_LB2) != 0;
+  }
+  if (int rc = set_device(c)) return rc;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_E = al(E.size() * sizeof(EnsElemD)), b_loc = al(loc.size() * sizeof(dd)),
+               b_M = al(M.size() * sizeof(dd)), b_G = al(G.size() * sizeof(EnsGrainD)),
+               b_SF = al(SF.size() * sizeof(SFArgsT<dd>));
+  if (int rc = ensure_work(c, b_E + b_loc + b_M + b_G + b_SF)) return rc;
+  char* w = static_cast<char*>(c->d_work);
+  EnsElemD* dE = reinterpret_cast<EnsElemD*>(w);
+  dd* dloc = reinterpret_cast<dd*>(w + b_E);
+  dd* dM = reinterpret_cast<dd*>(w + b_E + b_loc);
+  EnsGrainD* dG = reinterpret_cast<EnsGrainD*>(w + b_E + b_loc + b_M);
+  SFArgsT<dd>* dSF = reinterpret_cast<SFArgsT<dd>*>(w + b_E + b_loc + b_M + b_G);
+  {
+    const void* src[5] = {E.data(), loc.data(), M.data(), G.data(), SF.data()};
+    const size_t nb[5] = {E.size() * sizeof(EnsElemD), loc.size() * sizeof(dd), M.size() * sizeof(dd),
+                          G.size() * sizeof(EnsGrainD), SF.size() * sizeof(SFArgsT<dd>)};
+    if (int rc = stage_to_device(c, 5, src, nb, w)) return rc;
+  }
+  QViewE g{c->d_qx, c->d_qy, c->d_qz, c->npix};
+  LaunchTimer lt(c, true);
+  const dim3 grid(ceil_div(c->npix, 256), ndir);
+#define HGX_TAN_LAUNCH(S) \
+  k_ensemble_tangent<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, nloc_all, dM, dG, ng, dSF, I, dI)
+  switch (common) {
+    case HGX_SHAPE_SPHERE: HGX_TAN_LAUNCH(HGX_SHAPE_SPHERE); break;
+    case HGX_SHAPE_CYLINDER: HGX_TAN_LAUNCH(HGX_SHAPE_CYLINDER); break;
+    case HGX_SHAPE_BOX: HGX_TAN_LAUNCH(HGX_SHAPE_BOX); break;
+    default: HGX_TAN_LAUNCH(-1); break;
+  }
+#undef HGX_TAN_LAUNCH
+  HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}
+
 }  // namespace hgx
+
+extern "C" int hgx_ensemble_intensity_tangent(hgx_ctx* h, const hgx_ensemble_model* model, int ndir,
+                                              const hgx_ensemble_model* tangents, const double* fc, double* I,
+                                              double* dI) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  return ensemble_tangent_run(c, model, ndir, tangents, fc, I, dI);
+}
 
 extern "C" int hgx_ensemble_intensity(hgx_ctx* h, const hgx_ensemble_model* m, int precision, const double* fc,
                                       int accumulate, double* I) {

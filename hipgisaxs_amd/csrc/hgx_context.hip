@@ -45,6 +45,23 @@ int ensure_work(Context* c, size_t bytes) {
   return HGX_OK;
 }
 
+int ensure_result(Context* c, int64_t n) {
+  if (n <= c->result_cap) return HGX_OK;
+  if (c->d_result) {
+    HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    HGX_CHECK_HIP(c, hipFree(c->d_result));
+    c->d_result = nullptr;
+    c->result_cap = 0;
+  }
+  const int64_t cap = std::max<int64_t>(n, 4096);
+  if (hipMalloc(&c->d_result, (size_t)cap * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(c, HGX_ERR_OOM, "result buffer allocation failed");
+  }
+  c->result_cap = cap;
+  return HGX_OK;
+}
+
 int stage_to_device(Context* c, int n, const void* const* src, const size_t* bytes, char* dst) {
   size_t total = 0;
   for (int i = 0; i < n; ++i) total += (bytes[i] + 255) / 256 * 256;
@@ -333,6 +350,7 @@ int hgx_destroy(hgx_ctx* h) {
       (void)hipEventDestroy(e.second);
     }
   if (c->d_work) (void)hipFree(c->d_work);
+  if (c->d_result) (void)hipFree(c->d_result);
   if (c->d_flag) (void)hipFree(c->d_flag);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);

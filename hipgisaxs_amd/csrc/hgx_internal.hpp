@@ -225,6 +225,8 @@ struct Context {
   // scratch for partial sums
   void* d_work = nullptr;
   size_t work_bytes = 0;
+  double* d_result = nullptr;  // small results copied back to the host (distances and their tangents)
+  int64_t result_cap = 0;
   // device address of the device's static scratch array (kScratchBytes), shared by every context on
   // the device; null when it could not be resolved
   char* d_scratch = nullptr;
@@ -264,6 +266,8 @@ int set_error(Context* c, int code, const char* fmt, ...);
 #define HGX_CHECK_LAUNCH(ctx) HGX_CHECK_HIP(ctx, hipGetLastError())
 
 int ensure_work(Context* c, size_t bytes);
+// d_result with room for n doubles (grows; synchronises the stream before it replaces the buffer).
+int ensure_result(Context* c, int64_t n);
 // Copies n host tables (sizes in bytes) into device memory dst, each at a 256-byte aligned offset,
 // through the context's pinned staging buffer and one asynchronous copy on the context's stream; the
 // caller's tables may be released on return (no stream synchronisation).

@@ -230,6 +230,110 @@ void launch_terms(hipStream_t s, unsigned nb, int64_t n, const double* I, const 
   k_distance_terms<METRIC><<<nb, kDistWG, 0, s>>>(n, I, O, m, cst, part);
 }
 
+// ----- derivative of the distance along tangent images -----
+//
+// With u = I^(1/R), r = O^(1/R) (R = 2 or 3), the stats' sums v0 = sum m u^2, v1 = sum m |u r| and
+// ns = sqrt v0, nr = sqrt(sum m r^2):
+//   unit forms: D = sum m^2 e^2, e = r/nr - u/ns, and
+//               dD/du_p = -2 m^2 e/ns + 2 E m u/ns^3 with E = sum m^2 e u;
+//   c forms:    D = sum m^2 (r - c u)^2, c = v1/v0, and
+//               dD/du_p = -2 c m^2 (r - c u) - 2 F m (|r| - 2 c u)/v0 with F = sum m^2 (r - c u) u;
+// then dD/dI = dD/du u/(R I) (zero where I <= 0 or the pixel's term is dropped); chi^2 and L2 as in
+// metric_seed.
+template <int METRIC>
+__device__ inline void unit_or_c(double a, double b, double w, const double* cst, double& u, double& r, double& t,
+                                 bool& keep) {
+  constexpr int R = (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_SQRT_C_L2) ? 2 : 3;
+  u = root_of<R>(a);
+  r = root_of<R>(b);
+  if (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_CBRT_UNIT_L2)
+    t = r / cst[1] - u / cst[0];
+  else
+    t = r - cst[2] * u;
+  keep = R == 3 || isfinite(w * fabs(t));
+}
+
+// Pass 3: E (unit forms) or F (c forms) per block.
+template <int METRIC>
+__global__ __launch_bounds__(kDistWG) void k_distance_aux(int64_t n, const double* __restrict__ I,
+                                                          const double* __restrict__ O, const double* __restrict__ m,
+                                                          const double* __restrict__ cst, double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (int64_t p = (int64_t)blockIdx.x * kDistWG + threadIdx.x; p < n; p += (int64_t)gridDim.x * kDistWG) {
+    const double w = m ? m[p] : 1.0;
+    double u, r, t;
+    bool keep;
+    unit_or_c<METRIC>(I[p], O[p], w, cst, u, r, t, keep);
+    if (keep) v[0] += w * w * t * u;
+  }
+  block_store<1>(v, part);
+}
+
+// Pass 4: the seed g_p = dD/dI_p.
+template <int METRIC>
+__global__ __launch_bounds__(kDistWG) void k_distance_seed(int64_t n, const double* __restrict__ I,
+                                                           const double* __restrict__ O, const double* __restrict__ m,
+                                                           const double* __restrict__ cst, double* __restrict__ seed) {
+  const int64_t p = (int64_t)blockIdx.x * kDistWG + threadIdx.x;
+  if (p >= n) return;
+  const double w = m ? m[p] : 1.0;
+  const double a = I[p], b = O[p];
+  double g = 0.0;
+  if (METRIC == HGX_METRIC_CHI2) {
+    g = 2.0 * w * (a - b) / fmax(b, 1.0);
+  } else if (METRIC == HGX_METRIC_L2) {
+    g = 2.0 * w * w * (a - b);
+  } else {
+    constexpr int R = (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_SQRT_C_L2) ? 2 : 3;
+    double u, r, t;
+    bool keep;
+    unit_or_c<METRIC>(a, b, w, cst, u, r, t, keep);
+    const double ns = cst[0];
+    if (keep && a > 0.0 && ns > 0.0) {
+      double du;
+      if (METRIC == HGX_METRIC_SQRT_UNIT_L2 || METRIC == HGX_METRIC_CBRT_UNIT_L2)
+        du = -2.0 * w * w * t / ns + 2.0 * cst[4] * w * u / (ns * ns * ns);
+      else
+        du = -2.0 * cst[2] * w * w * t - 2.0 * cst[4] * w * (fabs(r) - 2.0 * cst[2] * u) / (ns * ns);
+      g = du * u / (R * a);
+    }
+  }
+  seed[p] = g;
+}
+
+// Per block and direction j = blockIdx.y: sum_p seed_p dI[j][p].
+__global__ __launch_bounds__(kDistWG) void k_seed_dots(int64_t n, const double* __restrict__ seed,
+                                                       const double* __restrict__ dI, double* __restrict__ part) {
+  const double* x = dI + (int64_t)blockIdx.y * n;
+  double v[1] = {0.0};
+  for (int64_t p = (int64_t)blockIdx.x * kDistWG + threadIdx.x; p < n; p += (int64_t)gridDim.x * kDistWG)
+    v[0] += seed[p] * x[p];
+  __shared__ double red[kDistWG];
+  red[threadIdx.x] = v[0];
+  __syncthreads();
+  for (int w = kDistWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[j] = sum_b part[j][b], one workgroup per direction, fixed order.
+__global__ __launch_bounds__(kDistWG) void k_dots_finish(int nb, const double* __restrict__ part,
+                                                         double* __restrict__ out) {
+  const double* x = part + (int64_t)blockIdx.x * nb;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kDistWG) v += x[b];
+  __shared__ double red[kDistWG];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kDistWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 }  // namespace hgx
 
 extern "C" int hgx_gaussian_smear(hgx_ctx* h, int ncol, int nrow, double sigma, int flags, double* I) {
@@ -251,10 +355,15 @@ extern "C" int hgx_gaussian_smear(hgx_ctx* h, int ncol, int nrow, double sigma, 
   }
   const int64_t npix = (int64_t)ncol * nrow;
   size_t tmp_b = ((npix * sizeof(double)) + 255) / 256 * 256;
-  if (int rc = ensure_work(c, tmp_b + gn * sizeof(double))) return rc;
+  if (int rc = ensure_work(c, tmp_b + (gn * sizeof(double) + 255) / 256 * 256)) return rc;
   double* tmp = static_cast<double*>(c->d_work);
   double* dg = reinterpret_cast<double*>(static_cast<char*>(c->d_work) + tmp_b);
-  HGX_CHECK_HIP(c, hipMemcpyAsync(dg, g.data(), gn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  {
+    // through the pinned staging buffer: the host table may go out of scope at once
+    const void* src[1] = {g.data()};
+    const size_t nbytes[1] = {gn * sizeof(double)};
+    if (int rc = stage_to_device(c, 1, src, nbytes, reinterpret_cast<char*>(dg))) return rc;
+  }
   const int compat = (flags & HGX_SMEAR_COMPAT) ? 1 : 0;
   {
     LaunchTimer lt(c, false);
@@ -268,27 +377,36 @@ extern "C" int hgx_gaussian_smear(hgx_ctx* h, int ncol, int nrow, double sigma, 
     k_smear_pass<<<ceil_div(npix, 256), 256, 0, c->stream>>>(nrow, ncol, ncol, 1, true, hw, dg, compat, tmp, I);
     HGX_CHECK_LAUNCH(c);
   }
-  // the host table must outlive the asynchronous copy
-  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return HGX_OK;
 }
 
-extern "C" int hgx_image_distance(hgx_ctx* h, int64_t n, int metric, const double* I, const double* O,
-                                  const double* mask, double* distance) {
-  Context* c = reinterpret_cast<Context*>(h);
-  if (!c) return HGX_ERR_BAD_HANDLE;
-  if (n < 1 || !I || !O || !distance) return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: bad arguments");
+namespace hgx {
+
+// Enqueues the distance passes; the value lands in device memory at *d_out. Workspace layout:
+// [block partials][constants: ns, nr, c, D, E|F][seed n][direction partials].
+static int distance_enqueue(Context* c, int64_t n, int metric, const double* I, const double* O, const double* mask,
+                            double* d_out, int ndir, const double* dI, double* d_dout) {
+  if (n < 1 || !I || !O) return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: bad arguments");
   if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2)
     return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: unknown metric %d", metric);
+  if (ndir < 0 || (ndir > 0 && (!dI || !d_dout)) || ndir > 4096)
+    return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: bad tangent arguments");
   if (int rc = set_device(c)) return rc;
   // a fixed grid per size keeps the summation order, and so the value, reproducible
   const unsigned nb = (unsigned)std::min<int64_t>((n + kDistWG - 1) / kDistWG, 1024);
-  const size_t part_b = ((size_t)nb * 3 * sizeof(double) + 255) / 256 * 256;
-  if (int rc = ensure_work(c, part_b + 256)) return rc;
-  double* part = static_cast<double*>(c->d_work);
-  double* cst = reinterpret_cast<double*>(static_cast<char*>(c->d_work) + part_b);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t part_b = al((size_t)nb * 3 * sizeof(double)), cst_b = 256;
+  const size_t seed_b = ndir > 0 ? al((size_t)n * sizeof(double)) : 0;
+  const size_t dpart_b = ndir > 0 ? al((size_t)ndir * nb * sizeof(double)) : 0;
+  if (int rc = ensure_work(c, part_b + cst_b + seed_b + dpart_b)) return rc;
+  char* w = static_cast<char*>(c->d_work);
+  double* part = reinterpret_cast<double*>(w);
+  double* cst = reinterpret_cast<double*>(w + part_b);
+  double* seed = reinterpret_cast<double*>(w + part_b + cst_b);
+  double* dpart = reinterpret_cast<double*>(w + part_b + cst_b + seed_b);
   hipStream_t s = c->stream;
-  if (metric >= HGX_METRIC_SQRT_UNIT_L2) {
+  const bool norm = metric >= HGX_METRIC_SQRT_UNIT_L2;
+  if (norm) {
     const bool cube = metric == HGX_METRIC_CBRT_UNIT_L2 || metric == HGX_METRIC_CBRT_C_L2;
     {
       LaunchTimer lt(c, false);
@@ -313,7 +431,72 @@ extern "C" int hgx_image_distance(hgx_ctx* h, int64_t n, int metric, const doubl
   }
   k_distance_finish<1><<<1, kDistWG, 0, s>>>((int)nb, false, part, cst + 3);
   HGX_CHECK_LAUNCH(c);
-  HGX_CHECK_HIP(c, hipMemcpyAsync(distance, cst + 3, sizeof(double), hipMemcpyDeviceToHost, s));
-  HGX_CHECK_HIP(c, hipStreamSynchronize(s));
+  HGX_CHECK_HIP(c, hipMemcpyAsync(d_out, cst + 3, sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (ndir == 0) return HGX_OK;
+  LaunchTimer lt(c, false);
+#define HGX_SEED(M)                                                                                \
+  do {                                                                                             \
+    if (M >= HGX_METRIC_SQRT_UNIT_L2) {                                                            \
+      k_distance_aux<M><<<nb, kDistWG, 0, s>>>(n, I, O, mask, cst, part);                          \
+      k_distance_finish<1><<<1, kDistWG, 0, s>>>((int)nb, false, part, cst + 4);                   \
+    }                                                                                              \
+    k_distance_seed<M><<<ceil_div(n, kDistWG), kDistWG, 0, s>>>(n, I, O, mask, cst, seed);         \
+  } while (0)
+  switch (metric) {
+    case HGX_METRIC_CHI2: HGX_SEED(HGX_METRIC_CHI2); break;
+    case HGX_METRIC_L2: HGX_SEED(HGX_METRIC_L2); break;
+    case HGX_METRIC_SQRT_UNIT_L2: HGX_SEED(HGX_METRIC_SQRT_UNIT_L2); break;
+    case HGX_METRIC_SQRT_C_L2: HGX_SEED(HGX_METRIC_SQRT_C_L2); break;
+    case HGX_METRIC_CBRT_UNIT_L2: HGX_SEED(HGX_METRIC_CBRT_UNIT_L2); break;
+    default: HGX_SEED(HGX_METRIC_CBRT_C_L2); break;
+  }
+#undef HGX_SEED
+  HGX_CHECK_LAUNCH(c);
+  k_seed_dots<<<dim3(nb, ndir), kDistWG, 0, s>>>(n, seed, dI, dpart);
+  HGX_CHECK_LAUNCH(c);
+  k_dots_finish<<<ndir, kDistWG, 0, s>>>((int)nb, dpart, d_dout);
+  HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}
+
+}  // namespace hgx
+
+extern "C" int hgx_image_distance(hgx_ctx* h, int64_t n, int metric, const double* I, const double* O,
+                                  const double* mask, double* distance) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!distance) return set_error(c, HGX_ERR_INVALID_ARG, "image_distance: bad arguments");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_result(c, 1)) return rc;
+  if (int rc = distance_enqueue(c, n, metric, I, O, mask, c->d_result, 0, nullptr, nullptr)) return rc;
+  HGX_CHECK_HIP(c, hipMemcpyAsync(distance, c->d_result, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return HGX_OK;
+}
+
+extern "C" int hgx_image_distance_async(hgx_ctx* h, int64_t n, int metric, const double* I, const double* O,
+                                        const double* mask, double* d_distance) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!d_distance) return set_error(c, HGX_ERR_INVALID_ARG, "image_distance_async: null output");
+  return distance_enqueue(c, n, metric, I, O, mask, d_distance, 0, nullptr, nullptr);
+}
+
+extern "C" int hgx_image_distance_tangent(hgx_ctx* h, int64_t n, int metric, const double* I, const double* O,
+                                          const double* mask, int ndir, const double* dI, double* distance,
+                                          double* ddistance) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!distance || ndir < 1 || !ddistance)
+    return set_error(c, HGX_ERR_INVALID_ARG, "image_distance_tangent: bad arguments");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = ensure_result(c, 1 + ndir)) return rc;
+  if (int rc = distance_enqueue(c, n, metric, I, O, mask, c->d_result, ndir, dI, c->d_result + 1)) return rc;
+  std::vector<double> host(1 + ndir);
+  HGX_CHECK_HIP(c, hipMemcpyAsync(host.data(), c->d_result, host.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+  HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  *distance = host[0];
+  for (int j = 0; j < ndir; ++j) ddistance[j] = host[1 + j];
   return HGX_OK;
 }

@@ -2,7 +2,7 @@
 // src/sf/sf.cpp:165-245 in closed Dirichlet form), shared by the SF kernel and the fused kernels.
 #pragma once
 
-#include "hgx_internal.hpp"
+#include "dual.hpp"
 
 namespace hgx {
 
@@ -25,23 +25,38 @@ __device__ inline cd dirichlet(cd x, double N) {
   return cdiv(csin_d(N * h), den);
 }
 
-struct SFArgs {
-  double la[3], lb[3], lc[3];
+template <class R>
+struct SFArgsT {
+  R la[3], lb[3], lc[3];
   double N[3];
-  double center[3];
-  double dw;  // sigma^2 / 2
+  R center[3];
+  R dw;  // sigma^2 / 2
   int compat_lb2;
 };
+using SFArgs = SFArgsT<double>;
+
+// D_N on duals: the plain value, the derivative d/dh (ddirichlet_dh) times dh = dx/2 (0 at a peak)
+__device__ inline dz dirichlet(dz x, double N) {
+  const cd v = dirichlet(x.v, N);
+  const double w = x.v.r - 2.0 * kPiD * rint(x.v.r * (0.5 / kPiD));
+  const cd h = 0.5 * cd{w, x.v.i};
+  if (cabs_d(csin_d(h)) < 1e-300) return dz{v, cd{0.0, 0.0}};
+  return dz{v, ddirichlet_dh(h, N) * (0.5 * x.d)};
+}
+
+__device__ inline bool nonzero(double x) { return x != 0.0; }
+__device__ inline bool nonzero(dd x) { return x.v != 0.0 || x.d != 0.0; }
 
 // S(q) for one rotated q-vector: product of the three Dirichlet kernels, the grain-centre phase
 // and the Debye-Waller damping.
-__device__ inline cd sf_value(const cd mq[3], const SFArgs& a) {
-  cd xa = mq[0] * a.la[0] + mq[1] * a.la[1] + mq[2] * a.la[2];
-  cd xb = mq[0] * a.lb[0] + mq[1] * a.lb[1] + (a.compat_lb2 ? cd{a.lb[2] * a.lb[2], 0.0} : mq[2] * a.lb[2]);
-  cd xc = mq[0] * a.lc[0] + mq[1] * a.lc[1] + mq[2] * a.lc[2];
-  cd s = dirichlet(xa, a.N[0]) * dirichlet(xb, a.N[1]) * dirichlet(xc, a.N[2]);
+template <class C, class R>
+__device__ inline C sf_value(const C mq[3], const SFArgsT<R>& a) {
+  C xa = mq[0] * a.la[0] + mq[1] * a.la[1] + mq[2] * a.la[2];
+  C xb = mq[0] * a.lb[0] + mq[1] * a.lb[1] + (a.compat_lb2 ? mkc(a.lb[2] * a.lb[2], R{}) : mq[2] * a.lb[2]);
+  C xc = mq[0] * a.lc[0] + mq[1] * a.lc[1] + mq[2] * a.lc[2];
+  C s = dirichlet(xa, a.N[0]) * dirichlet(xb, a.N[1]) * dirichlet(xc, a.N[2]);
   s = s * cexpi_d(mq[0] * a.center[0] + mq[1] * a.center[1] + mq[2] * a.center[2]);
-  if (a.dw != 0.0) s = exp(-a.dw * (norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]))) * s;
+  if (nonzero(a.dw)) s = exp(-a.dw * (norm2(mq[0]) + norm2(mq[1]) + norm2(mq[2]))) * s;
   return s;
 }
 

@@ -18,6 +18,7 @@ from .api import HgxError
 LIB_PATH = Path(__file__).resolve().parent / "libhgx_hig.so"
 
 NO_FUSE, SF_COMPAT_LB2, SMEAR_COMPAT, NO_SMEAR, MESH_NO_TRANSLATION, PROFILE = 1, 2, 4, 8, 16, 32
+FIT_FD_GRADIENT, FIT_SEQUENTIAL = 64, 128
 
 _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 _SIGS = {
@@ -43,6 +44,7 @@ _SIGS = {
     "hgx_hig_fit_algorithm": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_size_t, _ip, _dp]),
     "hgx_hig_fit_init": (C.c_int, [_vp]),
     "hgx_hig_fit_objective": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_double, _dp]),
+    "hgx_hig_fit_gradient": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]),
     "hgx_hig_fit_run": (C.c_int, [_vp, C.c_int, C.c_ulonglong, _dp, _dp, C.POINTER(C.c_longlong)]),
     "hgx_hig_fit_history": (C.c_int, [_vp, C.c_longlong, _dp, C.POINTER(C.c_longlong)]),
     "hgx_hig_last_error": (C.c_char_p, [_vp]),
@@ -236,6 +238,18 @@ class HigSimulation:
         self._check(self.lib.hgx_hig_fit_objective(self.h, xv.size, xv.ctypes.data_as(_dp), m, float(reg),
                                                    C.byref(v)), "fit_objective")
         return v.value
+
+    def gradient(self, x, metric="sqrt_unit_l2", reg: float = 0.0):
+        """The objective at x and its gradient in one tangent pass: (value, grad, computed), computed[j]
+        False where the pass does not cover variable j (its grad[j] is 0)."""
+        xv = np.ascontiguousarray(x, dtype=np.float64)
+        m = N.METRICS[metric] if isinstance(metric, str) else int(metric)
+        v = C.c_double()
+        g = np.zeros(xv.size)
+        done = (C.c_int * xv.size)()
+        self._check(self.lib.hgx_hig_fit_gradient(self.h, xv.size, xv.ctypes.data_as(_dp), m, float(reg), C.byref(v),
+                                                  g.ctypes.data_as(_dp), done), "fit_gradient")
+        return v.value, g, np.array([bool(d) for d in done])
 
     def fit(self, algorithm: int = 0, seed: int = 0) -> dict:
         """Runs algorithm block `algorithm`; returns the best x, its objective and the evaluation count."""

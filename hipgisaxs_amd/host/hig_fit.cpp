@@ -15,6 +15,7 @@
 #include <fstream>
 #include <functional>
 #include <limits>
+#include <memory>
 #include <random>
 #include <sstream>
 
@@ -380,6 +381,255 @@ bool HipGISAXS::fit_objective(const double* x, int metric, double reg, double* v
   return true;
 }
 
+namespace {
+
+// Whether two builds of every structure have the same counts (so that their difference is a tangent).
+bool same_layout(const std::vector<std::unique_ptr<EnsembleBuild>>& A, const std::vector<std::unique_ptr<EnsembleBuild>>& B) {
+  if (A.size() != B.size()) return false;
+  for (size_t s = 0; s < A.size(); ++s) {
+    const hgx_ensemble_model &a = A[s]->m, &b = B[s]->m;
+    if (a.nelem != b.nelem || a.ngrains != b.ngrains || a.sf_flags != b.sf_flags) return false;
+    for (int k = 0; k < 3; ++k)
+      if (a.repet[k] != b.repet[k]) return false;
+    for (int e = 0; e < a.nelem; ++e) {
+      const hgx_ensemble_element &x = a.elem[e], &y = b.elem[e];
+      if (x.shape != y.shape || x.nloc != y.nloc || x.params.flags != y.params.flags) return false;
+      for (int d = 0; d < 4; ++d)
+        if (x.params.n[d] != y.params.n[d]) return false;
+    }
+  }
+  return true;
+}
+
+// (P - M) / w field by field into D (same layout).
+void tangent_build(const EnsembleBuild& P, const EnsembleBuild& M, double w, EnsembleBuild& D) {
+  auto diff = [w](const std::vector<double>& a, const std::vector<double>& b) {
+    std::vector<double> r(a.size());
+    for (size_t i = 0; i < a.size(); ++i) r[i] = (a[i] - b[i]) / w;
+    return r;
+  };
+  D.el = P.el;
+  D.stores.resize(P.stores.size());
+  for (size_t i = 0; i < P.stores.size(); ++i) D.stores[i] = diff(P.stores[i], M.stores[i]);
+  D.locs.resize(P.locs.size());
+  for (size_t i = 0; i < P.locs.size(); ++i) D.locs[i] = diff(P.locs[i], M.locs[i]);
+  for (size_t e = 0; e < P.el.size(); ++e) {
+    hgx_ensemble_element& t = D.el[e];
+    const hgx_ensemble_element &a = P.el[e], &b = M.el[e];
+    t.params.tau = (a.params.tau - b.params.tau) / w;
+    t.params.eta = (a.params.eta - b.params.eta) / w;
+    for (int k = 0; k < 9; ++k) t.rot[k] = (a.rot[k] - b.rot[k]) / w;
+    t.dn2_re = (a.dn2_re - b.dn2_re) / w;
+    t.dn2_im = (a.dn2_im - b.dn2_im) / w;
+  }
+  D.rots = diff(P.rots, M.rots);
+  D.cen = diff(P.cen, M.cen);
+  D.wts = diff(P.wts, M.wts);
+  D.m = P.m;
+  for (int k = 0; k < 3; ++k) {
+    D.m.la[k] = (P.m.la[k] - M.m.la[k]) / w;
+    D.m.lb[k] = (P.m.lb[k] - M.m.lb[k]) / w;
+    D.m.lc[k] = (P.m.lc[k] - M.m.lc[k]) / w;
+  }
+  D.m.dw_sigma = (P.m.dw_sigma - M.m.dw_sigma) / w;
+  D.relink();
+}
+
+}  // namespace
+
+bool HipGISAXS::fit_gradient(const double* x, int metric, double reg, double* value, double* grad, int* computed) {
+  if (!fit_ready_) return fail("fit_init has not run");
+  if (configurations().size() != 1)
+    return fail("currently you can simulate only for single alpha_i, phi and tilt angles for fitting");
+  const size_t np = in_.fit_params.size();
+  std::vector<std::string> vars;
+  std::vector<double> mean, lo, hi;
+  double pnorm = 0.0;
+  size_t i = 0;
+  for (const auto& p : in_.fit_params) {
+    if (!update_param(p.second.variable, x[i])) return false;
+    vars.push_back(p.second.variable);
+    mean.push_back((p.second.max - p.second.min) / 2.0 + p.second.min);
+    lo.push_back(p.second.min);
+    hi.push_back(p.second.max);
+    pnorm += (x[i] - mean[i]) * (x[i] - mean[i]);
+    ++i;
+  }
+  for (size_t j = 0; j < np; ++j) {
+    computed[j] = 0;
+    grad[j] = 0.0;
+  }
+  if (!inited_ && !init()) return false;
+  if (npix_ != fit_npix_) return fail("the image size changed after fit_init");
+  // what the tangent pass covers: structures of analytic cells, the run_device checks passed
+  bool ok = !in_.structures.empty() && in_.compute.nslices <= 1 && in_.compute.structcorr == "nGnE";
+  for (const auto& s : in_.structures)
+    ok = ok && s.second.grain_type.empty() && !s.second.scaling_dist && !s.second.repetition_dist;
+  const double kDeg = 3.141592653589793 / 180;
+  const HigScattering& sc = in_.scattering;
+  const double alphai = sc.alphai_min * kDeg, phi = sc.phi_min * kDeg;
+  const GlibcRandom rng0 = rng_;
+  // every structure's model at the current values, grains drawn from rng0 as one run would draw them
+  auto build_all = [&](std::vector<std::unique_ptr<EnsembleBuild>>& out, std::vector<double>& ir,
+                       GlibcRandom& rng) -> bool {
+    out.clear();
+    for (const auto& s : in_.structures) {
+      std::vector<HigGrain> grains;
+      if (!make_grains(s.second, rng, phi, grains)) return false;
+      auto b = std::make_unique<EnsembleBuild>();
+      if (!ensemble_model(s.second, grains, *b, false)) return false;
+      out.push_back(std::move(b));
+    }
+    ir = iratios();
+    return true;
+  };
+  std::vector<std::unique_ptr<EnsembleBuild>> B0;
+  std::vector<double> ir0;
+  GlibcRandom rng = rng0;
+  ok = ok && build_all(B0, ir0, rng);
+  const GlibcRandom rng_after = rng;
+  // directions: variables that move the model's tables only, with builds of equal layout at x +- h
+  std::vector<int> dirs;
+  std::vector<std::vector<std::unique_ptr<EnsembleBuild>>> T;
+  std::vector<std::vector<double>> Tir;
+  for (size_t j = 0; ok && j < np; ++j) {
+    const std::string& v = vars[j];
+    if (v.compare(0, 15, "instrumentation") == 0 || v.compare(0, 5, "layer") == 0) continue;
+    // central differences of the host tables: they are smooth (mostly linear) in the variable, so a
+    // small step leaves only rounding, about 1e-16 |table| / h
+    const double h = 1e-6 * std::max({std::fabs(x[j]), 0.01 * std::fabs(hi[j] - lo[j]), 1e-3});
+    std::vector<std::unique_ptr<EnsembleBuild>> Bp, Bm;
+    std::vector<double> irp, irm;
+    GlibcRandom rp = rng0, rm = rng0;
+    const bool okp = update_param(v, x[j] + h) && build_all(Bp, irp, rp);
+    const bool okm = update_param(v, x[j] - h) && build_all(Bm, irm, rm);
+    if (!update_param(v, x[j])) return false;
+    err_.clear();
+    if (!okp || !okm || !same_layout(B0, Bp) || !same_layout(B0, Bm)) continue;
+    std::vector<std::unique_ptr<EnsembleBuild>> D;
+    for (size_t s = 0; s < B0.size(); ++s) {
+      D.push_back(std::make_unique<EnsembleBuild>());
+      tangent_build(*Bp[s], *Bm[s], 2 * h, *D.back());
+    }
+    std::vector<double> dir(ir0.size());
+    for (size_t s = 0; s < ir0.size(); ++s) dir[s] = (irp[s] - irm[s]) / (2 * h);
+    dirs.push_back((int)j);
+    T.push_back(std::move(D));
+    Tir.push_back(dir);
+  }
+  err_.clear();
+  rng_ = rng0;
+  if (dirs.empty()) return fit_objective(x, metric, reg, value);  // the caller differentiates
+  const int nd = (int)dirs.size();
+  const bool gisaxs = sc.expt == "gisaxs";
+  if (!ensure_grid(gisaxs ? alphai : -1.0, 0.0, 0.0)) return false;
+  double* d_fc = nullptr;
+  if (!fresnel_coeffs(alphai, &d_fc)) return false;
+  double* d_total = buffer(0, (size_t)npix_ * 8);
+  double* d_Is = buffer(1, (size_t)npix_ * 8);
+  double* d_dtot = buffer(3, (size_t)nd * npix_ * 8);
+  double* d_dIs = buffer(4, (size_t)nd * npix_ * 8);
+  if (!d_total || !d_Is || !d_dtot || !d_dIs) return fail("device allocation failed");
+  hgx_ctx* ctx = gpu_->get();
+  std::vector<hgx_ensemble_model> tm(nd);
+  for (size_t s = 0; s < B0.size(); ++s) {
+    for (int k = 0; k < nd; ++k) tm[k] = T[k][s]->m;
+    if (!hgx(hgx_ensemble_intensity_tangent(ctx, &B0[s]->m, nd, tm.data(), d_fc, d_Is, d_dIs),
+             "hgx_ensemble_intensity_tangent"))
+      return false;
+    // intensity = sum_s iratio_s I_s (src/sim/hipgisaxs_main.cpp:1351-1364) and its tangents
+    if (!hgx(hgx_axpy(ctx, npix_, ir0[s], d_Is, s > 0 ? 1 : 0, d_total), "hgx_axpy") ||
+        !hgx(hgx_axpy(ctx, (int64_t)nd * npix_, ir0[s], d_dIs, s > 0 ? 1 : 0, d_dtot), "hgx_axpy"))
+      return false;
+    for (int k = 0; k < nd; ++k)
+      if (Tir[k][s] != 0.0 && !hgx(hgx_axpy(ctx, npix_, Tir[k][s], d_Is, 1, d_dtot + (size_t)k * npix_), "hgx_axpy"))
+        return false;
+  }
+  // the smearing is linear: the same operator on the image and on every tangent
+  if (sc.smearing > DBL_MIN && !(flags_ & HGX_HIG_NO_SMEAR)) {
+    const int sf = (flags_ & HGX_HIG_SMEAR_COMPAT) ? HGX_SMEAR_COMPAT : 0;
+    for (int k = -1; k < nd; ++k)
+      if (!hgx(hgx_gaussian_smear(ctx, (int)ncol_, (int)nrow_, sc.smearing, sf, k < 0 ? d_total : d_dtot + (size_t)k * npix_),
+               "hgx_gaussian_smear"))
+        return false;
+  }
+  double D = 0.0;
+  std::vector<double> dD(nd);
+  if (!hgx(hgx_image_distance_tangent(ctx, npix_, metric, d_total, static_cast<const double*>(buf_[6]),
+                                      static_cast<const double*>(buf_[7]), nd, d_dtot, &D, dD.data()),
+           "hgx_image_distance_tangent"))
+    return false;
+  rng_ = rng_after;
+  const bool finite = std::isfinite(D);
+  double f = finite ? D + (reg / 2) * pnorm : 1e8;
+  for (int k = 0; k < nd; ++k) {
+    const int j = dirs[k];
+    grad[j] = finite ? dD[k] + reg * (x[j] - mean[j]) : 0.0;
+    computed[j] = 1;
+  }
+  if (precision_ == HGX_PREC_FP32C) {
+    // the value in the working precision, so that it compares with the fit's other evaluations
+    rng_ = rng0;
+    return fit_objective(x, metric, reg, value);
+  }
+  *value = f;
+  std::vector<double> row(x, x + np);
+  row.push_back(f);
+  history_.push_back(row);
+  return true;
+}
+
+bool HipGISAXS::fit_objective_many(const std::vector<std::vector<double>>& X, int metric, double reg,
+                                   std::vector<double>& out) {
+  out.clear();
+  if (!fit_ready_) return fail("fit_init has not run");
+  if (configurations().size() != 1)
+    return fail("currently you can simulate only for single alpha_i, phi and tilt angles for fitting");
+  const size_t n = X.size(), np = in_.fit_params.size();
+  if (n == 0) return true;
+  if (!device()) return false;
+  double* d_D = buffer(8, n * sizeof(double));
+  if (!d_D) return fail("device allocation failed");
+  std::vector<double> pnorm(n, 0.0);
+  for (size_t r = 0; r < n; ++r) {
+    if (X[r].size() != np) return fail("fit_objective_many: a row has the wrong length");
+    size_t i = 0;
+    for (const auto& p : in_.fit_params) {
+      if (!update_param(p.second.variable, X[r][i])) return false;
+      const double mean = (p.second.max - p.second.min) / 2.0 + p.second.min;
+      pnorm[r] += (X[r][i] - mean) * (X[r][i] - mean);
+      ++i;
+    }
+    if (!inited_ && !init()) return false;
+    if (npix_ != fit_npix_) return fail("the image size changed after fit_init");
+    const HigScattering& sc = in_.scattering;
+    const double* d_img = nullptr;
+    if (!run_device(sc.alphai_min, sc.phi_min, &d_img)) return false;
+    if (!d_img) {  // no structures: a zero image
+      double* d_zero = buffer(0, (size_t)npix_ * 8);
+      const std::vector<double> zeros((size_t)npix_, 0.0);
+      if (!d_zero || !hgx(hgx_memcpy_h2d(gpu_->get(), d_zero, zeros.data(), (size_t)npix_ * 8), "hgx_memcpy_h2d"))
+        return false;
+      d_img = d_zero;
+    }
+    // the next row's image overwrites buffer 0 only after this distance has read it (one stream)
+    if (!hgx(hgx_image_distance_async(gpu_->get(), npix_, metric, d_img, static_cast<const double*>(buf_[6]),
+                                      static_cast<const double*>(buf_[7]), d_D + r),
+             "hgx_image_distance_async"))
+      return false;
+  }
+  std::vector<double> D(n);
+  if (!hgx(hgx_memcpy_d2h(gpu_->get(), D.data(), d_D, n * sizeof(double)), "hgx_memcpy_d2h")) return false;
+  out.resize(n);
+  for (size_t r = 0; r < n; ++r) {
+    out[r] = std::isfinite(D[r]) ? D[r] + (reg / 2) * pnorm[r] : 1e8;
+    std::vector<double> row = X[r];
+    row.push_back(out[r]);
+    history_.push_back(row);
+  }
+  return true;
+}
+
 // ----- fit drivers -----------------------------------------------------------------------------------
 
 int metric_of(const std::string& name) {
@@ -400,9 +650,15 @@ struct Problem {
   double reg;
   std::vector<double> lo, hi, step, x0;
   int64_t evals = 0;
+  bool fd_only = false;     // LMVM gradients by central differences only (the reference's way)
+  int64_t grad_passes = 0;  // tangent passes of LMVM
   bool eval(const std::vector<double>& x, double& f) {
     ++evals;
     return sim.fit_objective(x.data(), metric, reg, &f);
+  }
+  bool eval_many(const std::vector<std::vector<double>>& X, std::vector<double>& f) {
+    evals += (int64_t)X.size();
+    return sim.fit_objective_many(X, metric, reg, f);
   }
 };
 
@@ -462,10 +718,19 @@ bool run_pso(Problem& P, const HigFitAlgorithm& a, uint64_t seed, std::vector<do
     }
   best.assign(d, 0.0);
   fbest = std::numeric_limits<double>::max();
+  const bool sequential = (P.sim.flags() & HGX_HIG_FIT_SEQUENTIAL) != 0;
+  std::vector<double> fgen;
   for (int gen = 0; gen < ng; ++gen) {
+    // the generation's objectives (one synchronisation), then the best-so-far updates in particle
+    // order, as the reference's loop makes them
+    if (!sequential && !P.eval_many(x, fgen)) return false;
     for (int i = 0; i < np; ++i) {
       double f;
-      if (!P.eval(x[i], f)) return false;
+      if (sequential) {
+        if (!P.eval(x[i], f)) return false;
+      } else {
+        f = fgen[i];
+      }
       if (pf[i] > f) {
         pf[i] = f;
         pb[i] = x[i];
@@ -533,10 +798,13 @@ bool run_bruteforce(Problem& P, std::vector<double>& best, double& fbest, std::s
 }
 
 // One solve of FitLMVMAlgo (hipgisaxs_fit_lmvm.cpp:43-180, 190-224): bounded limited-memory BFGS
-// from P.x0, gradients by central differences over +-step/2 (step, or 0.04 where the step is not
-// positive), divided by the same step (the reference divides by 0.04 whatever the step). Projected
-// search directions with a backtracking Armijo line search; stops when the projected gradient or
-// the relative decrease falls below the tolerance, or after 200 iterations.
+// from P.x0. The gradient comes from one tangent pass (HipGISAXS::fit_gradient: forward-mode
+// derivatives through the ensemble kernel) where the model allows; the remaining components, and
+// every component with P.fd_only, by the reference's central differences over +-step/2 (step, or
+// 0.04 where the step is not positive), divided by the same step (the reference divides by 0.04
+// whatever the step). Projected search directions with a backtracking Armijo line search; stops
+// when the projected gradient or the relative decrease falls below the tolerance, or after 200
+// iterations.
 bool run_lmvm(Problem& P, const HigFitAlgorithm& a, std::vector<double>& best, double& fbest, std::string&) {
   const int d = (int)P.lo.size(), mem = 6, max_iter = 200;
   const double tol = a.tolerance > 0 ? a.tolerance : 1e-6;
@@ -546,9 +814,17 @@ bool run_lmvm(Problem& P, const HigFitAlgorithm& a, std::vector<double>& best, d
   };
   project(x);
   auto grad = [&](const std::vector<double>& y, double& f, std::vector<double>& g) -> bool {
-    if (!P.eval(y, f)) return false;
     g.assign(d, 0.0);
+    std::vector<int> done(d, 0);
+    if (P.fd_only) {
+      if (!P.eval(y, f)) return false;
+    } else {
+      ++P.evals;
+      ++P.grad_passes;
+      if (!P.sim.fit_gradient(y.data(), P.metric, P.reg, &f, g.data(), done.data())) return false;
+    }
     for (int j = 0; j < d; ++j) {
+      if (done[j]) continue;
       const double h = P.step[j] > DBL_MIN ? P.step[j] : 0.04;
       std::vector<double> yp = y, ym = y;
       yp[j] += 0.5 * h;
@@ -696,6 +972,7 @@ bool fit_run(HipGISAXS& sim, int algo_index, uint64_t seed, std::vector<double>&
     return false;
   }
   Problem P{sim, metric, a.regularization, {}, {}, {}, {}};
+  P.fd_only = (sim.flags() & HGX_HIG_FIT_FD_GRADIENT) != 0;
   for (const auto& p : in.fit_params) {
     P.lo.push_back(p.second.min);
     P.hi.push_back(p.second.max);
@@ -829,6 +1106,21 @@ int hgx_hig_fit_objective(hgx_hig* h, int n, const double* x, int metric, double
     if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2) return fit_fail(h, "fit_objective: unknown metric");
     if (!h->sim.fit_ready() && !h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
     if (!h->sim.fit_objective(x, metric, reg, value)) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
+    return HGX_OK;
+  });
+}
+
+int hgx_hig_fit_gradient(hgx_hig* h, int n, const double* x, int metric, double reg, double* value, double* grad,
+                         int* computed) {
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!x || !value || !grad || !computed) return fit_fail(h, "fit_gradient: null argument");
+    if (n != (int)h->sim.input().fit_params.size())
+      return fit_fail(h, "fit_gradient: x must hold one value per fit parameter");
+    if (metric < HGX_METRIC_CHI2 || metric > HGX_METRIC_CBRT_C_L2) return fit_fail(h, "fit_gradient: unknown metric");
+    if (!h->sim.fit_ready() && !h->sim.fit_init()) return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
+    if (!h->sim.fit_gradient(x, metric, reg, value, grad, computed))
+      return fit_fail(h, h->sim.error(), HGX_ERR_INVALID_ARG);
     return HGX_OK;
   });
 }

@@ -158,11 +158,11 @@ void HipGISAXS::release_device_objects() {
   fc_key_.clear();
   if (gpu_ && gpu_->ok()) {
     for (auto& m : meshes_) hgx_mesh_release(gpu_->get(), m.second);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < kBuffers; ++i)
       if (buf_[i]) hgx_device_free(gpu_->get(), buf_[i]);
   }
   meshes_.clear();
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < kBuffers; ++i) {
     buf_[i] = nullptr;
     buf_bytes_[i] = 0;
   }
@@ -502,23 +502,97 @@ bool HipGISAXS::form_factor(const HigShape& sh, const Rot3& rot, const vec3& loc
 
 // One structure's grain sum into d_I (written, not accumulated): the body of run_gisaxs's structure
 // loop for the nGnE correlation (src/sim/hipgisaxs_main.cpp:775-1222).
-bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains,
-                                    double* d_I) {
+void EnsembleBuild::relink() {
+  m.nelem = (int)el.size();
+  m.elem = el.data();
+  for (size_t e = 0; e < el.size(); ++e) {
+    for (int d = 0; d < 4; ++d) {
+      el[e].params.val[d] = el[e].params.n[d] > 0 ? stores[8 * e + 2 * d].data() : nullptr;
+      el[e].params.wt[d] = el[e].params.n[d] > 0 ? stores[8 * e + 2 * d + 1].data() : nullptr;
+    }
+    el[e].loc = locs[e].data();
+  }
+  m.ngrains = (int)wts.size();
+  m.rot = rots.data();
+  m.center = cen.data();
+  m.weight = wts.data();
+}
+
+std::vector<double> HipGISAXS::iratios() const {
+  std::vector<double> r;
+  double sum = 0.0;
+  for (const auto& s : in_.structures) {
+    r.push_back(s.second.iratio);
+    sum += s.second.iratio;
+  }
+  if (sum != 1.0)
+    for (double& x : r) x /= sum;
+  return r;
+}
+
+bool HipGISAXS::ensemble_model(const HigStructure& s, const std::vector<HigGrain>& grains, EnsembleBuild& b,
+                               bool report) {
+  auto no = [&](const std::string& why) { return report ? fail(why) : false; };
   auto uit = in_.unitcells.find(s.unitcell_key);
-  if (uit == in_.unitcells.end()) return fail("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
+  if (uit == in_.unitcells.end()) return no("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
   const HigUnitcell& uc = uit->second;
-  for (const auto& e : uc.elements)
-    if (!in_.shapes.count(e.first)) return fail("unitcell '" + uc.key + "': no shape '" + e.first + "'");
-  const bool gisaxs = in_.scattering.expt == "gisaxs";
-  // The particle layer: the reference reads Structure::layer_order(), which no code path sets, so it
-  // is always 0 -- the vacuum (structure.cpp:358, hipgisaxs_main.cpp:780).
-  const int order = 0;
+  for (const auto& e : uc.elements) {
+    auto sh = in_.shapes.find(e.first);
+    if (sh == in_.shapes.end()) return no("unitcell '" + uc.key + "': no shape '" + e.first + "'");
+    if (sh->second.shape < 0) return no("shape '" + e.first + "' is not analytic");
+  }
+  vec3 a, bb, c;
+  std::string err;
+  if (!lattice_vectors(s, a, bb, c, err)) return no("structure '" + s.key + "': " + err);
+  const int ne = (int)uc.elements.size(), ng = (int)grains.size();
+  if (ne < 1 || ne > 64 || ng < 1) return no("structure '" + s.key + "': unit cell or grains out of range");
+  // the particle layer is the vacuum (structure_intensity)
   const double dnl[2] = {0.0, 0.0};
-  if (!ensure_grid(gisaxs ? alphai : -1.0, dnl[0], dnl[1])) return false;
-  const int nbranch = gisaxs ? 4 : 1;
-  const int64_t nq = npix_ * nbranch;
-  double* d_fc = nullptr;
-  if (gisaxs) {
+  b.el.assign(ne, hgx_ensemble_element{});
+  b.stores.assign(8 * (size_t)ne, {});
+  b.locs.assign(ne, {});
+  int e = 0;
+  for (const auto& it : uc.elements) {
+    const HigShape& sh = in_.shapes.at(it.first);
+    hgx_ensemble_element& E = b.el[e];
+    std::memset(&E, 0, sizeof(E));
+    if (!shape_params(sh, E.params, &b.stores[8 * (size_t)e])) return report ? false : (err_.clear(), false);
+    E.shape = sh.shape;
+    Rot3 re = Rot3(0, sh.xrot) * Rot3(1, sh.yrot) * Rot3(2, sh.zrot);
+    std::memcpy(E.rot, re.data(), sizeof(E.rot));
+    for (const auto& l : it.second) b.locs[e].insert(b.locs[e].end(), l.begin(), l.end());
+    E.nloc = (int)it.second.size();
+    E.dn2_re = dnl[0] - 2 * sh.delta;
+    E.dn2_im = dnl[1] - 2 * sh.beta;
+    ++e;
+  }
+  std::memset(&b.m, 0, sizeof(b.m));
+  // StructureFactor::compute_structure_factor scales the vectors (src/sf/sf.cpp:165-171)
+  for (int i = 0; i < 3; ++i) {
+    b.m.la[i] = a[i] * s.scaling[0];
+    b.m.lb[i] = bb[i] * s.scaling[1];
+    b.m.lc[i] = c[i] * s.scaling[2];
+    b.m.repet[i] = s.repetition[i];
+  }
+  b.m.dw_sigma = 0.0;
+  b.m.sf_flags = (flags_ & HGX_HIG_SF_COMPAT_LB2) ? HGX_SF_COMPAT_LB2 : 0;
+  b.rots.assign(9 * (size_t)ng, 0.0);
+  b.cen.assign(3 * (size_t)ng, 0.0);
+  b.wts.assign(ng, 0.0);
+  for (int g = 0; g < ng; ++g) {
+    std::memcpy(&b.rots[9 * (size_t)g], grains[g].rot.data(), 9 * sizeof(double));
+    std::memcpy(&b.cen[3 * (size_t)g], grains[g].center, 3 * sizeof(double));
+    b.wts[g] = grains[g].weight;
+  }
+  b.relink();
+  return true;
+}
+
+// The Fresnel coefficients of the layer stack for the particle layer at alphai, in buffer 2 (SAXS:
+// none), recomputed only when the stack, k0, the angle or the grid changed.
+bool HipGISAXS::fresnel_coeffs(double alphai, double** d_out) {
+  *d_out = nullptr;
+  if (in_.scattering.expt == "gisaxs") {
     // MultiLayer::init (src/model/multilayer.cpp:29-59): layers of order >= 0 top to bottom, then the
     // substrate (order -1, or an empty layer when the input has none)
     std::vector<double> de, be, th;
@@ -535,7 +609,9 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     de.push_back(sub != in_.layers.end() ? sub->second.delta : 0.0);
     be.push_back(sub != in_.layers.end() ? sub->second.beta : 0.0);
     th.push_back(sub != in_.layers.end() ? sub->second.thickness : 0.0);
-    d_fc = buffer(2, (size_t)nq * 16);
+    const int64_t nq = npix_ * 4;
+    const int order = 0;  // the particle layer (see structure_intensity)
+    double* d_fc = buffer(2, (size_t)nq * 16);
     if (!d_fc) return fail("device allocation failed");
     // the coefficients depend on the stack, k0, the incidence angle and the grid's rows: reuse them while
     // none of these changed (structures of one run, successive fit evaluations)
@@ -550,7 +626,27 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
         return false;
       fc_key_ = key;
     }
+    *d_out = d_fc;
   }
+  return true;
+}
+
+bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains,
+                                    double* d_I) {
+  auto uit = in_.unitcells.find(s.unitcell_key);
+  if (uit == in_.unitcells.end()) return fail("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
+  const HigUnitcell& uc = uit->second;
+  for (const auto& e : uc.elements)
+    if (!in_.shapes.count(e.first)) return fail("unitcell '" + uc.key + "': no shape '" + e.first + "'");
+  const bool gisaxs = in_.scattering.expt == "gisaxs";
+  // The particle layer: the reference reads Structure::layer_order(), which no code path sets, so it
+  // is always 0 -- the vacuum (structure.cpp:358, hipgisaxs_main.cpp:780).
+  const double dnl[2] = {0.0, 0.0};
+  if (!ensure_grid(gisaxs ? alphai : -1.0, dnl[0], dnl[1])) return false;
+  const int nbranch = gisaxs ? 4 : 1;
+  const int64_t nq = npix_ * nbranch;
+  double* d_fc = nullptr;
+  if (!fresnel_coeffs(alphai, &d_fc)) return false;
   vec3 a, b, c;
   std::string err;
   if (!lattice_vectors(s, a, b, c, err)) return fail("structure '" + s.key + "': " + err);
@@ -606,62 +702,22 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     counts_[0] += ng;
     counts_[2] += ng;
     counts_[3] += ng;
-    // the model arrays are read on the host during the call; synchronise before they go out of scope
-    return hgx(hgx_synchronize(gpu_->get()), "hgx_synchronize");
+    // the call stages the model's host arrays through pinned memory: nothing to wait for here
+    return true;
   }
 
-  // any analytic cell: the fused ensemble kernel (FP64) over all grains, elements and locations
-  bool analytic = !(flags_ & HGX_HIG_NO_FUSE);
-  for (const auto& e : uc.elements) analytic = analytic && in_.shapes.at(e.first).shape >= 0;
-  if (analytic) {
-    const int ne = (int)uc.elements.size();
-    std::vector<hgx_ensemble_element> el(ne);
-    std::vector<std::vector<double>> stores(8 * (size_t)ne), locs(ne);
-    int e = 0;
-    for (const auto& it : uc.elements) {
-      const HigShape& sh = in_.shapes.at(it.first);
-      hgx_ensemble_element& E = el[e];
-      std::memset(&E, 0, sizeof(E));
-      if (!shape_params(sh, E.params, &stores[8 * (size_t)e])) return false;
-      E.shape = sh.shape;
-      Rot3 re = Rot3(0, sh.xrot) * Rot3(1, sh.yrot) * Rot3(2, sh.zrot);
-      std::memcpy(E.rot, re.data(), sizeof(E.rot));
-      for (const auto& l : it.second) locs[e].insert(locs[e].end(), l.begin(), l.end());
-      E.nloc = (int)it.second.size();
-      E.loc = locs[e].data();
-      E.dn2_re = dnl[0] - 2 * sh.delta;
-      E.dn2_im = dnl[1] - 2 * sh.beta;
-      counts_[2] += (int64_t)ng * E.nloc;
-      ++e;
+  // any analytic cell: the fused ensemble kernel over all grains, elements and locations
+  if (!(flags_ & HGX_HIG_NO_FUSE)) {
+    EnsembleBuild b;
+    if (ensemble_model(s, grains, b, false)) {
+      for (const auto& e : b.el) counts_[2] += (int64_t)ng * e.nloc;
+      // the particles sit in the vacuum layer (dnl = 0 above), so q is real and FP32C takes any cell
+      if (!hgx(hgx_ensemble_intensity(gpu_->get(), &b.m, precision_, d_fc, 0, d_I), "hgx_ensemble_intensity"))
+        return false;
+      counts_[0] += ng;
+      counts_[3] += ng;
+      return true;
     }
-    hgx_ensemble_model m;
-    std::memset(&m, 0, sizeof(m));
-    m.nelem = ne;
-    m.elem = el.data();
-    for (int i = 0; i < 3; ++i) {
-      m.la[i] = la[i];
-      m.lb[i] = lb[i];
-      m.lc[i] = lc[i];
-      m.repet[i] = repet[i];
-    }
-    m.dw_sigma = 0.0;
-    m.sf_flags = sf_flags;
-    m.ngrains = ng;
-    std::vector<double> rots(9 * (size_t)ng), cen(3 * (size_t)ng), wts(ng);
-    for (int g = 0; g < ng; ++g) {
-      std::memcpy(&rots[9 * (size_t)g], grains[g].rot.data(), 9 * sizeof(double));
-      std::memcpy(&cen[3 * (size_t)g], grains[g].center, 3 * sizeof(double));
-      wts[g] = grains[g].weight;
-    }
-    m.rot = rots.data();
-    m.center = cen.data();
-    m.weight = wts.data();
-    // the particles sit in the vacuum layer (dnl = 0 above), so q is real and FP32C takes any cell
-    if (!hgx(hgx_ensemble_intensity(gpu_->get(), &m, precision_, d_fc, 0, d_I), "hgx_ensemble_intensity"))
-      return false;
-    counts_[0] += ng;
-    counts_[3] += ng;
-    return true;
   }
 
   // per grain: form factor (reused while the grain rotation repeats), structure factor, DWBA sum

@@ -54,6 +54,20 @@ struct HigGrain {
   double weight;        // gauss_weight * scaling weight
 };
 
+// The hgx_ensemble_model of one structure (an analytic unit cell seen by its grains) with the host
+// arrays it points into.
+struct EnsembleBuild {
+  std::vector<hgx_ensemble_element> el;
+  std::vector<std::vector<double>> stores;  // 8 per element: node values and weights per dimension
+  std::vector<std::vector<double>> locs;    // per element [nloc][3]
+  std::vector<double> rots, cen, wts;       // per grain
+  hgx_ensemble_model m{};
+  EnsembleBuild() = default;
+  EnsembleBuild(const EnsembleBuild&) = delete;
+  EnsembleBuild& operator=(const EnsembleBuild&) = delete;
+  void relink();  // points m and el at the arrays above
+};
+
 // Parsing, configurations and grain sampling run on the host alone; the device context is created
 // by the first call that needs it (init, the runs).
 class HipGISAXS {
@@ -103,8 +117,23 @@ class HipGISAXS {
   // under `metric` (HGX_METRIC_*) to the reference data, plus (reg/2) |x - range midpoints|^2; a
   // non-finite distance becomes 1e8. Appends (x, value) to the evaluation history.
   bool fit_objective(const double* x, int metric, double reg, double* value);
+  // The objective at x and its gradient in one pass: forward-mode derivatives through the ensemble
+  // kernel (hgx_ensemble_intensity_tangent) for every variable that moves only the model's tables
+  // (shape parameters and rotations, contrasts, locations, lattice vectors and scaling, grain
+  // centres, orientation angles and weights, iratio), with the tables' tangents from host central
+  // differences of their construction; the distance's derivative from hgx_image_distance_tangent.
+  // computed[j] = 0 for a variable it cannot differentiate so (instrument and layer values, which
+  // move the q-grid or the Fresnel coefficients; integer counts; cells with numeric shapes): the
+  // caller differentiates those otherwise. The value is the FP64 objective. Appends (x, value) to
+  // the history.
+  bool fit_gradient(const double* x, int metric, double reg, double* value, double* grad, int* computed);
+  // fit_objective of every row of X in turn, the images and distances queued back to back and the
+  // distances read back with one synchronisation (the generation of the particle swarm,
+  // src/analyzer/hipgisaxs_fit_pso.cpp:521-543); same values and history as the calls one by one.
+  bool fit_objective_many(const std::vector<std::vector<double>>& X, int metric, double reg, std::vector<double>& out);
   std::vector<std::vector<double>>& fit_history() { return history_; }
 
+  int flags() const { return flags_; }
   unsigned nrows() const { return nrow_; }
   unsigned ncols() const { return ncol_; }
   double k0() const { return k0_; }
@@ -131,6 +160,13 @@ class HipGISAXS {
   bool run_device(double alpha_i_deg, double phi_deg, const double** d_img);
   bool make_grains(const HigStructure& s, GlibcRandom& rng, double phi_rad, std::vector<HigGrain>& out);
   bool structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains, double* d_I);
+  // The Fresnel coefficients for the particle layer at alphai (buffer 2; NULL for SAXS), cached.
+  bool fresnel_coeffs(double alphai, double** d_fc);
+  // The ensemble model of structure s for `grains` when its unit cell is analytic (false otherwise,
+  // with the reason in err_ only when `report`).
+  bool ensemble_model(const HigStructure& s, const std::vector<HigGrain>& grains, EnsembleBuild& b, bool report);
+  // The normalised iratio weights of the structures (src/sim/hipgisaxs_main.cpp:1268-1282).
+  std::vector<double> iratios() const;
   bool form_factor(const HigShape& shape, const Rot3& rot, const vec3& loc, double* d_out);
   bool shape_params(const HigShape& shape, hgx_shape_params& p, std::vector<double> store[8]);
   int mesh_of(const HigShape& shape);
@@ -152,8 +188,12 @@ class HipGISAXS {
   uint64_t grid_gen_ = 0;          // bumped at every grid rebuild
   std::vector<double> fc_key_;     // inputs of the Fresnel coefficients now in buffer 2
   double grid_ai_ = 0.0, grid_dn_[2] = {0.0, 0.0};
-  void* buf_[8] = {};
-  size_t buf_bytes_[8] = {};
+  // device buffers: 0 image, 1 structure image, 2 Fresnel coefficients, 3-5 per-grain form and
+  // structure factors (tangent images in a gradient pass), 6 reference data, 7 mask, 8 a generation's
+  // distances
+  static constexpr int kBuffers = 9;
+  void* buf_[kBuffers] = {};
+  size_t buf_bytes_[kBuffers] = {};
   std::map<std::string, int> meshes_;
   bool fit_ready_ = false;
   int64_t fit_npix_ = 0;

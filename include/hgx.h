@@ -331,6 +331,18 @@ typedef struct hgx_ensemble_model {
 int hgx_ensemble_intensity(hgx_ctx* ctx, const hgx_ensemble_model* model, int precision, const double* fc,
                            int accumulate, double* I);
 
+/* Forward-mode derivatives of hgx_ensemble_intensity (FP64): I[p] as above (written, not accumulated)
+ * and dI[j][p] = d/dt I[p](model + t tangents[j]) for ndir (1..64) directions, all in one launch.
+ * tangents[j] has the model's counts (elements, shapes, node counts, locations, grains) and holds in
+ * every real field the derivative of the model's field along direction j: node values and weights
+ * (NULL arrays: zero), tau, eta, element rotations, locations, contrasts, lattice vectors, dw_sigma,
+ * grain rotations, centres and weights (NULL: zero); repetitions and shapes are not differentiated.
+ * fc is held fixed. The fit of HiG inputs uses it for the gradient of its objective in place of the
+ * reference's central differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). No counterpart in
+ * the reference. dI is device [ndir][npix]. */
+int hgx_ensemble_intensity_tangent(hgx_ctx* ctx, const hgx_ensemble_model* model, int ndir,
+                                   const hgx_ensemble_model* tangents, const double* fc, double* I, double* dI);
+
 /* ---- chi^2 and its gradient (forward / backward pair) ------------------------------------- */
 
 /* One-grain model fitted by hgx_chi2_*: I(p) = s |sum_b fc_b dn2 F(m_b) S(m_b)|^2 with
@@ -455,7 +467,7 @@ int hgx_axpy(hgx_ctx* ctx, int64_t n, double a, const double* x, int accumulate,
  * Convolutions::convolution_gaussian_2d (include/numerics/convolutions.hpp:128-175), applied by
  * run_gisaxs when the smearing sigma exceeds DBL_MIN (src/sim/hipgisaxs_main.cpp:1409-1431). By
  * default the weight follows the distance |k - i| over [i - h, i + h]; HGX_SMEAR_COMPAT keeps the
- * reference's window. Synchronises the stream. */
+ * reference's window. The weight table goes through pinned staging memory: no synchronisation. */
 int hgx_gaussian_smear(hgx_ctx* ctx, int ncol, int nrow, double sigma, int flags, double* I);
 
 /* Distance of a model image I to reference data O, both device [n], with an optional device mask
@@ -466,6 +478,17 @@ int hgx_gaussian_smear(hgx_ctx* ctx, int ncol, int nrow, double sigma, int flags
  * block partials summed in a fixed order. Writes the host scalar *distance; synchronises the stream. */
 int hgx_image_distance(hgx_ctx* ctx, int64_t n, int metric, const double* I, const double* O,
                        const double* mask, double* distance);
+/* hgx_image_distance with the value left in device memory (*d_distance, one double) and no
+ * synchronisation: distances of many images queued back to back and read back at once (the particle
+ * swarm's generation, src/analyzer/hipgisaxs_fit_pso.cpp:521-543). No counterpart in the reference. */
+int hgx_image_distance_async(hgx_ctx* ctx, int64_t n, int metric, const double* I, const double* O,
+                             const double* mask, double* d_distance);
+/* The distance (host *distance) and its derivatives along ndir tangent images dI (device [ndir][n]):
+ * ddistance[j] = sum_p dD/dI_p dI[j][p] (host [ndir]), dD/dI from the functor's closed form (the
+ * normalised forms through their norms and c); pixels whose term the functor drops, or with I <= 0
+ * under the root forms, contribute 0. One synchronisation. No counterpart in the reference. */
+int hgx_image_distance_tangent(hgx_ctx* ctx, int64_t n, int metric, const double* I, const double* O,
+                               const double* mask, int ndir, const double* dI, double* distance, double* ddistance);
 
 /* ---- host helpers ------------------------------------------------------------------------- */
 

@@ -25,6 +25,8 @@ typedef struct hgx_hig hgx_hig;
 #define HGX_HIG_MESH_NO_TRANSLATION 16 /* ignore unit-cell locations of mesh shapes, as the reference
                                           does (src/ff/ff.cpp:44) */
 #define HGX_HIG_PROFILE 32          /* time every kernel of a run with HIP events (hgx_hig_kernel_ms) */
+#define HGX_HIG_FIT_FD_GRADIENT 64  /* LMVM gradients by the reference's central differences only */
+#define HGX_HIG_FIT_SEQUENTIAL 128  /* particle swarm: one synchronisation per particle, as the reference */
 
 /* A driver on HIP device `device`. Reading inputs, hgx_hig_info, hgx_hig_config and hgx_hig_grains
  * run on the host; the device context (one hgx context) is created by the first call that needs it. */
@@ -99,8 +101,21 @@ int hgx_hig_fit_init(hgx_hig* h);
  * (reg / 2) |x - range midpoints|^2; a non-finite distance gives 1e8. Runs hgx_hig_fit_init first
  * when it has not run. */
 int hgx_hig_fit_objective(hgx_hig* h, int n, const double* x, int metric, double reg, double* value);
+/* The objective at x and its gradient in one pass (no counterpart in the reference, whose LMVM takes
+ * central differences, src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224): forward-mode derivatives
+ * through the ensemble kernel (hgx_ensemble_intensity_tangent) and the distance
+ * (hgx_image_distance_tangent) for every variable that moves only the model's tables, their tangents
+ * taken by central differences of the host's table construction. computed[j] = 1 where grad[j] was
+ * so formed, 0 for variables it does not cover (instrument and layer values, integer counts, cells
+ * with numeric shapes), whose grad[j] is 0. The value is the objective (FP64 when the FP32C option is
+ * off). */
+int hgx_hig_fit_gradient(hgx_hig* h, int n, const double* x, int metric, double reg, double* value, double* grad,
+                         int* computed);
 /* Runs algorithm block `algorithm`: pso (ParticleSwarmOptimization, base flavour, generator seeded
- * with `seed`), bruteforce, lmvm (bounded L-BFGS on central differences) or none_pounders (the
+ * with `seed`, each generation's objectives queued and read back with one synchronisation unless
+ * HGX_HIG_FIT_SEQUENTIAL), bruteforce, lmvm (bounded L-BFGS, gradients from hgx_hig_fit_gradient and
+ * central differences for the rest, or central differences alone with HGX_HIG_FIT_FD_GRADIENT) or
+ * none_pounders (the
  * objective at the initial values). x_best [nparams] and *value: the best point and its objective;
  * *evals: objective calls. */
 int hgx_hig_fit_run(hgx_hig* h, int algorithm, unsigned long long seed, double* x_best, double* value,
