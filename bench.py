@@ -102,15 +102,7 @@ class pixel_errors:
 def make_workload(nshapes=64, seed=SEED + 3):
     from hipgisaxs_amd import meshes
 
-    rng = np.random.default_rng(seed)
-    shapes = []
-    for _ in range(nshapes):
-        axes = rng.uniform(3.0, 15.0, 3)
-        target = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
-        side = int(rng.choice([128, 256, 512, 1024]))
-        tri = meshes.convex_polyhedron(rng, axes, target)
-        shapes.append({"tri": tri, "side": side, "axes": axes})
-    return shapes
+    return meshes.c3_shapes(nshapes, seed)
 
 
 def _cpu_model():
@@ -430,13 +422,12 @@ def bench_fit(evals=20):
             "workload": "cylinders R4 H12, hcp a=15 20x20x1, 512^2 DWBA, 6 parameters, FP64 adjoint"}
 
 
-def bench_grating(ncand=16, iters=2):
+def bench_grating(ncand=256, iters=1):
     """Trapezoidal grating lines (top 20 nm, sidewall 82 deg, height 40 nm; pitch 100 nm, 200 periods,
     length 100 nm) meshed to 106 912 triangles, 1024^2 q-space grid qy [-2, 2], qz [0, 2] with four
     DWBA branches over Si, 10 keV, 0.2 deg; seeded Poisson target at 1e4 peak counts. ncand candidates
-    (perturbed starts) run forward + adjoint + L-BFGS update per hipGraph step; the 256-candidate x
-    200-iteration run is extrapolated from the measured per-iteration time (candidates are independent
-    work in one launch, so the time is linear in their number)."""
+    (perturbed starts, 256 as configured) run forward + adjoint + L-BFGS update per hipGraph step; the
+    200-iteration run's time is the measured per-iteration time of all candidates times 200."""
     import torch
     from hipgisaxs_amd.api import Context, GratingPlan
 
@@ -492,7 +483,7 @@ def bench_grating(ncand=16, iters=2):
             "ms_forward_per_candidate": 1e3 * t_fwd, "grad_over_fwd": t_iter / ncand / t_fwd,
             "forward_pairs_per_candidate": pairs,
             "fwd_plus_adjoint_pair_rate": ncand * pairs / t_iter,
-            "extrapolated_s_256x200": 256 / ncand * t_iter * 200,
+            "s_for_200_iterations": t_iter * 200,
             "chi2_median_start": float(np.median(chi0)), "chi2_median_after": float(np.median(chi1)),
             "iterations_timed": iters,
             "workload": "trapezoid lines 106912 triangles, pitch 100 x 200, 1024^2 DWBA (4 branches), "
@@ -583,8 +574,10 @@ def bench_hig_examples(reps=3, cpu=True):
 def bench_hig_fit(cpu=True):
     """Fitting a .hig model (tests/golden/hig/fit_cylinders.hig at a 600 x 300 detector): objective
     evaluations per second of the particle-swarm driver (16 particles x 4 generations, each evaluation
-    = update_params + compute_gisaxs + the sqrt_unit_l2 distance on the device), and the CPU oracle's
-    objective timed on one evaluation of the same model."""
+    = update_params + compute_gisaxs + the sqrt_unit_l2 distance on the device), with each generation
+    read back once and particle by particle; the gradient's tangent pass against the 2P + 1 objective
+    evaluations of central differences; LMVM on tangent passes and on central differences; and the
+    CPU oracle's objective timed on one evaluation of the same model."""
     import tempfile
 
     from hipgisaxs_amd import hig as H
@@ -607,6 +600,39 @@ def bench_hig_fit(cpu=True):
         wall = time.perf_counter() - t0
         res = {"pixels": 600 * 300, "pso_evals": out["evals"], "pso_s": wall, "evals_per_s": out["evals"] / wall,
                "ms_per_eval": 1e3 * wall / out["evals"], "best": [float(x) for x in out["x"]]}
+        sim.close()
+        seq = H.HigSimulation(text=text, base_dir=d, flags=H.FIT_SEQUENTIAL)
+        seq.fit_init()
+        seq.objective([4.2, 48.5])
+        t0 = time.perf_counter()
+        out_s = seq.fit(algorithm=0, seed=1)
+        res["pso_particle_by_particle_s"] = time.perf_counter() - t0
+        res["pso_same_result"] = bool(np.array_equal(out_s["x"], out["x"]) and out_s["value"] == out["value"])
+        seq.close()
+        # the gradient: one tangent pass against 2P + 1 objective evaluations
+        sim = H.HigSimulation(text=text, base_dir=d)
+        sim.fit_init()
+        sim.gradient([4.2, 48.5])
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sim.gradient([4.2, 48.5])
+        t_grad = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sim.objective([4.2, 48.5])
+        t_obj = (time.perf_counter() - t0) / reps
+        res["gradient"] = {"tangent_pass_ms": 1e3 * t_grad, "objective_ms": 1e3 * t_obj,
+                           "central_differences_ms": 1e3 * 5 * t_obj, "tangent_over_objective": t_grad / t_obj}
+        fine = text.replace("step = 0.5 }", "step = 0.02 }").replace("step = 2.5 }", "step = 0.1 }")
+        for name, flags in (("lmvm_tangent", 0), ("lmvm_central_differences", H.FIT_FD_GRADIENT)):
+            s2 = H.HigSimulation(text=fine, base_dir=d, flags=flags)
+            s2.fit_init()
+            t0 = time.perf_counter()
+            o2 = s2.fit(algorithm=2)
+            res[name] = {"s": time.perf_counter() - t0, "evals": o2["evals"], "value": o2["value"],
+                         "x": [float(x) for x in o2["x"]]}
+            s2.close()
         sim.close()
         if cpu:
             params, _, _ = OH.read_fitting(text)
@@ -648,6 +674,8 @@ def main():
                     help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
     ap.add_argument("--hig-only", action="store_true",
                     help="only the .hig driver leg (the reference's example models), as one JSON object")
+    ap.add_argument("--leg", choices=("pyramid", "lattice", "fit", "grating", "hig"), default=None,
+                    help="only that measurement leg, as one JSON object (for profiling one workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -658,9 +686,13 @@ def main():
     import torch
     from hipgisaxs_amd.api import Batch, Context
 
-    if args.hig_only:
+    if args.hig_only or args.leg == "hig":
         print(json.dumps({"hig_examples": bench_hig_examples(cpu=not args.no_cpu),
                           "hig_fit": bench_hig_fit(cpu=not args.no_cpu)}))
+        return
+    if args.leg is not None:
+        legs = {"pyramid": bench_pyramid_fp64, "lattice": bench_lattice, "fit": bench_fit, "grating": bench_grating}
+        print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")

@@ -74,7 +74,7 @@ def build_peaks() -> Path:
 
 ADAPTER_SRCS = [PKG / "host" / "hgx_adapter.cpp"]
 ADAPTER_LIB = PKG / "libhgx_adapter.so"
-HIG_SRCS = [PKG / "host" / n for n in ("hig_input.cpp", "hig_sim.cpp", "hig_fit.cpp")]
+HIG_SRCS = [PKG / "host" / n for n in ("hig_input.cpp", "hig_sim.cpp", "hig_fit.cpp", "hull.cpp")]
 HIG_LIB = PKG / "libhgx_hig.so"
 SHIM_SRC = ROOT / "tests" / "cpp" / "hig_adapter_shim.cpp"
 SHIM_LIB = ROOT / "tests" / "cpp" / "libhig_shim.so"

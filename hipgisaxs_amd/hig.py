@@ -48,6 +48,7 @@ _SIGS = {
     "hgx_hig_fit_run": (C.c_int, [_vp, C.c_int, C.c_ulonglong, _dp, _dp, C.POINTER(C.c_longlong)]),
     "hgx_hig_fit_history": (C.c_int, [_vp, C.c_longlong, _dp, C.POINTER(C.c_longlong)]),
     "hgx_hig_last_error": (C.c_char_p, [_vp]),
+    "hgx_hig_convex_hull": (C.c_int64, [C.c_int64, _dp, C.c_int64, C.POINTER(C.c_int)]),
 }
 _lib = None
 
@@ -87,6 +88,21 @@ def read_mesh(path: str):
     if rc != 0:
         raise HgxError(f"hgx_hig_read_mesh: {N.ERRORS.get(rc, rc)}")
     return ("exact9" if w == 9 else "approx8"), rec
+
+
+def convex_hull(points) -> np.ndarray:
+    """hgx_hig_convex_hull: triangles [ntri][3] (point indices, counter-clockwise seen from outside)
+    of the convex hull of points [n][3]."""
+    lib = load()
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    nt = lib.hgx_hig_convex_hull(p.shape[0], p.ctypes.data_as(_dp), 0, None)
+    if nt < 0:
+        raise HgxError(f"hgx_hig_convex_hull: {N.ERRORS.get(nt, nt)}")
+    tri = np.zeros((nt, 3), dtype=np.int32)
+    got = lib.hgx_hig_convex_hull(p.shape[0], p.ctypes.data_as(_dp), nt, tri.ctypes.data_as(C.POINTER(C.c_int)))
+    if got != nt:
+        raise HgxError("hgx_hig_convex_hull: inconsistent triangle count")
+    return tri
 
 
 class HigSimulation:

@@ -121,16 +121,32 @@ def orient_outward(tri: np.ndarray, inside_point) -> np.ndarray:
 
 def convex_polyhedron(rng: np.random.Generator, semi_axes, target_tri: int) -> np.ndarray:
     """Convex hull of points on an ellipsoid (about target_tri triangles: a hull of n points in
-    general position has 2n - 4 faces)."""
-    from scipy.spatial import ConvexHull
+    general position has 2n - 4 faces), by the library's quickhull (hgx_hig_convex_hull)."""
+    from hipgisaxs_amd.hig import convex_hull
 
     n = max(4, int(round(target_tri / 2 + 2)))
     x = rng.standard_normal((n, 3))
     x /= np.linalg.norm(x, axis=1, keepdims=True)
     x *= np.asarray(semi_axes, dtype=np.float64)
-    hull = ConvexHull(x)
-    tri = x[hull.simplices].reshape(-1, 9)
+    tri = x[convex_hull(x)].reshape(-1, 9)
     return orient_outward(tri, x.mean(0))
+
+
+def c3_shapes(nshapes: int = 64, seed: int = 20261206):
+    """The batched-polyhedra workload: per shape, semi-axes U[3, 15]^3 nm, a target triangle count
+    log-uniform on [500, 50000], a square grid side from {128, 256, 512, 1024}, and the hull of points
+    on that ellipsoid. Returns dicts with tri, side, axes and target."""
+    import math
+
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for _ in range(nshapes):
+        axes = rng.uniform(3.0, 15.0, 3)
+        target = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
+        side = int(rng.choice([128, 256, 512, 1024]))
+        tri = convex_polyhedron(rng, axes, target)
+        shapes.append({"tri": tri, "side": side, "axes": axes, "target": target})
+    return shapes
 
 
 def mesh_volume(tri: np.ndarray) -> float:

@@ -77,6 +77,12 @@ int hgx_hig_save_tiff(const double* image, int ncol, int nrow, const char* path)
  * (RawShapeReader, src/file/rawshape_reader.cpp:39-62). Writes *kind and *ntri, and the records
  * into `records` when cap >= *ntri (pass cap = 0 to query the size). */
 int hgx_hig_read_mesh(const char* path, int* kind, int64_t* ntri, int64_t cap, double* records);
+/* Convex hull of n points (host [n][3]) by quickhull: returns the number of triangles and writes up
+ * to cap of them into tri [cap][3] (point indices, counter-clockwise seen from outside; tri may be
+ * NULL to count), or HGX_ERR_INVALID_ARG for fewer than 4 points, non-finite or coplanar input. Host
+ * only. No counterpart in the reference (its polyhedra come from mesh files); the generator of convex
+ * test shapes. */
+int64_t hgx_hig_convex_hull(int64_t n, const double* pts, int64_t cap, int* tri);
 /* ---- fitting (the analyzer's objective and fit drivers on the device) ------------------------- */
 
 /* HiGInput::update_params for one value (src/config/hig_input.cpp:2712-2832): the model value a fit
