@@ -716,6 +716,8 @@ def main():
     for s in shapes:
         s["mid"] = main_ctx.upload_mesh(s["tri"], "exact9")
         s["ntri"] = main_ctx.mesh_info(s["mid"])["ntri"]
+        if args.mode == "per-shape":  # per-shape calls run in the context that holds the shape's grid
+            s["mid_grid"] = grid_ctx[s["side"]].upload_mesh(s["tri"], "exact9")
         s["out32"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
         s["out64"] = torch.empty(2 * s["side"] ** 2, dtype=torch.float64, device=dev)
     pairs = float(sum(s["ntri"] * s["side"] ** 2 for s in shapes))
@@ -760,7 +762,7 @@ def main():
             return lambda graph: b.run(use_graph=graph), b
         def per_shape(graph):
             for s in shapes:
-                grid_ctx[s["side"]].ff_numeric(s["mid"], "approx", prec, flags=flags if prec == "fp32c" else 0,
+                grid_ctx[s["side"]].ff_numeric(s["mid_grid"], "approx", prec, flags=flags if prec == "fp32c" else 0,
                                                out=s["out32" if prec == "fp32c" else "out64"])
         return per_shape, None
 
