@@ -475,7 +475,9 @@ def bench_grating(ncand=256, iters=1):
     ctx.close()
     nslots = (256 // 8 * 208 // 8 + 2 * 208 // 16) * 128
     pairs = 4 * npix * nslots  # forward (q-point x triangle slot) pairs per candidate
-    mfma_flop = (48 + 54) * pairs * ncand  # forward (48 flop) + adjoint (54 flop) of matrix work per pair
+    # matrix work per pair: forward 48 flop (two complex products, 4 real products each, as 3 split-fp16
+    # products of 2 flop); adjoint 72 flop (three passes f = 1, qz, qz^2 of one complex product each)
+    mfma_flop = (48 + 72) * pairs * ncand
     return {"grad_evals_per_s": ncand / t_iter, "ms_per_iteration": 1e3 * t_iter, "candidates": ncand,
             "roofline": {"bound": "mfma", "achieved": mfma_flop / t_iter / 1e12, "peak": PEAK_FP16 / 1e12,
                          "unit": "TFLOP/s", "frac": mfma_flop / t_iter / PEAK_FP16,

@@ -457,9 +457,9 @@ def tune_product64(ctx: "Context", complex_mults: int = 4) -> None:
         raise HgxError("tune_product64: complex_mults must be 3 or 4")
 
 
-def tune_adjoint(ctx: "Context", operands: str = "f16x2", col_tiles: int = 2) -> None:
-    """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default; 2 or 4
-    32-column tiles per wave) or "fp32"."""
+def tune_adjoint(ctx: "Context", operands: str = "f16x2", col_tiles: int = 4) -> None:
+    """Operands of the FP32C product-grid record adjoint: "f16x2" (split fp16, default; 4 (default)
+    or 2 32-column tiles per wave) or "fp32"."""
     ops = {"fp32": N.PRODUCT_FP32, "f16x2": N.PRODUCT_F16X2}
     if operands not in ops or N.load().hgx_tune_adjoint(ctx.h, ops[operands], int(col_tiles)) != N.HGX_OK:
         raise HgxError("tune_adjoint: invalid operands / column tiles")

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt
   if (zo == 0) wexp[item * wexp_stride + y] = sw;
   const float sc = __builtin_ldexpf(1.0f, sw);
   const float* w = wt + item * wt_stride;
-  float f[3][8];
+  float f[2][8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int z = 8 * zo + e;
@@ -430,11 +430,10 @@ __global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt
     }
     f[0][e] = wr;
     f[1][e] = wi;
-    f[2][e] = wr + wi;
   }
   uint4* o = w16 + item * w16_stride;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 2; ++k) {
     h16x8 hi, lo;
     adj_split8(f[k], -1.0f, hi, lo);
     o[((int64_t)(2 * k) * nzo + zo) * ncol + y] = __builtin_bit_cast(uint4, hi);
@@ -442,17 +441,27 @@ __global__ __launch_bounds__(256) void k_adj_wsplit(const float* __restrict__ wt
   }
 }
 
+// -x for 8 packed fp16 values: flip each sign bit
+__device__ inline h16x8 adj_neg8(h16x8 x) {
+  uint4 u = __builtin_bit_cast(uint4, x);
+  u.x ^= 0x80008000u;
+  u.y ^= 0x80008000u;
+  u.z ^= 0x80008000u;
+  u.w ^= 0x80008000u;
+  return __builtin_bit_cast(h16x8, u);
+}
+
 __device__ inline f32x16a mfma16a(h16x8 a, h16x8 b, f32x16a c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
 template <bool CQ, int NY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 : 2, 8))) void k_adjoint_product16(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adjoint_product16(
     const AdjItem* __restrict__ items, AdjItem single, int upi) {
   constexpr int kAdjNY = NY;
   constexpr int kAdjYG = kAdjTile * NY;
-  // weights of one step: [buffer][form * 2 + half][z octet h][column]; per-z data of 256 z
-  __shared__ uint4 s_w[2][6][2][kAdjYG];
+  // weights of one step: [buffer][part * 2 + half][z octet h][column]; per-z data of 256 z
+  __shared__ uint4 s_w[2][kAdjW16Planes][2][kAdjYG];
   __shared__ float s_zb[2][kAdjZTab], s_zu[2][kAdjZTab], s_zfr[2][kAdjZTab], s_zfi[2][kAdjZTab];
   __shared__ float s_zab[2][CQ ? kAdjZTab : 1], s_zaz[2][CQ ? kAdjZTab : 1];
   __shared__ float s_red[256];
@@ -471,8 +480,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
   const double oz = it.borig[4 * blk + 2];
   const int y0 = yg * kAdjYG;
 
-  // A scale 2^sa: |f(z)| att <= 2^(14 - sa) / sqrt(2) over all z (the Gauss sum Ar + Ai is up to
-  // sqrt(2) |A|), with att <= exp(|Im qz| (|oz| + max |c_z - oz|)) over the block's triangles
+  // A scale 2^sa: |f(z)| att <= 2^(14 - sa) / sqrt(2) over all z, with
+  // att <= exp(|Im qz| (|oz| + max |c_z - oz|)) over the block's triangles
   float czmax = 0.0f;
   if (CQ) {
     s_red[tid] = tid < kBlockTri ? fabsf(it.rec32[8 * ((int64_t)blk * kBlockTri + tid) + 5]) : 0.0f;
@@ -537,8 +546,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
       s_zaz[buf][tid] = zaz;
     }
   };
-  // one step's weights: 6 forms-halves x 2 z octets x kAdjYG columns, kWPT uint4 per thread
-  constexpr int kWPT = 12 * kAdjYG / 256;
+  // one step's weights: 4 parts-halves x 2 z octets x kAdjYG columns, kWPT uint4 per thread
+  constexpr int kWPT = 2 * kAdjW16Planes * kAdjYG / 256;
   uint4 wv[kWPT];
   auto fetch = [&](int ks) {
 #pragma unroll
@@ -562,9 +571,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
   stage(0);
   __syncthreads();
 
-  f32x16a P[3][kAdjNY];
+  // P[0] = Re sum_z A B, P[1] = Im sum_z A B: four real products per complex product
+  f32x16a P[2][kAdjNY];
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 2; ++k)
 #pragma unroll
     for (int j = 0; j < kAdjNY; ++j)
 #pragma unroll
@@ -578,7 +588,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
     if (more) fetch(ks + 1);  // in flight during this step's work
     if (newz) ztable(chunk + 1);
     // A operands: triangle t0 + r at z = 16 ks + 8 h + e
-    float ar[3][8];
+    float ar[2][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int zl = (ks % (kAdjZTab / 16)) * 16 + 8 * h + e;
@@ -594,21 +604,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
       const float re = CQ ? fr * c - fi * sn : fr * c, im = CQ ? fr * sn + fi * c : fr * sn;
       ar[0][e] = re;
       ar[1][e] = im;
-      ar[2][e] = re + im;
     }
-    h16x8 AH[3], AL[3];
+    h16x8 AH[2], AL[2];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) adj_split8(ar[k], neg1, AH[k], AL[k]);
+    for (int k = 0; k < 2; ++k) adj_split8(ar[k], neg1, AH[k], AL[k]);
+    // -Ai: the halves of -x are the negated halves of x
+    const h16x8 NH = adj_neg8(AH[1]), NL = adj_neg8(AL[1]);
 #pragma unroll
-    for (int j = 0; j < kAdjNY; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const h16x8 BH = __builtin_bit_cast(h16x8, s_w[cur][2 * k][h][kAdjTile * j + r]);
-        const h16x8 BL = __builtin_bit_cast(h16x8, s_w[cur][2 * k + 1][h][kAdjTile * j + r]);
-        P[k][j] = mfma16a(AH[k], BH, P[k][j]);
-        P[k][j] = mfma16a(AL[k], BH, P[k][j]);
-        P[k][j] = mfma16a(AH[k], BL, P[k][j]);
-      }
+    for (int j = 0; j < kAdjNY; ++j) {
+      const h16x8 BRH = __builtin_bit_cast(h16x8, s_w[cur][0][h][kAdjTile * j + r]);
+      const h16x8 BRL = __builtin_bit_cast(h16x8, s_w[cur][1][h][kAdjTile * j + r]);
+      const h16x8 BIH = __builtin_bit_cast(h16x8, s_w[cur][2][h][kAdjTile * j + r]);
+      const h16x8 BIL = __builtin_bit_cast(h16x8, s_w[cur][3][h][kAdjTile * j + r]);
+      // Re: Ar Br - Ai Bi; Im: Ar Bi + Ai Br; each real product as hi hi + lo hi + hi lo
+      P[0][j] = mfma16a(AH[0], BRH, P[0][j]);
+      P[1][j] = mfma16a(AH[0], BIH, P[1][j]);
+      P[0][j] = mfma16a(AL[0], BRH, P[0][j]);
+      P[1][j] = mfma16a(AL[0], BIH, P[1][j]);
+      P[0][j] = mfma16a(AH[0], BRL, P[0][j]);
+      P[1][j] = mfma16a(AH[0], BIL, P[1][j]);
+      P[0][j] = mfma16a(NH, BIH, P[0][j]);
+      P[1][j] = mfma16a(AH[1], BRH, P[1][j]);
+      P[0][j] = mfma16a(NL, BIH, P[0][j]);
+      P[1][j] = mfma16a(AL[1], BRH, P[1][j]);
+      P[0][j] = mfma16a(NH, BIL, P[0][j]);
+      P[1][j] = mfma16a(AH[1], BRL, P[1][j]);
+    }
     // the other buffers' last reads were in the previous step (weights) or chunk (per-z data), which
     // every wave finished before the barrier that ended it
     if (more) stage(cur ^ 1);
@@ -636,8 +657,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 2 ? 1 
       const double ph = (qx0 * it.rec[8 * tc + 3] + qyv * it.rec[8 * tc + 4]) * kInvTwoPi;
       const float fph = (float)(ph - rint(ph));
       const double eyr = (double)__builtin_amdgcn_cosf(fph), eyi = (double)__builtin_amdgcn_sinf(fph);
-      const double p1 = P[0][j][k], p2 = P[1][j][k], p3 = P[2][j][k];
-      const double xr = (p1 - p2) * unscale, xi = (p3 - p1 - p2) * unscale;
+      const double xr = (double)P[0][j][k] * unscale, xi = (double)P[1][j][k] * unscale;
       double yr = yok ? xr * eyr - xi * eyi : 0.0;
       double yi = yok ? xr * eyi + xi * eyr : 0.0;
       for (int f = 0; f < nform; ++f) {
@@ -736,7 +756,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   const int nz = c->nbranch * c->nrow;
   const int nzo = (int)ceil_div(nz, 8);
   const size_t pm_b = f16 ? (size_t)adjoint_pmax_count(nz, c->ncol) * sizeof(float) : 0;
-  const size_t w16_b = f16 ? (size_t)6 * nzo * c->ncol * sizeof(uint4) : 0;
+  const size_t w16_b = f16 ? (size_t)kAdjW16Planes * nzo * c->ncol * sizeof(uint4) : 0;
   const size_t we_b = f16 ? (size_t)c->ncol * sizeof(int) : 0;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   if (int rc = ensure_work(c, al(rec_b) + w0_b + al(wt_b) + al(part_b) + al(r32_b) + al(pm_b) + al(w16_b) + we_b))

@@ -709,7 +709,7 @@ void group_eval(const GratingPlan& p, Context* c, hipStream_t s, int cand0, int 
   if (p.adj_f16) {
     const int nz = c->nbranch * c->nrow;
     adjoint_split_weights(s, gsize, p.d_wt, 2 * c->nq, nz, c->ncol, p.d_pmax, adjoint_pmax_count(nz, c->ncol), p.d_w16,
-                          6 * (int64_t)ceil_div(nz, 8) * c->ncol, p.d_wexp, c->ncol);
+                          kAdjW16Planes * (int64_t)ceil_div(nz, 8) * c->ncol, p.d_wexp, c->ncol);
   }
   adjoint_product_launch(s, p.d_aitems, none, p.upi_adj, gsize * p.upi_adj, p.cq, p.adj_variant);
   dim3 ggrad(p.ngb, gsize);
@@ -861,7 +861,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
   if (hipMalloc(&p.d_wt, sizeof(float) * 2 * c->nq * G) != hipSuccess) return fail("weights");
   if (p.adj_f16) {
     if (hipMalloc(&p.d_pmax, sizeof(float) * adjoint_pmax_count(nz, c->ncol) * G) != hipSuccess) return fail("weight maxima");
-    if (hipMalloc(&p.d_w16, sizeof(uint4) * 6 * (size_t)nzo * c->ncol * G) != hipSuccess) return fail("split weights");
+    if (hipMalloc(&p.d_w16, sizeof(uint4) * kAdjW16Planes * (size_t)nzo * c->ncol * G) != hipSuccess) return fail("split weights");
     if (hipMalloc(&p.d_wexp, sizeof(int) * (size_t)c->ncol * G) != hipSuccess) return fail("weight scales");
   }
   if (hipMalloc(&p.d_apart, sizeof(double) * 2 * 6 * p.nyt * gm.nslots * G) != hipSuccess) return fail("adjoint");
@@ -902,7 +902,7 @@ extern "C" int hgx_grating_plan_create(hgx_ctx* h, const hgx_grating_model* mode
     a.nz = nz;
     a.nyt = p.nyt;
     a.nzo = nzo;
-    a.w16 = p.adj_f16 ? p.d_w16 + (int64_t)i * 6 * nzo * c->ncol : nullptr;
+    a.w16 = p.adj_f16 ? p.d_w16 + (int64_t)i * kAdjW16Planes * nzo * c->ncol : nullptr;
     a.wexp = p.adj_f16 ? p.d_wexp + (int64_t)i * c->ncol : nullptr;
   }
   // forward units, item-major so a short last group runs a prefix

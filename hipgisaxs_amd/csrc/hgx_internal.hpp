@@ -131,6 +131,9 @@ struct ProductUnit {
   int chunk;
 };
 
+// Planes of the split-fp16 adjoint weights: the real and imaginary parts, each as high and low fp16.
+constexpr int kAdjW16Planes = 4;
+
 // One mesh for the product-grid adjoint kernel (adjoint.hip).
 struct AdjItem {
   const double* qx;     // grid arrays: qx[0] = qx0, qy[0..ncol) the qy axis, qz row z at qz + 2 z ncol
@@ -141,7 +144,7 @@ struct AdjItem {
   const float* rec32;   // [nblocks * kBlockTri][8] rotated FP32 records (A', c' - o_b)
   const double* borig;  // [nblocks][4] rotated block origins
   double* part;         // [6 forms][nyt][ntri] complex: B11, B1y, B1yy, Bz1, Bzy, Bzz per y-tile
-  const uint4* w16;     // split-fp16 weights: [3 forms][2 halves][nzo][ncol] of 8 fp16 (one z octet, one column)
+  const uint4* w16;     // split-fp16 weights: [kAdjW16Planes = re, im x hi, lo][nzo][ncol] of 8 fp16 (one z octet, one column)
   const int* wexp;      // [ncol] power-of-two column scales of w16
   int64_t ntri;
   int nblocks;
@@ -196,7 +199,7 @@ struct Tuning {
   int product_operands = HGX_PRODUCT_F16X2;
   int product_mults = 4;
   int product_mz = 2, product_two = 1, product_shared = 1;
-  int adjoint_variant = 2;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
+  int adjoint_variant = 4;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
   int product64_mults = 4;  // FP64 product-grid kernel: 4 or 3 (Gauss) real products per complex product
 };
 

@@ -538,8 +538,9 @@ int hgx_tune_product64(hgx_ctx* ctx, int complex_mults);
 int hgx_tune_product_tile(hgx_ctx* ctx, int row_tiles, int two_level, int shared_rows);
 
 /* Operands of the FP32C product-grid record adjoint (hgx_ff_numeric_adjoint and the grating fit):
- * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16) with col_tiles = 2 (default) or 4
- * 32-column tiles per wave, or HGX_PRODUCT_FP32 (v_mfma_f32_32x32x2_f32; col_tiles ignored).
+ * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16, 4 real products per complex
+ * product) with col_tiles = 4 (default) or 2 32-column tiles per wave, or HGX_PRODUCT_FP32
+ * (v_mfma_f32_32x32x2_f32; col_tiles ignored).
  * Read by each adjoint call of the context and at grating plan creation. No reference counterpart. */
 int hgx_tune_adjoint(hgx_ctx* ctx, int operands, int col_tiles);
 
