@@ -196,6 +196,7 @@ _SIGS = {
     "hgx_tune_adjoint": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hgx_caxpy": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_double, _vp, C.c_int, _vp]),
     "hgx_axpy": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_int, _vp]),
+    "hgx_axpy_sq": (C.c_int, [_vp, C.c_int64, C.c_double, _vp, C.c_int, _vp]),
     "hgx_gaussian_smear": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),
     "hgx_image_distance": (C.c_int, [_vp, C.c_int64, C.c_int, _vp, _vp, _vp, C.POINTER(C.c_double)]),
     "hgx_image_distance_async": (C.c_int, [_vp, C.c_int64, C.c_int, _vp, _vp, _vp, _vp]),

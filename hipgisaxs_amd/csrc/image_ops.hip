@@ -33,6 +33,15 @@ __global__ __launch_bounds__(256) void k_axpy(int64_t n, double a, const double*
   y[i] = accumulate ? y[i] + v : v;
 }
 
+// y = a x^2 (+ y), elementwise; x may be y (no restrict)
+__global__ __launch_bounds__(256) void k_axpy_sq(int64_t n, double a, const double* x, bool accumulate, double* y) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i];
+  const double v = a * xi * xi;
+  y[i] = accumulate ? y[i] + v : v;
+}
+
 // One 1-D pass of the smearing: out[line, i] = sum_k w_k in[line, k] / sum_k w_k over the window
 // around i, for every line of the image. (stride_i, stride_line) select rows or columns, so one
 // kernel serves both passes; consecutive threads take consecutive elements of the contiguous
@@ -91,6 +100,18 @@ extern "C" int hgx_axpy(hgx_ctx* h, int64_t n, double a, const double* x, int ac
   if (int rc = set_device(c)) return rc;
   LaunchTimer lt(c, false);
   k_axpy<<<ceil_div(n, 256), 256, 0, c->stream>>>(n, a, x, accumulate != 0, y);
+  HGX_CHECK_LAUNCH(c);
+  return HGX_OK;
+}
+
+extern "C" int hgx_axpy_sq(hgx_ctx* h, int64_t n, double a, const double* x, int accumulate, double* y) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (n < 0 || (n > 0 && (!x || !y))) return set_error(c, HGX_ERR_INVALID_ARG, "axpy_sq: bad arguments");
+  if (n == 0) return HGX_OK;
+  if (int rc = set_device(c)) return rc;
+  LaunchTimer lt(c, false);
+  k_axpy_sq<<<ceil_div(n, 256), 256, 0, c->stream>>>(n, a, x, accumulate != 0, y);
   HGX_CHECK_LAUNCH(c);
   return HGX_OK;
 }

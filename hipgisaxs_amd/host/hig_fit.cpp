@@ -662,14 +662,6 @@ struct Problem {
   }
 };
 
-// The uniform draw of woo::MTRandomNumberGenerator (include/woo/random/woo_mtrandom.hpp:28-33):
-// (mt19937_64() - min) / (max - min).
-struct MtUniform {
-  std::mt19937_64 g;
-  explicit MtUniform(uint64_t seed) : g(seed) {}
-  double next() { return (double)(g() - std::mt19937_64::min()) / (double)(std::mt19937_64::max() - std::mt19937_64::min()); }
-};
-
 // ParticleSwarmOptimization, base (inertia weight) flavour: particles drawn uniformly in the
 // limits with velocities uniform in [-max, max] (PSOParticle::init_random_uniform), each generation
 // evaluated particle by particle in index order, personal and global bests updated after each

@@ -441,6 +441,35 @@ bool HigInput::read_layer(const HigNode& n) {
   return true;
 }
 
+// One axis of a grain scaling distribution: { stat, mean, stddev, nsamples } (the mean / stddev /
+// nsamples / stat tokens under a lattice axis, src/config/hig_input.cpp:1206-1263, 1642-1652).
+bool HigInput::scaling_axis(const HigNode& n, int ax, HigStructure& s) {
+  for (const auto& f : n.fields) {
+    if (f.first == "stat") {
+      std::string st;
+      if (!string(f.second, st)) return false;
+      s.scale_stat[ax] = (st == "none" || st == "single") ? HigStructure::kStatNone
+                         : st == "gaussian"              ? HigStructure::kStatGaussian
+                         : st == "cauchy"                ? HigStructure::kStatCauchy
+                                                         : HigStructure::kStatOther;
+    } else if (f.first == "mean") {
+      if (!number(f.second, s.scaling[ax])) return false;
+    } else if (f.first == "stddev") {
+      if (!number(f.second, s.scale_sd[ax])) return false;
+    } else if (f.first == "nsamples") {
+      double x;
+      if (!number(f.second, x)) return false;
+      if (x < 1 || x > 100000 || x != std::floor(x)) return fail(f.second, "nsamples must be a positive integer");
+      s.scale_nvals[ax] = (int)x;
+    } else {
+      return fail(f.second, "unknown keyword '" + f.first + "' in a scaling distribution");
+    }
+  }
+  s.scaling_dist = false;
+  for (int i = 0; i < 3; ++i) s.scaling_dist = s.scaling_dist || s.scale_stat[i] != HigStructure::kStatNone;
+  return true;
+}
+
 bool HigInput::read_grain(const HigNode& n, HigStructure& s) {
   if (n.kind != HigNode::kObject) return fail(n, "grain must be an object");
   for (const auto& f : n.fields) {
@@ -466,6 +495,8 @@ bool HigInput::read_grain(const HigNode& n, HigStructure& s) {
           } else if (!string(g.second, s.hkl)) {
             return false;
           }
+        } else if ((g.first == "a" || g.first == "b" || g.first == "c") && g.second.kind == HigNode::kObject) {
+          if (!scaling_axis(g.second, g.first[0] - 'a', s)) return false;
         } else if (g.first == "a" || g.first == "b" || g.first == "c") {
           vec3& t = g.first == "a" ? s.a : g.first == "b" ? s.b : s.c;
           if (!vector(g.second, 3, t.data())) return false;
@@ -480,14 +511,41 @@ bool HigInput::read_grain(const HigNode& n, HigStructure& s) {
     } else if (k == "repetition") {
       if (!vector(v, 3, s.repetition.data())) return false;
     } else if (k == "repetitiondist") {
+      // { xrepetition = { stat, min, max }, y..., z... } (src/config/hig_input.cpp:248-262, 924-990)
+      if (v.kind != HigNode::kObject) return fail(v, "repetitiondist must be an object");
       s.repetition_dist = true;
+      for (const auto& r : v.fields) {
+        const int ax = r.first == "xrepetition" ? 0 : r.first == "yrepetition" ? 1 : r.first == "zrepetition" ? 2 : -1;
+        if (ax < 0) return fail(r.second, "unknown keyword '" + r.first + "' in repetitiondist");
+        if (r.second.kind != HigNode::kObject) return fail(r.second, r.first + " must be an object");
+        for (const auto& q : r.second.fields) {
+          if (q.first == "stat") {
+            std::string st;
+            if (!string(q.second, st)) return false;
+            s.rep_stat[ax] = (st == "none" || st == "single") ? 0 : st == "range" ? 1 : 2;
+          } else if (q.first == "min" || q.first == "max") {
+            double x;
+            if (!number(q.second, x)) return false;
+            if (x < 0 || x > 4294967295.0) return fail(q.second, "repetition bounds must be non-negative integers");
+            (q.first == "min" ? s.rep_min : s.rep_max)[ax] = std::floor(x);  // unsigned in the reference
+          } else if (q.first != "mean" && q.first != "stddev") {
+            return fail(q.second, "unknown keyword '" + q.first + "' in " + r.first);
+          }
+        }
+      }
     } else if (k == "scaling") {
       if (v.kind == HigNode::kNumber) {
         s.scaling = {v.num, v.num, v.num};
       } else if (v.kind == HigNode::kArray) {
         if (!vector(v, 3, s.scaling.data())) return false;
+      } else if (v.kind == HigNode::kObject) {
+        for (const auto& a : v.fields) {
+          if ((a.first != "a" && a.first != "b" && a.first != "c") || a.second.kind != HigNode::kObject)
+            return fail(a.second, "scaling distributions are given per axis: a|b|c = { stat, mean, stddev, nsamples }");
+          if (!scaling_axis(a.second, a.first[0] - 'a', s)) return false;
+        }
       } else {
-        s.scaling_dist = true;
+        return fail(v, "scaling can be a scalar, vector[3] or a distribution");
       }
     } else if (k == "transvec") {
       if (!vector(v, 3, s.transvec.data())) return false;

@@ -80,9 +80,20 @@ struct HigStructure {
   vec3 a{1.0, 0.0, 0.0}, b{0.0, 1.0, 0.0}, c{0.0, 0.0, 1.0};
   vec3 transvec{0.0, 0.0, 0.0};
   vec3 repetition{1.0, 1.0, 1.0};
-  vec3 scaling{1.0, 1.0, 1.0};
-  bool scaling_dist = false;
+  vec3 scaling{1.0, 1.0, 1.0};  // grain scaling means
+  // Grain scaling distribution per lattice axis (GrainScaling, src/model/structure.cpp:35-46): the
+  // statistic (none, gaussian or cauchy), its width and sample count. Set by `a|b|c = { stat, mean,
+  // stddev, nsamples }` inside the grain's lattice or scaling block.
+  enum { kStatNone = 0, kStatGaussian = 1, kStatCauchy = 2, kStatOther = 3 };
+  int scale_stat[3] = {kStatNone, kStatNone, kStatNone};
+  vec3 scale_sd{0.0, 0.0, 0.0};
+  int scale_nvals[3] = {40, 40, 40};
+  bool scaling_dist = false;  // some axis has a statistic other than none
+  // Grain repetition distribution (GrainRepetitions, include/model/structure.hpp:281-330): per axis
+  // unset (-1), none (0, the minimum), range (1) or another statistic (2), with its integer bounds.
   bool repetition_dist = false;
+  int rep_stat[3] = {-1, -1, -1};
+  double rep_min[3] = {0.0, 0.0, 0.0}, rep_max[3] = {0.0, 0.0, 0.0};
   // ensemble (src/model/structure.cpp:385-390, 236-246 defaults)
   vec3 spacing{0.0, 0.0, 0.0};
   vec3 maxgrains{1.0, 1.0, 1.0};
@@ -184,6 +195,7 @@ class HigInput {
   bool read_layer(const HigNode& n);
   bool read_structure(const HigNode& n);
   bool read_grain(const HigNode& n, HigStructure& s);
+  bool scaling_axis(const HigNode& n, int ax, HigStructure& s);
   bool read_ensemble(const HigNode& n, HigStructure& s);
   bool read_instrumentation(const HigNode& n);
   bool read_computation(const HigNode& n);

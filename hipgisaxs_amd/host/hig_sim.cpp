@@ -531,7 +531,9 @@ std::vector<double> HipGISAXS::iratios() const {
 }
 
 bool HipGISAXS::ensemble_model(const HigStructure& s, const std::vector<HigGrain>& grains, EnsembleBuild& b,
-                               bool report) {
+                               bool report, const vec3* scaling, const vec3* repet) {
+  const vec3& sc = scaling ? *scaling : s.scaling;
+  const vec3& rp = repet ? *repet : s.repetition;
   auto no = [&](const std::string& why) { return report ? fail(why) : false; };
   auto uit = in_.unitcells.find(s.unitcell_key);
   if (uit == in_.unitcells.end()) return no("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
@@ -569,10 +571,10 @@ bool HipGISAXS::ensemble_model(const HigStructure& s, const std::vector<HigGrain
   std::memset(&b.m, 0, sizeof(b.m));
   // StructureFactor::compute_structure_factor scales the vectors (src/sf/sf.cpp:165-171)
   for (int i = 0; i < 3; ++i) {
-    b.m.la[i] = a[i] * s.scaling[0];
-    b.m.lb[i] = bb[i] * s.scaling[1];
-    b.m.lc[i] = c[i] * s.scaling[2];
-    b.m.repet[i] = s.repetition[i];
+    b.m.la[i] = a[i] * sc[0];
+    b.m.lb[i] = bb[i] * sc[1];
+    b.m.lc[i] = c[i] * sc[2];
+    b.m.repet[i] = rp[i];
   }
   b.m.dw_sigma = 0.0;
   b.m.sf_flags = (flags_ & HGX_HIG_SF_COMPAT_LB2) ? HGX_SF_COMPAT_LB2 : 0;
@@ -631,8 +633,118 @@ bool HipGISAXS::fresnel_coeffs(double alphai, double** d_out) {
   return true;
 }
 
-bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains,
+namespace {
+
+// HipGISAXS::gaussian / cauchy (include/sim/hipgisaxs_main.hpp:151-172), with their constants
+double scaling_pdf(int stat, double x, double mean, double sd) {
+  const double kTiny = DBL_MIN, kSqrt2Pi = 2.506628, kPi = 3.141592653589793;
+  if (stat == HigStructure::kStatGaussian) {
+    if (sd < kTiny) return (x - mean < kTiny) ? 1.0 : 0.0;
+    return 1.0 / (sd * kSqrt2Pi) * std::exp(-(x - mean) * (x - mean) / (2 * sd * sd));
+  }
+  if (sd > kTiny) return 1.0 / (kPi * sd * (1.0 + ((x - mean) / sd) * ((x - mean) / sd)));
+  return (x - mean > kTiny) ? 0.0 : 1.0;
+}
+
+}  // namespace
+
+bool HipGISAXS::grain_sets(const HigStructure& s, const std::vector<HigGrain>& grains, std::vector<GrainSet>& out) {
+  out.clear();
+  const int ng = (int)grains.size();
+  // construct_scaling_distribution (src/sim/hipgisaxs_main.cpp:819-841, 2179-2226): each axis gives
+  // its mean (no statistic) or max_nvals samples in [mean - 4 sd, mean + 4 sd], one per cell of
+  // width 8 sd / (max_nvals + 1) at a uniform offset from one generator seeded 0, weighted by the pdf;
+  // an axis whose width is zero counts as one sample in max_nvals
+  int nvals[3];
+  for (int i = 0; i < 3; ++i)
+    nvals[i] = s.scale_stat[i] != HigStructure::kStatNone ? (s.scale_sd[i] == 0 ? 1 : s.scale_nvals[i]) : 1;
+  const int maxn = std::max(nvals[0], std::max(nvals[1], nvals[2]));
+  MtUniform rng(0);
+  std::vector<double> smp, wts;
+  for (int i = 0; i < 3; ++i) {
+    const int st = s.scale_stat[i];
+    if (st == HigStructure::kStatNone) {
+      smp.push_back(s.scaling[i]);
+      wts.push_back(1.0);
+    } else if (st == HigStructure::kStatGaussian || st == HigStructure::kStatCauchy) {
+      const double lo = s.scaling[i] - 4 * s.scale_sd[i], dx = 8 * s.scale_sd[i] / (maxn + 1);
+      for (int j = 0; j < maxn; ++j) {
+        const double t = lo + j * dx + rng.next() * dx;
+        smp.push_back(t);
+        wts.push_back(scaling_pdf(st, t, s.scaling[i], s.scale_sd[i]));
+      }
+    } else {
+      return fail("structure '" + s.key + "': unsupported distribution for scaling (none, gaussian or cauchy)");
+    }
+  }
+  // construct_repetition_distribution (:843-857, 2089-2176): per axis the minimum, or n values
+  // min + u (max - min) truncated, u from a generator seeded 0 for each axis; the axes are zipped to the
+  // shortest. n is the size of the scaling sample list (3 values even without a distribution), or the
+  // grain count for a list of one.
+  std::vector<vec3> reps;
+  if (s.repetition_dist) {
+    const int nr = smp.size() > 1 ? (int)smp.size() : ng;
+    std::vector<double> vals[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      if (s.rep_stat[ax] == 0) {
+        vals[ax].push_back(s.rep_min[ax]);
+      } else if (s.rep_stat[ax] == 1) {
+        if (s.rep_max[ax] < s.rep_min[ax])
+          return fail("structure '" + s.key + "': repetition range with max < min");
+        MtUniform r(0);
+        for (int k = 0; k < nr; ++k)
+          vals[ax].push_back(std::floor(s.rep_min[ax] + r.next() * (s.rep_max[ax] - s.rep_min[ax])));
+      } else {
+        return fail("structure '" + s.key + "': grain repetition distributions take stat none or range per axis");
+      }
+    }
+    const size_t n = std::min(vals[0].size(), std::min(vals[1].size(), vals[2].size()));
+    for (size_t k = 0; k < n; ++k) reps.push_back(vec3{vals[0][k], vals[1][k], vals[2][k]});
+  } else {
+    reps.push_back(s.repetition);
+  }
+  // the loop (:859-868, 1031-1061): grain scaling from the sample list at i, i + n, i + 2n (n = list
+  // size / 3) when some width is positive, else its first three entries, with weight entry i (or 0);
+  // repetitions by grain index modulo their count
+  const int nsc = (int)smp.size() / 3, nrep = (int)reps.size();
+  int nrs = 0;
+  if (nsc == nrep) nrs = nsc;
+  else if (nsc == 1) nrs = nrep;
+  else if (nrep == 1) nrs = nsc;
+  else return fail("structure '" + s.key + "': scaling and repetition distributions are not aligned");
+  bool isdist = false;
+  for (int i = 0; i < 3; ++i) isdist = isdist || s.scale_sd[i] > 0;
+  const int nclass = s.repetition_dist ? nrep : 1;
+  for (int is = 0; is < nrs; ++is) {
+    vec3 sc;
+    double w;
+    if (isdist) {
+      if (is + 2 * nsc >= (int)smp.size())
+        return fail("structure '" + s.key + "': the reference reads past its scaling samples here");
+      sc = {smp[is], smp[is + nsc], smp[is + 2 * nsc]};
+      w = wts[is];
+    } else {
+      sc = {smp[0], smp[1], smp[2]};
+      w = wts[0];
+    }
+    for (int r = 0; r < nclass; ++r) {
+      GrainSet gs;
+      gs.scaling = sc;
+      gs.repet = reps[r];
+      for (int g = 0; g < ng; ++g) {
+        if (s.repetition_dist && g % nrep != r) continue;
+        gs.grains.push_back(grains[g]);
+        gs.grains.back().weight *= w;
+      }
+      if (!gs.grains.empty()) out.push_back(std::move(gs));
+    }
+  }
+  return true;
+}
+
+bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const GrainSet& gs, bool accumulate,
                                     double* d_I) {
+  const std::vector<HigGrain>& grains = gs.grains;
   auto uit = in_.unitcells.find(s.unitcell_key);
   if (uit == in_.unitcells.end()) return fail("structure '" + s.key + "': no unitcell '" + s.unitcell_key + "'");
   const HigUnitcell& uc = uit->second;
@@ -653,11 +765,11 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
   // StructureFactor::compute_structure_factor scales the vectors (src/sf/sf.cpp:165-171)
   double la[3], lb[3], lc[3];
   for (int i = 0; i < 3; ++i) {
-    la[i] = a[i] * s.scaling[0];
-    lb[i] = b[i] * s.scaling[1];
-    lc[i] = c[i] * s.scaling[2];
+    la[i] = a[i] * gs.scaling[0];
+    lb[i] = b[i] * gs.scaling[1];
+    lc[i] = c[i] * gs.scaling[2];
   }
-  const double repet[3] = {s.repetition[0], s.repetition[1], s.repetition[2]};
+  const double repet[3] = {gs.repet[0], gs.repet[1], gs.repet[2]};
   const int sf_flags = (flags_ & HGX_HIG_SF_COMPAT_LB2) ? HGX_SF_COMPAT_LB2 : 0;
   const int ng = (int)grains.size();
 
@@ -698,7 +810,8 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     }
     m.rot = rots.data();
     m.weight = wts.data();
-    if (!hgx(hgx_lattice_intensity(gpu_->get(), &m, precision_, d_fc, 0, d_I), "hgx_lattice_intensity")) return false;
+    if (!hgx(hgx_lattice_intensity(gpu_->get(), &m, precision_, d_fc, accumulate ? 1 : 0, d_I), "hgx_lattice_intensity"))
+      return false;
     counts_[0] += ng;
     counts_[2] += ng;
     counts_[3] += ng;
@@ -709,10 +822,11 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
   // any analytic cell: the fused ensemble kernel over all grains, elements and locations
   if (!(flags_ & HGX_HIG_NO_FUSE)) {
     EnsembleBuild b;
-    if (ensemble_model(s, grains, b, false)) {
+    if (ensemble_model(s, grains, b, false, &gs.scaling, &gs.repet)) {
       for (const auto& e : b.el) counts_[2] += (int64_t)ng * e.nloc;
       // the particles sit in the vacuum layer (dnl = 0 above), so q is real and FP32C takes any cell
-      if (!hgx(hgx_ensemble_intensity(gpu_->get(), &b.m, precision_, d_fc, 0, d_I), "hgx_ensemble_intensity"))
+      if (!hgx(hgx_ensemble_intensity(gpu_->get(), &b.m, precision_, d_fc, accumulate ? 1 : 0, d_I),
+               "hgx_ensemble_intensity"))
         return false;
       counts_[0] += ng;
       counts_[3] += ng;
@@ -751,7 +865,8 @@ bool HipGISAXS::structure_intensity(const HigStructure& s, double alphai, const 
     }
     // the reference assigns instead of accumulating in its SAXS branch (:1102-1105), which keeps the
     // last grain only; both experiments accumulate here
-    if (!hgx(hgx_dwba_intensity(gpu_->get(), d_fc, d_sf, d_ff, G.weight, g > 0 ? 1 : 0, d_I), "hgx_dwba_intensity"))
+    if (!hgx(hgx_dwba_intensity(gpu_->get(), d_fc, d_sf, d_ff, G.weight, (accumulate || g > 0) ? 1 : 0, d_I),
+             "hgx_dwba_intensity"))
       return false;
   }
   counts_[1] += ng;
@@ -779,14 +894,18 @@ bool HipGISAXS::run_device(double alpha_i_deg, double phi_deg, const double** d_
   for (auto& c : counts_) c = 0;
   const HigCompute& cp = in_.compute;
   if (cp.nslices > 1) return fail("the slicing scheme (nslices > 1) is not implemented by the reference either");
-  if (cp.structcorr != "nGnE")
-    return fail("structcorrelation '" + cp.structcorr + "' is not supported (nGnE only)");
+  // structure correlation (src/sim/hipgisaxs_main.cpp:1205-1262, 1336-1398): nGnE sums iratio-weighted
+  // structure intensities; GnE squares each structure's grain sum, GE the iratio-weighted sum. The
+  // reference forms those grain sums by reading its summed real grain intensity as if it held one
+  // entry per grain, which is defined only for one grain per structure; nGE is not implemented there.
+  const std::string& corr = cp.structcorr;
+  if (corr == "nGE") return fail("structcorrelation 'nGE' is not implemented by the reference either");
+  if (corr != "nGnE" && corr != "GnE" && corr != "GE")
+    return fail("unknown structcorrelation '" + corr + "' (nGnE, GnE or GE)");
   for (const auto& s : in_.structures) {
     const HigStructure& st = s.second;
     if (!st.grain_type.empty())
       return fail("structure '" + st.key + "': " + st.grain_type + " structures are not supported");
-    if (st.scaling_dist || st.repetition_dist)
-      return fail("structure '" + st.key + "': scaling and repetition distributions are not supported");
   }
   const double alphai = alpha_i_deg * PI_ / 180;
   const double phi = phi_deg * PI_ / 180;
@@ -809,11 +928,25 @@ bool HipGISAXS::run_device(double alpha_i_deg, double phi_deg, const double** d_
   size_t si = 0;
   for (const auto& s : in_.structures) {
     if (!make_grains(s.second, rng_, phi, grains)) return false;
-    if (!structure_intensity(s.second, alphai, grains, d_Is)) return false;
-    // intensity = sum_s struct_intensity_s * iratio_s (:1351-1364)
-    if (!hgx(hgx_axpy(gpu_->get(), npix_, iratio[si], d_Is, si > 0 ? 1 : 0, d_total), "hgx_axpy")) return false;
+    if (corr != "nGnE" && grains.size() != 1)
+      return fail("structure '" + s.second.key + "': structcorrelation " + corr +
+                  " needs one grain per structure (the reference reads past its grain-intensity array "
+                  "for more, src/sim/hipgisaxs_main.cpp:1238-1241, 1254-1257)");
+    std::vector<GrainSet> sets;
+    if (!grain_sets(s.second, grains, sets)) return false;
+    for (size_t k = 0; k < sets.size(); ++k)
+      if (!structure_intensity(s.second, alphai, sets[k], k > 0, d_Is)) return false;
+    // intensity = sum_s struct_intensity_s * iratio_s (:1351-1364), struct_intensity = I_s^2 for GnE
+    if (corr == "GnE") {
+      if (!hgx(hgx_axpy_sq(gpu_->get(), npix_, iratio[si], d_Is, si > 0 ? 1 : 0, d_total), "hgx_axpy_sq"))
+        return false;
+    } else if (!hgx(hgx_axpy(gpu_->get(), npix_, iratio[si], d_Is, si > 0 ? 1 : 0, d_total), "hgx_axpy")) {
+      return false;
+    }
     ++si;
   }
+  // GE: |sum_s iratio_s c_s|^2 with c_s the (real) grain intensity (:1387-1396)
+  if (corr == "GE" && !hgx(hgx_axpy_sq(gpu_->get(), npix_, 1.0, d_total, 0, d_total), "hgx_axpy_sq")) return false;
   const double sigma = in_.scattering.smearing;
   if (sigma > DBL_MIN && !(flags_ & HGX_HIG_NO_SMEAR)) {
     int sf = (flags_ & HGX_HIG_SMEAR_COMPAT) ? HGX_SMEAR_COMPAT : 0;

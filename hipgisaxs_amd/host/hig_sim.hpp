@@ -10,6 +10,7 @@
 #include <new>
 #include <map>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -48,10 +49,27 @@ class GlibcRandom {
   int i_ = 0;  // ring position of r[i - 34]
 };
 
+// The uniform draw of woo::MTRandomNumberGenerator (include/woo/random/woo_mtrandom.hpp:28-33):
+// (mt19937_64() - min) / (max - min).
+struct MtUniform {
+  std::mt19937_64 g;
+  explicit MtUniform(uint64_t seed) : g(seed) {}
+  double next() { return (double)(g() - std::mt19937_64::min()) / (double)(std::mt19937_64::max() - std::mt19937_64::min()); }
+};
+
 struct HigGrain {
   Rot3 rot;             // rot_phi * r3 * r2 * r1
   double center[3];     // rot * dd + transvec
   double weight;        // gauss_weight * scaling weight
+};
+
+// One step of run_gisaxs's grain scaling / repetition loop (src/sim/hipgisaxs_main.cpp:1031-1061)
+// over the grains it applies to: the lattice scaling and repetitions of the step, each grain's
+// weight multiplied by the step's scaling weight.
+struct GrainSet {
+  vec3 scaling;
+  vec3 repet;
+  std::vector<HigGrain> grains;
 };
 
 // The hgx_ensemble_model of one structure (an analytic unit cell seen by its grains) with the host
@@ -159,12 +177,17 @@ class HipGISAXS {
   bool ensure_grid(double alphai_rad, double dnl_re, double dnl_im);
   bool run_device(double alpha_i_deg, double phi_deg, const double** d_img);
   bool make_grains(const HigStructure& s, GlibcRandom& rng, double phi_rad, std::vector<HigGrain>& out);
-  bool structure_intensity(const HigStructure& s, double alphai, const std::vector<HigGrain>& grains, double* d_I);
+  // The grain sets of structure s: one per scaling sample and repetition class, in the reference's
+  // loop order (a single set without distributions).
+  bool grain_sets(const HigStructure& s, const std::vector<HigGrain>& grains, std::vector<GrainSet>& out);
+  // d_I (+)= the DWBA intensity of one grain set.
+  bool structure_intensity(const HigStructure& s, double alphai, const GrainSet& gs, bool accumulate, double* d_I);
   // The Fresnel coefficients for the particle layer at alphai (buffer 2; NULL for SAXS), cached.
   bool fresnel_coeffs(double alphai, double** d_fc);
   // The ensemble model of structure s for `grains` when its unit cell is analytic (false otherwise,
   // with the reason in err_ only when `report`).
-  bool ensemble_model(const HigStructure& s, const std::vector<HigGrain>& grains, EnsembleBuild& b, bool report);
+  bool ensemble_model(const HigStructure& s, const std::vector<HigGrain>& grains, EnsembleBuild& b, bool report,
+                      const vec3* scaling = nullptr, const vec3* repet = nullptr);
   // The normalised iratio weights of the structures (src/sim/hipgisaxs_main.cpp:1268-1282).
   std::vector<double> iratios() const;
   bool form_factor(const HigShape& shape, const Rot3& rot, const vec3& loc, double* d_out);

@@ -458,6 +458,10 @@ int hgx_caxpy(hgx_ctx* ctx, int64_t n, double a_re, double a_im, const double* x
 /* y[0..n) (+)= a x[0..n) over real arrays (device): the iratio-weighted sum of structure intensities
  * (src/sim/hipgisaxs_main.cpp:1351-1364). */
 int hgx_axpy(hgx_ctx* ctx, int64_t n, double a, const double* x, int accumulate, double* y);
+/* y[0..n) (+)= a x[0..n)^2 over real arrays (device; x may be y): the grain- and ensemble-correlated
+ * structure sums GnE (sum_s iratio_s I_s^2) and GE ((sum_s iratio_s I_s)^2) of run_gisaxs
+ * (src/sim/hipgisaxs_main.cpp:1231-1262, 1371-1398). */
+int hgx_axpy_sq(hgx_ctx* ctx, int64_t n, double a, const double* x, int accumulate, double* y);
 
 /* Smearing window of the reference: [i - h, i + h) with the kernel weight indexed from the window
  * start, which shifts the kernel at the image edges (include/numerics/convolutions.hpp:141-147). */

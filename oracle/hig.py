@@ -149,6 +149,7 @@ def read_model(text):
         elif key == "structure":
             st = {"iratio": 1.0, "lattice": "cubic", "hkl": "100", "abc": None, "repetition": [1.0, 1.0, 1.0],
                   "scaling": [1.0, 1.0, 1.0], "transvec": [0.0, 0.0, 0.0], "maxgrains": [1, 1, 1],
+                  "scale_stat": ["none"] * 3, "scale_sd": [0.0] * 3, "scale_n": [40] * 3, "repdist": None,
                   "spacing": [0.0, 0.0, 0.0], "distribution": "regular", "stat": "single",
                   "rot": [{"axis": "x", "angles": [0.0, 0.0]}, {"axis": "y", "angles": [0.0, 0.0]},
                           {"axis": "z", "angles": [0.0, 0.0]}]}
@@ -163,10 +164,24 @@ def read_model(text):
                             ld = dict(gv)
                             st["lattice"] = ld.get("type", "cubic")
                             st["hkl"] = str(ld.get("hkl", "100"))
+                            for ax in "abc":  # a | b | c = { stat, mean, stddev, nsamples }: a scaling axis
+                                if ax in ld and _is_obj(ld[ax]):
+                                    _scaling_axis(st, "abc".index(ax), ld.pop(ax))
                             if "a" in ld:
                                 st["abc"] = (ld["a"], ld["b"], ld["c"])
                         elif gk == "scaling":
-                            st["scaling"] = [gv] * 3 if isinstance(gv, float) else list(gv)
+                            if _is_obj(gv):
+                                for ax, av in gv:
+                                    _scaling_axis(st, "abc".index(ax), av)
+                            else:
+                                st["scaling"] = [gv] * 3 if isinstance(gv, float) else list(gv)
+                        elif gk == "repetitiondist":
+                            st["repdist"] = [{"stat": None, "min": 0.0, "max": 0.0} for _ in range(3)]
+                            for rk, rv in gv:
+                                d = st["repdist"]["xyz".index(rk[0])]
+                                for qk, qv in rv:
+                                    if qk in ("stat", "min", "max"):
+                                        d[qk] = qv
                         elif gk in ("repetition", "transvec"):
                             st[gk] = list(gv)
                 elif k == "ensemble":
@@ -428,6 +443,83 @@ def smear(img, sigma, compat=False):
     return np.array([one_d(c) for c in tmp.T]).T
 
 
+def _is_obj(v):
+    return isinstance(v, list) and len(v) > 0 and all(isinstance(x, tuple) and len(x) == 2 for x in v)
+
+
+def _scaling_axis(st, ax, fields):
+    """One axis of a grain scaling distribution (GrainScaling, src/model/structure.cpp:35-46;
+    src/config/hig_input.cpp:1206-1263, 1642-1652)."""
+    for k, v in fields:
+        if k == "stat":
+            st["scale_stat"][ax] = "none" if v == "single" else v
+        elif k == "mean":
+            st["scaling"][ax] = v
+        elif k == "stddev":
+            st["scale_sd"][ax] = v
+        elif k == "nsamples":
+            st["scale_n"][ax] = int(v)
+
+
+def grain_sets(st, gr):
+    """run_gisaxs's grain scaling / repetition loop (src/sim/hipgisaxs_main.cpp:819-868, 1031-1061,
+    2089-2226) for the grains gr [(R, center, w)]: a list of (scaling[3], repetition[3], grains with
+    weights times the step's scaling weight), one per step and repetition class."""
+    stat, sd, mean = st["scale_stat"], st["scale_sd"], st["scaling"]
+    nv = [(1 if sd[i] == 0 else st["scale_n"][i]) if stat[i] != "none" else 1 for i in range(3)]
+    maxn = max(nv)
+    rng = MT19937_64(0)
+    smp, wts = [], []
+    for i in range(3):
+        if stat[i] == "none":
+            smp.append(mean[i])
+            wts.append(1.0)
+        elif stat[i] in ("gaussian", "cauchy"):
+            lo, dx = mean[i] - 4 * sd[i], 8 * sd[i] / (maxn + 1)
+            for j in range(maxn):
+                t = lo + j * dx + rng.uniform() * dx
+                smp.append(t)
+                if stat[i] == "gaussian":  # HipGISAXS::gaussian (include/sim/hipgisaxs_main.hpp:151-156)
+                    w = (1.0 if t - mean[i] < DBL_MIN else 0.0) if sd[i] < DBL_MIN else \
+                        1.0 / (sd[i] * 2.506628) * math.exp(-(t - mean[i]) ** 2 / (2 * sd[i] * sd[i]))
+                else:  # HipGISAXS::cauchy (:169-172)
+                    w = 1.0 / (PI_ * sd[i] * (1.0 + ((t - mean[i]) / sd[i]) ** 2)) if sd[i] > DBL_MIN else \
+                        (0.0 if t - mean[i] > DBL_MIN else 1.0)
+                wts.append(w)
+        else:
+            raise ValueError(f"scaling statistic {stat[i]}")
+    if st["repdist"] is not None:
+        n = len(smp) if len(smp) > 1 else len(gr)
+        vals = []
+        for d in st["repdist"]:
+            if d["stat"] in ("none", "single"):
+                vals.append([float(d["min"])])
+            elif d["stat"] == "range":
+                r = MT19937_64(0)
+                vals.append([float(math.floor(d["min"] + r.uniform() * (d["max"] - d["min"]))) for _ in range(n)])
+            else:
+                raise ValueError("repetition statistic")
+        reps = [list(v) for v in zip(*vals)]
+    else:
+        reps = [list(st["repetition"])]
+    nsc, nrep = len(smp) // 3, len(reps)
+    if nsc == nrep or nrep == 1:
+        nrs = nsc
+    elif nsc == 1:
+        nrs = nrep
+    else:
+        raise ValueError("scaling and repetition distributions are not aligned")
+    isdist = any(x > 0 for x in sd)
+    out = []
+    for i in range(nrs):
+        sc, w = ([smp[i], smp[i + nsc], smp[i + 2 * nsc]], wts[i]) if isdist else (smp[:3], wts[0])
+        for r in range(nrep if st["repdist"] is not None else 1):
+            gs = [(R, c, gw * w) for g, (R, c, gw) in enumerate(gr) if st["repdist"] is None or g % nrep == r]
+            if gs:
+                out.append((sc, reps[r], gs))
+    return out
+
+
 def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True, model=None):
     """Image [nrow][ncol] of configuration `config` (alpha_i outermost, as run_all_gisaxs loops) for
     .hig text (or an already read and edited `model`); grain orientations drawn from rand() after
@@ -472,35 +564,47 @@ def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", me
         ir = np.array([m["structures"][k]["iratio"] for k in keys], float)
         if ir.sum() != 1.0:
             ir = ir / ir.sum()
+        corr = cp["structcorrelation"]
+        if corr not in ("nGnE", "GnE", "GE"):
+            raise ValueError(f"structcorrelation {corr}")
         total = np.zeros(npix)
         for si, key in enumerate(keys):
             st = m["structures"][key]
             gr = grains(st, phi, draws)
             if ci != config:
                 continue
+            # GnE / GE (src/sim/hipgisaxs_main.cpp:1231-1262, 1371-1398) read one grain-intensity
+            # entry per grain from an array that holds one: defined for a single grain only
+            if corr != "nGnE" and len(gr) != 1:
+                raise ValueError("GnE/GE need one grain per structure")
             if st["abc"] is not None:
                 va, vb, vc = st["abc"]
             else:
                 va, vb, vc = _LATTICES.get((st["lattice"], st["hkl"]), _LATTICES.get((st["lattice"], None)))
-            s = st["scaling"]
-            la, lb, lc = (np.asarray(va, float) * s[0], np.asarray(vb, float) * s[1], np.asarray(vc, float) * s[2])
             els = m["unitcells"][st["unitcell"]]
             I = np.zeros(npix)
-            for R, center, w in gr:
-                ff = np.zeros(g.nq, complex)
-                for skey in sorted(els):
-                    sh = m["shapes"][skey]
-                    dn2 = 0j - complex(2 * sh["delta"], 2 * sh["beta"])
-                    srot = _mul(_mul(_mul(R, rotmat(0, sh["xrot"])), rotmat(1, sh["yrot"])), rotmat(2, sh["zrot"]))
-                    for loc in els[skey]:
-                        if sh["name"].endswith((".obj", ".dat")):
-                            ff += dn2 * mesh_ff(f"{base_dir}/{sh['name']}", g, srot, loc, mesh_translate)
-                        else:
-                            ff += dn2 * ff_analytic(sh["name"], shape_dims(sh), g, rot=srot, transvec=loc)
-                sf = sf_lattice(la, lb, lc, st["repetition"], center, g, 0.0, sf_flags, rot=R)
-                I = dwba_intensity(npix, nb, fc, sf, ff, w, I)
-            total += ir[si] * I
+            for s, repet, gset in grain_sets(st, gr):
+                la, lb, lc = (np.asarray(va, float) * s[0], np.asarray(vb, float) * s[1],
+                              np.asarray(vc, float) * s[2])
+                for R, center, w in gset:
+                    ff = np.zeros(g.nq, complex)
+                    for skey in sorted(els):
+                        sh = m["shapes"][skey]
+                        dn2 = 0j - complex(2 * sh["delta"], 2 * sh["beta"])
+                        srot = _mul(_mul(_mul(R, rotmat(0, sh["xrot"])), rotmat(1, sh["yrot"])),
+                                    rotmat(2, sh["zrot"]))
+                        for loc in els[skey]:
+                            if sh["name"].endswith((".obj", ".dat")):
+                                ff += dn2 * mesh_ff(f"{base_dir}/{sh['name']}", g, srot, loc, mesh_translate)
+                            else:
+                                ff += dn2 * ff_analytic(sh["name"], shape_dims(sh), g, rot=srot, transvec=loc)
+                    sf = sf_lattice(la, lb, lc, repet, center, g, 0.0, sf_flags, rot=R)
+                    I = dwba_intensity(npix, nb, fc, sf, ff, w, I)
+            # nGnE and GE sum iratio_s I_s (GE squares the sum below), GnE sums iratio_s I_s^2
+            total += ir[si] * (I * I if corr == "GnE" else I)
         if ci == config:
+            if corr == "GE":
+                total = total * total
             img = total.reshape(nrow, ncol)
             if smear_mode != "off" and sc["smearing"] > DBL_MIN:
                 img = smear(img, sc["smearing"], compat=smear_mode == "compat")

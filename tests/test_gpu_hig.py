@@ -179,6 +179,94 @@ def test_ensemble_kernel_matches_per_grain_calls_at_full_size():
         assert _rel(x, y) <= 1e-12
 
 
+def _one_grain_pair(corr):
+    t = _text("two_structures")
+    t = t.replace("maxgrains = [2 2 1]", "maxgrains = [1 1 1]").replace("maxgrains = [2 1 1]", "maxgrains = [1 1 1]")
+    return t.replace("resolution = [40 24]", f'resolution = [40 24], structcorrelation = "{corr}"')
+
+
+@pytest.mark.parametrize("corr", ["GnE", "GE"])
+def test_structure_correlations_match_oracle(corr):
+    """Grain/ensemble-correlated structure sums (one grain per structure, two structures, smearing on)."""
+    _say("structcorrelation", corr)
+    text = _one_grain_pair(corr)
+    img = H.HigSimulation(text=text).run_all()[0]  # the fixture scans angles: its first configuration
+    ref = OH.run(text, config=0)
+    r = _rel(img, ref)
+    _say(corr, "relL2", r)
+    assert r <= TOL64
+    # the correlated sums differ from the plain one
+    assert _rel(H.HigSimulation(text=_one_grain_pair("nGnE")).run_all()[0], ref) > 1e-2
+
+
+def test_structure_correlations_refused_where_undefined():
+    """nGE is unimplemented in the reference, and GnE / GE read past its per-structure array for more
+    than one grain: both are refused with an error, not computed."""
+    with pytest.raises(Exception, match="nGE"):
+        H.HigSimulation(text=_one_grain_pair("nGE")).run_all()
+    many = _text("two_structures").replace("resolution = [40 24]",
+                                            'resolution = [40 24], structcorrelation = "GnE"')
+    with pytest.raises(Exception, match="one grain"):
+        H.HigSimulation(text=many).run_all()
+
+
+def _spheres(grain):
+    return _text("spheres_bcc").replace("repetition = [100 100 10], scaling = 15", grain)
+
+
+_DISTS = {
+    # all three axes distributed: samples i, i + n, i + 2n of the list are one step's scaling
+    "scaling_abc": ('repetition = [20 20 4], scaling = { a = { stat = "gaussian", mean = 15, stddev = 0.3, '
+                    'nsamples = 3 }, b = { stat = "gaussian", mean = 15, stddev = 0.2, nsamples = 2 }, '
+                    'c = { stat = "cauchy", mean = 14, stddev = 0.25, nsamples = 3 } }'),
+    # one axis distributed (given in the lattice block): the reference's indexing of its sample list
+    "scaling_a_in_lattice": None,
+    # repetition ranges: three repetition entries, grains by index modulo 3, the loop run three times
+    "repetition_range": ('scaling = 15, repetitiondist = { xrepetition = { stat = "range", min = 5, max = 9 }, '
+                         'yrepetition = { stat = "range", min = 4, max = 8 }, '
+                         'zrepetition = { stat = "range", min = 2, max = 4 } }'),
+}
+
+
+def _dist_text(name):
+    if name == "scaling_a_in_lattice":
+        return _text("spheres_bcc").replace(
+            'lattice = { type = "bcc" }',
+            'lattice = { type = "bcc", a = { stat = "gaussian", mean = 15, stddev = 0.4, nsamples = 4 } }').replace(
+            "repetition = [100 100 10], scaling = 15", "repetition = [20 20 4], scaling = [15 16 14]")
+    return _spheres(_DISTS[name])
+
+
+@pytest.mark.parametrize("flags", [0, H.NO_FUSE], ids=["fused", "per_grain"])
+@pytest.mark.parametrize("name", list(_DISTS))
+def test_grain_distributions_match_oracle(name, flags):
+    """Grain scaling and repetition distributions (samples from the 64-bit Mersenne twister seeded 0)."""
+    _say("distribution", name, "flags", flags)
+    text = _dist_text(name)
+    img = H.HigSimulation(text=text, flags=flags).compute()
+    ref = OH.run(text)
+    r = _rel(img, ref)
+    _say(name, "relL2", r)
+    assert r <= TOL64
+    plain = OH.run(_spheres("repetition = [20 20 4], scaling = 15"))
+    assert _rel(ref, plain) > 1e-3
+
+
+def test_grain_distributions_refused_where_the_reference_fails():
+    both = _spheres('scaling = { a = { stat = "gaussian", mean = 15, stddev = 0.3, nsamples = 3 }, '
+                    'b = { stat = "gaussian", mean = 15, stddev = 0.3, nsamples = 3 }, '
+                    'c = { stat = "gaussian", mean = 15, stddev = 0.3, nsamples = 3 } }, '
+                    'repetitiondist = { xrepetition = { stat = "range", min = 5, max = 9 }, '
+                    'yrepetition = { stat = "range", min = 4, max = 8 }, zrepetition = { stat = "range", min = 2, max = 3 } }')
+    # nine scaling samples make nine repetition entries against three scaling steps
+    with pytest.raises(Exception, match="not aligned"):
+        H.HigSimulation(text=both).compute()
+    gauss = _spheres('scaling = 15, repetitiondist = { xrepetition = { stat = "gaussian", min = 5, max = 9 }, '
+                     'yrepetition = { stat = "range", min = 4, max = 8 }, zrepetition = { stat = "none", min = 2 } }')
+    with pytest.raises(Exception, match="none or range"):
+        H.HigSimulation(text=gauss).compute()
+
+
 def test_errors_are_reported_not_fatal():
     text = _text("spheres_bcc").replace('unitcell:key = "cell"', 'unitcell:key = "nope"')
     sim = H.HigSimulation(text=text)

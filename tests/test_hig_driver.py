@@ -169,6 +169,77 @@ def test_unsupported_features_fail_at_run_not_silently():
         sim.grains("bcc")
 
 
+def _one_grain_pair(corr):
+    """The two-structure fixture with one grain per structure and the given structcorrelation."""
+    t = (GOLD / "two_structures.hig").read_text()
+    t = t.replace("maxgrains = [2 2 1]", "maxgrains = [1 1 1]").replace("maxgrains = [2 1 1]", "maxgrains = [1 1 1]")
+    return t.replace("resolution = [40 24]", f'resolution = [40 24], structcorrelation = "{corr}"')
+
+
+def test_oracle_structure_correlations_pin():
+    """GE and GnE of the oracle through its own nGnE path: with iratios (3, 1) normalised to (3/4, 1/4),
+    nGnE gives A = 3/4 I1 + 1/4 I2, and iratios (1, 0) give I1 with the same grain draws, so I2 follows;
+    then GE = A^2 and GnE = 3/4 I1^2 + 1/4 I2^2, and more than one grain is refused."""
+    A = OH.run(_one_grain_pair("nGnE"), smear_mode="off")
+    I1 = OH.run(_one_grain_pair("nGnE").replace("iratio = 1,", "iratio = 0,"), smear_mode="off")
+    I2 = (A - 0.75 * I1) / 0.25
+    assert np.linalg.norm(I2) > 0.1 * np.linalg.norm(I1)
+    ge = OH.run(_one_grain_pair("GE"), smear_mode="off")
+    gne = OH.run(_one_grain_pair("GnE"), smear_mode="off")
+    assert np.allclose(ge, A * A, rtol=1e-13, atol=0)
+    assert np.allclose(gne, 0.75 * I1 * I1 + 0.25 * I2 * I2, rtol=1e-9, atol=1e-12 * np.abs(gne).max())
+    with pytest.raises(ValueError):
+        OH.run((GOLD / "two_structures.hig").read_text().replace(
+            "resolution = [40 24]", 'resolution = [40 24], structcorrelation = "GE"'))
+
+
+def _spheres(grain):
+    """The spheres fixture with its grain's repetition and scaling replaced."""
+    return (GOLD / "spheres_bcc.hig").read_text().replace("repetition = [100 100 10], scaling = 15", grain)
+
+
+_SCALING3 = ('repetition = [20 20 4], scaling = { a = { stat = "gaussian", mean = 15, stddev = 0.3, nsamples = 3 }, '
+             'b = { stat = "gaussian", mean = 15, stddev = 0.2, nsamples = 2 }, '
+             'c = { stat = "cauchy", mean = 14, stddev = 0.25, nsamples = 3 } }')
+
+
+def test_oracle_scaling_distribution_pins():
+    """The oracle's grain scaling loop: every sample sits in its cell of [mean - 4 sd, mean + 4 sd]
+    (8 sd / (n + 1) wide, n the largest sample count) with the pdf of the reference's constants as
+    its weight, and the image is the sum over samples of weight^2 times the image at that fixed
+    scaling; zero widths give the plain image."""
+    st = OH.read_model(_spheres(_SCALING3))["structures"]["bcc"]
+    sets = OH.grain_sets(st, [(None, None, 1.0)])
+    assert len(sets) == 3
+    means, sds = [15.0, 15.0, 14.0], [0.3, 0.2, 0.25]
+    for j, (sc, rep, g) in enumerate(sets):
+        assert rep == [20.0, 20.0, 4.0]
+        for ax in range(3):
+            lo, dx = means[ax] - 4 * sds[ax], 8 * sds[ax] / 4
+            assert lo + j * dx <= sc[ax] < lo + (j + 1) * dx
+        x = (sc[0] - 15.0) / 0.3
+        assert g[0][2] == pytest.approx(math.exp(-0.5 * x * x) / (0.3 * 2.506628), rel=1e-14)
+    img = OH.run(_spheres(_SCALING3), smear_mode="off")
+    ref = sum(g[0][2] ** 2 * OH.run(_spheres(f"repetition = [20 20 4], scaling = [{sc[0]!r} {sc[1]!r} {sc[2]!r}]"),
+                                     smear_mode="off") for sc, _, g in sets)
+    assert np.allclose(img, ref, rtol=1e-12, atol=0)
+    zero = _SCALING3.replace("stddev = 0.25", "stddev = 0").replace("stddev = 0.3", "stddev = 0").replace(
+        "stddev = 0.2", "stddev = 0").replace("mean = 14", "mean = 15")
+    assert np.allclose(OH.run(_spheres(zero), smear_mode="off"),
+                       OH.run(_spheres("repetition = [20 20 4], scaling = 15"), smear_mode="off"), rtol=1e-14, atol=0)
+
+
+def test_oracle_repetition_range_pins():
+    """A repetition range of one value per axis gives the fixed repetitions; without a scaling
+    distribution the reference's loop runs over three repetition entries (its sample list has three
+    values) and adds each grain three times."""
+    rd = ("scaling = 15, repetitiondist = { xrepetition = { stat = \"range\", min = 7, max = 7 }, "
+          "yrepetition = { stat = \"range\", min = 6, max = 6 }, zrepetition = { stat = \"range\", min = 3, max = 3 } }")
+    img = OH.run(_spheres(rd), smear_mode="off")
+    ref = OH.run(_spheres("repetition = [7 6 3], scaling = 15"), smear_mode="off")
+    assert np.allclose(img, 3 * ref, rtol=1e-13, atol=0)
+
+
 def test_read_mesh_files():
     kind, tri = H.read_mesh(str(GOLD / "tetra.obj"))
     assert kind == "exact9" and tri.shape == (4, 9)
