@@ -11,10 +11,13 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <iterator>
 #include <limits>
+#include <map>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -268,6 +271,83 @@ bool read_ascii_image(const std::string& path, std::vector<double>& data, unsign
   return true;
 }
 
+// EDFReader (src/file/edf_reader.cpp:24-150): a header of `key = value ;` items (white space dropped)
+// closed by '}', read in 512-byte blocks; the data start at the block after the one holding the '}'
+// and are EDF_BinarySize bytes of 32-bit floats in the machine's (little-endian) order, Dim_1 columns
+// by Dim_2 rows, whatever DataType and ByteOrder say.
+bool read_edf_image(const std::string& path, std::vector<double>& data, unsigned& ncol, unsigned& nrow,
+                    std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    err = "could not open the EDF file " + path;
+    return false;
+  }
+  const std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  auto white = [](char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r'; };
+  std::map<std::string, std::string> header;
+  size_t p = 0, close = std::string::npos;
+  while (close == std::string::npos) {
+    std::string key, value;
+    bool got_key = false;
+    for (; p < bytes.size(); ++p) {  // the key, up to '='; a '}' here ends the header
+      const char c = bytes[p];
+      if (c == '=') {
+        got_key = true;
+        ++p;
+        break;
+      }
+      if (c == '}') {
+        close = p;
+        break;
+      }
+      if (!white(c)) key += c;
+    }
+    if (close != std::string::npos) break;
+    if (!got_key) {
+      err = "EDF file " + path + ": the header is not closed";
+      return false;
+    }
+    bool got_value = false;
+    for (; p < bytes.size(); ++p) {
+      const char c = bytes[p];
+      if (c == ';') {
+        got_value = true;
+        ++p;
+        break;
+      }
+      if (!white(c)) value += c;
+    }
+    if (!got_value) {
+      err = "EDF file " + path + ": a header item has no ';'";
+      return false;
+    }
+    header[key] = value;
+    size_t q = p;  // header_done: the next character that is not white space closes it
+    while (q < bytes.size() && white(bytes[q])) ++q;
+    if (q < bytes.size() && bytes[q] == '}') close = q;
+  }
+  auto num = [&](const char* k) { return std::atol(header[k].c_str()); };  // atoi / atol as the reference
+  const long cols = num("Dim_1"), rows = num("Dim_2"), nbytes = num("EDF_BinarySize");
+  const size_t start = (close / 512 + 1) * 512;
+  if (cols <= 0 || rows <= 0 || nbytes < 0 || (size_t)nbytes / 4 != (size_t)cols * (size_t)rows) {
+    err = "EDF file " + path + ": mismatch in reference data size (Dim_1 x Dim_2 against EDF_BinarySize / 4)";
+    return false;
+  }
+  if (start + (size_t)nbytes > bytes.size()) {
+    err = "EDF file " + path + ": shorter than its header says";
+    return false;
+  }
+  data.resize((size_t)cols * rows);
+  for (size_t i = 0; i < data.size(); ++i) {
+    float v;
+    std::memcpy(&v, bytes.data() + start + 4 * i, 4);
+    data[i] = v;
+  }
+  ncol = (unsigned)cols;
+  nrow = (unsigned)rows;
+  return true;
+}
+
 }  // namespace
 
 bool HipGISAXS::update_param(const std::string& variable, double value) {
@@ -296,13 +376,14 @@ bool HipGISAXS::fit_init() {
   const HigFitReference& ref = in_.reference;
   const size_t dot = ref.path.find_last_of('.');
   const std::string ext = dot == std::string::npos ? "" : ref.path.substr(dot + 1);
-  if (ext == "edf" || ext == "EDF") return fail("EDF reference data are not supported (ASCII .out/.dat/.txt only)");
-  if (ext != "txt" && ext != "TXT" && ext != "dat" && ext != "DAT" && ext != "out" && ext != "OUT")
+  const bool edf = ext == "edf" || ext == "EDF";
+  if (!edf && ext != "txt" && ext != "TXT" && ext != "dat" && ext != "DAT" && ext != "out" && ext != "OUT")
     return fail("error in reference data file format (extension of '" + ref.path + "')");
   std::vector<double> O;
   unsigned ncol = 0, nrow = 0;
   std::string err;
-  if (!read_ascii_image(ref.path, O, ncol, nrow, err)) return fail(err);
+  if (!(edf ? read_edf_image(ref.path, O, ncol, nrow, err) : read_ascii_image(ref.path, O, ncol, nrow, err)))
+    return fail(err);
   if (!prepare()) return false;
   if (ncol != ncol_ || nrow != nrow_) {
     // override_qregion (src/sim/hipgisaxs_main.cpp:236-276): the reference's region and size

@@ -99,7 +99,8 @@ int hgx_hig_fit_param(hgx_hig* h, int i, char* key, size_t key_cap, double* rang
  * tol_reg = (tolerance, regularization). Host only. */
 int hgx_hig_fit_algorithm(hgx_hig* h, int i, char* name, size_t name_cap, int* metric, double* tol_reg);
 /* HipGISAXSObjectiveFunction::set_reference_data (src/analyzer/objective_func_hipgisaxs.cpp:110-170):
- * reads the reference data (ASCII rows) and mask, takes the reference region and size as the output
+ * reads the reference data (ASCII rows, or an ESRF data file: EDFReader, src/file/edf_reader.cpp:24-150)
+ * and mask, takes the reference region and size as the output
  * region when the sizes differ (HipGISAXS::override_qregion), and keeps both on the device. */
 int hgx_hig_fit_init(hgx_hig* h);
 /* HipGISAXSObjectiveFunction::operator() (:245-332): sets the n fit parameters to x, computes the

@@ -725,6 +725,41 @@ def read_ascii_image(path):
     return np.array(rows, dtype=np.float64)
 
 
+def read_edf_image(path):
+    """EDFReader (src/file/edf_reader.cpp:24-150): `key = value ;` header items with white space
+    dropped, closed by '}'; the data start at the 512-byte block after the one holding the '}' and are
+    EDF_BinarySize bytes of little-endian float32, Dim_1 columns by Dim_2 rows (DataType and ByteOrder
+    are not read)."""
+    raw = open(path, "rb").read()
+    white = b" \n\t\r"
+    head, p, close = {}, 0, None
+    while close is None:
+        key, value = b"", b""
+        while raw[p:p + 1] not in (b"=", b"}"):
+            if raw[p] not in white:
+                key += raw[p:p + 1]
+            p += 1
+        if raw[p:p + 1] == b"}":
+            close = p
+            break
+        p += 1
+        while raw[p:p + 1] != b";":
+            if raw[p] not in white:
+                value += raw[p:p + 1]
+            p += 1
+        p += 1
+        head[key.decode()] = value.decode()
+        q = p
+        while raw[q] in white:
+            q += 1
+        if raw[q:q + 1] == b"}":
+            close = q
+    cols, rows, nb = (int(head.get(k, "0")) for k in ("Dim_1", "Dim_2", "EDF_BinarySize"))
+    start = (close // 512 + 1) * 512
+    assert nb // 4 == cols * rows, "mismatch in reference data size"
+    return np.frombuffer(raw[start:start + nb], dtype="<f4").astype(np.float64).reshape(rows, cols)
+
+
 def objective(text, params, x, metric, O, mask=None, reg=0.0, base_dir=".", seed=1):
     """HipGISAXSObjectiveFunction::operator(): the image of the edited model, its distance to O under
     `metric` (oracle.metrics), plus (reg / 2) |x - range midpoints|^2; non-finite -> 1e8."""

@@ -206,6 +206,37 @@ def test_reference_region_overrides_the_output_region(tmp_path):
     assert sim.objective(TRUTH, "sqrt_c_l2") <= 1e-12
 
 
+def _write_edf(path, img):
+    rows, cols = img.shape
+    items = ["HeaderID = EH:000001:000000:000000", "ByteOrder = LowByteFirst", "DataType = FloatValue",
+             f"Dim_1 = {cols}", f"Dim_2 = {rows}", f"EDF_BinarySize = {4 * rows * cols}"]
+    head = "{\n" + "".join(f"{it} ;\n" for it in items)
+    head += " " * (1024 - len(head) - 2) + "}\n"
+    Path(path).write_bytes(head.encode() + np.asarray(img, "<f4").tobytes())
+
+
+def test_edf_reference_data(fitdir, tmp_path):
+    """Reference data in an ESRF data file (EDFReader: float32 pixels after the header blocks) give the
+    objective of the same float32 values written as ASCII, and the oracle's with its EDF reader."""
+    O32 = OH.read_ascii_image(str(fitdir / "fit_cylinders_ref.out")).astype(np.float32)
+    _write_edf(tmp_path / "ref.edf", O32)
+    np.savetxt(tmp_path / "ref.out", O32.astype(np.float64), fmt="%.17g")
+    t_edf = FIT.replace("fit_cylinders_ref.out", "ref.edf")
+    t_out = FIT.replace("fit_cylinders_ref.out", "ref.out")
+    assert t_edf != FIT
+    s_edf = H.HigSimulation(text=t_edf, base_dir=str(tmp_path))
+    s_out = H.HigSimulation(text=t_out, base_dir=str(tmp_path))
+    params, _, _ = OH.read_fitting(t_edf)
+    O = OH.read_edf_image(str(tmp_path / "ref.edf"))
+    x = [5.5, 49.0]
+    for metric in ("sqrt_unit_l2", "chi2"):
+        a, b = s_edf.objective(x, metric), s_out.objective(x, metric)
+        want = OH.objective(t_edf, params, x, metric, O)
+        _say("edf", metric, a, b, want)
+        assert a == pytest.approx(b, rel=1e-12)
+        assert a == pytest.approx(want, rel=1e-9)
+
+
 def test_mask_file_weights_the_distance(fitdir, tmp_path):
     """A referencedata mask (whitespace-separated weights, read_mask_data) weights every functor as the
     oracle's mask does; a zero mask over the lower half changes the distance."""

@@ -127,6 +127,31 @@ def test_fit_init_reports_missing_reference_data(tmp_path):
         H.HigSimulation(text=edf, base_dir=str(tmp_path)).fit_init()
 
 
+def _write_edf(path, img, header_blocks=1, first_key="HeaderID = EH:000001:000000:000000"):
+    """An ESRF data file: a '{ key = value ; ... }' header padded with spaces to header_blocks x 512
+    bytes, then the image as little-endian float32, row by row."""
+    rows, cols = img.shape
+    items = [first_key, "Image = 1", "ByteOrder = LowByteFirst", "DataType = FloatValue",
+             f"Dim_1 = {cols}", f"Dim_2 = {rows}", f"Size = {4 * rows * cols}", f"EDF_BinarySize = {4 * rows * cols}"]
+    head = "{\n" + "".join(f"{it} ;\n" for it in items)
+    head += " " * (512 * header_blocks - len(head) - 2) + "}\n"
+    assert len(head) == 512 * header_blocks
+    Path(path).write_bytes(head.encode() + np.asarray(img, "<f4").tobytes())
+
+
+def test_oracle_edf_reader_pins(tmp_path):
+    """The oracle's EDF reader returns the float32 image a file holds, for a header of one or of two
+    512-byte blocks, with an item split across the block boundary."""
+    rng = np.random.default_rng(3)
+    img = rng.gamma(2.0, 50.0, (7, 11)).astype(np.float32)
+    _write_edf(tmp_path / "a.edf", img)
+    got = OH.read_edf_image(str(tmp_path / "a.edf"))
+    assert got.shape == (7, 11) and np.array_equal(got, img.astype(np.float64))
+    # a long first item pushes Dim_* across the first block boundary
+    _write_edf(tmp_path / "b.edf", img, header_blocks=2, first_key="HeaderID = " + "X" * 470)
+    assert np.array_equal(OH.read_edf_image(str(tmp_path / "b.edf")), img.astype(np.float64))
+
+
 def test_mersenne_twister_64_is_the_standard_generator():
     """[rand.predef]: the 10000th output of a default-constructed std::mt19937_64 is
     9981545732273789042; the uniform draw is (g - min) / (max - min) in double."""
