@@ -28,6 +28,7 @@ PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
 SF_COMPAT_LB2 = 1
 SF_FP32C = 2
 SMEAR_COMPAT = 1
+SHAPE_COMPAT_PYRAMID, SHAPE_TRUNCATED_SPHERE = 1, 2
 HGX_DIST_REFERENCE = 1
 METRICS = {"chi2": 0, "l2": 1, "sqrt_unit_l2": 2, "sqrt_c_l2": 3, "cbrt_unit_l2": 4, "cbrt_c_l2": 5}
 SHAPES = {"sphere": 0, "cylinder": 1, "box": 2, "cube": 3, "prism3": 4, "prism6": 5, "pyramid": 6}

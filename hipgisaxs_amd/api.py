@@ -197,8 +197,9 @@ class Context:
         """dims: per shape dimension a (values, weights) pair of 1-D arrays."""
         sp = N.ShapeParams()
         keep = []
-        if len(dims) != N.SHAPE_DIMS[shape]:
-            raise HgxError(f"{shape} takes {N.SHAPE_DIMS[shape]} dimensions")
+        nd = 2 if shape == "sphere" and flags & N.SHAPE_TRUNCATED_SPHERE else N.SHAPE_DIMS[shape]
+        if len(dims) != nd:
+            raise HgxError(f"{shape} takes {nd} dimensions")
         for d, (v, w) in enumerate(dims):
             v, w = _f64(v).ravel(), _f64(w).ravel()
             keep += [v, w]

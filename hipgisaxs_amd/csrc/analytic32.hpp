@@ -230,6 +230,7 @@ __device__ inline bool fp32_conditioned(const double m[3], const Nodes& P) {
     return l * d >= 0.05;
   }
   if (SHAPE == HGX_SHAPE_PYRAMID) return fabs(m[0]) * min_node(P, 0) >= 0.5 && fabs(m[1]) * min_node(P, 1) >= 0.5;
+  if (SHAPE == HGX_SHAPE_SPHERE) return !(P.flags & HGX_SHAPE_TRUNCATED_SPHERE);  // the cap: FP64
   return true;
 }
 

@@ -4,6 +4,8 @@
 // gives the value, (dz, dd) the value with its directional derivative (dual.hpp).
 #pragma once
 
+#include <type_traits>
+
 #include "dual.hpp"
 
 namespace hgx {
@@ -98,8 +100,39 @@ struct ShapeFF;
 // sphere, src/ff/ff_ana_sphere.cpp:33-45 and :98-110
 template <>
 struct ShapeFF<HGX_SHAPE_SPHERE> {
+  // the reference GPU's cap of height H (ff_ana_sphere_gpu.cu:53-86, 100-108): trapezoid over
+  // kCapSlices disks, each 0 where |q_par| < 1e-13 or r Re q_par < 1e-13
+  static constexpr int kCapSlices = 512;
+  __device__ static cd cap(const cd mq[3], double Rr, double H) {
+    const cd qp = csqrt_d(mq[0] * mq[0] + mq[1] * mq[1]);
+    if (cabs_d(qp) < 1e-13) return cd{0.0, 0.0};
+    const double cen = H - Rr, dh = H / (double)(kCapSlices - 1);
+    cd out{0.0, 0.0};
+    for (int s = 0; s < kCapSlices; ++s) {
+      const double z = s * dh - cen;
+      // the last slice can pass the pole by a rounding step (511 (H / 511) > H): an empty disk there,
+      // where the reference takes the root of a negative number
+      const double rz = sqrt(fmax(Rr * Rr - z * z, 0.0));
+      const double t2 = rz * qp.r;
+      if (t2 < 1e-13) continue;
+      const double disk = 2 * kPi * rz * rz * (::j1(t2) / t2);
+      out = out + ((s == 0 || s == kCapSlices - 1) ? disk : 2.0 * disk) * cexpi_d(mq[2] * z);
+    }
+    return (0.5 * dh) * out;
+  }
   template <class C, class R>
   __device__ static C eval(const C mq[3], const NodesT<R>& P) {
+    if constexpr (std::is_same<R, double>::value) {
+      if ((P.flags & HGX_SHAPE_TRUNCATED_SPHERE) && P.n[1] > 0) {
+        C acc = cconst<C>(0.0);
+        for (int i = 0; i < P.n[0]; ++i)
+          for (int k = 0; k < P.n[1]; ++k) {
+            const double Rr = P.val[0][i], H = P.val[1][k];
+            acc = acc + (P.wt[1][k] * P.wt[0][i]) * (cexpi_d(mq[2] * (H - Rr)) * cap(mq, Rr, H));
+          }
+        return acc;
+      }
+    }
     C qval = csqrt_d(mq[0] * mq[0] + mq[1] * mq[1] + mq[2] * mq[2]);
     C acc = cconst<C>(0.0);
     if (cabs_d(qval) < 2.2250738585072014e-308) return acc;

@@ -322,7 +322,6 @@ __global__ __launch_bounds__(256) void k_ensemble_sum(int64_t npix, int ny, cons
 
 using namespace hgx;
 
-static const int kEnsDims[7] = {1, 2, 3, 1, 2, 2, 4};
 
 namespace hgx {
 
@@ -377,7 +376,7 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     Nodes& P = E[e].P;
     for (int d = 0; d < 4; ++d) {
       P.n[d] = 0;
-      if (d >= kEnsDims[el.shape]) continue;
+      if (d >= analytic_dims(el.shape, el.params.flags)) continue;
       int n = el.params.n[d];
       if (n < 1 || n > HGX_MAX_NODES || !el.params.val[d] || !el.params.wt[d])
         return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: element %d dimension %d needs 1..%d nodes", e,
@@ -388,6 +387,8 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
         P.wt[d][i] = el.params.wt[d][i];
       }
     }
+    if (!analytic_heights_ok(el.shape, el.params.flags, el.params.n, el.params.val))
+      return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_intensity: element %d: truncated sphere heights must lie in [0, 2 R]", e);
     P.tan_tau = std::tan(el.params.tau);
     P.sin_eta = std::sin(el.params.eta);
     P.cos_eta = std::cos(el.params.eta);
@@ -526,7 +527,9 @@ static int ensemble_tangent_run(Context* c, const hgx_ensemble_model* m, int ndi
     const hgx_ensemble_element& el = m->elem[e];
     if (el.shape < 0 || el.shape > HGX_SHAPE_PYRAMID || el.nloc < 1 || !el.loc)
       return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: element %d needs a shape and locations", e);
-    for (int d = 0; d < kEnsDims[el.shape]; ++d)
+    if (el.shape == HGX_SHAPE_SPHERE && (el.params.flags & HGX_SHAPE_TRUNCATED_SPHERE))
+      return set_error(c, HGX_ERR_UNSUPPORTED, "ensemble_tangent: element %d: truncated spheres have no tangent", e);
+    for (int d = 0; d < analytic_dims(el.shape, el.params.flags); ++d)
       if (el.params.n[d] < 1 || el.params.n[d] > HGX_MAX_NODES || !el.params.val[d] || !el.params.wt[d])
         return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: element %d dimension %d needs 1..%d nodes", e, d,
                          HGX_MAX_NODES);
@@ -541,7 +544,7 @@ static int ensemble_tangent_run(Context* c, const hgx_ensemble_model* m, int ndi
       if (tj.elem[e].shape != m->elem[e].shape || tj.elem[e].nloc != m->elem[e].nloc)
         return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: tangent %d element %d differs in shape or locations",
                          j, e);
-      for (int d = 0; d < kEnsDims[m->elem[e].shape]; ++d)
+      for (int d = 0; d < analytic_dims(m->elem[e].shape, m->elem[e].params.flags); ++d)
         if ((tj.elem[e].params.val[d] || tj.elem[e].params.wt[d]) && tj.elem[e].params.n[d] != m->elem[e].params.n[d])
           return set_error(c, HGX_ERR_INVALID_ARG, "ensemble_tangent: tangent %d element %d dimension %d node count", j,
                            e, d);
@@ -562,7 +565,7 @@ static int ensemble_tangent_run(Context* c, const hgx_ensemble_model* m, int ndi
       EnsElemD& D = E[(size_t)j * ne + e];
       std::memset(&D, 0, sizeof(D));
       for (int d = 0; d < 4; ++d) {
-        if (d >= kEnsDims[el.shape]) continue;
+        if (d >= analytic_dims(el.shape, el.params.flags)) continue;
         D.P.n[d] = el.params.n[d];
         for (int i = 0; i < el.params.n[d]; ++i) {
           D.P.val[d][i] = dd{el.params.val[d][i], te.params.val[d] ? te.params.val[d][i] : 0.0};

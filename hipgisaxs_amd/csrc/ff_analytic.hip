@@ -54,8 +54,6 @@ __global__ __launch_bounds__(256) void k_ff_analytic32(QViewA g, Rot R, Nodes P,
 
 using namespace hgx;
 
-static const int kDims[7] = {1, 2, 3, 1, 2, 2, 4};
-
 extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* params, int precision,
                                const double rot[9], const double transvec[3], double* ff) {
   Context* c = reinterpret_cast<Context*>(h);
@@ -68,7 +66,7 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   Nodes P;
   for (int d = 0; d < 4; ++d) {
     P.n[d] = 0;
-    if (d >= kDims[shape]) continue;
+    if (d >= analytic_dims(shape, params->flags)) continue;
     int n = params->n[d];
     if (n < 1 || n > HGX_MAX_NODES || !params->val[d] || !params->wt[d])
       return set_error(c, HGX_ERR_INVALID_ARG, "ff_analytic: dimension %d needs 1..%d nodes with values and weights", d,
@@ -79,6 +77,8 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
       P.wt[d][i] = params->wt[d][i];
     }
   }
+  if (!analytic_heights_ok(shape, params->flags, params->n, params->val))
+    return set_error(c, HGX_ERR_INVALID_ARG, "ff_analytic: truncated sphere heights must lie in [0, 2 R]");
   P.tan_tau = std::tan(params->tau);
   P.sin_eta = std::sin(params->eta);
   P.cos_eta = std::cos(params->eta);

@@ -131,6 +131,22 @@ struct ProductUnit {
   int chunk;
 };
 
+// Node dimensions of an analytic shape (HGX_SHAPE_* order); a truncated sphere adds its heights.
+inline int analytic_dims(int shape, int flags) {
+  static const int k[7] = {1, 2, 3, 1, 2, 2, 4};
+  if (shape == HGX_SHAPE_SPHERE && (flags & HGX_SHAPE_TRUNCATED_SPHERE)) return 2;
+  return k[shape];
+}
+// A truncated sphere's heights must not exceed the diameter of any of its radii (the slice radii
+// sqrt(R^2 - z^2) are real); true when the nodes are usable.
+inline bool analytic_heights_ok(int shape, int flags, const int n[4], const double* const val[4]) {
+  if (shape != HGX_SHAPE_SPHERE || !(flags & HGX_SHAPE_TRUNCATED_SPHERE)) return true;
+  for (int i = 0; i < n[0]; ++i)
+    for (int k = 0; k < n[1]; ++k)
+      if (!(val[1][k] >= 0.0 && val[1][k] <= 2.0 * val[0][i])) return false;
+  return true;
+}
+
 // Planes of the split-fp16 adjoint weights: the real and imaginary parts, each as high and low fp16.
 constexpr int kAdjW16Planes = 4;
 

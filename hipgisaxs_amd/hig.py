@@ -18,7 +18,7 @@ from .api import HgxError
 LIB_PATH = Path(__file__).resolve().parent / "libhgx_hig.so"
 
 NO_FUSE, SF_COMPAT_LB2, SMEAR_COMPAT, NO_SMEAR, MESH_NO_TRANSLATION, PROFILE = 1, 2, 4, 8, 16, 32
-FIT_FD_GRADIENT, FIT_SEQUENTIAL = 64, 128
+FIT_FD_GRADIENT, FIT_SEQUENTIAL, TRUNCATED_SPHERE = 64, 128, 256
 
 _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 _SIGS = {

@@ -543,7 +543,9 @@ bool HipGISAXS::fit_gradient(const double* x, int metric, double reg, double* va
   if (!inited_ && !init()) return false;
   if (npix_ != fit_npix_) return fail("the image size changed after fit_init");
   // what the tangent pass covers: structures of analytic cells, the run_device checks passed
-  bool ok = !in_.structures.empty() && in_.compute.nslices <= 1 && in_.compute.structcorr == "nGnE";
+  // (truncated spheres have no tangent: those inputs take central differences)
+  bool ok = !in_.structures.empty() && in_.compute.nslices <= 1 && in_.compute.structcorr == "nGnE" &&
+            !(flags_ & HGX_HIG_TRUNCATED_SPHERE);
   for (const auto& s : in_.structures)
     ok = ok && s.second.grain_type.empty() && !s.second.scaling_dist && !s.second.repetition_dist;
   const double kDeg = 3.141592653589793 / 180;

@@ -422,7 +422,15 @@ bool HipGISAXS::shape_params(const HigShape& sh, hgx_shape_params& p, std::vecto
   std::memset(&p, 0, sizeof(p));
   std::vector<std::vector<std::string>> dims;
   switch (sh.shape) {
-    case HGX_SHAPE_SPHERE: dims = {{"radius"}}; break;
+    case HGX_SHAPE_SPHERE:
+      // the reference GPU path cuts spheres that carry a height (ff_ana_sphere.cpp:53-57, GPU :180-188)
+      if ((flags_ & HGX_HIG_TRUNCATED_SPHERE) && sh.params.count("height")) {
+        dims = {{"radius"}, {"height"}};
+        p.flags |= HGX_SHAPE_TRUNCATED_SPHERE;
+      } else {
+        dims = {{"radius"}};
+      }
+      break;
     case HGX_SHAPE_CYLINDER: dims = {{"radius"}, {"height"}}; break;
     case HGX_SHAPE_BOX: dims = {{"xsize", "edge"}, {"ysize", "edge"}, {"height", "edge"}}; break;
     case HGX_SHAPE_CUBE: dims = {{"height"}}; break;

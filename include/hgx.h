@@ -60,7 +60,7 @@ extern "C" {
 #define HGX_SF_FP32C 2      /* hgx_sf_lattice in FP32C: phases reduced in FP64, FP32 Dirichlet kernels (real q) */
 
 /* analytic shapes */
-#define HGX_SHAPE_SPHERE 0   /* dims: [radius] */
+#define HGX_SHAPE_SPHERE 0   /* dims: [radius] (+ [height] with HGX_SHAPE_TRUNCATED_SPHERE) */
 #define HGX_SHAPE_CYLINDER 1 /* dims: [radius, height] */
 #define HGX_SHAPE_BOX 2      /* dims: [xsize, ysize, height] */
 #define HGX_SHAPE_CUBE 3     /* dims: [edge] */
@@ -89,6 +89,14 @@ typedef struct hgx_shape_params {
  * have the opposite sign of the published truncated-pyramid form factor while the +-i factors keep
  * theirs; by default the published form (the volume integral of the pyramid) is used. */
 #define HGX_SHAPE_COMPAT_PYRAMID 1
+/* Sphere with heights (node dimension 1): the cap of height h cut from the top of the sphere and
+ * resting on z = 0, as the reference's GPU path evaluates a sphere with a height parameter
+ * (src/ff/gpu/ff_ana_sphere_gpu.cu:53-113): e^{i q_z (h - R)} times a 512-point trapezoid over
+ * z in [R - h, R] of the disks 2 pi r^2 J1(r Re q_par) / (r Re q_par) e^{i q_z z}, r^2 = R^2 - z^2, each
+ * disk 0 where |q_par| or r Re q_par is below 1e-13; every height at most the diameter of every
+ * radius. Without the flag heights are not read, as the reference CPU path (src/ff/ff_ana_sphere.cpp:
+ * 98-110). FP32C evaluates the cap in FP64 (its translation phase in FP32). */
+#define HGX_SHAPE_TRUNCATED_SPHERE 2
 
 /* ---- context ----------------------------------------------------------------------------- */
 

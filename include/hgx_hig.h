@@ -27,6 +27,9 @@ typedef struct hgx_hig hgx_hig;
 #define HGX_HIG_PROFILE 32          /* time every kernel of a run with HIP events (hgx_hig_kernel_ms) */
 #define HGX_HIG_FIT_FD_GRADIENT 64  /* LMVM gradients by the reference's central differences only */
 #define HGX_HIG_FIT_SEQUENTIAL 128  /* particle swarm: one synchronisation per particle, as the reference */
+#define HGX_HIG_TRUNCATED_SPHERE 256 /* spheres with a height parameter cut as the reference's GPU path cuts
+                                        them (HGX_SHAPE_TRUNCATED_SPHERE); by default heights are ignored,
+                                        as its CPU path does */
 
 /* A driver on HIP device `device`. Reading inputs, hgx_hig_info, hgx_hig_config and hgx_hig_grains
  * run on the host; the device context (one hgx context) is created by the first call that needs it. */

@@ -203,6 +203,40 @@ def _dims(dims):
     return out
 
 
+def ff_truncated_sphere(dims, g: Grid, rot=None, transvec=None, slices=512) -> np.ndarray:
+    """The reference GPU's sphere with heights (src/ff/gpu/ff_ana_sphere_gpu.cu:53-113): per radius R and
+    height h, e^{i mq_z (h - R)} times the trapezoid (`slices` points, ends weighted 1/2) over z in
+    [R - h, R] of the disks 2 pi r^2 J1(r Re q_par)/(r Re q_par) e^{i mq_z z}, r = sqrt(R^2 - z^2), a disk
+    being 0 where |q_par| < 1e-13 or r Re q_par < 1e-13; summed with the node weights, times e^{i mq.t}."""
+    from scipy.special import j1
+
+    (rv, rw), (hv, hw) = ((np.asarray(v, float), np.asarray(w, float)) for v, w in dims)
+    R = np.eye(3) if rot is None else np.asarray(rot, float).reshape(3, 3)
+    pix = np.arange(g.nq) % g.npix
+    q = np.stack([np.asarray(g.qx, float)[pix] + 0j, np.asarray(g.qy, float)[pix] + 0j, np.asarray(g.qz, complex)])
+    mq = R @ q
+    qp = np.sqrt(mq[0] * mq[0] + mq[1] * mq[1])
+    ok = np.abs(qp) >= 1e-13
+    out = np.zeros(g.nq, complex)
+    for r0, wr in zip(rv, rw):
+        for h0, wh in zip(hv, hw):
+            dh = h0 / (slices - 1)
+            acc = np.zeros(g.nq, complex)
+            for s in range(slices):
+                z = s * dh - (h0 - r0)
+                rz = np.sqrt(max(r0 * r0 - z * z, 0.0))  # the reference: NaN past the pole by rounding
+                t2 = rz * qp.real
+                use = ok & (t2 >= 1e-13)
+                disk = np.zeros(g.nq)
+                disk[use] = 2 * np.pi * rz * rz * j1(t2[use]) / t2[use]
+                acc += (1.0 if s in (0, slices - 1) else 2.0) * disk * np.exp(1j * mq[2] * z)
+            out += wh * wr * np.exp(1j * mq[2] * (h0 - r0)) * (0.5 * dh) * acc
+    if transvec is not None:
+        t = np.asarray(transvec, float)
+        out *= np.exp(1j * (mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]))
+    return out
+
+
 def ff_analytic(shape: str, dims, g: Grid, rot=None, transvec=None, tau=0.0, eta=0.0, compat=0) -> np.ndarray:
     npix, nq, qx, qy, _, R = _qargs(g, rot)
     qz = _c2r(g.qz)

@@ -22,8 +22,8 @@ import re
 
 import numpy as np
 
-from . import (dwba_intensity, ff_analytic, ff_approx, fresnel, param_distribution, qgrid_qspace, qz_extended,
-               sf_lattice, tri9_to_rec8)
+from . import (dwba_intensity, ff_analytic, ff_approx, ff_truncated_sphere, fresnel, param_distribution, qgrid_qspace,
+               qz_extended, sf_lattice, tri9_to_rec8)
 
 PI_ = 3.141592653589793
 SQRT_2PI_ = 2.506628
@@ -520,10 +520,12 @@ def grain_sets(st, gr):
     return out
 
 
-def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True, model=None):
+def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", mesh_translate=True, model=None,
+        truncated_sphere=False):
     """Image [nrow][ncol] of configuration `config` (alpha_i outermost, as run_all_gisaxs loops) for
     .hig text (or an already read and edited `model`); grain orientations drawn from rand() after
-    srand(seed) (earlier configurations draw first)."""
+    srand(seed) (earlier configurations draw first). truncated_sphere: spheres with a height parameter
+    as the reference's GPU path cuts them (its CPU path ignores the height)."""
     m = read_model(text) if model is None else model
     sc, cp = m["scattering"], m["compute"]
     keV = sc["photon"] / 1000 if sc["unit"] == "ev" else sc["photon"]
@@ -596,6 +598,11 @@ def run(text, seed=1, sf_flags=0, smear_mode="fixed", config=0, base_dir=".", me
                         for loc in els[skey]:
                             if sh["name"].endswith((".obj", ".dat")):
                                 ff += dn2 * mesh_ff(f"{base_dir}/{sh['name']}", g, srot, loc, mesh_translate)
+                            elif truncated_sphere and sh["name"] == "sphere" and "height" in sh["params"]:
+                                hp = sh["params"]["height"]
+                                hd = param_distribution(hp["min"], hp["max"], int(hp["nvalues"]), _STAT[hp["stat"]],
+                                                        hp["p1"], hp["p2"], reference=True)
+                                ff += dn2 * ff_truncated_sphere(shape_dims(sh) + [hd], g, rot=srot, transvec=loc)
                             else:
                                 ff += dn2 * ff_analytic(sh["name"], shape_dims(sh), g, rot=srot, transvec=loc)
                     sf = sf_lattice(la, lb, lc, repet, center, g, 0.0, sf_flags, rot=R)

@@ -99,6 +99,33 @@ def test_analytic_fp32c_matches_oracle(gpu_ctx, oracle_mod, det_grid, shape):
     assert r.max() <= 1e-2
 
 
+@pytest.mark.parametrize("precision,dnl", [("fp64", 0j), ("fp64", 2e-6 + 1e-8j), ("fp32c", 0j)])
+def test_truncated_sphere_matches_oracle(gpu_ctx, oracle_mod, precision, dnl):
+    """Spheres cut to a height (the reference GPU path's 512-slice cap) with radius and height
+    distributions, a rotation and a translation, on real and absorbing-layer branches; FP32C evaluates
+    the cap in FP64 and the translation phase in FP32. A height above a diameter is refused."""
+    from hipgisaxs_amd import _native as N
+    from hipgisaxs_amd.api import HgxError
+
+    O = oracle_mod
+    g = O.qz_extended(O.qgrid_qspace(40, 30, -1.5, 0.0, 1.5, 1.5, AI, K0), K0, AI, dnl)
+    _upload(gpu_ctx, g)
+    dims = [([4.5, 5.0], [0.4, 0.6]), ([6.0, 8.5], [0.7, 0.3])]
+    R = _rot(0.4, 0.1)
+    tv = np.array([1.0, -2.0, 0.5])
+    ref = O.ff_truncated_sphere(dims, g, R, tv)
+    got = _gpu(gpu_ctx, gpu_ctx.ff_analytic("sphere", dims, precision, rot=R, transvec=tv,
+                                            flags=N.SHAPE_TRUNCATED_SPHERE))
+    print(f"truncated sphere {precision} dnl={dnl}: relL2 {rel_l2(got, ref):.2e}")
+    # FP32C: the cap in FP64, the translation phase in FP32
+    assert rel_l2(got, ref) <= (1e-10 if precision == "fp64" else 1e-6)
+    # without the flag the heights are not read: the full sphere
+    full = _gpu(gpu_ctx, gpu_ctx.ff_analytic("sphere", dims[:1], "fp64", rot=R, transvec=tv))
+    assert rel_l2(full, O.ff_analytic("sphere", dims[:1], g, R, tv)) <= 1e-10
+    with pytest.raises(HgxError):
+        gpu_ctx.ff_analytic("sphere", [([4.0], [1.0]), ([8.5], [1.0])], "fp64", flags=N.SHAPE_TRUNCATED_SPHERE)
+
+
 def test_analytic_fp32c_refuses_complex_q(gpu_ctx, oracle_mod):
     from hipgisaxs_amd.api import HgxError
 

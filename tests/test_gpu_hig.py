@@ -267,6 +267,23 @@ def test_grain_distributions_refused_where_the_reference_fails():
         H.HigSimulation(text=gauss).compute()
 
 
+@pytest.mark.parametrize("flags", [0, H.NO_FUSE], ids=["fused", "per_grain"])
+def test_truncated_spheres_match_oracle(flags):
+    """Spheres with a height parameter: the reference's GPU path cuts them (driver flag), its CPU path
+    ignores the height (default)."""
+    text = _spheres("repetition = [20 20 4], scaling = 15").replace(
+        'param = { type = "radius", min = 5 }',
+        'param = { type = "radius", min = 5 }, param = { type = "height", min = 6, max = 8, stat = "uniform", nvalues = 3 }')
+    cut = H.HigSimulation(text=text, flags=flags | H.TRUNCATED_SPHERE).compute()
+    ref = OH.run(text, truncated_sphere=True)
+    r = _rel(cut, ref)
+    _say("truncated spheres relL2", r)
+    assert r <= TOL64
+    full = H.HigSimulation(text=text, flags=flags).compute()
+    assert _rel(full, OH.run(text)) <= TOL64
+    assert _rel(full, ref) > 1e-2
+
+
 def test_errors_are_reported_not_fatal():
     text = _text("spheres_bcc").replace('unitcell:key = "cell"', 'unitcell:key = "nope"')
     sim = H.HigSimulation(text=text)

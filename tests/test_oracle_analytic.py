@@ -26,6 +26,49 @@ def test_sphere_is_spherical_bessel(oracle_mod):
     assert rel_l2(O.ff_analytic("sphere", [(r, w)], g), ref) < 1e-12
 
 
+def _cap_gauss(qx, qy, qz, R, h, n=96):
+    """The cap of height h cut from the top of a sphere of radius R and resting on z = 0, by
+    Gauss-Legendre quadrature over its disks: sum of 2 pi r^2 J1(q_par r)/(q_par r) e^{i qz z'}, z' in [0, h]."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    zp = 0.5 * h * (x + 1)                # height above the base
+    z = zp - (h - R)                      # from the sphere centre
+    r = np.sqrt(np.maximum(R * R - z * z, 0.0))
+    qp = np.sqrt(qx ** 2 + qy ** 2)[:, None]
+    t = qp * r[None, :]
+    disk = 2 * np.pi * r ** 2 * np.where(t > 0, special.j1(t) / np.where(t > 0, t, 1), 0.5)
+    return 0.5 * h * (disk * np.exp(1j * qz[:, None] * zp[None, :]) * w[None, :]).sum(axis=1)
+
+
+def test_truncated_sphere_is_the_cap_integral(oracle_mod):
+    """The reference GPU's truncated sphere (512-point trapezoid over disks): at h = 2R the disk
+    integral is the sphere (Gauss-Legendre against the closed form), a fine trapezoid converges to the
+    Gauss value for any cap, 512 points stay within the trapezoid error, and q_par = 0 gives 0."""
+    O = oracle_mod
+    rng = np.random.default_rng(7)
+    n = 60
+    qx, qy = rng.uniform(-1.2, 1.2, (2, n))
+    qz = rng.uniform(0.0, 1.2, n)
+    qy[0] = qx[0] = 0.0  # q_par = 0
+    g = O.Grid(n, 1, qx, qy, qz + 0j)
+    R = 5.0
+    full = O.ff_analytic("sphere", [([R], [1.0])], g)
+    sel = np.arange(1, n)
+    assert rel_l2(_cap_gauss(qx, qy, qz, R, 2 * R)[sel], full[sel]) < 1e-10
+    for h in (2 * R, 1.3 * R, 0.4 * R):
+        ref = _cap_gauss(qx, qy, qz, R, h)
+        coarse = O.ff_truncated_sphere([([R], [1.0]), ([h], [1.0])], g)
+        half = O.ff_truncated_sphere([([R], [1.0]), ([h], [1.0])], g, slices=1023)
+        assert coarse[0] == 0 and half[0] == 0
+        e512, e1023 = rel_l2(coarse[sel], ref[sel]), rel_l2(half[sel], ref[sel])
+        # the trapezoid rule's second-order error: half the spacing, a quarter of the error
+        assert e512 < 3e-5 and abs(e512 / e1023 - 4.0) < 0.01
+    # node weights and the transvec phase
+    two = O.ff_truncated_sphere([([R, 4.0], [0.3, 0.7]), ([6.0, 7.5], [0.6, 0.4])], g, transvec=[1.0, -2.0, 0.5])
+    want = sum(wr * wh * O.ff_truncated_sphere([([r], [1.0]), ([hh], [1.0])], g)
+               for r, wr in zip([R, 4.0], [0.3, 0.7]) for hh, wh in zip([6.0, 7.5], [0.6, 0.4]))
+    assert rel_l2(two, want * np.exp(1j * (qx - 2 * qy + 0.5 * qz))) < 1e-13
+
+
 def test_cylinder_is_bessel_j1(oracle_mod):
     O = oracle_mod
     g, qx, qy, qz = _grid(O)
