@@ -25,6 +25,7 @@ MESH_APPROX8, MESH_EXACT9 = 0, 1
 FF_APPROX, FF_EXACT = 0, 1
 FLAG_EXACT_REFERENCE_SIGN, FLAG_ACC_KAHAN, FLAG_NO_MFMA, FLAG_EXACT_CPU_THRESHOLD = 1, 2, 4, 8
 PRODUCT_FP32, PRODUCT_F16X2 = 0, 1
+PATH_PAIR, PATH_PRODUCT, PATH_EWALD = 0, 1, 2
 SF_COMPAT_LB2 = 1
 SF_FP32C = 2
 SMEAR_COMPAT = 1
@@ -160,6 +161,9 @@ _SIGS = {
     "hgx_qgrid_info": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hgx_qgrid_is_product": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "hgx_qgrid_is_ewald": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "hgx_ff_numeric_path": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "hgx_qgrid_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "hgx_mesh_upload": (C.c_int, [_vp, C.c_int, C.c_int64, _vp, C.POINTER(C.c_int)]),
     "hgx_mesh_release": (C.c_int, [_vp, C.c_int]),

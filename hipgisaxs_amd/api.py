@@ -135,6 +135,21 @@ class Context:
         self._check(self.lib.hgx_qgrid_is_product(self.h, C.byref(v)), "qgrid_is_product")
         return bool(v.value)
 
+    def is_ewald(self):
+        """(whether the grid has the Ewald-sphere form, the tile half-range of c times max |v| in 1/nm)"""
+        v, dv = C.c_int(), C.c_double()
+        self._check(self.lib.hgx_qgrid_is_ewald(self.h, C.byref(v), C.byref(dv)), "qgrid_is_ewald")
+        return bool(v.value), dv.value
+
+    def ff_numeric_path(self, mesh_id, method="approx", precision="fp64", flags=0):
+        """("pair" | "product" | "ewald", series orders or -1): the kernels ff_numeric would run"""
+        m = {"approx": N.FF_APPROX, "exact": N.FF_EXACT}[method]
+        p = {"fp64": N.PREC_FP64, "fp32c": N.PREC_FP32C}[precision]
+        path, orders = C.c_int(), C.c_int()
+        self._check(self.lib.hgx_ff_numeric_path(self.h, mesh_id, m, p, flags, C.byref(path), C.byref(orders)),
+                    "ff_numeric_path")
+        return {N.PATH_PAIR: "pair", N.PATH_PRODUCT: "product", N.PATH_EWALD: "ewald"}[path.value], orders.value
+
     def qgrid_get(self):
         i = self.info()
         qx = np.empty(i["npix"])

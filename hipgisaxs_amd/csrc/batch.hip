@@ -228,6 +228,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     if (prod) {
       // product grid: complex matrix products on the matrix cores (product_mfma.hip)
       is_prod[i] = 1;
+      d.e.series_qr = kSeriesQRSplit;
       d.bchunk = std::min<int64_t>(product_blocks_per_unit(c), m.nblocks);
       ProductItem& pi = pit[i];
       pi.qx = gc->d_qx;

@@ -536,6 +536,15 @@ struct OptTimer {
   }
 };
 
+// Series orders of the Ewald-grid matrix-core path for this call, or -1 when another kernel runs:
+// the FP32C approximate method on a grid with the Ewald factorization that is not a product grid,
+// with the matrix cores and block accumulation allowed, and few enough orders for the mesh's radius.
+int ewald_path_orders(const Context* c, const Mesh& m, int method, int precision, int flags) {
+  if (method != HGX_FF_APPROX || precision != HGX_PREC_FP32C || c->product || !c->ew.ok) return -1;
+  if (flags & (HGX_FLAG_NO_MFMA | HGX_FLAG_ACC_KAHAN)) return -1;
+  return ewald_orders(c, m.radius);
+}
+
 int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags, const double rot[9],
                    const double transvec[3], double* ff, const Work& work) {
   const bool timed = work.fixed == nullptr;
@@ -563,6 +572,7 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     // product_fp64.hip)
     const bool f64 = precision == HGX_PREC_FP64;
     e.use_series = f64 ? 0 : 1;
+    e.series_qr = kSeriesQRSplit;
     const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
                                              R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
     ProductItem it{};
@@ -627,6 +637,74 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
       product_launch_single(c->stream, it, cq, pmode);
     }
     HGX_CHECK_LAUNCH(c);
+    {
+      OptTimer lt(c, false, timed);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
+      HGX_CHECK_LAUNCH(c);
+    }
+    return HGX_OK;
+  }
+  const int ew_orders = ewald_path_orders(c, m, method, precision, flags);
+  if (ew_orders >= 0) {
+    // Ewald-sphere grid (QGrid::create, detector): matrix products with a short series in the
+    // per-row scale of q_par (ewald_mfma.hip)
+    e.use_series = 1;
+    e.series_qr = kSeriesQRSplit;
+    const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
+                                             R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
+    EwaldItem it{};
+    it.qz = c->d_qz;
+    it.row = c->ew.d_row;
+    it.col = c->ew.d_col;
+    it.tile = c->ew.d_tile;
+    it.perm = c->ew.d_perm;
+    it.nblocks = m.nblocks;
+    it.nq = c->nq;
+    it.ncol = c->ncol;
+    it.nrow = c->nrow;
+    it.ntile = c->ew.ntile;
+    it.snmax = (float)m.snmax;
+    it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
+                      m.radius);
+    // the expansion point: the mesh origin in the rotated frame of the records, R^T o
+    it.ox = R.m[0] * m.origin[0] + R.m[3] * m.origin[1] + R.m[6] * m.origin[2];
+    it.oy = R.m[1] * m.origin[0] + R.m[4] * m.origin[1] + R.m[7] * m.origin[2];
+    it.oz = R.m[2] * m.origin[0] + R.m[5] * m.origin[1] + R.m[8] * m.origin[2];
+    const int64_t tiles = (int64_t)it.ntile * ceil_div(it.ncol, kProductWaves * kProductTile);
+    int64_t want = std::max<int64_t>(1, (int64_t)(c->tune.target_wg + tiles - 1) / tiles);
+    want = std::min<int64_t>(want, m.nblocks);
+    it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit(c));
+    const size_t rec_bytes =
+        identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
+    {
+      const int64_t lim = chunk_limit(work, c->nq, rec_bytes);
+      if ((m.nblocks + it.bchunk - 1) / it.bchunk > lim) it.bchunk = (m.nblocks + lim - 1) / lim;
+    }
+    it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
+    if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
+    const size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
+    char* w = nullptr;
+    if (part_bytes + rec_bytes > 0)
+      if (int rc = take_work(c, work, part_bytes + rec_bytes, &w)) return rc;
+    it.part = it.nchunks > 1 ? reinterpret_cast<double*>(w) : nullptr;
+    it.out = ff;
+    if (identity) {
+      it.rec = m.d_rec32;
+      it.borig = m.d_borig;
+    } else {
+      float* rr = reinterpret_cast<float*>(w + part_bytes);
+      double* ro = reinterpret_cast<double*>(w + part_bytes + (size_t)m.nblocks * kBlockTri * 8 * sizeof(float));
+      {
+        OptTimer lt(c, false, timed);
+        if (int rc = product_rotate_mesh(c, m, R, rr, ro, c->stream)) return rc;
+      }
+      it.rec = rr;
+      it.borig = ro;
+    }
+    {
+      OptTimer lt(c, true, timed);
+      if (int rc = ewald_launch(c, it, cq, ew_orders)) return rc;
+    }
     {
       OptTimer lt(c, false, timed);
       k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
@@ -761,4 +839,28 @@ extern "C" int hgx_ff_numeric_noalloc(hgx_ctx* h, int mesh_id, int method, int p
   if (!c->d_scratch)
     return set_error(c, HGX_ERR_DEVICE, "ff_numeric_noalloc: the scratch array of the device was not resolved");
   return ff_numeric_run(c, mesh_id, method, precision, flags, rot, transvec, ff, Work{c->d_scratch, kScratchBytes});
+}
+
+extern "C" int hgx_ff_numeric_path(const hgx_ctx* h, int mesh_id, int method, int precision, int flags, int* path,
+                                   int* orders) {
+  const Context* c = reinterpret_cast<const Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (!path) return HGX_ERR_INVALID_ARG;
+  if (!c->d_qz) return HGX_ERR_NO_QGRID;
+  if (mesh_id < 0 || mesh_id >= (int)c->meshes.size() || !c->meshes[mesh_id].live) return HGX_ERR_BAD_HANDLE;
+  if (method != HGX_FF_APPROX && method != HGX_FF_EXACT) return HGX_ERR_INVALID_ARG;
+  if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C) return HGX_ERR_INVALID_ARG;
+  const int k = ewald_path_orders(c, c->meshes[mesh_id], method, precision, flags);
+  if (orders) *orders = k;
+  if (method == HGX_FF_APPROX && c->product && !(flags & HGX_FLAG_NO_MFMA)) *path = HGX_PATH_PRODUCT;
+  else *path = k >= 0 ? HGX_PATH_EWALD : HGX_PATH_PAIR;
+  return HGX_OK;
+}
+
+extern "C" int hgx_qgrid_is_ewald(const hgx_ctx* h, int* is_ewald, double* dv) {
+  const Context* c = reinterpret_cast<const Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (is_ewald) *is_ewald = c->ew.ok ? 1 : 0;
+  if (dv) *dv = c->ew.ok ? c->ew.dmax * c->ew.vmax : 0.0;
+  return HGX_OK;
 }

@@ -166,6 +166,8 @@ static void free_qgrid(Context* c) {
   c->ncol = c->nrow = c->nbranch = 0;
   c->have_alpha = false;
   c->product = false;
+  c->qz_rows = false;
+  ewald_free(c);
 }
 
 // Allocates grid arrays for nrow x ncol pixels and nbranch branches.
@@ -268,19 +270,21 @@ __global__ void k_qz_extended(int64_t npix, const double* __restrict__ qz_in, do
   }
 }
 
-// Clears *flag unless every q-point equals the product of the grid's axes: qx = qx[0],
-// qy = qy[column] of row 0 and qz = qz of column 0 of its (branch, row).
+// flag[0] is cleared unless every q-point equals the product of the grid's axes: qx = qx[0], qy =
+// qy[column] of row 0 and qz = qz of column 0 of its (branch, row); flag[1] unless qz depends on the
+// (branch, row) only.
 __global__ void k_check_product(int ncol, int64_t npix, int nbranch, const double* __restrict__ qx,
                                 const double* __restrict__ qy, const double* __restrict__ qz, int* __restrict__ flag) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npix) return;
   int64_t row = p / ncol, j = p % ncol;
-  bool ok = qx[p] == qx[0] && qy[p] == qy[j];
+  bool rows = true;
   for (int b = 0; b < nbranch; ++b) {
     int64_t q = b * npix + p, q0 = b * npix + row * ncol;
-    ok = ok && qz[2 * q] == qz[2 * q0] && qz[2 * q + 1] == qz[2 * q0 + 1];
+    rows = rows && qz[2 * q] == qz[2 * q0] && qz[2 * q + 1] == qz[2 * q0 + 1];
   }
-  if (!ok) *flag = 0;  // every writer stores the same value
+  if (!(rows && qx[p] == qx[0] && qy[p] == qy[j])) flag[0] = 0;  // every writer stores the same value
+  if (!rows) flag[1] = 0;
 }
 
 }  // namespace hgx
@@ -289,22 +293,26 @@ using namespace hgx;
 
 static Context* ctx_of(hgx_ctx* c) { return reinterpret_cast<Context*>(c); }
 
-// Decides whether the current grid is a product grid (run once per grid change; synchronous).
+// Decides whether the current grid is a product grid and whether it has the Ewald-grid
+// factorization (run once per grid change; synchronous).
 static int update_product(Context* c) {
   c->product = false;
-  if (!c->d_flag && hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess) {
+  c->qz_rows = false;
+  ewald_free(c);
+  if (!c->d_flag && hipMalloc(&c->d_flag, 2 * sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     return set_error(c, HGX_ERR_OOM, "q-grid: flag allocation failed");
   }
-  int one = 1, got = 0;
-  HGX_CHECK_HIP(c, hipMemcpyAsync(c->d_flag, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  int one[2] = {1, 1}, got[2] = {0, 0};
+  HGX_CHECK_HIP(c, hipMemcpyAsync(c->d_flag, one, sizeof(one), hipMemcpyHostToDevice, c->stream));
   k_check_product<<<ceil_div(c->npix, 256), 256, 0, c->stream>>>(c->ncol, c->npix, c->nbranch, c->d_qx, c->d_qy,
                                                                  c->d_qz, c->d_flag);
   HGX_CHECK_LAUNCH(c);
-  HGX_CHECK_HIP(c, hipMemcpyAsync(&got, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HGX_CHECK_HIP(c, hipMemcpyAsync(got, c->d_flag, sizeof(got), hipMemcpyDeviceToHost, c->stream));
   HGX_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-  c->product = got != 0;
-  return HGX_OK;
+  c->product = got[0] != 0;
+  c->qz_rows = got[1] != 0;
+  return c->product || !c->qz_rows ? HGX_OK : ewald_update(c);
 }
 static const Context* ctx_of(const hgx_ctx* c) { return reinterpret_cast<const Context*>(c); }
 
@@ -482,7 +490,7 @@ int hgx_qgrid_qspace(hgx_ctx* h, int ncol, int nrow, double qy_min, double qz_mi
   HGX_CHECK_LAUNCH(c);
   c->qz_real = true;
   c->have_alpha = true;
-  return HGX_OK;
+  return update_product(c);
 }
 
 int hgx_qgrid_explicit(hgx_ctx* h, int ncol, int nrow, double qy_min, double qz_min, double qy_max,
@@ -514,7 +522,7 @@ int hgx_qgrid_detector(hgx_ctx* h, int ncol, int nrow, double pixel, double dist
   HGX_CHECK_LAUNCH(c);
   c->qz_real = true;
   c->have_alpha = true;
-  return HGX_OK;
+  return update_product(c);
 }
 
 int hgx_qgrid_set(hgx_ctx* h, int ncol, int nrow, int nbranch, const double* qx, const double* qy,
@@ -557,8 +565,9 @@ int hgx_qz_extended(hgx_ctx* h, double k0, double alpha_i, double dnl_re, double
   c->nq = 4 * c->npix;
   // kzi and kzf are real exactly when the layer is lossless and every kzf0^2 >= k0^2 dnl
   c->qz_real = (dnl_re == 0.0 && dnl_im == 0.0);
-  // the branches depend on the row only, so a product grid stays one
-  return c->product ? update_product(c) : HGX_OK;
+  // the branches depend on the row only, so a product grid stays one; the Ewald-grid tables are
+  // rebuilt for the four branches
+  return update_product(c);
 }
 
 int hgx_qgrid_info(const hgx_ctx* h, int64_t* npix, int64_t* nq, int* nrow, int* ncol, int* nbranch) {

@@ -206,6 +206,50 @@ inline int product_ycols(int mode) {
 }
 
 
+// ----- Ewald-sphere q-grids on the matrix cores (ewald_mfma.hip) --------------------------------
+//
+// QGrid::create (src/model/qgrid.cpp:76-84) and the detector geometry give
+//     q(row i, column j, branch b) = (k0 (cos af_i cos tth_j - cos ai), k0 cos af_i sin tth_j, qz_{b,i}),
+// which is q = (a_i + c_i vx_j, c_i vy_j, qz_{b,i}) with a_i = k0 (cos af_i - cos ai), c_i = cos af_i
+// and v_j = k0 (cos tth_j - 1, sin tth_j): a row part, and a small column vector scaled per row. Any
+// grid of that form (fitted from the grid arrays, not assumed from how they were made) runs the
+// approximate-triangle sum on the matrix cores: rows sorted by c are tiled 64 at a time, the tile's
+// mid value cbar goes into the column operands exactly and exp(i (c_i - cbar) v_j . (c_t - o)) is
+// expanded in a short series whose terms are products of row and column factors.
+struct EwaldGrid {
+  bool ok = false;
+  int ntile = 0;            // 64-row tiles of the branch-major rows sorted by c
+  double dmax = 0.0;        // largest half-range D of c over a tile
+  double vmax = 0.0;        // largest |v_j|
+  void* d_mem = nullptr;    // one allocation holding the four tables below
+  const double* d_row = nullptr;   // [nrow][2] (a_i, c_i)
+  const double* d_col = nullptr;   // [ncol][2] (vx_j, vy_j)
+  const double* d_tile = nullptr;  // [ntile][2] (cbar, D)
+  const int* d_perm = nullptr;     // [ntile * 64] branch-major row of each tile slot, -1 past the end
+};
+
+// One form factor for the Ewald-grid kernel; records and block origins rotated by R^T as for the
+// product kernel, o = R^T (mesh origin) the expansion point.
+struct EwaldItem {
+  const double* qz;     // grid qz: row z = branch * nrow + row at qz + 2 z ncol
+  const double* row;
+  const double* col;
+  const double* tile;
+  const int* perm;
+  const float* rec;     // [nblocks * kBlockTri][8] rotated s*n (3), c - o_b (3), 0, 0
+  const double* borig;  // [nblocks][4] rotated block origins
+  double ox, oy, oz;
+  int64_t nblocks;
+  int64_t bchunk;
+  int64_t nq;
+  int nchunks;
+  int ncol, nrow, ntile;
+  float snmax;          // max_t |s_t n_t|
+  float rmax;           // bound on |c_t| (complex qz attenuation)
+  double* part;         // [nchunks][nq] complex partial sums (nchunks > 1)
+  double* out;          // [nq] complex raw sums (nchunks == 1)
+};
+
 // Launch-shape settings of the numeric kernels, per context (hgx_tune_*; the autotuner sets them).
 // Batches and grating plans copy what they need when they are created.
 struct Tuning {
@@ -232,7 +276,9 @@ struct Context {
   int64_t npix = 0, nq = 0;
   bool qz_real = true;      // all Im(qz) == 0
   bool product = false;     // q(row, col, branch) = (qx0, qy[col], qz[branch, row]) exactly
-  int* d_flag = nullptr;    // device scratch flag for the product-grid check
+  bool qz_rows = false;     // qz depends on (branch, row) only
+  EwaldGrid ew;             // the Ewald-grid factorization, when the grid has one
+  int* d_flag = nullptr;    // device scratch flags of the grid checks (2 ints)
   bool have_alpha = false;
   double* d_qx = nullptr;   // [npix]
   double* d_qy = nullptr;   // [npix]
@@ -468,6 +514,13 @@ int adjoint_product_variant(const Context* c);  // the context's setting (hgx_tu
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq, int variant);
 // Tuning: triangle blocks per work unit and the kernel variant (kProduct* above); the FP64 variant.
+// Ewald-grid tables of the current grid (ewald_mfma.hip): fitted after every grid change, freed with
+// the grid. ewald_orders gives the series orders K (0..3) the matrix-core path needs for a mesh of
+// this radius about its origin, or -1 when it would need more (the per-pair kernels then run).
+int ewald_update(Context* c);
+void ewald_free(Context* c);
+int ewald_orders(const Context* c, double radius);
+int ewald_launch(Context* c, const EwaldItem& it, bool cq, int orders);
 int product_blocks_per_unit(const Context* c);
 int product_mode(const Context* c);
 inline int product_mode64(const Context* c);

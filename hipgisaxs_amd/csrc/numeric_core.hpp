@@ -9,6 +9,10 @@ namespace hgx {
 
 constexpr int kWG = 256;
 constexpr double kSeriesQR = 0.1;
+// Series threshold of the split-fp16 matrix-core paths: their operands carry about 22 bits, so the
+// direct sum's cancellation near q = 0 (a loss of about 3 / (|q| R)) is kept further away; the order-8
+// series is still good to 5e-10 of F at |q| R = 0.3.
+constexpr double kSeriesQRSplit = 0.3;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kInvTwoPi = 0.15915494309189533576888376337251;
 
@@ -28,6 +32,7 @@ struct Epi {
   double origin[3];
   double radius;
   int use_series;
+  double series_qr = kSeriesQR;  // the series replaces the direct sum below |q| R = series_qr
 };
 
 // Loads and rotates q-point q; with CQ = false the grid is known to be real and the imaginary
@@ -42,7 +47,7 @@ __device__ inline void load_q(const QView& g, const Rot& R, int64_t q, cd mq[3])
 
 // sum_{k=0..K} (i^k / k!) sum_t s_t (q.n_t) (q.(c_t - o))^k from the mesh moments. Monomials are
 // built by repeated multiplication (no indexed arrays, so nothing spills to scratch); this runs
-// only for the few q-points with |q| R < kSeriesQR.
+// only for the few q-points with |q| R below the series threshold.
 __device__ inline cd cpow_small(cd z, int n) {
   cd r = cd{1.0, 0.0};
   for (int i = 0; i < n; ++i) r = r * z;
@@ -87,7 +92,7 @@ __device__ inline cd finish_q(const Epi& e, const cd mq[3], cd P) {
   cd F;
   if (q2 == 0.0) {
     F = cd{e.volume, 0.0};
-  } else if (e.use_series && sqrt(q2) * e.radius < kSeriesQR) {
+  } else if (e.use_series && sqrt(q2) * e.radius < e.series_qr) {
     cd S = moment_series(e.moments, mq);
     cd ph = mq[0] * e.origin[0] + mq[1] * e.origin[1] + mq[2] * e.origin[2];
     F = (1.0 / q2) * mul_mi(S * cexpi_d(ph));

@@ -23,10 +23,9 @@
 // added in FP64 by the finishing kernel, which also applies -i/|q|^2 and the small-|q| series
 // exactly as for the per-pair path.
 #include "numeric_core.hpp"
+#include "split16.hpp"
 
 namespace hgx {
-
-typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 int product_blocks_per_unit(const Context* c) { return c->tune.product_bpu; }
 int product_mode(const Context* c) {
@@ -65,9 +64,6 @@ __global__ __launch_bounds__(256) void k_rotate_records(const float* __restrict_
 __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
 }
-
-// accumulator register rr of lane (h = lane / 32) holds tile row 8 (rr / 4) + 4 h + rr % 4
-__device__ inline int acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (rr & 3); }
 
 template <bool CQ, bool GAUSS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_ff_product32(const ProductItem* __restrict__ items,
@@ -248,28 +244,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // The three cross terms are three MFMAs on whole operand registers (high x high, low x high,
 // high x low), so no operand is copied: per row tile and complex part, 3 MFMAs per 4 triangles.
 
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-
-// two FP32 values to fp16, rounded to nearest (v_cvt_pk_f16_f32)
-__device__ inline f16x2 pk16(float a, float b) { return __builtin_convertvector((f32x2){a, b}, f16x2); }
-
-// (v.y, -v.x): the pair for the real part from the pair for the imaginary part (exact)
-__device__ inline f16x2 swap_neg(f16x2 v) { return (f16x2){v[1], -v[0]}; }
-
-__device__ inline f16x8 cat4(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
-  typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-  f16x4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
-  f16x4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
-  return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
-}
-
-__device__ inline f32x16 mfma16(f16x8 a, f16x8 b, f32x16 c) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-}
-
 // One triangle's y-side operand pairs for this lane's column: for the imaginary part (Y1i, Y1r),
 // (Eyi, Eyr) and for the real part (Y1r, -Y1i), (Eyr, -Eyi); high halves h*, low halves l*.
 struct YSplit {
@@ -328,14 +302,6 @@ __device__ inline void split_z(const float4 ra, const float4 rc, float uz, float
   o.h1 = pk16(z2r, z2i);
   o.l0 = pk16(ezr - (float)o.h0[0], ezi - (float)o.h0[1]);
   o.l1 = pk16(z2r - (float)o.h1[0], z2i - (float)o.h1[1]);
-}
-
-// smallest e >= 0 with v <= 2^e (v = 0 gives 0)
-__device__ inline int bound_exp(float v) {
-  if (!(v > 0.0f)) return 0;
-  int e;
-  (void)frexpf(v, &e);  // v = m 2^e, m in [1/2, 1)
-  return max(e, 0);
 }
 
 // MZ: row tiles per wave; TWO: each 128-triangle block's sums go to separate accumulators and are

@@ -156,6 +156,13 @@ int hgx_qgrid_info(const hgx_ctx* ctx, int64_t* npix, int64_t* nq, int* nrow, in
  * FP32C approximate-method form factors on such grids run as complex matrix products on the
  * matrix cores. No reference counterpart. */
 int hgx_qgrid_is_product(const hgx_ctx* ctx, int* is_product);
+/* Whether the current grid has the Ewald-sphere form of QGrid::create (src/model/qgrid.cpp:76-84) and
+ * of detector grids: q = (a_i + c_i vx_j, c_i vy_j, qz(branch, row i)) for row values a_i, c_i and
+ * column values v_j, fitted from the grid arrays to 1e-10 (1 + max|q|) whenever the grid changes.
+ * dv (may be NULL) receives the largest half-range of c over a 64-row tile of the rows sorted by c,
+ * times the largest |v_j| (1/nm): a mesh of radius R about its origin needs series orders for
+ * x = dv R (see hgx_ff_numeric_path). No reference counterpart. */
+int hgx_qgrid_is_ewald(const hgx_ctx* ctx, int* is_ewald, double* dv);
 /* Copies the grid back to host arrays (any pointer may be NULL). */
 int hgx_qgrid_get(hgx_ctx* ctx, double* qx, double* qy, double* qz_ext, double* alpha_f);
 
@@ -188,6 +195,17 @@ int hgx_mesh_info(const hgx_ctx* ctx, int mesh_id, int64_t* ntri, double* volume
  * At q = 0 the result is the mesh volume (the reference divides 0/0 there). */
 int hgx_ff_numeric(hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, const double rot[9],
                    const double transvec[3], double* ff);
+
+/* kernels of hgx_ff_numeric (hgx_ff_numeric_path) */
+#define HGX_PATH_PAIR 0    /* per-q-point VALU kernels (any grid; both methods) */
+#define HGX_PATH_PRODUCT 1 /* approximate method on a product grid, on the matrix cores */
+#define HGX_PATH_EWALD 2   /* FP32C approximate method on an Ewald-sphere grid (hgx_qgrid_is_ewald) that is
+                              not a product grid, on the matrix cores with a series of *orders + 1 terms
+                              in the per-row scale of q_par; used when x = dv R needs at most 3 orders */
+/* Which kernels hgx_ff_numeric would run for these arguments on the current grid: *path = HGX_PATH_*,
+ * *orders (may be NULL) the Ewald series orders (-1 on the other paths). No reference counterpart. */
+int hgx_ff_numeric_path(const hgx_ctx* ctx, int mesh_id, int method, int precision, int flags, int* path,
+                        int* orders);
 
 /* hgx_ff_numeric without any device allocation: the same kernels and arguments, with their partial
  * sums and rotated records in a fixed per-device scratch array of 32 MiB instead of the context's
