@@ -312,6 +312,99 @@ def bench_pyramid_fp64(reps=5):
                         "exact-triangle FF of the same mesh and grid"}
 
 
+def ewald_flop_per_pair(orders):
+    """Matrix-core flop per pair the Ewald-grid kernel executes with series orders 0..K: split-fp16
+    orders 0 and 1 take 48 (two complex multiply-adds x 4 real products x 3 cross terms), single-fp16
+    orders from 2 up take 16."""
+    return 48 * min(orders + 1, 2) + 16 * max(orders - 1, 0)
+
+
+def bench_ewald(nshapes=64, reps=2):
+    """The 64 seeded convex polyhedra of the batched workload on the reference's own q-grids: each on
+    QGrid::create's q-space grid of its side (qy [-2, 2], qz [0, 2] 1/nm, 10 keV, alpha_i 0.2 deg) with
+    the four DWBA branches of a particle in vacuum, FP32C approximate method, one call per shape. The
+    Ewald-grid matrix-core kernel against the per-pair FP32C kernel on the same calls, both against
+    the FP64 per-pair kernel (intensity relL2 and per-pixel errors)."""
+    import torch
+    from hipgisaxs_amd import _native as N
+    from hipgisaxs_amd.api import Context
+
+    k0 = 2 * math.pi / (1.23984 / 10.0)
+    ai = math.radians(0.2)
+    shapes = make_workload(nshapes)
+    ctxs = {}
+    for side in sorted({s["side"] for s in shapes}):
+        c = Context(0)
+        c.qgrid_qspace(side, side, -2.0, 0.0, 2.0, 2.0, ai, k0)
+        c.qz_extended(k0, ai, 0j)
+        ctxs[side] = c
+    pairs = 0.0
+    orders = {}
+    flop_exec = 0.0
+    for s in shapes:
+        c = ctxs[s["side"]]
+        s["mid"] = c.upload_mesh(s["tri"], "exact9")
+        s["ntri"] = c.mesh_info(s["mid"])["ntri"]
+        path, k = c.ff_numeric_path(s["mid"], "approx", "fp32c")
+        orders[f"{path}:{k}"] = orders.get(f"{path}:{k}", 0) + 1
+        p = float(s["ntri"]) * c.info()["nq"]
+        pairs += p
+        flop_exec += p * (ewald_flop_per_pair(k) if path == "ewald" else float("nan"))
+        s["out"] = {m: torch.empty(2 * c.info()["nq"], dtype=torch.float64, device="cuda:0")
+                    for m in ("ewald", "pair", "fp64")}
+
+    def run(kind, flags, prec, n):
+        for c in ctxs.values():
+            c.profile_begin()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            for s in shapes:
+                ctxs[s["side"]].ff_numeric(s["mid"], "approx", prec, flags=flags, out=s["out"][kind])
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / n
+        prof = [c.profile_end() for c in ctxs.values()]
+        kms = sum(p["main_ms"] for p in prof) / n
+        return t, kms
+
+    sched = {}
+    for pipe in (1, 0):  # the pipelined schedule, then the default one (whose outputs stay)
+        for c in ctxs.values():
+            c.lib.hgx_tune_ewald(c.h, pipe)
+        run("ewald", 0, "fp32c", 1)  # warm-up (first launches)
+        sched["pipelined" if pipe else "default"] = run("ewald", 0, "fp32c", reps)
+    t_ew, k_ew = sched["default"]
+    t_pp, k_pp = run("pair", N.FLAG_NO_MFMA, "fp32c", reps)
+    t_64, k_64 = run("fp64", 0, "fp64", 1)
+    pix = pixel_errors()
+    worst = 0.0
+    for s in shapes:
+        I64, Iew = intensity(s["out"]["fp64"]), intensity(s["out"]["ewald"])
+        pix.add(Iew, I64)
+        worst = max(worst, float(torch.linalg.vector_norm(Iew - I64) / torch.linalg.vector_norm(I64)))
+    num = sum(float(torch.linalg.vector_norm(intensity(s["out"]["ewald"]) - intensity(s["out"]["fp64"]))) ** 2
+              for s in shapes)
+    den = sum(float(torch.linalg.vector_norm(intensity(s["out"]["fp64"]))) ** 2 for s in shapes)
+    for c in ctxs.values():
+        c.close()
+    ok = not math.isnan(flop_exec)
+    return {"pairs": pairs, "series_orders": orders,
+            "ewald": {"value": pairs / t_ew, "ms": 1e3 * t_ew, "kernel_ms": k_ew, "kernel": "k_ff_ewald16",
+                      "roofline": {"bound": "mfma", "achieved": 16 * pairs / (k_ew * 1e-3) / 1e12,
+                                   "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
+                                   "frac": 16 * pairs / (k_ew * 1e-3) / PEAK_FP16, "flop_per_pair": 16,
+                                   "executed_tflops": flop_exec / (k_ew * 1e-3) / 1e12 if ok else None}},
+            "ewald_pipelined": {"value": pairs / sched["pipelined"][0], "kernel_ms": sched["pipelined"][1]},
+            "fp32c_per_pair": {"value": pairs / t_pp, "ms": 1e3 * t_pp, "kernel_ms": k_pp},
+            "fp64_per_pair": {"value": pairs / t_64, "ms": 1e3 * t_64, "kernel_ms": k_64},
+            "speedup_vs_fp32c_per_pair": t_pp / t_ew, "speedup_vs_fp64_per_pair": t_64 / t_ew,
+            "relL2_intensity_vs_fp64": math.sqrt(num / den), "relL2_worst_shape": worst,
+            "pixel_error_vs_fp64": pix.result(),
+            "workload": f"{nshapes} convex polyhedra of the batched workload, each on QGrid::create's q-space "
+                        "grid of its side (128-1024, qy [-2, 2], qz [0, 2], 10 keV, 0.2 deg) with 4 DWBA branches, "
+                        "FP32C approximate method, one call per shape"}
+
+
 def bench_lattice(reps=5):
     """Spheres R ~ N(5, 0.5) nm (21 nodes on [3, 7]) on bcc a = 20 nm, 10x10x5, 8 azimuths, DW 0.5 nm,
     1024^2 detector (2 m, 172 um), 12 keV, alpha_i = 0.15 deg, DWBA over Si: fused kernel, both paths."""
@@ -676,7 +769,7 @@ def main():
                     help="per-pair VALU kernels for FP32C even on product grids (no matrix cores)")
     ap.add_argument("--hig-only", action="store_true",
                     help="only the .hig driver leg (the reference's example models), as one JSON object")
-    ap.add_argument("--leg", choices=("pyramid", "lattice", "fit", "grating", "hig"), default=None,
+    ap.add_argument("--leg", choices=("pyramid", "lattice", "fit", "grating", "hig", "ewald"), default=None,
                     help="only that measurement leg, as one JSON object (for profiling one workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates HBM byte counters) and exit")
@@ -693,7 +786,8 @@ def main():
                           "hig_fit": bench_hig_fit(cpu=not args.no_cpu)}))
         return
     if args.leg is not None:
-        legs = {"pyramid": bench_pyramid_fp64, "lattice": bench_lattice, "fit": bench_fit, "grating": bench_grating}
+        legs = {"pyramid": bench_pyramid_fp64, "lattice": bench_lattice, "fit": bench_fit, "grating": bench_grating,
+                "ewald": bench_ewald}
         print(json.dumps({args.leg: legs[args.leg]()}))
         return
     if args.copy_probe:
@@ -932,6 +1026,8 @@ def main():
         print(f"[bench] fit done", file=sys.stderr, flush=True)
         result["grating_fit"] = bench_grating()
         print(f"[bench] grating done", file=sys.stderr, flush=True)
+        result["ewald_grids"] = bench_ewald(args.shapes)
+        print(f"[bench] ewald grids done", file=sys.stderr, flush=True)
         result["hig_examples"] = bench_hig_examples(cpu=not args.no_cpu)
         result["hig_fit"] = bench_hig_fit(cpu=not args.no_cpu)
         print(f"[bench] hig examples done", file=sys.stderr, flush=True)

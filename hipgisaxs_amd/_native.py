@@ -162,6 +162,7 @@ _SIGS = {
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hgx_qgrid_is_product": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "hgx_qgrid_is_ewald": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "hgx_tune_ewald": (C.c_int, [_vp, C.c_int]),
     "hgx_ff_numeric_path": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "hgx_qgrid_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -45,12 +45,13 @@ struct EwCol {
   float fy;      // 2^sy
 };
 
-// One triangle's column operand pairs for the orders 0 .. NO-1 (imaginary-part form: (Y1i, Y1r),
-// (Eyi, Eyr)); low halves for the first NS orders.
+// One triangle's column operand pairs for the orders 0 .. NO-1, in the imaginary-part form (Y1i, Y1r),
+// (Eyi, Eyr) and the real-part form (Y1r, -Y1i), (Eyr, -Eyi), each built by one conversion (the
+// negation is a source modifier); low halves for the first NS orders.
 template <int NO, int NS>
 struct EwTri {
-  f16x2 h0[NO], h1[NO];
-  f16x2 l0[NS > 0 ? NS : 1], l1[NS > 0 ? NS : 1];
+  f16x2 i0[NO], i1[NO], r0[NO], r1[NO];
+  f16x2 li0[NS], li1[NS], lr0[NS], lr1[NS];
 };
 
 template <int NO, int NS>
@@ -58,7 +59,7 @@ __device__ inline void ew_tri(const float4 ra, const float4 rc, const EwCol& k, 
   const float snx = ra.x, sny = ra.y, dcx = ra.w, dcy = rc.x;
   const float phy = __builtin_fmaf(k.uy, dcy, __builtin_fmaf(k.ux, dcx, by));
   float er = __builtin_amdgcn_cosf(phy) * k.fy, ei = __builtin_amdgcn_sinf(phy) * k.fy;
-  const float gam = __builtin_fmaf(k.gy, sny, k.gx * snx);                // s (v . n)
+  const float gam = __builtin_fmaf(k.gy, sny, k.gx * snx);                         // s (v . n)
   const float f = k.Df * __builtin_fmaf(k.gy, dcy, __builtin_fmaf(k.gx, dcx, pb));  // D v . (c - o)
 #pragma unroll
   for (int n = 0; n < NO; ++n) {
@@ -70,11 +71,17 @@ __device__ inline void ew_tri(const float4 ra, const float4 rc, const EwCol& k, 
       ei = ni;
     }
     const float yr = gam * er, yi = gam * ei;
-    o.h0[n] = pk16(yi, yr);
-    o.h1[n] = pk16(ei, er);
+    o.i0[n] = pk16(yi, yr);
+    o.i1[n] = pk16(ei, er);
+    o.r0[n] = pk16(yr, -yi);
+    o.r1[n] = pk16(er, -ei);
     if (n < NS) {
-      o.l0[n] = pk16(yi - (float)o.h0[n][0], yr - (float)o.h0[n][1]);
-      o.l1[n] = pk16(ei - (float)o.h1[n][0], er - (float)o.h1[n][1]);
+      const float dyi = yi - (float)o.i0[n][0], dyr = yr - (float)o.i0[n][1];
+      const float dei = ei - (float)o.i1[n][0], der = er - (float)o.i1[n][1];
+      o.li0[n] = pk16(dyi, dyr);
+      o.li1[n] = pk16(dei, der);
+      o.lr0[n] = pk16(dyr, -dyi);
+      o.lr1[n] = pk16(der, -dei);
     }
   }
 }
@@ -84,7 +91,7 @@ __device__ inline void ew_tri(const float4 ra, const float4 rc, const EwCol& k, 
 template <int NO, int NS>
 struct EwStep {
   f16x8 hre[NO], him[NO];
-  f16x8 lre[NS > 0 ? NS : 1], lim[NS > 0 ? NS : 1];
+  f16x8 lre[NS], lim[NS];
 };
 
 template <int NO, int NS>
@@ -96,21 +103,25 @@ __device__ inline EwStep<NO, NS> ew_step(const float4* __restrict__ rb, int ps, 
   EwStep<NO, NS> o;
 #pragma unroll
   for (int n = 0; n < NO; ++n) {
-    o.him[n] = cat4(T.h0[n], T.h1[n], U.h0[n], U.h1[n]);
-    o.hre[n] = cat4(swap_neg(T.h0[n]), swap_neg(T.h1[n]), swap_neg(U.h0[n]), swap_neg(U.h1[n]));
+    o.him[n] = cat4(T.i0[n], T.i1[n], U.i0[n], U.i1[n]);
+    o.hre[n] = cat4(T.r0[n], T.r1[n], U.r0[n], U.r1[n]);
   }
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
-    o.lim[n] = cat4(T.l0[n], T.l1[n], U.l0[n], U.l1[n]);
-    o.lre[n] = cat4(swap_neg(T.l0[n]), swap_neg(T.l1[n]), swap_neg(U.l0[n]), swap_neg(U.l1[n]));
+    o.lim[n] = cat4(T.li0[n], T.li1[n], U.li0[n], U.li1[n]);
+    o.lre[n] = cat4(T.lr0[n], T.lr1[n], U.lr0[n], U.lr1[n]);
   }
   return o;
 }
 
 // One work unit per workgroup: 64 sorted rows (tile zt) x 128 columns (column group yg) over one chunk
 // of triangle blocks; units run chunk-major, column group fastest.
-template <bool CQ, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) void k_ff_ewald16(EwaldItem it, int nyg) {
+// PIPE: the column operands of step p + 1 are built during step p and held in registers (one wave per
+// SIMD fits); otherwise each step builds its own before its products, which keeps the kernel within
+// two waves per SIMD, so two workgroups share a compute unit and cover each other's barriers.
+template <bool CQ, int K, bool PIPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 2, 8))) void k_ff_ewald16(EwaldItem it,
+                                                                                                  int nyg) {
   constexpr int MZ = 2;
   constexpr int NO = K + 1;
   constexpr int NS = K >= 1 ? 2 : 1;
@@ -230,7 +241,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) voi
     float by, pb, bz, bza, byn = 0.0f, pbn = 0.0f, bzn = 0.0f, bzan = 0.0f;
     phases(b0, by, pb, bz, bza);
     build(s_rec[0], 0, bz, bza, 0);
-    EwStep<NO, NS> Y = ew_step<NO, NS>(s_rec[0], 0, h, kc, by, pb);
+    EwStep<NO, NS> Y;
+    if constexpr (PIPE) Y = ew_step<NO, NS>(s_rec[0], 0, h, kc, by, pb);
     for (int lb = 0; lb < nb; ++lb) {
       const int cur = lb & 1;
       const bool more = lb + 1 < nb;
@@ -240,15 +252,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) voi
       for (int m = 0; m < MZ; ++m)
 #pragma unroll
         for (int k = 0; k < 16; ++k) PR[m][k] = PI[m][k] = 0.0f;
-      // step p: products of step p, operands of step p + 1 (the next block's step 0 after the last
-      // step; after the unit's last step they are built from valid records and never read)
+      // step p: products of step p, row operands of step p + 1 (the next block's step 0 after the
+      // last step; after the unit's last step they are built from valid records and never read), and
+      // the column operands of step p + 1 (PIPE) or of step p
       auto step = [&](int p) {
         const int buf = p & 1;
         const bool wrap = p == kSteps - 1;
         const float4* rb = s_rec[wrap && more ? cur ^ 1 : cur];
         const int ps = wrap ? 0 : p + 1;
         build(rb, ps, wrap ? bzn : bz, wrap ? bzan : bza, buf ^ 1);
-        const EwStep<NO, NS> Yn = ew_step<NO, NS>(rb, ps, h, kc, wrap ? byn : by, wrap ? pbn : pb);
+        EwStep<NO, NS> Yn;
+        if constexpr (PIPE) Yn = ew_step<NO, NS>(rb, ps, h, kc, wrap ? byn : by, wrap ? pbn : pb);
+        else Y = ew_step<NO, NS>(s_rec[cur], p, h, kc, by, pb);
 #pragma unroll
         for (int m = 0; m < MZ; ++m) {
 #pragma unroll
@@ -272,7 +287,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) voi
           __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
           __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
         }
-        Y = Yn;
+        if constexpr (PIPE) Y = Yn;
       };
       __syncthreads();
       if (more) {
@@ -325,13 +340,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) voi
     }
 }
 
-template <bool CQ>
+template <bool CQ, bool PIPE>
 void launch_orders(hipStream_t s, const EwaldItem& it, int orders, unsigned grid, int nyg) {
   switch (orders) {
-    case 0: k_ff_ewald16<CQ, 0><<<grid, 256, 0, s>>>(it, nyg); break;
-    case 1: k_ff_ewald16<CQ, 1><<<grid, 256, 0, s>>>(it, nyg); break;
-    case 2: k_ff_ewald16<CQ, 2><<<grid, 256, 0, s>>>(it, nyg); break;
-    default: k_ff_ewald16<CQ, 3><<<grid, 256, 0, s>>>(it, nyg); break;
+    case 0: k_ff_ewald16<CQ, 0, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
+    case 1: k_ff_ewald16<CQ, 1, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
+    case 2: k_ff_ewald16<CQ, 2, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
+    default: k_ff_ewald16<CQ, 3, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
   }
 }
 
@@ -342,8 +357,14 @@ int ewald_launch(Context* c, const EwaldItem& it, bool cq, int orders) {
   const int nyg = (int)ceil_div(it.ncol, kProductWaves * kProductTile);
   const int64_t units = (int64_t)nyg * it.ntile * it.nchunks;
   if (units <= 0 || units > INT32_MAX) return set_error(c, HGX_ERR_INVALID_ARG, "ewald: %lld work units", (long long)units);
-  if (cq) launch_orders<true>(c->stream, it, orders, (unsigned)units, nyg);
-  else launch_orders<false>(c->stream, it, orders, (unsigned)units, nyg);
+  const bool pipe = c->tune.ewald_pipe != 0;
+  if (cq) {
+    if (pipe) launch_orders<true, true>(c->stream, it, orders, (unsigned)units, nyg);
+    else launch_orders<true, false>(c->stream, it, orders, (unsigned)units, nyg);
+  } else {
+    if (pipe) launch_orders<false, true>(c->stream, it, orders, (unsigned)units, nyg);
+    else launch_orders<false, false>(c->stream, it, orders, (unsigned)units, nyg);
+  }
   HGX_CHECK_LAUNCH(c);
   return HGX_OK;
 }
@@ -464,3 +485,13 @@ int ewald_orders(const Context* c, double radius) {
 }
 
 }  // namespace hgx
+
+using namespace hgx;
+
+extern "C" int hgx_tune_ewald(hgx_ctx* h, int pipelined) {
+  Context* c = reinterpret_cast<Context*>(h);
+  if (!c) return HGX_ERR_BAD_HANDLE;
+  if (pipelined != 0 && pipelined != 1) return HGX_ERR_INVALID_ARG;
+  c->tune.ewald_pipe = pipelined;
+  return HGX_OK;
+}

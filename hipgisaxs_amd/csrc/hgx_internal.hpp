@@ -261,6 +261,7 @@ struct Tuning {
   int product_mz = 2, product_two = 1, product_shared = 1;
   int adjoint_variant = 4;  // 0: FP32 operands; 2 or 4: split fp16 with 2 or 4 column tiles per wave
   int product64_mults = 4;  // FP64 product-grid kernel: 4 or 3 (Gauss) real products per complex product
+  int ewald_pipe = 0;       // Ewald-grid kernel: column operands built a step ahead (1) or in their step (0)
 };
 
 struct Context {

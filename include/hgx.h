@@ -567,6 +567,11 @@ int hgx_tune_product64(hgx_ctx* ctx, int complex_mults);
  * counterpart. */
 int hgx_tune_product_tile(hgx_ctx* ctx, int row_tiles, int two_level, int shared_rows);
 
+/* Schedule of the Ewald-grid kernel (HGX_PATH_EWALD): pipelined = 1 builds each step's column
+ * operands during the previous step's matrix products (one wave per SIMD), 0 (default) in their own
+ * step (two waves per SIMD). Read by each call of the context. No reference counterpart. */
+int hgx_tune_ewald(hgx_ctx* ctx, int pipelined);
+
 /* Operands of the FP32C product-grid record adjoint (hgx_ff_numeric_adjoint and the grating fit):
  * HGX_PRODUCT_F16X2 (default: split fp16 on v_mfma_f32_32x32x16_f16, 4 real products per complex
  * product) with col_tiles = 4 (default) or 2 32-column tiles per wave, or HGX_PRODUCT_FP32

@@ -113,6 +113,24 @@ def test_rotated_and_offcentre_meshes(gpu_ctx, oracle_mod, meshes_up, case, mesh
     _check(gpu_ctx, oracle_mod, g, tri, mid, _rot(0.3, -0.2, 0.7))
 
 
+@pytest.mark.parametrize("case", ["ex01_dwba", "ex02_absorbing"])
+def test_both_schedules_agree(gpu_ctx, oracle_mod, meshes_up, case):
+    """The pipelined schedule (column operands a step ahead) against the default one: both within the
+    bar, and equal to each other up to FP32 accumulation order (they run the same products)."""
+    g = _build(gpu_ctx, oracle_mod, case)
+    tri, mid = meshes_up["polyhedron"]
+    outs = []
+    try:
+        for pipe in (1, 0):
+            assert gpu_ctx.lib.hgx_tune_ewald(gpu_ctx.h, pipe) == 0
+            _check(gpu_ctx, oracle_mod, g, tri, mid, _rot(0.1, 0.4, -0.3))
+            outs.append(gpu_ctx.as_complex(gpu_ctx.ff_numeric(mid, "approx", "fp32c",
+                                                              rot=_rot(0.1, 0.4, -0.3))).cpu().numpy())
+    finally:
+        gpu_ctx.lib.hgx_tune_ewald(gpu_ctx.h, 0)
+    assert rel_l2(outs[0], outs[1]) <= 1e-12
+
+
 def _ewald_grid(ncol, nrow, af, tth, ai, k0):
     """Host arrays of QGrid::create's formula (src/model/qgrid.cpp:80-82) for given exit angles."""
     A, T = np.meshgrid(af, tth, indexing="ij")
