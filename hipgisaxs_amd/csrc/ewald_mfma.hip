@@ -73,15 +73,15 @@ __device__ inline void ew_tri(const float4 ra, const float4 rc, const EwCol& k, 
     const float yr = gam * er, yi = gam * ei;
     o.i0[n] = pk16(yi, yr);
     o.i1[n] = pk16(ei, er);
-    o.r0[n] = pk16(yr, -yi);
-    o.r1[n] = pk16(er, -ei);
+    o.r0[n] = pk16n(yr, yi);
+    o.r1[n] = pk16n(er, ei);
     if (n < NS) {
       const float dyi = yi - (float)o.i0[n][0], dyr = yr - (float)o.i0[n][1];
       const float dei = ei - (float)o.i1[n][0], der = er - (float)o.i1[n][1];
       o.li0[n] = pk16(dyi, dyr);
       o.li1[n] = pk16(dei, der);
-      o.lr0[n] = pk16(dyr, -dyi);
-      o.lr1[n] = pk16(der, -dei);
+      o.lr0[n] = pk16n(dyr, dyi);
+      o.lr1[n] = pk16n(der, dei);
     }
   }
 }

@@ -23,6 +23,15 @@ __device__ inline int acc_row(int rr, int h) { return 8 * (rr >> 2) + 4 * h + (r
 // two FP32 values to fp16, rounded to nearest (v_cvt_pk_f16_f32)
 __device__ inline f16x2 pk16(float a, float b) { return __builtin_convertvector((f32x2){a, b}, f16x2); }
 
+// (a, -b) to fp16, rounded to nearest: one v_cvt_pk_f16_f32 with the negation as a source modifier
+// (the compiler emits a separate sign flip for pk16(a, -b)); fp16(-b) = -fp16(b), so the pair equals
+// pk16(a, -b) bit for bit. A register-only instruction: nothing is loaded or stored.
+__device__ inline f16x2 pk16n(float a, float b) {
+  f16x2 r;
+  asm("v_cvt_pk_f16_f32 %0, %1, -%2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // (v.y, -v.x): the pair for the real part from the pair for the imaginary part (exact)
 __device__ inline f16x2 swap_neg(f16x2 v) { return (f16x2){v[1], -v[0]}; }
 
