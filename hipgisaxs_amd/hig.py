@@ -35,6 +35,7 @@ _SIGS = {
     "hgx_hig_grains": (C.c_int, [_vp, C.c_char_p, C.c_int, _vp, _vp, _vp, _ip]),
     "hgx_hig_last_counts": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp]),
     "hgx_hig_kernel_ms": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)]),
+    "hgx_hig_last_paths": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "hgx_hig_save": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
     "hgx_hig_save_tiff": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p]),
     "hgx_hig_read_mesh": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_int64), C.c_int64, _vp]),
@@ -186,6 +187,12 @@ class HigSimulation:
         self._check(self.lib.hgx_hig_last_counts(self.h, c, C.byref(ms)), "last_counts")
         return {"fused_grains": c[0], "grains": c[1], "form_factors": c[2], "structure_factors": c[3],
                 "ms": ms.value}
+
+    def last_paths(self) -> dict:
+        """Numeric form-factor calls of the last run by kernel path."""
+        c = (C.c_int64 * 3)()
+        self._check(self.lib.hgx_hig_last_paths(self.h, c), "last_paths")
+        return {"pair": c[0], "product": c[1], "ewald": c[2]}
 
     def kernel_ms(self) -> dict:
         """Kernel times of the last run (needs flags with PROFILE)."""

@@ -499,6 +499,10 @@ bool HipGISAXS::form_factor(const HigShape& sh, const Rot3& rot, const vec3& loc
     int id = mesh_of(sh);
     if (id < 0) return false;
     const double* tv = (flags_ & HGX_HIG_MESH_NO_TRANSLATION) ? nullptr : loc.data();
+    int path = HGX_PATH_PAIR;
+    if (!hgx(hgx_ff_numeric_path(gpu_->get(), id, HGX_FF_APPROX, precision_, 0, &path, nullptr), "hgx_ff_numeric_path"))
+      return false;
+    ++paths_[path == HGX_PATH_PRODUCT ? 1 : path == HGX_PATH_EWALD ? 2 : 0];
     return hgx(hgx_ff_numeric(gpu_->get(), id, HGX_FF_APPROX, precision_, 0, rot.data(), tv, d_out), "hgx_ff_numeric");
   }
   hgx_shape_params p;
@@ -900,6 +904,7 @@ bool HipGISAXS::run_device(double alpha_i_deg, double phi_deg, const double** d_
   if (!loaded_) return fail("no input");
   if (!inited_ && !init()) return false;
   for (auto& c : counts_) c = 0;
+  for (auto& c : paths_) c = 0;
   const HigCompute& cp = in_.compute;
   if (cp.nslices > 1) return fail("the slicing scheme (nslices > 1) is not implemented by the reference either");
   // structure correlation (src/sim/hipgisaxs_main.cpp:1205-1262, 1336-1398): nGnE sums iratio-weighted
@@ -1174,6 +1179,15 @@ int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms) {
     if (!h) return HGX_ERR_BAD_HANDLE;
     if (counts) h->sim.last_counts(counts);
     if (ms) *ms = h->sim.last_run_ms();
+    return HGX_OK;
+  });
+}
+
+int hgx_hig_last_paths(hgx_hig* h, int64_t paths[3]) {
+  return hgx_hig_guard(h, [&]() -> int {
+    if (!h) return HGX_ERR_BAD_HANDLE;
+    if (!paths) return HGX_ERR_INVALID_ARG;
+    h->sim.last_paths(paths);
     return HGX_OK;
   });
 }

@@ -160,6 +160,10 @@ class HipGISAXS {
   // work counts of the last run: grains through the fused kernel, grains through per-grain
   // calls, form-factor and structure-factor evaluations
   void last_counts(int64_t counts[4]) const;
+  // numeric form-factor calls of the last run by kernel path: per-pair, product grid, Ewald grid
+  void last_paths(int64_t paths[3]) const {
+    for (int i = 0; i < 3; ++i) paths[i] = paths_[i];
+  }
   double last_run_ms() const { return last_ms_; }
   // kernel times of the last run (HGX_HIG_PROFILE): main ms, main launches, aux ms, aux launches
   void last_kernel_times(double* main_ms, int64_t* main_n, double* aux_ms, int64_t* aux_n) const {
@@ -222,6 +226,7 @@ class HipGISAXS {
   int64_t fit_npix_ = 0;
   std::vector<std::vector<double>> history_;
   int64_t counts_[4] = {0, 0, 0, 0};
+  int64_t paths_[3] = {0, 0, 0};
   double last_ms_ = 0.0;
   double kms_[2] = {0.0, 0.0};
   int64_t kn_[2] = {0, 0};

@@ -63,6 +63,10 @@ int hgx_hig_grains(hgx_hig* h, const char* key, int cap, double* rot, double* ce
 /* Work of the last run: counts[0] grains through the fused lattice kernel, [1] grains through
  * per-grain calls, [2] form-factor evaluations, [3] structure-factor evaluations; *ms its wall time. */
 int hgx_hig_last_counts(hgx_hig* h, int64_t counts[4], double* ms);
+/* Numeric (mesh) form-factor calls of the last run by the kernels they took (hgx_ff_numeric_path of
+ * include/hgx.h): paths[0] per-pair kernels, [1] product grid on the matrix cores, [2] Ewald-sphere
+ * grid on the matrix cores. No reference counterpart. */
+int hgx_hig_last_paths(hgx_hig* h, int64_t paths[3]);
 /* With HGX_HIG_PROFILE: summed durations (ms) and launch counts of the last run's kernels, "main" the
  * form-factor / structure-factor / lattice kernels, "aux" the rest (hgx_profile_end of include/hgx.h). */
 int hgx_hig_kernel_ms(hgx_hig* h, double* main_ms, int64_t* main_launches, double* aux_ms, int64_t* aux_launches);

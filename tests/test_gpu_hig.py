@@ -95,11 +95,17 @@ def test_reference_compatible_switches(name, flags, kw):
 def test_fp32c_paths(name):
     _say(name, "fp32c")
     text = _text(name)
-    img = H.HigSimulation(text=text, base_dir=str(GOLD), precision="fp32c").compute()
+    sim = H.HigSimulation(text=text, base_dir=str(GOLD), precision="fp32c")
+    img = sim.compute()
     ref = OH.run(text, base_dir=str(GOLD))
     r = _rel(img, ref)
     _say(name, "fp32c relL2", r)
     assert r <= TOL32
+    if name == "mesh_tetra":
+        # the mesh's form factors on the input's q-space grid (QGrid::create) run on the matrix cores
+        paths = sim.last_paths()
+        _say(name, "numeric form-factor paths", paths)
+        assert paths["ewald"] > 0 and paths["pair"] == 0
 
 
 def test_seed_changes_orientations_only_where_drawn():
