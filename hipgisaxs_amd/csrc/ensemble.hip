@@ -36,10 +36,24 @@ struct EnsElem {
   double dn2r, dn2i;
 };
 
+// One grain's structure-factor phases in the q frame. The lattice phase of direction k at q is
+// (R_g q).l_k = q.(R_g^T l_k), so the rotated lattice vectors (and R_g^T c_g for the grain-centre
+// phase) replace the rotation of every q-point; a rotated vector with no z component gives the same
+// phase on every DWBA branch, which differ only in q_z, so its factor is taken once per grain.
 struct EnsGrain {
-  double rot[9];     // R_g, for the structure factor
-  double center[3];  // grain centre
-  double w;          // grain weight
+  double L[4][3];  // R_g^T l_a, R_g^T l_b, R_g^T l_c, R_g^T c_g, times `scale` (1, or 1/2pi for turns)
+  double off[4];   // constant phases (the reference's lb[2]*lb[2] in place of mq[2]*lb[2], :212), scaled
+  double rot[9];   // R_g, for the Debye-Waller factor when some R_g is not orthogonal
+  double w;        // grain weight
+  int flat;        // bit k: L[k][2] == 0 (the factor is the same on every branch)
+};
+
+// The structure-factor constants shared by every grain: Dirichlet orders, Debye-Waller sigma^2/2 and
+// whether |R_g q| = |q| for every grain (the factor then depends on the branch only).
+struct EnsSF {
+  double N[3];
+  double dw;
+  int dw_invariant;
 };
 
 struct QViewE {
@@ -88,61 +102,141 @@ __device__ inline cd cell64(int o, double qx, double qy, cd qz, const EnsElem* _
   return ff;
 }
 
-// SHAPE >= 0: every element has that shape (one form-factor function inlined, fewer registers);
-// SHAPE = -1: mixed cells, the shape chosen per element. HOIST: the cell's form-factor sum is the same
-// for every grain (the host checked it: spheres and cylinders whose F depends on q only through q.q
-// and the third row of R_g R_e, which every grain shares, with locations that every grain sees
-// alike), so it is evaluated once per branch and the grain loop runs the structure factor alone.
-template <int SHAPE, bool HOIST>
+// Per-branch structure factor of grain gr at q = (qx, qy, qz_b), FP64, from the grain's table:
+// the branch-independent factors (flat directions) are passed in as Sflat.
+__device__ inline cd sf_branch64(const EnsGrain& gr, const double u[4], cd Sflat, cd qz, const EnsSF& sf) {
+  cd S = Sflat;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (!(gr.flat & (1 << k))) S = S * dirichlet(cd{u[k] + qz.r * gr.L[k][2], qz.i * gr.L[k][2]}, sf.N[k]);
+  if (!(gr.flat & 8)) S = S * cexpi_d(cd{u[3] + qz.r * gr.L[3][2], qz.i * gr.L[3][2]});
+  return S;
+}
+
+// |R_g q|^2 (the Debye-Waller argument) for a grain whose rotation is not orthogonal
+__device__ inline double rot_norm2(const double* __restrict__ R, double qx, double qy, cd qz) {
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double re = R[3 * j] * qx + R[3 * j + 1] * qy + R[3 * j + 2] * qz.r, im = R[3 * j + 2] * qz.i;
+    s += re * re + im * im;
+  }
+  return s;
+}
+
+// The branch factors every grain shares: the Fresnel coefficient and, when every R_g is orthogonal,
+// exp(-sigma^2/2 |R q|^2) = exp(-sigma^2/2 |q|^2).
+__device__ inline cd branch_factor64(const double* __restrict__ fc, int64_t q, double qx, double qy, cd qz,
+                                     const EnsSF& sf) {
+  cd f = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
+  if (sf.dw != 0.0 && sf.dw_invariant) f = exp(-sf.dw * (qx * qx + qy * qy + qz.r * qz.r + qz.i * qz.i)) * f;
+  return f;
+}
+
+// Fused kernel, the cell's form-factor sum per grain. SHAPE >= 0: every element has that shape (one
+// form-factor function inlined, fewer registers); SHAPE = -1: mixed cells, the shape chosen per element.
+template <int SHAPE>
 __global__ __launch_bounds__(256) void k_ensemble64(QViewE g, int nb, const double* __restrict__ fc,
                                                     const EnsElem* __restrict__ E, int nelem,
                                                     const double* __restrict__ loc, const double* __restrict__ M,
                                                     const EnsGrain* __restrict__ G, int ngrains, int per_y,
-                                                    SFArgs sfb, double* __restrict__ part) {
+                                                    EnsSF sf, double* __restrict__ part) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npix) return;
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
-  cd qzb[4], fcb[4], ffh[4];
+  cd qzb[4], fcb[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     if (b >= nb) break;
     const int64_t q = p + b * g.npix;
     qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
-    fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
-    if (HOIST) ffh[b] = fcb[b] * cell64<SHAPE>(g0, qx, qy, qzb[b], E, nelem, loc, M);
+    fcb[b] = branch_factor64(fc, q, qx, qy, qzb[b], sf);
   }
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
-    SFArgs sf = sfb;
-    sf.center[0] = G[o].center[0];
-    sf.center[1] = G[o].center[1];
-    sf.center[2] = G[o].center[2];
+    const EnsGrain& gr = G[o];
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    cd Sflat = cd{1.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
+    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
     cd amp = cd{0.0, 0.0};
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       if (b >= nb) break;
-      cd mg[3];
-      rotate_m(G[o].rot, qx, qy, qzb[b], mg);
-      const cd S = sf_value(mg, sf);
-      if (HOIST) {
-        amp = amp + ffh[b] * S;
-      } else {
-        cd ff = cd{0.0, 0.0};
-        for (int e = 0; e < nelem; ++e) {
-          cd mq[3];
-          rotate_m(M + 9 * ((int64_t)o * nelem + e), qx, qy, qzb[b], mq);
-          const cd F = SHAPE >= 0 ? shape_ff<(SHAPE >= 0 ? SHAPE : 0)>(mq, E[e].P) : ff_any(E[e].shape, mq, E[e].P);
-          const cd dn2 = cd{E[e].dn2r, E[e].dn2i};
-          for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
-            const double* t = loc + 3 * l;
-            ff = ff + dn2 * (F * cexpi_d(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2]));
-          }
-        }
-        amp = amp + fcb[b] * S * ff;
-      }
+      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
+      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
+      amp = amp + fcb[b] * S * cell64<SHAPE>(o, qx, qy, qzb[b], E, nelem, loc, M);
     }
-    amp = G[o].w * amp;
+    amp = gr.w * amp;
+    acc += amp.r * amp.r + amp.i * amp.i;
+  }
+  part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
+// Grain-invariant cells (the host checked it: spheres and cylinders whose F depends on q only through
+// q.q and the third row of R_g R_e, which every grain shares, with locations that every grain sees
+// alike): the cell's form-factor sum times the branch factor, once per (pixel, branch), into ffh
+// [nb][npix] complex. A kernel of its own, so that the registers and scratch of the form-factor code
+// (the sphere cap's FP64 Bessel function, the FP64 fallbacks of the FP32C forms) do not size the
+// structure-factor loop over the grains.
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_cell64(QViewE g, int nb, const double* __restrict__ fc,
+                                                         const EnsElem* __restrict__ E, int nelem,
+                                                         const double* __restrict__ loc, const double* __restrict__ M,
+                                                         EnsSF sf, double* __restrict__ ffh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const double qx = g.qx[p], qy = g.qy[p];
+  for (int b = 0; b < nb; ++b) {
+    const int64_t q = p + b * g.npix;
+    const cd qz = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+    const cd f = branch_factor64(fc, q, qx, qy, qz, sf) * cell64<SHAPE>(0, qx, qy, qz, E, nelem, loc, M);
+    ffh[2 * q] = f.r;
+    ffh[2 * q + 1] = f.i;
+  }
+}
+
+// The grain loop of a grain-invariant cell: sum_g |w_g sum_b ffh_b S_g(q_b)|^2 over this y-slice.
+__global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const double* __restrict__ ffh,
+                                                       const EnsGrain* __restrict__ G, int ngrains, int per_y,
+                                                       EnsSF sf, double* __restrict__ part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
+  const double qx = g.qx[p], qy = g.qy[p];
+  cd qzb[4], F[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= nb) break;
+    const int64_t q = p + b * g.npix;
+    qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+  }
+  double acc = 0.0;
+  for (int o = g0; o < g1; ++o) {
+    const EnsGrain& gr = G[o];
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    cd Sflat = cd{1.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
+    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
+    cd amp = cd{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= nb) break;
+      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
+      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
+      amp = amp + F[b] * S;
+    }
+    amp = gr.w * amp;
     acc += amp.r * amp.r + amp.i * amp.i;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
@@ -171,48 +265,133 @@ __device__ inline cd cell32(int o, double qx, double qy, double qz, const EnsEle
   return ff;
 }
 
-template <int SHAPE, bool HOIST>
+// FP64 phase in turns reduced to [-1/2, 1/2], as FP32
+__device__ inline float red_turn(double t) { return (float)(t - rint(t)); }
+
+// The FP32C structure factor of grain gr on branch b: the branch-independent factors Df, Pf (flat
+// directions) and the rest from the FP64 phases in turns.
+__device__ inline cf sf_branch32(const EnsGrain& gr, const double u[4], float Df, cf Pf, double qz, const float N[3],
+                                 const EnsSF& sf, double qx, double qy) {
+  float D = Df;
+  if (!(gr.flat & 1)) D *= dirichlet_turn(red_turn(u[0] + qz * gr.L[0][2]), N[0]);
+  if (!(gr.flat & 2)) D *= dirichlet_turn(red_turn(u[1] + qz * gr.L[1][2]), N[1]);
+  if (!(gr.flat & 4)) D *= dirichlet_turn(red_turn(u[2] + qz * gr.L[2][2]), N[2]);
+  const cf P = (gr.flat & 8) ? Pf : expi_turn(red_turn(u[3] + qz * gr.L[3][2]));
+  if (sf.dw != 0.0 && !sf.dw_invariant) D *= __expf((float)(-sf.dw * rot_norm2(gr.rot, qx, qy, cd{qz, 0.0})));
+  return D * P;
+}
+
+// the branch-independent factors of grain gr (FP32C)
+__device__ inline void sf_flat32(const EnsGrain& gr, const double u[4], const float N[3], float& Df, cf& Pf) {
+  Df = 1.0f;
+  Pf = cf{1.0f, 0.0f};
+  if (gr.flat & 1) Df *= dirichlet_turn(red_turn(u[0]), N[0]);
+  if (gr.flat & 2) Df *= dirichlet_turn(red_turn(u[1]), N[1]);
+  if (gr.flat & 4) Df *= dirichlet_turn(red_turn(u[2]), N[2]);
+  if (gr.flat & 8) Pf = expi_turn(red_turn(u[3]));
+}
+
+__device__ inline double branch_dw32(double qx, double qy, double qz, const EnsSF& sf) {
+  return (sf.dw != 0.0 && sf.dw_invariant) ? (double)__expf((float)(-sf.dw * (qx * qx + qy * qy + qz * qz))) : 1.0;
+}
+
+template <int SHAPE>
 __global__ __launch_bounds__(256) void k_ensemble32(QViewE g, int nb, const double* __restrict__ fc,
                                                     const EnsElem* __restrict__ E, int nelem,
                                                     const double* __restrict__ loc, const double* __restrict__ M,
                                                     const EnsGrain* __restrict__ G, int ngrains, int per_y,
-                                                    SFArgs sfb, double* __restrict__ part) {
+                                                    EnsSF sf, double* __restrict__ part) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npix) return;
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
+  const float N[3] = {(float)sf.N[0], (float)sf.N[1], (float)sf.N[2]};
   double qzb[4];
-  cd fcb[4], ffh[4];
+  cd fcb[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     if (b >= nb) break;
     const int64_t q = p + b * g.npix;
     qzb[b] = g.qz[2 * q];
-    fcb[b] = fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0};
-    if (HOIST) ffh[b] = fcb[b] * cell32<SHAPE>(g0, qx, qy, qzb[b], E, nelem, loc, M);
+    fcb[b] = branch_dw32(qx, qy, qzb[b], sf) * (fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0});
   }
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
-    SFArgs sf = sfb;
-    sf.center[0] = G[o].center[0];
-    sf.center[1] = G[o].center[1];
-    sf.center[2] = G[o].center[2];
+    const EnsGrain& gr = G[o];
+    // in-plane parts of the phases, in turns (FP64)
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    float Df;
+    cf Pf;
+    sf_flat32(gr, u, N, Df, Pf);
     cd amp = cd{0.0, 0.0};
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       if (b >= nb) break;
-      double mg[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) mg[j] = G[o].rot[3 * j] * qx + G[o].rot[3 * j + 1] * qy + G[o].rot[3 * j + 2] * qzb[b];
-      const cf S = sf32e(mg, sf);
-      const cd Sd = cd{(double)S.r, (double)S.i};
-      if (HOIST) {
-        amp = amp + ffh[b] * Sd;
-      } else {
-        amp = amp + fcb[b] * (Sd * cell32<SHAPE>(o, qx, qy, qzb[b], E, nelem, loc, M));
-      }
+      const cf S = sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
+      amp = amp + fcb[b] * (cd{(double)S.r, (double)S.i} * cell32<SHAPE>(o, qx, qy, qzb[b], E, nelem, loc, M));
     }
-    amp = G[o].w * amp;
+    amp = gr.w * amp;
+    acc += amp.r * amp.r + amp.i * amp.i;
+  }
+  part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
+// FP32C grain-invariant cell (see k_ensemble_cell64)
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_cell32(QViewE g, int nb, const double* __restrict__ fc,
+                                                         const EnsElem* __restrict__ E, int nelem,
+                                                         const double* __restrict__ loc, const double* __restrict__ M,
+                                                         EnsSF sf, double* __restrict__ ffh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const double qx = g.qx[p], qy = g.qy[p];
+  for (int b = 0; b < nb; ++b) {
+    const int64_t q = p + b * g.npix;
+    const double qz = g.qz[2 * q];
+    const cd f = branch_dw32(qx, qy, qz, sf) * (fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0});
+    const cd v = f * cell32<SHAPE>(0, qx, qy, qz, E, nelem, loc, M);
+    ffh[2 * q] = v.r;
+    ffh[2 * q + 1] = v.i;
+  }
+}
+
+// FP32C grain loop of a grain-invariant cell
+__global__ __launch_bounds__(256) void k_ensemble_sf32(QViewE g, int nb, const double* __restrict__ ffh,
+                                                       const EnsGrain* __restrict__ G, int ngrains, int per_y,
+                                                       EnsSF sf, double* __restrict__ part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
+  const double qx = g.qx[p], qy = g.qy[p];
+  const float N[3] = {(float)sf.N[0], (float)sf.N[1], (float)sf.N[2]};
+  double qzb[4];
+  cd F[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= nb) break;
+    const int64_t q = p + b * g.npix;
+    qzb[b] = g.qz[2 * q];
+    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+  }
+  double acc = 0.0;
+  for (int o = g0; o < g1; ++o) {
+    const EnsGrain& gr = G[o];
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    float Df;
+    cf Pf;
+    sf_flat32(gr, u, N, Df, Pf);
+    cd amp = cd{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= nb) break;
+      const cf S = sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
+      amp = amp + F[b] * cd{(double)S.r, (double)S.i};
+    }
+    amp = gr.w * amp;
     acc += amp.r * amp.r + amp.i * amp.i;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
@@ -408,11 +587,39 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   // M[g][e] = R_g R_e with the summation order of the reference's RotMatrix_t product
   std::vector<double> Mm(9 * (size_t)ng * ne);
   std::vector<EnsGrain> G(ng);
+  // the grains' phase tables: R_g^T l_k, R_g^T c_g (turns for FP32C), the compat constant, flat bits
+  const double scale = precision == HGX_PREC_FP32C ? 1.0 / (2.0 * kPiD) : 1.0;
+  const bool compat_lb2 = (m->sf_flags & HGX_SF_COMPAT_LB2) != 0;
+  bool orthogonal = true;
   for (int o = 0; o < ng; ++o) {
     const double* Rg = m->rot + 9 * (size_t)o;
-    for (int k = 0; k < 9; ++k) G[o].rot[k] = Rg[k];
-    for (int k = 0; k < 3; ++k) G[o].center[k] = m->center ? m->center[3 * (size_t)o + k] : 0.0;
-    G[o].w = m->weight ? m->weight[o] : 1.0;
+    EnsGrain& gr = G[o];
+    std::memset(&gr, 0, sizeof(gr));
+    for (int k = 0; k < 9; ++k) gr.rot[k] = Rg[k];
+    double vec[4][3];
+    for (int k = 0; k < 3; ++k) {
+      vec[0][k] = m->la[k];
+      vec[1][k] = m->lb[k];
+      vec[2][k] = m->lc[k];
+      vec[3][k] = m->center ? m->center[3 * (size_t)o + k] : 0.0;
+    }
+    if (compat_lb2) {
+      // src/sf/sf.cpp:212: lb[2] * lb[2] in place of lb[2] * mq[2], a constant phase
+      gr.off[1] = m->lb[2] * m->lb[2] * scale;
+      vec[1][2] = 0.0;
+    }
+    for (int v = 0; v < 4; ++v)
+      for (int k = 0; k < 3; ++k)  // (R^T v)_k = sum_j R[j][k] v_j
+        gr.L[v][k] = (Rg[k] * vec[v][0] + Rg[3 + k] * vec[v][1] + Rg[6 + k] * vec[v][2]) * scale;
+    for (int v = 0; v < 4; ++v)
+      if (gr.L[v][2] == 0.0) gr.flat |= 1 << v;
+    gr.w = m->weight ? m->weight[o] : 1.0;
+    for (int i = 0; i < 3 && orthogonal; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double d = 0.0;
+        for (int k = 0; k < 3; ++k) d += Rg[3 * k + i] * Rg[3 * k + j];
+        if (std::fabs(d - (i == j ? 1.0 : 0.0)) > 1e-12) orthogonal = false;
+      }
     for (int e = 0; e < ne; ++e) {
       const double* Re = m->elem[e].rot;
       double* out = &Mm[9 * ((size_t)o * ne + e)];
@@ -425,16 +632,10 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     }
   }
   const bool hoist = cell_is_grain_invariant(m, Mm);
-  SFArgs sfb;
-  for (int k = 0; k < 3; ++k) {
-    sfb.la[k] = m->la[k];
-    sfb.lb[k] = m->lb[k];
-    sfb.lc[k] = m->lc[k];
-    sfb.N[k] = m->repet[k] < 1 ? 1 : m->repet[k];  // src/sf/sf.cpp:165-167
-    sfb.center[k] = 0.0;
-  }
+  EnsSF sfb;
+  for (int k = 0; k < 3; ++k) sfb.N[k] = m->repet[k] < 1 ? 1 : m->repet[k];  // src/sf/sf.cpp:165-167
   sfb.dw = 0.5 * m->dw_sigma * m->dw_sigma;
-  sfb.compat_lb2 = (m->sf_flags & HGX_SF_COMPAT_LB2) != 0;
+  sfb.dw_invariant = orthogonal ? 1 : 0;
   if (int rc = set_device(c)) return rc;
   // grains split over blockIdx.y so that a launch has at least ~2048 workgroups
   const unsigned nbx = ceil_div(c->npix, 256);
@@ -444,14 +645,15 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t b_part = al((size_t)ny * c->npix * sizeof(double)), b_E = al(ne * sizeof(EnsElem)),
                b_loc = al(loc.size() * sizeof(double)), b_M = al(Mm.size() * sizeof(double)),
-               b_G = al(ng * sizeof(EnsGrain));
-  if (int rc = ensure_work(c, b_part + b_E + b_loc + b_M + b_G)) return rc;
+               b_G = al(ng * sizeof(EnsGrain)), b_ffh = hoist ? al((size_t)c->nq * 2 * sizeof(double)) : 0;
+  if (int rc = ensure_work(c, b_part + b_E + b_loc + b_M + b_G + b_ffh)) return rc;
   char* w = static_cast<char*>(c->d_work);
   double* part = reinterpret_cast<double*>(w);
   EnsElem* dE = reinterpret_cast<EnsElem*>(w + b_part);
   double* dloc = reinterpret_cast<double*>(w + b_part + b_E);
   double* dM = reinterpret_cast<double*>(w + b_part + b_E + b_loc);
   EnsGrain* dG = reinterpret_cast<EnsGrain*>(w + b_part + b_E + b_loc + b_M);
+  double* ffh = hoist ? reinterpret_cast<double*>(w + b_part + b_E + b_loc + b_M + b_G) : nullptr;
   {
     const void* src[4] = {E.data(), loc.data(), Mm.data(), G.data()};
     const size_t nb[4] = {ne * sizeof(EnsElem), loc.size() * sizeof(double), Mm.size() * sizeof(double),
@@ -459,36 +661,31 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     if (int rc = stage_to_device(c, 4, src, nb, reinterpret_cast<char*>(dE))) return rc;
   }
   QViewE g{c->d_qx, c->d_qy, c->d_qz, c->npix};
-  if (precision == HGX_PREC_FP32C) {
-    LaunchTimer lt(c, true);
-    const dim3 grid(nbx, ny);
-#define HGX_ENS32_LAUNCH(S)                                                                                    \
-  if (hoist)                                                                                                   \
-    k_ensemble32<S, true><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, \
-                                                       part);                                                  \
-  else                                                                                                         \
-    k_ensemble32<S, false><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
-    switch (common) {
-      case HGX_SHAPE_SPHERE: HGX_ENS32_LAUNCH(HGX_SHAPE_SPHERE); break;
-      case HGX_SHAPE_CYLINDER: HGX_ENS32_LAUNCH(HGX_SHAPE_CYLINDER); break;
-      case HGX_SHAPE_BOX: HGX_ENS32_LAUNCH(HGX_SHAPE_BOX); break;
-      case HGX_SHAPE_CUBE: HGX_ENS32_LAUNCH(HGX_SHAPE_CUBE); break;
-      case HGX_SHAPE_PRISM3: HGX_ENS32_LAUNCH(HGX_SHAPE_PRISM3); break;
-      case HGX_SHAPE_PRISM6: HGX_ENS32_LAUNCH(HGX_SHAPE_PRISM6); break;
-      case HGX_SHAPE_PYRAMID: HGX_ENS32_LAUNCH(HGX_SHAPE_PYRAMID); break;
-      default: HGX_ENS32_LAUNCH(-1); break;
+  const dim3 grid(nbx, ny);
+  const bool f32 = precision == HGX_PREC_FP32C;
+  if (hoist) {
+    {
+      LaunchTimer lt(c, true);
+#define HGX_ENS_CELL(S)                                                                                     \
+  if (f32) k_ensemble_cell32<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh); \
+  else k_ensemble_cell64<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh)
+      switch (common) {
+        case HGX_SHAPE_SPHERE: HGX_ENS_CELL(HGX_SHAPE_SPHERE); break;
+        case HGX_SHAPE_CYLINDER: HGX_ENS_CELL(HGX_SHAPE_CYLINDER); break;
+        default: HGX_ENS_CELL(-1); break;
+      }
+#undef HGX_ENS_CELL
+      HGX_CHECK_LAUNCH(c);
     }
-#undef HGX_ENS32_LAUNCH
+    LaunchTimer lt(c, true);
+    if (f32) k_ensemble_sf32<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
+    else k_ensemble_sf64<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
     HGX_CHECK_LAUNCH(c);
   } else {
     LaunchTimer lt(c, true);
-    const dim3 grid(nbx, ny);
-#define HGX_ENS_LAUNCH(S)                                                                                      \
-  if (hoist)                                                                                                   \
-    k_ensemble64<S, true><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, \
-                                                       part);                                                  \
-  else                                                                                                         \
-    k_ensemble64<S, false><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
+#define HGX_ENS_LAUNCH(S)                                                                                       \
+  if (f32) k_ensemble32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part); \
+  else k_ensemble64<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
     switch (common) {
       case HGX_SHAPE_SPHERE: HGX_ENS_LAUNCH(HGX_SHAPE_SPHERE); break;
       case HGX_SHAPE_CYLINDER: HGX_ENS_LAUNCH(HGX_SHAPE_CYLINDER); break;

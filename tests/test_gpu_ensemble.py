@@ -82,6 +82,47 @@ def test_mixed_cell_matches_oracle(gpu_ctx, oracle_mod, absorbing):
     assert r <= 1e-10
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-5)])
+@pytest.mark.parametrize("case", ["flat", "tilted", "compat", "skew"])
+def test_structure_factor_forms(gpu_ctx, oracle_mod, precision, tol, case):
+    """The structure factor of the fused kernel in each of its forms, with Debye-Waller damping and
+    3-D grain centres: grains rotated about z only (the in-plane lattice directions and the centre
+    phase the same on every DWBA branch), tilted grains (every factor per branch), the reference's
+    lb[2]*lb[2] term (src/sf/sf.cpp:212, a constant phase), and a grain matrix that is not a rotation
+    (the Debye-Waller factor per grain). Against the oracle's per-grain composition."""
+    from hipgisaxs_amd.api import ensemble_intensity
+
+    O = oracle_mod
+    g, fc = _grid(O, gpu_ctx, False)
+    el = [{"shape": "sphere", "dims": [([3.0, 3.5], [0.5, 0.5])], "rot": np.eye(3),
+           "locations": [[0, 0, 0]], "dn2": -1.2e-5}]
+    # l_b with a z component, so that the reference's lb[2]*lb[2] differs from lb[2]*mq[2]
+    la, lb, lc, rep = (20.0, 0, 0), (0, 20.0, 4.0 if case == "compat" else 0.0), (10.0, 10.0, 10.0), (6, 5, 3)
+    rng = np.random.default_rng(11)
+    angles = rng.uniform(0, 2 * np.pi, 6)
+    if case == "flat":
+        rots = [_rz(a) for a in angles]
+        centers = np.c_[rng.uniform(-30, 30, (6, 2)), np.zeros(6)]
+    else:
+        rots = [_rz(a) @ _rx(b) for a, b in zip(angles, rng.uniform(0.02, 0.3, 6))]
+        centers = rng.uniform(-30, 30, (6, 3))
+    if case == "skew":
+        rots[2] = rots[2] @ np.diag([1.0, 1.0, 1.02])
+    flags = 1 if case == "compat" else 0
+    weights = rng.uniform(0.5, 1.5, 6)
+    dw = 0.7
+    got = ensemble_intensity(gpu_ctx, el, la, lb, lc, rep, np.stack(rots), centers=centers, weights=weights,
+                             dw_sigma=dw, sf_flags=flags, fc=_dev(fc), precision=precision).cpu().numpy()
+    I = None
+    for R, c, w in zip(rots, centers, weights):
+        ff = el[0]["dn2"] * O.ff_analytic("sphere", el[0]["dims"], g, rot=R)
+        sf = O.sf_lattice(la, lb, lc, rep, c, g, dw_sigma=dw, flags=flags, rot=R)
+        I = O.dwba_intensity(g.npix, g.nq // g.npix, fc, sf, ff, w, I)
+    r = rel_l2(got, I)
+    print(case, precision, "relL2", r)
+    assert r <= tol
+
+
 @pytest.mark.parametrize("shape,dims", [
     ("box", [([30.0, 32.0], [0.5, 0.5]), ([10.0], [1.0]), ([40.0, 44.0, 48.0], [0.2, 0.5, 0.3])]),
     ("cube", [([9.0, 11.0], [0.6, 0.4])]),
