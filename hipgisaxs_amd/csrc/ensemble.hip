@@ -397,6 +397,202 @@ __global__ __launch_bounds__(256) void k_ensemble_sf32(QViewE g, int nb, const d
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
 }
 
+// ----- separable cells -------------------------------------------------------------------------
+//
+// A box (any node distribution: its node triple sum is the product of three per-axis sums) or a
+// one-node cube is F(mq) = F_par(mq_x, mq_y) F_z(mq_z). When the cell holds that one element and
+// every grain's M_g = R_g R_e keeps the z axis (third row and column (0, 0, 1): rotations about z, as
+// the reference's in-plane orientation ranges give), mq_z = q_z on every grain and mq_x, mq_y do not
+// depend on the branch; with the element's locations at one height t_z the location sum splits the
+// same way. So F_z(q_z) e^{i q_z t_z} joins the branch factor once per (pixel, branch) and the grain loop
+// evaluates F_par and the in-plane location phases once per (pixel, grain) instead of the whole cell
+// per (pixel, grain, branch).
+
+template <int SHAPE>
+__device__ inline double fpar64(double mx, double my, const Nodes& P) {
+  if (SHAPE == HGX_SHAPE_CUBE) {
+    const double L = P.val[0][0];
+    return (P.wt[0][0] * L * L) * sinc_free(cd{0.5 * L * mx, 0.0}).r * sinc_free(cd{0.5 * L * my, 0.0}).r;
+  }
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < P.n[0]; ++i) a += (P.wt[0][i] * P.val[0][i]) * sinc_free(cd{0.5 * P.val[0][i] * mx, 0.0}).r;
+  for (int i = 0; i < P.n[1]; ++i) b += (P.wt[1][i] * P.val[1][i]) * sinc_free(cd{0.5 * P.val[1][i] * my, 0.0}).r;
+  return a * b;
+}
+
+template <int SHAPE>
+__device__ inline cd fz64(cd qz, const Nodes& P) {
+  const int d = SHAPE == HGX_SHAPE_CUBE ? 0 : 2;
+  const int n = SHAPE == HGX_SHAPE_CUBE ? 1 : P.n[2];
+  cd acc = cd{0.0, 0.0};
+  for (int k = 0; k < n; ++k) {
+    const double H = P.val[d][k];
+    const cd hz = (0.5 * H) * qz;
+    acc = acc + ((SHAPE == HGX_SHAPE_CUBE ? 1.0 : P.wt[d][k]) * H) * (cexpi_d(hz) * sinc_free(hz));
+  }
+  return acc;
+}
+
+template <int SHAPE>
+__device__ inline float fpar32(double mx, double my, const Nodes& P) {
+  if (SHAPE == HGX_SHAPE_CUBE) {
+    const double L = P.val[0][0];
+    return (float)(P.wt[0][0] * L * L) * sinc32(mx, L) * sinc32(my, L);
+  }
+  float a = 0.0f, b = 0.0f;
+  for (int i = 0; i < P.n[0]; ++i) a += (float)(P.wt[0][i] * P.val[0][i]) * sinc32(mx, P.val[0][i]);
+  for (int i = 0; i < P.n[1]; ++i) b += (float)(P.wt[1][i] * P.val[1][i]) * sinc32(my, P.val[1][i]);
+  return a * b;
+}
+
+template <int SHAPE>
+__device__ inline cf fz32(double qz, const Nodes& P) {
+  const int d = SHAPE == HGX_SHAPE_CUBE ? 0 : 2;
+  const int n = SHAPE == HGX_SHAPE_CUBE ? 1 : P.n[2];
+  cf acc = cf{0.0f, 0.0f};
+  for (int k = 0; k < n; ++k) {
+    const double H = P.val[d][k];
+    acc = acc + ((float)((SHAPE == HGX_SHAPE_CUBE ? 1.0 : P.wt[d][k]) * H) * sinc32(qz, H)) *
+                    expi_turn(turns_of(0.5 * H * qz));
+  }
+  return acc;
+}
+
+// Branch factors of a separable cell: fc_b, the Debye-Waller factor of orthogonal grains, dn2,
+// F_z(q_z) and e^{i q_z t_z}, into ffh [nb][npix] complex.
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_zcell64(QViewE g, int nb, const double* __restrict__ fc,
+                                                          const EnsElem* __restrict__ E, double tz, EnsSF sf,
+                                                          double* __restrict__ ffh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const double qx = g.qx[p], qy = g.qy[p];
+  const cd dn2 = cd{E[0].dn2r, E[0].dn2i};
+  for (int b = 0; b < nb; ++b) {
+    const int64_t q = p + b * g.npix;
+    const cd qz = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+    const cd v = branch_factor64(fc, q, qx, qy, qz, sf) * dn2 * fz64<SHAPE>(qz, E[0].P) * cexpi_d(tz * qz);
+    ffh[2 * q] = v.r;
+    ffh[2 * q + 1] = v.i;
+  }
+}
+
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_zcell32(QViewE g, int nb, const double* __restrict__ fc,
+                                                          const EnsElem* __restrict__ E, double tz, EnsSF sf,
+                                                          double* __restrict__ ffh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const double qx = g.qx[p], qy = g.qy[p];
+  const cd dn2 = cd{E[0].dn2r, E[0].dn2i};
+  for (int b = 0; b < nb; ++b) {
+    const int64_t q = p + b * g.npix;
+    const double qz = g.qz[2 * q];
+    const cd f = branch_dw32(qx, qy, qz, sf) * (fc ? cd{fc[2 * q], fc[2 * q + 1]} : cd{1.0, 0.0});
+    const cf z = fz32<SHAPE>(qz, E[0].P) * expi_turn(turns_of(tz * qz));
+    const cd v = f * dn2 * cd{(double)z.r, (double)z.i};
+    ffh[2 * q] = v.r;
+    ffh[2 * q + 1] = v.i;
+  }
+}
+
+// The grain loop of a separable cell, FP64: sum_g |w_g F_par(g) L(g) sum_b ffh_b S_g(q_b)|^2 with
+// L(g) the in-plane location phases.
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const double* __restrict__ ffh,
+                                                        const EnsElem* __restrict__ E, const double* __restrict__ loc,
+                                                        const double* __restrict__ M, const EnsGrain* __restrict__ G,
+                                                        int ngrains, int per_y, EnsSF sf, double* __restrict__ part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
+  const double qx = g.qx[p], qy = g.qy[p];
+  cd qzb[4], F[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= nb) break;
+    const int64_t q = p + b * g.npix;
+    qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
+    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+  }
+  const int l0 = E[0].loc0, l1 = E[0].loc0 + E[0].nloc;
+  double acc = 0.0;
+  for (int o = g0; o < g1; ++o) {
+    const EnsGrain& gr = G[o];
+    const double* Mo = M + 9 * (int64_t)o;
+    const double mx = Mo[0] * qx + Mo[1] * qy, my = Mo[3] * qx + Mo[4] * qy;
+    cd lph = cd{0.0, 0.0};
+    for (int l = l0; l < l1; ++l) lph = lph + cexpi_d(cd{mx * loc[3 * l] + my * loc[3 * l + 1], 0.0});
+    const cd A = fpar64<SHAPE>(mx, my, E[0].P) * lph;
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    cd Sflat = cd{1.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
+    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
+    cd sum = cd{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= nb) break;
+      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
+      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
+      sum = sum + F[b] * S;
+    }
+    const cd amp = gr.w * (A * sum);
+    acc += amp.r * amp.r + amp.i * amp.i;
+  }
+  part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_sep32(QViewE g, int nb, const double* __restrict__ ffh,
+                                                        const EnsElem* __restrict__ E, const double* __restrict__ loc,
+                                                        const double* __restrict__ M, const EnsGrain* __restrict__ G,
+                                                        int ngrains, int per_y, EnsSF sf, double* __restrict__ part) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
+  const double qx = g.qx[p], qy = g.qy[p];
+  const float N[3] = {(float)sf.N[0], (float)sf.N[1], (float)sf.N[2]};
+  double qzb[4];
+  cd F[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (b >= nb) break;
+    const int64_t q = p + b * g.npix;
+    qzb[b] = g.qz[2 * q];
+    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+  }
+  const int l0 = E[0].loc0, l1 = E[0].loc0 + E[0].nloc;
+  double acc = 0.0;
+  for (int o = g0; o < g1; ++o) {
+    const EnsGrain& gr = G[o];
+    const double* Mo = M + 9 * (int64_t)o;
+    const double mx = Mo[0] * qx + Mo[1] * qy, my = Mo[3] * qx + Mo[4] * qy;
+    cf lph = cf{0.0f, 0.0f};
+    for (int l = l0; l < l1; ++l) lph = lph + expi_turn(turns_of(mx * loc[3 * l] + my * loc[3 * l + 1]));
+    const float fp = fpar32<SHAPE>(mx, my, E[0].P);
+    double u[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+    float Df;
+    cf Pf;
+    sf_flat32(gr, u, N, Df, Pf);
+    cd sum = cd{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b >= nb) break;
+      const cf S = sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
+      sum = sum + F[b] * cd{(double)S.r, (double)S.i};
+    }
+    const cd amp = (gr.w * (double)fp) * (cd{(double)lph.r, (double)lph.i} * sum);
+    acc += amp.r * amp.r + amp.i * amp.i;
+  }
+  part[(int64_t)blockIdx.y * g.npix + p] = acc;
+}
+
 // ----- forward-mode tangents (FP64) -------------------------------------------------------------
 
 // One element with dual nodes, contrast and its location range; grains with dual rotation, centre
@@ -534,6 +730,25 @@ static bool cell_is_grain_invariant(const hgx_ensemble_model* m, const std::vect
   return true;
 }
 
+// Whether the cell is one box (or one-node cube) element whose F(M_g q) = F_par(mq_x, mq_y) F_z(q_z) on
+// every grain (each M_g keeps the z axis to 1e-15) with all its locations at one height (*tz).
+static bool cell_is_separable(const hgx_ensemble_model* m, const std::vector<double>& Mm, double* tz) {
+  if (m->nelem != 1) return false;
+  const hgx_ensemble_element& e = m->elem[0];
+  if (!(e.shape == HGX_SHAPE_BOX || (e.shape == HGX_SHAPE_CUBE && e.params.n[0] == 1))) return false;
+  if (e.nloc < 1 || !e.loc) return false;
+  for (int l = 1; l < e.nloc; ++l)
+    if (e.loc[3 * l + 2] != e.loc[2]) return false;
+  for (int o = 0; o < m->ngrains; ++o) {
+    const double* M = &Mm[9 * (size_t)o];
+    if (std::fabs(M[2]) > 1e-15 || std::fabs(M[5]) > 1e-15 || std::fabs(M[6]) > 1e-15 || std::fabs(M[7]) > 1e-15 ||
+        std::fabs(M[8] - 1.0) > 1e-15)
+      return false;
+  }
+  *tz = e.loc[2];
+  return true;
+}
+
 static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, const double* fc, int accumulate,
                         double* I) {
   if (!c->d_qz) return set_error(c, HGX_ERR_NO_QGRID, "ensemble_intensity: no q-grid");
@@ -632,6 +847,8 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     }
   }
   const bool hoist = cell_is_grain_invariant(m, Mm);
+  double tz = 0.0;
+  const bool sep = !hoist && cell_is_separable(m, Mm, &tz);
   EnsSF sfb;
   for (int k = 0; k < 3; ++k) sfb.N[k] = m->repet[k] < 1 ? 1 : m->repet[k];  // src/sf/sf.cpp:165-167
   sfb.dw = 0.5 * m->dw_sigma * m->dw_sigma;
@@ -645,7 +862,7 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t b_part = al((size_t)ny * c->npix * sizeof(double)), b_E = al(ne * sizeof(EnsElem)),
                b_loc = al(loc.size() * sizeof(double)), b_M = al(Mm.size() * sizeof(double)),
-               b_G = al(ng * sizeof(EnsGrain)), b_ffh = hoist ? al((size_t)c->nq * 2 * sizeof(double)) : 0;
+               b_G = al(ng * sizeof(EnsGrain)), b_ffh = hoist || sep ? al((size_t)c->nq * 2 * sizeof(double)) : 0;
   if (int rc = ensure_work(c, b_part + b_E + b_loc + b_M + b_G + b_ffh)) return rc;
   char* w = static_cast<char*>(c->d_work);
   double* part = reinterpret_cast<double*>(w);
@@ -653,7 +870,7 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   double* dloc = reinterpret_cast<double*>(w + b_part + b_E);
   double* dM = reinterpret_cast<double*>(w + b_part + b_E + b_loc);
   EnsGrain* dG = reinterpret_cast<EnsGrain*>(w + b_part + b_E + b_loc + b_M);
-  double* ffh = hoist ? reinterpret_cast<double*>(w + b_part + b_E + b_loc + b_M + b_G) : nullptr;
+  double* ffh = hoist || sep ? reinterpret_cast<double*>(w + b_part + b_E + b_loc + b_M + b_G) : nullptr;
   {
     const void* src[4] = {E.data(), loc.data(), Mm.data(), G.data()};
     const size_t nb[4] = {ne * sizeof(EnsElem), loc.size() * sizeof(double), Mm.size() * sizeof(double),
@@ -680,6 +897,26 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     LaunchTimer lt(c, true);
     if (f32) k_ensemble_sf32<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
     else k_ensemble_sf64<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
+    HGX_CHECK_LAUNCH(c);
+  } else if (sep) {
+    const int sh = m->elem[0].shape;
+    {
+      LaunchTimer lt(c, true);
+#define HGX_ENS_Z(S)                                                                                   \
+  if (f32) k_ensemble_zcell32<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, tz, sfb, ffh); \
+  else k_ensemble_zcell64<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, tz, sfb, ffh)
+      if (sh == HGX_SHAPE_BOX) HGX_ENS_Z(HGX_SHAPE_BOX);
+      else HGX_ENS_Z(HGX_SHAPE_CUBE);
+#undef HGX_ENS_Z
+      HGX_CHECK_LAUNCH(c);
+    }
+    LaunchTimer lt(c, true);
+#define HGX_ENS_SEP(S)                                                                                                \
+  if (f32) k_ensemble_sep32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part); \
+  else k_ensemble_sep64<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part)
+    if (sh == HGX_SHAPE_BOX) HGX_ENS_SEP(HGX_SHAPE_BOX);
+    else HGX_ENS_SEP(HGX_SHAPE_CUBE);
+#undef HGX_ENS_SEP
     HGX_CHECK_LAUNCH(c);
   } else {
     LaunchTimer lt(c, true);

@@ -123,6 +123,40 @@ def test_structure_factor_forms(gpu_ctx, oracle_mod, precision, tol, case):
     assert r <= tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32c", 1e-5)])
+@pytest.mark.parametrize("shape,dims", [
+    ("box", [([30.0, 34.0], [0.6, 0.4]), ([10.0, 11.0, 12.0], [0.2, 0.5, 0.3]), ([40.0, 44.0], [0.5, 0.5])]),
+    ("cube", [([9.0], [1.0])]),
+])
+def test_separable_cells(gpu_ctx, oracle_mod, precision, tol, shape, dims):
+    """Box and one-node cube cells under grains rotated about z, with an element rotation about z and
+    two locations at one height: the kernel pair that evaluates the in-plane factor once per grain and
+    the vertical factor once per branch. Debye-Waller damping, 3-D grain centres, DWBA branches."""
+    from hipgisaxs_amd.api import ensemble_intensity
+
+    O = oracle_mod
+    g, fc = _grid(O, gpu_ctx, False)
+    el = [{"shape": shape, "dims": dims, "rot": _rz(0.3), "locations": [[0, 0, 2.0], [25, 7, 2.0]], "dn2": -1e-5}]
+    la, lb, lc, rep = (60.0, 0, 0), (0, 45.0, 0), (0, 0, 50.0), (4, 6, 1)
+    rng = np.random.default_rng(5)
+    rots = [_rz(a) for a in rng.uniform(0, np.pi, 7)]
+    centers = rng.uniform(-40, 40, (7, 3))
+    weights = rng.uniform(0.5, 1.5, 7)
+    got = ensemble_intensity(gpu_ctx, el, la, lb, lc, rep, np.stack(rots), centers=centers, weights=weights,
+                             dw_sigma=0.4, fc=_dev(fc), precision=precision).cpu().numpy()
+    I = None
+    for R, c, w in zip(rots, centers, weights):
+        M = R @ el[0]["rot"]
+        ff = 0
+        for t in el[0]["locations"]:
+            ff = ff + el[0]["dn2"] * O.ff_analytic(shape, dims, g, rot=M, transvec=t)
+        sf = O.sf_lattice(la, lb, lc, rep, c, g, dw_sigma=0.4, rot=R)
+        I = O.dwba_intensity(g.npix, g.nq // g.npix, fc, sf, ff, w, I)
+    r = rel_l2(got, I)
+    print(shape, precision, "separable relL2", r)
+    assert r <= tol
+
+
 @pytest.mark.parametrize("shape,dims", [
     ("box", [([30.0, 32.0], [0.5, 0.5]), ([10.0], [1.0]), ([40.0, 44.0, 48.0], [0.2, 0.5, 0.3])]),
     ("cube", [([9.0, 11.0], [0.6, 0.4])]),
