@@ -405,6 +405,22 @@ def bench_ewald(nshapes=64, reps=2):
                         "FP32C approximate method, one call per shape"}
 
 
+# Algorithmic work of the fused ensemble kernels as implemented (add/mul 1 flop, FMA 2; sin/cos/exp
+# counted apart as T). Per (pixel, grain) of a lattice whose in-plane directions and grain centre give
+# branch-independent factors (rotations about z): four in-plane phases (16), two Dirichlet kernels and
+# the centre phase shared by the four branches (9 + 6 T), per branch the out-of-plane lattice phase,
+# its Dirichlet kernel, the product with the centre phase and the complex multiply-add with the
+# hoisted cell value (16 + 2 T, x4), weight and |amp|^2 (5): 94 flop + 14 T. A separable box cell adds
+# its in-plane factor, rotated q and location phase per grain: 28 flop + 4 T. Per (pixel, branch) the
+# hoisted cell (its nodes x ~55-60 flop) and the branch factor (8).
+ENS_GRAIN_FLOP = 94
+ENS_SEP_FLOP = 28
+
+
+def ensemble_flop(npix, nbranch, cell_per_branch, ngrains, per_grain=ENS_GRAIN_FLOP):
+    return npix * (nbranch * (cell_per_branch + 8) + ngrains * per_grain)
+
+
 def bench_lattice(reps=5):
     """Spheres R ~ N(5, 0.5) nm (21 nodes on [3, 7]) on bcc a = 20 nm, 10x10x5, 8 azimuths, DW 0.5 nm,
     1024^2 detector (2 m, 172 um), 12 keV, alpha_i = 0.15 deg, DWBA over Si: fused kernel, both paths."""
@@ -445,10 +461,10 @@ def bench_lattice(reps=5):
     pix = pixel_errors()
     pix.add(res["fp32c"], res["fp64"])
     out["pixel_error_fp32c_vs_fp64"] = pix.result()
-    # VALU roofline on the work the kernel must do: the 8 azimuths rotate about z, so the sphere form
-    # factor (about 55 flop per (q, node)) is evaluated once per q-point and branch; per (q, grain) the
-    # lattice factor (about 155 flop) and the DWBA term (about 15 flop); transcendentals not counted
-    flop_step = 4 * 1024 * 1024 * (v.size * 55 + 8 * (155 + 15))
+    # VALU roofline on the work the kernels do (ensemble_flop): the 8 azimuths rotate about z, so the
+    # sphere form factor (about 55 flop per node) is evaluated once per (pixel, branch) and the grain
+    # loop runs the structure factor alone; transcendentals not counted
+    flop_step = ensemble_flop(1024 * 1024, 4, v.size * 55, 8)
     for prec, peak in (("fp32c", PEAK_FP32), ("fp64", PEAK_FP64)):
         a = flop_step / (out[prec]["kernel_ms"] * 1e-3)
         out[prec]["roofline"] = {"bound": "valu", "achieved": a / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
@@ -637,13 +653,13 @@ def bench_hig_examples(reps=3, cpu=True):
             res[prec]["kernels"] = sim.kernel_ms()
             sim.close()
         ng_ = 625 if name != "01-pyramid" else 1
-        if name.startswith(("02", "03")):
-            # fused ensemble kernel, VALU roofline with the lattice leg's flop model: the grains rotate
-            # about z, so the particle form factor (sphere about 55 flop; cylinder 60) is evaluated once
-            # per (q, branch); per (q, grain) the lattice factor (155) and the DWBA term (15 per
-            # branch); transcendentals not counted
-            ffl = 55 if name.startswith("02") else 60
-            flop = 4 * img.size * (ffl + ng_ * (155 + 15))
+        if name.startswith(("02", "03", "07")):
+            # fused ensemble kernels, VALU roofline on the work they do (ensemble_flop): the grains
+            # rotate about z, so the sphere (about 55 flop) and cylinder (60) form factors are evaluated
+            # once per (pixel, branch) and the grain loop runs the structure factor; the boxes of 07
+            # split into a vertical factor per branch (about 12 flop) and an in-plane one per grain
+            ffl = {"02": 55, "03": 60, "07": 12}[name[:2]]
+            flop = ensemble_flop(img.size, 4, ffl, ng_, ENS_GRAIN_FLOP + (ENS_SEP_FLOP if name.startswith("07") else 0))
             for prec, peak in (("fp64", PEAK_FP64), ("fp32c", PEAK_FP32)):
                 a_ = flop / (res[prec]["kernels"]["main_ms"] * 1e-3)
                 res[prec]["roofline"] = {"bound": "valu", "achieved": a_ / 1e12, "peak": peak / 1e12,

@@ -202,6 +202,7 @@ __global__ __launch_bounds__(256) void k_ensemble_cell64(QViewE g, int nb, const
 }
 
 // The grain loop of a grain-invariant cell: sum_g |w_g sum_b ffh_b S_g(q_b)|^2 over this y-slice.
+template <int NB>
 __global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const double* __restrict__ ffh,
                                                        const EnsGrain* __restrict__ G, int ngrains, int per_y,
                                                        EnsSF sf, double* __restrict__ part) {
@@ -209,10 +210,9 @@ __global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const d
   if (p >= g.npix) return;
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
-  cd qzb[4], F[4];
+  cd qzb[NB], F[NB];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (b >= nb) break;
+  for (int b = 0; b < NB; ++b) {
     const int64_t q = p + b * g.npix;
     qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
     F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
@@ -230,8 +230,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const d
     if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
     cd amp = cd{0.0, 0.0};
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b >= nb) break;
+    for (int b = 0; b < NB; ++b) {
       cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
       if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
       amp = amp + F[b] * S;
@@ -358,6 +357,7 @@ __global__ __launch_bounds__(256) void k_ensemble_cell32(QViewE g, int nb, const
 }
 
 // FP32C grain loop of a grain-invariant cell
+template <int NB>
 __global__ __launch_bounds__(256) void k_ensemble_sf32(QViewE g, int nb, const double* __restrict__ ffh,
                                                        const EnsGrain* __restrict__ G, int ngrains, int per_y,
                                                        EnsSF sf, double* __restrict__ part) {
@@ -366,14 +366,15 @@ __global__ __launch_bounds__(256) void k_ensemble_sf32(QViewE g, int nb, const d
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
   const float N[3] = {(float)sf.N[0], (float)sf.N[1], (float)sf.N[2]};
-  double qzb[4];
-  cd F[4];
+  // the branch factors in FP32: the sum over four branches is the grain's amplitude, each term an
+  // FP32 structure factor times an FP64 cell value, so FP32 products lose nothing at the FP32C bar
+  double qzb[NB];
+  cf F[NB];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (b >= nb) break;
+  for (int b = 0; b < NB; ++b) {
     const int64_t q = p + b * g.npix;
     qzb[b] = g.qz[2 * q];
-    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+    F[b] = cf{(float)ffh[2 * q], (float)ffh[2 * q + 1]};
   }
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
@@ -384,15 +385,13 @@ __global__ __launch_bounds__(256) void k_ensemble_sf32(QViewE g, int nb, const d
     float Df;
     cf Pf;
     sf_flat32(gr, u, N, Df, Pf);
-    cd amp = cd{0.0, 0.0};
+    cf amp = cf{0.0f, 0.0f};
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b >= nb) break;
-      const cf S = sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
-      amp = amp + F[b] * cd{(double)S.r, (double)S.i};
+    for (int b = 0; b < NB; ++b) {
+      amp = amp + F[b] * sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
     }
-    amp = gr.w * amp;
-    acc += amp.r * amp.r + amp.i * amp.i;
+    const double ar = gr.w * (double)amp.r, ai = gr.w * (double)amp.i;
+    acc += ar * ar + ai * ai;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
 }
@@ -498,7 +497,7 @@ __global__ __launch_bounds__(256) void k_ensemble_zcell32(QViewE g, int nb, cons
 
 // The grain loop of a separable cell, FP64: sum_g |w_g F_par(g) L(g) sum_b ffh_b S_g(q_b)|^2 with
 // L(g) the in-plane location phases.
-template <int SHAPE>
+template <int SHAPE, int NB>
 __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const double* __restrict__ ffh,
                                                         const EnsElem* __restrict__ E, const double* __restrict__ loc,
                                                         const double* __restrict__ M, const EnsGrain* __restrict__ G,
@@ -507,10 +506,9 @@ __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const 
   if (p >= g.npix) return;
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
-  cd qzb[4], F[4];
+  cd qzb[NB], F[NB];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (b >= nb) break;
+  for (int b = 0; b < NB; ++b) {
     const int64_t q = p + b * g.npix;
     qzb[b] = cd{g.qz[2 * q], g.qz[2 * q + 1]};
     F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
@@ -534,8 +532,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const 
     if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
     cd sum = cd{0.0, 0.0};
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b >= nb) break;
+    for (int b = 0; b < NB; ++b) {
       cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
       if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
       sum = sum + F[b] * S;
@@ -546,7 +543,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const 
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
 }
 
-template <int SHAPE>
+template <int SHAPE, int NB>
 __global__ __launch_bounds__(256) void k_ensemble_sep32(QViewE g, int nb, const double* __restrict__ ffh,
                                                         const EnsElem* __restrict__ E, const double* __restrict__ loc,
                                                         const double* __restrict__ M, const EnsGrain* __restrict__ G,
@@ -556,14 +553,13 @@ __global__ __launch_bounds__(256) void k_ensemble_sep32(QViewE g, int nb, const 
   const int g0 = blockIdx.y * per_y, g1 = min(ngrains, g0 + per_y);
   const double qx = g.qx[p], qy = g.qy[p];
   const float N[3] = {(float)sf.N[0], (float)sf.N[1], (float)sf.N[2]};
-  double qzb[4];
-  cd F[4];
+  double qzb[NB];
+  cf F[NB];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (b >= nb) break;
+  for (int b = 0; b < NB; ++b) {
     const int64_t q = p + b * g.npix;
     qzb[b] = g.qz[2 * q];
-    F[b] = cd{ffh[2 * q], ffh[2 * q + 1]};
+    F[b] = cf{(float)ffh[2 * q], (float)ffh[2 * q + 1]};
   }
   const int l0 = E[0].loc0, l1 = E[0].loc0 + E[0].nloc;
   double acc = 0.0;
@@ -580,15 +576,14 @@ __global__ __launch_bounds__(256) void k_ensemble_sep32(QViewE g, int nb, const 
     float Df;
     cf Pf;
     sf_flat32(gr, u, N, Df, Pf);
-    cd sum = cd{0.0, 0.0};
+    cf sum = cf{0.0f, 0.0f};
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      if (b >= nb) break;
-      const cf S = sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
-      sum = sum + F[b] * cd{(double)S.r, (double)S.i};
+    for (int b = 0; b < NB; ++b) {
+      sum = sum + F[b] * sf_branch32(gr, u, Df, Pf, qzb[b], N, sf, qx, qy);
     }
-    const cd amp = (gr.w * (double)fp) * (cd{(double)lph.r, (double)lph.i} * sum);
-    acc += amp.r * amp.r + amp.i * amp.i;
+    const cf a = (fp * lph) * sum;
+    const double ar = gr.w * (double)a.r, ai = gr.w * (double)a.i;
+    acc += ar * ar + ai * ai;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
 }
@@ -895,8 +890,11 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
       HGX_CHECK_LAUNCH(c);
     }
     LaunchTimer lt(c, true);
-    if (f32) k_ensemble_sf32<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
-    else k_ensemble_sf64<<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dG, ng, per_y, sfb, part);
+    const bool four = c->nbranch == 4;
+    if (f32 && four) k_ensemble_sf32<4><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
+    else if (f32) k_ensemble_sf32<1><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
+    else if (four) k_ensemble_sf64<4><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
+    else k_ensemble_sf64<1><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
     HGX_CHECK_LAUNCH(c);
   } else if (sep) {
     const int sh = m->elem[0].shape;
@@ -911,12 +909,22 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
       HGX_CHECK_LAUNCH(c);
     }
     LaunchTimer lt(c, true);
-#define HGX_ENS_SEP(S)                                                                                                \
-  if (f32) k_ensemble_sep32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part); \
-  else k_ensemble_sep64<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part)
-    if (sh == HGX_SHAPE_BOX) HGX_ENS_SEP(HGX_SHAPE_BOX);
-    else HGX_ENS_SEP(HGX_SHAPE_CUBE);
+#define HGX_ENS_SEP_NB(S, B)                                                                                    \
+  if (f32) k_ensemble_sep32<S, B><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part); \
+  else k_ensemble_sep64<S, B><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part)
+#define HGX_ENS_SEP(S)                  \
+  if (c->nbranch == 4) {                \
+    HGX_ENS_SEP_NB(S, 4);               \
+  } else {                              \
+    HGX_ENS_SEP_NB(S, 1);               \
+  }
+    if (sh == HGX_SHAPE_BOX) {
+      HGX_ENS_SEP(HGX_SHAPE_BOX)
+    } else {
+      HGX_ENS_SEP(HGX_SHAPE_CUBE)
+    }
 #undef HGX_ENS_SEP
+#undef HGX_ENS_SEP_NB
     HGX_CHECK_LAUNCH(c);
   } else {
     LaunchTimer lt(c, true);

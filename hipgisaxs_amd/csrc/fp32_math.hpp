@@ -16,14 +16,15 @@ constexpr float kPiF = 3.14159265358979323846f;
 __device__ inline float sin_turn(float t) { return __builtin_amdgcn_sinf(t); }
 __device__ inline float cos_turn(float t) { return __builtin_amdgcn_cosf(t); }
 
-// sin(pi * u) with relative accuracy also for small |u| (u in turns / 2, i.e. half-turns)
+// sin(pi * u) with relative accuracy also for small |u| (u in turns / 2, i.e. half-turns): the Taylor
+// series below |pi u| = 1/8, the hardware sine above. Both are evaluated and one is selected, so a
+// wave whose lanes fall on both sides runs no divergent branches.
 __device__ inline float sin_pi_accurate(float u) {
-  float y = kPiF * u;
-  if (fabsf(y) < 0.125f) {
-    float y2 = y * y;
-    return y * (1.0f - y2 * (1.0f / 6.0f) * (1.0f - y2 * (1.0f / 20.0f) * (1.0f - y2 * (1.0f / 42.0f))));
-  }
-  return sin_turn(0.5f * u);
+  const float y = kPiF * u;
+  const float y2 = y * y;
+  const float t = y * (1.0f - y2 * (1.0f / 6.0f) * (1.0f - y2 * (1.0f / 20.0f) * (1.0f - y2 * (1.0f / 42.0f))));
+  const float h = sin_turn(0.5f * u);
+  return fabsf(y) < 0.125f ? t : h;
 }
 
 // J0 and J1 of a real argument, Abramowitz & Stegun, Handbook of Mathematical Functions,
@@ -72,10 +73,12 @@ __device__ inline cf expi_turn(float t) { return cf{cos_turn(t), sin_turn(t)}; }
 
 // Dirichlet kernel D_N(x) = sin(N x / 2) / sin(x / 2) for x already wrapped to (-pi, pi] and
 // given in turns (u = x / 2 pi in (-1/2, 1/2]); the x -> 0 limit is N.
+// The quotient takes the hardware reciprocal (v_rcp_f32, 1 ulp) instead of the IEEE division sequence:
+// den is a sine, never near the float range limits, and the result carries FP32 error anyway.
 __device__ inline float dirichlet_turn(float u, float N) {
-  float den = sin_pi_accurate(u);
-  if (fabsf(den) < 1e-30f) return N;
-  return sin_pi_accurate(N * u) / den;
+  const float den = sin_pi_accurate(u);
+  const float v = sin_pi_accurate(N * u) * __builtin_amdgcn_rcpf(den);
+  return fabsf(den) < 1e-30f ? N : v;
 }
 
 }  // namespace hgx
