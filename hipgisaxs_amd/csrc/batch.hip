@@ -140,6 +140,9 @@ void launch_main(const Batch& b, hipStream_t s) {
   if (b.npunits > 0)
     product_launch_units(s, static_cast<const ProductItem*>(b.d_pitems), static_cast<const ProductUnit*>(b.d_punits),
                          b.npunits, b.cq, b.pmode);
+  for (int k = 0; k < 4; ++k)  // Ewald-grid items, one launch per series order
+    ewald_launch_units(s, static_cast<const EwaldItem*>(b.d_eitems),
+                       static_cast<const EwaldUnit*>(b.d_eunits) + b.eunit_off[k], b.neunits[k], b.cq, k, b.epipe);
   if (b.nunits == 0) return;
   switch (b.qpt) {
     case 1: launch_batch<1>(b, s); break;
@@ -166,6 +169,8 @@ void free_batch_state(BatchState& b) {
   if (b.d_pitems) (void)hipFree(b.d_pitems);
   if (b.d_punits) (void)hipFree(b.d_punits);
   if (b.d_prot) (void)hipFree(b.d_prot);
+  if (b.d_eitems) (void)hipFree(b.d_eitems);
+  if (b.d_eunits) (void)hipFree(b.d_eunits);
   b = BatchState();
 }
 }  // namespace hgx
@@ -191,10 +196,13 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   b.flags = flags;
   b.qpt = c->tune.batch_qpt;
   b.pmode = precision == HGX_PREC_FP64 ? product_mode64(c) : product_mode(c);
+  b.epipe = c->tune.ewald_pipe != 0;
   const int64_t bpu = c->tune.batch_bpu;
   std::vector<BatchItemDev> it(n);
   std::vector<ProductItem> pit(n);
+  std::vector<EwaldItem> eit(n);
   std::vector<char> is_prod(n, 0);
+  std::vector<int> eord(n, -1);  // series orders of Ewald-grid items, -1 for the others
   std::vector<int64_t> part_off(n, -1), rot_off(n, -1);
   int64_t part_total = 0, rot_total = 0;  // doubles, bytes
   for (int i = 0; i < n; ++i) {
@@ -225,6 +233,26 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     d.ntri = m.ntri;
     d.bchunk = std::min<int64_t>(bpu, m.nblocks);
     const bool prod = gc->product && !(flags & HGX_FLAG_NO_MFMA);
+    // Ewald-sphere grid (QGrid::create, detector), FP32C: matrix products with a short series
+    // (ewald_mfma.hip), as hgx_ff_numeric takes them
+    if (!prod && precision == HGX_PREC_FP32C && gc->ew.ok && !(flags & (HGX_FLAG_NO_MFMA | HGX_FLAG_ACC_KAHAN)))
+      eord[i] = ewald_orders(gc, m.radius);
+    if (eord[i] >= 0) {
+      d.e.series_qr = kSeriesQRSplit;
+      d.bchunk = std::min<int64_t>(product_blocks_per_unit(c), m.nblocks);
+      EwaldItem& ei = eit[i];
+      ewald_fill_item(gc, m, d.R, ei);
+      ei.bchunk = d.bchunk;
+      bool identity = true;
+      for (int k = 0; k < 9; ++k) identity = identity && x.rot[k] == ((k % 4 == 0) ? 1.0 : 0.0);
+      if (identity) {
+        ei.rec = m.d_rec32;
+        ei.borig = m.d_borig;
+      } else {
+        rot_off[i] = rot_total;
+        rot_total += (int64_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double));
+      }
+    }
     if (prod) {
       // product grid: complex matrix products on the matrix cores (product_mfma.hip)
       is_prod[i] = 1;
@@ -257,6 +285,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     }
     d.nchunks = (int)((m.nblocks + d.bchunk - 1) / d.bchunk);
     pit[i].nchunks = d.nchunks;
+    eit[i].nchunks = d.nchunks;
     d.out = x.ff;
     d.part = nullptr;
     if (d.nchunks > 1) {
@@ -277,11 +306,28 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     ProductUnit u;
   };
   std::vector<PU> pus;
+  struct EU {
+    double cost;
+    EwaldUnit u;
+  };
+  std::vector<EU> eus[4];
   std::vector<FinishUnit> fin;
   const int64_t qtile = (int64_t)kWG * b.qpt;
   for (int i = 0; i < n; ++i) {
     const BatchItemDev& d = it[i];
     for (int64_t qb = 0; qb < (d.g.nq + kWG - 1) / kWG; ++qb) fin.push_back(FinishUnit{i, (int)qb});
+    if (eord[i] >= 0) {
+      const EwaldItem& ei = eit[i];
+      const int nyg = ewald_col_groups(ei.ncol);
+      // matrix work grows with the orders: split fp16 for orders 0 and 1, single fp16 above
+      const double w = 3.0 * std::min(eord[i] + 1, 2) + std::max(eord[i] - 1, 0);
+      for (int ch = 0; ch < d.nchunks; ++ch) {
+        int64_t nb = std::min<int64_t>(d.bchunk, d.nblocks - ch * d.bchunk);
+        for (int zt = 0; zt < ei.ntile; ++zt)
+          for (int yg = 0; yg < nyg; ++yg) eus[eord[i]].push_back(EU{w * (double)nb, EwaldUnit{i, zt, yg, ch}});
+      }
+      continue;
+    }
     if (is_prod[i]) {
       const ProductItem& pi = pit[i];
       int nzt = (int)ceil_div(pi.nz, product_zrows(b.pmode)), nyt = (int)ceil_div(pi.ncol, product_ycols(b.pmode));
@@ -307,7 +353,14 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   for (size_t k = 0; k < us.size(); ++k) units[k] = us[k].u;
   std::vector<ProductUnit> punits(pus.size());
   for (size_t k = 0; k < pus.size(); ++k) punits[k] = pus[k].u;
-  if (units.size() > 0x7fffffff || punits.size() > 0x7fffffff)
+  std::vector<EwaldUnit> eunits;
+  for (int k = 0; k < 4; ++k) {
+    std::stable_sort(eus[k].begin(), eus[k].end(), [](const EU& a, const EU& b2) { return a.cost > b2.cost; });
+    b.eunit_off[k] = (int)eunits.size();
+    b.neunits[k] = (int)eus[k].size();
+    for (const EU& e : eus[k]) eunits.push_back(e.u);
+  }
+  if (units.size() > 0x7fffffff || punits.size() > 0x7fffffff || eunits.size() > 0x7fffffff)
     return set_error(c, HGX_ERR_INVALID_ARG, "batch_create: too many work units");
   b.nunits = (int)units.size();
   b.npunits = (int)punits.size();
@@ -327,8 +380,8 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   };
   if (part_total > 0 && hipMalloc(&b.d_part, part_total * sizeof(double)) != hipSuccess) return fail("partials");
   for (int i = 0; i < n; ++i)
-    if (part_off[i] >= 0) it[i].part = pit[i].part = b.d_part + part_off[i];
-  for (int i = 0; i < n; ++i) pit[i].out = it[i].out;
+    if (part_off[i] >= 0) it[i].part = pit[i].part = eit[i].part = b.d_part + part_off[i];
+  for (int i = 0; i < n; ++i) pit[i].out = eit[i].out = it[i].out;
   if (rot_total > 0) {
     if (hipMalloc(&b.d_prot, rot_total) != hipSuccess) return fail("rotated records");
     for (int i = 0; i < n; ++i) {
@@ -350,9 +403,16 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
         free_batch_state(b);
         return rc;
       }
-      pit[i].rec = rr;
-      pit[i].borig = ro;
+      pit[i].rec = eit[i].rec = rr;
+      pit[i].borig = eit[i].borig = ro;
     }
+  }
+  if (!eunits.empty()) {
+    if (hipMalloc(&b.d_eitems, eit.size() * sizeof(EwaldItem)) != hipSuccess) return fail("Ewald-grid items");
+    if (hipMalloc(&b.d_eunits, eunits.size() * sizeof(EwaldUnit)) != hipSuccess) return fail("Ewald-grid units");
+    if (!h2d(b.d_eitems, eit.data(), eit.size() * sizeof(EwaldItem)) ||
+        !h2d(b.d_eunits, eunits.data(), eunits.size() * sizeof(EwaldUnit)))
+      return copy_fail();
   }
   if (b.npunits > 0) {
     if (hipMalloc(&b.d_pitems, pit.size() * sizeof(ProductItem)) != hipSuccess) return fail("product items");
@@ -434,7 +494,8 @@ extern "C" int hgx_batch_info(hgx_ctx* h, int batch_id, int64_t* units, double* 
   if (!c) return HGX_ERR_BAD_HANDLE;
   auto& B = batches(c);
   if (batch_id < 0 || batch_id >= (int)B.size() || !B[batch_id].live) return HGX_ERR_BAD_HANDLE;
-  if (units) *units = (int64_t)B[batch_id].nunits + B[batch_id].npunits;
+  const Batch& bb = B[batch_id];
+  if (units) *units = (int64_t)bb.nunits + bb.npunits + bb.neunits[0] + bb.neunits[1] + bb.neunits[2] + bb.neunits[3];
   if (pairs) *pairs = B[batch_id].pairs;
   return HGX_OK;
 }

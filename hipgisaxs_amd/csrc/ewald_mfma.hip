@@ -119,18 +119,29 @@ __device__ inline EwStep<NO, NS> ew_step(const float4* __restrict__ rb, int ps, 
 // PIPE: the column operands of step p + 1 are built during step p and held in registers (one wave per
 // SIMD fits); otherwise each step builds its own before its products, which keeps the kernel within
 // two waves per SIMD, so two workgroups share a compute unit and cover each other's barriers.
+// units == nullptr: the single item `single`, units decoded from blockIdx (chunk-major, column group
+// fastest); else unit blockIdx.x of a batch's table, its item from `items`.
 template <bool CQ, int K, bool PIPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 2, 8))) void k_ff_ewald16(EwaldItem it,
-                                                                                                  int nyg) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 2, 8))) void k_ff_ewald16(
+    const EwaldItem* __restrict__ items, const EwaldUnit* __restrict__ units, EwaldItem single, int nyg) {
   constexpr int MZ = 2;
   constexpr int NO = K + 1;
   constexpr int NS = K >= 1 ? 2 : 1;
   constexpr int kSteps = kBlockTri / 4;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int uidx = blockIdx.x;
-  const int yg = uidx % nyg;
-  const int zt = (uidx / nyg) % it.ntile;
-  const int chunk = uidx / (nyg * it.ntile);
+  int yg, zt, chunk;
+  if (units) {
+    const EwaldUnit u = units[uidx];
+    yg = u.yg;
+    zt = u.tile;
+    chunk = u.chunk;
+  } else {
+    yg = uidx % nyg;
+    zt = (uidx / nyg) % single.ntile;
+    chunk = uidx / (nyg * single.ntile);
+  }
+  const EwaldItem it = units ? items[units[uidx].item] : single;
   const int r = lane & 31, h = lane >> 5;
   const int ncol = it.ncol;
   const int y0 = (yg * kProductWaves + wave) * kProductTile;
@@ -341,12 +352,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 
 }
 
 template <bool CQ, bool PIPE>
-void launch_orders(hipStream_t s, const EwaldItem& it, int orders, unsigned grid, int nyg) {
+void launch_orders(hipStream_t s, const EwaldItem* items, const EwaldUnit* units, const EwaldItem& it, int orders,
+                   unsigned grid, int nyg) {
   switch (orders) {
-    case 0: k_ff_ewald16<CQ, 0, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
-    case 1: k_ff_ewald16<CQ, 1, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
-    case 2: k_ff_ewald16<CQ, 2, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
-    default: k_ff_ewald16<CQ, 3, PIPE><<<grid, 256, 0, s>>>(it, nyg); break;
+    case 0: k_ff_ewald16<CQ, 0, PIPE><<<grid, 256, 0, s>>>(items, units, it, nyg); break;
+    case 1: k_ff_ewald16<CQ, 1, PIPE><<<grid, 256, 0, s>>>(items, units, it, nyg); break;
+    case 2: k_ff_ewald16<CQ, 2, PIPE><<<grid, 256, 0, s>>>(items, units, it, nyg); break;
+    default: k_ff_ewald16<CQ, 3, PIPE><<<grid, 256, 0, s>>>(items, units, it, nyg); break;
+  }
+}
+
+void launch_any(hipStream_t s, const EwaldItem* items, const EwaldUnit* units, const EwaldItem& it, int orders,
+                unsigned grid, int nyg, bool cq, bool pipe) {
+  if (cq) {
+    if (pipe) launch_orders<true, true>(s, items, units, it, orders, grid, nyg);
+    else launch_orders<true, false>(s, items, units, it, orders, grid, nyg);
+  } else {
+    if (pipe) launch_orders<false, true>(s, items, units, it, orders, grid, nyg);
+    else launch_orders<false, false>(s, items, units, it, orders, grid, nyg);
   }
 }
 
@@ -357,16 +380,36 @@ int ewald_launch(Context* c, const EwaldItem& it, bool cq, int orders) {
   const int nyg = (int)ceil_div(it.ncol, kProductWaves * kProductTile);
   const int64_t units = (int64_t)nyg * it.ntile * it.nchunks;
   if (units <= 0 || units > INT32_MAX) return set_error(c, HGX_ERR_INVALID_ARG, "ewald: %lld work units", (long long)units);
-  const bool pipe = c->tune.ewald_pipe != 0;
-  if (cq) {
-    if (pipe) launch_orders<true, true>(c->stream, it, orders, (unsigned)units, nyg);
-    else launch_orders<true, false>(c->stream, it, orders, (unsigned)units, nyg);
-  } else {
-    if (pipe) launch_orders<false, true>(c->stream, it, orders, (unsigned)units, nyg);
-    else launch_orders<false, false>(c->stream, it, orders, (unsigned)units, nyg);
-  }
+  launch_any(c->stream, nullptr, nullptr, it, orders, (unsigned)units, nyg, cq, c->tune.ewald_pipe != 0);
   HGX_CHECK_LAUNCH(c);
   return HGX_OK;
+}
+
+void ewald_launch_units(hipStream_t s, const EwaldItem* d_items, const EwaldUnit* d_units, int nunits, bool cq,
+                        int orders, bool pipelined) {
+  if (nunits <= 0) return;
+  EwaldItem none{};
+  launch_any(s, d_items, d_units, none, orders, (unsigned)nunits, 1, cq, pipelined);
+}
+
+void ewald_fill_item(const Context* gc, const Mesh& m, const Rot& R, EwaldItem& it) {
+  it.qz = gc->d_qz;
+  it.row = gc->ew.d_row;
+  it.col = gc->ew.d_col;
+  it.tile = gc->ew.d_tile;
+  it.perm = gc->ew.d_perm;
+  it.nblocks = m.nblocks;
+  it.nq = gc->nq;
+  it.ncol = gc->ncol;
+  it.nrow = gc->nrow;
+  it.ntile = gc->ew.ntile;
+  it.snmax = (float)m.snmax;
+  it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
+                    m.radius);
+  // the expansion point: the mesh origin in the rotated frame of the records, R^T o
+  it.ox = R.m[0] * m.origin[0] + R.m[3] * m.origin[1] + R.m[6] * m.origin[2];
+  it.oy = R.m[1] * m.origin[0] + R.m[4] * m.origin[1] + R.m[7] * m.origin[2];
+  it.oz = R.m[2] * m.origin[0] + R.m[5] * m.origin[1] + R.m[8] * m.origin[2];
 }
 
 void ewald_free(Context* c) {

@@ -653,24 +653,8 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     const bool identity = rot == nullptr || (R.m[0] == 1 && R.m[1] == 0 && R.m[2] == 0 && R.m[3] == 0 &&
                                              R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
     EwaldItem it{};
-    it.qz = c->d_qz;
-    it.row = c->ew.d_row;
-    it.col = c->ew.d_col;
-    it.tile = c->ew.d_tile;
-    it.perm = c->ew.d_perm;
-    it.nblocks = m.nblocks;
-    it.nq = c->nq;
-    it.ncol = c->ncol;
-    it.nrow = c->nrow;
-    it.ntile = c->ew.ntile;
-    it.snmax = (float)m.snmax;
-    it.rmax = (float)(std::sqrt(m.origin[0] * m.origin[0] + m.origin[1] * m.origin[1] + m.origin[2] * m.origin[2]) +
-                      m.radius);
-    // the expansion point: the mesh origin in the rotated frame of the records, R^T o
-    it.ox = R.m[0] * m.origin[0] + R.m[3] * m.origin[1] + R.m[6] * m.origin[2];
-    it.oy = R.m[1] * m.origin[0] + R.m[4] * m.origin[1] + R.m[7] * m.origin[2];
-    it.oz = R.m[2] * m.origin[0] + R.m[5] * m.origin[1] + R.m[8] * m.origin[2];
-    const int64_t tiles = (int64_t)it.ntile * ceil_div(it.ncol, kProductWaves * kProductTile);
+    ewald_fill_item(c, m, R, it);
+    const int64_t tiles = (int64_t)it.ntile * ewald_col_groups(it.ncol);
     int64_t want = std::max<int64_t>(1, (int64_t)(c->tune.target_wg + tiles - 1) / tiles);
     want = std::min<int64_t>(want, m.nblocks);
     it.bchunk = std::min<int64_t>((m.nblocks + want - 1) / want, product_blocks_per_unit(c));

@@ -79,6 +79,12 @@ struct BatchState {
   void* d_pitems = nullptr;
   void* d_punits = nullptr;
   void* d_prot = nullptr;
+  // Ewald-grid items (FP32C on QGrid::create / detector grids): units grouped by series order
+  void* d_eitems = nullptr;
+  void* d_eunits = nullptr;
+  int neunits[4] = {0, 0, 0, 0};
+  int eunit_off[4] = {0, 0, 0, 0};
+  bool epipe = false;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   double pairs = 0.0;
@@ -515,6 +521,15 @@ int adjoint_product_variant(const Context* c);  // the context's setting (hgx_tu
 void adjoint_product_launch(hipStream_t s, const AdjItem* d_items, const AdjItem& single, int upi, int nunits,
                             bool cq, int variant);
 // Tuning: triangle blocks per work unit and the kernel variant (kProduct* above); the FP64 variant.
+// A work unit of a batch's Ewald-grid items: one 64-row tile x 128-column group of one item over one
+// chunk of triangle blocks.
+struct EwaldUnit {
+  int item;
+  int tile;
+  int yg;
+  int chunk;
+};
+
 // Ewald-grid tables of the current grid (ewald_mfma.hip): fitted after every grid change, freed with
 // the grid. ewald_orders gives the series orders K (0..3) the matrix-core path needs for a mesh of
 // this radius about its origin, or -1 when it would need more (the per-pair kernels then run).
@@ -522,6 +537,14 @@ int ewald_update(Context* c);
 void ewald_free(Context* c);
 int ewald_orders(const Context* c, double radius);
 int ewald_launch(Context* c, const EwaldItem& it, bool cq, int orders);
+// The grid, mesh and rotation fields of an item (expansion origin R^T o, tables, sizes); the caller
+// sets rec, borig, bchunk, nchunks, part and out.
+void ewald_fill_item(const Context* grid, const Mesh& m, const Rot& R, EwaldItem& it);
+// Column groups (128 columns) of a grid of ncol columns.
+inline int ewald_col_groups(int ncol) { return (ncol + kProductWaves * kProductTile - 1) / (kProductWaves * kProductTile); }
+// A batch's units of one series order (d_units), in one launch.
+void ewald_launch_units(hipStream_t s, const EwaldItem* d_items, const EwaldUnit* d_units, int nunits, bool cq,
+                        int orders, bool pipelined);
 int product_blocks_per_unit(const Context* c);
 int product_mode(const Context* c);
 inline int product_mode64(const Context* c);

@@ -91,3 +91,59 @@ def test_batch_refuses_a_replaced_grid_or_released_mesh(gpu_ctx):
         b.release()
     finally:
         gpu_ctx.release_mesh(m0)
+
+
+def test_batch_on_reference_grids_takes_the_ewald_kernel(gpu_ctx, oracle_mod):
+    """Items on QGrid::create q-space grids (src/model/qgrid.cpp:76-84) with DWBA branches, rotated meshes
+    of different sizes and series orders, FP32C: one batch (direct and three hipGraph replays) against
+    single hgx_ff_numeric calls on the same grids (the same Ewald-grid kernel: equal to FP32 accumulation
+    order) and against the oracle (relL2 <= 1e-5)."""
+    import torch
+
+    from hipgisaxs_amd import meshes
+    from hipgisaxs_amd.api import Batch, Context
+
+    O = oracle_mod
+    k0, ai = 2 * np.pi / (1.23984 / 10.0), np.radians(0.16)
+    rng = np.random.default_rng(21)
+    specs = {"a": (60, 40, 0.0, 0.0, 3.0, 2.5), "b": (96, 64, -1.0, 0.0, 1.0, 1.0)}
+    grids, hgrids = {}, {}
+    for key, args in specs.items():
+        c = Context(0)
+        c.qgrid_qspace(*args, ai, k0)
+        c.qz_extended(k0, ai, 0j)
+        grids[key] = c
+        hgrids[key] = O.qz_extended(O.qgrid_qspace(*args, ai, k0), k0, ai, 0j)
+    shapes = []
+    for k in range(5):
+        tri = meshes.convex_polyhedron(rng, rng.uniform(3, 12, 3), int(rng.integers(200, 1500)))
+        shapes.append((tri, "ab"[k % 2], _rz(0.4 * k)))
+    try:
+        outs, items, singles = [], [], []
+        for tri, key, R in shapes:
+            gc = grids[key]
+            mid = gpu_ctx.upload_mesh(tri, "exact9")
+            gmid = gc.upload_mesh(tri, "exact9")
+            path, orders = gc.ff_numeric_path(gmid, "approx", "fp32c")
+            assert path == "ewald", (path, orders)
+            nq = gc.info()["nq"]
+            o = torch.full((2 * nq,), float("nan"), dtype=torch.float64, device="cuda")
+            outs.append(o)
+            items.append((gc, mid, R, o))
+            singles.append(gc.as_complex(gc.ff_numeric(gmid, "approx", "fp32c", rot=R)).cpu().numpy())
+        b = Batch(gpu_ctx, items, precision="fp32c")
+        for use_graph in (False, True, True, True):
+            for o in outs:
+                o.fill_(float("nan"))
+            b.run(use_graph=use_graph)
+            torch.cuda.synchronize()
+            for (tri, key, R), o, single in zip(shapes, outs, singles):
+                got = gpu_ctx.as_complex(o).cpu().numpy()
+                assert np.all(np.isfinite(got))
+                ref = O.ff_approx(O.tri9_to_rec8(tri), hgrids[key], R)
+                assert rel_l2(got, ref) <= 1e-5, (use_graph, key)
+                assert rel_l2(got, single) <= 1e-6, (use_graph, key)
+        b.release()
+    finally:
+        for c in grids.values():
+            c.close()
