@@ -376,6 +376,27 @@ def bench_ewald(nshapes=64, reps=2):
     t_ew, k_ew = sched["default"]
     t_pp, k_pp = run("pair", N.FLAG_NO_MFMA, "fp32c", reps)
     t_64, k_64 = run("fp64", 0, "fp64", 1)
+    # the same 64 calls as one batch (meshes in one context, each item on its shape's grid), replayed
+    # from its hipGraph: one launch per series order plus the finishing launch
+    from hipgisaxs_amd.api import Batch
+
+    main = Context(0)
+    items = []
+    for s in shapes:
+        s["out"]["batch"] = torch.empty_like(s["out"]["ewald"])
+        items.append((ctxs[s["side"]], main.upload_mesh(s["tri"], "exact9"), None, s["out"]["batch"]))
+    bat = Batch(main, items, precision="fp32c")
+    bat.run(use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bat.run(use_graph=True)
+    torch.cuda.synchronize()
+    t_bat = (time.perf_counter() - t0) / reps
+    bnum = sum(float(torch.linalg.vector_norm(s["out"]["batch"] - s["out"]["ewald"])) ** 2 for s in shapes)
+    bden = sum(float(torch.linalg.vector_norm(s["out"]["ewald"])) ** 2 for s in shapes)
+    bat.release()
+    main.close()
     pix = pixel_errors()
     worst = 0.0
     for s in shapes:
@@ -395,6 +416,8 @@ def bench_ewald(nshapes=64, reps=2):
                                    "frac": 16 * pairs / (k_ew * 1e-3) / PEAK_FP16, "flop_per_pair": 16,
                                    "executed_tflops": flop_exec / (k_ew * 1e-3) / 1e12 if ok else None}},
             "ewald_pipelined": {"value": pairs / sched["pipelined"][0], "kernel_ms": sched["pipelined"][1]},
+            "ewald_batch_graph": {"value": pairs / t_bat, "ms": 1e3 * t_bat,
+                                  "relL2_vs_single_calls": math.sqrt(bnum / bden)},
             "fp32c_per_pair": {"value": pairs / t_pp, "ms": 1e3 * t_pp, "kernel_ms": k_pp},
             "fp64_per_pair": {"value": pairs / t_64, "ms": 1e3 * t_64, "kernel_ms": k_64},
             "speedup_vs_fp32c_per_pair": t_pp / t_ew, "speedup_vs_fp64_per_pair": t_64 / t_ew,
