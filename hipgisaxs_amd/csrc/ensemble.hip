@@ -124,6 +124,22 @@ __device__ inline double rot_norm2(const double* __restrict__ R, double qx, doub
   return s;
 }
 
+// D_N of a real phase (the real-argument case of dirichlet() in lattice_sf.hpp: the same wrap into
+// (-pi, pi] and the same limit N where the denominator vanishes), and e^{iu} by one sincos
+__device__ inline double dirichlet_real(double x, double N) {
+  const double w = x - 2.0 * kPiD * rint(x * (0.5 / kPiD));
+  const double h = 0.5 * w;
+  const double den = sin(h);
+  if (fabs(den) < 1e-300) return N;
+  return sin(N * h) / den;
+}
+
+__device__ inline cd expi_real(double u) {
+  double sn, cs;
+  sincos(u, &sn, &cs);
+  return cd{cs, sn};
+}
+
 // The branch factors every grain shares: the Fresnel coefficient and, when every R_g is orthogonal,
 // exp(-sigma^2/2 |R q|^2) = exp(-sigma^2/2 |q|^2).
 __device__ inline cd branch_factor64(const double* __restrict__ fc, int64_t q, double qx, double qy, cd qz,
@@ -201,8 +217,50 @@ __global__ __launch_bounds__(256) void k_ensemble_cell64(QViewE g, int nb, const
   }
 }
 
+// sum_b F_b S_g(q_b) for grain gr, FP64; CQ = false: real q_z, so every phase is real and the
+// structure factor is a real Dirichlet product times the centre phase (real sines, one sincos).
+template <int NB, bool CQ>
+__device__ __attribute__((always_inline)) inline cd branch_sum64(const EnsGrain& gr, double qx, double qy, const cd (&qzb)[NB], const cd (&F)[NB],
+                                  const EnsSF& sf) {
+  double u[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
+  cd sum = cd{0.0, 0.0};
+  if constexpr (CQ) {
+    cd Sflat = cd{1.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
+    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
+      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
+      sum = sum + F[b] * S;
+    }
+  } else {
+    double Df = 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (gr.flat & (1 << k)) Df *= dirichlet_real(u[k], sf.N[k]);
+    const cd Pf = (gr.flat & 8) ? expi_real(u[3]) : cd{1.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double qz = qzb[b].r;
+      double D = Df;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (!(gr.flat & (1 << k))) D *= dirichlet_real(u[k] + qz * gr.L[k][2], sf.N[k]);
+      if (sf.dw != 0.0 && !sf.dw_invariant) D *= exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b]));
+      const cd P = (gr.flat & 8) ? Pf : expi_real(u[3] + qz * gr.L[3][2]);
+      sum = sum + D * (F[b] * P);
+    }
+  }
+  return sum;
+}
+
 // The grain loop of a grain-invariant cell: sum_g |w_g sum_b ffh_b S_g(q_b)|^2 over this y-slice.
-template <int NB>
+template <int NB, bool CQ>
 __global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const double* __restrict__ ffh,
                                                        const EnsGrain* __restrict__ G, int ngrains, int per_y,
                                                        EnsSF sf, double* __restrict__ part) {
@@ -220,22 +278,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sf64(QViewE g, int nb, const d
   double acc = 0.0;
   for (int o = g0; o < g1; ++o) {
     const EnsGrain& gr = G[o];
-    double u[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
-    cd Sflat = cd{1.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
-    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
-    cd amp = cd{0.0, 0.0};
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
-      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
-      amp = amp + F[b] * S;
-    }
-    amp = gr.w * amp;
+    const cd amp = gr.w * branch_sum64<NB, CQ>(gr, qx, qy, qzb, F, sf);
     acc += amp.r * amp.r + amp.i * amp.i;
   }
   part[(int64_t)blockIdx.y * g.npix + p] = acc;
@@ -497,7 +540,7 @@ __global__ __launch_bounds__(256) void k_ensemble_zcell32(QViewE g, int nb, cons
 
 // The grain loop of a separable cell, FP64: sum_g |w_g F_par(g) L(g) sum_b ffh_b S_g(q_b)|^2 with
 // L(g) the in-plane location phases.
-template <int SHAPE, int NB>
+template <int SHAPE, int NB, bool CQ>
 __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const double* __restrict__ ffh,
                                                         const EnsElem* __restrict__ E, const double* __restrict__ loc,
                                                         const double* __restrict__ M, const EnsGrain* __restrict__ G,
@@ -522,21 +565,7 @@ __global__ __launch_bounds__(256) void k_ensemble_sep64(QViewE g, int nb, const 
     cd lph = cd{0.0, 0.0};
     for (int l = l0; l < l1; ++l) lph = lph + cexpi_d(cd{mx * loc[3 * l] + my * loc[3 * l + 1], 0.0});
     const cd A = fpar64<SHAPE>(mx, my, E[0].P) * lph;
-    double u[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) u[k] = qx * gr.L[k][0] + qy * gr.L[k][1] + gr.off[k];
-    cd Sflat = cd{1.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (gr.flat & (1 << k)) Sflat = Sflat * dirichlet(cd{u[k], 0.0}, sf.N[k]);
-    if (gr.flat & 8) Sflat = Sflat * cexpi_d(cd{u[3], 0.0});
-    cd sum = cd{0.0, 0.0};
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      cd S = sf_branch64(gr, u, Sflat, qzb[b], sf);
-      if (sf.dw != 0.0 && !sf.dw_invariant) S = exp(-sf.dw * rot_norm2(gr.rot, qx, qy, qzb[b])) * S;
-      sum = sum + F[b] * S;
-    }
+    const cd sum = branch_sum64<NB, CQ>(gr, qx, qy, qzb, F, sf);
     const cd amp = gr.w * (A * sum);
     acc += amp.r * amp.r + amp.i * amp.i;
   }
@@ -893,8 +922,10 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     const bool four = c->nbranch == 4;
     if (f32 && four) k_ensemble_sf32<4><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
     else if (f32) k_ensemble_sf32<1><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
-    else if (four) k_ensemble_sf64<4><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
-    else k_ensemble_sf64<1><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
+    else if (four && c->qz_real) k_ensemble_sf64<4, false><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
+    else if (four) k_ensemble_sf64<4, true><<<grid, 256, 0, c->stream>>>(g, 4, ffh, dG, ng, per_y, sfb, part);
+    else if (c->qz_real) k_ensemble_sf64<1, false><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
+    else k_ensemble_sf64<1, true><<<grid, 256, 0, c->stream>>>(g, 1, ffh, dG, ng, per_y, sfb, part);
     HGX_CHECK_LAUNCH(c);
   } else if (sep) {
     const int sh = m->elem[0].shape;
@@ -911,7 +942,8 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
     LaunchTimer lt(c, true);
 #define HGX_ENS_SEP_NB(S, B)                                                                                    \
   if (f32) k_ensemble_sep32<S, B><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part); \
-  else k_ensemble_sep64<S, B><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part)
+  else if (c->qz_real) k_ensemble_sep64<S, B, false><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part); \
+  else k_ensemble_sep64<S, B, true><<<grid, 256, 0, c->stream>>>(g, B, ffh, dE, dloc, dM, dG, ng, per_y, sfb, part)
 #define HGX_ENS_SEP(S)                  \
   if (c->nbranch == 4) {                \
     HGX_ENS_SEP_NB(S, 4);               \
