@@ -234,9 +234,19 @@ __device__ inline bool fp32_conditioned(const double m[3], const Nodes& P) {
   return true;
 }
 
+// the plain sphere (kShapePlainSphere) is the sphere's FP32 form, always conditioned
+template <>
+__device__ inline cf ff32<kShapePlainSphere>(const double m[3], const Nodes& P) {
+  return ff32<HGX_SHAPE_SPHERE>(m, P);
+}
+
 // FP32C form factor at a real rotated q-point, returned in FP64.
 template <int SHAPE>
 __device__ inline cd ff_fp32c(const double m[3], const Nodes& P) {
+  if constexpr (SHAPE == kShapePlainSphere) {
+    const cf F = ff32<SHAPE>(m, P);
+    return cd{(double)F.r, (double)F.i};
+  }
   if (!fp32_conditioned<SHAPE>(m, P)) {
     const cd mq[3] = {cd{m[0], 0.0}, cd{m[1], 0.0}, cd{m[2], 0.0}};
     return shape_ff<SHAPE>(mq, P);

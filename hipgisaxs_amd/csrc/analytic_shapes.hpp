@@ -133,6 +133,11 @@ struct ShapeFF<HGX_SHAPE_SPHERE> {
         return acc;
       }
     }
+    return plain(mq, P);
+  }
+  // the full sphere (any flags ignored)
+  template <class C, class R>
+  __device__ static C plain(const C mq[3], const NodesT<R>& P) {
     C qval = csqrt_d(mq[0] * mq[0] + mq[1] * mq[1] + mq[2] * mq[2]);
     C acc = cconst<C>(0.0);
     if (cabs_d(qval) < 2.2250738585072014e-308) return acc;
@@ -144,6 +149,19 @@ struct ShapeFF<HGX_SHAPE_SPHERE> {
       acc = acc + (P.wt[0][i] * (4 * kPi * Rr * Rr * Rr)) * (c0 * c1);
     }
     return acc;
+  }
+};
+
+// An internal shape id for spheres the host knows are not truncated: the same formula without the
+// cap's code path (its FP64 Bessel function over 512 slices), so that kernels instantiated for it keep
+// their registers and need no scratch.
+constexpr int kShapePlainSphere = 8;
+
+template <>
+struct ShapeFF<kShapePlainSphere> {
+  template <class C, class R>
+  __device__ static C eval(const C mq[3], const NodesT<R>& P) {
+    return ShapeFF<HGX_SHAPE_SPHERE>::plain(mq, P);
   }
 };
 

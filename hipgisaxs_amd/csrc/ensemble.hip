@@ -821,6 +821,11 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   int common = E[0].shape;
   for (int e = 1; e < ne; ++e)
     if (E[e].shape != common) common = -1;
+  // spheres without a cap run the lean instantiation (no truncated-sphere code)
+  bool capped = false;
+  for (int e = 0; e < ne; ++e)
+    capped = capped || (E[e].shape == HGX_SHAPE_SPHERE && (E[e].P.flags & HGX_SHAPE_TRUNCATED_SPHERE) && E[e].P.n[1] > 0);
+  if (common == HGX_SHAPE_SPHERE && !capped) common = kShapePlainSphere;
   if (precision == HGX_PREC_FP32C && !c->qz_real)
     return set_error(c, HGX_ERR_UNSUPPORTED, "ensemble_intensity: FP32C needs real q");
   // M[g][e] = R_g R_e with the summation order of the reference's RotMatrix_t product
@@ -911,6 +916,7 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   if (f32) k_ensemble_cell32<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh); \
   else k_ensemble_cell64<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh)
       switch (common) {
+        case kShapePlainSphere: HGX_ENS_CELL(kShapePlainSphere); break;
         case HGX_SHAPE_SPHERE: HGX_ENS_CELL(HGX_SHAPE_SPHERE); break;
         case HGX_SHAPE_CYLINDER: HGX_ENS_CELL(HGX_SHAPE_CYLINDER); break;
         default: HGX_ENS_CELL(-1); break;
@@ -964,6 +970,7 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   if (f32) k_ensemble32<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part); \
   else k_ensemble64<S><<<grid, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, dG, ng, per_y, sfb, part)
     switch (common) {
+      case kShapePlainSphere: HGX_ENS_LAUNCH(kShapePlainSphere); break;
       case HGX_SHAPE_SPHERE: HGX_ENS_LAUNCH(HGX_SHAPE_SPHERE); break;
       case HGX_SHAPE_CYLINDER: HGX_ENS_LAUNCH(HGX_SHAPE_CYLINDER); break;
       case HGX_SHAPE_BOX: HGX_ENS_LAUNCH(HGX_SHAPE_BOX); break;

@@ -89,13 +89,18 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   Vec3 tv = make_vec(transvec);
   int has_tv = transvec != nullptr;
   unsigned nb = ceil_div(c->nq, 256);
+  // spheres without a cap run the lean instantiation (no truncated-sphere code)
+  const bool plain_sphere = shape == HGX_SHAPE_SPHERE && !((params->flags & HGX_SHAPE_TRUNCATED_SPHERE) && P.n[1] > 0);
   if (precision == HGX_PREC_FP32C) {
     if (!c->qz_real)
       return set_error(c, HGX_ERR_UNSUPPORTED, "ff_analytic: FP32C needs real q (a particle in a non-absorbing layer)");
     LaunchTimer lt(c, true);
 #define HGX_A32(S) k_ff_analytic32<S><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff)
     switch (shape) {
-      case HGX_SHAPE_SPHERE: HGX_A32(HGX_SHAPE_SPHERE); break;
+      case HGX_SHAPE_SPHERE:
+        if (plain_sphere) HGX_A32(kShapePlainSphere);
+        else HGX_A32(HGX_SHAPE_SPHERE);
+        break;
       case HGX_SHAPE_CYLINDER: HGX_A32(HGX_SHAPE_CYLINDER); break;
       case HGX_SHAPE_BOX: HGX_A32(HGX_SHAPE_BOX); break;
       case HGX_SHAPE_CUBE: HGX_A32(HGX_SHAPE_CUBE); break;
@@ -109,7 +114,10 @@ extern "C" int hgx_ff_analytic(hgx_ctx* h, int shape, const hgx_shape_params* pa
   }
   LaunchTimer lt(c, true);
   switch (shape) {
-    case HGX_SHAPE_SPHERE: k_ff_analytic<HGX_SHAPE_SPHERE><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff); break;
+    case HGX_SHAPE_SPHERE:
+      if (plain_sphere) k_ff_analytic<kShapePlainSphere><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff);
+      else k_ff_analytic<HGX_SHAPE_SPHERE><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff);
+      break;
     case HGX_SHAPE_CYLINDER: k_ff_analytic<HGX_SHAPE_CYLINDER><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff); break;
     case HGX_SHAPE_BOX: k_ff_analytic<HGX_SHAPE_BOX><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff); break;
     case HGX_SHAPE_CUBE: k_ff_analytic<HGX_SHAPE_CUBE><<<nb, 256, 0, c->stream>>>(g, R, P, tv, has_tv, ff); break;
