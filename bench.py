@@ -376,6 +376,7 @@ def bench_ewald(nshapes=64, reps=2):
     t_ew, k_ew = sched["default"]
     t_pp, k_pp = run("pair", N.FLAG_NO_MFMA, "fp32c", reps)
     t_64, k_64 = run("fp64", 0, "fp64", 1)
+    adj = bench_adjoint_any_grid(ctxs, shapes[::4])
     # the same 64 calls as one batch (meshes in one context, each item on its shape's grid), replayed
     # from its hipGraph: one launch per series order plus the finishing launch
     from hipgisaxs_amd.api import Batch
@@ -422,10 +423,43 @@ def bench_ewald(nshapes=64, reps=2):
             "fp64_per_pair": {"value": pairs / t_64, "ms": 1e3 * t_64, "kernel_ms": k_64},
             "speedup_vs_fp32c_per_pair": t_pp / t_ew, "speedup_vs_fp64_per_pair": t_64 / t_ew,
             "relL2_intensity_vs_fp64": math.sqrt(num / den), "relL2_worst_shape": worst,
-            "pixel_error_vs_fp64": pix.result(),
+            "pixel_error_vs_fp64": pix.result(), "record_adjoint": adj,
             "workload": f"{nshapes} convex polyhedra of the batched workload, each on QGrid::create's q-space "
                         "grid of its side (128-1024, qy [-2, 2], qz [0, 2], 10 keV, 0.2 deg) with 4 DWBA branches, "
                         "FP32C approximate method, one call per shape"}
+
+
+def bench_adjoint_any_grid(ctxs, shapes):
+    """Record adjoint of L = Re sum_q W(q) F(q) (hgx_ff_numeric_adjoint) for the given shapes on their
+    q-space grids with seeded random W: FP32C (the VALU kernel on these non-product grids) against FP64,
+    rate in (q-point, triangle) pairs per second and the gradients' relL2."""
+    import torch
+
+    gen = torch.Generator(device="cuda:0").manual_seed(7)
+    W = {side: torch.randn(2 * c.info()["nq"], dtype=torch.float64, device="cuda:0", generator=gen)
+         for side, c in ctxs.items()}
+    pairs = sum(float(s["ntri"]) * ctxs[s["side"]].info()["nq"] for s in shapes)
+
+    def run(prec):
+        for c in ctxs.values():
+            c.profile_begin()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        grads = [ctxs[s["side"]].ff_numeric_adjoint(s["mid"], W[s["side"]], prec) for s in shapes]
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        kms = sum(c.profile_end()["main_ms"] for c in ctxs.values())
+        return t, kms, grads
+
+    run("fp32c")  # warm-up (first launches, workspace)
+    t32, k32, g32 = run("fp32c")
+    t64, k64, g64 = run("fp64")
+    num = sum(float(torch.linalg.vector_norm(a - b)) ** 2 for a, b in zip(g32, g64))
+    den = sum(float(torch.linalg.vector_norm(b)) ** 2 for b in g64)
+    return {"shapes": len(shapes), "pairs": pairs,
+            "fp32c": {"value": pairs / t32, "ms": 1e3 * t32, "kernel_ms": k32, "kernel": "k_adjoint32"},
+            "fp64": {"value": pairs / t64, "ms": 1e3 * t64, "kernel_ms": k64, "kernel": "k_adjoint64"},
+            "speedup_fp32c_vs_fp64": t64 / t32, "relL2_fp32c_vs_fp64": math.sqrt(num / den)}
 
 
 # Algorithmic work of the fused ensemble kernels as implemented (add/mul 1 flop, FMA 2; sin/cos/exp

@@ -16,6 +16,9 @@
 // Plain FP64 (any grid): one thread per triangle, q-points in chunks (wave-uniform q data through
 // scalar loads), FP64 partials per chunk reduced in order.
 //
+// FP32C on any other grid: the same loop in FP32 on block-relative records, the block's phase folded
+// into the weights staged in LDS (k_adjoint32).
+//
 // FP32C on product grids q = (qx0, qy_y, qz_z): the sums are bilinear forms
 //     B[f, g](t) = sum_{z, y} f(z) g(y) Wt(z, y) Ez_t(z) Ey_t(y)
 // with f in {1, qz, qz^2} and g in {1, qy, qy^2}; V_t and M_t are combinations of the six forms
@@ -158,6 +161,116 @@ __global__ __launch_bounds__(256) void k_adjoint64_finish(const double* __restri
 __global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// ----- FP32C, any grid (VALU) ----------------------------------------------------------------------
+// One workgroup = one 128-triangle block and one chunk of q-points; thread i takes triangle i of the
+// block, whose FP32 record holds A' and c' - o_b. Per stage of 128 q-points each thread prepares one
+// q-point in LDS: its FP32 components, in turns and as they are, and its weight folded with the block's
+// phase, omega(q) = Wt(q) exp(i q . o_b) (the FP64 phase reduced to turns before the hardware sin/cos,
+// times exp(-Im(q_z) o_bz)). The inner loop is then FP32 only: exp(i q . (c' - o_b)) from the
+// block-relative phase in turns, and every thread reads the same q-point (an LDS broadcast). Sums run
+// over one stage in FP32 and are added to FP64 accumulators; the chunk partials are reduced in order by
+// k_adjoint64_finish.
+constexpr int kAdjQStage = kBlockTri;
+
+template <bool CQ>
+__global__ __launch_bounds__(kBlockTri) void k_adjoint32(QView g, const float* __restrict__ wt,
+                                                         const float* __restrict__ rec,
+                                                         const double* __restrict__ borig, int64_t ntri,
+                                                         int64_t qchunk, double* __restrict__ part) {
+  __shared__ float4 sq[2 * kAdjQStage];
+  __shared__ float2 sqi[CQ ? kAdjQStage : 1];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t t = (int64_t)b * kBlockTri + tid;  // records are padded to whole blocks
+  const int chunk = blockIdx.y;
+  const int64_t q0 = (int64_t)chunk * qchunk, q1 = min(g.nq, q0 + qchunk);
+  const float* r = rec + 8 * t;
+  const float ax = r[0], ay = r[1], az = r[2], cx = r[3], cy = r[4], cz = r[5];
+  const double ox = borig[4 * b], oy = borig[4 * b + 1], oz = borig[4 * b + 2];
+  double gA[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
+  for (int64_t s0 = q0; s0 < q1; s0 += kAdjQStage) {
+    // this thread's q-point of the stage; past the chunk's end a zero weight and q = 0 add nothing
+    const int64_t q = s0 + tid;
+    float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v = u;
+    float2 vi = make_float2(0.0f, 0.0f);
+    if (q < q1) {
+      const int64_t p = q % g.npix;
+      const double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = CQ ? g.qz[2 * q + 1] : 0.0;
+      const double tb = (qx * ox + qy * oy + zr * oz) * kInvTwoPi;
+      const float fr = (float)(tb - rint(tb));
+      double cb = (double)__builtin_amdgcn_cosf(fr), sb = (double)__builtin_amdgcn_sinf(fr);
+      if (CQ) {
+        const double m = exp(-zi * oz);
+        cb *= m;
+        sb *= m;
+      }
+      const double wr = wt[2 * q], wi = wt[2 * q + 1];
+      u = make_float4((float)(qx * kInvTwoPi), (float)(qy * kInvTwoPi), (float)(zr * kInvTwoPi),
+                      (float)(wr * cb - wi * sb));
+      v = make_float4((float)qx, (float)qy, (float)zr, (float)(wr * sb + wi * cb));
+      if (CQ) vi = make_float2((float)(zi * 1.4426950408889634), (float)zi);
+    }
+    __syncthreads();  // every thread has read the previous stage
+    sq[2 * tid] = u;
+    sq[2 * tid + 1] = v;
+    if (CQ) sqi[tid] = vi;
+    __syncthreads();
+    float fA0 = 0.0f, fA1 = 0.0f, fA2 = 0.0f, fc0 = 0.0f, fc1 = 0.0f, fc2 = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < kAdjQStage; ++k) {
+      const float4 a = sq[2 * k], e = sq[2 * k + 1];
+      const float ph = a.x * cx + a.y * cy + a.z * cz;
+      const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
+      // wa = omega exp(i q . (c' - o_b)); qa = q . A'; wc = i wa qa
+      float war = a.w * cs - e.w * sn, wai = a.w * sn + e.w * cs;
+      const float qar = e.x * ax + e.y * ay + e.z * az;
+      if (!CQ) {
+        const float wcr = -wai * qar;
+        fA0 += war * e.x;
+        fA1 += war * e.y;
+        fA2 += war * e.z;
+        fc0 += wcr * e.x;
+        fc1 += wcr * e.y;
+        fc2 += wcr * e.z;
+      } else {
+        const float2 ei = sqi[k];
+        const float att = __builtin_amdgcn_exp2f(-(ei.x * cz));
+        war *= att;
+        wai *= att;
+        const float qai = ei.y * az;
+        const float wcr = -(war * qai + wai * qar), wci = war * qar - wai * qai;
+        fA0 += war * e.x;
+        fA1 += war * e.y;
+        fA2 += war * e.z - wai * ei.y;
+        fc0 += wcr * e.x;
+        fc1 += wcr * e.y;
+        fc2 += wcr * e.z - wci * ei.y;
+      }
+    }
+    gA[0] += fA0;
+    gA[1] += fA1;
+    gA[2] += fA2;
+    gc[0] += fc0;
+    gc[1] += fc1;
+    gc[2] += fc2;
+  }
+  if (t >= ntri) return;
+  double* o = part + 6 * ((int64_t)chunk * ntri + t);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o[k] = gA[k];
+    o[3 + k] = gc[k];
+  }
+}
+
+// q-chunks of the FP32C any-grid kernel: enough workgroups to fill the chip (about 8192), whole stages
+// per chunk, at most one chunk per stage
+static void adjoint32_chunks(int64_t nq, int64_t nblocks, int64_t* qchunk, int* nqc) {
+  const int64_t stages = ceil_div(nq, kAdjQStage);
+  const int64_t want = std::max<int64_t>(1, std::min<int64_t>(stages, ceil_div(8192, std::max<int64_t>(nblocks, 1))));
+  *qchunk = ceil_div(stages, want) * kAdjQStage;
+  *nqc = (int)ceil_div(nq, *qchunk);
 }
 
 // ----- FP32C on product grids (matrix cores) -------------------------------------------------------
@@ -730,9 +843,9 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   if (!W || !grad) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: null weights or output");
   if (precision != HGX_PREC_FP64 && precision != HGX_PREC_FP32C)
     return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric_adjoint: bad precision %d", precision);
-  if (precision == HGX_PREC_FP32C && !c->product)
-    return set_error(c, HGX_ERR_UNSUPPORTED, "ff_numeric_adjoint: FP32C needs a product q-grid (use FP64)");
   const Mesh& m = c->meshes[mesh_id];
+  // FP32C on a grid that is not a product grid (the q-space and detector grids): the VALU kernel
+  const bool any32 = precision == HGX_PREC_FP32C && !c->product;
   if (int rc = set_device(c)) return rc;
   QView g{c->d_qx, c->d_qy, c->d_qz, c->npix, c->nq};
   Rot R = make_rot(rot);
@@ -748,10 +861,14 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
                            ? (size_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double)) : 0;
   const size_t w0_b = 256;
   const size_t wt_b = (size_t)c->nq * 2 * (precision == HGX_PREC_FP64 ? sizeof(double) : sizeof(float));
+  int64_t qchunk32 = 0;
+  int nqc32 = 0;
+  if (any32) adjoint32_chunks(c->nq, m.nblocks, &qchunk32, &nqc32);
   const size_t part_b = precision == HGX_PREC_FP64 ? (size_t)nqc * m.ntri * 6 * sizeof(double)
+                        : any32                    ? (size_t)nqc32 * m.ntri * 6 * sizeof(double)
                                                    : (size_t)kAdjForms * nyt * m.ntri * 2 * sizeof(double);
   // split-fp16 weights (FP32C with the fp16 matrix cores): column maxima, 12 B per q-point, column scales
-  const int variant = precision == HGX_PREC_FP32C ? adjoint_product_variant(c) : 0;
+  const int variant = precision == HGX_PREC_FP32C && !any32 ? adjoint_product_variant(c) : 0;
   const bool f16 = variant > 0;
   const int nz = c->nbranch * c->nrow;
   const int nzo = (int)ceil_div(nz, 8);
@@ -809,6 +926,20 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
     if (int rc = product_rotate_mesh(c, m, R, rr, ro, c->stream)) return rc;
     it.rec32 = rr;
     it.borig = ro;
+  }
+  if (any32) {
+    dim3 grid((unsigned)m.nblocks, (unsigned)nqc32);
+    {
+      LaunchTimer lt(c, true);
+      if (cq) k_adjoint32<true><<<grid, kBlockTri, 0, c->stream>>>(g, static_cast<const float*>(wt), it.rec32, it.borig,
+                                                                  m.ntri, qchunk32, part);
+      else k_adjoint32<false><<<grid, kBlockTri, 0, c->stream>>>(g, static_cast<const float*>(wt), it.rec32, it.borig,
+                                                                 m.ntri, qchunk32, part);
+    }
+    HGX_CHECK_LAUNCH(c);
+    k_adjoint64_finish<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(part, nqc32, m.ntri, rrec, R, w0, m.d_src, grad);
+    HGX_CHECK_LAUNCH(c);
+    return HGX_OK;
   }
   it.part = part;
   it.w16 = w16;

@@ -223,9 +223,9 @@ int hgx_ff_numeric_noalloc(hgx_ctx* ctx, int mesh_id, int method, int precision,
  * hgx_ff_numeric(HGX_FF_APPROX) for the same mesh and rotation and W complex [2*nq] on the device,
  * writes grad[6 * ntri_uploaded] (device) = (dL/d(s n)_t, dL/dc_t) per triangle in the order the mesh
  * was uploaded (zero for triangles dropped as degenerate). Gradients of any mesh parameterisation
- * follow by the chain rule through (s n_t, c_t). HGX_PREC_FP64 works on any grid; HGX_PREC_FP32C needs
- * a product grid (matrix cores). No reference counterpart: the reference differentiates by finite
- * differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). */
+ * follow by the chain rule through (s n_t, c_t). Both precisions work on any grid; HGX_PREC_FP32C runs
+ * the matrix cores on product grids and FP32 vector arithmetic elsewhere. No reference counterpart: the
+ * reference differentiates by finite differences (src/analyzer/hipgisaxs_fit_lmvm.cpp:190-224). */
 int hgx_ff_numeric_adjoint(hgx_ctx* ctx, int mesh_id, int precision, int flags, const double rot[9],
                            const double* W, double* grad);
 

@@ -1,7 +1,8 @@
 """Record adjoint of the approximate numeric form factor (hgx_ff_numeric_adjoint) against the oracle's
-record adjoint (itself pinned by finite differences in test_oracle_adjoint.py): plain FP64 on a
-non-product (detector) grid and FP32C on product grids (matrix cores), with rotations, complex DWBA
-branches, a q = 0 point, a degenerate triangle and the caller's triangle order."""
+record adjoint (itself pinned by finite differences in test_oracle_adjoint.py): plain FP64 and FP32C
+(VALU kernel) on the reference's non-product grids (detector, QGrid::create q-space), FP32C on product
+grids (matrix cores), with rotations, complex DWBA branches, a q = 0 point, a degenerate triangle and
+the caller's triangle order."""
 import numpy as np
 import pytest
 
@@ -62,8 +63,57 @@ def test_adjoint_fp64_detector_grid(oracle_mod):
         e = rel_l2(got, ref)
         print(f"fp64 adjoint relL2 {e:.2e}")
         assert e <= 1e-10
-        with pytest.raises(Exception):
-            c.ff_numeric_adjoint(mid, Wd, "fp32c", rot=R)  # not a product grid
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("grid,dwba,rotated,ntri", [
+    ("detector", 3e-6 + 2e-8j, True, 500),   # complex DWBA branches
+    ("qspace", 0j, False, 1500),             # real branches, a partial last triangle block
+    ("qspace", None, True, 3000),            # Born, one branch
+])
+def test_adjoint_fp32c_any_grid(oracle_mod, grid, dwba, rotated, ntri):
+    import torch
+
+    from hipgisaxs_amd.api import Context
+
+    O = oracle_mod
+    rng = np.random.default_rng(ntri)
+    tri = _mesh(rng, ntri)
+    R = _rot(0.4, -0.3) if rotated else None
+    c = Context(0)
+    try:
+        if grid == "detector":
+            args = (40, 30, 0.172, 2000.0, 19.5, 0.0, AI, K0)
+            c.qgrid_detector(*args)
+            g = O.qgrid_detector(*args)
+        else:
+            # an even column count: no column at q_y = 0, so no q-point within rounding of q = 0, where
+            # 1/|q|^2 is ill-posed in any two evaluations of the grid
+            args = (56, 41, -2.0, 0.0, 2.0, 2.0, AI, K0)
+            c.qgrid_qspace(*args)
+            g = O.qgrid_qspace(*args)
+        if dwba is not None:
+            c.qz_extended(K0, AI, dwba)
+            g = O.qz_extended(g, K0, AI, dwba)
+        assert not c.is_product()
+        mid = c.upload_mesh(tri, "exact9")
+        W = rng.normal(size=g.nq) + 1j * rng.normal(size=g.nq)
+        Wd = torch.from_numpy(np.stack([W.real, W.imag], -1).ravel()).cuda()
+        ref = _oracle_grad(O, tri, g, W, R)
+        got64 = c.ff_numeric_adjoint(mid, Wd, "fp64", rot=R).cpu().numpy()
+        got32 = c.ff_numeric_adjoint(mid, Wd, "fp32c", rot=R).cpu().numpy()
+        e64, e32 = rel_l2(got64, ref), rel_l2(got32, ref)
+        # per triangle: the FP32C row against the FP64 row, relative to the largest row of its half
+        # (dL/dA, dL/dc), so that no triangle's gradient is lost under the image-wide norm
+        d = np.abs(got32 - got64)
+        row = max(np.abs(got64[:, :3]).max(), 1e-300), max(np.abs(got64[:, 3:]).max(), 1e-300)
+        worst = max(d[:, :3].max() / row[0], d[:, 3:].max() / row[1])
+        print(f"{grid} dwba={dwba} rot={rotated} ntri={ntri}: fp64 {e64:.2e} fp32c {e32:.2e} worst {worst:.2e}")
+        assert np.all(got32[5] == 0.0)
+        assert e64 <= 1e-10
+        assert e32 <= 1e-5
+        assert worst <= 1e-5
     finally:
         c.close()
 
