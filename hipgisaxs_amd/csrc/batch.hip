@@ -27,6 +27,7 @@ struct BatchItemDev {
   int nchunks;
   double* part;
   double* out;
+  float* part32;  // matrix-core items: FP32 partial sums instead of part
 };
 
 struct BatchUnit {
@@ -95,16 +96,9 @@ __global__ __launch_bounds__(kWG) void k_batch_finish(const BatchItemDev* __rest
   if (q >= it.g.nq) return;
   cd mq[3];
   load_q(it.g, it.R, q, mq);
-  double pr = 0.0, pi = 0.0;
-  if (it.nchunks == 1) {
-    pr = it.out[2 * q];
-    pi = it.out[2 * q + 1];
-  }
-  for (int c = 0; it.nchunks > 1 && c < it.nchunks; ++c) {
-    pr += it.part[2 * ((int64_t)c * it.g.nq + q)];
-    pi += it.part[2 * ((int64_t)c * it.g.nq + q) + 1];
-  }
-  cd F = finish_q(it.e, mq, cd{pr, pi});
+  const cd raw = it.nchunks == 1 ? cd{it.out[2 * q], it.out[2 * q + 1]}
+                                  : sum_chunks(it.nchunks, it.part, it.part32, it.g.nq, q);
+  cd F = finish_q(it.e, mq, raw);
   it.out[2 * q] = F.r;
   it.out[2 * q + 1] = F.i;
 }
@@ -204,6 +198,7 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   std::vector<char> is_prod(n, 0);
   std::vector<int> eord(n, -1);  // series orders of Ewald-grid items, -1 for the others
   std::vector<int64_t> part_off(n, -1), rot_off(n, -1);
+  std::vector<char> f32part(n, 0);
   int64_t part_total = 0, rot_total = 0;  // doubles, bytes
   for (int i = 0; i < n; ++i) {
     const hgx_batch_item& x = items[i];
@@ -288,9 +283,13 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
     eit[i].nchunks = d.nchunks;
     d.out = x.ff;
     d.part = nullptr;
+    d.part32 = nullptr;
+    // FP32C matrix-core items keep FP32 partial sums (half the doubles of space)
+    f32part[i] = precision == HGX_PREC_FP32C && (prod || eord[i] >= 0);
     if (d.nchunks > 1) {
       part_off[i] = part_total;
-      part_total += (int64_t)d.nchunks * gc->nq * 2;
+      part_total += (int64_t)d.nchunks * gc->nq * (f32part[i] ? 1 : 2);
+      part_total = (part_total + 31) & ~(int64_t)31;  // every item's partials 256-byte aligned
     }
     b.cq = b.cq || !gc->qz_real;
     b.pairs += (double)gc->nq * (double)m.ntri;
@@ -380,7 +379,10 @@ extern "C" int hgx_batch_create(hgx_ctx* h, int n, const hgx_batch_item* items, 
   };
   if (part_total > 0 && hipMalloc(&b.d_part, part_total * sizeof(double)) != hipSuccess) return fail("partials");
   for (int i = 0; i < n; ++i)
-    if (part_off[i] >= 0) it[i].part = pit[i].part = eit[i].part = b.d_part + part_off[i];
+    if (part_off[i] >= 0 && f32part[i])
+      it[i].part32 = pit[i].part32 = eit[i].part32 = reinterpret_cast<float*>(b.d_part + part_off[i]);
+    else if (part_off[i] >= 0)
+      it[i].part = pit[i].part = eit[i].part = b.d_part + part_off[i];
   for (int i = 0; i < n; ++i) pit[i].out = eit[i].out = it[i].out;
   if (rot_total > 0) {
     if (hipMalloc(&b.d_prot, rot_total) != hipSuccess) return fail("rotated records");

@@ -345,9 +345,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 
       sincos((cs - cbar) * vo, &sn, &cn);
       const double re = (double)CR[m][k] * unscale, im = (double)CI[m][k] * unscale;
       const int64_t q = (int64_t)zs * ncol + yj;
-      double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)chunk * it.nq + q);
-      dst[0] = re * cn - im * sn;
-      dst[1] = re * sn + im * cn;
+      store_raw(it.nchunks, it.out, it.part, it.part32, (int64_t)chunk * it.nq + q, q, re * cn - im * sn,
+                re * sn + im * cn);
     }
 }
 

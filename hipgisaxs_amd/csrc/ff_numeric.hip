@@ -79,21 +79,13 @@ __global__ __launch_bounds__(kWG) void k_ff_approx32(QView g, Rot R, const float
 // Reduces partial sums over chunks in chunk order (or takes the raw sum the hot kernel left in
 // `out` when there is one chunk) and applies the per-q epilogue, including the small-|q| series.
 __global__ __launch_bounds__(kWG) void k_ff_finish(QView g, Rot R, int nchunks, const double* __restrict__ part,
-                                                   double* __restrict__ out, Epi e) {
+                                                   const float* __restrict__ part32, double* __restrict__ out, Epi e) {
   int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
   if (q >= g.nq) return;
   cd mq[3];
   load_q(g, R, q, mq);
-  double pr = 0.0, pi = 0.0;
-  if (nchunks == 1) {
-    pr = out[2 * q];
-    pi = out[2 * q + 1];
-  }
-  for (int c = 0; nchunks > 1 && c < nchunks; ++c) {
-    pr += part[2 * ((int64_t)c * g.nq + q)];
-    pi += part[2 * ((int64_t)c * g.nq + q) + 1];
-  }
-  cd F = finish_q(e, mq, cd{pr, pi});
+  const cd raw = nchunks == 1 ? cd{out[2 * q], out[2 * q + 1]} : sum_chunks(nchunks, part, part32, g.nq, q);
+  cd F = finish_q(e, mq, raw);
   out[2 * q] = F.r;
   out[2 * q + 1] = F.i;
 }
@@ -516,10 +508,10 @@ int take_work(Context* c, const Work& w, size_t bytes, char** out) {
 
 // Fewest chunk count reductions: with a fixed buffer, at most as many chunks of nq complex partial
 // sums as fit beside `reserved` bytes (one chunk needs none: it writes the output directly).
-int64_t chunk_limit(const Work& w, int64_t nq, size_t reserved) {
+int64_t chunk_limit(const Work& w, int64_t nq, size_t reserved, size_t value_bytes = sizeof(double)) {
   if (!w.fixed) return INT64_MAX;
   if (reserved >= w.cap) return 1;
-  const int64_t n = (int64_t)((w.cap - reserved) / ((size_t)nq * 2 * sizeof(double)));
+  const int64_t n = (int64_t)((w.cap - reserved) / ((size_t)nq * 2 * value_bytes));
   return n >= 2 ? n : 1;
 }
 
@@ -600,16 +592,19 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
                                 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
     {
       // a fixed buffer may hold fewer chunks: longer chunks then (their FP32 sums span more blocks)
-      const int64_t lim = chunk_limit(work, c->nq, rec_bytes);
+      const int64_t lim = chunk_limit(work, c->nq, rec_bytes, f64 ? sizeof(double) : sizeof(float));
       if ((m.nblocks + it.bchunk - 1) / it.bchunk > lim) it.bchunk = (m.nblocks + lim - 1) / lim;
     }
     it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
     if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
-    size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
+    // partial sums of the chunks: FP64 for the FP64 kernel, FP32 for the FP32C ones (their sums are FP32)
+    const size_t pbytes = f64 ? sizeof(double) : sizeof(float);
+    size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * pbytes : 0;
     char* w = nullptr;
     if (part_bytes + rec_bytes > 0)
       if (int rc = take_work(c, work, part_bytes + rec_bytes, &w)) return rc;
-    it.part = it.nchunks > 1 ? reinterpret_cast<double*>(w) : nullptr;
+    it.part = it.nchunks > 1 && f64 ? reinterpret_cast<double*>(w) : nullptr;
+    it.part32 = it.nchunks > 1 && !f64 ? reinterpret_cast<float*>(w) : nullptr;
     it.out = ff;
     if (identity) {
       it.rec = m.d_rec32;
@@ -639,7 +634,7 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     HGX_CHECK_LAUNCH(c);
     {
       OptTimer lt(c, false, timed);
-      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, it.part32, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
     return HGX_OK;
@@ -661,16 +656,18 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     const size_t rec_bytes =
         identity ? 0 : (size_t)m.nblocks * kBlockTri * 8 * sizeof(float) + (size_t)m.nblocks * 4 * sizeof(double);
     {
-      const int64_t lim = chunk_limit(work, c->nq, rec_bytes);
+      const int64_t lim = chunk_limit(work, c->nq, rec_bytes, sizeof(float));
       if ((m.nblocks + it.bchunk - 1) / it.bchunk > lim) it.bchunk = (m.nblocks + lim - 1) / lim;
     }
     it.nchunks = (int)((m.nblocks + it.bchunk - 1) / it.bchunk);
     if ((double)tiles * it.nchunks > 2.0e9) return set_error(c, HGX_ERR_INVALID_ARG, "ff_numeric: grid too large");
-    const size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(double) : 0;
+    // FP32 partial sums of the chunks (the kernel's sums are FP32)
+    const size_t part_bytes = it.nchunks > 1 ? (size_t)it.nchunks * c->nq * 2 * sizeof(float) : 0;
     char* w = nullptr;
     if (part_bytes + rec_bytes > 0)
       if (int rc = take_work(c, work, part_bytes + rec_bytes, &w)) return rc;
-    it.part = it.nchunks > 1 ? reinterpret_cast<double*>(w) : nullptr;
+    it.part = nullptr;
+    it.part32 = it.nchunks > 1 ? reinterpret_cast<float*>(w) : nullptr;
     it.out = ff;
     if (identity) {
       it.rec = m.d_rec32;
@@ -691,7 +688,7 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     }
     {
       OptTimer lt(c, false, timed);
-      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, ff, e);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, it.nchunks, it.part, it.part32, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
     return HGX_OK;
@@ -731,7 +728,7 @@ int ff_numeric_run(Context* c, int mesh_id, int method, int precision, int flags
     HGX_CHECK_LAUNCH(c);
     {
       OptTimer lt(c, false, timed);
-      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, ff, e);
+      k_ff_finish<<<ceil_div(c->nq, kWG), kWG, 0, c->stream>>>(g, R, p.nchunks, part, nullptr, ff, e);
       HGX_CHECK_LAUNCH(c);
     }
     return HGX_OK;

@@ -127,6 +127,7 @@ struct ProductItem {
   int pad;
   double* part;         // [nchunks][nq] complex partial sums (nchunks > 1)
   double* out;          // [nq] complex raw sums (nchunks == 1)
+  float* part32 = nullptr;  // FP32C kernels: FP32 partial sums here instead of part (exact: FP32 sums times 2^k)
 };
 
 // A work unit: one 32 (z) x 32 (y) tile of one item over one chunk of triangle blocks.
@@ -254,6 +255,7 @@ struct EwaldItem {
   float rmax;           // bound on |c_t| (complex qz attenuation)
   double* part;         // [nchunks][nq] complex partial sums (nchunks > 1)
   double* out;          // [nq] complex raw sums (nchunks == 1)
+  float* part32 = nullptr;  // FP32C kernels: FP32 partial sums here instead of part (exact: FP32 sums times 2^k)
 };
 
 // Launch-shape settings of the numeric kernels, per context (hgx_tune_*; the autotuner sets them).

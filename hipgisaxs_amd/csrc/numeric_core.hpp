@@ -7,6 +7,69 @@
 
 namespace hgx {
 
+// Raw sum of one q-point over one chunk of triangles: in place (FP64) when the item has one chunk, else
+// into the chunk's slot of the partial sums, FP32 (part32: the matrix-core kernels' FP32 sums, half the
+// traffic) or FP64 (part).
+__device__ inline void store_raw(int nchunks, double* out, double* part, float* part32, int64_t slot, int64_t q,
+                                 double re, double im) {
+  if (nchunks == 1) {
+    out[2 * q] = re;
+    out[2 * q + 1] = im;
+  } else if (part32) {
+    *reinterpret_cast<float2*>(part32 + 2 * slot) = make_float2((float)re, (float)im);
+  } else {
+    part[2 * slot] = re;
+    part[2 * slot + 1] = im;
+  }
+}
+
+// Sum over chunks of one q-point's partial sums, in chunk order, in FP64. Four chunks' loads are issued
+// before their additions (one thread per q-point has little else to hide memory latency with).
+__device__ inline cd sum_chunks(int nchunks, const double* part, const float* part32, int64_t nq, int64_t q) {
+  double pr = 0.0, pi = 0.0;
+  int c = 0;
+  if (part32) {
+    const float2* p = reinterpret_cast<const float2*>(part32) + q;
+    for (; c + 4 <= nchunks; c += 4) {
+      const float2 v0 = p[(int64_t)c * nq], v1 = p[(int64_t)(c + 1) * nq];
+      const float2 v2 = p[(int64_t)(c + 2) * nq], v3 = p[(int64_t)(c + 3) * nq];
+      pr += (double)v0.x;
+      pi += (double)v0.y;
+      pr += (double)v1.x;
+      pi += (double)v1.y;
+      pr += (double)v2.x;
+      pi += (double)v2.y;
+      pr += (double)v3.x;
+      pi += (double)v3.y;
+    }
+    for (; c < nchunks; ++c) {
+      const float2 v = p[(int64_t)c * nq];
+      pr += (double)v.x;
+      pi += (double)v.y;
+    }
+  } else {
+    const double2* p = reinterpret_cast<const double2*>(part) + q;
+    for (; c + 4 <= nchunks; c += 4) {
+      const double2 v0 = p[(int64_t)c * nq], v1 = p[(int64_t)(c + 1) * nq];
+      const double2 v2 = p[(int64_t)(c + 2) * nq], v3 = p[(int64_t)(c + 3) * nq];
+      pr += v0.x;
+      pi += v0.y;
+      pr += v1.x;
+      pi += v1.y;
+      pr += v2.x;
+      pi += v2.y;
+      pr += v3.x;
+      pi += v3.y;
+    }
+    for (; c < nchunks; ++c) {
+      const double2 v = p[(int64_t)c * nq];
+      pr += v.x;
+      pi += v.y;
+    }
+  }
+  return cd{pr, pi};
+}
+
 constexpr int kWG = 256;
 constexpr double kSeriesQR = 0.1;
 // Series threshold of the split-fp16 matrix-core paths: their operands carry about 22 bits, so the
@@ -45,42 +108,40 @@ __device__ inline void load_q(const QView& g, const Rot& R, int64_t q, cd mq[3])
   if (!CQ) mq[0].i = mq[1].i = mq[2].i = 0.0;
 }
 
-// sum_{k=0..K} (i^k / k!) sum_t s_t (q.n_t) (q.(c_t - o))^k from the mesh moments. Monomials are
-// built by repeated multiplication (no indexed arrays, so nothing spills to scratch); this runs
-// only for the few q-points with |q| R below the series threshold.
-__device__ inline cd cpow_small(cd z, int n) {
-  cd r = cd{1.0, 0.0};
-  for (int i = 0; i < n; ++i) r = r * z;
-  return r;
-}
-
-__device__ inline double fact_small(int n) {
-  double f = 1.0;
-  for (int j = 2; j <= n; ++j) f *= j;
-  return f;
-}
-
+// sum_{k=0..K} (i^k / k!) sum_t s_t (q.n_t) (q.(c_t - o))^k from the mesh moments. The moments are
+// stored per order k, then per component i of n, then for a = k..0, b = k - a..0 (c = k - a - b):
+// M_i[k; a, b] = sum_t s_t n_ti (c - o)_x^a (c - o)_y^b (c - o)_z^c. With the multinomial expansion
+// of (q.(c - o))^k the sum is
+//     sum_{a + b + c <= K} i^{a+b+c} (x^a / a!) (y^b / b!) (z^c / c!) sum_i q_i M_i[a + b + c; a, b],
+// (x, y, z) = q, evaluated in one pass over the monomials: the three scaled powers are updated by one
+// multiplication each, the moment index has a closed form, and loop counters are wave-uniform. This
+// runs only for the few q-points with |q| R below the series threshold, but for the whole wave that
+// holds one, so it is kept short.
 __device__ inline cd moment_series(const double* __restrict__ M, const cd mq[3]) {
+  static constexpr double kInv[kSeriesOrder + 1] = {1.0,       1.0,       1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0,
+                                                    1.0 / 5.0, 1.0 / 6.0, 1.0 / 7.0, 1.0 / 8.0};
+  static_assert(kSeriesOrder == 8, "table of 1/n");
   cd total = cd{0.0, 0.0};
-  cd ik = cd{1.0, 0.0};
-  int idx = 0;
-  for (int k = 0; k <= kSeriesOrder; ++k) {
-    const double fk = fact_small(k);
-    cd Tk = cd{0.0, 0.0};
-    for (int i = 0; i < 3; ++i) {
-      cd s = cd{0.0, 0.0};
-      for (int a = k; a >= 0; --a) {
-        const cd xa = cpow_small(mq[0], a);
-        for (int b = k - a; b >= 0; --b) {
-          const int c = k - a - b;
-          const double coef = fk / (fact_small(a) * fact_small(b) * fact_small(c));  // k!/(a! b! c!)
-          s = s + (coef * M[idx++]) * (xa * cpow_small(mq[1], b) * cpow_small(mq[2], c));
-        }
+  cd xa = cd{1.0, 0.0};
+  for (int a = 0; a <= kSeriesOrder; ++a) {
+    if (a > 0) xa = kInv[a] * (xa * mq[0]);
+    cd yb = cd{1.0, 0.0};
+    for (int b = 0; a + b <= kSeriesOrder; ++b) {
+      if (b > 0) yb = kInv[b] * (yb * mq[1]);
+      const cd xy = xa * yb;
+      cd zc = cd{1.0, 0.0};
+      for (int c = 0; a + b + c <= kSeriesOrder; ++c) {
+        if (c > 0) zc = kInv[c] * (zc * mq[2]);
+        const int k = a + b + c, A = k - a;
+        const int nk = (k + 1) * (k + 2) / 2;                            // monomials of order k
+        const int idx = k * (k + 1) * (k + 2) / 2 + A * (A + 1) / 2 + (A - b);  // component 0
+        const cd w = M[idx] * mq[0] + M[idx + nk] * mq[1] + M[idx + 2 * nk] * mq[2];
+        cd t = (xy * zc) * w;
+        if (k & 1) t = mul_i(t);
+        if (k & 2) t = -t;
+        total = total + t;
       }
-      Tk = Tk + mq[i] * s;
     }
-    total = total + (1.0 / fk) * (ik * Tk);
-    ik = mul_i(ik);
   }
   return total;
 }

@@ -208,14 +208,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int zi = z0 + acc_row(k, h);
     if (zi >= nz) continue;
     const int64_t q = (int64_t)zi * ncol + yj;
-    double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)u.chunk * it.nq + q);
-    if (GAUSS) {
-      dst[0] = (double)C1[k] - (double)C2[k];
-      dst[1] = (double)C3[k] - (double)C1[k] - (double)C2[k];
-    } else {
-      dst[0] = (double)C1[k];
-      dst[1] = (double)C2[k];
-    }
+    const int64_t slot = (int64_t)u.chunk * it.nq + q;
+    if (GAUSS)
+      store_raw(it.nchunks, it.out, it.part, it.part32, slot, q, (double)C1[k] - (double)C2[k],
+                (double)C3[k] - (double)C1[k] - (double)C2[k]);
+    else
+      store_raw(it.nchunks, it.out, it.part, it.part32, slot, q, (double)C1[k], (double)C2[k]);
   }
 }
 
@@ -300,8 +298,8 @@ __device__ inline void split_z(const float4 ra, const float4 rc, float uz, float
   }
   o.h0 = pk16(ezr, ezi);
   o.h1 = pk16(z2r, z2i);
-  o.l0 = pk16(ezr - (float)o.h0[0], ezi - (float)o.h0[1]);
-  o.l1 = pk16(z2r - (float)o.h1[0], z2i - (float)o.h1[1]);
+  o.l0 = lo16(o.h0, ezr, ezi);
+  o.l1 = lo16(o.h1, z2r, z2i);
 }
 
 // MZ: row tiles per wave; TWO: each 128-triangle block's sums go to separate accumulators and are
@@ -453,9 +451,8 @@ void k_ff_product16(const ProductItem* __restrict__ items, const ProductUnit* __
       const int zi = z0 + kProductTile * m + row;
       if (yj >= ncol || zi >= nz) continue;
       const int64_t q = (int64_t)zi * ncol + yj;
-      double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)u.chunk * it.nq + q);
-      dst[0] = (double)CR[m][k] * unscale;
-      dst[1] = (double)CI[m][k] * unscale;
+      store_raw(it.nchunks, it.out, it.part, it.part32, (int64_t)u.chunk * it.nq + q, q, (double)CR[m][k] * unscale,
+                (double)CI[m][k] * unscale);
     }
 }
 
@@ -690,9 +687,8 @@ void k_ff_product16s(const ProductItem* __restrict__ items, const ProductUnit* _
       const int zi = z0 + kProductTile * m + row;
       if (yj >= ncol || zi >= nz) continue;
       const int64_t q = (int64_t)zi * ncol + yj;
-      double* dst = it.nchunks == 1 ? it.out + 2 * q : it.part + 2 * ((int64_t)u.chunk * it.nq + q);
-      dst[0] = (double)CR[m][k] * unscale;
-      dst[1] = (double)CI[m][k] * unscale;
+      store_raw(it.nchunks, it.out, it.part, it.part32, (int64_t)u.chunk * it.nq + q, q, (double)CR[m][k] * unscale,
+                (double)CI[m][k] * unscale);
     }
 }
 

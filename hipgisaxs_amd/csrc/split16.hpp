@@ -32,6 +32,26 @@ __device__ inline f16x2 pk16n(float a, float b) {
   return r;
 }
 
+// The low halves of a pair whose high halves are h = pk16(a, b): (fp16(a - h.x), fp16(b - h.y)), each
+// by one mixed-precision FMA (v_fma_mixlo_f16 / v_fma_mixhi_f16: an fp16 and two FP32 sources, an
+// exact FP32 difference, one rounding to fp16), so the high halves are not converted back to FP32.
+// Bit for bit the split of pk16(a - (float)h.x, b - (float)h.y). Register-only instructions.
+__device__ inline f16x2 lo16(f16x2 h, float a, float b) {
+  f16x2 r;
+  asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
+  asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(r) : "v"(h), "v"(b));
+  return r;
+}
+
+// The low halves of the swapped, negated pair (b, -a) from h = pk16(a, b): (fp16(b - h.y),
+// fp16(h.x - a)), equal to swap_neg(lo16(h, a, b)) but for the sign of a zero half.
+__device__ inline f16x2 lo16sn(f16x2 h, float a, float b) {
+  f16x2 r;
+  asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(b));
+  asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "+v"(r) : "v"(h), "v"(a));
+  return r;
+}
+
 // (v.y, -v.x): the pair for the real part from the pair for the imaginary part (exact)
 __device__ inline f16x2 swap_neg(f16x2 v) { return (f16x2){v[1], -v[0]}; }
 
