@@ -44,8 +44,8 @@ PEAK_FP16 = 2.5e15  # MI355X dense fp16 matrix-core peak
 # HBM bytes per launch of the product kernel on this workload and of the finishing kernel that
 # follows it, per work-unit shape, from rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy) in KiB per dispatch (profiles/k_ff_product16s_64polyhedra_batch.md).
-PRODUCT_TRAFFIC_BYTES = {(64, 4, "f16x2", 2, True, True): ((99408.8 * 2 + 694528.0) * 1024,
-                                                           (617116.5 * 2 + 306688.0) * 1024)}
+PRODUCT_TRAFFIC_BYTES = {(64, 4, "f16x2", 2, True, True): ((85599.1 * 2 + 413824.0) * 1024,
+                                                           (522742.7 * 2 + 306688.0) * 1024)}
 
 
 def ALGO_BYTES(shapes):
@@ -1062,7 +1062,7 @@ def main():
         if len(shapes) == 64 and not args.kahan and pcfg in PRODUCT_TRAFFIC_BYTES:
             kb, fb = PRODUCT_TRAFFIC_BYTES[pcfg]
             result["roofline"]["traffic"] = kb
-            # the whole step: the finishing kernel reads every chunk's FP64 partials once more
+            # the whole step: the finishing kernel reads every chunk's FP32 partials once more
             result["roofline"]["traffic_step"] = kb + fb if fb is not None else None
             result["roofline"]["algorithmic_bytes"] = ALGO_BYTES(shapes)
     if b32 is not None:
