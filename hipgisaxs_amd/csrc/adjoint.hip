@@ -65,22 +65,39 @@ __global__ __launch_bounds__(256) void k_adj_weights(QView g, const double* __re
   wt[2 * q + 1] = (T)(-W[2 * q] * s);
 }
 
-// w0 = sum over q-points with |q| = 0 of Re W(q): one workgroup, fixed-order tree (deterministic).
-__global__ __launch_bounds__(256) void k_adj_q0(QView g, const double* __restrict__ W, double* __restrict__ w0) {
+// w0 = sum over q-points with |q| = 0 of Re W(q), in a fixed order (deterministic): workgroup b sums
+// the q-points [b per, (b + 1) per) by a fixed-order tree into parts[b]; k_adj_q0_sum adds the parts.
+constexpr int kAdjQ0Parts = 512;
+
+__device__ inline double block_sum256(double v) {
   __shared__ double s[256];
-  double acc = 0.0;
-  for (int64_t q = threadIdx.x; q < g.nq; q += 256) {
-    int64_t p = q % g.npix;
-    double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = g.qz[2 * q + 1];
-    if (qx * qx + qy * qy + zr * zr + zi * zi == 0.0) acc += W[2 * q];
-  }
-  s[threadIdx.x] = acc;
+  s[threadIdx.x] = v;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *w0 = s[0];
+  return s[0];
+}
+
+__global__ __launch_bounds__(256) void k_adj_q0(QView g, const double* __restrict__ W, int64_t per,
+                                                double* __restrict__ parts) {
+  const int64_t q0 = (int64_t)blockIdx.x * per, q1 = min(g.nq, q0 + per);
+  double acc = 0.0;
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
+    int64_t p = q % g.npix;
+    double qx = g.qx[p], qy = g.qy[p], zr = g.qz[2 * q], zi = g.qz[2 * q + 1];
+    if (qx * qx + qy * qy + zr * zr + zi * zi == 0.0) acc += W[2 * q];
+  }
+  const double t = block_sum256(acc);
+  if (threadIdx.x == 0) parts[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_adj_q0_sum(const double* __restrict__ parts, int n, double* __restrict__ w0) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += parts[i];
+  const double t = block_sum256(acc);
+  if (threadIdx.x == 0) *w0 = t;
 }
 
 // ----- plain FP64, any grid ----------------------------------------------------------------------
@@ -859,7 +876,7 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
                                            R.m[4] == 1 && R.m[5] == 0 && R.m[6] == 0 && R.m[7] == 0 && R.m[8] == 1);
   const size_t r32_b = (precision == HGX_PREC_FP32C && !identity)
                            ? (size_t)m.nblocks * (kBlockTri * 8 * sizeof(float) + 4 * sizeof(double)) : 0;
-  const size_t w0_b = 256;
+  const size_t w0_b = 256 + kAdjQ0Parts * sizeof(double);  // w0, then the parts of its sum
   const size_t wt_b = (size_t)c->nq * 2 * (precision == HGX_PREC_FP64 ? sizeof(double) : sizeof(float));
   int64_t qchunk32 = 0;
   int nqc32 = 0;
@@ -889,7 +906,14 @@ extern "C" int hgx_ff_numeric_adjoint(hgx_ctx* h, int mesh_id, int precision, in
   int* wexp = reinterpret_cast<int*>(r32 + al(r32_b) + al(pm_b) + al(w16_b));
   k_adj_rotate<<<ceil_div(m.ntri, 256), 256, 0, c->stream>>>(m.d_rec64, m.ntri, R, rrec);
   HGX_CHECK_LAUNCH(c);
-  k_adj_q0<<<1, 256, 0, c->stream>>>(g, W, w0);
+  {
+    const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>(kAdjQ0Parts, ceil_div(c->nq, 256)));
+    const int64_t per = ceil_div(c->nq, nparts);
+    double* parts = w0 + 32;
+    k_adj_q0<<<nparts, 256, 0, c->stream>>>(g, W, per, parts);
+    HGX_CHECK_LAUNCH(c);
+    k_adj_q0_sum<<<1, 256, 0, c->stream>>>(parts, nparts, w0);
+  }
   HGX_CHECK_LAUNCH(c);
   if (m.ntri_in > m.ntri) {
     k_adj_zero<<<ceil_div(m.ntri_in * 6, 256), 256, 0, c->stream>>>(grad, m.ntri_in * 6);
