@@ -228,8 +228,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PIPE ? 1 : 
       vh.y = __builtin_bit_cast(unsigned, h1);
       reinterpret_cast<uint2*>(&s_zh[buf][n][zm][hw][zrr])[sw] = vh;
       if (n < NS) {
-        const f16x2 l0 = pk16(v[0] - (float)h0[0], v[1] - (float)h0[1]);
-        const f16x2 l1 = pk16(v[2] - (float)h1[0], v[3] - (float)h1[1]);
+        const f16x2 l0 = lo16(h0, v[0], v[1]);
+        const f16x2 l1 = lo16(h1, v[2], v[3]);
         uint2 vl;
         vl.x = __builtin_bit_cast(unsigned, l0);
         vl.y = __builtin_bit_cast(unsigned, l1);

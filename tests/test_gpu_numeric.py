@@ -350,3 +350,38 @@ def test_exact_fp32c_fine_pyramid_full_grid(gpu_ctx, oracle_mod):
         print(f"exact fp32c 2048-triangle pyramid, 256^2 x 4: relL2 vs fp64 {e:.2e}")
         assert e <= 1e-5
     gpu_ctx.release_mesh(mid)
+
+
+@pytest.mark.parametrize("layout", ["product", "scattered"])
+def test_approx_fp32c_small_q_series_matches_fp64(gpu_ctx, oracle_mod, layout):
+    """The approximate method's small-|q| moment series (|q| R below 0.3 on the matrix cores, 0.1 on
+    the per-pair kernel) against the FP64 direct sum, on q-points down to |q| ~ 1e-5 and q = 0 (the
+    volume): a product grid whose rows and columns approach zero, and scattered q-points along random
+    directions scaled by 1..1e-5 (a one-row grid, per-pair kernel)."""
+    from hipgisaxs_amd import meshes
+
+    rng = np.random.default_rng(23)
+    tri = meshes.convex_polyhedron(rng, np.array([6.0, 4.0, 9.0]), 1500)
+    mid = gpu_ctx.upload_mesh(tri, "exact9")
+    if layout == "product":
+        ax = np.array([0.0, 1e-5, -3e-5, 1e-4, -3e-4, 1e-3, 3e-3, -1e-2, 2e-2, 0.04, -0.05, 0.07, 0.1, 0.3, -1.0])
+        nc = nr = ax.size
+        qy = np.tile(ax, nr)
+        qz = np.repeat(ax, nc) + 0j
+        qx = np.zeros(nc * nr)
+        gpu_ctx.qgrid_set(nc, nr, qx, qy, qz)
+        assert gpu_ctx.is_product()
+    else:
+        n = 64
+        d = rng.normal(size=(3, n))
+        d *= rng.uniform(0.02, 0.2, n) / np.linalg.norm(d, axis=0)
+        scales = np.concatenate([np.full(n, 10.0 ** -k) for k in range(0, 5)] + [np.zeros(1)])
+        qq = np.concatenate([d] * 5 + [np.zeros((3, 1))], axis=1) * scales
+        gpu_ctx.qgrid_set(qq.shape[1], 1, qq[0], qq[1], qq[2] + 0j)
+        assert not gpu_ctx.is_product()
+    ref = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp64"))
+    got = _gpu(gpu_ctx, gpu_ctx.ff_numeric(mid, "approx", "fp32c"))
+    err = np.abs(got - ref) / np.abs(ref)
+    print(f"{layout}: small-q fp32c vs fp64 max rel {err.max():.2e} over {err.size} q-points")
+    assert err.max() <= 1e-5
+    gpu_ctx.release_mesh(mid)
