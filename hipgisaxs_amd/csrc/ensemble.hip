@@ -126,9 +126,26 @@ __device__ inline double rot_norm2(const double* __restrict__ R, double qx, doub
 
 // D_N of a real phase (the real-argument case of dirichlet() in lattice_sf.hpp: the same wrap into
 // (-pi, pi] and the same limit N where the denominator vanishes), and e^{iu} by one sincos
+// D_N(x) = sin(N x / 2) / sin(x / 2) for real x. For an integer N up to 48 (a lattice's repetition
+// count, the same on every lane) it is U_{N-1}(cos(x / 2)), the Chebyshev polynomial of the second
+// kind, by its three-term recurrence U_{k+1} = 2 c U_k - U_{k-1}: one cosine and N - 2 multiply-adds
+// in place of two sines and a division, with the peak value N exact and no 0/0; larger N keep the
+// sines (the recurrence would cost more).
 __device__ inline double dirichlet_real(double x, double N) {
   const double w = x - 2.0 * kPiD * rint(x * (0.5 / kPiD));
   const double h = 0.5 * w;
+  if (N <= 48.0 && N == rint(N)) {
+    const int n = (int)N;
+    if (n <= 1) return 1.0;
+    const double c2 = 2.0 * cos(h);
+    double a = 1.0, b = c2;  // U_0, U_1
+    for (int k = 2; k < n; ++k) {
+      const double t = c2 * b - a;
+      a = b;
+      b = t;
+    }
+    return b;
+  }
   const double den = sin(h);
   if (fabs(den) < 1e-300) return N;
   return sin(N * h) / den;
