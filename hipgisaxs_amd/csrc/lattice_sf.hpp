@@ -16,10 +16,26 @@ constexpr double kPiD = 3.141592653589793238462643383279502884;
 // exp(i p x') with x' = wrap(Re x) + i Im x (wrap into (-pi, pi]), so the product is the
 // Dirichlet kernel D_N(x') = sin(N x'/2) / sin(x'/2), evaluated here in that closed form, which
 // keeps full relative accuracy next to the Bragg peaks where the ratio form cancels.
+// For an integer N up to 48 (a lattice's repetition count, the same on every lane) the same value is
+// U_{N-1}(cos h), h = x'/2, the Chebyshev polynomial of the second kind, by its three-term recurrence
+// (stable: for |cos h| > 1 it follows the growing solution): one complex cosine and N - 2 complex
+// multiply-adds in place of two complex sines and a division, the peak N exact.
 __device__ inline cd dirichlet(cd x, double N) {
   double w = x.r - 2.0 * kPiD * rint(x.r * (0.5 / kPiD));
   cd xp = cd{w, x.i};
   cd h = 0.5 * xp;
+  if (N <= 48.0 && N == rint(N)) {
+    const int n = (int)N;
+    if (n <= 1) return cd{1.0, 0.0};
+    const cd c2 = 2.0 * ccos_d(h);
+    cd a = cd{1.0, 0.0}, b = c2;  // U_0, U_1
+    for (int k = 2; k < n; ++k) {
+      const cd t = c2 * b - a;
+      a = b;
+      b = t;
+    }
+    return b;
+  }
   cd den = csin_d(h);
   if (cabs_d(den) < 1e-300) return cd{N, 0.0};
   return cdiv(csin_d(N * h), den);
