@@ -234,6 +234,56 @@ __global__ __launch_bounds__(256) void k_ensemble_cell64(QViewE g, int nb, const
   }
 }
 
+// The same pre-pass for cells of plain spheres on real q (a particle in vacuum, as the reference's
+// examples place it): |M q| and its components are real, so each node takes one real sincos for the
+// sphere kernel (sin x - x cos x) / x^3 and one for exp(i q_z R), where the complex path evaluates complex
+// sines (src/ff/ff_ana_sphere.cpp:33-45 forms the same kernel on complex q).
+__global__ __launch_bounds__(256) void k_ensemble_cell64_sphere_real(QViewE g, int nb, const double* __restrict__ fc,
+                                                                     const EnsElem* __restrict__ E, int nelem,
+                                                                     const double* __restrict__ loc,
+                                                                     const double* __restrict__ M, EnsSF sf,
+                                                                     double* __restrict__ ffh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npix) return;
+  const double qx = g.qx[p], qy = g.qy[p];
+  for (int b = 0; b < nb; ++b) {
+    const int64_t q = p + b * g.npix;
+    const double qz = g.qz[2 * q];
+    cd cell = cd{0.0, 0.0};
+    for (int e = 0; e < nelem; ++e) {
+      const double* m = M + 9 * (int64_t)e;
+      double mq[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) mq[j] = m[3 * j] * qx + m[3 * j + 1] * qy + m[3 * j + 2] * qz;
+      const double qv = sqrt(mq[0] * mq[0] + mq[1] * mq[1] + mq[2] * mq[2]);
+      cd F = cd{0.0, 0.0};
+      if (qv >= 2.2250738585072014e-308) {
+        const Nodes& P = E[e].P;
+        for (int i = 0; i < P.n[0]; ++i) {
+          const double R = P.val[0][i], x = qv * R;
+          double sx, cx, sz, cz;
+          sincos(x, &sx, &cx);
+          sincos(mq[2] * R, &sz, &cz);
+          const double core = (sx - x * cx) / (x * x * x);
+          const double a = P.wt[0][i] * (4 * kPi * R * R * R) * core;
+          F = F + cd{a * cz, a * sz};
+        }
+      }
+      cd ph = cd{0.0, 0.0};
+      for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
+        const double* t = loc + 3 * l;
+        double s, c;
+        sincos(mq[0] * t[0] + mq[1] * t[1] + mq[2] * t[2], &s, &c);
+        ph = ph + cd{c, s};
+      }
+      cell = cell + cd{E[e].dn2r, E[e].dn2i} * (F * ph);
+    }
+    const cd f = branch_factor64(fc, q, qx, qy, cd{qz, 0.0}, sf) * cell;
+    ffh[2 * q] = f.r;
+    ffh[2 * q + 1] = f.i;
+  }
+}
+
 // sum_b F_b S_g(q_b) for grain gr, FP64; CQ = false: real q_z, so every phase is real and the
 // structure factor is a real Dirichlet product times the centre phase (real sines, one sincos).
 template <int NB, bool CQ>
@@ -933,7 +983,11 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
   if (f32) k_ensemble_cell32<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh); \
   else k_ensemble_cell64<S><<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh)
       switch (common) {
-        case kShapePlainSphere: HGX_ENS_CELL(kShapePlainSphere); break;
+        case kShapePlainSphere:
+          if (!f32 && c->qz_real)
+            k_ensemble_cell64_sphere_real<<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh);
+          else HGX_ENS_CELL(kShapePlainSphere);
+          break;
         case HGX_SHAPE_SPHERE: HGX_ENS_CELL(HGX_SHAPE_SPHERE); break;
         case HGX_SHAPE_CYLINDER: HGX_ENS_CELL(HGX_SHAPE_CYLINDER); break;
         default: HGX_ENS_CELL(-1); break;
