@@ -234,15 +234,55 @@ __global__ __launch_bounds__(256) void k_ensemble_cell64(QViewE g, int nb, const
   }
 }
 
-// The same pre-pass for cells of plain spheres on real q (a particle in vacuum, as the reference's
-// examples place it): |M q| and its components are real, so each node takes one real sincos for the
-// sphere kernel (sin x - x cos x) / x^3 and one for exp(i q_z R), where the complex path evaluates complex
-// sines (src/ff/ff_ana_sphere.cpp:33-45 forms the same kernel on complex q).
-__global__ __launch_bounds__(256) void k_ensemble_cell64_sphere_real(QViewE g, int nb, const double* __restrict__ fc,
-                                                                     const EnsElem* __restrict__ E, int nelem,
-                                                                     const double* __restrict__ loc,
-                                                                     const double* __restrict__ M, EnsSF sf,
-                                                                     double* __restrict__ ffh) {
+// The same pre-pass for cells of one plain sphere or cylinder shape on real q (a particle in vacuum, as
+// the reference's examples place it): M q is real, so the sphere takes one real sincos per node for
+// (sin x - x cos x) / x^3 and one for exp(i q_z R), and the cylinder one sincos per height node for both
+// sinc(q_z H / 2) and exp(i q_z H / 2), where the complex path evaluates complex sines and divisions
+// (src/ff/ff_ana_sphere.cpp:33-45 and src/ff/ff_ana_cylinder.cpp:34-42 form the same kernels on complex q).
+template <int SHAPE>
+__device__ inline cd real_cell_ff(const double mq[3], const Nodes& P);
+
+template <>
+__device__ inline cd real_cell_ff<kShapePlainSphere>(const double mq[3], const Nodes& P) {
+  const double qv = sqrt(mq[0] * mq[0] + mq[1] * mq[1] + mq[2] * mq[2]);
+  cd F = cd{0.0, 0.0};
+  if (qv < 2.2250738585072014e-308) return F;
+  for (int i = 0; i < P.n[0]; ++i) {
+    const double R = P.val[0][i], x = qv * R;
+    double sx, cx, sz, cz;
+    sincos(x, &sx, &cx);
+    sincos(mq[2] * R, &sz, &cz);
+    const double a = P.wt[0][i] * (4 * kPi * R * R * R) * ((sx - x * cx) / (x * x * x));
+    F = F + cd{a * cz, a * sz};
+  }
+  return F;
+}
+
+template <>
+__device__ inline cd real_cell_ff<HGX_SHAPE_CYLINDER>(const double mq[3], const Nodes& P) {
+  const double qp = sqrt(mq[0] * mq[0] + mq[1] * mq[1]);
+  cd F = cd{0.0, 0.0};
+  for (int i = 0; i < P.n[0]; ++i) {
+    const double R = P.val[0][i], t = qp * R;
+    const double bess = t * t > 1.0e-13 ? ::j1(t) / t : 0.0;  // J1(t)/t, 0 below |t|^2 = 1e-13
+    for (int k = 0; k < P.n[1]; ++k) {
+      const double H = P.val[1][k], hz = 0.5 * H * mq[2];
+      double sz, cz;
+      sincos(hz, &sz, &cz);
+      const double sinc = fabs(hz) > 1.0e-14 ? sz / hz : 1.0 - hz * hz / 6.0 + hz * hz * hz * hz / 120.0;
+      const double a = (P.wt[0][i] * P.wt[1][k]) * (2. * kPi * R * R * H) * (sinc * bess);
+      F = F + cd{a * cz, a * sz};
+    }
+  }
+  return F;
+}
+
+template <int SHAPE>
+__global__ __launch_bounds__(256) void k_ensemble_cell64_real(QViewE g, int nb, const double* __restrict__ fc,
+                                                              const EnsElem* __restrict__ E, int nelem,
+                                                              const double* __restrict__ loc,
+                                                              const double* __restrict__ M, EnsSF sf,
+                                                              double* __restrict__ ffh) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npix) return;
   const double qx = g.qx[p], qy = g.qy[p];
@@ -255,20 +295,7 @@ __global__ __launch_bounds__(256) void k_ensemble_cell64_sphere_real(QViewE g, i
       double mq[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) mq[j] = m[3 * j] * qx + m[3 * j + 1] * qy + m[3 * j + 2] * qz;
-      const double qv = sqrt(mq[0] * mq[0] + mq[1] * mq[1] + mq[2] * mq[2]);
-      cd F = cd{0.0, 0.0};
-      if (qv >= 2.2250738585072014e-308) {
-        const Nodes& P = E[e].P;
-        for (int i = 0; i < P.n[0]; ++i) {
-          const double R = P.val[0][i], x = qv * R;
-          double sx, cx, sz, cz;
-          sincos(x, &sx, &cx);
-          sincos(mq[2] * R, &sz, &cz);
-          const double core = (sx - x * cx) / (x * x * x);
-          const double a = P.wt[0][i] * (4 * kPi * R * R * R) * core;
-          F = F + cd{a * cz, a * sz};
-        }
-      }
+      const cd F = real_cell_ff<SHAPE>(mq, E[e].P);
       cd ph = cd{0.0, 0.0};
       for (int l = E[e].loc0; l < E[e].loc0 + E[e].nloc; ++l) {
         const double* t = loc + 3 * l;
@@ -985,11 +1012,17 @@ static int ensemble_run(Context* c, const hgx_ensemble_model* m, int precision, 
       switch (common) {
         case kShapePlainSphere:
           if (!f32 && c->qz_real)
-            k_ensemble_cell64_sphere_real<<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh);
+            k_ensemble_cell64_real<kShapePlainSphere>
+                <<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh);
           else HGX_ENS_CELL(kShapePlainSphere);
           break;
         case HGX_SHAPE_SPHERE: HGX_ENS_CELL(HGX_SHAPE_SPHERE); break;
-        case HGX_SHAPE_CYLINDER: HGX_ENS_CELL(HGX_SHAPE_CYLINDER); break;
+        case HGX_SHAPE_CYLINDER:
+          if (!f32 && c->qz_real)
+            k_ensemble_cell64_real<HGX_SHAPE_CYLINDER>
+                <<<nbx, 256, 0, c->stream>>>(g, c->nbranch, fc, dE, ne, dloc, dM, sfb, ffh);
+          else HGX_ENS_CELL(HGX_SHAPE_CYLINDER);
+          break;
         default: HGX_ENS_CELL(-1); break;
       }
 #undef HGX_ENS_CELL
